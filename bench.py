@@ -1,0 +1,157 @@
+"""Benchmark: aligned frames/s on the 512x512, 30-node workload on one MI355X.
+
+Workload: 2-D 512x512 uint16 photon-count frames (600-blob
+phantom, mean ~50 photons/px, Poisson noise), 200 frames, 30 resampling points
+(nD = 31 nodes, 2 components), trust region to 1e-4 relative change of the SSD
+or 50 iterations. A step aligns the whole 200-frame batch from the zero start,
+with frames already resident in HBM.
+
+Prints one JSON line: value = frames/s of the batched form; roofline of the fused
+warp+SSD+Jtr pass (algorithmic bytes / HIP-event-timed kernel time); the plain
+one-frame-per-launch path and per-frame latency beside it; the CPU oracle timed
+on a bounded sample of the same frames on this host (also the parity check of the
+measured results, <= 0.02 px).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "frame-aligner_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "aligned frames/s @512²,30 nodes (≤0.02px err); p50/p99 latency; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=200)
+    ap.add_argument("--plain-frames", type=int, default=40)
+    ap.add_argument("--cpu-frames", type=int, default=64)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=20260)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames are "
+              f"independent, see the sharding note in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import fa_amd as F
+    from fa_amd import synth
+
+    nD, shape = 31, (1, 512, 512)
+    t0 = time.time()
+    st = synth.make_stack(shape, args.frames, nD, seed=args.seed, n_blobs=600, mean=50.0,
+                          dtype="uint16")
+    log(f"generated {args.frames} frames in {time.time() - t0:.1f}s")
+    ref_w = synth.with_offset(st["reference"])
+    tr = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+
+    ctx = F.Context(0)
+    ref = F.Reference(ctx, ref_w, nD, chunk_px=args.chunk)
+    frames = [ref.register(f) for f in st["frames"]]  # uint16 counts, +1 on the device
+    batch = F.Batch(frames, **tr)
+    batch.set_timing(True)
+    for _ in range(args.warmup):
+        batch.run()
+    tot_wall = 0.0
+    pass_ms = 0.0
+    pass_bytes = 0.0
+    launches = 0
+    rounds = 0
+    for k in range(args.steps):
+        t = time.perf_counter()
+        xs, stats, tm = batch.run()
+        tot_wall += time.perf_counter() - t
+        pass_ms += tm["pass_ms"]
+        pass_bytes += tm["pass_bytes"]
+        launches += tm["pass_launches"]
+        rounds += tm["tr_rounds"]
+        log(f"step {k}: {tm['wall_ms']:.1f} ms, {tm['pass_launches']} passes, "
+            f"pass {tm['pass_ms']:.2f} ms")
+    n = args.frames
+    fps = n * args.steps / tot_wall
+    ms_per_step = 1e3 * tot_wall / args.steps
+    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
+    evals = [s["evaluations"] for s in stats]
+
+    # plain path: one frame per launch sequence
+    lat = []
+    npl = min(args.plain_frames, n)
+    for i in range(min(2, npl)):
+        frames[i].align(**tr)
+    x_plain = []
+    for i in range(npl):
+        t = time.perf_counter()
+        x, _ = frames[i].align(**tr)
+        lat.append(time.perf_counter() - t)
+        x_plain.append(x)
+    lat = np.array(lat) * 1e3
+    plain_fps = npl / (lat.sum() * 1e-3)
+    plain_same = all(np.array_equal(a, b) for a, b in zip(x_plain, xs[:npl]))
+
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded blob phantom, Poisson counts; no dataset)",
+        "config": {"workload": "512x512 uint16, 200 frames, 30 resampling points (nD=31), "
+                               "TR to 1e-4 rel or 50 iterations, batched",
+                   "frames": n, "nD": nD, "shape": list(shape), "chunk_px": ref.n_chunks and args.chunk},
+        "mpx_per_s": round(fps * 512 * 512 / 1e6, 1),
+        "evaluations_per_frame": {"mean": float(np.mean(evals)), "max": int(np.max(evals))},
+        "plain": {"frames_per_s": round(plain_fps, 2), "frames": npl,
+                  "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                 "p99": round(float(np.percentile(lat, 99)), 3)},
+                  "same_result_as_batched": plain_same},
+        "batched_vs_plain": round(fps / plain_fps, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_pass (fused warp + SSD + Jtr)",
+                     "launches": launches, "avg_launch_us": round(1e3 * pass_ms / max(launches, 1), 2),
+                     "bytes_per_launch": pass_bytes / max(launches, 1)},
+    }
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        ncpu = min(args.cpu_frames, n)
+        orc = O.Oracle().set_reference(ref_w, nD)
+        fw = np.stack([synth.with_offset(f) for f in st["frames"][:ncpu]])
+        t = time.perf_counter()
+        x_o, s_o, secs = orc.align_many(fw, threads=args.cpu_threads, **tr)
+        wall = time.perf_counter() - t
+        err = max(float(np.max(np.abs(xs[i] - x_o[i]))) for i in range(ncpu))
+        out["cpu_baseline"] = {"value": round(ncpu / wall, 3), "unit": "frames/s",
+                               "cores": args.cpu_threads, "kind": "port",
+                               "sample": f"first {ncpu} of the {n} frames, one frame per thread, "
+                                         f"{wall:.1f}s wall, {secs.sum():.1f}s CPU",
+                               "latency_ms": {"p50": round(1e3 * float(np.percentile(secs, 50)), 1),
+                                              "p99": round(1e3 * float(np.percentile(secs, 99)), 1)}}
+        out["max_disp_err_px_vs_oracle"] = err
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,868 @@
+// C ABI of the gfx950 warp-registration library (include/frame_aligner.h).
+//
+// Host side of the drop-in boundary: validates arguments, owns device memory behind opaque
+// handles, builds the per-reference tables (node intervals, work chunks) and drives the
+// kernels of fa_kernels.hip on the context's stream. Exceptions never cross the ABI.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fa_kernels.hpp"
+
+using namespace fa;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_err(int code, const std::string& msg)
+{
+   g_last_error = msg;
+   return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+   do {                                                                                           \
+      hipError_t e_ = (expr);                                                                     \
+      if (e_ != hipSuccess)                                                                       \
+         return set_err(FA_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+   } while (0)
+
+#define CHK(expr)                                                                                 \
+   do {                                                                                           \
+      int r_ = (expr);                                                                            \
+      if (r_ != FA_OK) return r_;                                                                 \
+   } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t count)
+{
+   *p = nullptr;
+   if (count == 0) count = 1;
+   if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return set_err(FA_E_OOM, "Not enough memory to allocate");
+   }
+   return FA_OK;
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+   if (p) (void)hipFree((void*)p);
+   p = nullptr;
+}
+
+constexpr int kDefaultChunk = 1024;
+constexpr int kLdsCapFloats = 12288;  // 48 KB of staged frame rows per workgroup
+constexpr int kBlocksPerCu = 4;
+
+// ImageScanParameters ctor (ImageScanParameters.h:7-20) then processScanParameters
+// (FrameWarper.cpp:223-236).
+void derive_scan(const fa_scan_params& in, int nz, int ny, int nx, int nD, double* o6, ScanDev* sd)
+{
+   int sx = in.n_x > 0 ? in.n_x : nx;
+   int sy = in.n_y > 0 ? in.n_y : ny;
+   int sz = in.n_z > 0 ? in.n_z : nz;
+   sx = std::max(sx, 1);
+   sy = std::max(sy, 1);
+   double line = in.line_duration, interline = in.interline_duration, interframe = in.interframe_duration;
+   if (interline < line) interline = 1.01 * line;
+   double pixel = line / sx;
+   double frame = sy * interline;
+   double stack = (sz - 1) * interframe + frame;
+   double cap = nz * frame / (nD - nz);
+   if ((interframe - frame) > cap) {
+      interframe = frame + cap;
+      stack = (nz - 1) * interframe + frame;
+   }
+   o6[0] = line;
+   o6[1] = interline;
+   o6[2] = pixel;
+   o6[3] = frame;
+   o6[4] = interframe;
+   o6[5] = stack;
+   sd->interframe = interframe;
+   sd->interline = interline;
+   sd->pixel = pixel;
+   sd->stack = stack;
+   sd->nDm1 = (double)(nD - 1);
+}
+
+struct DeviceGuard {
+   explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
+};
+
+}  // namespace
+
+// ---- batch ----------------------------------------------------------------------------------
+
+struct fa_batch {
+   fa_context* ctx = nullptr;
+   std::vector<fa_frame*> frames;
+   fa_tr_params prm{};
+   std::vector<BatchFrame> hframes;
+   std::vector<int64_t> x_off;  // offset of each frame's parameter vector in x_inout
+   int64_t total_work = 0, total_x = 0, total_vec = 0, total_fac = 0, total_D = 0;
+   int grid = 1;
+   int timing = 0;
+   RefDev* d_refs = nullptr;
+   BatchFrame* d_frames = nullptr;
+   TrState* d_state = nullptr;
+   double* d_x = nullptr;
+   double* d_vec = nullptr;
+   double* d_fac = nullptr;
+   double* d_D = nullptr;
+   double* d_partials = nullptr;
+   int32_t* d_active_frames = nullptr;
+   int64_t* d_active_base = nullptr;
+   Counters* d_counters = nullptr;
+   Counters* h_counters = nullptr;  // pinned
+   std::vector<hipEvent_t> ev;      // start/end pairs
+   float* d_warp = nullptr;         // output buffer of warps (one-frame batches)
+   int64_t warp_px = 0;
+
+   BatchDev dev() const
+   {
+      BatchDev B;
+      B.refs = d_refs;
+      B.frames = d_frames;
+      B.state = d_state;
+      B.x = d_x;
+      B.vec = d_vec;
+      B.fac = d_fac;
+      B.Dtrial = d_D;
+      B.partials = d_partials;
+      B.active_frames = d_active_frames;
+      B.active_base = d_active_base;
+      B.counters = d_counters;
+      B.prm.radius = prm.radius;
+      B.prm.min_delta = prm.min_delta;
+      B.prm.rel_delta = prm.rel_delta;
+      B.prm.sub_eps = prm.sub_eps;
+      B.prm.max_iter = prm.max_iter;
+      B.prm.max_evals = prm.max_evals;
+      B.prm.sub_max_iter = prm.sub_max_iter;
+      B.prm.skip_repeat = prm.skip_repeat;
+      B.n_frames = (int32_t)frames.size();
+      return B;
+   }
+
+   PassArgs pass_args() const
+   {
+      PassArgs a;
+      a.refs = d_refs;
+      a.frames = d_frames;
+      a.Dtrial = d_D;
+      a.partials = d_partials;
+      a.active_frames = d_active_frames;
+      a.active_base = d_active_base;
+      a.counters = d_counters;
+      a.lds_cap = kLdsCapFloats;
+      a.use_lds = 1;
+      return a;
+   }
+
+   ~fa_batch()
+   {
+      for (auto e : ev) (void)hipEventDestroy(e);
+      dfree(d_refs);
+      dfree(d_frames);
+      dfree(d_state);
+      dfree(d_x);
+      dfree(d_vec);
+      dfree(d_fac);
+      dfree(d_D);
+      dfree(d_partials);
+      dfree(d_active_frames);
+      dfree(d_active_base);
+      dfree(d_counters);
+      dfree(d_warp);
+      if (h_counters) (void)hipHostFree(h_counters);
+   }
+};
+
+namespace {
+
+int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out)
+{
+   *out = nullptr;
+   if (n <= 0 || !frames) return set_err(FA_E_ARG, "empty batch");
+   fa_context* ctx = nullptr;
+   for (int i = 0; i < n; i++) {
+      if (!frames[i] || !frames[i]->ref) return set_err(FA_E_UNKNOWN_FRAME, "Unrecognised frame");
+      if (!ctx) ctx = frames[i]->ref->ctx;
+      if (frames[i]->ref->ctx != ctx) return set_err(FA_E_ARG, "frames of different contexts");
+   }
+   auto* b = new fa_batch();
+   b->ctx = ctx;
+   b->frames.assign(frames, frames + n);
+   if (prm) b->prm = *prm;
+   else fa_tr_default(&b->prm);
+
+   std::map<fa_reference*, int> ref_idx;
+   std::vector<RefDev> refs;
+   b->hframes.resize(n);
+   b->x_off.resize(n);
+   for (int i = 0; i < n; i++) {
+      fa_reference* r = frames[i]->ref;
+      auto it = ref_idx.find(r);
+      int ri;
+      if (it == ref_idx.end()) {
+         ri = (int)refs.size();
+         ref_idx[r] = ri;
+         refs.push_back(r->dev);
+      } else {
+         ri = it->second;
+      }
+      BatchFrame& bf = b->hframes[i];
+      bf.f.data = frames[i]->d_data;
+      bf.f.dtype = frames[i]->dtype;
+      bf.f.offset = frames[i]->offset;
+      bf.ref = ri;
+      bf.n = r->nD * r->n_dim;
+      bf.work_base = b->total_work;
+      bf.x_base = b->total_x;
+      bf.vec_base = b->total_vec;
+      bf.fac_base = b->total_fac;
+      bf.d_base = b->total_D;
+      b->x_off[i] = b->total_x;
+      b->total_work += (int64_t)r->chunks.size();
+      b->total_x += bf.n;
+      b->total_vec += 4 * (int64_t)bf.n;
+      b->total_fac += 2 * (int64_t)r->nD * r->n_dim * r->n_dim;
+      b->total_D += 3 * (int64_t)r->nD;
+   }
+   int rc = FA_OK;
+   auto fail = [&](int code) {
+      delete b;
+      return code;
+   };
+   if ((rc = dalloc(&b->d_refs, refs.size())) || (rc = dalloc(&b->d_frames, n)) ||
+       (rc = dalloc(&b->d_state, n)) || (rc = dalloc(&b->d_x, b->total_x)) ||
+       (rc = dalloc(&b->d_vec, b->total_vec)) || (rc = dalloc(&b->d_fac, b->total_fac)) ||
+       (rc = dalloc(&b->d_D, b->total_D)) ||
+       (rc = dalloc(&b->d_partials, (size_t)b->total_work * kPassStride)) ||
+       (rc = dalloc(&b->d_active_frames, n)) || (rc = dalloc(&b->d_active_base, n + 1)) ||
+       (rc = dalloc(&b->d_counters, 1)))
+      return fail(rc);
+   if (hipHostMalloc((void**)&b->h_counters, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      b->h_counters = nullptr;
+      return fail(set_err(FA_E_OOM, "Not enough memory to allocate"));
+   }
+   hipStream_t s = ctx->stream;
+   if (hipMemcpyAsync(b->d_refs, refs.data(), refs.size() * sizeof(RefDev), hipMemcpyHostToDevice, s) !=
+           hipSuccess ||
+       hipMemcpyAsync(b->d_frames, b->hframes.data(), n * sizeof(BatchFrame), hipMemcpyHostToDevice, s) !=
+           hipSuccess ||
+       hipMemsetAsync(b->d_state, 0, n * sizeof(TrState), s) != hipSuccess ||
+       hipMemsetAsync(b->d_counters, 0, sizeof(Counters), s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(set_err(FA_E_DEVICE, "batch upload failed"));
+   }
+   b->grid = (int)std::max<int64_t>(1, std::min<int64_t>(b->total_work, (int64_t)ctx->n_cu * kBlocksPerCu));
+   *out = b;
+   return FA_OK;
+}
+
+int event_pair(fa_batch* b, size_t i, hipEvent_t* s, hipEvent_t* e)
+{
+   while (b->ev.size() < 2 * (i + 1)) {
+      hipEvent_t x;
+      HIPCHK(hipEventCreate(&x));
+      b->ev.push_back(x);
+   }
+   *s = b->ev[2 * i];
+   *e = b->ev[2 * i + 1];
+   return FA_OK;
+}
+
+constexpr size_t kMaxTimedRounds = 4096;
+
+int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timing* timing)
+{
+   auto t0 = std::chrono::steady_clock::now();
+   fa_context* ctx = b->ctx;
+   hipStream_t s = ctx->stream;
+   const int n = (int)b->frames.size();
+   BatchDev B = b->dev();
+   PassArgs pa = b->pass_args();
+   HIPCHK(hipMemcpyAsync(b->d_x, x_inout, b->total_x * sizeof(double), hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemsetAsync(b->d_counters, 0, sizeof(Counters), s));
+   HIPCHK(launch_tr_init(B, 0, s));
+   const int64_t max_rounds = b->prm.max_evals > 0 ? (int64_t)b->prm.max_evals + 2 : 100000;
+   const int K = 4;
+   size_t timed = 0;
+   int64_t round = 0;
+   bool finished = false;
+   while (!finished && round < max_rounds) {
+      for (int r = 0; r < K && round < max_rounds; r++, round++) {
+         HIPCHK(launch_schedule(B, 1, s));
+         hipEvent_t es = nullptr, ee = nullptr;
+         if (b->timing && timed < kMaxTimedRounds) {
+            CHK(event_pair(b, timed, &es, &ee));
+            HIPCHK(hipEventRecord(es, s));
+         }
+         HIPCHK(launch_pass(pa, b->grid, s));
+         if (es) {
+            HIPCHK(hipEventRecord(ee, s));
+            timed++;
+         }
+         HIPCHK(launch_tr_update(B, s));
+      }
+      HIPCHK(launch_schedule(B, 0, s));
+      HIPCHK(hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(Counters), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      finished = b->h_counters->n_active_frames == 0;
+   }
+   std::vector<TrState> st(n);
+   HIPCHK(hipMemcpyAsync(x_inout, b->d_x, b->total_x * sizeof(double), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(st.data(), b->d_state, n * sizeof(TrState), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   if (stats)
+      for (int i = 0; i < n; i++) {
+         stats[i].iterations = st[i].iterations;
+         stats[i].evaluations = st[i].evaluations;
+         stats[i].rejected = st[i].rejected;
+         stats[i].status = st[i].done ? st[i].status : FA_STOP_MAX_EVALS;
+         stats[i].f_final = st[i].f;
+         stats[i].radius_final = st[i].radius;
+      }
+   if (timing) {
+      double pass_ms = 0;
+      for (size_t i = 0; i < timed; i++) {
+         float ms = 0;
+         HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
+         pass_ms += ms;
+      }
+      timing->pass_ms = pass_ms;
+      timing->pass_launches = (int64_t)timed;
+      timing->pass_bytes = (double)b->h_counters->bytes;
+      timing->tr_rounds = (int64_t)b->h_counters->rounds;
+      timing->wall_ms =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+   }
+   return FA_OK;
+}
+
+// One fused pass over every frame at D (concatenated nD x 3 per frame).
+int batch_eval(fa_batch* b, const double* D, double* err, double* jac)
+{
+   hipStream_t s = b->ctx->stream;
+   const int n = (int)b->frames.size();
+   BatchDev B = b->dev();
+   HIPCHK(hipMemcpyAsync(b->d_D, D, b->total_D * sizeof(double), hipMemcpyHostToDevice, s));
+   HIPCHK(launch_tr_init(B, 1, s));
+   HIPCHK(launch_schedule(B, 1, s));
+   HIPCHK(launch_pass(b->pass_args(), b->grid, s));
+   HIPCHK(launch_reduce_eval(B, s));
+   std::vector<TrState> st(n);
+   std::vector<double> vec;
+   HIPCHK(hipMemcpyAsync(st.data(), b->d_state, n * sizeof(TrState), hipMemcpyDeviceToHost, s));
+   if (jac) {
+      vec.resize(b->total_vec);
+      HIPCHK(hipMemcpyAsync(vec.data(), b->d_vec, b->total_vec * sizeof(double), hipMemcpyDeviceToHost, s));
+   }
+   HIPCHK(hipStreamSynchronize(s));
+   for (int i = 0; i < n; i++) {
+      if (err) err[i] = st[i].f;
+      if (jac) {
+         const BatchFrame& bf = b->hframes[i];
+         std::memcpy(jac + b->x_off[i], vec.data() + bf.vec_base + bf.n, bf.n * sizeof(double));
+      }
+   }
+   return FA_OK;
+}
+
+int single_batch(fa_frame* f, fa_batch** out)
+{
+   if (!f->single) CHK(batch_build(&f, 1, nullptr, &f->single));
+   *out = f->single;
+   return FA_OK;
+}
+
+}  // namespace
+
+// ---- C ABI ------------------------------------------------------------------------------------
+
+extern "C" {
+
+const char* fa_last_error(void) { return g_last_error.c_str(); }
+
+void fa_tr_default(fa_tr_params* p)
+{
+   p->radius = 40;
+   p->min_delta = 1e-8;
+   p->rel_delta = 0;
+   p->max_iter = 0;
+   p->max_evals = 0;
+   p->sub_eps = 0.1;
+   p->sub_max_iter = 20;
+   p->skip_repeat = 0;
+}
+
+int fa_create(int device, fa_context** out)
+{
+   if (!out) return set_err(FA_E_ARG, "null output");
+   *out = nullptr;
+   int count = 0;
+   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+      (void)hipGetLastError();
+      return set_err(FA_E_DEVICE, "No supported GPU");
+   }
+   hipDeviceProp_t prop;
+   HIPCHK(hipGetDeviceProperties(&prop, device));
+   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      return set_err(FA_E_DEVICE, std::string("No supported GPU (found ") + prop.gcnArchName + ")");
+   HIPCHK(hipSetDevice(device));
+   auto* c = new fa_context();
+   c->device = device;
+   c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return set_err(FA_E_DEVICE, "stream creation failed");
+   }
+   *out = c;
+   return FA_OK;
+}
+
+int fa_destroy(fa_context* ctx)
+{
+   if (!ctx) return FA_OK;
+   DeviceGuard g(ctx->device);
+   (void)hipStreamSynchronize(ctx->stream);
+   (void)hipStreamDestroy(ctx->stream);
+   delete ctx;
+   return FA_OK;
+}
+
+void* fa_context_stream(fa_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, int32_t nx,
+                     int32_t nD, const fa_scan_params* scan, int32_t chunk_px, fa_reference** out)
+{
+   if (!ctx || !ref || !scan || !out) return set_err(FA_E_ARG, "null argument");
+   *out = nullptr;
+   if (nz < 1 || ny < 2 || nx < 2) return set_err(FA_E_ARG, "Reference must be three dimensional");
+   const int64_t n_px = (int64_t)nz * ny * nx;
+   if (n_px >= INT32_MAX) return set_err(FA_E_ARG, "volume too large");
+   if (nD < 2 || nD - 1 > 65535 || nD <= nz) return set_err(FA_E_ARG, "invalid number of nodes");
+   if (scan->bidirectional) return set_err(FA_E_ARG, "bidirectional scans are not supported");
+   std::lock_guard<std::mutex> lk(ctx->mu);
+   DeviceGuard g(ctx->device);
+   hipStream_t s = ctx->stream;
+
+   auto* r = new fa_reference();
+   r->ctx = ctx;
+   r->nz = nz;
+   r->ny = ny;
+   r->nx = nx;
+   r->nD = nD;
+   r->n_dim = nz > 1 ? 3 : 2;
+   r->n_px = n_px;
+   r->chunk_px = chunk_px > 0 ? chunk_px : kDefaultChunk;
+   derive_scan(*scan, nz, ny, nx, nD, r->scan_host, &r->scan);
+   const int m = r->n_dim;
+
+   float* d_T = nullptr;
+   uint16_t* d_Di = nullptr;
+   int32_t* d_begin = nullptr;
+   int32_t* d_flags = nullptr;
+   double* d_hpart = nullptr;
+   double* d_info = nullptr;
+   auto cleanup = [&]() {
+      dfree(d_T);
+      dfree(d_Di);
+      dfree(d_begin);
+      dfree(d_flags);
+      dfree(d_hpart);
+      dfree(d_info);
+   };
+   auto fail = [&](int code) {
+      cleanup();
+      fa_reference_release(r);
+      return code;
+   };
+   int rc;
+   if ((rc = dalloc(&d_T, n_px)) || (rc = dalloc(&d_Di, n_px)) || (rc = dalloc(&d_begin, nD)) ||
+       (rc = dalloc(&d_flags, 1)) || (rc = dalloc(&d_info, 2)) ||
+       (rc = dalloc(&r->d_rg, (size_t)n_px * (1 + m))))
+      return fail(rc);
+
+   // node map on the device, interval starts back to the host
+   std::vector<int32_t> begin(nD, INT32_MAX);
+   uint16_t last_i = 0;
+   int32_t flags = 0;
+   if (hipMemcpyAsync(d_T, ref, n_px * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(d_begin, begin.data(), nD * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemsetAsync(d_flags, 0, sizeof(int32_t), s) != hipSuccess ||
+       launch_node_map(r->scan, nx, ny, n_px, d_Di, nullptr, s) != hipSuccess ||
+       launch_interval_begin(d_Di, n_px, nD, d_begin, d_flags, s) != hipSuccess ||
+       hipMemcpyAsync(begin.data(), d_begin, nD * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipMemcpyAsync(&last_i, d_Di + (n_px - 1), sizeof(uint16_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipMemcpyAsync(&flags, d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(set_err(FA_E_DEVICE, "node map failed"));
+   }
+   if (flags & 2)
+      return fail(set_err(FA_E_ARG, "scan timing is not monotonic in scan order "
+                                    "(interframe_duration < frame duration)"));
+   if ((flags & 1) || last_i != nD - 2) return fail(set_err(FA_E_NODE_MAP, "Something went wrong"));
+
+   // D_range exactly as precomputeInterp leaves it (FrameWarper.cpp:269-283)
+   r->D_range.assign(2 * nD, 0);
+   int last = -1;
+   for (int i = 0; i <= nD - 2; i++) {
+      if (begin[i] == INT32_MAX) continue;
+      r->D_range[2 * i] = begin[i];
+      if (last >= 0) r->D_range[2 * last + 1] = begin[i] - 1;
+      last = i;
+   }
+   r->D_range[2 * last + 1] = (int32_t)(n_px - 1);
+
+   // work chunks: within one interval and one plane, at most chunk_px pixels
+   const int64_t plane = (int64_t)nx * ny;
+   r->ichunk.assign(nD, 0);
+   for (int i = 0; i <= nD - 2; i++) {
+      r->ichunk[i] = (int32_t)r->chunks.size();
+      if (begin[i] == INT32_MAX) continue;
+      int64_t b0 = r->D_range[2 * i], e0 = r->D_range[2 * i + 1];
+      for (int64_t p = b0; p <= e0;) {
+         int64_t plane_end = (p / plane + 1) * plane - 1;
+         int64_t q = std::min<int64_t>({e0, plane_end, p + r->chunk_px - 1});
+         r->chunks.push_back(Chunk{(int32_t)p, (int32_t)(q + 1), i, (int32_t)e0});
+         p = q + 1;
+      }
+   }
+   r->ichunk[nD - 1] = (int32_t)r->chunks.size();
+   const int nc = (int)r->chunks.size();
+   const int mm = m * m;
+   if ((rc = dalloc(&r->d_chunks, nc)) || (rc = dalloc(&r->d_ichunk, nD)) ||
+       (rc = dalloc(&r->d_Hdiag, (size_t)nD * mm)) || (rc = dalloc(&r->d_Hsub, (size_t)nD * mm)) ||
+       (rc = dalloc(&r->d_L0d, (size_t)nD * mm)) || (rc = dalloc(&r->d_L0s, (size_t)nD * mm)) ||
+       (rc = dalloc(&d_hpart, (size_t)nc * kHessStride)))
+      return fail(rc);
+
+   RefDev& R = r->dev;
+   R.rg = r->d_rg;
+   R.chunks = r->d_chunks;
+   R.ichunk = r->d_ichunk;
+   R.Hdiag = r->d_Hdiag;
+   R.Hsub = r->d_Hsub;
+   R.L0d = r->d_L0d;
+   R.L0s = r->d_L0s;
+   R.scan = r->scan;
+   R.nx = nx;
+   R.ny = ny;
+   R.nz = nz;
+   R.nD = nD;
+   R.n_dim = m;
+   R.n_chunks = nc;
+   R.h_pd = 0;
+   R.eig_lo = 0;
+   R.n_px = n_px;
+
+   double info[2] = {0, 0};
+   r->Hdiag.assign((size_t)nD * mm, 0);
+   r->Hsub.assign((size_t)nD * mm, 0);
+   if (hipMemcpyAsync(r->d_chunks, r->chunks.data(), nc * sizeof(Chunk), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(r->d_ichunk, r->ichunk.data(), nD * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemsetAsync(r->d_Hsub, 0, (size_t)nD * mm * sizeof(double), s) != hipSuccess ||
+       hipMemsetAsync(r->d_L0s, 0, (size_t)nD * mm * sizeof(double), s) != hipSuccess ||
+       launch_sobel(d_T, nz, ny, nx, m, r->d_rg, s) != hipSuccess ||
+       launch_hessian(R, d_hpart, r->d_Hdiag, r->d_Hsub, r->d_L0d, r->d_L0s, d_info, s) != hipSuccess ||
+       hipMemcpyAsync(info, d_info, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipMemcpyAsync(r->Hdiag.data(), r->d_Hdiag, (size_t)nD * mm * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipMemcpyAsync(r->Hsub.data(), r->d_Hsub, (size_t)nD * mm * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(set_err(FA_E_DEVICE, "reference set-up failed"));
+   }
+   r->h_pd = info[0] != 0.0;
+   r->eig_lo = info[1];
+   R.h_pd = r->h_pd;
+   R.eig_lo = r->eig_lo;
+   cleanup();
+   *out = r;
+   return FA_OK;
+}
+
+int fa_reference_release(fa_reference* r)
+{
+   if (!r) return FA_OK;
+   DeviceGuard g(r->ctx->device);
+   dfree(r->d_rg);
+   dfree(r->d_chunks);
+   dfree(r->d_ichunk);
+   dfree(r->d_Hdiag);
+   dfree(r->d_Hsub);
+   dfree(r->d_L0d);
+   dfree(r->d_L0s);
+   delete r;
+   return FA_OK;
+}
+
+int fa_reference_info(const fa_reference* r, int32_t* o)
+{
+   if (!r || !o) return set_err(FA_E_ARG, "null argument");
+   o[0] = r->nz;
+   o[1] = r->ny;
+   o[2] = r->nx;
+   o[3] = r->nD;
+   o[4] = r->n_dim;
+   o[5] = (int32_t)r->chunks.size();
+   return FA_OK;
+}
+
+int fa_get_hessian(const fa_reference* r, double* H)
+{
+   if (!r || !H) return set_err(FA_E_ARG, "null argument");
+   const int m = r->n_dim, n = r->nD * m, mm = m * m;
+   std::fill(H, H + (size_t)n * n, 0.0);
+   for (int k = 0; k < r->nD; k++)
+      for (int a = 0; a < m; a++)
+         for (int c = 0; c < m; c++) {
+            H[(size_t)(k * m + a) * n + k * m + c] = r->Hdiag[(size_t)k * mm + a * m + c];
+            if (k + 1 < r->nD) {
+               double v = r->Hsub[(size_t)k * mm + a * m + c];
+               H[(size_t)((k + 1) * m + a) * n + k * m + c] = v;
+               H[(size_t)(k * m + c) * n + (k + 1) * m + a] = v;
+            }
+         }
+   return FA_OK;
+}
+
+int fa_get_node_map(const fa_reference* r, uint16_t* Di, float* Df, int32_t* D_range)
+{
+   if (!r) return set_err(FA_E_ARG, "null argument");
+   if (D_range) std::memcpy(D_range, r->D_range.data(), r->D_range.size() * sizeof(int32_t));
+   if (!Di && !Df) return FA_OK;
+   std::lock_guard<std::mutex> lk(r->ctx->mu);
+   DeviceGuard g(r->ctx->device);
+   hipStream_t s = r->ctx->stream;
+   uint16_t* d_Di = nullptr;
+   float* d_Df = nullptr;
+   CHK(dalloc(&d_Di, r->n_px));
+   if (dalloc(&d_Df, r->n_px) != FA_OK) {
+      dfree(d_Di);
+      return FA_E_OOM;
+   }
+   int rc = FA_OK;
+   if (launch_node_map(r->scan, r->nx, r->ny, r->n_px, d_Di, d_Df, s) != hipSuccess ||
+       (Di && hipMemcpyAsync(Di, d_Di, r->n_px * sizeof(uint16_t), hipMemcpyDeviceToHost, s) != hipSuccess) ||
+       (Df && hipMemcpyAsync(Df, d_Df, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess) ||
+       hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      rc = set_err(FA_E_DEVICE, "node map failed");
+   }
+   dfree(d_Di);
+   dfree(d_Df);
+   return rc;
+}
+
+int fa_get_gradient(const fa_reference* r, float* gx, float* gy, float* gz)
+{
+   if (!r) return set_err(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(r->ctx->mu);
+   DeviceGuard g(r->ctx->device);
+   const int m = r->n_dim;
+   std::vector<float> rg((size_t)r->n_px * (1 + m));
+   HIPCHK(hipMemcpyAsync(rg.data(), r->d_rg, rg.size() * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
+   HIPCHK(hipStreamSynchronize(r->ctx->stream));
+   for (int64_t p = 0; p < r->n_px; p++) {
+      if (gx) gx[p] = rg[p * (1 + m) + 1];
+      if (gy) gy[p] = rg[p * (1 + m) + 2];
+      if (gz) gz[p] = m == 3 ? rg[p * (1 + m) + 3] : 0.f;
+   }
+   return FA_OK;
+}
+
+int fa_register_frame(fa_reference* r, const void* data, int32_t dtype, float add_offset, fa_frame** out)
+{
+   if (!r || !data || !out) return set_err(FA_E_ARG, "null argument");
+   *out = nullptr;
+   if (dtype != FA_F32 && dtype != FA_U16) return set_err(FA_E_ARG, "unsupported frame dtype");
+   std::lock_guard<std::mutex> lk(r->ctx->mu);
+   DeviceGuard g(r->ctx->device);
+   auto* f = new fa_frame();
+   f->ref = r;
+   f->dtype = dtype;
+   f->offset = add_offset;
+   size_t bytes = (size_t)r->n_px * (dtype == FA_U16 ? 2 : 4);
+   char* d = nullptr;
+   int rc = dalloc(&d, bytes);
+   if (rc) {
+      delete f;
+      return rc;
+   }
+   f->d_data = d;
+   if (hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, r->ctx->stream) != hipSuccess ||
+       hipStreamSynchronize(r->ctx->stream) != hipSuccess) {
+      (void)hipGetLastError();
+      dfree(d);
+      delete f;
+      return set_err(FA_E_DEVICE, "frame upload failed");
+   }
+   r->live_frames++;
+   *out = f;
+   return FA_OK;
+}
+
+int fa_deregister_frame(fa_frame* f)
+{
+   if (!f) return FA_OK;
+   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   DeviceGuard g(f->ref->ctx->device);
+   delete f->single;
+   f->single = nullptr;
+   void* d = f->d_data;
+   dfree(d);
+   f->ref->live_frames--;
+   delete f;
+   return FA_OK;
+}
+
+int fa_get_error(fa_frame* f, const double* D, double* err)
+{
+   if (!f || !D || !err) return set_err(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   DeviceGuard g(f->ref->ctx->device);
+   fa_batch* b;
+   CHK(single_batch(f, &b));
+   return batch_eval(b, D, err, nullptr);
+}
+
+int fa_get_jacobian(fa_frame* f, const double* D, double* jac, double* err)
+{
+   if (!f || !D || !jac) return set_err(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   DeviceGuard g(f->ref->ctx->device);
+   fa_batch* b;
+   CHK(single_batch(f, &b));
+   double e = 0;
+   CHK(batch_eval(b, D, &e, jac));
+   if (err) *err = e;
+   return FA_OK;
+}
+
+int fa_warp_interpolated(fa_frame* f, const double* D, float* out, float invalid_value)
+{
+   if (!f || !D || !out) return set_err(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   DeviceGuard g(f->ref->ctx->device);
+   fa_batch* b;
+   CHK(single_batch(f, &b));
+   fa_reference* r = f->ref;
+   hipStream_t s = r->ctx->stream;
+   if (!b->d_warp) {
+      CHK(dalloc(&b->d_warp, r->n_px));
+      b->warp_px = r->n_px;
+   }
+   FrameDev F;
+   F.data = f->d_data;
+   F.dtype = f->dtype;
+   F.offset = f->offset;
+   int grid = (int)std::min<int64_t>((int64_t)r->chunks.size(), (int64_t)r->ctx->n_cu * kBlocksPerCu);
+   HIPCHK(hipMemcpyAsync(b->d_D, D, 3 * (size_t)r->nD * sizeof(double), hipMemcpyHostToDevice, s));
+   HIPCHK(launch_warp(r->dev, F, b->d_D, b->d_warp, invalid_value, kLdsCapFloats, 1, grid, s));
+   HIPCHK(hipMemcpyAsync(out, b->d_warp, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   return FA_OK;
+}
+
+int fa_warp_point(const fa_reference* r, const double* D, int32_t x, int32_t y, int32_t z,
+                  int32_t binning, double* out3)
+{
+   if (!r || !D || !out3) return set_err(FA_E_ARG, "null argument");
+   if (binning < 1) return set_err(FA_E_ARG, "spatial binning must be >= 1");
+   double factor = 1.0 / binning;
+   int xs = (int)(x / factor);
+   int ys = (int)(y / factor);
+   out3[0] = out3[1] = out3[2] = 0;
+   if (xs < 0 || xs >= r->nx || ys < 0 || ys >= r->ny || z < 0 || z >= r->nz) return FA_OK;
+   double t = (double)z * r->scan.interframe + (double)ys * r->scan.interline + (double)xs * r->scan.pixel;
+   double u = t / r->scan.stack * r->scan.nDm1;
+   double ip = std::trunc(u);
+   double f = (double)(float)(u - ip);
+   int i = (int)ip;
+   for (int c = 0; c < 3; c++) {
+      double a = (c < r->n_dim) ? D[3 * (i + 1) + c] : 0.0, b = (c < r->n_dim) ? D[3 * i + c] : 0.0;
+      out3[c] = (f * a + (1 - f) * b) * factor;
+   }
+   return FA_OK;
+}
+
+int fa_align(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_align_stats* stats)
+{
+   if (!f || !x_inout) return set_err(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   DeviceGuard g(f->ref->ctx->device);
+   fa_batch* b;
+   CHK(single_batch(f, &b));
+   if (prm) b->prm = *prm;
+   else fa_tr_default(&b->prm);
+   return batch_run(b, x_inout, stats, nullptr);
+}
+
+int fa_batch_create(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out)
+{
+   if (!out || !frames || n <= 0 || !frames[0] || !frames[0]->ref) return set_err(FA_E_ARG, "invalid batch");
+   fa_context* ctx = frames[0]->ref->ctx;
+   std::lock_guard<std::mutex> lk(ctx->mu);
+   DeviceGuard g(ctx->device);
+   return batch_build(frames, n, prm, out);
+}
+
+int fa_batch_destroy(fa_batch* b)
+{
+   if (!b) return FA_OK;
+   std::lock_guard<std::mutex> lk(b->ctx->mu);
+   DeviceGuard g(b->ctx->device);
+   (void)hipStreamSynchronize(b->ctx->stream);
+   delete b;
+   return FA_OK;
+}
+
+int fa_batch_set_timing(fa_batch* b, int32_t enable)
+{
+   if (!b) return set_err(FA_E_ARG, "null batch");
+   b->timing = enable ? 1 : 0;
+   return FA_OK;
+}
+
+int fa_batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timing* timing)
+{
+   if (!b || !x_inout) return set_err(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(b->ctx->mu);
+   DeviceGuard g(b->ctx->device);
+   return batch_run(b, x_inout, stats, timing);
+}
+
+int fa_batch_eval(fa_batch* b, const double* D, double* err, double* jac)
+{
+   if (!b || !D) return set_err(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(b->ctx->mu);
+   DeviceGuard g(b->ctx->device);
+   return batch_eval(b, D, err, jac);
+}
+
+int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, double* x_inout,
+                   fa_align_stats* stats)
+{
+   fa_batch* b = nullptr;
+   CHK(fa_batch_create(frames, n, prm, &b));
+   int rc = fa_batch_run(b, x_inout, stats, nullptr);
+   fa_batch_destroy(b);
+   return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+// Internal structures of the gfx950 warp-registration library: device views of references
+// and frames, the per-frame trust-region state, and the host objects behind the C ABI
+// handles of include/frame_aligner.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "frame_aligner.h"
+
+namespace fa {
+
+// Derived scan timing in double (ImageScanParameters.h:7-20 after
+// processScanParameters, FrameWarper.cpp:223-236): t(z,y,x) = z*interframe +
+// y*interline + x*pixel and u = t / stack * (nD - 1).
+struct ScanDev {
+   double interframe, interline, pixel, stack, nDm1;
+};
+
+// One work item of the fused passes: the pixels [p0, p1) of node interval k inside one
+// plane. end_p is the last pixel of interval k, which the reference leaves out of the
+// steepest-descent images, JtJ and Jtr (its consumers iterate [begin, end),
+// FrameWarper.cpp:309-344) but keeps in the SSD.
+struct Chunk {
+   int32_t p0, p1, k, end_p;
+};
+
+// Doubles written per work item by the fused pass: SSD, Jtr of node k (n_dim values,
+// weight 1-f), Jtr of node k+1 (n_dim values, weight f); padded to 8.
+constexpr int kPassStride = 8;
+// Per work item of the JtJ pass: aa (upper, n_dim(n_dim+1)/2), bb (same), ba (n_dim^2);
+// 21 in 3-D, padded to 24.
+constexpr int kHessStride = 24;
+constexpr int kBlock = 256;  // threads per fused-pass workgroup (4 wave64)
+
+struct RefDev {
+   const float* rg;        // per pixel (R, gx, gy[, gz]): 1 + n_dim floats, interleaved
+   const Chunk* chunks;    // work items, ordered by interval then pixel
+   const int32_t* ichunk;  // first chunk of interval k (k = 0..nD-2), ichunk[nD-1] = n_chunks
+   const double* Hdiag;    // nD blocks n_dim x n_dim
+   const double* Hsub;     // nD-1 blocks, block (k+1, k)
+   const double* L0d;      // Cholesky factor of H (valid when h_pd)
+   const double* L0s;
+   ScanDev scan;
+   int32_t nx, ny, nz, nD, n_dim, n_chunks, h_pd, pad;
+   double eig_lo;          // Gershgorin lower bound of the spectrum of H
+   int64_t n_px;
+};
+
+struct FrameDev {
+   const void* data;
+   int32_t dtype;  // FA_F32 / FA_U16
+   float offset;
+};
+
+// Per frame of a batch.
+struct BatchFrame {
+   FrameDev f;
+   int32_t ref;        // index into the batch's reference table
+   int32_t n;          // nD * n_dim
+   int64_t work_base;  // first work item of this frame (all frames active)
+   int64_t x_base;     // offset of this frame's parameter vector x
+   int64_t vec_base;   // offset of this frame's work vectors (p, g, t1, t2) in the pool
+   int64_t fac_base;   // offset of this frame's factor scratch
+   int64_t d_base;     // offset of this frame's nD x 3 trial displacement
+};
+
+struct TrParamsDev {
+   double radius, min_delta, rel_delta, sub_eps;
+   int32_t max_iter, max_evals, sub_max_iter, skip_repeat;
+};
+
+struct TrState {
+   double f, radius, radius0, prev_f, lambda, pnorm;
+   int32_t phase, stale, been_used, iterations, evaluations, rejected, status, done, sub_iters,
+       pad;
+};
+
+// Counters the schedule kernel maintains for the host and the pass kernel.
+struct Counters {
+   int32_t n_active_frames;
+   int32_t pad;
+   int64_t n_active_work;
+   unsigned long long bytes;  // algorithmic bytes scheduled so far
+   unsigned long long rounds;
+};
+
+struct PassArgs {
+   const RefDev* refs;
+   const BatchFrame* frames;
+   const double* Dtrial;
+   double* partials;
+   const int32_t* active_frames;
+   const int64_t* active_base;
+   const Counters* counters;
+   int32_t lds_cap;   // floats of dynamic LDS available for staging
+   int32_t use_lds;
+};
+
+struct BatchDev {
+   const RefDev* refs;
+   BatchFrame* frames;
+   TrState* state;
+   double* x;         // parameter vectors, concatenated in batch order
+   double* vec;       // per frame: p, g, t1, t2 (4 n doubles)
+   double* fac;       // per frame: Ld, Ls (2 nD n_dim^2 doubles)
+   double* Dtrial;
+   const double* partials;
+   int32_t* active_frames;
+   int64_t* active_base;
+   Counters* counters;
+   TrParamsDev prm;
+   int32_t n_frames;
+};
+
+}  // namespace fa
+
+// ---- host objects behind the C handles ----------------------------------------------------
+
+struct fa_context {
+   int device = 0;
+   int n_cu = 256;
+   hipStream_t stream = nullptr;
+   std::mutex mu;
+};
+
+struct fa_reference {
+   fa_context* ctx = nullptr;
+   int nz = 0, ny = 0, nx = 0, nD = 0, n_dim = 0, chunk_px = 0;
+   int64_t n_px = 0;
+   fa::ScanDev scan{};
+   double scan_host[6] = {0};
+   std::vector<int32_t> D_range;  // nD x 2
+   std::vector<fa::Chunk> chunks;
+   std::vector<int32_t> ichunk;
+   std::vector<double> Hdiag, Hsub;
+   int h_pd = 0;
+   double eig_lo = 0;
+   // device
+   float* d_rg = nullptr;
+   fa::Chunk* d_chunks = nullptr;
+   int32_t* d_ichunk = nullptr;
+   double* d_Hdiag = nullptr;
+   double* d_Hsub = nullptr;
+   double* d_L0d = nullptr;
+   double* d_L0s = nullptr;
+   fa::RefDev dev{};
+   int live_frames = 0;
+};
+
+struct fa_frame {
+   fa_reference* ref = nullptr;
+   int dtype = FA_F32;
+   float offset = 0;
+   void* d_data = nullptr;
+   fa_batch* single = nullptr;  // cached one-frame batch for fa_align and single evaluations
+};

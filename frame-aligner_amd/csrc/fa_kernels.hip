@@ -1,0 +1,1046 @@
+// gfx950 kernels of the warp-registration path.
+//
+// Precision: the node map (which interval a pixel belongs to, its fraction) and the sample
+// position (which taps, whether a sample exists) are computed in FP64 with the reference's
+// operation order, because they decide integer indices that must match the reference
+// exactly. Interpolation, residuals and per-workgroup sums run in FP32 with compensated
+// (Kahan) accumulation; partial sums cross workgroups as FP64, and JtJ / Jtr reductions and
+// the trust-region solve are FP64.
+#include "fa_kernels.hpp"
+
+#include <cfloat>
+#include <cmath>
+
+namespace fa {
+
+// ---- compensated FP32 accumulation -------------------------------------------------------
+// The pair (s, c) holds the value s - c (Kahan's running compensation).
+struct KF {
+   float s, c;
+};
+
+__device__ __forceinline__ void kadd(KF& a, float x)
+{
+   float y = x - a.c;
+   float t = a.s + y;
+   a.c = (t - a.s) - y;
+   a.s = t;
+}
+
+// Sum of two compensated values: TwoSum of the leading parts, errors folded into c.
+__device__ __forceinline__ KF kcomb(KF a, KF b)
+{
+   float s = a.s + b.s;
+   float bb = s - a.s;
+   float err = (a.s - (s - bb)) + (b.s - bb);
+   KF r;
+   r.s = s;
+   r.c = (a.c + b.c) - err;
+   return r;
+}
+
+template <int NV>
+__device__ __forceinline__ void wave_reduce(KF (&acc)[NV])
+{
+#pragma unroll
+   for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+         KF o;
+         o.s = __shfl_xor(acc[v].s, off, 64);
+         o.c = __shfl_xor(acc[v].c, off, 64);
+         acc[v] = kcomb(acc[v], o);
+      }
+   }
+}
+
+// Block (kBlock threads) reduction; thread 0 writes NV doubles to out. Ends with a barrier.
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(KF (&acc)[NV], KF (*sred)[NV], double* out)
+{
+   wave_reduce<NV>(acc);
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   if (lane == 0)
+#pragma unroll
+      for (int v = 0; v < NV; v++) sred[wave][v] = acc[v];
+   __syncthreads();
+   if (threadIdx.x == 0) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+         KF t = sred[0][v];
+         for (int w = 1; w < kBlock / 64; w++) t = kcomb(t, sred[w][v]);
+         out[v] = (double)t.s - (double)t.c;
+      }
+   }
+   __syncthreads();
+}
+
+// ---- node map (FrameWarper.cpp:239-293) -------------------------------------------------
+// u = (z*interframe + y*interline + x*pixel) / stack * (nD-1), i = integer part, f = rest,
+// each operation separately rounded as the reference's C++ states them.
+__device__ __forceinline__ double node_coord(const ScanDev& sc, int z, int y, int x)
+{
+#pragma clang fp contract(off)
+   double t = (double)z * sc.interframe + (double)y * sc.interline + (double)x * sc.pixel;
+   return t / sc.stack * sc.nDm1;
+}
+
+__global__ void k_node_map(ScanDev sc, int nx, int ny, int64_t n, uint16_t* Di, float* Df)
+{
+   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (p >= n) return;
+   int x = (int)(p % nx);
+   int64_t r = p / nx;
+   int y = (int)(r % ny);
+   int z = (int)(r / ny);
+   double u = node_coord(sc, z, y, x);
+   double ip = trunc(u);
+   double f = u - ip;
+   Di[p] = (uint16_t)(int)ip;
+   if (Df) Df[p] = (float)f;
+}
+
+// First pixel of every interval; flags: 1 = interval index out of range, 2 = the map is not
+// monotonic in scan order.
+__global__ void k_interval_begin(const uint16_t* Di, int64_t n, int nD, int32_t* begin,
+                                 int32_t* flags)
+{
+   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (p >= n) return;
+   int i = Di[p];
+   if (i > nD - 2) atomicOr(flags, 1);
+   if (p == 0 || Di[p - 1] != i) {
+      if (p > 0 && Di[p - 1] > i) atomicOr(flags, 2);
+      if (i < nD) atomicMin(&begin[i], (int32_t)p);
+   }
+}
+
+// ---- Sobel (Sobel3d.cpp:8-84), bit-exact -------------------------------------------------
+// One 3-tap pass at position pos of an axis of length n: interior c0*m(-1) + c1*m(0) +
+// c2*m(+1); first sample e1[0]*m(0) + e1[1]*m(+1); last sample e2[0]*m(0) + e2[1]*m(-1); all
+// in float, left to right. A single-plane Z pass is the identity (Sobel3d.cpp:40-44).
+struct Taps {
+   float c0, c1, c2, a0, a1, b0, b1;
+};
+
+template <class G>
+__device__ __forceinline__ float filt(const Taps& t, int pos, int n, bool identity, G get)
+{
+#pragma clang fp contract(off)
+   if (identity) return get(0);
+   if (pos == 0) return t.a0 * get(0) + t.a1 * get(1);
+   if (pos == n - 1) return t.b0 * get(0) + t.b1 * get(-1);
+   return t.c0 * get(-1) + t.c1 * get(0) + t.c2 * get(1);
+}
+
+__global__ void k_sobel(const float* __restrict__ T, int nz, int ny, int nx, int n_dim,
+                        float* __restrict__ rg)
+{
+   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   int64_t n = (int64_t)nz * ny * nx;
+   if (p >= n) return;
+   const int x = (int)(p % nx);
+   const int64_t r = p / nx;
+   const int y = (int)(r % ny);
+   const int z = (int)(r / ny);
+   const Taps S{1.0f / 4.0f, 2.0f / 4.0f, 1.0f / 4.0f, 2.0f / 3.0f, 2.0f / 3.0f, 2.0f / 3.0f, 2.0f / 3.0f};
+   const Taps Dv{-0.5f, 0.0f, 0.5f, -0.5f, 0.5f, 0.5f, -0.5f};
+   const bool zid = nz == 1;
+   auto Tat = [&](int zz, int yy, int xx) { return T[((int64_t)zz * ny + yy) * nx + xx]; };
+   auto Sx = [&](int zz, int yy, int xx) {
+      return filt(S, xx, nx, false, [&](int o) { return Tat(zz, yy, xx + o); });
+   };
+   auto Sy = [&](int zz, int yy, int xx) {
+      return filt(S, yy, ny, false, [&](int o) { return Tat(zz, yy + o, xx); });
+   };
+   // gx = Dx(Sz(Sy(T)))
+   float gx = filt(Dv, x, nx, false, [&](int ox) {
+      return filt(S, z, nz, zid, [&](int oz) { return Sy(z + oz, y, x + ox); });
+   });
+   // gy = Dy(Sz(Sx(T)))
+   float gy = filt(Dv, y, ny, false, [&](int oy) {
+      return filt(S, z, nz, zid, [&](int oz) { return Sx(z + oz, y + oy, x); });
+   });
+   float* o = rg + p * (1 + n_dim);
+   o[0] = T[p];
+   o[1] = gx;
+   o[2] = gy;
+   if (n_dim == 3) {
+      // gz = Dz(Sy(Sx(T)))
+      float gz = filt(Dv, z, nz, zid, [&](int oz) {
+         return filt(S, y, ny, false, [&](int oy) { return Sx(z + oz, y + oy, x); });
+      });
+      o[3] = gz;
+   }
+}
+
+// ---- shared per-pixel helpers -------------------------------------------------------------
+
+template <typename FT>
+__device__ __forceinline__ float fload(const FrameDev& F, int64_t i)
+{
+   return (float)((const FT*)F.data)[i] + F.offset;
+}
+
+// Pixel coordinates of the pixels a thread visits in [p0, p1) of one plane: start at
+// p0 + tid and advance by kBlock.
+struct PixelWalk {
+   int x, y;
+   __device__ __forceinline__ void init(int q, int nx)
+   {
+      y = (int)((float)q * (1.0f / (float)nx));
+      x = q - y * nx;
+      while (x < 0) { x += nx; y--; }
+      while (x >= nx) { x -= nx; y++; }
+   }
+   __device__ __forceinline__ void step(int nx)
+   {
+      x += kBlock;
+      while (x >= nx) { x -= nx; y++; }
+   }
+};
+
+// ---- JtJ partial sums (FrameWarper.cpp:296-395) -----------------------------------------
+// Per pixel p of interval k (except its last pixel): a = (1-f) grad T (node k),
+// b = f grad T (node k+1), each product formed in double and stored as float as the
+// reference stores VI_dW_dp; float products a_i a_j, b_i b_j, b_i a_j accumulated.
+template <int M>
+__device__ void hess_chunk(const RefDev& R, const Chunk ch, double* out, KF (*sred)[21])
+{
+   constexpr int NA = M * (M + 1) / 2;
+   constexpr int NV = 2 * NA + M * M;
+   KF acc[NV];
+#pragma unroll
+   for (int v = 0; v < NV; v++) acc[v] = KF{0.f, 0.f};
+   const int nx = R.nx, ny = R.ny;
+   const int64_t plane_sz = (int64_t)nx * ny;
+   const int plane = (int)(ch.p0 / plane_sz);
+   const int q0 = (int)(ch.p0 - plane * plane_sz);
+   PixelWalk w;
+   w.init(q0 + threadIdx.x, nx);
+   for (int p = ch.p0 + threadIdx.x; p < ch.p1; p += kBlock, w.step(nx)) {
+      if (p == ch.end_p) continue;
+      double u = node_coord(R.scan, plane, w.y, w.x);
+      double fd = (double)(float)(u - trunc(u));
+      const float* g = R.rg + (int64_t)p * (1 + M);
+      float a[M], b[M];
+#pragma unroll
+      for (int c = 0; c < M; c++) {
+         a[c] = (float)((double)g[1 + c] * (1.0 - fd));
+         b[c] = (float)((double)g[1 + c] * fd);
+      }
+      int v = 0;
+#pragma unroll
+      for (int i = 0; i < M; i++)
+#pragma unroll
+         for (int j = i; j < M; j++) kadd(acc[v++], a[i] * a[j]);
+#pragma unroll
+      for (int i = 0; i < M; i++)
+#pragma unroll
+         for (int j = i; j < M; j++) kadd(acc[v++], b[i] * b[j]);
+#pragma unroll
+      for (int i = 0; i < M; i++)
+#pragma unroll
+         for (int j = 0; j < M; j++) kadd(acc[v++], b[i] * a[j]);
+   }
+   block_reduce_store<NV>(acc, (KF(*)[NV])sred, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_hess_partials(RefDev R, double* partials)
+{
+   __shared__ KF sred[kBlock / 64][21];
+   for (int c = blockIdx.x; c < R.n_chunks; c += gridDim.x) {
+      Chunk ch = R.chunks[c];
+      double* out = partials + (int64_t)c * kHessStride;
+      if (R.n_dim == 2) hess_chunk<2>(R, ch, out, sred);
+      else hess_chunk<3>(R, ch, out, sred);
+   }
+}
+
+// ---- small FP64 linear algebra (block-tridiagonal Cholesky) -----------------------------
+
+__device__ bool d_chol_small(int m, double* A)
+{
+   for (int j = 0; j < m; j++) {
+      double s = A[j * m + j];
+      for (int k = 0; k < j; k++) s -= A[j * m + k] * A[j * m + k];
+      if (!(s > 0) || !isfinite(s)) return false;
+      double l = sqrt(s);
+      A[j * m + j] = l;
+      for (int i = j + 1; i < m; i++) {
+         double t = A[i * m + j];
+         for (int k = 0; k < j; k++) t -= A[i * m + k] * A[j * m + k];
+         A[i * m + j] = t / l;
+      }
+      for (int i = 0; i < j; i++) A[i * m + j] = 0;
+   }
+   return true;
+}
+
+// L L^T = H + lambda I with H block-tridiagonal (diag blocks, sub-diagonal blocks (k+1,k)).
+__device__ bool d_bt_chol(int nb, int m, const double* diag, const double* sub, double lambda,
+                          double* Ld, double* Ls)
+{
+   const int mm = m * m;
+   for (int k = 0; k < nb; k++) {
+      double A[9];
+      for (int i = 0; i < mm; i++) A[i] = diag[k * mm + i];
+      for (int i = 0; i < m; i++) A[i * m + i] += lambda;
+      if (k > 0) {
+         const double* C = &Ls[(k - 1) * mm];
+         for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) {
+               double s = 0;
+               for (int t = 0; t < m; t++) s += C[i * m + t] * C[j * m + t];
+               A[i * m + j] -= s;
+            }
+      }
+      if (!d_chol_small(m, A)) return false;
+      for (int i = 0; i < mm; i++) Ld[k * mm + i] = A[i];
+      if (k + 1 < nb) {
+         const double* B = &sub[k * mm];
+         double* X = &Ls[k * mm];
+         for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) {
+               double t = B[i * m + j];
+               for (int q = 0; q < j; q++) t -= X[i * m + q] * A[j * m + q];
+               X[i * m + j] = t / A[j * m + j];
+            }
+      }
+   }
+   return true;
+}
+
+__device__ void d_bt_fwd(int nb, int m, const double* Ld, const double* Ls, const double* b,
+                         double* y)
+{
+   const int mm = m * m;
+   for (int k = 0; k < nb; k++) {
+      double r[3];
+      for (int i = 0; i < m; i++) {
+         r[i] = b[k * m + i];
+         if (k > 0)
+            for (int t = 0; t < m; t++) r[i] -= Ls[(k - 1) * mm + i * m + t] * y[(k - 1) * m + t];
+      }
+      const double* L = &Ld[k * mm];
+      for (int i = 0; i < m; i++) {
+         double t = r[i];
+         for (int q = 0; q < i; q++) t -= L[i * m + q] * y[k * m + q];
+         y[k * m + i] = t / L[i * m + i];
+      }
+   }
+}
+
+__device__ void d_bt_bwd(int nb, int m, const double* Ld, const double* Ls, const double* y,
+                         double* x)
+{
+   const int mm = m * m;
+   for (int k = nb - 1; k >= 0; k--) {
+      double r[3];
+      for (int i = 0; i < m; i++) {
+         r[i] = y[k * m + i];
+         if (k + 1 < nb)
+            for (int t = 0; t < m; t++) r[i] -= Ls[k * mm + t * m + i] * x[(k + 1) * m + t];
+      }
+      const double* L = &Ld[k * mm];
+      for (int i = m - 1; i >= 0; i--) {
+         double t = r[i];
+         for (int q = i + 1; q < m; q++) t -= L[q * m + i] * x[k * m + q];
+         x[k * m + i] = t / L[i * m + i];
+      }
+   }
+}
+
+__device__ double d_norm(int n, const double* v)
+{
+   double s = 0;
+   for (int i = 0; i < n; i++) s += v[i] * v[i];
+   return sqrt(s);
+}
+
+// H entry (i, j) of the block-tridiagonal storage.
+__device__ double h_get(const RefDev& R, int i, int j)
+{
+   const int m = R.n_dim;
+   int bi = i / m, bj = j / m, a = i % m, b = j % m;
+   if (bi == bj) return R.Hdiag[bi * m * m + a * m + b];
+   if (bi == bj + 1) return R.Hsub[bj * m * m + a * m + b];
+   if (bj == bi + 1) return R.Hsub[bi * m * m + b * m + a];
+   return 0;
+}
+
+// ---- JtJ assembly, factorisation and spectrum bound (one workgroup) --------------------
+__global__ void k_hess_assemble(RefDev R, const double* partials, double* Hdiag, double* Hsub,
+                                double* L0d, double* L0s, double* info)
+{
+   const int m = R.n_dim, mm = m * m;
+   const int NA = m * (m + 1) / 2;
+   for (int k = threadIdx.x; k < R.nD; k += blockDim.x) {
+      double hd[9] = {0}, hs[9] = {0};
+      // interval k: aa into the diagonal block of node k, ba into block (k+1, k)
+      if (k <= R.nD - 2)
+         for (int c = R.ichunk[k]; c < R.ichunk[k + 1]; c++) {
+            const double* P = partials + (int64_t)c * kHessStride;
+            int v = 0;
+            for (int i = 0; i < m; i++)
+               for (int j = i; j < m; j++) hd[i * m + j] += P[v++];
+            v = 2 * NA;
+            for (int i = 0; i < m; i++)
+               for (int j = 0; j < m; j++) hs[i * m + j] += P[v++];
+         }
+      // interval k-1: bb into the diagonal block of node k
+      if (k >= 1)
+         for (int c = R.ichunk[k - 1]; c < R.ichunk[k]; c++) {
+            const double* P = partials + (int64_t)c * kHessStride;
+            int v = NA;
+            for (int i = 0; i < m; i++)
+               for (int j = i; j < m; j++) hd[i * m + j] += P[v++];
+         }
+      for (int i = 0; i < m; i++)
+         for (int j = 0; j < i; j++) hd[i * m + j] = hd[j * m + i];
+      for (int i = 0; i < mm; i++) {
+         Hdiag[k * mm + i] = hd[i];
+         if (k <= R.nD - 2) Hsub[k * mm + i] = hs[i];
+      }
+   }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      R.Hdiag = Hdiag;
+      R.Hsub = Hsub;
+      const int n = R.nD * m;
+      double eig_lo = INFINITY;
+      for (int i = 0; i < n; i++) {
+         double c = h_get(R, i, i), r = 0;
+         int j0 = (i / m - 1) * m, j1 = (i / m + 2) * m;
+         if (j0 < 0) j0 = 0;
+         if (j1 > n) j1 = n;
+         for (int j = j0; j < j1; j++)
+            if (j != i) r += fabs(h_get(R, i, j));
+         if (c - r < eig_lo) eig_lo = c - r;
+      }
+      info[0] = d_bt_chol(R.nD, m, Hdiag, Hsub, 0.0, L0d, L0s) ? 1.0 : 0.0;
+      info[1] = eig_lo;
+   }
+}
+
+// ---- the fused pass ------------------------------------------------------------------------
+// For every pixel of a work item: node fraction, displacement d = f D[k+1] + (1-f) D[k]
+// (warpPoint, FrameWarper.cpp:397-419), zero-padded bilinear/trilinear pull-sample of the
+// frame at p + d (warpImageInterpolated, FrameWarper.cpp:31-69), residual against the
+// reference and SSD over samples that exist (computeErrorImage, FrameWarper.cpp:4-28), and
+// Jtr rows (computeJacobian, FrameWarper.cpp:147-167). The frame rows the work item can
+// reach are staged in LDS when they fit.
+enum { MODE_PASS = 0, MODE_WARP = 1 };
+
+template <int M, typename FT, int MODE>
+__device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
+                           const double* __restrict__ D, float* lds, int lds_cap, bool want_lds,
+                           double* out, float* wout, float invalid, KF (*sred)[7])
+{
+   constexpr int NV = 1 + 2 * M;
+   const int nx = R.nx, ny = R.ny, nz = R.nz;
+   const int64_t plane_sz = (int64_t)nx * ny;
+   const int k = ch.k;
+   double Dk[3], Dk1[3];
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      Dk[c] = (c < M) ? D[3 * k + c] : 0.0;
+      Dk1[c] = (c < M) ? D[3 * (k + 1) + c] : 0.0;
+   }
+   const int plane = (int)(ch.p0 / plane_sz);
+   const int q0 = (int)(ch.p0 - plane * plane_sz);
+   const int r0 = q0 / nx, r1 = (int)((ch.p1 - 1 - plane * plane_sz) / nx);
+
+   // Rows (and planes) the work item can reach: d is a convex combination of D[k], D[k+1].
+   bool use = false;
+   int za = 0, zb = 0, ya = 0, yb = -1;
+   if (want_lds) {
+      double ylo = fmin(Dk[1], Dk1[1]), yhi = fmax(Dk[1], Dk1[1]);
+      double zlo = fmin(Dk[2], Dk1[2]), zhi = fmax(Dk[2], Dk1[2]);
+      if (fabs(ylo) < 1e6 && fabs(yhi) < 1e6 && fabs(zlo) < 1e6 && fabs(zhi) < 1e6) {
+         ya = max(0, (int)floor(r0 + ylo) - 1);
+         yb = min(ny - 1, (int)floor(r1 + yhi) + 2);
+         if (M == 3) {
+            za = max(0, (int)floor(plane + zlo) - 1);
+            zb = min(nz - 1, (int)floor(plane + zhi) + 2);
+         }
+         if (yb >= ya && zb >= za)
+            use = (int64_t)(zb - za + 1) * (yb - ya + 1) * nx <= lds_cap;
+      }
+   }
+   const int nrows = yb - ya + 1;
+   if (use) {
+      const int psz = nrows * nx;
+      for (int zi = 0; zi <= zb - za; zi++) {
+         const int64_t base = ((int64_t)(za + zi) * ny + ya) * nx;
+         for (int i = threadIdx.x; i < psz; i += kBlock) lds[zi * psz + i] = fload<FT>(F, base + i);
+      }
+      __syncthreads();
+   }
+   auto tap = [&](int zz, int yy, int xx) -> float {
+      if ((unsigned)xx >= (unsigned)nx || (unsigned)yy >= (unsigned)ny) return 0.f;
+      if (M == 3 && (unsigned)zz >= (unsigned)nz) return 0.f;
+      if (use && zz >= za && zz <= zb && yy >= ya && yy <= yb)
+         return lds[((zz - za) * nrows + (yy - ya)) * nx + xx];
+      return fload<FT>(F, ((int64_t)zz * ny + yy) * nx + xx);
+   };
+
+   KF acc[NV];
+#pragma unroll
+   for (int v = 0; v < NV; v++) acc[v] = KF{0.f, 0.f};
+
+   PixelWalk w;
+   w.init(q0 + threadIdx.x, nx);
+   int row_y = -1;
+   double row_t = 0;
+   for (int p = ch.p0 + threadIdx.x; p < ch.p1; p += kBlock, w.step(nx)) {
+      double fd;
+      {
+#pragma clang fp contract(off)
+         if (w.y != row_y) {
+            row_y = w.y;
+            row_t = (double)plane * R.scan.interframe + (double)w.y * R.scan.interline;
+         }
+         double t = row_t + (double)w.x * R.scan.pixel;
+         double u = t / R.scan.stack * R.scan.nDm1;
+         fd = (double)(float)(u - trunc(u));
+      }
+      float v = 0.f;
+      {
+         double s[3];
+         {
+#pragma clang fp contract(off)
+#pragma unroll
+            for (int c = 0; c < 3; c++) s[c] = fd * Dk1[c] + (1.0 - fd) * Dk[c];
+            s[0] += (double)w.x;
+            s[1] += (double)w.y;
+            s[2] += (double)plane;
+         }
+         if (fabs(s[0]) < 1e9 && fabs(s[1]) < 1e9 && fabs(s[2]) < 1e9) {
+            double fx = floor(s[0]), fy = floor(s[1]);
+            int x0 = (int)fx, y0 = (int)fy;
+            double wx = s[0] - fx, wy = s[1] - fy;
+            float wx1 = (float)wx, wx0 = (float)(1.0 - wx);
+            float wy1 = (float)wy, wy0 = (float)(1.0 - wy);
+            if (M == 2) {
+               float t00 = tap(0, y0, x0), t01 = tap(0, y0, x0 + 1);
+               float t10 = tap(0, y0 + 1, x0), t11 = tap(0, y0 + 1, x0 + 1);
+               v = wy0 * (wx0 * t00 + wx1 * t01) + wy1 * (wx0 * t10 + wx1 * t11);
+            } else {
+               double fz = floor(s[2]);
+               int z0 = (int)fz;
+               double wz = s[2] - fz;
+               float wz1 = (float)wz, wz0 = (float)(1.0 - wz);
+               float t000 = tap(z0, y0, x0), t001 = tap(z0, y0, x0 + 1);
+               float t010 = tap(z0, y0 + 1, x0), t011 = tap(z0, y0 + 1, x0 + 1);
+               float t100 = tap(z0 + 1, y0, x0), t101 = tap(z0 + 1, y0, x0 + 1);
+               float t110 = tap(z0 + 1, y0 + 1, x0), t111 = tap(z0 + 1, y0 + 1, x0 + 1);
+               float v0 = wy0 * (wx0 * t000 + wx1 * t001) + wy1 * (wx0 * t010 + wx1 * t011);
+               float v1 = wy0 * (wx0 * t100 + wx1 * t101) + wy1 * (wx0 * t110 + wx1 * t111);
+               v = wz0 * v0 + wz1 * v1;
+            }
+         }
+      }
+      if (MODE == MODE_WARP) {
+         wout[p] = (v == 0.f) ? invalid : v;
+         continue;
+      }
+      if (v == 0.f) continue;  // no sample: residual 0, outside the SSD
+      const float* rg = R.rg + (int64_t)p * (1 + M);
+      float e = v - rg[0];
+      kadd(acc[0], e * e);
+      if (p != ch.end_p) {
+#pragma unroll
+         for (int c = 0; c < M; c++) {
+            float a = (float)((double)rg[1 + c] * (1.0 - fd));
+            float b = (float)((double)rg[1 + c] * fd);
+            kadd(acc[1 + c], a * e);
+            kadd(acc[1 + M + c], b * e);
+         }
+      }
+   }
+   if (MODE == MODE_PASS) block_reduce_store<NV>(acc, (KF(*)[NV])sred, out);
+   else __syncthreads();
+}
+
+template <int MODE>
+__device__ __forceinline__ void dispatch_chunk(const RefDev& R, const FrameDev& F,
+                                               const Chunk ch, const double* D, float* lds,
+                                               int lds_cap, bool use_lds, double* out,
+                                               float* wout, float invalid, KF (*sred)[7])
+{
+   if (R.n_dim == 2) {
+      if (F.dtype == FA_U16)
+         pass_chunk<2, uint16_t, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid, sred);
+      else
+         pass_chunk<2, float, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid, sred);
+   } else {
+      if (F.dtype == FA_U16)
+         pass_chunk<3, uint16_t, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid, sred);
+      else
+         pass_chunk<3, float, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid, sred);
+   }
+}
+
+// Persistent over the active work list: workgroup b takes a contiguous share of the items,
+// with workgroups renumbered so that each XCD (workgroups are dealt to the 8 XCDs round
+// robin) owns one contiguous range and the frame rows it stages stay in its L2.
+__global__ __launch_bounds__(kBlock) void k_pass(PassArgs a)
+{
+   extern __shared__ float lds[];
+   __shared__ KF sred[kBlock / 64][7];
+   const int64_t W = a.counters->n_active_work;
+   const int nfa = a.counters->n_active_frames;
+   if (W <= 0 || nfa <= 0) return;
+   const int G = gridDim.x;
+   int lb = blockIdx.x;
+   if (G % 8 == 0) lb = (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+   const int64_t w0 = W * lb / G, w1 = W * (lb + 1) / G;
+   int j = 0;
+   int64_t jb = 0, je = -1;
+   for (int64_t wi = w0; wi < w1; wi++) {
+      if (wi >= je || wi < jb) {
+         int lo = 0, hi = nfa - 1;
+         while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (a.active_base[mid] <= wi) lo = mid;
+            else hi = mid - 1;
+         }
+         j = lo;
+         jb = a.active_base[j];
+         je = a.active_base[j + 1];
+      }
+      const BatchFrame& bf = a.frames[a.active_frames[j]];
+      const RefDev& R = a.refs[bf.ref];
+      const int c = (int)(wi - jb);
+      const Chunk ch = R.chunks[c];
+      double* out = a.partials + (bf.work_base + c) * kPassStride;
+      dispatch_chunk<MODE_PASS>(R, bf.f, ch, a.Dtrial + bf.d_base, lds, a.lds_cap, a.use_lds != 0,
+                                out, nullptr, 0.f, sred);
+   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_warp(RefDev R, FrameDev F, const double* D,
+                                                 float* out, float invalid, int lds_cap,
+                                                 int use_lds)
+{
+   extern __shared__ float lds[];
+   __shared__ KF sred[kBlock / 64][7];
+   for (int c = blockIdx.x; c < R.n_chunks; c += gridDim.x) {
+      dispatch_chunk<MODE_WARP>(R, F, R.chunks[c], D, lds, lds_cap, use_lds != 0, nullptr, out,
+                                invalid, sred);
+   }
+}
+
+// ---- scheduling: compact the active frames and their work items -----------------------
+// account = 0 only refreshes the counters the host polls.
+__global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
+{
+   __shared__ int64_t s_work[1024];
+   __shared__ int32_t s_fr[1024];
+   __shared__ unsigned long long s_bytes[1024];
+   const int t = threadIdx.x, T = blockDim.x;
+   const int n = B.n_frames;
+   const int f0 = (int)((int64_t)n * t / T), f1 = (int)((int64_t)n * (t + 1) / T);
+   int64_t work = 0;
+   int32_t fr = 0;
+   unsigned long long bytes = 0;
+   for (int f = f0; f < f1; f++) {
+      if (B.state[f].done) continue;
+      const RefDev& R = B.refs[B.frames[f].ref];
+      work += R.n_chunks;
+      fr++;
+      bytes += (unsigned long long)R.n_px *
+               (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + 4 * (1 + R.n_dim));
+   }
+   s_work[t] = work;
+   s_fr[t] = fr;
+   s_bytes[t] = bytes;
+   __syncthreads();
+   // inclusive scan (Hillis-Steele)
+   for (int off = 1; off < T; off <<= 1) {
+      int64_t aw = t >= off ? s_work[t - off] : 0;
+      int32_t af = t >= off ? s_fr[t - off] : 0;
+      unsigned long long ab = t >= off ? s_bytes[t - off] : 0;
+      __syncthreads();
+      s_work[t] += aw;
+      s_fr[t] += af;
+      s_bytes[t] += ab;
+      __syncthreads();
+   }
+   int64_t wbase = s_work[t] - work;
+   int32_t fbase = s_fr[t] - fr;
+   for (int f = f0; f < f1; f++) {
+      if (B.state[f].done) continue;
+      B.active_frames[fbase] = f;
+      B.active_base[fbase] = wbase;
+      wbase += B.refs[B.frames[f].ref].n_chunks;
+      fbase++;
+   }
+   if (t == T - 1) {
+      B.active_base[s_fr[t]] = s_work[t];
+      B.counters->n_active_frames = s_fr[t];
+      B.counters->n_active_work = s_work[t];
+      if (account) {
+         B.counters->bytes += s_bytes[t];
+         B.counters->rounds += 1;
+      }
+   }
+}
+
+// ---- per-frame reductions of the pass partials ------------------------------------------
+// SSD: lanes take chunks strided, then a fixed tree; Jtr: one node per lane, chunks of
+// interval k-1 (weight f) and interval k (weight 1-f) in order. Fixed order, so the result
+// does not depend on which other frames share the launch.
+__device__ double reduce_ssd(const RefDev& R, const double* P, double* sh)
+{
+   double s = 0;
+   for (int c = threadIdx.x; c < R.n_chunks; c += blockDim.x) s += P[(int64_t)c * kPassStride];
+   sh[threadIdx.x] = s;
+   __syncthreads();
+   for (int off = blockDim.x / 2; off >= 1; off >>= 1) {
+      if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+      __syncthreads();
+   }
+   double r = sh[0];
+   __syncthreads();
+   return r;
+}
+
+__device__ void reduce_jtr(const RefDev& R, const double* P, double* g)
+{
+   const int m = R.n_dim;
+   for (int k = threadIdx.x; k < R.nD; k += blockDim.x) {
+      double acc[3] = {0, 0, 0};
+      if (k >= 1)
+         for (int c = R.ichunk[k - 1]; c < R.ichunk[k]; c++)
+            for (int d = 0; d < m; d++) acc[d] += P[(int64_t)c * kPassStride + 1 + m + d];
+      if (k <= R.nD - 2)
+         for (int c = R.ichunk[k]; c < R.ichunk[k + 1]; c++)
+            for (int d = 0; d < m; d++) acc[d] += P[(int64_t)c * kPassStride + 1 + d];
+      for (int d = 0; d < m; d++) g[k * m + d] = acc[d];
+   }
+}
+
+__global__ __launch_bounds__(64) void k_reduce_eval(BatchDev B)
+{
+   __shared__ double sh[64];
+   const int j = blockIdx.x;
+   const BatchFrame& bf = B.frames[j];
+   const RefDev& R = B.refs[bf.ref];
+   const double* P = B.partials + bf.work_base * kPassStride;
+   double f = reduce_ssd(R, P, sh);
+   double* g = B.vec + bf.vec_base + (int64_t)bf.n;
+   reduce_jtr(R, P, g);
+   if (threadIdx.x == 0) B.state[j].f = f;
+}
+
+// ---- device trust region ---------------------------------------------------------------
+// dlib find_min_trust_region + objective_delta_stop_strategy, as the reference calls it
+// (FrameWarpAligner.cpp:205-209), over f = SSD, g = Jtr, B = H (OptimisationModel.cpp:45-63).
+// The subproblem is Nocedal & Wright Alg. 4.3 on the block-tridiagonal H.
+
+struct TrScratch {
+   double *x, *p, *g, *t1, *t2, *Ld, *Ls;
+};
+
+__device__ int d_tr_subproblem(const RefDev& R, const TrScratch& S, const double* g,
+                               double radius, double eps, int max_iter, double* p,
+                               double* lam_out)
+{
+   const int m = R.n_dim, nb = R.nD, n = nb * m;
+   double gnorm = d_norm(n, g);
+   for (int i = 0; i < n; i++) p[i] = 0;
+   *lam_out = 0;
+   if (gnorm == 0) return 1;
+   const double eig_lo = R.eig_lo;
+   double lam = 0, lam_lo = 0, lam_hi = gnorm / radius + (eig_lo < 0 ? -eig_lo : 0);
+   const double *Ld = nullptr, *Ls = nullptr;
+   int it;
+   for (it = 0; it < max_iter; it++) {
+      bool ok;
+      if (lam == 0 && R.h_pd) {
+         Ld = R.L0d;
+         Ls = R.L0s;
+         ok = true;
+      } else if (lam == 0) {
+         ok = false;
+      } else {
+         ok = d_bt_chol(nb, m, R.Hdiag, R.Hsub, lam, S.Ld, S.Ls);
+         Ld = S.Ld;
+         Ls = S.Ls;
+      }
+      if (!ok) {
+         lam_lo = lam;
+         lam = 0.5 * (lam_lo + lam_hi);
+         continue;
+      }
+      for (int i = 0; i < n; i++) S.t2[i] = -g[i];
+      d_bt_fwd(nb, m, Ld, Ls, S.t2, S.t1);
+      d_bt_bwd(nb, m, Ld, Ls, S.t1, p);
+      *lam_out = lam;
+      double pn = d_norm(n, p);
+      if (lam == 0 && pn <= radius) {
+         it++;
+         break;
+      }
+      if (fabs(pn - radius) <= eps * radius) {
+         it++;
+         break;
+      }
+      if (pn > radius) lam_lo = lam;
+      else lam_hi = lam;
+      d_bt_fwd(nb, m, Ld, Ls, p, S.t1);
+      double qn = d_norm(n, S.t1);
+      double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
+      if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
+      lam = lam_new;
+   }
+   return it;
+}
+
+__device__ double d_quad(const RefDev& R, const double* p)
+{
+   const int m = R.n_dim, mm = m * m, nb = R.nD;
+   double s = 0;
+   for (int k = 0; k < nb; k++) {
+      for (int a = 0; a < m; a++)
+         for (int b = 0; b < m; b++) s += p[k * m + a] * R.Hdiag[k * mm + a * m + b] * p[k * m + b];
+      if (k + 1 < nb)
+         for (int a = 0; a < m; a++)
+            for (int b = 0; b < m; b++)
+               s += 2.0 * p[(k + 1) * m + a] * R.Hsub[k * mm + a * m + b] * p[k * m + b];
+   }
+   return s;
+}
+
+__device__ TrScratch scratch_of(const BatchDev& B, const BatchFrame& bf)
+{
+   TrScratch S;
+   double* v = B.vec + bf.vec_base;
+   const int64_t n = bf.n;
+   S.x = B.x + bf.x_base;
+   S.p = v;
+   S.g = v + n;
+   S.t1 = v + 2 * n;
+   S.t2 = v + 3 * n;
+   S.Ld = B.fac + bf.fac_base;
+   const RefDev& R = B.refs[bf.ref];
+   S.Ls = S.Ld + (int64_t)R.nD * R.n_dim * R.n_dim;
+   return S;
+}
+
+// Starts every frame: x (already uploaded) is the first evaluation point.
+__global__ void k_tr_init(BatchDev B, int eval_only)
+{
+   const int j = blockIdx.x;
+   const BatchFrame& bf = B.frames[j];
+   const RefDev& R = B.refs[bf.ref];
+   TrScratch S = scratch_of(B, bf);
+   double* Dt = B.Dtrial + bf.d_base;
+   const int m = R.n_dim;
+   if (!eval_only)
+      for (int i = threadIdx.x; i < R.nD; i += blockDim.x)
+         for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < m ? S.x[i * m + c] : 0.0;
+   if (threadIdx.x == 0) {
+      TrState st = {};
+      st.radius = B.prm.radius;
+      st.radius0 = B.prm.radius;
+      st.phase = 0;
+      B.state[j] = st;
+   }
+}
+
+__global__ __launch_bounds__(64) void k_tr_update(BatchDev B)
+{
+   __shared__ double sh[64];
+   __shared__ int s_accept;
+   const int j = blockIdx.x;
+   if (B.state[j].done) return;
+   const BatchFrame& bf = B.frames[j];
+   const RefDev& R = B.refs[bf.ref];
+   const TrParamsDev& prm = B.prm;
+   TrScratch S = scratch_of(B, bf);
+   const double* P = B.partials + bf.work_base * kPassStride;
+   const int n = bf.n, m = R.n_dim;
+   const double f_new = reduce_ssd(R, P, sh);
+
+   TrState st = B.state[j];
+   // Decide on the evaluated point (lane 0), then reduce Jtr on all lanes if it is kept.
+   if (threadIdx.x == 0) {
+      int accept = 0;
+      if (st.phase == 0) {
+         st.f = f_new;
+         st.evaluations = 1;
+         st.stale = 0;
+         accept = 2;  // first point: take f and g, no step bookkeeping
+      } else {
+         st.evaluations++;
+         double gp = 0;
+         for (int i = 0; i < n; i++) gp += S.g[i] * S.p[i];
+         double pred = -(0.5 * d_quad(R, S.p) + gp);
+         double meas = st.f - f_new;
+         if (fabs(pred) <= fabs(meas) * DBL_EPSILON) {
+            st.status = FA_STOP_NO_PROGRESS;
+            st.done = 1;
+         } else {
+            double rho = meas / fabs(pred);
+            if (!isfinite(rho)) {
+               st.status = FA_STOP_NONFINITE;
+               st.done = 1;
+            } else {
+               int collapse = 0;
+               if (rho < 0.25) {
+                  st.radius *= 0.25;
+                  if (st.radius <= st.radius0 * DBL_EPSILON) collapse = 1;
+               } else if (rho > 0.75 && st.sub_iters > 1) {
+                  st.radius = 2 * st.radius < 1000 ? 2 * st.radius : 1000;
+               }
+               if (rho > 0) {
+                  for (int i = 0; i < n; i++) S.x[i] = S.x[i] + S.p[i];
+                  st.f = f_new;
+                  st.stale = 0;
+                  accept = 1;
+               } else {
+                  st.stale = 1;
+                  st.rejected++;
+                  if (prm.skip_repeat && st.lambda == 0 && !collapse) {
+                     double pn = d_norm(n, S.p);
+                     while (pn <= st.radius) {
+                        st.radius *= 0.25;
+                        st.rejected++;
+                        if (st.radius <= st.radius0 * DBL_EPSILON) {
+                           collapse = 1;
+                           break;
+                        }
+                     }
+                  }
+               }
+               if (collapse) {
+                  st.status = FA_STOP_RADIUS;
+                  st.done = 1;
+               }
+            }
+         }
+      }
+      s_accept = accept;
+   }
+   __syncthreads();
+   if (s_accept) reduce_jtr(R, P, S.g);
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      if (!st.done && !st.stale) {
+         st.iterations++;
+         if (st.been_used) {
+            double df = fabs(st.f - st.prev_f);
+            if (prm.max_iter && st.iterations > prm.max_iter) {
+               st.status = FA_STOP_MAX_ITER;
+               st.done = 1;
+            } else if (df < prm.min_delta) {
+               st.status = FA_STOP_ABS_DELTA;
+               st.done = 1;
+            } else if (prm.rel_delta > 0 && df < prm.rel_delta * fabs(st.prev_f)) {
+               st.status = FA_STOP_REL_DELTA;
+               st.done = 1;
+            }
+         }
+         st.been_used = 1;
+         st.prev_f = st.f;
+      }
+      if (!st.done && prm.max_evals && st.evaluations >= prm.max_evals) {
+         st.status = FA_STOP_MAX_EVALS;
+         st.done = 1;
+      }
+      if (!st.done) {
+         double lam;
+         st.sub_iters = d_tr_subproblem(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
+         st.lambda = lam;
+         st.phase = 1;
+      }
+      B.state[j] = st;
+   }
+   __syncthreads();
+   if (!B.state[j].done) {
+      double* Dt = B.Dtrial + bf.d_base;
+      for (int i = threadIdx.x; i < R.nD; i += blockDim.x)
+         for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < m ? S.x[i * m + c] + S.p[i * m + c] : 0.0;
+   }
+}
+
+// ---- launchers ----------------------------------------------------------------------------
+
+static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_node_map(const ScanDev& sc, int nx, int ny, int64_t n, uint16_t* Di, float* Df,
+                           hipStream_t s)
+{
+   hipLaunchKernelGGL(k_node_map, dim3(blocks_for(n, 256)), dim3(256), 0, s, sc, nx, ny, n, Di, Df);
+   return hipGetLastError();
+}
+
+hipError_t launch_interval_begin(const uint16_t* Di, int64_t n, int nD, int32_t* begin,
+                                 int32_t* flags, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_interval_begin, dim3(blocks_for(n, 256)), dim3(256), 0, s, Di, n, nD, begin,
+                      flags);
+   return hipGetLastError();
+}
+
+hipError_t launch_sobel(const float* T, int nz, int ny, int nx, int n_dim, float* rg, hipStream_t s)
+{
+   int64_t n = (int64_t)nz * ny * nx;
+   hipLaunchKernelGGL(k_sobel, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, nz, ny, nx, n_dim, rg);
+   return hipGetLastError();
+}
+
+hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, double* Hsub,
+                          double* L0d, double* L0s, double* info, hipStream_t s)
+{
+   unsigned g = (unsigned)std::min<int64_t>(R.n_chunks, 65535);
+   hipLaunchKernelGGL(k_hess_partials, dim3(g), dim3(kBlock), 0, s, R, partials);
+   hipError_t e = hipGetLastError();
+   if (e != hipSuccess) return e;
+   hipLaunchKernelGGL(k_hess_assemble, dim3(1), dim3(256), 0, s, R, (const double*)partials, Hdiag,
+                      Hsub, L0d, L0s, info);
+   return hipGetLastError();
+}
+
+hipError_t launch_pass(const PassArgs& a, int grid, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_pass, dim3(grid), dim3(kBlock), (size_t)a.lds_cap * sizeof(float), s, a);
+   return hipGetLastError();
+}
+
+hipError_t launch_warp(const RefDev& R, const FrameDev& F, const double* D, float* out,
+                       float invalid, int lds_cap, int use_lds, int grid, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_warp, dim3(grid), dim3(kBlock), (size_t)lds_cap * sizeof(float), s, R, F, D,
+                      out, invalid, lds_cap, use_lds);
+   return hipGetLastError();
+}
+
+hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_schedule, dim3(1), dim3(1024), 0, s, B, account);
+   return hipGetLastError();
+}
+
+hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_tr_init, dim3(B.n_frames), dim3(64), 0, s, B, eval_only);
+   return hipGetLastError();
+}
+
+hipError_t launch_tr_update(const BatchDev& B, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_tr_update, dim3(B.n_frames), dim3(64), 0, s, B);
+   return hipGetLastError();
+}
+
+hipError_t launch_reduce_eval(const BatchDev& B, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_reduce_eval, dim3(B.n_frames), dim3(64), 0, s, B);
+   return hipGetLastError();
+}
+
+}  // namespace fa

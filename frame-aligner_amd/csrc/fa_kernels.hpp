@@ -1,0 +1,23 @@
+// Launchers of the gfx950 kernels (fa_kernels.hip).
+#pragma once
+
+#include "fa_internal.hpp"
+
+namespace fa {
+
+hipError_t launch_node_map(const ScanDev& sc, int nx, int ny, int64_t n, uint16_t* Di, float* Df,
+                           hipStream_t s);
+hipError_t launch_interval_begin(const uint16_t* Di, int64_t n, int nD, int32_t* begin,
+                                 int32_t* flags, hipStream_t s);
+hipError_t launch_sobel(const float* T, int nz, int ny, int nx, int n_dim, float* rg, hipStream_t s);
+hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, double* Hsub,
+                          double* L0d, double* L0s, double* info, hipStream_t s);
+hipError_t launch_pass(const PassArgs& a, int grid, hipStream_t s);
+hipError_t launch_warp(const RefDev& R, const FrameDev& F, const double* D, float* out,
+                       float invalid, int lds_cap, int use_lds, int grid, hipStream_t s);
+hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s);
+hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s);
+hipError_t launch_tr_update(const BatchDev& B, hipStream_t s);
+hipError_t launch_reduce_eval(const BatchDev& B, hipStream_t s);
+
+}  // namespace fa

@@ -1,0 +1,395 @@
+"""Python front end of the gfx950 warp-registration library (libfa_hip.so).
+
+Mirrors the reference's warper interface (FrameAligner/FrameWarper.h:95-143,
+GpuFrameWarper.h:31-72): setReference, registerFrame / deregisterFrame,
+getError, getJacobian, warpImageInterpolated, warpPoint, H; plus the device
+trust-region loop of FrameWarpAligner::addFrame (FrameWarpAligner.cpp:200-215)
+for one frame (align) or many (Batch).
+
+Everything runs through the C ABI of include/frame_aligner.h. There is no CPU
+fallback: if the HIP library is missing or no gfx950 GPU is present, the
+calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libfa_hip.so")
+
+FA_F32, FA_U16 = 0, 1
+STOP_NAMES = {1: "abs_delta", 2: "rel_delta", 3: "max_iter", 4: "no_progress", 5: "nonfinite",
+              6: "radius", 7: "max_evals"}
+
+
+class FaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+class ScanParams(C.Structure):
+    """ImageScanParameters constructor arguments (ImageScanParameters.h:7-20)."""
+    _fields_ = [("line_duration", C.c_double), ("interline_duration", C.c_double),
+                ("interframe_duration", C.c_double), ("n_x", C.c_int32), ("n_y", C.c_int32),
+                ("n_z", C.c_int32), ("bidirectional", C.c_int32)]
+
+
+class TrParams(C.Structure):
+    _fields_ = [("radius", C.c_double), ("min_delta", C.c_double), ("rel_delta", C.c_double),
+                ("max_iter", C.c_int32), ("max_evals", C.c_int32), ("sub_eps", C.c_double),
+                ("sub_max_iter", C.c_int32), ("skip_repeat", C.c_int32)]
+
+
+class AlignStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("evaluations", C.c_int32), ("rejected", C.c_int32),
+                ("status", C.c_int32), ("f_final", C.c_double), ("radius_final", C.c_double)]
+
+
+class BatchTiming(C.Structure):
+    _fields_ = [("wall_ms", C.c_double), ("pass_ms", C.c_double), ("pass_launches", C.c_int64),
+                ("pass_bytes", C.c_double), ("tr_rounds", C.c_int64)]
+
+
+EXPORTS = [
+    "fa_last_error", "fa_tr_default", "fa_create", "fa_destroy", "fa_set_reference",
+    "fa_reference_release", "fa_reference_info", "fa_get_hessian", "fa_get_node_map",
+    "fa_get_gradient", "fa_register_frame", "fa_deregister_frame", "fa_get_error",
+    "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_point", "fa_align", "fa_batch_create",
+    "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_run", "fa_batch_eval", "fa_align_batch",
+    "fa_context_stream",
+]
+
+_lib = None
+
+
+def load_library(path=None):
+    """Loads libfa_hip.so (built by frame-aligner_amd/build.sh); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise FaError(-4, f"HIP library not built: {path} (run frame-aligner_amd/build.sh)")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    i32 = C.c_int32
+    L.fa_last_error.restype = C.c_char_p
+    L.fa_tr_default.argtypes = [C.POINTER(TrParams)]
+    L.fa_tr_default.restype = None
+    L.fa_create.argtypes = [C.c_int, C.POINTER(P)]
+    L.fa_destroy.argtypes = [P]
+    L.fa_set_reference.argtypes = [P, P, i32, i32, i32, i32, C.POINTER(ScanParams), i32, C.POINTER(P)]
+    L.fa_reference_release.argtypes = [P]
+    L.fa_reference_info.argtypes = [P, P]
+    L.fa_get_hessian.argtypes = [P, P]
+    L.fa_get_node_map.argtypes = [P, P, P, P]
+    L.fa_get_gradient.argtypes = [P, P, P, P]
+    L.fa_register_frame.argtypes = [P, P, i32, C.c_float, C.POINTER(P)]
+    L.fa_deregister_frame.argtypes = [P]
+    L.fa_get_error.argtypes = [P, P, P]
+    L.fa_get_jacobian.argtypes = [P, P, P, P]
+    L.fa_warp_interpolated.argtypes = [P, P, P, C.c_float]
+    L.fa_warp_point.argtypes = [P, P, i32, i32, i32, i32, P]
+    L.fa_align.argtypes = [P, C.POINTER(TrParams), P, C.POINTER(AlignStats)]
+    L.fa_batch_create.argtypes = [P, i32, C.POINTER(TrParams), C.POINTER(P)]
+    L.fa_batch_destroy.argtypes = [P]
+    L.fa_batch_set_timing.argtypes = [P, i32]
+    L.fa_batch_run.argtypes = [P, P, P, C.POINTER(BatchTiming)]
+    L.fa_batch_eval.argtypes = [P, P, P, P]
+    L.fa_align_batch.argtypes = [P, i32, C.POINTER(TrParams), P, P]
+    L.fa_context_stream.argtypes = [P]
+    L.fa_context_stream.restype = P
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise FaError(rc, _lib.fa_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def tr_params(**kw):
+    """Trust-region controls; defaults are the reference's call (radius 40, |df| < 1e-8)."""
+    L = load_library()
+    p = TrParams()
+    L.fa_tr_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def stats_dict(s):
+    d = {f: getattr(s, f) for f, _ in AlignStats._fields_}
+    d["stop"] = STOP_NAMES.get(d["status"], "?")
+    return d
+
+
+class Context:
+    """One GPU (GpuFrameWarper's device, GpuFrameWarper.cpp:10-44)."""
+
+    def __init__(self, device=0):
+        self.L = load_library()
+        h = C.c_void_p()
+        _check(self.L.fa_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return self.L.fa_context_stream(self.h)
+
+
+class Reference:
+    """Per-acquisition state resident in HBM: reference, Sobel gradient, node tables,
+    block-tridiagonal JtJ and its factor (AbstractFrameWarper::setReference)."""
+
+    def __init__(self, ctx, reference, nD, line=1.0, interline=1.0, interframe=None, chunk_px=0):
+        self.ctx = ctx
+        self.L = ctx.L
+        ref = np.ascontiguousarray(reference, dtype=np.float32)
+        if ref.ndim == 2:
+            ref = ref[None]
+        nz, ny, nx = ref.shape
+        if interframe is None:
+            interframe = ny * max(interline, line)
+        sp = ScanParams(line, interline, interframe, nx, ny, nz, 0)
+        h = C.c_void_p()
+        _check(self.L.fa_set_reference(ctx.h, _ptr(ref), nz, ny, nx, nD, C.byref(sp), chunk_px, C.byref(h)))
+        self.h = h
+        info = np.zeros(6, np.int32)
+        _check(self.L.fa_reference_info(self.h, _ptr(info)))
+        self.shape = (nz, ny, nx)
+        self.nD = nD
+        self.n_dim = int(info[4])
+        self.n_chunks = int(info[5])
+        self.n_px = nz * ny * nx
+
+    def release(self):
+        if getattr(self, "h", None):
+            self.L.fa_reference_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    @property
+    def H(self):
+        n = self.nD * self.n_dim
+        H = np.zeros((n, n))
+        _check(self.L.fa_get_hessian(self.h, _ptr(H)))
+        return H
+
+    def node_map(self):
+        Di = np.zeros(self.n_px, np.uint16)
+        Df = np.zeros(self.n_px, np.float32)
+        Dr = np.zeros((self.nD, 2), np.int32)
+        _check(self.L.fa_get_node_map(self.h, _ptr(Di), _ptr(Df), _ptr(Dr)))
+        return Di, Df, Dr
+
+    def gradient(self):
+        g = [np.zeros(self.n_px, np.float32) for _ in range(3)]
+        _check(self.L.fa_get_gradient(self.h, *[_ptr(a) for a in g]))
+        return [a.reshape(self.shape) for a in g]
+
+    def warp_point(self, D, x, y, z, spatial_binning=1):
+        D = as_D(D, self.nD, self.n_dim)
+        o = np.zeros(3)
+        _check(self.L.fa_warp_point(self.h, _ptr(D), x, y, z, spatial_binning, _ptr(o)))
+        return o
+
+    def register(self, frame, offset=None):
+        return Frame(self, frame, offset)
+
+
+def as_D(D, nD, n_dim):
+    """nD x 3 doubles from an nD x 3 array or a D2col vector (OptimisationModel.cpp:14-42)."""
+    D = np.asarray(D, np.float64)
+    if D.ndim == 1:
+        x = D.reshape(nD, n_dim)
+        D = np.zeros((nD, 3))
+        D[:, :n_dim] = x
+    return np.ascontiguousarray(D)
+
+
+class Frame:
+    """A frame resident in HBM (GpuFrameWarper::registerFrame). uint16 frames are kept as
+    counts and sampled as count + offset; offset defaults to 1 for uint16 (the aligner's
+    +1.0f, FrameWarpAligner.cpp:142) and 0 for float frames (already offset)."""
+
+    def __init__(self, ref, frame, offset=None):
+        self.ref = ref
+        self.L = ref.L
+        a = np.ascontiguousarray(frame)
+        if a.dtype == np.uint16:
+            dt = FA_U16
+            off = 1.0 if offset is None else offset
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            dt = FA_F32
+            off = 0.0 if offset is None else offset
+        if a.size != ref.n_px:
+            raise FaError(-1, "Frame is incorrect size")
+        h = C.c_void_p()
+        _check(self.L.fa_register_frame(ref.h, _ptr(a), dt, float(off), C.byref(h)))
+        self.h = h
+
+    def release(self):
+        if getattr(self, "h", None):
+            self.L.fa_deregister_frame(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def error(self, D):
+        D = as_D(D, self.ref.nD, self.ref.n_dim)
+        e = C.c_double()
+        _check(self.L.fa_get_error(self.h, _ptr(D), C.byref(e)))
+        return e.value
+
+    def jacobian(self, D):
+        D = as_D(D, self.ref.nD, self.ref.n_dim)
+        jac = np.zeros(self.ref.nD * self.ref.n_dim)
+        e = C.c_double()
+        _check(self.L.fa_get_jacobian(self.h, _ptr(D), _ptr(jac), C.byref(e)))
+        return e.value, jac
+
+    def warp_interpolated(self, D, invalid_value=0.0):
+        D = as_D(D, self.ref.nD, self.ref.n_dim)
+        out = np.zeros(self.ref.n_px, np.float32)
+        _check(self.L.fa_warp_interpolated(self.h, _ptr(D), _ptr(out), float(invalid_value)))
+        return out.reshape(self.ref.shape)
+
+    def align(self, x0=None, **kw):
+        n = self.ref.nD * self.ref.n_dim
+        x = np.zeros(n) if x0 is None else np.array(x0, np.float64).reshape(n).copy()
+        p = tr_params(**kw)
+        s = AlignStats()
+        _check(self.L.fa_align(self.h, C.byref(p), _ptr(x), C.byref(s)))
+        return x, stats_dict(s)
+
+
+class Batch:
+    """Many frames aligned together: every trust-region round is one fused pass over all
+    active frames plus one device update."""
+
+    def __init__(self, frames, **kw):
+        self.frames = list(frames)
+        self.L = self.frames[0].L
+        arr = (C.c_void_p * len(self.frames))(*[f.h.value for f in self.frames])
+        self._arr = arr
+        p = tr_params(**kw)
+        h = C.c_void_p()
+        _check(self.L.fa_batch_create(arr, len(self.frames), C.byref(p), C.byref(h)))
+        self.h = h
+        self.sizes = [f.ref.nD * f.ref.n_dim for f in self.frames]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on=True):
+        _check(self.L.fa_batch_set_timing(self.h, 1 if on else 0))
+
+    def run(self, x0=None):
+        x = np.zeros(self.offsets[-1]) if x0 is None else np.concatenate(
+            [np.asarray(v, np.float64).ravel() for v in x0])
+        st = (AlignStats * len(self.frames))()
+        tm = BatchTiming()
+        _check(self.L.fa_batch_run(self.h, _ptr(x), st, C.byref(tm)))
+        xs = [x[self.offsets[i]:self.offsets[i + 1]].copy() for i in range(len(self.frames))]
+        timing = {f: getattr(tm, f) for f, _ in BatchTiming._fields_}
+        return xs, [stats_dict(s) for s in st], timing
+
+    def eval(self, Ds):
+        D = np.concatenate([as_D(d, f.ref.nD, f.ref.n_dim).ravel() for d, f in zip(Ds, self.frames)])
+        err = np.zeros(len(self.frames))
+        jac = np.zeros(self.offsets[-1])
+        _check(self.L.fa_batch_eval(self.h, _ptr(D), _ptr(err), _ptr(jac)))
+        return err, [jac[self.offsets[i]:self.offsets[i + 1]].copy() for i in range(len(self.frames))]
+
+
+class HipFrameWarper:
+    """The reference's AbstractFrameWarper interface (FrameWarper.h:95-143) on the GPU.
+    Frames are keyed by their data pointer, as the reference keys them by cv::Mat::data
+    (GpuFrameWarper.h:55)."""
+
+    def __init__(self, device=0, ctx=None):
+        self.ctx = ctx or Context(device)
+        self.ref = None
+        self.frames = {}
+
+    def setReference(self, reference, nD, line_duration=1.0, interline_duration=1.0,
+                     interframe_duration=None):
+        self.ref = Reference(self.ctx, reference, nD, line_duration, interline_duration,
+                             interframe_duration)
+        self.n_dim = self.ref.n_dim
+        self.dims = list(self.ref.shape)
+        self.nD = nD
+
+    @property
+    def H(self):
+        return self.ref.H
+
+    def _key(self, frame):
+        return frame.__array_interface__["data"][0]
+
+    def registerFrame(self, frame):
+        self.frames[self._key(frame)] = self.ref.register(frame, offset=0.0)
+
+    def deregisterFrame(self, frame):
+        f = self.frames.pop(self._key(frame), None)
+        if f is not None:
+            f.release()
+
+    def _frame(self, frame):
+        f = self.frames.get(self._key(frame))
+        if f is None:
+            raise FaError(-3, "Unrecognised frame")
+        return f
+
+    def getError(self, frame, D):
+        return self._frame(frame).error(D)
+
+    def getJacobian(self, frame, D):
+        return self._frame(frame).jacobian(D)[1]
+
+    def warpImageInterpolated(self, frame, D, invalid_value=0):
+        return self._frame(frame).warp_interpolated(D, invalid_value)
+
+    def warpPoint(self, D, x, y, z, spatial_binning=1):
+        return self.ref.warp_point(D, x, y, z, spatial_binning)
+
+    def clearTemp(self):
+        pass

@@ -1,0 +1,174 @@
+/*
+ * frame_aligner.h - C ABI of the MI355X (gfx950) warp-registration library.
+ *
+ * Drop-in boundary for the non-rigid warp path of flimfit/frame-aligner.
+ * The entry points are what a host written against the reference's
+ * AbstractFrameWarper (FrameAligner/FrameWarper.h:95-143) and its
+ * trust-region caller (FrameAligner/FrameWarpAligner.cpp:117-223,
+ * FrameAligner/OptimisationModel.cpp:45-63) binds: plain pointers and sizes,
+ * no C++ or torch types, status codes instead of exceptions.
+ *
+ * Conventions
+ *  - Volumes are (nz, ny, nx) with x fastest (FrameAligner/Cv3dUtils.cpp:6-8).
+ *    A 2-D image is nz = 1; n_dim = 3 when nz > 1, else 2
+ *    (FrameAligner/FrameWarper.cpp:198).
+ *  - D, the node displacements, is nD x 3 doubles (x, y, z) per node, as the
+ *    reference's std::vector<cv::Point3d>; z is ignored in 2-D.
+ *  - Parameter vectors (trust region x, Jacobian) use the reference's D2col
+ *    layout: [x0, y0, (z0), x1, y1, ...] (FrameAligner/OptimisationModel.cpp:14-27).
+ *  - Every call returns FA_OK (0) or a negative FA_E_* code; fa_last_error()
+ *    returns the calling thread's last message. Calls on distinct frames may
+ *    run from different threads; a reference is read-only after set-up.
+ */
+#ifndef FRAME_ALIGNER_H
+#define FRAME_ALIGNER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FA_OK 0
+#define FA_E_ARG (-1)           /* invalid argument / size ("Frame is incorrect size") */
+#define FA_E_OOM (-2)           /* device allocation failed ("Not enough memory to allocate") */
+#define FA_E_UNKNOWN_FRAME (-3) /* handle not registered ("Unrecognised frame") */
+#define FA_E_DEVICE (-4)        /* HIP runtime error or no supported GPU ("No supported GPU") */
+#define FA_E_NODE_MAP (-5)      /* node map inconsistent ("Something went wrong") */
+#define FA_E_STATE (-6)         /* call out of order */
+
+typedef struct fa_context fa_context;
+typedef struct fa_reference fa_reference;
+typedef struct fa_frame fa_frame;
+typedef struct fa_batch fa_batch;
+
+/* Raw constructor arguments of ImageScanParameters (FrameAligner/ImageScanParameters.h:7-20);
+   the library applies the constructor's fix-ups and processScanParameters
+   (FrameAligner/FrameWarper.cpp:223-236). bidirectional must be 0: the reference never
+   initialises it (ImageScanParameters.h:31) and its bidirectional node map is inconsistent. */
+typedef struct fa_scan_params {
+   double line_duration;
+   double interline_duration;
+   double interframe_duration;
+   int32_t n_x, n_y, n_z;
+   int32_t bidirectional;
+} fa_scan_params;
+
+enum fa_dtype { FA_F32 = 0, FA_U16 = 1 };
+
+/* find_min_trust_region(objective_delta_stop_strategy(min_delta), model, x, radius)
+   (FrameAligner/FrameWarpAligner.cpp:205-209), plus the relative stop and caps the
+   benchmark configurations ask for. */
+typedef struct fa_tr_params {
+   double radius;        /* initial radius (reference: 40) */
+   double min_delta;     /* stop when |f - f_prev| < min_delta (reference: 1e-8) */
+   double rel_delta;     /* stop when |f - f_prev| < rel_delta * |f_prev|; 0 = off */
+   int32_t max_iter;     /* stop-strategy iterations; 0 = unlimited (reference) */
+   int32_t max_evals;    /* cap on objective evaluations; 0 = unlimited */
+   double sub_eps;       /* subproblem boundary tolerance (0.1) */
+   int32_t sub_max_iter; /* subproblem iterations (20) */
+   int32_t skip_repeat;  /* 1: skip re-evaluating a rejected step proposed unchanged */
+} fa_tr_params;
+
+enum fa_stop {
+   FA_STOP_ABS_DELTA = 1,
+   FA_STOP_REL_DELTA = 2,
+   FA_STOP_MAX_ITER = 3,
+   FA_STOP_NO_PROGRESS = 4,
+   FA_STOP_NONFINITE = 5,
+   FA_STOP_RADIUS = 6,
+   FA_STOP_MAX_EVALS = 7
+};
+
+typedef struct fa_align_stats {
+   int32_t iterations;   /* stop-strategy calls */
+   int32_t evaluations;  /* objective evaluations (fused passes) */
+   int32_t rejected;     /* rejected steps */
+   int32_t status;       /* FA_STOP_* */
+   double f_final;       /* SSD at the returned x */
+   double radius_final;
+} fa_align_stats;
+
+/* Device-side timing of a batch run, filled when fa_batch_set_timing(b, 1). */
+typedef struct fa_batch_timing {
+   double wall_ms;       /* host wall time of the run */
+   double pass_ms;       /* sum of fused-pass kernel durations (HIP events on the batch stream) */
+   int64_t pass_launches;
+   double pass_bytes;    /* algorithmic bytes of those launches: active pixels x bytes/pixel */
+   int64_t tr_rounds;    /* launch rounds (pass + update) issued */
+} fa_batch_timing;
+
+const char* fa_last_error(void);
+void fa_tr_default(fa_tr_params* p);
+
+/* Replaces GpuFrameWarper() / getGpuSupportInformation (FrameAligner/GpuFrameWarper.cpp:10-44). */
+int fa_create(int device, fa_context** out);
+int fa_destroy(fa_context* ctx);
+
+/* AbstractFrameWarper::setReference(reference, nD, image_params)
+   (FrameAligner/FrameWarper.cpp:186-205; GPU set-up GpuFrameWarper.cpp:75-152).
+   ref: nz*ny*nx floats, already smoothed and offset by the caller, as the reference's
+   aligner passes it (FrameWarpAligner.cpp:86-102). Builds the node map, Sobel gradient,
+   block-tridiagonal JtJ and its Cholesky factor on the device. chunk_px = 0 picks the
+   default work-item size (pixels per workgroup task). */
+int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, int32_t nx,
+                     int32_t nD, const fa_scan_params* scan, int32_t chunk_px,
+                     fa_reference** out);
+int fa_reference_release(fa_reference* ref);
+/* out[0..5] = nz, ny, nx, nD, n_dim, number of work chunks. */
+int fa_reference_info(const fa_reference* ref, int32_t* out6);
+/* AbstractFrameWarper::H (FrameWarper.h:134), dense (nD*n_dim)^2 row-major. */
+int fa_get_hessian(const fa_reference* ref, double* H);
+/* Node map Di/Df and D_range (FrameWarper.h:132-136); any pointer may be NULL.
+   D_range is nD x (begin, end). */
+int fa_get_node_map(const fa_reference* ref, uint16_t* Di, float* Df, int32_t* D_range);
+/* Sobel gradient of the reference (Sobel3d.cpp:60-84); gz may be NULL in 2-D. */
+int fa_get_gradient(const fa_reference* ref, float* gx, float* gy, float* gz);
+
+/* GpuFrameWarper::registerFrame / deregisterFrame (FrameAligner/GpuFrameWarper.cpp:46-73).
+   data: nz*ny*nx values of dtype in host memory, copied to a resident device slot.
+   Samples are data + add_offset (the aligner's +1.0f, FrameWarpAligner.cpp:142, can be
+   applied here for raw uint16 counts; pass 0 for frames already offset). */
+int fa_register_frame(fa_reference* ref, const void* data, int32_t dtype, float add_offset,
+                      fa_frame** out);
+int fa_deregister_frame(fa_frame* frame);
+
+/* getError (FrameWarper.h:103; CPU FrameWarper.cpp:169-176, GPU GpuFrameWarper.cpp:182-190):
+   SSD of the warped frame against the reference over valid pixels. */
+int fa_get_error(fa_frame* frame, const double* D, double* err);
+/* getJacobian (FrameWarper.h:104; FrameWarper.cpp:147-167,178-182; GpuFrameWarper.cpp:192-235):
+   Jtr at D (nD*n_dim, D2col layout). err (may be NULL) receives the SSD at D. */
+int fa_get_jacobian(fa_frame* frame, const double* D, double* jac, double* err);
+/* warpImageInterpolated (FrameWarper.h:106; FrameWarper.cpp:31-69): out gets nz*ny*nx floats;
+   samples that are exactly zero become invalid_value. */
+int fa_warp_interpolated(fa_frame* frame, const double* D, float* out, float invalid_value);
+/* warpPoint (FrameWarper.h:110; FrameWarper.cpp:397-419). */
+int fa_warp_point(const fa_reference* ref, const double* D, int32_t x, int32_t y, int32_t z,
+                  int32_t spatial_binning, double* out3);
+
+/* The trust-region loop of FrameWarpAligner::addFrame (FrameWarpAligner.cpp:200-215) run
+   on the device for one frame: x_inout (nD*n_dim) is the start and receives the result. */
+int fa_align(fa_frame* frame, const fa_tr_params* prm, double* x_inout, fa_align_stats* stats);
+
+/* Batched form: many frames (any references, sizes and dtypes) aligned together, each
+   trust-region round one fused pass over every active frame plus one device update. */
+int fa_batch_create(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
+                    fa_batch** out);
+int fa_batch_destroy(fa_batch* b);
+int fa_batch_set_timing(fa_batch* b, int32_t enable);
+/* x_inout: concatenated per-frame parameter vectors; stats: n entries (may be NULL). */
+int fa_batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timing* timing);
+/* One fused pass over every frame of the batch at the given D (concatenated nD x 3 per
+   frame): err[n] SSDs and jac (concatenated nD*n_dim per frame); either may be NULL. */
+int fa_batch_eval(fa_batch* b, const double* D, double* err, double* jac);
+int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
+                   double* x_inout, fa_align_stats* stats);
+
+/* The HIP stream the context launches on (hipStream_t), for callers that time it. */
+void* fa_context_stream(fa_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

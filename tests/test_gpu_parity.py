@@ -1,0 +1,180 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle on seeded inputs.
+
+Tolerances: SSD within 1e-4 relative, JtJ / Jtr within 1e-3
+relative, converged node displacements within 0.02 px. Index work (node map,
+interval ranges) and the Sobel gradient, which the reference computes in plain
+float with a fixed operation order, are compared bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import fa_amd as F
+from helpers import rel_err, small_stack
+
+pytestmark = pytest.mark.gpu
+
+ERR_TOL = 1e-4
+JAC_TOL = 1e-3
+D_TOL = 0.02
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = F.Context(0)
+    yield c
+    c.close()
+
+
+CASES = [
+    ("2d", (1, 96, 80), 9, dict(line=1.0, interline=1.0)),
+    ("2d_interline", (1, 64, 72), 13, dict(line=1.0, interline=1.3)),
+    ("3d", (6, 40, 36), 6 * 3 + 1, dict(line=1.0, interline=1.0)),
+]
+
+
+def _pair(ctx, shape, nD, scan, seed=3, **kw):
+    st = small_stack(shape, nD=nD, seed=seed, **kw)
+    orc = O.Oracle().set_reference(st["ref_w"], nD, **scan)
+    ref = F.Reference(ctx, st["ref_w"], nD, **scan)
+    return st, orc, ref
+
+
+@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+def test_node_map_bit_exact(ctx, name, shape, nD, scan):
+    st, orc, ref = _pair(ctx, shape, nD, scan)
+    Di_o, Df_o, Dr_o = orc.node_map()
+    Di_g, Df_g, Dr_g = ref.node_map()
+    assert np.array_equal(Di_o, Di_g)
+    assert np.array_equal(Df_o.astype(np.float32), Df_g)
+    assert np.array_equal(Dr_o, Dr_g)
+
+
+@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+def test_sobel_bit_exact(ctx, name, shape, nD, scan):
+    st, orc, ref = _pair(ctx, shape, nD, scan)
+    go = orc.gradient()
+    gg = ref.gradient()
+    for c in range(ref.n_dim):
+        assert np.array_equal(go[c].astype(np.float32), gg[c]), f"component {c}"
+
+
+@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+def test_hessian(ctx, name, shape, nD, scan):
+    st, orc, ref = _pair(ctx, shape, nD, scan)
+    Ho, Hg = orc.hessian(), ref.H
+    assert rel_err(Hg, Ho) < JAC_TOL
+    # block-tridiagonal structure: zero beyond the first off-diagonal block
+    m = ref.n_dim
+    for i in range(nD):
+        for j in range(nD):
+            if abs(i - j) > 1:
+                assert not Hg[i * m:(i + 1) * m, j * m:(j + 1) * m].any()
+
+
+def _displacements(st, nD, rng):
+    yield np.zeros((nD, 3))
+    yield st["D_true"][0]
+    yield st["D_true"][0] + rng.normal(0, 0.7, (nD, 3))
+    yield rng.normal(0, 4.0, (nD, 3))
+
+
+@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+@pytest.mark.parametrize("dtype", ["float32", "uint16"])
+def test_error_and_jacobian(ctx, name, shape, nD, scan, dtype):
+    st, orc, ref = _pair(ctx, shape, nD, scan, dtype=dtype)
+    rng = np.random.default_rng(11)
+    for f_i in range(len(st["frames"])):
+        fr = ref.register(st["frames"][f_i])  # uint16 counts get +1 on the device
+        fw = st["frames_w"][f_i]
+        for D in _displacements(st, nD, rng):
+            if ref.n_dim == 2:
+                D = D.copy()
+                D[:, 2] = 0
+            e_o, j_o = orc.get_jacobian(fw, D)
+            e_g, j_g = fr.jacobian(D)
+            assert abs(e_g - e_o) <= ERR_TOL * abs(e_o), (e_g, e_o)
+            assert abs(fr.error(D) - e_g) <= 1e-12 * abs(e_g)
+            assert rel_err(j_g, j_o) < JAC_TOL, rel_err(j_g, j_o)
+        fr.release()
+
+
+@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+def test_warp_interpolated(ctx, name, shape, nD, scan):
+    st, orc, ref = _pair(ctx, shape, nD, scan)
+    fr = ref.register(st["frames_w"][0])
+    rng = np.random.default_rng(5)
+    for D in _displacements(st, nD, rng):
+        for invalid in (0.0, -1.0):
+            wo = orc.warp_interpolated(st["frames_w"][0], D, invalid)
+            wg = fr.warp_interpolated(D, invalid)
+            assert np.array_equal(wo == invalid, wg == invalid)
+            assert np.max(np.abs(wg - wo)) <= 1e-5 * np.max(np.abs(wo))
+
+
+def test_zero_and_integer_shift_exact(ctx):
+    st, orc, ref = _pair(ctx, (1, 64, 64), 9, dict(line=1.0, interline=1.0))
+    fr = ref.register(st["ref_w"])
+    assert fr.error(np.zeros((9, 3))) == 0.0
+    D = np.zeros((9, 3))
+    D[:, 0] = 2
+    D[:, 1] = -3
+    w = fr.warp_interpolated(D, -7.0)
+    expect = np.full((1, 64, 64), -7.0, np.float32)
+    expect[0, 3:, :62] = st["ref_w"][0, :61, 2:]
+    assert np.array_equal(w, expect)
+
+
+@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+def test_align_matches_oracle(ctx, name, shape, nD, scan):
+    st, orc, ref = _pair(ctx, shape, nD, scan)
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    for f_i in range(len(st["frames"])):
+        fw = st["frames_w"][f_i]
+        x_o, s_o = orc.align(fw, **kw)
+        fr = ref.register(fw)
+        x_g, s_g = fr.align(**kw)
+        assert np.max(np.abs(x_g - x_o)) <= D_TOL, (np.max(np.abs(x_g - x_o)), s_o, s_g)
+        assert abs(s_g["f_final"] - s_o["f_final"]) <= ERR_TOL * s_o["f_final"]
+        fr.release()
+
+
+def test_batch_same_alone_and_in_mix(ctx):
+    """Each frame's result is bit-identical alone and inside a mixed batch (two references of
+    different size, uint16 and float frames)."""
+    a, orc_a, ref_a = _pair(ctx, (1, 96, 80), 9, dict(line=1.0, interline=1.0), seed=21)
+    b, orc_b, ref_b = _pair(ctx, (1, 64, 72), 13, dict(line=1.0, interline=1.3), seed=22,
+                            dtype="uint16")
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    frames = [ref_a.register(f) for f in a["frames_w"]] + [ref_b.register(f) for f in b["frames"]]
+    alone = [f.align(**kw) for f in frames]
+    batch = F.Batch(frames, **kw)
+    xs, stats, _ = batch.run()
+    for (x1, s1), x2, s2 in zip(alone, xs, stats):
+        assert np.array_equal(x1, x2)
+        assert s1["evaluations"] == s2["evaluations"]
+    # and against the oracle
+    for i, fw in enumerate(a["frames_w"]):
+        x_o, _ = orc_a.align(fw, **kw)
+        assert np.max(np.abs(xs[i] - x_o)) <= D_TOL
+    xs2, _, _ = batch.run()
+    for x1, x2 in zip(xs, xs2):
+        assert np.array_equal(x1, x2)
+
+
+def test_errors_are_reported(ctx):
+    st, orc, ref = _pair(ctx, (1, 64, 64), 9, dict(line=1.0, interline=1.0))
+    with pytest.raises(F.FaError):
+        ref.register(np.zeros((10, 10), np.float32))
+    with pytest.raises(F.FaError):
+        F.Reference(ctx, st["ref_w"], 1)
+    with pytest.raises(F.FaError):
+        F.Reference(ctx, np.zeros((1, 1, 8), np.float32), 4)
+    fr = ref.register(st["frames_w"][0])
+    # displacement far outside: every sample missing, SSD 0, Jtr 0
+    D = np.full((9, 3), 1e7)
+    e, j = fr.jacobian(D)
+    assert e == 0.0 and not j.any()
+    D = np.full((9, 3), np.nan)
+    e, j = fr.jacobian(D)
+    assert e == 0.0 and not j.any()
