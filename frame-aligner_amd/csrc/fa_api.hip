@@ -437,6 +437,8 @@ int fa_create(int device, fa_context** out)
 int fa_destroy(fa_context* ctx)
 {
    if (!ctx) return FA_OK;
+   if (ctx->live_refs > 0)
+      return set_err(FA_E_STATE, "context still has live references; release them first");
    DeviceGuard g(ctx->device);
    (void)hipStreamSynchronize(ctx->stream);
    (void)hipStreamDestroy(ctx->stream);
@@ -592,6 +594,8 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    R.h_pd = r->h_pd;
    R.eig_lo = r->eig_lo;
    cleanup();
+   ctx->live_refs++;
+   r->counted = true;
    *out = r;
    return FA_OK;
 }
@@ -599,7 +603,10 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
 int fa_reference_release(fa_reference* r)
 {
    if (!r) return FA_OK;
+   if (r->live_frames > 0)
+      return set_err(FA_E_STATE, "reference still has registered frames; deregister them first");
    DeviceGuard g(r->ctx->device);
+   if (r->counted) r->ctx->live_refs--;
    dfree(r->d_rg);
    dfree(r->d_chunks);
    dfree(r->d_ichunk);

@@ -126,6 +126,7 @@ struct fa_context {
    int n_cu = 256;
    hipStream_t stream = nullptr;
    std::mutex mu;
+   int live_refs = 0;  // references not yet released; fa_destroy refuses while > 0
 };
 
 struct fa_reference {
@@ -150,6 +151,7 @@ struct fa_reference {
    double* d_L0s = nullptr;
    fa::RefDev dev{};
    int live_frames = 0;
+   bool counted = false;  // included in ctx->live_refs
 };
 
 struct fa_frame {

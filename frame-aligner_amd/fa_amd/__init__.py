@@ -12,6 +12,7 @@ calls raise.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -139,9 +140,19 @@ class Context:
         h = C.c_void_p()
         _check(self.L.fa_create(device, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
+        """Releases every batch, frame and reference made on this context, then the context."""
         if getattr(self, "h", None):
+            kids = list(self._children)
+            for kind in (Batch, Frame, Reference):
+                for k in kids:
+                    if isinstance(k, kind):
+                        k._release_handle()
             self.L.fa_destroy(self.h)
             self.h = None
 
@@ -173,6 +184,7 @@ class Reference:
         h = C.c_void_p()
         _check(self.L.fa_set_reference(ctx.h, _ptr(ref), nz, ny, nx, nD, C.byref(sp), chunk_px, C.byref(h)))
         self.h = h
+        ctx._adopt(self)
         info = np.zeros(6, np.int32)
         _check(self.L.fa_reference_info(self.h, _ptr(info)))
         self.shape = (nz, ny, nx)
@@ -181,10 +193,12 @@ class Reference:
         self.n_chunks = int(info[5])
         self.n_px = nz * ny * nx
 
-    def release(self):
+    def _release_handle(self):
         if getattr(self, "h", None):
             self.L.fa_reference_release(self.h)
             self.h = None
+
+    release = _release_handle
 
     def __del__(self):
         try:
@@ -252,11 +266,14 @@ class Frame:
         h = C.c_void_p()
         _check(self.L.fa_register_frame(ref.h, _ptr(a), dt, float(off), C.byref(h)))
         self.h = h
+        ref.ctx._adopt(self)
 
-    def release(self):
+    def _release_handle(self):
         if getattr(self, "h", None):
             self.L.fa_deregister_frame(self.h)
             self.h = None
+
+    release = _release_handle
 
     def __del__(self):
         try:
@@ -305,13 +322,16 @@ class Batch:
         h = C.c_void_p()
         _check(self.L.fa_batch_create(arr, len(self.frames), C.byref(p), C.byref(h)))
         self.h = h
+        self.frames[0].ref.ctx._adopt(self)
         self.sizes = [f.ref.nD * f.ref.n_dim for f in self.frames]
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
 
-    def close(self):
+    def _release_handle(self):
         if getattr(self, "h", None):
             self.L.fa_batch_destroy(self.h)
             self.h = None
+
+    close = _release_handle
 
     def __del__(self):
         try:
