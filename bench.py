@@ -38,7 +38,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=200)
     ap.add_argument("--plain-frames", type=int, default=40)
-    ap.add_argument("--cpu-frames", type=int, default=64)
+    ap.add_argument("--cpu-frames", type=int, default=200)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)

@@ -59,7 +59,8 @@ void dfree(T*& p)
    p = nullptr;
 }
 
-constexpr int kDefaultChunk = 1024;
+constexpr int kDefaultChunk = 2048;
+constexpr int kTrLdsMaxDoubles = 7680;  // 60 KB for the trust-region update
 constexpr int kLdsCapFloats = 12288;  // 48 KB of staged frame rows per workgroup
 constexpr int kBlocksPerCu = 4;
 
@@ -112,6 +113,7 @@ struct fa_batch {
    int64_t total_work = 0, total_x = 0, total_vec = 0, total_fac = 0, total_D = 0;
    int grid = 1;
    int timing = 0;
+   int tr_lds = 0;
    RefDev* d_refs = nullptr;
    BatchFrame* d_frames = nullptr;
    TrState* d_state = nullptr;
@@ -151,6 +153,7 @@ struct fa_batch {
       B.prm.sub_max_iter = prm.sub_max_iter;
       B.prm.skip_repeat = prm.skip_repeat;
       B.n_frames = (int32_t)frames.size();
+      B.tr_lds = tr_lds;
       return B;
    }
 
@@ -238,6 +241,8 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       b->total_vec += 4 * (int64_t)bf.n;
       b->total_fac += 2 * (int64_t)r->nD * r->n_dim * r->n_dim;
       b->total_D += 3 * (int64_t)r->nD;
+      const int need = 5 * bf.n + 6 * r->nD * r->n_dim * r->n_dim;
+      if (need <= kTrLdsMaxDoubles) b->tr_lds = std::max(b->tr_lds, need);
    }
    int rc = FA_OK;
    auto fail = [&](int code) {
@@ -496,7 +501,8 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    int rc;
    if ((rc = dalloc(&d_T, n_px)) || (rc = dalloc(&d_Di, n_px)) || (rc = dalloc(&d_begin, nD)) ||
        (rc = dalloc(&d_flags, 1)) || (rc = dalloc(&d_info, 2)) ||
-       (rc = dalloc(&r->d_rg, (size_t)n_px * (1 + m))))
+       (rc = dalloc(&r->d_rg4, (size_t)n_px)) ||
+       (m == 3 && (rc = dalloc(&r->d_df, (size_t)n_px))))
       return fail(rc);
 
    // node map on the device, interval starts back to the host
@@ -555,7 +561,8 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
       return fail(rc);
 
    RefDev& R = r->dev;
-   R.rg = r->d_rg;
+   R.rg4 = r->d_rg4;
+   R.df = r->d_df;
    R.chunks = r->d_chunks;
    R.ichunk = r->d_ichunk;
    R.Hdiag = r->d_Hdiag;
@@ -580,7 +587,7 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
        hipMemcpyAsync(r->d_ichunk, r->ichunk.data(), nD * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
        hipMemsetAsync(r->d_Hsub, 0, (size_t)nD * mm * sizeof(double), s) != hipSuccess ||
        hipMemsetAsync(r->d_L0s, 0, (size_t)nD * mm * sizeof(double), s) != hipSuccess ||
-       launch_sobel(d_T, nz, ny, nx, m, r->d_rg, s) != hipSuccess ||
+       launch_reference_record(d_T, nz, ny, nx, m, r->scan, r->d_rg4, r->d_df, s) != hipSuccess ||
        launch_hessian(R, d_hpart, r->d_Hdiag, r->d_Hsub, r->d_L0d, r->d_L0s, d_info, s) != hipSuccess ||
        hipMemcpyAsync(info, d_info, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
        hipMemcpyAsync(r->Hdiag.data(), r->d_Hdiag, (size_t)nD * mm * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -607,7 +614,8 @@ int fa_reference_release(fa_reference* r)
       return set_err(FA_E_STATE, "reference still has registered frames; deregister them first");
    DeviceGuard g(r->ctx->device);
    if (r->counted) r->ctx->live_refs--;
-   dfree(r->d_rg);
+   dfree(r->d_rg4);
+   dfree(r->d_df);
    dfree(r->d_chunks);
    dfree(r->d_ichunk);
    dfree(r->d_Hdiag);
@@ -682,13 +690,13 @@ int fa_get_gradient(const fa_reference* r, float* gx, float* gy, float* gz)
    std::lock_guard<std::mutex> lk(r->ctx->mu);
    DeviceGuard g(r->ctx->device);
    const int m = r->n_dim;
-   std::vector<float> rg((size_t)r->n_px * (1 + m));
-   HIPCHK(hipMemcpyAsync(rg.data(), r->d_rg, rg.size() * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
+   std::vector<float> rg((size_t)r->n_px * 4);
+   HIPCHK(hipMemcpyAsync(rg.data(), r->d_rg4, rg.size() * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
    HIPCHK(hipStreamSynchronize(r->ctx->stream));
    for (int64_t p = 0; p < r->n_px; p++) {
-      if (gx) gx[p] = rg[p * (1 + m) + 1];
-      if (gy) gy[p] = rg[p * (1 + m) + 2];
-      if (gz) gz[p] = m == 3 ? rg[p * (1 + m) + 3] : 0.f;
+      if (gx) gx[p] = rg[p * 4 + 1];
+      if (gy) gy[p] = rg[p * 4 + 2];
+      if (gz) gz[p] = m == 3 ? rg[p * 4 + 3] : 0.f;
    }
    return FA_OK;
 }

@@ -36,9 +36,17 @@ constexpr int kPassStride = 8;
 // 21 in 3-D, padded to 24.
 constexpr int kHessStride = 24;
 constexpr int kBlock = 256;  // threads per fused-pass workgroup (4 wave64)
+// Zero columns staged on each side of a frame row in LDS: taps need no bounds checks while
+// |dx| stays below kPad - 2 over a work item.
+constexpr int kPad = 16;
 
+// Per-pixel reference record read by every pass. 2-D: (R, gx, gy, Df) in one float4;
+// 3-D: (R, gx, gy, gz) plus Df in a separate float array. Df is the node fraction of
+// precomputeInterp (FrameWarper.cpp:258-267), computed once in FP64 and stored as float
+// exactly as the reference stores it.
 struct RefDev {
-   const float* rg;        // per pixel (R, gx, gy[, gz]): 1 + n_dim floats, interleaved
+   const float4* rg4;
+   const float* df;        // 3-D only (2-D keeps Df in rg4.w)
    const Chunk* chunks;    // work items, ordered by interval then pixel
    const int32_t* ichunk;  // first chunk of interval k (k = 0..nD-2), ichunk[nD-1] = n_chunks
    const double* Hdiag;    // nD blocks n_dim x n_dim
@@ -115,6 +123,7 @@ struct BatchDev {
    Counters* counters;
    TrParamsDev prm;
    int32_t n_frames;
+   int32_t tr_lds;    // doubles of dynamic LDS for the trust-region update (0: global memory)
 };
 
 }  // namespace fa
@@ -142,7 +151,8 @@ struct fa_reference {
    int h_pd = 0;
    double eig_lo = 0;
    // device
-   float* d_rg = nullptr;
+   float4* d_rg4 = nullptr;
+   float* d_df = nullptr;
    fa::Chunk* d_chunks = nullptr;
    int32_t* d_ichunk = nullptr;
    double* d_Hdiag = nullptr;

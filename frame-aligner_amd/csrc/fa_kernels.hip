@@ -133,8 +133,10 @@ __device__ __forceinline__ float filt(const Taps& t, int pos, int n, bool identi
    return t.c0 * get(-1) + t.c1 * get(0) + t.c2 * get(1);
 }
 
-__global__ void k_sobel(const float* __restrict__ T, int nz, int ny, int nx, int n_dim,
-                        float* __restrict__ rg)
+// Builds the per-pixel reference record: Sobel gradient (bit-exact to Sobel3d.cpp) and the
+// node fraction Df (FrameWarper.cpp:258-267).
+__global__ void k_reference_record(const float* __restrict__ T, int nz, int ny, int nx, int n_dim,
+                                   ScanDev sc, float4* __restrict__ rg4, float* __restrict__ df)
 {
    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    int64_t n = (int64_t)nz * ny * nx;
@@ -161,16 +163,17 @@ __global__ void k_sobel(const float* __restrict__ T, int nz, int ny, int nx, int
    float gy = filt(Dv, y, ny, false, [&](int oy) {
       return filt(S, z, nz, zid, [&](int oz) { return Sx(z + oz, y + oy, x); });
    });
-   float* o = rg + p * (1 + n_dim);
-   o[0] = T[p];
-   o[1] = gx;
-   o[2] = gy;
+   double u = node_coord(sc, z, y, x);
+   float f = (float)(u - trunc(u));
    if (n_dim == 3) {
       // gz = Dz(Sy(Sx(T)))
       float gz = filt(Dv, z, nz, zid, [&](int oz) {
          return filt(S, y, ny, false, [&](int oy) { return Sx(z + oz, y + oy, x); });
       });
-      o[3] = gz;
+      rg4[p] = make_float4(T[p], gx, gy, gz);
+      df[p] = f;
+   } else {
+      rg4[p] = make_float4(T[p], gx, gy, f);
    }
 }
 
@@ -212,22 +215,16 @@ __device__ void hess_chunk(const RefDev& R, const Chunk ch, double* out, KF (*sr
    KF acc[NV];
 #pragma unroll
    for (int v = 0; v < NV; v++) acc[v] = KF{0.f, 0.f};
-   const int nx = R.nx, ny = R.ny;
-   const int64_t plane_sz = (int64_t)nx * ny;
-   const int plane = (int)(ch.p0 / plane_sz);
-   const int q0 = (int)(ch.p0 - plane * plane_sz);
-   PixelWalk w;
-   w.init(q0 + threadIdx.x, nx);
-   for (int p = ch.p0 + threadIdx.x; p < ch.p1; p += kBlock, w.step(nx)) {
+   for (int p = ch.p0 + threadIdx.x; p < ch.p1; p += kBlock) {
       if (p == ch.end_p) continue;
-      double u = node_coord(R.scan, plane, w.y, w.x);
-      double fd = (double)(float)(u - trunc(u));
-      const float* g = R.rg + (int64_t)p * (1 + M);
+      const float4 rec = R.rg4[p];
+      const double fd = (double)(M == 3 ? R.df[p] : rec.w);
+      const float g[3] = {rec.y, rec.z, rec.w};
       float a[M], b[M];
 #pragma unroll
       for (int c = 0; c < M; c++) {
-         a[c] = (float)((double)g[1 + c] * (1.0 - fd));
-         b[c] = (float)((double)g[1 + c] * fd);
+         a[c] = (float)((double)g[c] * (1.0 - fd));
+         b[c] = (float)((double)g[c] * fd);
       }
       int v = 0;
 #pragma unroll
@@ -258,97 +255,157 @@ __global__ __launch_bounds__(kBlock) void k_hess_partials(RefDev R, double* part
 }
 
 // ---- small FP64 linear algebra (block-tridiagonal Cholesky) -----------------------------
+// Templated on the block size so the per-block work stays in registers; the operation order
+// is the same as the oracle's (oracle/fa_oracle.c).
 
-__device__ bool d_chol_small(int m, double* A)
+template <int M>
+__device__ __forceinline__ bool chol_block(double (&A)[M][M])
 {
-   for (int j = 0; j < m; j++) {
-      double s = A[j * m + j];
-      for (int k = 0; k < j; k++) s -= A[j * m + k] * A[j * m + k];
+#pragma unroll
+   for (int j = 0; j < M; j++) {
+      double s = A[j][j];
+#pragma unroll
+      for (int k = 0; k < j; k++) s -= A[j][k] * A[j][k];
       if (!(s > 0) || !isfinite(s)) return false;
-      double l = sqrt(s);
-      A[j * m + j] = l;
-      for (int i = j + 1; i < m; i++) {
-         double t = A[i * m + j];
-         for (int k = 0; k < j; k++) t -= A[i * m + k] * A[j * m + k];
-         A[i * m + j] = t / l;
+      const double l = sqrt(s);
+      A[j][j] = l;
+#pragma unroll
+      for (int i = j + 1; i < M; i++) {
+         double t = A[i][j];
+#pragma unroll
+         for (int k = 0; k < j; k++) t -= A[i][k] * A[j][k];
+         A[i][j] = t / l;
       }
-      for (int i = 0; i < j; i++) A[i * m + j] = 0;
+#pragma unroll
+      for (int i = 0; i < j; i++) A[i][j] = 0;
    }
    return true;
 }
 
 // L L^T = H + lambda I with H block-tridiagonal (diag blocks, sub-diagonal blocks (k+1,k)).
-__device__ bool d_bt_chol(int nb, int m, const double* diag, const double* sub, double lambda,
-                          double* Ld, double* Ls)
+template <int M>
+__device__ bool bt_chol(int nb, const double* diag, const double* sub, double lambda, double* Ld,
+                        double* Ls)
 {
-   const int mm = m * m;
+   constexpr int MM = M * M;
+   double C[M][M];
+#pragma unroll
+   for (int i = 0; i < M; i++)
+#pragma unroll
+      for (int j = 0; j < M; j++) C[i][j] = 0;
    for (int k = 0; k < nb; k++) {
-      double A[9];
-      for (int i = 0; i < mm; i++) A[i] = diag[k * mm + i];
-      for (int i = 0; i < m; i++) A[i * m + i] += lambda;
+      double A[M][M];
+#pragma unroll
+      for (int i = 0; i < M; i++)
+#pragma unroll
+         for (int j = 0; j < M; j++) A[i][j] = diag[k * MM + i * M + j];
+#pragma unroll
+      for (int i = 0; i < M; i++) A[i][i] += lambda;
       if (k > 0) {
-         const double* C = &Ls[(k - 1) * mm];
-         for (int i = 0; i < m; i++)
-            for (int j = 0; j < m; j++) {
+#pragma unroll
+         for (int i = 0; i < M; i++)
+#pragma unroll
+            for (int j = 0; j < M; j++) {
                double s = 0;
-               for (int t = 0; t < m; t++) s += C[i * m + t] * C[j * m + t];
-               A[i * m + j] -= s;
+#pragma unroll
+               for (int t = 0; t < M; t++) s += C[i][t] * C[j][t];
+               A[i][j] -= s;
             }
       }
-      if (!d_chol_small(m, A)) return false;
-      for (int i = 0; i < mm; i++) Ld[k * mm + i] = A[i];
+      if (!chol_block<M>(A)) return false;
+#pragma unroll
+      for (int i = 0; i < M; i++)
+#pragma unroll
+         for (int j = 0; j < M; j++) Ld[k * MM + i * M + j] = A[i][j];
       if (k + 1 < nb) {
-         const double* B = &sub[k * mm];
-         double* X = &Ls[k * mm];
-         for (int i = 0; i < m; i++)
-            for (int j = 0; j < m; j++) {
-               double t = B[i * m + j];
-               for (int q = 0; q < j; q++) t -= X[i * m + q] * A[j * m + q];
-               X[i * m + j] = t / A[j * m + j];
+#pragma unroll
+         for (int i = 0; i < M; i++)
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+               double t = sub[k * MM + i * M + j];
+#pragma unroll
+               for (int q = 0; q < j; q++) t -= C[i][q] * A[j][q];
+               C[i][j] = t / A[j][j];
             }
+#pragma unroll
+         for (int i = 0; i < M; i++)
+#pragma unroll
+            for (int j = 0; j < M; j++) Ls[k * MM + i * M + j] = C[i][j];
       }
    }
    return true;
 }
 
-__device__ void d_bt_fwd(int nb, int m, const double* Ld, const double* Ls, const double* b,
-                         double* y)
+// y = L^-1 b.
+template <int M>
+__device__ void bt_fwd(int nb, const double* Ld, const double* Ls, const double* b, double* y)
 {
-   const int mm = m * m;
+   constexpr int MM = M * M;
+   double yp[M];
+#pragma unroll
+   for (int i = 0; i < M; i++) yp[i] = 0;
    for (int k = 0; k < nb; k++) {
-      double r[3];
-      for (int i = 0; i < m; i++) {
-         r[i] = b[k * m + i];
+      double r[M], yk[M];
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+         r[i] = b[k * M + i];
          if (k > 0)
-            for (int t = 0; t < m; t++) r[i] -= Ls[(k - 1) * mm + i * m + t] * y[(k - 1) * m + t];
+#pragma unroll
+            for (int t = 0; t < M; t++) r[i] -= Ls[(k - 1) * MM + i * M + t] * yp[t];
       }
-      const double* L = &Ld[k * mm];
-      for (int i = 0; i < m; i++) {
+      const double* L = &Ld[k * MM];
+#pragma unroll
+      for (int i = 0; i < M; i++) {
          double t = r[i];
-         for (int q = 0; q < i; q++) t -= L[i * m + q] * y[k * m + q];
-         y[k * m + i] = t / L[i * m + i];
+#pragma unroll
+         for (int q = 0; q < i; q++) t -= L[i * M + q] * yk[q];
+         yk[i] = t / L[i * M + i];
+      }
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+         y[k * M + i] = yk[i];
+         yp[i] = yk[i];
       }
    }
 }
 
-__device__ void d_bt_bwd(int nb, int m, const double* Ld, const double* Ls, const double* y,
-                         double* x)
+// x = L^-T y.
+template <int M>
+__device__ void bt_bwd(int nb, const double* Ld, const double* Ls, const double* y, double* x)
 {
-   const int mm = m * m;
+   constexpr int MM = M * M;
+   double xn[M];
+#pragma unroll
+   for (int i = 0; i < M; i++) xn[i] = 0;
    for (int k = nb - 1; k >= 0; k--) {
-      double r[3];
-      for (int i = 0; i < m; i++) {
-         r[i] = y[k * m + i];
+      double r[M], xk[M];
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+         r[i] = y[k * M + i];
          if (k + 1 < nb)
-            for (int t = 0; t < m; t++) r[i] -= Ls[k * mm + t * m + i] * x[(k + 1) * m + t];
+#pragma unroll
+            for (int t = 0; t < M; t++) r[i] -= Ls[k * MM + t * M + i] * xn[t];
       }
-      const double* L = &Ld[k * mm];
-      for (int i = m - 1; i >= 0; i--) {
+      const double* L = &Ld[k * MM];
+#pragma unroll
+      for (int i = M - 1; i >= 0; i--) {
          double t = r[i];
-         for (int q = i + 1; q < m; q++) t -= L[q * m + i] * x[k * m + q];
-         x[k * m + i] = t / L[i * m + i];
+#pragma unroll
+         for (int q = i + 1; q < M; q++) t -= L[q * M + i] * xk[q];
+         xk[i] = t / L[i * M + i];
+      }
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+         x[k * M + i] = xk[i];
+         xn[i] = xk[i];
       }
    }
+}
+
+__device__ bool d_bt_chol(int nb, int m, const double* diag, const double* sub, double lambda,
+                          double* Ld, double* Ls)
+{
+   return m == 2 ? bt_chol<2>(nb, diag, sub, lambda, Ld, Ls) : bt_chol<3>(nb, diag, sub, lambda, Ld, Ls);
 }
 
 __device__ double d_norm(int n, const double* v)
@@ -432,6 +489,113 @@ __global__ void k_hess_assemble(RefDev R, const double* partials, double* Hdiag,
 // reach are staged in LDS when they fit.
 enum { MODE_PASS = 0, MODE_WARP = 1 };
 
+// Block reduction used by the fused pass: the compensated pairs of each thread are summed as
+// two plain float trees (leading parts and compensations) and combined once per wave.
+template <int NV>
+__device__ __forceinline__ void block_reduce_fast(KF (&acc)[NV], KF (*sred)[NV], double* out)
+{
+#pragma unroll
+   for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+         acc[v].s += __shfl_xor(acc[v].s, off, 64);
+         acc[v].c += __shfl_xor(acc[v].c, off, 64);
+      }
+   }
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   if (lane == 0)
+#pragma unroll
+      for (int v = 0; v < NV; v++) sred[wave][v] = acc[v];
+   __syncthreads();
+   if (threadIdx.x < NV) {
+      const int v = threadIdx.x;
+      double t = 0;
+      for (int w = 0; w < kBlock / 64; w++) t += (double)sred[w][v].s - (double)sred[w][v].c;
+      out[v] = t;
+   }
+   __syncthreads();
+}
+
+// Frame samples a work item can reach, staged in LDS: planes [za, za+nzs) and rows
+// [ya, ya+nrows) of those planes, each row as kPad zeros, the nx samples, kPad zeros
+// (stride floats). Rows and planes outside the frame are staged as zeros, which gives the
+// zero padding of getZeroPadded (FrameWarper.h:39-47) without a test per tap.
+struct Stage {
+   int za, nzs, ya, nrows, stride;
+};
+
+template <typename FT>
+__device__ __forceinline__ float4 load4(const FrameDev& F, int64_t i);
+
+template <>
+__device__ __forceinline__ float4 load4<uint16_t>(const FrameDev& F, int64_t i)
+{
+   const uint2 w = *(const uint2*)((const uint16_t*)F.data + i);
+   return make_float4((float)(w.x & 0xffffu) + F.offset, (float)(w.x >> 16) + F.offset,
+                      (float)(w.y & 0xffffu) + F.offset, (float)(w.y >> 16) + F.offset);
+}
+
+template <>
+__device__ __forceinline__ float4 load4<float>(const FrameDev& F, int64_t i)
+{
+   const float4 w = *(const float4*)((const float*)F.data + i);
+   return make_float4(w.x + F.offset, w.y + F.offset, w.z + F.offset, w.w + F.offset);
+}
+
+template <typename FT>
+__device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, float* lds)
+{
+   const int nx = R.nx, ny = R.ny, nz = R.nz;
+   const int rows = S.nzs * S.nrows;
+   const int tid = threadIdx.x;
+   const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+   if ((nx & 3) == 0) {
+      constexpr int P4 = kPad / 4;
+      for (int i = tid; i < rows * 2 * P4; i += kBlock) {
+         const int r = i / (2 * P4), j = i - r * (2 * P4);
+         const int col = j < P4 ? 4 * j : kPad + nx + 4 * (j - P4);
+         *(float4*)&lds[r * S.stride + col] = zero4;
+      }
+      const int nx4 = nx >> 2;
+      auto row4 = [&](int r, int c) {
+         const int zi = r / S.nrows;
+         const int yy = S.ya + (r - zi * S.nrows), zz = S.za + zi;
+         float4 v = zero4;
+         if ((unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz)
+            v = load4<FT>(F, ((int64_t)zz * ny + yy) * nx + 4 * c);
+         *(float4*)&lds[r * S.stride + kPad + 4 * c] = v;
+      };
+      if (nx4 <= kBlock) {
+         const int rpi = kBlock / nx4, tr = tid / nx4, tc = tid - tr * nx4;
+         if (tr < rpi)
+            for (int r = tr; r < rows; r += rpi) row4(r, tc);
+      } else {
+         for (int r = 0; r < rows; r++)
+            for (int c = tid; c < nx4; c += kBlock) row4(r, c);
+      }
+   } else {
+      for (int r = 0; r < rows; r++) {
+         const int zi = r / S.nrows;
+         const int yy = S.ya + (r - zi * S.nrows), zz = S.za + zi;
+         const bool in = (unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz;
+         for (int c = tid; c < S.stride; c += kBlock) {
+            const int xx = c - kPad;
+            float v = 0.f;
+            if (in && (unsigned)xx < (unsigned)nx) v = fload<FT>(F, ((int64_t)zz * ny + yy) * nx + xx);
+            lds[r * S.stride + c] = v;
+         }
+      }
+   }
+   __syncthreads();
+}
+
+// For every pixel of a work item: displacement d = f D[k+1] + (1-f) D[k] (warpPoint,
+// FrameWarper.cpp:397-419), zero-padded bilinear/trilinear pull-sample of the frame at p + d
+// (warpImageInterpolated, FrameWarper.cpp:31-69), residual against the reference and SSD over
+// samples that exist (computeErrorImage, FrameWarper.cpp:4-28), and the two Jacobian rows
+// (computeJacobian, FrameWarper.cpp:147-167). The sample position is formed in FP64 as the
+// reference does, so the tap indices and the "no sample" decision match it; the interpolation
+// and the sums are FP32 with compensated accumulation.
 template <int M, typename FT, int MODE>
 __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
                            const double* __restrict__ D, float* lds, int lds_cap, bool want_lds,
@@ -441,47 +605,47 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    const int nx = R.nx, ny = R.ny, nz = R.nz;
    const int64_t plane_sz = (int64_t)nx * ny;
    const int k = ch.k;
-   double Dk[3], Dk1[3];
+   double D0[3], D1[3];
 #pragma unroll
    for (int c = 0; c < 3; c++) {
-      Dk[c] = (c < M) ? D[3 * k + c] : 0.0;
-      Dk1[c] = (c < M) ? D[3 * (k + 1) + c] : 0.0;
+      D0[c] = (c < M) ? D[3 * k + c] : 0.0;
+      D1[c] = (c < M) ? D[3 * (k + 1) + c] : 0.0;
    }
    const int plane = (int)(ch.p0 / plane_sz);
    const int q0 = (int)(ch.p0 - plane * plane_sz);
    const int r0 = q0 / nx, r1 = (int)((ch.p1 - 1 - plane * plane_sz) / nx);
 
-   // Rows (and planes) the work item can reach: d is a convex combination of D[k], D[k+1].
+   // d is a convex combination of D[k] and D[k+1]: bound the reachable samples.
+   Stage S{0, 1, 0, 0, 0};
    bool use = false;
-   int za = 0, zb = 0, ya = 0, yb = -1;
    if (want_lds) {
-      double ylo = fmin(Dk[1], Dk1[1]), yhi = fmax(Dk[1], Dk1[1]);
-      double zlo = fmin(Dk[2], Dk1[2]), zhi = fmax(Dk[2], Dk1[2]);
-      if (fabs(ylo) < 1e6 && fabs(yhi) < 1e6 && fabs(zlo) < 1e6 && fabs(zhi) < 1e6) {
-         ya = max(0, (int)floor(r0 + ylo) - 1);
-         yb = min(ny - 1, (int)floor(r1 + yhi) + 2);
+      const double lim = kPad - 3;
+      double lo[3], hi[3];
+      bool ok = true;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         lo[c] = fmin(D0[c], D1[c]);
+         hi[c] = fmax(D0[c], D1[c]);
+         ok = ok && lo[c] > -1e6 && hi[c] < 1e6;
+      }
+      ok = ok && lo[0] > -lim && hi[0] < lim;
+      if (ok) {
+         S.ya = (int)floor(r0 + lo[1]) - 1;
+         S.nrows = (int)floor(r1 + hi[1]) + 2 - S.ya + 1;
          if (M == 3) {
-            za = max(0, (int)floor(plane + zlo) - 1);
-            zb = min(nz - 1, (int)floor(plane + zhi) + 2);
+            S.za = (int)floor(plane + lo[2]) - 1;
+            S.nzs = (int)floor(plane + hi[2]) + 2 - S.za + 1;
          }
-         if (yb >= ya && zb >= za)
-            use = (int64_t)(zb - za + 1) * (yb - ya + 1) * nx <= lds_cap;
+         S.stride = ((nx + 2 * kPad) + 3) & ~3;
+         use = (int64_t)S.nzs * S.nrows * S.stride <= lds_cap;
       }
    }
-   const int nrows = yb - ya + 1;
-   if (use) {
-      const int psz = nrows * nx;
-      for (int zi = 0; zi <= zb - za; zi++) {
-         const int64_t base = ((int64_t)(za + zi) * ny + ya) * nx;
-         for (int i = threadIdx.x; i < psz; i += kBlock) lds[zi * psz + i] = fload<FT>(F, base + i);
-      }
-      __syncthreads();
-   }
-   auto tap = [&](int zz, int yy, int xx) -> float {
+   if (use) stage_frame<FT>(R, F, S, lds);
+   const int lds_lim = S.nzs * S.nrows * S.stride - (M == 3 ? S.nrows * S.stride : 0) - S.stride - 2;
+
+   auto tap_g = [&](int zz, int yy, int xx) -> float {
       if ((unsigned)xx >= (unsigned)nx || (unsigned)yy >= (unsigned)ny) return 0.f;
       if (M == 3 && (unsigned)zz >= (unsigned)nz) return 0.f;
-      if (use && zz >= za && zz <= zb && yy >= ya && yy <= yb)
-         return lds[((zz - za) * nrows + (yy - ya)) * nx + xx];
       return fload<FT>(F, ((int64_t)zz * ny + yy) * nx + xx);
    };
 
@@ -491,75 +655,75 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
 
    PixelWalk w;
    w.init(q0 + threadIdx.x, nx);
-   int row_y = -1;
-   double row_t = 0;
    for (int p = ch.p0 + threadIdx.x; p < ch.p1; p += kBlock, w.step(nx)) {
-      double fd;
-      {
-#pragma clang fp contract(off)
-         if (w.y != row_y) {
-            row_y = w.y;
-            row_t = (double)plane * R.scan.interframe + (double)w.y * R.scan.interline;
-         }
-         double t = row_t + (double)w.x * R.scan.pixel;
-         double u = t / R.scan.stack * R.scan.nDm1;
-         fd = (double)(float)(u - trunc(u));
-      }
-      float v = 0.f;
-      {
-         double s[3];
-         {
-#pragma clang fp contract(off)
+      const float4 rec = R.rg4[p];
+      const float f = (M == 3) ? R.df[p] : rec.w;
+      const double fd = (double)f, omf = 1.0 - fd;
+      double sp[3];
 #pragma unroll
-            for (int c = 0; c < 3; c++) s[c] = fd * Dk1[c] + (1.0 - fd) * Dk[c];
-            s[0] += (double)w.x;
-            s[1] += (double)w.y;
-            s[2] += (double)plane;
+      for (int c = 0; c < M; c++) sp[c] = fd * D1[c] + omf * D0[c];
+      sp[0] += (double)w.x;
+      sp[1] += (double)w.y;
+      if (M == 3) sp[2] += (double)plane;
+      float v = 0.f;
+      if (use) {
+         const double fx = floor(sp[0]), fy = floor(sp[1]);
+         const int x0 = (int)fx, y0 = (int)fy;
+         const double ax = sp[0] - fx, ay = sp[1] - fy;
+         const float wx1 = (float)ax, wx0 = (float)(1.0 - ax);
+         const float wy1 = (float)ay, wy0 = (float)(1.0 - ay);
+         int idx = (y0 - S.ya) * S.stride + x0 + kPad;
+         if (M == 3) idx += ((int)floor(sp[2]) - S.za) * S.nrows * S.stride;
+         idx = min(max(idx, 0), lds_lim);
+         const float* L = lds + idx;
+         float v0 = wy0 * (wx0 * L[0] + wx1 * L[1]) + wy1 * (wx0 * L[S.stride] + wx1 * L[S.stride + 1]);
+         if (M == 3) {
+            const double fz = floor(sp[2]);
+            const double az = sp[2] - fz;
+            const float wz1 = (float)az, wz0 = (float)(1.0 - az);
+            const float* L2 = L + S.nrows * S.stride;
+            float v1 = wy0 * (wx0 * L2[0] + wx1 * L2[1]) + wy1 * (wx0 * L2[S.stride] + wx1 * L2[S.stride + 1]);
+            v = wz0 * v0 + wz1 * v1;
+         } else {
+            v = v0;
          }
-         if (fabs(s[0]) < 1e9 && fabs(s[1]) < 1e9 && fabs(s[2]) < 1e9) {
-            double fx = floor(s[0]), fy = floor(s[1]);
-            int x0 = (int)fx, y0 = (int)fy;
-            double wx = s[0] - fx, wy = s[1] - fy;
-            float wx1 = (float)wx, wx0 = (float)(1.0 - wx);
-            float wy1 = (float)wy, wy0 = (float)(1.0 - wy);
-            if (M == 2) {
-               float t00 = tap(0, y0, x0), t01 = tap(0, y0, x0 + 1);
-               float t10 = tap(0, y0 + 1, x0), t11 = tap(0, y0 + 1, x0 + 1);
-               v = wy0 * (wx0 * t00 + wx1 * t01) + wy1 * (wx0 * t10 + wx1 * t11);
-            } else {
-               double fz = floor(s[2]);
-               int z0 = (int)fz;
-               double wz = s[2] - fz;
-               float wz1 = (float)wz, wz0 = (float)(1.0 - wz);
-               float t000 = tap(z0, y0, x0), t001 = tap(z0, y0, x0 + 1);
-               float t010 = tap(z0, y0 + 1, x0), t011 = tap(z0, y0 + 1, x0 + 1);
-               float t100 = tap(z0 + 1, y0, x0), t101 = tap(z0 + 1, y0, x0 + 1);
-               float t110 = tap(z0 + 1, y0 + 1, x0), t111 = tap(z0 + 1, y0 + 1, x0 + 1);
-               float v0 = wy0 * (wx0 * t000 + wx1 * t001) + wy1 * (wx0 * t010 + wx1 * t011);
-               float v1 = wy0 * (wx0 * t100 + wx1 * t101) + wy1 * (wx0 * t110 + wx1 * t111);
-               v = wz0 * v0 + wz1 * v1;
-            }
+      } else if (fabs(sp[0]) < 1e9 && fabs(sp[1]) < 1e9 && (M == 2 || fabs(sp[2]) < 1e9)) {
+         const double fx = floor(sp[0]), fy = floor(sp[1]);
+         const int x0 = (int)fx, y0 = (int)fy;
+         const double ax = sp[0] - fx, ay = sp[1] - fy;
+         const float wx1 = (float)ax, wx0 = (float)(1.0 - ax);
+         const float wy1 = (float)ay, wy0 = (float)(1.0 - ay);
+         const int z0 = M == 3 ? (int)floor(sp[2]) : 0;
+         float v0 = wy0 * (wx0 * tap_g(z0, y0, x0) + wx1 * tap_g(z0, y0, x0 + 1)) +
+                    wy1 * (wx0 * tap_g(z0, y0 + 1, x0) + wx1 * tap_g(z0, y0 + 1, x0 + 1));
+         if (M == 3) {
+            const double az = sp[2] - floor(sp[2]);
+            const float wz1 = (float)az, wz0 = (float)(1.0 - az);
+            float v1 = wy0 * (wx0 * tap_g(z0 + 1, y0, x0) + wx1 * tap_g(z0 + 1, y0, x0 + 1)) +
+                       wy1 * (wx0 * tap_g(z0 + 1, y0 + 1, x0) + wx1 * tap_g(z0 + 1, y0 + 1, x0 + 1));
+            v = wz0 * v0 + wz1 * v1;
+         } else {
+            v = v0;
          }
       }
       if (MODE == MODE_WARP) {
          wout[p] = (v == 0.f) ? invalid : v;
          continue;
       }
-      if (v == 0.f) continue;  // no sample: residual 0, outside the SSD
-      const float* rg = R.rg + (int64_t)p * (1 + M);
-      float e = v - rg[0];
-      kadd(acc[0], e * e);
-      if (p != ch.end_p) {
+      if (v != 0.f) {  // no sample: residual 0, outside the SSD
+         const float e = v - rec.x;
+         kadd(acc[0], e * e);
+         const float ej = (p != ch.end_p) ? e : 0.f;  // the interval's last pixel has no row
+         const float omff = (float)omf;
+         const float g[3] = {rec.y, rec.z, rec.w};
 #pragma unroll
          for (int c = 0; c < M; c++) {
-            float a = (float)((double)rg[1 + c] * (1.0 - fd));
-            float b = (float)((double)rg[1 + c] * fd);
-            kadd(acc[1 + c], a * e);
-            kadd(acc[1 + M + c], b * e);
+            kadd(acc[1 + c], (g[c] * omff) * ej);
+            kadd(acc[1 + M + c], (g[c] * f) * ej);
          }
       }
    }
-   if (MODE == MODE_PASS) block_reduce_store<NV>(acc, (KF(*)[NV])sred, out);
+   if (MODE == MODE_PASS) block_reduce_fast<NV>(acc, (KF(*)[NV])sred, out);
    else __syncthreads();
 }
 
@@ -651,7 +815,7 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
       work += R.n_chunks;
       fr++;
       bytes += (unsigned long long)R.n_px *
-               (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + 4 * (1 + R.n_dim));
+               (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + (R.n_dim == 3 ? 20 : 16));
    }
    s_work[t] = work;
    s_fr[t] = fr;
@@ -744,11 +908,12 @@ struct TrScratch {
    double *x, *p, *g, *t1, *t2, *Ld, *Ls;
 };
 
+template <int M>
 __device__ int d_tr_subproblem(const RefDev& R, const TrScratch& S, const double* g,
                                double radius, double eps, int max_iter, double* p,
                                double* lam_out)
 {
-   const int m = R.n_dim, nb = R.nD, n = nb * m;
+   const int nb = R.nD, n = nb * M;
    double gnorm = d_norm(n, g);
    for (int i = 0; i < n; i++) p[i] = 0;
    *lam_out = 0;
@@ -766,7 +931,7 @@ __device__ int d_tr_subproblem(const RefDev& R, const TrScratch& S, const double
       } else if (lam == 0) {
          ok = false;
       } else {
-         ok = d_bt_chol(nb, m, R.Hdiag, R.Hsub, lam, S.Ld, S.Ls);
+         ok = bt_chol<M>(nb, R.Hdiag, R.Hsub, lam, S.Ld, S.Ls);
          Ld = S.Ld;
          Ls = S.Ls;
       }
@@ -776,8 +941,8 @@ __device__ int d_tr_subproblem(const RefDev& R, const TrScratch& S, const double
          continue;
       }
       for (int i = 0; i < n; i++) S.t2[i] = -g[i];
-      d_bt_fwd(nb, m, Ld, Ls, S.t2, S.t1);
-      d_bt_bwd(nb, m, Ld, Ls, S.t1, p);
+      bt_fwd<M>(nb, Ld, Ls, S.t2, S.t1);
+      bt_bwd<M>(nb, Ld, Ls, S.t1, p);
       *lam_out = lam;
       double pn = d_norm(n, p);
       if (lam == 0 && pn <= radius) {
@@ -790,7 +955,7 @@ __device__ int d_tr_subproblem(const RefDev& R, const TrScratch& S, const double
       }
       if (pn > radius) lam_lo = lam;
       else lam_hi = lam;
-      d_bt_fwd(nb, m, Ld, Ls, p, S.t1);
+      bt_fwd<M>(nb, Ld, Ls, p, S.t1);
       double qn = d_norm(n, S.t1);
       double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
       if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
@@ -799,17 +964,23 @@ __device__ int d_tr_subproblem(const RefDev& R, const TrScratch& S, const double
    return it;
 }
 
+template <int M>
 __device__ double d_quad(const RefDev& R, const double* p)
 {
-   const int m = R.n_dim, mm = m * m, nb = R.nD;
+   constexpr int MM = M * M;
+   const int nb = R.nD;
    double s = 0;
    for (int k = 0; k < nb; k++) {
-      for (int a = 0; a < m; a++)
-         for (int b = 0; b < m; b++) s += p[k * m + a] * R.Hdiag[k * mm + a * m + b] * p[k * m + b];
+#pragma unroll
+      for (int a = 0; a < M; a++)
+#pragma unroll
+         for (int b = 0; b < M; b++) s += p[k * M + a] * R.Hdiag[k * MM + a * M + b] * p[k * M + b];
       if (k + 1 < nb)
-         for (int a = 0; a < m; a++)
-            for (int b = 0; b < m; b++)
-               s += 2.0 * p[(k + 1) * m + a] * R.Hsub[k * mm + a * m + b] * p[k * m + b];
+#pragma unroll
+         for (int a = 0; a < M; a++)
+#pragma unroll
+            for (int b = 0; b < M; b++)
+               s += 2.0 * p[(k + 1) * M + a] * R.Hsub[k * MM + a * M + b] * p[k * M + b];
    }
    return s;
 }
@@ -851,18 +1022,53 @@ __global__ void k_tr_init(BatchDev B, int eval_only)
    }
 }
 
+// One wave per frame. The frame's vectors and the reference's JtJ blocks and factor are
+// copied into LDS first (when they fit), because the triangular solves are a sequential chain
+// that would otherwise wait on a global-memory round trip at every block.
 __global__ __launch_bounds__(64) void k_tr_update(BatchDev B)
 {
+   extern __shared__ double smem[];
    __shared__ double sh[64];
    __shared__ int s_accept;
    const int j = blockIdx.x;
    if (B.state[j].done) return;
    const BatchFrame& bf = B.frames[j];
-   const RefDev& R = B.refs[bf.ref];
+   RefDev R = B.refs[bf.ref];
    const TrParamsDev& prm = B.prm;
-   TrScratch S = scratch_of(B, bf);
+   const TrScratch G = scratch_of(B, bf);
+   TrScratch S = G;
    const double* P = B.partials + bf.work_base * kPassStride;
    const int n = bf.n, m = R.n_dim;
+   const int nb = R.nD * m * m;
+   const bool in_lds = 5 * n + 6 * nb <= B.tr_lds;
+   if (in_lds) {
+      double* q = smem;
+      S.x = q;
+      S.p = q + n;
+      S.g = q + 2 * n;
+      S.t1 = q + 3 * n;
+      S.t2 = q + 4 * n;
+      q += 5 * n;
+      double *hd = q, *hs = q + nb, *l0d = q + 2 * nb, *l0s = q + 3 * nb;
+      S.Ld = q + 4 * nb;
+      S.Ls = q + 5 * nb;
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+         S.x[i] = G.x[i];
+         S.p[i] = G.p[i];
+         S.g[i] = G.g[i];
+      }
+      for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+         hd[i] = R.Hdiag[i];
+         hs[i] = R.Hsub[i];
+         l0d[i] = R.L0d[i];
+         l0s[i] = R.L0s[i];
+      }
+      R.Hdiag = hd;
+      R.Hsub = hs;
+      R.L0d = l0d;
+      R.L0s = l0s;
+      __syncthreads();
+   }
    const double f_new = reduce_ssd(R, P, sh);
 
    TrState st = B.state[j];
@@ -878,7 +1084,7 @@ __global__ __launch_bounds__(64) void k_tr_update(BatchDev B)
          st.evaluations++;
          double gp = 0;
          for (int i = 0; i < n; i++) gp += S.g[i] * S.p[i];
-         double pred = -(0.5 * d_quad(R, S.p) + gp);
+         double pred = -(0.5 * (m == 2 ? d_quad<2>(R, S.p) : d_quad<3>(R, S.p)) + gp);
          double meas = st.f - f_new;
          if (fabs(pred) <= fabs(meas) * DBL_EPSILON) {
             st.status = FA_STOP_NO_PROGRESS;
@@ -953,7 +1159,8 @@ __global__ __launch_bounds__(64) void k_tr_update(BatchDev B)
       }
       if (!st.done) {
          double lam;
-         st.sub_iters = d_tr_subproblem(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
+         st.sub_iters = m == 2 ? d_tr_subproblem<2>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam)
+                              : d_tr_subproblem<3>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
          st.lambda = lam;
          st.phase = 1;
       }
@@ -965,6 +1172,12 @@ __global__ __launch_bounds__(64) void k_tr_update(BatchDev B)
       for (int i = threadIdx.x; i < R.nD; i += blockDim.x)
          for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < m ? S.x[i * m + c] + S.p[i * m + c] : 0.0;
    }
+   if (in_lds)
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+         G.x[i] = S.x[i];
+         G.p[i] = S.p[i];
+         G.g[i] = S.g[i];
+      }
 }
 
 // ---- launchers ----------------------------------------------------------------------------
@@ -986,10 +1199,12 @@ hipError_t launch_interval_begin(const uint16_t* Di, int64_t n, int nD, int32_t*
    return hipGetLastError();
 }
 
-hipError_t launch_sobel(const float* T, int nz, int ny, int nx, int n_dim, float* rg, hipStream_t s)
+hipError_t launch_reference_record(const float* T, int nz, int ny, int nx, int n_dim,
+                                   const ScanDev& sc, float4* rg4, float* df, hipStream_t s)
 {
    int64_t n = (int64_t)nz * ny * nx;
-   hipLaunchKernelGGL(k_sobel, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, nz, ny, nx, n_dim, rg);
+   hipLaunchKernelGGL(k_reference_record, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, nz, ny, nx,
+                      n_dim, sc, rg4, df);
    return hipGetLastError();
 }
 
@@ -1033,7 +1248,7 @@ hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s)
 
 hipError_t launch_tr_update(const BatchDev& B, hipStream_t s)
 {
-   hipLaunchKernelGGL(k_tr_update, dim3(B.n_frames), dim3(64), 0, s, B);
+   hipLaunchKernelGGL(k_tr_update, dim3(B.n_frames), dim3(64), (size_t)B.tr_lds * sizeof(double), s, B);
    return hipGetLastError();
 }
 

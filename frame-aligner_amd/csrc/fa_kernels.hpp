@@ -9,7 +9,8 @@ hipError_t launch_node_map(const ScanDev& sc, int nx, int ny, int64_t n, uint16_
                            hipStream_t s);
 hipError_t launch_interval_begin(const uint16_t* Di, int64_t n, int nD, int32_t* begin,
                                  int32_t* flags, hipStream_t s);
-hipError_t launch_sobel(const float* T, int nz, int ny, int nx, int n_dim, float* rg, hipStream_t s);
+hipError_t launch_reference_record(const float* T, int nz, int ny, int nx, int n_dim,
+                                   const ScanDev& sc, float4* rg4, float* df, hipStream_t s);
 hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, double* Hsub,
                           double* L0d, double* L0s, double* info, hipStream_t s);
 hipError_t launch_pass(const PassArgs& a, int grid, hipStream_t s);

@@ -135,7 +135,11 @@ def test_align_matches_oracle(ctx, name, shape, nD, scan):
         fr = ref.register(fw)
         x_g, s_g = fr.align(**kw)
         assert np.max(np.abs(x_g - x_o)) <= D_TOL, (np.max(np.abs(x_g - x_o)), s_o, s_g)
-        assert abs(s_g["f_final"] - s_o["f_final"]) <= ERR_TOL * s_o["f_final"]
+        # the relative stop rule fixes the final SSD only to ~rel_delta, so the SSDs are
+        # compared at the same point: the GPU's result
+        e_o = orc.get_error(fw, orc.col2D(x_g))
+        assert abs(s_g["f_final"] - e_o) <= ERR_TOL * e_o
+        assert abs(s_g["f_final"] - s_o["f_final"]) <= 10 * kw["rel_delta"] * s_o["f_final"]
         fr.release()
 
 
