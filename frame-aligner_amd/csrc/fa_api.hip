@@ -59,9 +59,9 @@ void dfree(T*& p)
    p = nullptr;
 }
 
-constexpr int kDefaultChunk = 2048;
+constexpr int kDefaultChunk = 4096;
 constexpr int kTrLdsMaxDoubles = 7680;  // 60 KB for the trust-region update
-constexpr int kLdsCapFloats = 12288;  // 48 KB of staged frame rows per workgroup
+constexpr int kLdsCapFloats = 15360;  // 60 KB of staged frame rows per workgroup at most
 constexpr int kBlocksPerCu = 4;
 
 // ImageScanParameters ctor (ImageScanParameters.h:7-20) then processScanParameters
@@ -111,9 +111,12 @@ struct fa_batch {
    std::vector<BatchFrame> hframes;
    std::vector<int64_t> x_off;  // offset of each frame's parameter vector in x_inout
    int64_t total_work = 0, total_x = 0, total_vec = 0, total_fac = 0, total_D = 0;
-   int grid = 1;
+   int grid2 = 0, grid3 = 0;  // pass grids for the 2-D and 3-D frames (0: none of that kind)
+   int64_t work2 = 0, work3 = 0;
+   fa_tuning tune{};
    int timing = 0;
    int tr_lds = 0;
+   std::vector<int> user;  // internal index -> caller's index
    RefDev* d_refs = nullptr;
    BatchFrame* d_frames = nullptr;
    TrState* d_state = nullptr;
@@ -167,8 +170,8 @@ struct fa_batch {
       a.active_frames = d_active_frames;
       a.active_base = d_active_base;
       a.counters = d_counters;
-      a.lds_cap = kLdsCapFloats;
-      a.use_lds = 1;
+      a.lds_cap = tune.lds_floats;
+      a.use_lds = tune.use_lds;
       return a;
    }
 
@@ -193,6 +196,35 @@ struct fa_batch {
 
 namespace {
 
+// Default LDS budget: the rows one work item spans plus room for the rows (and, in 3-D, the
+// planes) its displacement range reaches, for the widest frame of the batch.
+int default_lds_floats(const std::vector<fa_frame*>& frames)
+{
+   int64_t best = 0;
+   for (fa_frame* f : frames) {
+      const fa_reference* r = f->ref;
+      const int64_t stride = ((r->nx + 2 * kPad) + 3) & ~3;
+      const int64_t rows = (r->chunk_px + r->nx - 1) / r->nx + 8;
+      best = std::max(best, rows * stride * (r->n_dim == 3 ? 3 : 1));
+   }
+   return (int)std::min<int64_t>(best, kLdsCapFloats);
+}
+
+void default_tuning(fa_tuning* t, const std::vector<fa_frame*>& frames)
+{
+   t->lds_floats = default_lds_floats(frames);
+   t->blocks_per_cu = kBlocksPerCu;
+   t->use_lds = 1;
+   t->check_every = 4;
+}
+
+void set_grids(fa_batch* b)
+{
+   const int64_t cap = (int64_t)b->ctx->n_cu * std::max(1, b->tune.blocks_per_cu);
+   b->grid2 = b->work2 ? (int)std::max<int64_t>(1, std::min(b->work2, cap)) : 0;
+   b->grid3 = b->work3 ? (int)std::max<int64_t>(1, std::min(b->work3, cap)) : 0;
+}
+
 int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out)
 {
    *out = nullptr;
@@ -208,12 +240,30 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
    b->frames.assign(frames, frames + n);
    if (prm) b->prm = *prm;
    else fa_tr_default(&b->prm);
+   default_tuning(&b->tune, b->frames);
+
+   // caller's layout of x and D
+   b->x_off.resize(n);
+   std::vector<int64_t> d_off(n);
+   for (int i = 0; i < n; i++) {
+      fa_reference* r = frames[i]->ref;
+      b->x_off[i] = b->total_x;
+      d_off[i] = b->total_D;
+      b->total_x += (int64_t)r->nD * r->n_dim;
+      b->total_D += 3 * (int64_t)r->nD;
+   }
+   // internal order: 2-D frames first (k_pass<2> takes them), caller's order within a kind
+   for (int i = 0; i < n; i++)
+      if (frames[i]->ref->n_dim == 2) b->user.push_back(i);
+   for (int i = 0; i < n; i++)
+      if (frames[i]->ref->n_dim == 3) b->user.push_back(i);
 
    std::map<fa_reference*, int> ref_idx;
    std::vector<RefDev> refs;
    b->hframes.resize(n);
-   b->x_off.resize(n);
-   for (int i = 0; i < n; i++) {
+   int tr_need = 0;
+   for (int j = 0; j < n; j++) {
+      const int i = b->user[j];
       fa_reference* r = frames[i]->ref;
       auto it = ref_idx.find(r);
       int ri;
@@ -224,26 +274,26 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       } else {
          ri = it->second;
       }
-      BatchFrame& bf = b->hframes[i];
+      BatchFrame& bf = b->hframes[j];
       bf.f.data = frames[i]->d_data;
       bf.f.dtype = frames[i]->dtype;
       bf.f.offset = frames[i]->offset;
       bf.ref = ri;
       bf.n = r->nD * r->n_dim;
       bf.work_base = b->total_work;
-      bf.x_base = b->total_x;
+      bf.x_base = b->x_off[i];
       bf.vec_base = b->total_vec;
       bf.fac_base = b->total_fac;
-      bf.d_base = b->total_D;
-      b->x_off[i] = b->total_x;
+      bf.d_base = d_off[i];
       b->total_work += (int64_t)r->chunks.size();
-      b->total_x += bf.n;
+      (r->n_dim == 2 ? b->work2 : b->work3) += (int64_t)r->chunks.size();
       b->total_vec += 4 * (int64_t)bf.n;
       b->total_fac += 2 * (int64_t)r->nD * r->n_dim * r->n_dim;
-      b->total_D += 3 * (int64_t)r->nD;
-      const int need = 5 * bf.n + 6 * r->nD * r->n_dim * r->n_dim;
-      if (need <= kTrLdsMaxDoubles) b->tr_lds = std::max(b->tr_lds, need);
+      const int need = 5 * bf.n + 4 * r->nD * r->n_dim * r->n_dim;
+      tr_need = std::max(tr_need, need);
    }
+   // the update works from LDS when every frame's vectors and JtJ blocks fit
+   b->tr_lds = tr_need <= kTrLdsMaxDoubles ? tr_need : 0;
    int rc = FA_OK;
    auto fail = [&](int code) {
       delete b;
@@ -273,8 +323,17 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       (void)hipGetLastError();
       return fail(set_err(FA_E_DEVICE, "batch upload failed"));
    }
-   b->grid = (int)std::max<int64_t>(1, std::min<int64_t>(b->total_work, (int64_t)ctx->n_cu * kBlocksPerCu));
+   set_grids(b);
    *out = b;
+   return FA_OK;
+}
+
+// Enqueues the fused pass over the active frames of every kind.
+int launch_passes(fa_batch* b, hipStream_t s)
+{
+   const PassArgs pa = b->pass_args();
+   if (b->grid2) HIPCHK(launch_pass(pa, 2, b->grid2, s));
+   if (b->grid3) HIPCHK(launch_pass(pa, 3, b->grid3, s));
    return FA_OK;
 }
 
@@ -299,12 +358,11 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
    hipStream_t s = ctx->stream;
    const int n = (int)b->frames.size();
    BatchDev B = b->dev();
-   PassArgs pa = b->pass_args();
    HIPCHK(hipMemcpyAsync(b->d_x, x_inout, b->total_x * sizeof(double), hipMemcpyHostToDevice, s));
    HIPCHK(hipMemsetAsync(b->d_counters, 0, sizeof(Counters), s));
    HIPCHK(launch_tr_init(B, 0, s));
    const int64_t max_rounds = b->prm.max_evals > 0 ? (int64_t)b->prm.max_evals + 2 : 100000;
-   const int K = 4;
+   const int K = std::max(1, b->tune.check_every);
    size_t timed = 0;
    int64_t round = 0;
    bool finished = false;
@@ -316,7 +374,7 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
             CHK(event_pair(b, timed, &es, &ee));
             HIPCHK(hipEventRecord(es, s));
          }
-         HIPCHK(launch_pass(pa, b->grid, s));
+         CHK(launch_passes(b, s));
          if (es) {
             HIPCHK(hipEventRecord(ee, s));
             timed++;
@@ -333,13 +391,14 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
    HIPCHK(hipMemcpyAsync(st.data(), b->d_state, n * sizeof(TrState), hipMemcpyDeviceToHost, s));
    HIPCHK(hipStreamSynchronize(s));
    if (stats)
-      for (int i = 0; i < n; i++) {
-         stats[i].iterations = st[i].iterations;
-         stats[i].evaluations = st[i].evaluations;
-         stats[i].rejected = st[i].rejected;
-         stats[i].status = st[i].done ? st[i].status : FA_STOP_MAX_EVALS;
-         stats[i].f_final = st[i].f;
-         stats[i].radius_final = st[i].radius;
+      for (int j = 0; j < n; j++) {
+         fa_align_stats& o = stats[b->user[j]];
+         o.iterations = st[j].iterations;
+         o.evaluations = st[j].evaluations;
+         o.rejected = st[j].rejected;
+         o.status = st[j].done ? st[j].status : FA_STOP_MAX_EVALS;
+         o.f_final = st[j].f;
+         o.radius_final = st[j].radius;
       }
    if (timing) {
       double pass_ms = 0;
@@ -367,7 +426,7 @@ int batch_eval(fa_batch* b, const double* D, double* err, double* jac)
    HIPCHK(hipMemcpyAsync(b->d_D, D, b->total_D * sizeof(double), hipMemcpyHostToDevice, s));
    HIPCHK(launch_tr_init(B, 1, s));
    HIPCHK(launch_schedule(B, 1, s));
-   HIPCHK(launch_pass(b->pass_args(), b->grid, s));
+   CHK(launch_passes(b, s));
    HIPCHK(launch_reduce_eval(B, s));
    std::vector<TrState> st(n);
    std::vector<double> vec;
@@ -377,10 +436,11 @@ int batch_eval(fa_batch* b, const double* D, double* err, double* jac)
       HIPCHK(hipMemcpyAsync(vec.data(), b->d_vec, b->total_vec * sizeof(double), hipMemcpyDeviceToHost, s));
    }
    HIPCHK(hipStreamSynchronize(s));
-   for (int i = 0; i < n; i++) {
-      if (err) err[i] = st[i].f;
+   for (int j = 0; j < n; j++) {
+      const int i = b->user[j];
+      if (err) err[i] = st[j].f;
       if (jac) {
-         const BatchFrame& bf = b->hframes[i];
+         const BatchFrame& bf = b->hframes[j];
          std::memcpy(jac + b->x_off[i], vec.data() + bf.vec_base + bf.n, bf.n * sizeof(double));
       }
    }
@@ -788,7 +848,7 @@ int fa_warp_interpolated(fa_frame* f, const double* D, float* out, float invalid
    F.offset = f->offset;
    int grid = (int)std::min<int64_t>((int64_t)r->chunks.size(), (int64_t)r->ctx->n_cu * kBlocksPerCu);
    HIPCHK(hipMemcpyAsync(b->d_D, D, 3 * (size_t)r->nD * sizeof(double), hipMemcpyHostToDevice, s));
-   HIPCHK(launch_warp(r->dev, F, b->d_D, b->d_warp, invalid_value, kLdsCapFloats, 1, grid, s));
+   HIPCHK(launch_warp(r->dev, F, b->d_D, b->d_warp, invalid_value, b->tune.lds_floats, b->tune.use_lds, grid, s));
    HIPCHK(hipMemcpyAsync(out, b->d_warp, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
    HIPCHK(hipStreamSynchronize(s));
    return FA_OK;
@@ -851,6 +911,25 @@ int fa_batch_set_timing(fa_batch* b, int32_t enable)
 {
    if (!b) return set_err(FA_E_ARG, "null batch");
    b->timing = enable ? 1 : 0;
+   return FA_OK;
+}
+
+int fa_batch_get_tuning(const fa_batch* b, fa_tuning* t)
+{
+   if (!b || !t) return set_err(FA_E_ARG, "null argument");
+   *t = b->tune;
+   return FA_OK;
+}
+
+int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t)
+{
+   if (!b || !t) return set_err(FA_E_ARG, "null argument");
+   if (t->lds_floats < 0 || t->lds_floats > 15360 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
+       t->check_every < 1)
+      return set_err(FA_E_ARG, "tuning out of range (lds_floats <= 15360, 1 <= blocks_per_cu <= 64)");
+   std::lock_guard<std::mutex> lk(b->ctx->mu);
+   b->tune = *t;
+   set_grids(b);
    return FA_OK;
 }
 

@@ -93,6 +93,7 @@ struct Counters {
    int32_t n_active_frames;
    int32_t pad;
    int64_t n_active_work;
+   int64_t n_active_work2;    // work items of active 2-D frames (they come first)
    unsigned long long bytes;  // algorithmic bytes scheduled so far
    unsigned long long rounds;
 };

@@ -268,13 +268,14 @@ __device__ __forceinline__ bool chol_block(double (&A)[M][M])
       for (int k = 0; k < j; k++) s -= A[j][k] * A[j][k];
       if (!(s > 0) || !isfinite(s)) return false;
       const double l = sqrt(s);
+      const double il = 1.0 / l;
       A[j][j] = l;
 #pragma unroll
       for (int i = j + 1; i < M; i++) {
          double t = A[i][j];
 #pragma unroll
          for (int k = 0; k < j; k++) t -= A[i][k] * A[j][k];
-         A[i][j] = t / l;
+         A[i][j] = t * il;
       }
 #pragma unroll
       for (int i = 0; i < j; i++) A[i][j] = 0;
@@ -284,7 +285,7 @@ __device__ __forceinline__ bool chol_block(double (&A)[M][M])
 
 // L L^T = H + lambda I with H block-tridiagonal (diag blocks, sub-diagonal blocks (k+1,k)).
 template <int M>
-__device__ bool bt_chol(int nb, const double* diag, const double* sub, double lambda, double* Ld,
+__device__ __forceinline__ bool bt_chol(int nb, const double* diag, const double* sub, double lambda, double* Ld,
                         double* Ls)
 {
    constexpr int MM = M * M;
@@ -325,7 +326,7 @@ __device__ bool bt_chol(int nb, const double* diag, const double* sub, double la
                double t = sub[k * MM + i * M + j];
 #pragma unroll
                for (int q = 0; q < j; q++) t -= C[i][q] * A[j][q];
-               C[i][j] = t / A[j][j];
+               C[i][j] = t * (1.0 / A[j][j]);
             }
 #pragma unroll
          for (int i = 0; i < M; i++)
@@ -338,7 +339,7 @@ __device__ bool bt_chol(int nb, const double* diag, const double* sub, double la
 
 // y = L^-1 b.
 template <int M>
-__device__ void bt_fwd(int nb, const double* Ld, const double* Ls, const double* b, double* y)
+__device__ __forceinline__ void bt_fwd(int nb, const double* Ld, const double* Ls, const double* b, double* y)
 {
    constexpr int MM = M * M;
    double yp[M];
@@ -354,12 +355,15 @@ __device__ void bt_fwd(int nb, const double* Ld, const double* Ls, const double*
             for (int t = 0; t < M; t++) r[i] -= Ls[(k - 1) * MM + i * M + t] * yp[t];
       }
       const double* L = &Ld[k * MM];
+      double il[M];
+#pragma unroll
+      for (int i = 0; i < M; i++) il[i] = 1.0 / L[i * M + i];
 #pragma unroll
       for (int i = 0; i < M; i++) {
          double t = r[i];
 #pragma unroll
          for (int q = 0; q < i; q++) t -= L[i * M + q] * yk[q];
-         yk[i] = t / L[i * M + i];
+         yk[i] = t * il[i];
       }
 #pragma unroll
       for (int i = 0; i < M; i++) {
@@ -371,7 +375,7 @@ __device__ void bt_fwd(int nb, const double* Ld, const double* Ls, const double*
 
 // x = L^-T y.
 template <int M>
-__device__ void bt_bwd(int nb, const double* Ld, const double* Ls, const double* y, double* x)
+__device__ __forceinline__ void bt_bwd(int nb, const double* Ld, const double* Ls, const double* y, double* x)
 {
    constexpr int MM = M * M;
    double xn[M];
@@ -387,12 +391,15 @@ __device__ void bt_bwd(int nb, const double* Ld, const double* Ls, const double*
             for (int t = 0; t < M; t++) r[i] -= Ls[k * MM + t * M + i] * xn[t];
       }
       const double* L = &Ld[k * MM];
+      double il[M];
+#pragma unroll
+      for (int i = 0; i < M; i++) il[i] = 1.0 / L[i * M + i];
 #pragma unroll
       for (int i = M - 1; i >= 0; i--) {
          double t = r[i];
 #pragma unroll
          for (int q = i + 1; q < M; q++) t -= L[q * M + i] * xk[q];
-         xk[i] = t / L[i * M + i];
+         xk[i] = t * il[i];
       }
 #pragma unroll
       for (int i = 0; i < M; i++) {
@@ -408,7 +415,7 @@ __device__ bool d_bt_chol(int nb, int m, const double* diag, const double* sub, 
    return m == 2 ? bt_chol<2>(nb, diag, sub, lambda, Ld, Ls) : bt_chol<3>(nb, diag, sub, lambda, Ld, Ls);
 }
 
-__device__ double d_norm(int n, const double* v)
+__device__ __forceinline__ double d_norm(int n, const double* v)
 {
    double s = 0;
    for (int i = 0; i < n; i++) s += v[i] * v[i];
@@ -653,11 +660,26 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
 #pragma unroll
    for (int v = 0; v < NV; v++) acc[v] = KF{0.f, 0.f};
 
+   // Each thread takes kUnroll pixels per step (kBlock apart) and issues their reference
+   // records before using any, so several loads are in flight per wave.
+   constexpr int kUnroll = 4;
    PixelWalk w;
    w.init(q0 + threadIdx.x, nx);
-   for (int p = ch.p0 + threadIdx.x; p < ch.p1; p += kBlock, w.step(nx)) {
-      const float4 rec = R.rg4[p];
-      const float f = (M == 3) ? R.df[p] : rec.w;
+   for (int pb = ch.p0 + threadIdx.x; pb < ch.p1; pb += kUnroll * kBlock) {
+      float4 recs[kUnroll];
+      float fs[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+         const int pu = min(pb + u * kBlock, ch.p1 - 1);
+         recs[u] = R.rg4[pu];
+         fs[u] = (M == 3) ? R.df[pu] : recs[u].w;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++, w.step(nx)) {
+      const int p = pb + u * kBlock;
+      if (p >= ch.p1) break;
+      const float4 rec = recs[u];
+      const float f = fs[u];
       const double fd = (double)f, omf = 1.0 - fd;
       double sp[3];
 #pragma unroll
@@ -722,6 +744,7 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
             kadd(acc[1 + M + c], (g[c] * f) * ej);
          }
       }
+      }
    }
    if (MODE == MODE_PASS) block_reduce_fast<NV>(acc, (KF(*)[NV])sred, out);
    else __syncthreads();
@@ -748,18 +771,23 @@ __device__ __forceinline__ void dispatch_chunk(const RefDev& R, const FrameDev& 
 
 // Persistent over the active work list: workgroup b takes a contiguous share of the items,
 // with workgroups renumbered so that each XCD (workgroups are dealt to the 8 XCDs round
-// robin) owns one contiguous range and the frame rows it stages stay in its L2.
+// robin) owns one contiguous range and the frame rows it stages stay in its L2. The 2-D
+// frames of a batch come first; k_pass<2> takes their items, k_pass<3> the rest, so each
+// variant is compiled for its own register budget.
+template <int M>
 __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a)
 {
    extern __shared__ float lds[];
    __shared__ KF sred[kBlock / 64][7];
-   const int64_t W = a.counters->n_active_work;
+   const int64_t W2 = a.counters->n_active_work2;
+   const int64_t Wb = M == 2 ? 0 : W2;
+   const int64_t W = (M == 2 ? W2 : a.counters->n_active_work) - Wb;
    const int nfa = a.counters->n_active_frames;
    if (W <= 0 || nfa <= 0) return;
    const int G = gridDim.x;
    int lb = blockIdx.x;
    if (G % 8 == 0) lb = (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
-   const int64_t w0 = W * lb / G, w1 = W * (lb + 1) / G;
+   const int64_t w0 = Wb + W * lb / G, w1 = Wb + W * (lb + 1) / G;
    int j = 0;
    int64_t jb = 0, je = -1;
    for (int64_t wi = w0; wi < w1; wi++) {
@@ -779,8 +807,12 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a)
       const int c = (int)(wi - jb);
       const Chunk ch = R.chunks[c];
       double* out = a.partials + (bf.work_base + c) * kPassStride;
-      dispatch_chunk<MODE_PASS>(R, bf.f, ch, a.Dtrial + bf.d_base, lds, a.lds_cap, a.use_lds != 0,
-                                out, nullptr, 0.f, sred);
+      if (bf.f.dtype == FA_U16)
+         pass_chunk<M, uint16_t, MODE_PASS>(R, bf.f, ch, a.Dtrial + bf.d_base, lds, a.lds_cap,
+                                            a.use_lds != 0, out, nullptr, 0.f, sred);
+      else
+         pass_chunk<M, float, MODE_PASS>(R, bf.f, ch, a.Dtrial + bf.d_base, lds, a.lds_cap,
+                                         a.use_lds != 0, out, nullptr, 0.f, sred);
    }
 }
 
@@ -801,33 +833,38 @@ __global__ __launch_bounds__(kBlock) void k_warp(RefDev R, FrameDev F, const dou
 __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
 {
    __shared__ int64_t s_work[1024];
+   __shared__ int64_t s_work2[1024];
    __shared__ int32_t s_fr[1024];
    __shared__ unsigned long long s_bytes[1024];
    const int t = threadIdx.x, T = blockDim.x;
    const int n = B.n_frames;
    const int f0 = (int)((int64_t)n * t / T), f1 = (int)((int64_t)n * (t + 1) / T);
-   int64_t work = 0;
+   int64_t work = 0, work2 = 0;
    int32_t fr = 0;
    unsigned long long bytes = 0;
    for (int f = f0; f < f1; f++) {
       if (B.state[f].done) continue;
       const RefDev& R = B.refs[B.frames[f].ref];
       work += R.n_chunks;
+      if (R.n_dim == 2) work2 += R.n_chunks;
       fr++;
       bytes += (unsigned long long)R.n_px *
                (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + (R.n_dim == 3 ? 20 : 16));
    }
    s_work[t] = work;
+   s_work2[t] = work2;
    s_fr[t] = fr;
    s_bytes[t] = bytes;
    __syncthreads();
    // inclusive scan (Hillis-Steele)
    for (int off = 1; off < T; off <<= 1) {
       int64_t aw = t >= off ? s_work[t - off] : 0;
+      int64_t aw2 = t >= off ? s_work2[t - off] : 0;
       int32_t af = t >= off ? s_fr[t - off] : 0;
       unsigned long long ab = t >= off ? s_bytes[t - off] : 0;
       __syncthreads();
       s_work[t] += aw;
+      s_work2[t] += aw2;
       s_fr[t] += af;
       s_bytes[t] += ab;
       __syncthreads();
@@ -845,6 +882,7 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
       B.active_base[s_fr[t]] = s_work[t];
       B.counters->n_active_frames = s_fr[t];
       B.counters->n_active_work = s_work[t];
+      B.counters->n_active_work2 = s_work2[t];
       if (account) {
          B.counters->bytes += s_bytes[t];
          B.counters->rounds += 1;
@@ -852,125 +890,201 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
    }
 }
 
-// ---- per-frame reductions of the pass partials ------------------------------------------
-// SSD: lanes take chunks strided, then a fixed tree; Jtr: one node per lane, chunks of
-// interval k-1 (weight f) and interval k (weight 1-f) in order. Fixed order, so the result
-// does not depend on which other frames share the launch.
-__device__ double reduce_ssd(const RefDev& R, const double* P, double* sh)
-{
-   double s = 0;
-   for (int c = threadIdx.x; c < R.n_chunks; c += blockDim.x) s += P[(int64_t)c * kPassStride];
-   sh[threadIdx.x] = s;
-   __syncthreads();
-   for (int off = blockDim.x / 2; off >= 1; off >>= 1) {
-      if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-      __syncthreads();
-   }
-   double r = sh[0];
-   __syncthreads();
-   return r;
-}
-
-__device__ void reduce_jtr(const RefDev& R, const double* P, double* g)
-{
-   const int m = R.n_dim;
-   for (int k = threadIdx.x; k < R.nD; k += blockDim.x) {
-      double acc[3] = {0, 0, 0};
-      if (k >= 1)
-         for (int c = R.ichunk[k - 1]; c < R.ichunk[k]; c++)
-            for (int d = 0; d < m; d++) acc[d] += P[(int64_t)c * kPassStride + 1 + m + d];
-      if (k <= R.nD - 2)
-         for (int c = R.ichunk[k]; c < R.ichunk[k + 1]; c++)
-            for (int d = 0; d < m; d++) acc[d] += P[(int64_t)c * kPassStride + 1 + d];
-      for (int d = 0; d < m; d++) g[k * m + d] = acc[d];
-   }
-}
-
-__global__ __launch_bounds__(64) void k_reduce_eval(BatchDev B)
-{
-   __shared__ double sh[64];
-   const int j = blockIdx.x;
-   const BatchFrame& bf = B.frames[j];
-   const RefDev& R = B.refs[bf.ref];
-   const double* P = B.partials + bf.work_base * kPassStride;
-   double f = reduce_ssd(R, P, sh);
-   double* g = B.vec + bf.vec_base + (int64_t)bf.n;
-   reduce_jtr(R, P, g);
-   if (threadIdx.x == 0) B.state[j].f = f;
-}
-
 // ---- device trust region ---------------------------------------------------------------
 // dlib find_min_trust_region + objective_delta_stop_strategy, as the reference calls it
 // (FrameWarpAligner.cpp:205-209), over f = SSD, g = Jtr, B = H (OptimisationModel.cpp:45-63).
 // The subproblem is Nocedal & Wright Alg. 4.3 on the block-tridiagonal H.
+//
+// One wave (64 lanes) per frame. Every lane runs the same scalar logic on wave-reduced values,
+// so the control flow is uniform; vector work is spread over the lanes. The block-triangular
+// solves are recurrences y_k = A_k y_(k-1) + c_k and are evaluated as a parallel prefix scan
+// of affine maps (composition is associative): each lane composes the maps of its segment of
+// blocks, the lanes scan the segment maps, and each lane replays its segment from the value
+// entering it. Only the block Cholesky of H + lambda I (lambda > 0) is sequential.
 
 struct TrScratch {
    double *x, *p, *g, *t1, *t2, *Ld, *Ls;
 };
 
-template <int M>
-__device__ int d_tr_subproblem(const RefDev& R, const TrScratch& S, const double* g,
-                               double radius, double eps, int max_iter, double* p,
-                               double* lam_out)
+__device__ __forceinline__ double wave_sum(double v)
 {
-   const int nb = R.nD, n = nb * M;
-   double gnorm = d_norm(n, g);
-   for (int i = 0; i < n; i++) p[i] = 0;
-   *lam_out = 0;
-   if (gnorm == 0) return 1;
-   const double eig_lo = R.eig_lo;
-   double lam = 0, lam_lo = 0, lam_hi = gnorm / radius + (eig_lo < 0 ? -eig_lo : 0);
-   const double *Ld = nullptr, *Ls = nullptr;
-   int it;
-   for (it = 0; it < max_iter; it++) {
-      bool ok;
-      if (lam == 0 && R.h_pd) {
-         Ld = R.L0d;
-         Ls = R.L0s;
-         ok = true;
-      } else if (lam == 0) {
-         ok = false;
-      } else {
-         ok = bt_chol<M>(nb, R.Hdiag, R.Hsub, lam, S.Ld, S.Ls);
-         Ld = S.Ld;
-         Ls = S.Ls;
-      }
-      if (!ok) {
-         lam_lo = lam;
-         lam = 0.5 * (lam_lo + lam_hi);
-         continue;
-      }
-      for (int i = 0; i < n; i++) S.t2[i] = -g[i];
-      bt_fwd<M>(nb, Ld, Ls, S.t2, S.t1);
-      bt_bwd<M>(nb, Ld, Ls, S.t1, p);
-      *lam_out = lam;
-      double pn = d_norm(n, p);
-      if (lam == 0 && pn <= radius) {
-         it++;
-         break;
-      }
-      if (fabs(pn - radius) <= eps * radius) {
-         it++;
-         break;
-      }
-      if (pn > radius) lam_lo = lam;
-      else lam_hi = lam;
-      bt_fwd<M>(nb, Ld, Ls, p, S.t1);
-      double qn = d_norm(n, S.t1);
-      double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
-      if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
-      lam = lam_new;
-   }
-   return it;
+#pragma unroll
+   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+   return __shfl(v, 0, 64);
+}
+
+__device__ __forceinline__ double wave_dot(int n, const double* a, const double* b)
+{
+   double s = 0;
+   for (int i = threadIdx.x; i < n; i += 64) s += a[i] * b[i];
+   return wave_sum(s);
 }
 
 template <int M>
-__device__ double d_quad(const RefDev& R, const double* p)
+struct Aff {
+   double A[M][M];
+   double c[M];
+};
+
+template <int M>
+__device__ __forceinline__ Aff<M> aff_identity()
+{
+   Aff<M> F;
+#pragma unroll
+   for (int i = 0; i < M; i++) {
+      F.c[i] = 0;
+#pragma unroll
+      for (int j = 0; j < M; j++) F.A[i][j] = i == j ? 1.0 : 0.0;
+   }
+   return F;
+}
+
+// G after F: y -> G.A (F.A y + F.c) + G.c
+template <int M>
+__device__ __forceinline__ Aff<M> aff_then(const Aff<M>& F, const Aff<M>& G)
+{
+   Aff<M> H;
+#pragma unroll
+   for (int i = 0; i < M; i++) {
+      double c = G.c[i];
+#pragma unroll
+      for (int q = 0; q < M; q++) c += G.A[i][q] * F.c[q];
+      H.c[i] = c;
+#pragma unroll
+      for (int j = 0; j < M; j++) {
+         double a = 0;
+#pragma unroll
+         for (int q = 0; q < M; q++) a += G.A[i][q] * F.A[q][j];
+         H.A[i][j] = a;
+      }
+   }
+   return H;
+}
+
+template <int M>
+__device__ __forceinline__ Aff<M> aff_shfl_up(const Aff<M>& F, int d)
+{
+   Aff<M> O;
+#pragma unroll
+   for (int i = 0; i < M; i++) {
+      O.c[i] = __shfl_up(F.c[i], d, 64);
+#pragma unroll
+      for (int j = 0; j < M; j++) O.A[i][j] = __shfl_up(F.A[i][j], d, 64);
+   }
+   return O;
+}
+
+// Inverse of a lower-triangular block.
+template <int M>
+__device__ __forceinline__ void tri_inv(const double* L, double (&Li)[M][M])
+{
+#pragma unroll
+   for (int i = 0; i < M; i++) {
+      const double il = 1.0 / L[i * M + i];
+#pragma unroll
+      for (int j = 0; j < M; j++) {
+         double t = i == j ? 1.0 : 0.0;
+#pragma unroll
+         for (int q = 0; q < i; q++) t -= L[i * M + q] * Li[q][j];
+         Li[i][j] = j <= i ? t * il : 0.0;
+      }
+   }
+}
+
+// Forward (L y = b) or backward (L^T x = b) solve with the block factor (Ld, Ls) of a
+// block-tridiagonal matrix, across the wave. The caller synchronises before using out.
+template <int M, bool BWD>
+__device__ void bt_solve_scan(int nb, const double* Ld, const double* Ls, const double* b,
+                              double* out)
+{
+   constexpr int MM = M * M;
+   const int lane = threadIdx.x;
+   const int per = (nb + 63) / 64;
+   const int s0 = lane * per, s1 = min(nb, s0 + per);
+   // scan position s visits block k = s (forward) or nb-1-s (backward)
+   auto term = [&](int s, Aff<M>& F) {
+      const int k = BWD ? nb - 1 - s : s;
+      double Li[M][M];
+      tri_inv<M>(&Ld[k * MM], Li);
+      if (!BWD) {
+         // y_k = Li (b_k - L(k,k-1) y_(k-1))
+#pragma unroll
+         for (int i = 0; i < M; i++) {
+            double c = 0;
+#pragma unroll
+            for (int q = 0; q < M; q++) c += Li[i][q] * b[k * M + q];
+            F.c[i] = c;
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+               double a = 0;
+               if (k > 0)
+#pragma unroll
+                  for (int q = 0; q < M; q++) a -= Li[i][q] * Ls[(k - 1) * MM + q * M + j];
+               F.A[i][j] = a;
+            }
+         }
+      } else {
+         // x_k = Li^T (b_k - L(k+1,k)^T x_(k+1))
+#pragma unroll
+         for (int i = 0; i < M; i++) {
+            double c = 0;
+#pragma unroll
+            for (int q = 0; q < M; q++) c += Li[q][i] * b[k * M + q];
+            F.c[i] = c;
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+               double a = 0;
+               if (k + 1 < nb)
+#pragma unroll
+                  for (int q = 0; q < M; q++) a -= Li[q][i] * Ls[k * MM + j * M + q];
+               F.A[i][j] = a;
+            }
+         }
+      }
+   };
+   Aff<M> S = aff_identity<M>();
+   for (int sp = s0; sp < s1; sp++) {
+      Aff<M> F;
+      term(sp, F);
+      S = aff_then<M>(S, F);
+   }
+#pragma unroll
+   for (int off = 1; off < 64; off <<= 1) {
+      Aff<M> O = aff_shfl_up<M>(S, off);
+      if (lane >= off) S = aff_then<M>(O, S);
+   }
+   Aff<M> P = aff_shfl_up<M>(S, 1);
+   double y[M];
+#pragma unroll
+   for (int i = 0; i < M; i++) y[i] = lane > 0 ? P.c[i] : 0.0;
+   for (int sp = s0; sp < s1; sp++) {
+      Aff<M> F;
+      term(sp, F);
+      const int k = BWD ? nb - 1 - sp : sp;
+      double yn[M];
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+         double t = F.c[i];
+#pragma unroll
+         for (int q = 0; q < M; q++) t += F.A[i][q] * y[q];
+         yn[i] = t;
+      }
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+         out[k * M + i] = yn[i];
+         y[i] = yn[i];
+      }
+   }
+}
+
+// p' H p over the lanes.
+template <int M>
+__device__ __forceinline__ double wave_quad(const RefDev& R, const double* p)
 {
    constexpr int MM = M * M;
    const int nb = R.nD;
    double s = 0;
-   for (int k = 0; k < nb; k++) {
+   for (int k = threadIdx.x; k < nb; k += 64) {
 #pragma unroll
       for (int a = 0; a < M; a++)
 #pragma unroll
@@ -982,10 +1096,74 @@ __device__ double d_quad(const RefDev& R, const double* p)
             for (int b = 0; b < M; b++)
                s += 2.0 * p[(k + 1) * M + a] * R.Hsub[k * MM + a * M + b] * p[k * M + b];
    }
-   return s;
+   return wave_sum(s);
 }
 
-__device__ TrScratch scratch_of(const BatchDev& B, const BatchFrame& bf)
+template <int M>
+__device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* g, double radius,
+                             double eps, int max_iter, double* p, double* lam_out)
+{
+   const int nb = R.nD, n = nb * M;
+   const double gnorm = sqrt(wave_dot(n, g, g));
+   for (int i = threadIdx.x; i < n; i += 64) p[i] = 0;
+   *lam_out = 0;
+   __syncthreads();
+   if (gnorm == 0) return 1;
+   const double eig_lo = R.eig_lo;
+   double lam = 0, lam_lo = 0, lam_hi = gnorm / radius + (eig_lo < 0 ? -eig_lo : 0);
+   int it;
+   for (it = 0; it < max_iter; it++) {
+      const double *Ld, *Ls;
+      bool ok;
+      if (lam == 0 && R.h_pd) {
+         Ld = R.L0d;
+         Ls = R.L0s;
+         ok = true;
+      } else if (lam == 0) {
+         Ld = Ls = nullptr;
+         ok = false;
+      } else {
+         int okl = 0;
+         if (threadIdx.x == 0) okl = bt_chol<M>(nb, R.Hdiag, R.Hsub, lam, S.Ld, S.Ls) ? 1 : 0;
+         ok = __shfl(okl, 0, 64) != 0;
+         __syncthreads();
+         Ld = S.Ld;
+         Ls = S.Ls;
+      }
+      if (!ok) {
+         lam_lo = lam;
+         lam = 0.5 * (lam_lo + lam_hi);
+         continue;
+      }
+      for (int i = threadIdx.x; i < n; i += 64) S.t2[i] = -g[i];
+      __syncthreads();
+      bt_solve_scan<M, false>(nb, Ld, Ls, S.t2, S.t1);
+      __syncthreads();
+      bt_solve_scan<M, true>(nb, Ld, Ls, S.t1, p);
+      __syncthreads();
+      *lam_out = lam;
+      const double pn = sqrt(wave_dot(n, p, p));
+      if (lam == 0 && pn <= radius) {
+         it++;
+         break;
+      }
+      if (fabs(pn - radius) <= eps * radius) {
+         it++;
+         break;
+      }
+      if (pn > radius) lam_lo = lam;
+      else lam_hi = lam;
+      bt_solve_scan<M, false>(nb, Ld, Ls, p, S.t1);
+      __syncthreads();
+      const double qn = sqrt(wave_dot(n, S.t1, S.t1));
+      double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
+      if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
+      lam = lam_new;
+   }
+   return it;
+}
+
+__device__ __forceinline__ TrScratch scratch_of(const BatchDev& B, const BatchFrame& bf)
 {
    TrScratch S;
    double* v = B.vec + bf.vec_base;
@@ -999,6 +1177,44 @@ __device__ TrScratch scratch_of(const BatchDev& B, const BatchFrame& bf)
    const RefDev& R = B.refs[bf.ref];
    S.Ls = S.Ld + (int64_t)R.nD * R.n_dim * R.n_dim;
    return S;
+}
+
+// ---- per-frame reductions of the pass partials ------------------------------------------
+// SSD: lanes take chunks strided, then a fixed butterfly; Jtr: one node per lane, chunks of
+// interval k-1 (weight f) and interval k (weight 1-f) in order. Fixed order, so the result
+// does not depend on which other frames share the launch.
+__device__ __forceinline__ double reduce_ssd(const RefDev& R, const double* P)
+{
+   double s = 0;
+   for (int c = threadIdx.x; c < R.n_chunks; c += 64) s += P[(int64_t)c * kPassStride];
+   return wave_sum(s);
+}
+
+__device__ __forceinline__ void reduce_jtr(const RefDev& R, const double* P, double* g)
+{
+   const int m = R.n_dim;
+   for (int k = threadIdx.x; k < R.nD; k += 64) {
+      double acc[3] = {0, 0, 0};
+      if (k >= 1)
+         for (int c = R.ichunk[k - 1]; c < R.ichunk[k]; c++)
+            for (int d = 0; d < m; d++) acc[d] += P[(int64_t)c * kPassStride + 1 + m + d];
+      if (k <= R.nD - 2)
+         for (int c = R.ichunk[k]; c < R.ichunk[k + 1]; c++)
+            for (int d = 0; d < m; d++) acc[d] += P[(int64_t)c * kPassStride + 1 + d];
+      for (int d = 0; d < m; d++) g[k * m + d] = acc[d];
+   }
+}
+
+__global__ __launch_bounds__(64) void k_reduce_eval(BatchDev B)
+{
+   const int j = blockIdx.x;
+   const BatchFrame& bf = B.frames[j];
+   const RefDev& R = B.refs[bf.ref];
+   const double* P = B.partials + bf.work_base * kPassStride;
+   const double f = reduce_ssd(R, P);
+   double* g = B.vec + bf.vec_base + (int64_t)bf.n;
+   reduce_jtr(R, P, g);
+   if (threadIdx.x == 0) B.state[j].f = f;
 }
 
 // Starts every frame: x (already uploaded) is the first evaluation point.
@@ -1025,23 +1241,19 @@ __global__ void k_tr_init(BatchDev B, int eval_only)
 // One wave per frame. The frame's vectors and the reference's JtJ blocks and factor are
 // copied into LDS first (when they fit), because the triangular solves are a sequential chain
 // that would otherwise wait on a global-memory round trip at every block.
-__global__ __launch_bounds__(64) void k_tr_update(BatchDev B)
+template <int M, bool LDS>
+__device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
 {
-   extern __shared__ double smem[];
-   __shared__ double sh[64];
-   __shared__ int s_accept;
-   const int j = blockIdx.x;
-   if (B.state[j].done) return;
+   constexpr int MM = M * M;
    const BatchFrame& bf = B.frames[j];
    RefDev R = B.refs[bf.ref];
    const TrParamsDev& prm = B.prm;
    const TrScratch G = scratch_of(B, bf);
    TrScratch S = G;
    const double* P = B.partials + bf.work_base * kPassStride;
-   const int n = bf.n, m = R.n_dim;
-   const int nb = R.nD * m * m;
-   const bool in_lds = 5 * n + 6 * nb <= B.tr_lds;
-   if (in_lds) {
+   const int n = bf.n, nb = R.nD;
+   if (LDS) {
+      // vectors, JtJ blocks and the factor scratch of lambda > 0 in LDS
       double* q = smem;
       S.x = q;
       S.p = q + n;
@@ -1049,135 +1261,133 @@ __global__ __launch_bounds__(64) void k_tr_update(BatchDev B)
       S.t1 = q + 3 * n;
       S.t2 = q + 4 * n;
       q += 5 * n;
-      double *hd = q, *hs = q + nb, *l0d = q + 2 * nb, *l0s = q + 3 * nb;
-      S.Ld = q + 4 * nb;
-      S.Ls = q + 5 * nb;
-      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      double *hd = q, *hs = q + nb * MM;
+      S.Ld = q + 2 * nb * MM;
+      S.Ls = q + 3 * nb * MM;
+      for (int i = threadIdx.x; i < n; i += 64) {
          S.x[i] = G.x[i];
          S.p[i] = G.p[i];
          S.g[i] = G.g[i];
       }
-      for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      for (int i = threadIdx.x; i < nb * MM; i += 64) {
          hd[i] = R.Hdiag[i];
          hs[i] = R.Hsub[i];
-         l0d[i] = R.L0d[i];
-         l0s[i] = R.L0s[i];
       }
       R.Hdiag = hd;
       R.Hsub = hs;
-      R.L0d = l0d;
-      R.L0s = l0s;
       __syncthreads();
    }
-   const double f_new = reduce_ssd(R, P, sh);
-
+   const double f_new = reduce_ssd(R, P);
    TrState st = B.state[j];
-   // Decide on the evaluated point (lane 0), then reduce Jtr on all lanes if it is kept.
-   if (threadIdx.x == 0) {
-      int accept = 0;
-      if (st.phase == 0) {
-         st.f = f_new;
-         st.evaluations = 1;
-         st.stale = 0;
-         accept = 2;  // first point: take f and g, no step bookkeeping
+   int accept = 0;
+   if (st.phase == 0) {
+      st.f = f_new;
+      st.evaluations = 1;
+      st.stale = 0;
+      accept = 1;  // first point: take f and g, no step bookkeeping
+   } else {
+      st.evaluations++;
+      const double gp = wave_dot(n, S.g, S.p);
+      const double pred = -(0.5 * wave_quad<M>(R, S.p) + gp);
+      const double meas = st.f - f_new;
+      if (fabs(pred) <= fabs(meas) * DBL_EPSILON) {
+         st.status = FA_STOP_NO_PROGRESS;
+         st.done = 1;
       } else {
-         st.evaluations++;
-         double gp = 0;
-         for (int i = 0; i < n; i++) gp += S.g[i] * S.p[i];
-         double pred = -(0.5 * (m == 2 ? d_quad<2>(R, S.p) : d_quad<3>(R, S.p)) + gp);
-         double meas = st.f - f_new;
-         if (fabs(pred) <= fabs(meas) * DBL_EPSILON) {
-            st.status = FA_STOP_NO_PROGRESS;
+         const double rho = meas / fabs(pred);
+         if (!isfinite(rho)) {
+            st.status = FA_STOP_NONFINITE;
             st.done = 1;
          } else {
-            double rho = meas / fabs(pred);
-            if (!isfinite(rho)) {
-               st.status = FA_STOP_NONFINITE;
-               st.done = 1;
+            int collapse = 0;
+            if (rho < 0.25) {
+               st.radius *= 0.25;
+               if (st.radius <= st.radius0 * DBL_EPSILON) collapse = 1;
+            } else if (rho > 0.75 && st.sub_iters > 1) {
+               st.radius = 2 * st.radius < 1000 ? 2 * st.radius : 1000;
+            }
+            if (rho > 0) {
+               for (int i = threadIdx.x; i < n; i += 64) S.x[i] = S.x[i] + S.p[i];
+               st.f = f_new;
+               st.stale = 0;
+               accept = 1;
             } else {
-               int collapse = 0;
-               if (rho < 0.25) {
-                  st.radius *= 0.25;
-                  if (st.radius <= st.radius0 * DBL_EPSILON) collapse = 1;
-               } else if (rho > 0.75 && st.sub_iters > 1) {
-                  st.radius = 2 * st.radius < 1000 ? 2 * st.radius : 1000;
-               }
-               if (rho > 0) {
-                  for (int i = 0; i < n; i++) S.x[i] = S.x[i] + S.p[i];
-                  st.f = f_new;
-                  st.stale = 0;
-                  accept = 1;
-               } else {
-                  st.stale = 1;
-                  st.rejected++;
-                  if (prm.skip_repeat && st.lambda == 0 && !collapse) {
-                     double pn = d_norm(n, S.p);
-                     while (pn <= st.radius) {
-                        st.radius *= 0.25;
-                        st.rejected++;
-                        if (st.radius <= st.radius0 * DBL_EPSILON) {
-                           collapse = 1;
-                           break;
-                        }
+               st.stale = 1;
+               st.rejected++;
+               if (prm.skip_repeat && st.lambda == 0 && !collapse) {
+                  // the unchanged interior step would be proposed, evaluated and rejected
+                  // again while it fits the shrunken region
+                  const double pn = sqrt(wave_dot(n, S.p, S.p));
+                  while (pn <= st.radius) {
+                     st.radius *= 0.25;
+                     st.rejected++;
+                     if (st.radius <= st.radius0 * DBL_EPSILON) {
+                        collapse = 1;
+                        break;
                      }
                   }
                }
-               if (collapse) {
-                  st.status = FA_STOP_RADIUS;
-                  st.done = 1;
-               }
+            }
+            if (collapse) {
+               st.status = FA_STOP_RADIUS;
+               st.done = 1;
             }
          }
       }
-      s_accept = accept;
    }
+   if (accept) reduce_jtr(R, P, S.g);
    __syncthreads();
-   if (s_accept) reduce_jtr(R, P, S.g);
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      if (!st.done && !st.stale) {
-         st.iterations++;
-         if (st.been_used) {
-            double df = fabs(st.f - st.prev_f);
-            if (prm.max_iter && st.iterations > prm.max_iter) {
-               st.status = FA_STOP_MAX_ITER;
-               st.done = 1;
-            } else if (df < prm.min_delta) {
-               st.status = FA_STOP_ABS_DELTA;
-               st.done = 1;
-            } else if (prm.rel_delta > 0 && df < prm.rel_delta * fabs(st.prev_f)) {
-               st.status = FA_STOP_REL_DELTA;
-               st.done = 1;
-            }
+   if (!st.done && !st.stale) {
+      st.iterations++;
+      if (st.been_used) {
+         const double df = fabs(st.f - st.prev_f);
+         if (prm.max_iter && st.iterations > prm.max_iter) {
+            st.status = FA_STOP_MAX_ITER;
+            st.done = 1;
+         } else if (df < prm.min_delta) {
+            st.status = FA_STOP_ABS_DELTA;
+            st.done = 1;
+         } else if (prm.rel_delta > 0 && df < prm.rel_delta * fabs(st.prev_f)) {
+            st.status = FA_STOP_REL_DELTA;
+            st.done = 1;
          }
-         st.been_used = 1;
-         st.prev_f = st.f;
       }
-      if (!st.done && prm.max_evals && st.evaluations >= prm.max_evals) {
-         st.status = FA_STOP_MAX_EVALS;
-         st.done = 1;
-      }
-      if (!st.done) {
-         double lam;
-         st.sub_iters = m == 2 ? d_tr_subproblem<2>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam)
-                              : d_tr_subproblem<3>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
-         st.lambda = lam;
-         st.phase = 1;
-      }
-      B.state[j] = st;
+      st.been_used = 1;
+      st.prev_f = st.f;
    }
-   __syncthreads();
-   if (!B.state[j].done) {
+   if (!st.done && prm.max_evals && st.evaluations >= prm.max_evals) {
+      st.status = FA_STOP_MAX_EVALS;
+      st.done = 1;
+   }
+   if (!st.done) {
+      double lam;
+      st.sub_iters = tr_subproblem<M>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
+      st.lambda = lam;
+      st.phase = 1;
       double* Dt = B.Dtrial + bf.d_base;
-      for (int i = threadIdx.x; i < R.nD; i += blockDim.x)
-         for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < m ? S.x[i * m + c] + S.p[i * m + c] : 0.0;
+      for (int i = threadIdx.x; i < nb; i += 64)
+#pragma unroll
+         for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < M ? S.x[i * M + c] + S.p[i * M + c] : 0.0;
    }
-   if (in_lds)
-      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+   if (threadIdx.x == 0) B.state[j] = st;
+   if (LDS) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < n; i += 64) {
          G.x[i] = S.x[i];
          G.p[i] = S.p[i];
          G.g[i] = S.g[i];
       }
+   }
+}
+
+template <bool LDS>
+__global__ __launch_bounds__(64) void k_tr_update(BatchDev B)
+{
+   extern __shared__ double smem[];
+   const int j = blockIdx.x;
+   if (B.state[j].done) return;
+   if (B.refs[B.frames[j].ref].n_dim == 2) tr_update_frame<2, LDS>(B, j, smem);
+   else tr_update_frame<3, LDS>(B, j, smem);
 }
 
 // ---- launchers ----------------------------------------------------------------------------
@@ -1220,9 +1430,12 @@ hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, doub
    return hipGetLastError();
 }
 
-hipError_t launch_pass(const PassArgs& a, int grid, hipStream_t s)
+hipError_t launch_pass(const PassArgs& a, int n_dim, int grid, hipStream_t s)
 {
-   hipLaunchKernelGGL(k_pass, dim3(grid), dim3(kBlock), (size_t)a.lds_cap * sizeof(float), s, a);
+   if (n_dim == 2)
+      hipLaunchKernelGGL(k_pass<2>, dim3(grid), dim3(kBlock), (size_t)a.lds_cap * sizeof(float), s, a);
+   else
+      hipLaunchKernelGGL(k_pass<3>, dim3(grid), dim3(kBlock), (size_t)a.lds_cap * sizeof(float), s, a);
    return hipGetLastError();
 }
 
@@ -1248,7 +1461,10 @@ hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s)
 
 hipError_t launch_tr_update(const BatchDev& B, hipStream_t s)
 {
-   hipLaunchKernelGGL(k_tr_update, dim3(B.n_frames), dim3(64), (size_t)B.tr_lds * sizeof(double), s, B);
+   if (B.tr_lds > 0)
+      hipLaunchKernelGGL(k_tr_update<true>, dim3(B.n_frames), dim3(64), (size_t)B.tr_lds * sizeof(double), s, B);
+   else
+      hipLaunchKernelGGL(k_tr_update<false>, dim3(B.n_frames), dim3(64), 0, s, B);
    return hipGetLastError();
 }
 

@@ -13,7 +13,7 @@ hipError_t launch_reference_record(const float* T, int nz, int ny, int nx, int n
                                    const ScanDev& sc, float4* rg4, float* df, hipStream_t s);
 hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, double* Hsub,
                           double* L0d, double* L0s, double* info, hipStream_t s);
-hipError_t launch_pass(const PassArgs& a, int grid, hipStream_t s);
+hipError_t launch_pass(const PassArgs& a, int n_dim, int grid, hipStream_t s);
 hipError_t launch_warp(const RefDev& R, const FrameDev& F, const double* D, float* out,
                        float invalid, int lds_cap, int use_lds, int grid, hipStream_t s);
 hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s);

@@ -49,6 +49,12 @@ class AlignStats(C.Structure):
                 ("status", C.c_int32), ("f_final", C.c_double), ("radius_final", C.c_double)]
 
 
+class Tuning(C.Structure):
+    """Launch shape of the fused pass (fa_tuning in include/frame_aligner.h)."""
+    _fields_ = [("lds_floats", C.c_int32), ("blocks_per_cu", C.c_int32), ("use_lds", C.c_int32),
+                ("check_every", C.c_int32)]
+
+
 class BatchTiming(C.Structure):
     _fields_ = [("wall_ms", C.c_double), ("pass_ms", C.c_double), ("pass_launches", C.c_int64),
                 ("pass_bytes", C.c_double), ("tr_rounds", C.c_int64)]
@@ -59,7 +65,8 @@ EXPORTS = [
     "fa_reference_release", "fa_reference_info", "fa_get_hessian", "fa_get_node_map",
     "fa_get_gradient", "fa_register_frame", "fa_deregister_frame", "fa_get_error",
     "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_point", "fa_align", "fa_batch_create",
-    "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_run", "fa_batch_eval", "fa_align_batch",
+    "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_get_tuning", "fa_batch_set_tuning",
+    "fa_batch_run", "fa_batch_eval", "fa_align_batch",
     "fa_context_stream",
 ]
 
@@ -98,6 +105,8 @@ def load_library(path=None):
     L.fa_batch_create.argtypes = [P, i32, C.POINTER(TrParams), C.POINTER(P)]
     L.fa_batch_destroy.argtypes = [P]
     L.fa_batch_set_timing.argtypes = [P, i32]
+    L.fa_batch_get_tuning.argtypes = [P, C.POINTER(Tuning)]
+    L.fa_batch_set_tuning.argtypes = [P, C.POINTER(Tuning)]
     L.fa_batch_run.argtypes = [P, P, P, C.POINTER(BatchTiming)]
     L.fa_batch_eval.argtypes = [P, P, P, P]
     L.fa_align_batch.argtypes = [P, i32, C.POINTER(TrParams), P, P]
@@ -341,6 +350,18 @@ class Batch:
 
     def set_timing(self, on=True):
         _check(self.L.fa_batch_set_timing(self.h, 1 if on else 0))
+
+    @property
+    def tuning(self):
+        t = Tuning()
+        _check(self.L.fa_batch_get_tuning(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = Tuning(**self.tuning)
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        _check(self.L.fa_batch_set_tuning(self.h, C.byref(t)))
 
     def run(self, x0=None):
         x = np.zeros(self.offsets[-1]) if x0 is None else np.concatenate(

@@ -98,6 +98,18 @@ typedef struct fa_batch_timing {
    int64_t tr_rounds;    /* launch rounds (pass + update) issued */
 } fa_batch_timing;
 
+/* Launch shape of the fused pass, chosen per batch by the autotuner (fa_amd.tune):
+   lds_floats - floats of LDS per workgroup for staged frame rows (0 disables staging;
+                at most 15360); work items whose reachable rows need more use global loads;
+   blocks_per_cu - persistent workgroups per compute unit;
+   use_lds - 0 forces global loads; check_every - trust-region rounds between host polls. */
+typedef struct fa_tuning {
+   int32_t lds_floats;
+   int32_t blocks_per_cu;
+   int32_t use_lds;
+   int32_t check_every;
+} fa_tuning;
+
 const char* fa_last_error(void);
 void fa_tr_default(fa_tr_params* p);
 
@@ -156,6 +168,8 @@ int fa_batch_create(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
                     fa_batch** out);
 int fa_batch_destroy(fa_batch* b);
 int fa_batch_set_timing(fa_batch* b, int32_t enable);
+int fa_batch_get_tuning(const fa_batch* b, fa_tuning* t);
+int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t);
 /* x_inout: concatenated per-frame parameter vectors; stats: n entries (may be NULL). */
 int fa_batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timing* timing);
 /* One fused pass over every frame of the batch at the given D (concatenated nD x 3 per

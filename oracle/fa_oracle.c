@@ -583,11 +583,12 @@ static int chol_small(int m, double* A)
       for (int k = 0; k < j; k++) s -= A[j * m + k] * A[j * m + k];
       if (!(s > 0) || !isfinite(s)) return 0;
       double l = sqrt(s);
+      double il = 1.0 / l;
       A[j * m + j] = l;
       for (int i = j + 1; i < m; i++) {
          double t = A[i * m + j];
          for (int k = 0; k < j; k++) t -= A[i * m + k] * A[j * m + k];
-         A[i * m + j] = t / l;
+         A[i * m + j] = t * il;
       }
       for (int i = 0; i < j; i++) A[i * m + j] = 0;
    }
@@ -621,7 +622,7 @@ int orc_bt_cholesky(int nb, int m, const double* diag, const double* sub, double
             for (int j = 0; j < m; j++) {
                double t = B[i * m + j];
                for (int q = 0; q < j; q++) t -= X[i * m + q] * A[j * m + q];
-               X[i * m + j] = t / A[j * m + j];
+               X[i * m + j] = t * (1.0 / A[j * m + j]);
             }
       }
    }
@@ -642,7 +643,7 @@ static void bt_forward(int nb, int m, const double* Ld, const double* Ls, const 
       for (int i = 0; i < m; i++) {
          double t = r[i];
          for (int q = 0; q < i; q++) t -= L[i * m + q] * y[k * m + q];
-         y[k * m + i] = t / L[i * m + i];
+         y[k * m + i] = t * (1.0 / L[i * m + i]);
       }
    }
 }
@@ -661,7 +662,7 @@ static void bt_backward(int nb, int m, const double* Ld, const double* Ls, const
       for (int i = m - 1; i >= 0; i--) {
          double t = r[i];
          for (int q = i + 1; q < m; q++) t -= L[q * m + i] * x[k * m + q];
-         x[k * m + i] = t / L[i * m + i];
+         x[k * m + i] = t * (1.0 / L[i * m + i]);
       }
    }
 }
