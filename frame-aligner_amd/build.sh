@@ -14,3 +14,7 @@ OUT="$HERE/lib/libfa_hip.so"
   -o "$OUT.tmp"
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"
+# C++ host mirror check program (plain g++ against the C ABI)
+g++ -O2 -std=c++17 -I"$ROOT/include" -I"$HERE/host" "$HERE/host/fa_host_check.cpp" \
+  -L"$HERE/lib" -lfa_hip -Wl,-rpath,'$ORIGIN' -o "$HERE/lib/fa_host_check"
+echo "built $HERE/lib/fa_host_check"
