@@ -31,6 +31,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_probe(mib):
+    """One plain device-to-device copy of mib MiB (hipMemcpy), whose rocprof FETCH_SIZE and
+    WRITE_SIZE give the scale of those counters for a known byte count."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return mib
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -43,6 +60,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB copied device-to-device once (calibrates rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames are "
@@ -132,6 +151,9 @@ def main():
                      "launches": launches, "avg_launch_us": round(1e3 * pass_ms / max(launches, 1), 2),
                      "bytes_per_launch": pass_bytes / max(launches, 1)},
     }
+
+    if args.copy_probe:
+        out["copy_probe_mib"] = copy_probe(args.copy_probe)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
