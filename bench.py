@@ -25,6 +25,19 @@ import numpy as np  # noqa: E402
 
 METRIC = "aligned frames/s @512²,30 nodes (≤0.02px err); p50/p99 latency; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pass_traffic_512x512_200f_nD31.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_pass launch from the committed rocprof FETCH_SIZE/WRITE_SIZE summary
+    (counters scaled by a plain copy of known size), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -146,7 +159,8 @@ def main():
                   "same_result_as_batched": plain_same},
         "batched_vs_plain": round(fps / plain_fps, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(),
                      "kernel": "k_pass (fused warp + SSD + Jtr)",
                      "launches": launches, "avg_launch_us": round(1e3 * pass_ms / max(launches, 1), 2),
                      "bytes_per_launch": pass_bytes / max(launches, 1)},
