@@ -668,6 +668,11 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    for (int pb = ch.p0 + threadIdx.x; pb < ch.p1; pb += kUnroll * kBlock) {
       float4 recs[kUnroll];
       float fs[kUnroll];
+      // plain FP32 sums over the kUnroll pixels of this step, folded into the compensated
+      // accumulators once per step
+      float part[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) part[v] = 0.f;
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
          const int pu = min(pb + u * kBlock, ch.p1 - 1);
@@ -734,17 +739,20 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
       }
       if (v != 0.f) {  // no sample: residual 0, outside the SSD
          const float e = v - rec.x;
-         kadd(acc[0], e * e);
+         part[0] += e * e;
          const float ej = (p != ch.end_p) ? e : 0.f;  // the interval's last pixel has no row
          const float omff = (float)omf;
          const float g[3] = {rec.y, rec.z, rec.w};
 #pragma unroll
          for (int c = 0; c < M; c++) {
-            kadd(acc[1 + c], (g[c] * omff) * ej);
-            kadd(acc[1 + M + c], (g[c] * f) * ej);
+            part[1 + c] += (g[c] * omff) * ej;
+            part[1 + M + c] += (g[c] * f) * ej;
          }
       }
       }
+      if (MODE == MODE_PASS)
+#pragma unroll
+         for (int v = 0; v < NV; v++) kadd(acc[v], part[v]);
    }
    if (MODE == MODE_PASS) block_reduce_fast<NV>(acc, (KF(*)[NV])sred, out);
    else __syncthreads();

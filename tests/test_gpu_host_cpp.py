@@ -15,10 +15,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "frame-aligner_amd", "lib", "fa_host_check")
 
 
-@pytest.mark.parametrize("shape,nD", [((1, 80, 72), 9), ((4, 32, 28), 4 * 3 + 1)])
-def test_cpp_host_mirror(tmp_path, shape, nD):
+# well-posed cases: the converged displacement is stable under start perturbations
+@pytest.mark.parametrize("shape,nD,gen", [((1, 80, 72), 9, {}),
+                                          ((4, 32, 28), 4 * 3 + 1, dict(n_blobs=120, sigma=(1.5, 4.0)))])
+def test_cpp_host_mirror(tmp_path, shape, nD, gen):
     assert os.path.exists(EXE), "build frame-aligner_amd/build.sh first"
-    st = small_stack(shape, nD=nD, n_frames=1, seed=31)
+    st = small_stack(shape, nD=nD, n_frames=1, seed=31, **gen)
     ref, fr = st["ref_w"], st["frames_w"][0]
     D = st["D_true"][0].copy()
     if shape[0] == 1:

@@ -10,7 +10,7 @@ import pytest
 
 import oracle as O
 import fa_amd as F
-from helpers import rel_err, small_stack
+from helpers import PARITY_CASES, rel_err, small_stack
 
 pytestmark = pytest.mark.gpu
 
@@ -26,14 +26,12 @@ def ctx():
     c.close()
 
 
-CASES = [
-    ("2d", (1, 96, 80), 9, dict(line=1.0, interline=1.0)),
-    ("2d_interline", (1, 64, 72), 13, dict(line=1.0, interline=1.3)),
-    ("3d", (6, 40, 36), 6 * 3 + 1, dict(line=1.0, interline=1.0)),
-]
+CASES = [c[:4] for c in PARITY_CASES]
+GEN = {c[0]: c[4] for c in PARITY_CASES}
 
 
-def _pair(ctx, shape, nD, scan, seed=3, **kw):
+def _pair(ctx, shape, nD, scan, seed=3, name=None, **kw):
+    kw = {**GEN.get(name, {}), **kw}
     st = small_stack(shape, nD=nD, seed=seed, **kw)
     orc = O.Oracle().set_reference(st["ref_w"], nD, **scan)
     ref = F.Reference(ctx, st["ref_w"], nD, **scan)
@@ -42,7 +40,7 @@ def _pair(ctx, shape, nD, scan, seed=3, **kw):
 
 @pytest.mark.parametrize("name,shape,nD,scan", CASES)
 def test_node_map_bit_exact(ctx, name, shape, nD, scan):
-    st, orc, ref = _pair(ctx, shape, nD, scan)
+    st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
     Di_o, Df_o, Dr_o = orc.node_map()
     Di_g, Df_g, Dr_g = ref.node_map()
     assert np.array_equal(Di_o, Di_g)
@@ -52,7 +50,7 @@ def test_node_map_bit_exact(ctx, name, shape, nD, scan):
 
 @pytest.mark.parametrize("name,shape,nD,scan", CASES)
 def test_sobel_bit_exact(ctx, name, shape, nD, scan):
-    st, orc, ref = _pair(ctx, shape, nD, scan)
+    st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
     go = orc.gradient()
     gg = ref.gradient()
     for c in range(ref.n_dim):
@@ -61,7 +59,7 @@ def test_sobel_bit_exact(ctx, name, shape, nD, scan):
 
 @pytest.mark.parametrize("name,shape,nD,scan", CASES)
 def test_hessian(ctx, name, shape, nD, scan):
-    st, orc, ref = _pair(ctx, shape, nD, scan)
+    st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
     Ho, Hg = orc.hessian(), ref.H
     assert rel_err(Hg, Ho) < JAC_TOL
     # block-tridiagonal structure: zero beyond the first off-diagonal block
@@ -82,7 +80,7 @@ def _displacements(st, nD, rng):
 @pytest.mark.parametrize("name,shape,nD,scan", CASES)
 @pytest.mark.parametrize("dtype", ["float32", "uint16"])
 def test_error_and_jacobian(ctx, name, shape, nD, scan, dtype):
-    st, orc, ref = _pair(ctx, shape, nD, scan, dtype=dtype)
+    st, orc, ref = _pair(ctx, shape, nD, scan, name=name, dtype=dtype)
     rng = np.random.default_rng(11)
     for f_i in range(len(st["frames"])):
         fr = ref.register(st["frames"][f_i], offset=1.0)  # the aligner's +1 on the device
@@ -101,7 +99,7 @@ def test_error_and_jacobian(ctx, name, shape, nD, scan, dtype):
 
 @pytest.mark.parametrize("name,shape,nD,scan", CASES)
 def test_warp_interpolated(ctx, name, shape, nD, scan):
-    st, orc, ref = _pair(ctx, shape, nD, scan)
+    st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
     fr = ref.register(st["frames_w"][0])
     rng = np.random.default_rng(5)
     for D in _displacements(st, nD, rng):
@@ -127,7 +125,7 @@ def test_zero_and_integer_shift_exact(ctx):
 
 @pytest.mark.parametrize("name,shape,nD,scan", CASES)
 def test_align_matches_oracle(ctx, name, shape, nD, scan):
-    st, orc, ref = _pair(ctx, shape, nD, scan)
+    st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
     kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
     for f_i in range(len(st["frames"])):
         fw = st["frames_w"][f_i]
