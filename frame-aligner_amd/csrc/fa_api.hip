@@ -59,10 +59,9 @@ void dfree(T*& p)
    p = nullptr;
 }
 
-constexpr int kDefaultChunk = 4096;
 constexpr int kTrLdsMaxDoubles = 7680;  // 60 KB for the trust-region update
 constexpr int kLdsCapFloats = 15360;  // 60 KB of staged frame rows per workgroup at most
-constexpr int kBlocksPerCu = 4;
+constexpr int kBlocksPerCu = 5;
 
 // ImageScanParameters ctor (ImageScanParameters.h:7-20) then processScanParameters
 // (FrameWarper.cpp:223-236).
@@ -106,6 +105,14 @@ struct DeviceGuard {
 
 struct fa_batch {
    fa_context* ctx = nullptr;
+   hipStream_t stream = nullptr;   // the batch's own stream (default priority)
+   // A batch of many frames runs as parts on their own streams, whose trust-region rounds
+   // interleave so that one part's update overlaps another part's pass. Leaf batches have no
+   // parts. part_first[i] is the caller's index of part i's first frame.
+   std::vector<fa_batch*> parts;
+   std::vector<int> part_first;
+   bool finished = false;
+   size_t timed = 0;
    std::vector<fa_frame*> frames;
    fa_tr_params prm{};
    std::vector<BatchFrame> hframes;
@@ -177,6 +184,8 @@ struct fa_batch {
 
    ~fa_batch()
    {
+      for (fa_batch* q : parts) delete q;
+      if (stream) (void)hipStreamSynchronize(stream);
       for (auto e : ev) (void)hipEventDestroy(e);
       dfree(d_refs);
       dfree(d_frames);
@@ -191,6 +200,7 @@ struct fa_batch {
       dfree(d_counters);
       dfree(d_warp);
       if (h_counters) (void)hipHostFree(h_counters);
+      if (stream) (void)hipStreamDestroy(stream);
    }
 };
 
@@ -204,7 +214,7 @@ int default_lds_floats(const std::vector<fa_frame*>& frames)
    for (fa_frame* f : frames) {
       const fa_reference* r = f->ref;
       const int64_t stride = ((r->nx + 2 * kPad) + 3) & ~3;
-      const int64_t rows = (r->chunk_px + r->nx - 1) / r->nx + 8;
+      const int64_t rows = (r->chunk_px + r->nx - 1) / r->nx + 6;
       best = std::max(best, rows * stride * (r->n_dim == 3 ? 3 : 1));
    }
    return (int)std::min<int64_t>(best, kLdsCapFloats);
@@ -312,7 +322,12 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       b->h_counters = nullptr;
       return fail(set_err(FA_E_OOM, "Not enough memory to allocate"));
    }
-   hipStream_t s = ctx->stream;
+   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      b->stream = nullptr;
+      return fail(set_err(FA_E_DEVICE, "stream creation failed"));
+   }
+   hipStream_t s = b->stream;
    if (hipMemcpyAsync(b->d_refs, refs.data(), refs.size() * sizeof(RefDev), hipMemcpyHostToDevice, s) !=
            hipSuccess ||
        hipMemcpyAsync(b->d_frames, b->hframes.data(), n * sizeof(BatchFrame), hipMemcpyHostToDevice, s) !=
@@ -350,49 +365,59 @@ int event_pair(fa_batch* b, size_t i, hipEvent_t* s, hipEvent_t* e)
 }
 
 constexpr size_t kMaxTimedRounds = 4096;
+constexpr int kSplitFrames = 64;  // batches of at least this many frames run as two parts
 
-int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timing* timing)
+int64_t max_rounds(const fa_batch* b) { return b->prm.max_evals > 0 ? (int64_t)b->prm.max_evals + 2 : 100000; }
+
+// Phases of a run on a leaf batch, all enqueued on its stream.
+int run_begin(fa_batch* b, const double* x)
 {
-   auto t0 = std::chrono::steady_clock::now();
-   fa_context* ctx = b->ctx;
-   hipStream_t s = ctx->stream;
-   const int n = (int)b->frames.size();
-   BatchDev B = b->dev();
-   HIPCHK(hipMemcpyAsync(b->d_x, x_inout, b->total_x * sizeof(double), hipMemcpyHostToDevice, s));
+   hipStream_t s = b->stream;
+   HIPCHK(hipMemcpyAsync(b->d_x, x, b->total_x * sizeof(double), hipMemcpyHostToDevice, s));
    HIPCHK(hipMemsetAsync(b->d_counters, 0, sizeof(Counters), s));
-   HIPCHK(launch_tr_init(B, 0, s));
-   const int64_t max_rounds = b->prm.max_evals > 0 ? (int64_t)b->prm.max_evals + 2 : 100000;
-   const int K = std::max(1, b->tune.check_every);
-   size_t timed = 0;
-   int64_t round = 0;
-   bool finished = false;
-   while (!finished && round < max_rounds) {
-      for (int r = 0; r < K && round < max_rounds; r++, round++) {
-         HIPCHK(launch_schedule(B, 1, s));
-         hipEvent_t es = nullptr, ee = nullptr;
-         if (b->timing && timed < kMaxTimedRounds) {
-            CHK(event_pair(b, timed, &es, &ee));
-            HIPCHK(hipEventRecord(es, s));
-         }
-         CHK(launch_passes(b, s));
-         if (es) {
-            HIPCHK(hipEventRecord(ee, s));
-            timed++;
-         }
-         HIPCHK(launch_tr_update(B, s));
-      }
-      HIPCHK(launch_schedule(B, 0, s));
-      HIPCHK(hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(Counters), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      finished = b->h_counters->n_active_frames == 0;
+   HIPCHK(launch_tr_init(b->dev(), 0, s));
+   b->finished = false;
+   b->timed = 0;
+   return FA_OK;
+}
+
+int run_round(fa_batch* b)
+{
+   hipStream_t s = b->stream;
+   const BatchDev B = b->dev();
+   HIPCHK(launch_schedule(B, 1, s));
+   hipEvent_t es = nullptr, ee = nullptr;
+   if (b->timing && b->timed < kMaxTimedRounds) {
+      CHK(event_pair(b, b->timed, &es, &ee));
+      HIPCHK(hipEventRecord(es, s));
    }
+   CHK(launch_passes(b, s));
+   if (es) {
+      HIPCHK(hipEventRecord(ee, s));
+      b->timed++;
+   }
+   HIPCHK(launch_tr_update(B, s));
+   return FA_OK;
+}
+
+// Refreshes the counters the host polls and copies them back (asynchronously).
+int run_poll(fa_batch* b)
+{
+   HIPCHK(launch_schedule(b->dev(), 0, b->stream));
+   HIPCHK(hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(Counters), hipMemcpyDeviceToHost, b->stream));
+   return FA_OK;
+}
+
+int run_finish(fa_batch* b, double* x, fa_align_stats* stats, int first_user)
+{
+   const int n = (int)b->frames.size();
    std::vector<TrState> st(n);
-   HIPCHK(hipMemcpyAsync(x_inout, b->d_x, b->total_x * sizeof(double), hipMemcpyDeviceToHost, s));
-   HIPCHK(hipMemcpyAsync(st.data(), b->d_state, n * sizeof(TrState), hipMemcpyDeviceToHost, s));
-   HIPCHK(hipStreamSynchronize(s));
+   HIPCHK(hipMemcpyAsync(x, b->d_x, b->total_x * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+   HIPCHK(hipMemcpyAsync(st.data(), b->d_state, n * sizeof(TrState), hipMemcpyDeviceToHost, b->stream));
+   HIPCHK(hipStreamSynchronize(b->stream));
    if (stats)
       for (int j = 0; j < n; j++) {
-         fa_align_stats& o = stats[b->user[j]];
+         fa_align_stats& o = stats[first_user + b->user[j]];
          o.iterations = st[j].iterations;
          o.evaluations = st[j].evaluations;
          o.rejected = st[j].rejected;
@@ -400,27 +425,84 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
          o.f_final = st[j].f;
          o.radius_final = st[j].radius;
       }
-   if (timing) {
-      double pass_ms = 0;
-      for (size_t i = 0; i < timed; i++) {
-         float ms = 0;
-         HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
-         pass_ms += ms;
+   return FA_OK;
+}
+
+int add_timing(fa_batch* b, fa_batch_timing* t)
+{
+   double pass_ms = 0;
+   for (size_t i = 0; i < b->timed; i++) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
+      pass_ms += ms;
+   }
+   t->pass_ms += pass_ms;
+   t->pass_launches += (int64_t)b->timed;
+   t->pass_bytes += (double)b->h_counters->bytes;
+   t->tr_rounds = std::max<int64_t>(t->tr_rounds, (int64_t)b->h_counters->rounds);
+   return FA_OK;
+}
+
+// Runs every leaf (a leaf batch alone, or the parts of a split batch) to completion,
+// interleaving their rounds: rounds are enqueued part by part on their own streams and the
+// host polls every check_every rounds.
+int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timing* timing)
+{
+   auto t0 = std::chrono::steady_clock::now();
+   std::vector<fa_batch*> leaves;
+   std::vector<int64_t> xoff;
+   std::vector<int> first;
+   if (b->parts.empty()) {
+      leaves.push_back(b);
+      xoff.push_back(0);
+      first.push_back(0);
+   } else {
+      int64_t o = 0;
+      for (size_t i = 0; i < b->parts.size(); i++) {
+         leaves.push_back(b->parts[i]);
+         xoff.push_back(o);
+         first.push_back(b->part_first[i]);
+         o += b->parts[i]->total_x;
       }
-      timing->pass_ms = pass_ms;
-      timing->pass_launches = (int64_t)timed;
-      timing->pass_bytes = (double)b->h_counters->bytes;
-      timing->tr_rounds = (int64_t)b->h_counters->rounds;
+   }
+   for (size_t i = 0; i < leaves.size(); i++) CHK(run_begin(leaves[i], x_inout + xoff[i]));
+   const int K = std::max(1, b->tune.check_every);
+   int64_t round = 0;
+   const int64_t limit = max_rounds(b);
+   for (;;) {
+      bool any = false;
+      for (int r = 0; r < K && round < limit; r++, round++)
+         for (fa_batch* q : leaves)
+            if (!q->finished) {
+               CHK(run_round(q));
+               any = true;
+            }
+      if (!any) break;
+      for (fa_batch* q : leaves)
+         if (!q->finished) CHK(run_poll(q));
+      bool all = true;
+      for (fa_batch* q : leaves)
+         if (!q->finished) {
+            HIPCHK(hipStreamSynchronize(q->stream));
+            q->finished = q->h_counters->n_active_frames == 0;
+            all = all && q->finished;
+         }
+      if (all || round >= limit) break;
+   }
+   for (size_t i = 0; i < leaves.size(); i++) CHK(run_finish(leaves[i], x_inout + xoff[i], stats, first[i]));
+   if (timing) {
+      *timing = fa_batch_timing{};
+      for (fa_batch* q : leaves) CHK(add_timing(q, timing));
       timing->wall_ms =
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
    }
    return FA_OK;
 }
 
-// One fused pass over every frame at D (concatenated nD x 3 per frame).
-int batch_eval(fa_batch* b, const double* D, double* err, double* jac)
+// One fused pass over every frame of a leaf batch at D (concatenated nD x 3 per frame).
+int leaf_eval(fa_batch* b, const double* D, double* err, double* jac)
 {
-   hipStream_t s = b->ctx->stream;
+   hipStream_t s = b->stream;
    const int n = (int)b->frames.size();
    BatchDev B = b->dev();
    HIPCHK(hipMemcpyAsync(b->d_D, D, b->total_D * sizeof(double), hipMemcpyHostToDevice, s));
@@ -444,6 +526,51 @@ int batch_eval(fa_batch* b, const double* D, double* err, double* jac)
          std::memcpy(jac + b->x_off[i], vec.data() + bf.vec_base + bf.n, bf.n * sizeof(double));
       }
    }
+   return FA_OK;
+}
+
+int batch_eval(fa_batch* b, const double* D, double* err, double* jac)
+{
+   if (b->parts.empty()) return leaf_eval(b, D, err, jac);
+   int64_t xo = 0, dof = 0;
+   for (size_t i = 0; i < b->parts.size(); i++) {
+      fa_batch* q = b->parts[i];
+      CHK(leaf_eval(q, D + dof, err ? err + b->part_first[i] : nullptr, jac ? jac + xo : nullptr));
+      xo += q->total_x;
+      dof += q->total_D;
+   }
+   return FA_OK;
+}
+
+// A batch for the caller: a leaf, or two leaves over the two halves of the frames (balanced
+// by pixel count) when it is large enough to overlap their rounds.
+int batch_create_split(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out)
+{
+   if (n < kSplitFrames) return batch_build(frames, n, prm, out);
+   int64_t total = 0;
+   for (int i = 0; i < n; i++) total += frames[i]->ref->n_px;
+   int cut = 0;
+   for (int64_t acc = 0; cut < n - 1 && 2 * (acc + frames[cut]->ref->n_px) <= total; cut++)
+      acc += frames[cut]->ref->n_px;
+   cut = std::max(1, cut);
+   fa_batch* a = nullptr;
+   fa_batch* c = nullptr;
+   CHK(batch_build(frames, cut, prm, &a));
+   int rc = batch_build(frames + cut, n - cut, prm, &c);
+   if (rc != FA_OK) {
+      delete a;
+      return rc;
+   }
+   auto* b = new fa_batch();
+   b->ctx = a->ctx;
+   b->frames.assign(frames, frames + n);
+   b->prm = a->prm;
+   b->tune = a->tune;
+   b->parts = {a, c};
+   b->part_first = {0, cut};
+   b->total_x = a->total_x + c->total_x;
+   b->total_D = a->total_D + c->total_D;
+   *out = b;
    return FA_OK;
 }
 
@@ -535,7 +662,8 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    r->nD = nD;
    r->n_dim = nz > 1 ? 3 : 2;
    r->n_px = n_px;
-   r->chunk_px = chunk_px > 0 ? chunk_px : kDefaultChunk;
+   // default work item: about 8 frame rows, 2048..8192 pixels
+   r->chunk_px = chunk_px > 0 ? chunk_px : (int)std::min<int64_t>(8192, std::max<int64_t>(2048, 8 * (int64_t)nx));
    derive_scan(*scan, nz, ny, nx, nD, r->scan_host, &r->scan);
    const int m = r->n_dim;
 
@@ -837,7 +965,7 @@ int fa_warp_interpolated(fa_frame* f, const double* D, float* out, float invalid
    fa_batch* b;
    CHK(single_batch(f, &b));
    fa_reference* r = f->ref;
-   hipStream_t s = r->ctx->stream;
+   hipStream_t s = b->stream;
    if (!b->d_warp) {
       CHK(dalloc(&b->d_warp, r->n_px));
       b->warp_px = r->n_px;
@@ -894,7 +1022,7 @@ int fa_batch_create(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
    fa_context* ctx = frames[0]->ref->ctx;
    std::lock_guard<std::mutex> lk(ctx->mu);
    DeviceGuard g(ctx->device);
-   return batch_build(frames, n, prm, out);
+   return batch_create_split(frames, n, prm, out);
 }
 
 int fa_batch_destroy(fa_batch* b)
@@ -902,7 +1030,6 @@ int fa_batch_destroy(fa_batch* b)
    if (!b) return FA_OK;
    std::lock_guard<std::mutex> lk(b->ctx->mu);
    DeviceGuard g(b->ctx->device);
-   (void)hipStreamSynchronize(b->ctx->stream);
    delete b;
    return FA_OK;
 }
@@ -911,6 +1038,7 @@ int fa_batch_set_timing(fa_batch* b, int32_t enable)
 {
    if (!b) return set_err(FA_E_ARG, "null batch");
    b->timing = enable ? 1 : 0;
+   for (fa_batch* q : b->parts) q->timing = b->timing;
    return FA_OK;
 }
 
@@ -929,7 +1057,11 @@ int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t)
       return set_err(FA_E_ARG, "tuning out of range (lds_floats <= 15360, 1 <= blocks_per_cu <= 64)");
    std::lock_guard<std::mutex> lk(b->ctx->mu);
    b->tune = *t;
-   set_grids(b);
+   if (b->parts.empty()) set_grids(b);
+   for (fa_batch* q : b->parts) {
+      q->tune = *t;
+      set_grids(q);
+   }
    return FA_OK;
 }
 

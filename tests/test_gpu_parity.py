@@ -180,3 +180,26 @@ def test_errors_are_reported(ctx):
     D = np.full((9, 3), np.nan)
     e, j = fr.jacobian(D)
     assert e == 0.0 and not j.any()
+
+
+def test_split_batch_same_as_alone(ctx):
+    """A batch of >= 64 frames runs as two parts on two streams with interleaved rounds; every
+    frame's result is bit-identical to aligning it alone."""
+    a, _, ref_a = _pair(ctx, (1, 48, 40), 7, dict(line=1.0, interline=1.0), seed=41, n_frames=36)
+    b, _, ref_b = _pair(ctx, (1, 40, 56), 9, dict(line=1.0, interline=1.0), seed=42, n_frames=36,
+                        dtype="uint16")
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    frames = [ref_a.register(f) for f in a["frames_w"]] + [ref_b.register(f) for f in b["frames"]]
+    batch = F.Batch(frames, **kw)
+    batch.set_timing(True)
+    xs, stats, timing = batch.run()
+    assert timing["pass_launches"] > 0 and timing["pass_bytes"] > 0
+    for i in (0, 17, 35, 36, 50, 71):
+        x1, s1 = frames[i].align(**kw)
+        assert np.array_equal(x1, xs[i]), i
+        assert s1["evaluations"] == stats[i]["evaluations"]
+    D = [np.zeros((f.ref.nD, 3)) for f in frames]
+    err, jac = batch.eval(D)
+    for i in (3, 40):
+        e, j = frames[i].jacobian(D[i])
+        assert e == err[i] and np.array_equal(j, jac[i])
