@@ -113,6 +113,13 @@ struct fa_batch {
    std::vector<int> part_first;
    bool finished = false;
    size_t timed = 0;
+   // hipGraph of check_every trust-region rounds (schedule, pass, update), replayed per poll;
+   // gev holds its 2 x check_every timing events when it was captured with timing on
+   hipGraphExec_t gexec = nullptr;
+   int graph_rounds = 0, graph_timing = 0;
+   double graph_pass_ms = 0;
+   fa_tr_params graph_prm{};
+   std::vector<hipEvent_t> gev;
    std::vector<fa_frame*> frames;
    fa_tr_params prm{};
    std::vector<BatchFrame> hframes;
@@ -137,7 +144,8 @@ struct fa_batch {
    Counters* d_counters = nullptr;
    Counters* h_counters = nullptr;  // pinned
    std::vector<hipEvent_t> ev;      // start/end pairs
-   float* d_warp = nullptr;         // output buffer of warps (one-frame batches)
+   float* d_warp = nullptr;         // output buffers of warps (one-frame batches)
+   float* d_cov = nullptr;
    int64_t warp_px = 0;
 
    BatchDev dev() const
@@ -182,10 +190,19 @@ struct fa_batch {
       return a;
    }
 
+   void drop_graph()
+   {
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+      for (auto e : gev) (void)hipEventDestroy(e);
+      gev.clear();
+   }
+
    ~fa_batch()
    {
       for (fa_batch* q : parts) delete q;
       if (stream) (void)hipStreamSynchronize(stream);
+      drop_graph();
       for (auto e : ev) (void)hipEventDestroy(e);
       dfree(d_refs);
       dfree(d_frames);
@@ -199,6 +216,7 @@ struct fa_batch {
       dfree(d_active_base);
       dfree(d_counters);
       dfree(d_warp);
+      dfree(d_cov);
       if (h_counters) (void)hipHostFree(h_counters);
       if (stream) (void)hipStreamDestroy(stream);
    }
@@ -226,6 +244,7 @@ void default_tuning(fa_tuning* t, const std::vector<fa_frame*>& frames)
    t->blocks_per_cu = kBlocksPerCu;
    t->use_lds = 1;
    t->check_every = 4;
+   t->use_graph = 1;
 }
 
 void set_grids(fa_batch* b)
@@ -378,6 +397,7 @@ int run_begin(fa_batch* b, const double* x)
    HIPCHK(launch_tr_init(b->dev(), 0, s));
    b->finished = false;
    b->timed = 0;
+   b->graph_pass_ms = 0;
    return FA_OK;
 }
 
@@ -430,16 +450,75 @@ int run_finish(fa_batch* b, double* x, fa_align_stats* stats, int first_user)
 
 int add_timing(fa_batch* b, fa_batch_timing* t)
 {
-   double pass_ms = 0;
-   for (size_t i = 0; i < b->timed; i++) {
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
-      pass_ms += ms;
-   }
+   double pass_ms = b->graph_pass_ms;
+   if (!b->gexec || !b->tune.use_graph)
+      for (size_t i = 0; i < b->timed; i++) {
+         float ms = 0;
+         HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
+         pass_ms += ms;
+      }
    t->pass_ms += pass_ms;
    t->pass_launches += (int64_t)b->timed;
    t->pass_bytes += (double)b->h_counters->bytes;
    t->tr_rounds = std::max<int64_t>(t->tr_rounds, (int64_t)b->h_counters->rounds);
+   return FA_OK;
+}
+
+// Captures check_every rounds into a graph (once per tuning / parameter set).
+int ensure_graph(fa_batch* b, int K)
+{
+   if (b->gexec && b->graph_rounds == K && b->graph_timing == b->timing &&
+       std::memcmp(&b->graph_prm, &b->prm, sizeof(fa_tr_params)) == 0)
+      return FA_OK;
+   b->drop_graph();
+   if (b->timing)
+      for (int i = 0; i < 2 * K; i++) {
+         hipEvent_t e;
+         HIPCHK(hipEventCreate(&e));
+         b->gev.push_back(e);
+      }
+   hipStream_t s = b->stream;
+   const BatchDev B = b->dev();
+   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+   int rc = FA_OK;
+   for (int r = 0; r < K && rc == FA_OK; r++) {
+      if (launch_schedule(B, 1, s) != hipSuccess) rc = set_err(FA_E_DEVICE, "graph capture failed");
+      if (rc == FA_OK && b->timing && hipEventRecord(b->gev[2 * r], s) != hipSuccess)
+         rc = set_err(FA_E_DEVICE, "graph capture failed");
+      if (rc == FA_OK) rc = launch_passes(b, s);
+      if (rc == FA_OK && b->timing && hipEventRecord(b->gev[2 * r + 1], s) != hipSuccess)
+         rc = set_err(FA_E_DEVICE, "graph capture failed");
+      if (rc == FA_OK && launch_tr_update(B, s) != hipSuccess) rc = set_err(FA_E_DEVICE, "graph capture failed");
+   }
+   hipGraph_t graph = nullptr;
+   hipError_t e = hipStreamEndCapture(s, &graph);
+   if (rc != FA_OK || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      (void)hipGetLastError();
+      return rc != FA_OK ? rc : set_err(FA_E_DEVICE, "graph capture failed");
+   }
+   e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+   (void)hipGraphDestroy(graph);
+   if (e != hipSuccess) {
+      b->gexec = nullptr;
+      (void)hipGetLastError();
+      return set_err(FA_E_DEVICE, "graph instantiation failed");
+   }
+   b->graph_rounds = K;
+   b->graph_timing = b->timing;
+   b->graph_prm = b->prm;
+   return FA_OK;
+}
+
+// Adds the pass durations of the last graph replay.
+int collect_graph_timing(fa_batch* b)
+{
+   for (int r = 0; r < b->graph_rounds; r++) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, b->gev[2 * r], b->gev[2 * r + 1]));
+      b->graph_pass_ms += ms;
+   }
+   b->timed += (size_t)b->graph_rounds;
    return FA_OK;
 }
 
@@ -465,18 +544,30 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
          o += b->parts[i]->total_x;
       }
    }
-   for (size_t i = 0; i < leaves.size(); i++) CHK(run_begin(leaves[i], x_inout + xoff[i]));
    const int K = std::max(1, b->tune.check_every);
+   const bool graphs = b->tune.use_graph != 0;
+   if (graphs)
+      for (fa_batch* q : leaves) CHK(ensure_graph(q, K));
+   for (size_t i = 0; i < leaves.size(); i++) CHK(run_begin(leaves[i], x_inout + xoff[i]));
    int64_t round = 0;
    const int64_t limit = max_rounds(b);
    for (;;) {
       bool any = false;
-      for (int r = 0; r < K && round < limit; r++, round++)
+      if (graphs) {
          for (fa_batch* q : leaves)
             if (!q->finished) {
-               CHK(run_round(q));
+               HIPCHK(hipGraphLaunch(q->gexec, q->stream));
                any = true;
             }
+         round += K;
+      } else {
+         for (int r = 0; r < K && round < limit; r++, round++)
+            for (fa_batch* q : leaves)
+               if (!q->finished) {
+                  CHK(run_round(q));
+                  any = true;
+               }
+      }
       if (!any) break;
       for (fa_batch* q : leaves)
          if (!q->finished) CHK(run_poll(q));
@@ -484,6 +575,7 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
       for (fa_batch* q : leaves)
          if (!q->finished) {
             HIPCHK(hipStreamSynchronize(q->stream));
+            if (graphs && q->timing) CHK(collect_graph_timing(q));
             q->finished = q->h_counters->n_active_frames == 0;
             all = all && q->finished;
          }
@@ -982,6 +1074,35 @@ int fa_warp_interpolated(fa_frame* f, const double* D, float* out, float invalid
    return FA_OK;
 }
 
+int fa_warp_image(fa_frame* f, const double* D, float* wimg, float* coverage, int32_t apply_offset)
+{
+   if (!f || !D || !wimg || !coverage) return set_err(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   DeviceGuard g(f->ref->ctx->device);
+   fa_batch* b;
+   CHK(single_batch(f, &b));
+   fa_reference* r = f->ref;
+   hipStream_t s = b->stream;
+   if (!b->d_warp) {
+      CHK(dalloc(&b->d_warp, r->n_px));
+      b->warp_px = r->n_px;
+   }
+   if (!b->d_cov) CHK(dalloc(&b->d_cov, r->n_px));
+   FrameDev F;
+   F.data = f->d_data;
+   F.dtype = f->dtype;
+   F.offset = f->offset;
+   int grid = (int)std::min<int64_t>((int64_t)r->chunks.size(), (int64_t)r->ctx->n_cu * kBlocksPerCu);
+   HIPCHK(hipMemcpyAsync(b->d_D, D, 3 * (size_t)r->nD * sizeof(double), hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemsetAsync(b->d_warp, 0, r->n_px * sizeof(float), s));
+   HIPCHK(hipMemsetAsync(b->d_cov, 0, r->n_px * sizeof(float), s));
+   HIPCHK(launch_splat(r->dev, F, b->d_D, b->d_warp, b->d_cov, apply_offset ? f->offset : 0.f, grid, s));
+   HIPCHK(hipMemcpyAsync(wimg, b->d_warp, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(coverage, b->d_cov, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   return FA_OK;
+}
+
 int fa_warp_point(const fa_reference* r, const double* D, int32_t x, int32_t y, int32_t z,
                   int32_t binning, double* out3)
 {
@@ -1053,14 +1174,18 @@ int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t)
 {
    if (!b || !t) return set_err(FA_E_ARG, "null argument");
    if (t->lds_floats < 0 || t->lds_floats > 15360 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
-       t->check_every < 1)
+       t->check_every < 1 || t->check_every > 64)
       return set_err(FA_E_ARG, "tuning out of range (lds_floats <= 15360, 1 <= blocks_per_cu <= 64)");
    std::lock_guard<std::mutex> lk(b->ctx->mu);
    b->tune = *t;
-   if (b->parts.empty()) set_grids(b);
+   if (b->parts.empty()) {
+      set_grids(b);
+      b->drop_graph();
+   }
    for (fa_batch* q : b->parts) {
       q->tune = *t;
       set_grids(q);
+      q->drop_graph();
    }
    return FA_OK;
 }

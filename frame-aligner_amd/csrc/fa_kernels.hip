@@ -836,6 +836,70 @@ __global__ __launch_bounds__(kBlock) void k_warp(RefDev R, FrameDev F, const dou
    }
 }
 
+// ---- forward splat (AbstractFrameWarper::warpImage, FrameWarper.cpp:71-136) -------------
+// Each source pixel p is pushed to p - d(p) with trilinear weights into wimg (weight x value)
+// and coverage (weight). loc1 stays at loc0 on the last row/column/plane, and only loc0 must
+// lie in the volume (z checked only with more than one plane). The clamp of loc at
+// FrameWarper.cpp:87-94 runs after loc0 and the fractions are taken and has no effect.
+// Weights are formed in FP64; the float accumulation uses atomics, so the summation order
+// differs from the reference's scan order.
+template <int M, typename FT>
+__device__ void splat_chunk(const RefDev& R, const FrameDev& F, const Chunk ch, const double* D,
+                            float* wimg, float* cov, float offset)
+{
+   const int nx = R.nx, ny = R.ny, nz = R.nz;
+   const int64_t plane_sz = (int64_t)nx * ny;
+   const int k = ch.k;
+   const int plane = (int)(ch.p0 / plane_sz);
+   const int q0 = (int)(ch.p0 - plane * plane_sz);
+   PixelWalk w;
+   w.init(q0 + threadIdx.x, nx);
+   for (int p = ch.p0 + threadIdx.x; p < ch.p1; p += kBlock, w.step(nx)) {
+      const double fd = (double)(M == 3 ? R.df[p] : R.rg4[p].w);
+      double l[3];
+      l[0] = (double)w.x - (fd * D[3 * (k + 1) + 0] + (1.0 - fd) * D[3 * k + 0]);
+      l[1] = (double)w.y - (fd * D[3 * (k + 1) + 1] + (1.0 - fd) * D[3 * k + 1]);
+      l[2] = M == 3 ? (double)plane - (fd * D[3 * (k + 1) + 2] + (1.0 - fd) * D[3 * k + 2]) : 0.0;
+      if (!(fabs(l[0]) < 1e9 && fabs(l[1]) < 1e9 && fabs(l[2]) < 1e9)) continue;
+      const double fl[3] = {floor(l[0]), floor(l[1]), floor(l[2])};
+      const int x0 = (int)fl[0], y0 = (int)fl[1], z0 = (int)fl[2];
+      const double fx = l[0] - fl[0], fy = l[1] - fl[1], fz = l[2] - fl[2];
+      if (x0 < 0 || y0 < 0 || x0 > nx - 1 || y0 > ny - 1) continue;
+      if (nz > 1 && (z0 < 0 || z0 > nz - 1)) continue;
+      const int x1 = x0 < nx - 1 ? x0 + 1 : x0, y1 = y0 < ny - 1 ? y0 + 1 : y0;
+      const int z1 = z0 < nz - 1 ? z0 + 1 : z0;
+      const double v = (double)((float)((const FT*)F.data)[p] + offset);
+      const int zz[2] = {z0, z1}, yy[2] = {y0, y1}, xx[2] = {x0, x1};
+      const double wz[2] = {1 - fz, fz}, wy[2] = {1 - fy, fy}, wx[2] = {1 - fx, fx};
+#pragma unroll
+      for (int a = 0; a < (M == 3 ? 2 : 1); a++)
+#pragma unroll
+         for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+               const double wt = wy[b] * wx[c] * wz[a];
+               const int64_t o = ((int64_t)zz[a] * ny + yy[b]) * nx + xx[c];
+               atomicAdd(&wimg[o], (float)(wt * v));
+               atomicAdd(&cov[o], (float)wt);
+            }
+   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_splat(RefDev R, FrameDev F, const double* D, float* wimg,
+                                                  float* cov, float offset)
+{
+   for (int c = blockIdx.x; c < R.n_chunks; c += gridDim.x) {
+      const Chunk ch = R.chunks[c];
+      if (R.n_dim == 2) {
+         if (F.dtype == FA_U16) splat_chunk<2, uint16_t>(R, F, ch, D, wimg, cov, offset);
+         else splat_chunk<2, float>(R, F, ch, D, wimg, cov, offset);
+      } else {
+         if (F.dtype == FA_U16) splat_chunk<3, uint16_t>(R, F, ch, D, wimg, cov, offset);
+         else splat_chunk<3, float>(R, F, ch, D, wimg, cov, offset);
+      }
+   }
+}
+
 // ---- scheduling: compact the active frames and their work items -----------------------
 // account = 0 only refreshes the counters the host polls.
 __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
@@ -1452,6 +1516,13 @@ hipError_t launch_warp(const RefDev& R, const FrameDev& F, const double* D, floa
 {
    hipLaunchKernelGGL(k_warp, dim3(grid), dim3(kBlock), (size_t)lds_cap * sizeof(float), s, R, F, D,
                       out, invalid, lds_cap, use_lds);
+   return hipGetLastError();
+}
+
+hipError_t launch_splat(const RefDev& R, const FrameDev& F, const double* D, float* wimg, float* cov,
+                        float offset, int grid, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_splat, dim3(grid), dim3(kBlock), 0, s, R, F, D, wimg, cov, offset);
    return hipGetLastError();
 }
 

@@ -16,6 +16,8 @@ hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, doub
 hipError_t launch_pass(const PassArgs& a, int n_dim, int grid, hipStream_t s);
 hipError_t launch_warp(const RefDev& R, const FrameDev& F, const double* D, float* out,
                        float invalid, int lds_cap, int use_lds, int grid, hipStream_t s);
+hipError_t launch_splat(const RefDev& R, const FrameDev& F, const double* D, float* wimg, float* cov,
+                        float offset, int grid, hipStream_t s);
 hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s);
 hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s);
 hipError_t launch_tr_update(const BatchDev& B, hipStream_t s);

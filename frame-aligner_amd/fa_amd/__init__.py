@@ -52,7 +52,7 @@ class AlignStats(C.Structure):
 class Tuning(C.Structure):
     """Launch shape of the fused pass (fa_tuning in include/frame_aligner.h)."""
     _fields_ = [("lds_floats", C.c_int32), ("blocks_per_cu", C.c_int32), ("use_lds", C.c_int32),
-                ("check_every", C.c_int32)]
+                ("check_every", C.c_int32), ("use_graph", C.c_int32)]
 
 
 class BatchTiming(C.Structure):
@@ -64,7 +64,7 @@ EXPORTS = [
     "fa_last_error", "fa_tr_default", "fa_create", "fa_destroy", "fa_set_reference",
     "fa_reference_release", "fa_reference_info", "fa_get_hessian", "fa_get_node_map",
     "fa_get_gradient", "fa_register_frame", "fa_deregister_frame", "fa_get_error",
-    "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_point", "fa_align", "fa_batch_create",
+    "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_image", "fa_warp_point", "fa_align", "fa_batch_create",
     "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_get_tuning", "fa_batch_set_tuning",
     "fa_batch_run", "fa_batch_eval", "fa_align_batch",
     "fa_context_stream",
@@ -101,6 +101,7 @@ def load_library(path=None):
     L.fa_get_jacobian.argtypes = [P, P, P, P]
     L.fa_warp_interpolated.argtypes = [P, P, P, C.c_float]
     L.fa_warp_point.argtypes = [P, P, i32, i32, i32, i32, P]
+    L.fa_warp_image.argtypes = [P, P, P, P, i32]
     L.fa_align.argtypes = [P, C.POINTER(TrParams), P, C.POINTER(AlignStats)]
     L.fa_batch_create.argtypes = [P, i32, C.POINTER(TrParams), C.POINTER(P)]
     L.fa_batch_destroy.argtypes = [P]
@@ -309,6 +310,14 @@ class Frame:
         _check(self.L.fa_warp_interpolated(self.h, _ptr(D), _ptr(out), float(invalid_value)))
         return out.reshape(self.ref.shape)
 
+    def warp_image(self, D, apply_offset=False):
+        """Forward splat (warpImage): returns (wimg, coverage)."""
+        D = as_D(D, self.ref.nD, self.ref.n_dim)
+        w = np.zeros(self.ref.n_px, np.float32)
+        c = np.zeros(self.ref.n_px, np.float32)
+        _check(self.L.fa_warp_image(self.h, _ptr(D), _ptr(w), _ptr(c), 1 if apply_offset else 0))
+        return w.reshape(self.ref.shape), c.reshape(self.ref.shape)
+
     def align(self, x0=None, **kw):
         n = self.ref.nD * self.ref.n_dim
         x = np.zeros(n) if x0 is None else np.array(x0, np.float64).reshape(n).copy()
@@ -428,6 +437,9 @@ class HipFrameWarper:
 
     def warpImageInterpolated(self, frame, D, invalid_value=0):
         return self._frame(frame).warp_interpolated(D, invalid_value)
+
+    def warpImage(self, frame, D):
+        return self._frame(frame).warp_image(D)
 
     def warpPoint(self, D, x, y, z, spatial_binning=1):
         return self.ref.warp_point(D, x, y, z, spatial_binning)

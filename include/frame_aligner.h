@@ -102,12 +102,14 @@ typedef struct fa_batch_timing {
    lds_floats - floats of LDS per workgroup for staged frame rows (0 disables staging;
                 at most 15360); work items whose reachable rows need more use global loads;
    blocks_per_cu - persistent workgroups per compute unit;
-   use_lds - 0 forces global loads; check_every - trust-region rounds between host polls. */
+   use_lds - 0 forces global loads; check_every - trust-region rounds between host polls;
+   use_graph - 1 replays the check_every rounds as one captured hipGraph. */
 typedef struct fa_tuning {
    int32_t lds_floats;
    int32_t blocks_per_cu;
    int32_t use_lds;
    int32_t check_every;
+   int32_t use_graph;
 } fa_tuning;
 
 const char* fa_last_error(void);
@@ -154,6 +156,13 @@ int fa_get_jacobian(fa_frame* frame, const double* D, double* jac, double* err);
 /* warpImageInterpolated (FrameWarper.h:106; FrameWarper.cpp:31-69): out gets nz*ny*nx floats;
    samples that are exactly zero become invalid_value. */
 int fa_warp_interpolated(fa_frame* frame, const double* D, float* out, float invalid_value);
+/* warpImage (FrameWarper.h:107; FrameWarper.cpp:71-136): forward splat of every source pixel
+   to p - d(p) with trilinear weights; wimg gets the weighted values, coverage the weights
+   (nz*ny*nx floats each). apply_offset = 0 splats the raw registered values (the aligner
+   splats the raw frame, FrameWarpAligner.cpp:317-346); 1 adds the registration offset.
+   warpImageNormalised (FrameWarper.cpp:138-144) is this followed by a type conversion. */
+int fa_warp_image(fa_frame* frame, const double* D, float* wimg, float* coverage,
+                  int32_t apply_offset);
 /* warpPoint (FrameWarper.h:110; FrameWarper.cpp:397-419). */
 int fa_warp_point(const fa_reference* ref, const double* D, int32_t x, int32_t y, int32_t z,
                   int32_t spatial_binning, double* out3);

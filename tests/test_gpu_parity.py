@@ -203,3 +203,47 @@ def test_split_batch_same_as_alone(ctx):
     for i in (3, 40):
         e, j = frames[i].jacobian(D[i])
         assert e == err[i] and np.array_equal(j, jac[i])
+
+
+@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+def test_warp_image_splat(ctx, name, shape, nD, scan):
+    st, orc, ref = _pair(ctx, shape, nD, scan, name=name, dtype="uint16")
+    fr = ref.register(st["frames"][0])  # uint16 counts, +1 registered for alignment
+    raw = st["frames"][0].astype(np.float32)
+    w0, c0 = fr.warp_image(np.zeros((nD, 3)))
+    assert np.array_equal(w0, raw) and np.array_equal(c0, np.ones(shape, np.float32))
+    rng = np.random.default_rng(9)
+    for D in (st["D_true"][0], rng.normal(0, 3.0, (nD, 3))):
+        if ref.n_dim == 2:
+            D = D.copy()
+            D[:, 2] = 0
+        wo, co = orc.warp_image(raw, D)
+        wg, cg = fr.warp_image(D)
+        assert np.max(np.abs(wg - wo)) <= 1e-5 * max(1.0, np.max(np.abs(wo)))
+        assert np.max(np.abs(cg - co)) <= 1e-5
+        w1, _ = fr.warp_image(D, apply_offset=True)
+        np.testing.assert_allclose(w1 - wg, cg, rtol=1e-4, atol=1e-4)
+
+
+def test_graph_replays_match_eager(ctx):
+    """Trust-region rounds replayed as captured hipGraphs give the same bits as eager launches,
+    on every replay, with eager work and synchronisations in between."""
+    a, _, ref_a = _pair(ctx, (1, 64, 56), 9, dict(line=1.0, interline=1.0), seed=51, n_frames=6)
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    frames = [ref_a.register(f) for f in a["frames_w"]]
+    batch = F.Batch(frames, **kw)
+    batch.set_tuning(use_graph=0)
+    x_eager, s_eager, _ = batch.run()
+    batch.set_tuning(use_graph=1, check_every=3)
+    batch.set_timing(True)
+    for rep in range(4):
+        xs, stats, timing = batch.run()
+        for x1, x2, s1, s2 in zip(x_eager, xs, s_eager, stats):
+            assert np.array_equal(x1, x2), rep
+            assert s1["evaluations"] == s2["evaluations"]
+        assert timing["pass_launches"] > 0
+        e, _ = frames[rep].jacobian(np.zeros((9, 3)))  # eager work between replays
+        assert e > 0
+    batch.set_tuning(use_graph=0)
+    xs, _, _ = batch.run()
+    assert all(np.array_equal(x1, x2) for x1, x2 in zip(x_eager, xs))

@@ -433,3 +433,52 @@ def test_fp64_mode_close_to_faithful():
     eb, jb = b.get_jacobian(st["frames_w"][0], D)
     assert ea == pytest.approx(eb, rel=1e-5)
     assert np.linalg.norm(ja - jb) < 1e-4 * np.linalg.norm(jb)
+
+
+# ---- forward splat (FrameWarper.cpp:71-136) ----
+
+@pytest.mark.parametrize("shape", [(1, 26, 30), (4, 12, 14)])
+def test_splat_identity_and_mass(shape):
+    nz = shape[0]
+    nD = nz * 2 + 1 if nz > 1 else 5
+    st = small_stack(shape, nD=nD, n_frames=1, seed=17)
+    o = O.Oracle(fp64=True).set_reference(st["ref_w"], nD)
+    f = st["frames_w"][0].astype(np.float64)
+    w, c = o.warp_image(st["frames_w"][0], np.zeros((nD, 3)))
+    assert np.array_equal(w, f) and np.array_equal(c, np.ones(shape))
+    # any displacement: each source that lands spreads total weight 1 (mass is conserved)
+    D = np.random.default_rng(4).normal(0, 2.0, (nD, 3))
+    if nz == 1:
+        D[:, 2] = 0
+    w, c = o.warp_image(st["frames_w"][0], D)
+    Di, Df, _ = o.node_map()
+    fr = Df.reshape(shape)
+    i = Di.reshape(shape).astype(int)
+    d = [fr * D[i + 1, k] + (1 - fr) * D[i, k] for k in range(3)]
+    zz, yy, xx = np.meshgrid(*[np.arange(n, dtype=float) for n in shape], indexing="ij")
+    l0 = [np.floor(xx - d[0]), np.floor(yy - d[1]), np.floor(zz - d[2])]
+    ok = (l0[0] >= 0) & (l0[0] <= shape[2] - 1) & (l0[1] >= 0) & (l0[1] <= shape[1] - 1)
+    if nz > 1:
+        ok &= (l0[2] >= 0) & (l0[2] <= nz - 1)
+    assert c.sum() == pytest.approx(ok.sum(), rel=1e-12)
+    assert w.sum() == pytest.approx(f[ok].sum(), rel=1e-12)
+
+
+def test_splat_integer_and_half_shift():
+    st = small_stack((1, 20, 22), nD=5, n_frames=1, seed=18)
+    o = O.Oracle(fp64=True).set_reference(st["ref_w"], 5)
+    f = st["frames_w"][0].astype(np.float64)[0]
+    D = np.zeros((5, 3))
+    D[:, 0], D[:, 1] = 2, -1  # source (x, y) lands on (x - 2, y + 1)
+    w, c = o.warp_image(st["frames_w"][0], D)
+    expect = np.zeros((20, 22))
+    expect[1:, :20] = f[:19, 2:]
+    assert np.array_equal(w[0], expect)
+    assert np.array_equal(c[0], (expect != 0).astype(float))
+    D = np.zeros((5, 3))
+    D[:, 0] = -0.5  # each source splits evenly between x + 0 and x + 1 (last column keeps both)
+    w, c = o.warp_image(st["frames_w"][0], D)
+    expect = 0.5 * f.copy()
+    expect[:, 1:] += 0.5 * f[:, :-1]
+    expect[:, -1] += 0.5 * f[:, -1]
+    np.testing.assert_allclose(w[0], expect, rtol=1e-15)
