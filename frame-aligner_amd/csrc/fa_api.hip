@@ -113,13 +113,10 @@ struct fa_batch {
    std::vector<int> part_first;
    bool finished = false;
    size_t timed = 0;
-   // hipGraph of check_every trust-region rounds (schedule, pass, update), replayed per poll;
-   // gev holds its 2 x check_every timing events when it was captured with timing on
+   // hipGraph of check_every trust-region rounds (schedule, pass, update), replayed per poll
    hipGraphExec_t gexec = nullptr;
-   int graph_rounds = 0, graph_timing = 0;
-   double graph_pass_ms = 0;
+   int graph_rounds = 0;
    fa_tr_params graph_prm{};
-   std::vector<hipEvent_t> gev;
    std::vector<fa_frame*> frames;
    fa_tr_params prm{};
    std::vector<BatchFrame> hframes;
@@ -194,8 +191,6 @@ struct fa_batch {
    {
       if (gexec) (void)hipGraphExecDestroy(gexec);
       gexec = nullptr;
-      for (auto e : gev) (void)hipEventDestroy(e);
-      gev.clear();
    }
 
    ~fa_batch()
@@ -243,7 +238,7 @@ void default_tuning(fa_tuning* t, const std::vector<fa_frame*>& frames)
    t->lds_floats = default_lds_floats(frames);
    t->blocks_per_cu = kBlocksPerCu;
    t->use_lds = 1;
-   t->check_every = 4;
+   t->check_every = 8;
    t->use_graph = 1;
 }
 
@@ -397,7 +392,6 @@ int run_begin(fa_batch* b, const double* x)
    HIPCHK(launch_tr_init(b->dev(), 0, s));
    b->finished = false;
    b->timed = 0;
-   b->graph_pass_ms = 0;
    return FA_OK;
 }
 
@@ -450,13 +444,12 @@ int run_finish(fa_batch* b, double* x, fa_align_stats* stats, int first_user)
 
 int add_timing(fa_batch* b, fa_batch_timing* t)
 {
-   double pass_ms = b->graph_pass_ms;
-   if (!b->gexec || !b->tune.use_graph)
-      for (size_t i = 0; i < b->timed; i++) {
-         float ms = 0;
-         HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
-         pass_ms += ms;
-      }
+   double pass_ms = 0;
+   for (size_t i = 0; i < b->timed; i++) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
+      pass_ms += ms;
+   }
    t->pass_ms += pass_ms;
    t->pass_launches += (int64_t)b->timed;
    t->pass_bytes += (double)b->h_counters->bytes;
@@ -467,27 +460,16 @@ int add_timing(fa_batch* b, fa_batch_timing* t)
 // Captures check_every rounds into a graph (once per tuning / parameter set).
 int ensure_graph(fa_batch* b, int K)
 {
-   if (b->gexec && b->graph_rounds == K && b->graph_timing == b->timing &&
-       std::memcmp(&b->graph_prm, &b->prm, sizeof(fa_tr_params)) == 0)
+   if (b->gexec && b->graph_rounds == K && std::memcmp(&b->graph_prm, &b->prm, sizeof(fa_tr_params)) == 0)
       return FA_OK;
    b->drop_graph();
-   if (b->timing)
-      for (int i = 0; i < 2 * K; i++) {
-         hipEvent_t e;
-         HIPCHK(hipEventCreate(&e));
-         b->gev.push_back(e);
-      }
    hipStream_t s = b->stream;
    const BatchDev B = b->dev();
    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
    int rc = FA_OK;
    for (int r = 0; r < K && rc == FA_OK; r++) {
       if (launch_schedule(B, 1, s) != hipSuccess) rc = set_err(FA_E_DEVICE, "graph capture failed");
-      if (rc == FA_OK && b->timing && hipEventRecord(b->gev[2 * r], s) != hipSuccess)
-         rc = set_err(FA_E_DEVICE, "graph capture failed");
       if (rc == FA_OK) rc = launch_passes(b, s);
-      if (rc == FA_OK && b->timing && hipEventRecord(b->gev[2 * r + 1], s) != hipSuccess)
-         rc = set_err(FA_E_DEVICE, "graph capture failed");
       if (rc == FA_OK && launch_tr_update(B, s) != hipSuccess) rc = set_err(FA_E_DEVICE, "graph capture failed");
    }
    hipGraph_t graph = nullptr;
@@ -505,20 +487,7 @@ int ensure_graph(fa_batch* b, int K)
       return set_err(FA_E_DEVICE, "graph instantiation failed");
    }
    b->graph_rounds = K;
-   b->graph_timing = b->timing;
    b->graph_prm = b->prm;
-   return FA_OK;
-}
-
-// Adds the pass durations of the last graph replay.
-int collect_graph_timing(fa_batch* b)
-{
-   for (int r = 0; r < b->graph_rounds; r++) {
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, b->gev[2 * r], b->gev[2 * r + 1]));
-      b->graph_pass_ms += ms;
-   }
-   b->timed += (size_t)b->graph_rounds;
    return FA_OK;
 }
 
@@ -545,7 +514,9 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
       }
    }
    const int K = std::max(1, b->tune.check_every);
-   const bool graphs = b->tune.use_graph != 0;
+   // per-pass HIP event timing needs eager launches: events recorded inside a captured graph
+   // cannot be read back with hipEventElapsedTime
+   const bool graphs = b->tune.use_graph != 0 && !b->timing;
    if (graphs)
       for (fa_batch* q : leaves) CHK(ensure_graph(q, K));
    for (size_t i = 0; i < leaves.size(); i++) CHK(run_begin(leaves[i], x_inout + xoff[i]));
@@ -575,7 +546,6 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
       for (fa_batch* q : leaves)
          if (!q->finished) {
             HIPCHK(hipStreamSynchronize(q->stream));
-            if (graphs && q->timing) CHK(collect_graph_timing(q));
             q->finished = q->h_counters->n_active_frames == 0;
             all = all && q->finished;
          }

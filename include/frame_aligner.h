@@ -103,7 +103,8 @@ typedef struct fa_batch_timing {
                 at most 15360); work items whose reachable rows need more use global loads;
    blocks_per_cu - persistent workgroups per compute unit;
    use_lds - 0 forces global loads; check_every - trust-region rounds between host polls;
-   use_graph - 1 replays the check_every rounds as one captured hipGraph. */
+   use_graph - 1 replays the check_every rounds as one captured hipGraph (runs with
+               fa_batch_set_timing on launch eagerly, to time each pass with HIP events). */
 typedef struct fa_tuning {
    int32_t lds_floats;
    int32_t blocks_per_cu;

@@ -221,8 +221,8 @@ def test_warp_image_splat(ctx, name, shape, nD, scan):
         wg, cg = fr.warp_image(D)
         assert np.max(np.abs(wg - wo)) <= 1e-5 * max(1.0, np.max(np.abs(wo)))
         assert np.max(np.abs(cg - co)) <= 1e-5
-        w1, _ = fr.warp_image(D, apply_offset=True)
-        np.testing.assert_allclose(w1 - wg, cg, rtol=1e-4, atol=1e-4)
+        w1, _ = fr.warp_image(D, apply_offset=True)  # + offset x coverage, up to float rounding
+        assert np.max(np.abs((w1 - wg) - cg)) <= 1e-5 * max(1.0, np.max(np.abs(w1)))
 
 
 def test_graph_replays_match_eager(ctx):
@@ -235,13 +235,11 @@ def test_graph_replays_match_eager(ctx):
     batch.set_tuning(use_graph=0)
     x_eager, s_eager, _ = batch.run()
     batch.set_tuning(use_graph=1, check_every=3)
-    batch.set_timing(True)
     for rep in range(4):
         xs, stats, timing = batch.run()
         for x1, x2, s1, s2 in zip(x_eager, xs, s_eager, stats):
             assert np.array_equal(x1, x2), rep
             assert s1["evaluations"] == s2["evaluations"]
-        assert timing["pass_launches"] > 0
         e, _ = frames[rep].jacobian(np.zeros((9, 3)))  # eager work between replays
         assert e > 0
     batch.set_tuning(use_graph=0)
