@@ -8,6 +8,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <string>
 #include <vector>
@@ -128,6 +129,7 @@ struct fa_batch {
    int timing = 0;
    int tr_lds = 0;
    std::vector<int> user;  // internal index -> caller's index
+   int n_slots = 0;        // serving batches: number of slots (frames are admitted into them)
    RefDev* d_refs = nullptr;
    BatchFrame* d_frames = nullptr;
    TrState* d_state = nullptr;
@@ -167,7 +169,7 @@ struct fa_batch {
       B.prm.max_evals = prm.max_evals;
       B.prm.sub_max_iter = prm.sub_max_iter;
       B.prm.skip_repeat = prm.skip_repeat;
-      B.n_frames = (int32_t)frames.size();
+      B.n_frames = n_slots > 0 ? n_slots : (int32_t)frames.size();
       B.tr_lds = tr_lds;
       return B;
    }
@@ -645,6 +647,210 @@ int single_batch(fa_frame* f, fa_batch** out)
 
 }  // namespace
 
+// ---- serving: continuous batching over fixed slots ----------------------------------------
+
+struct fa_server {
+   struct Req {
+      fa_frame* f;
+      double* x;
+      int64_t tag;
+   };
+   fa_context* ctx = nullptr;
+   fa_batch* b = nullptr;  // slot batch: n_slots frames' worth of state, sized for the largest reference
+   std::vector<fa_reference*> refs;
+   int S = 0, nmax = 0;
+   std::deque<Req> pending;
+   std::vector<int> free_slots;
+   std::vector<Req> slot_req;
+   std::vector<char> busy;
+   std::deque<fa_server_result> results;
+   BatchFrame* h_bf = nullptr;  // pinned staging
+   double* h_x = nullptr;
+   int32_t* h_slots = nullptr;
+   TrState* h_state = nullptr;
+   double* h_xall = nullptr;
+   int32_t* d_slots = nullptr;
+
+   ~fa_server()
+   {
+      delete b;
+      if (h_bf) (void)hipHostFree(h_bf);
+      if (h_x) (void)hipHostFree(h_x);
+      if (h_slots) (void)hipHostFree(h_slots);
+      if (h_state) (void)hipHostFree(h_state);
+      if (h_xall) (void)hipHostFree(h_xall);
+      dfree(d_slots);
+   }
+};
+
+namespace {
+
+template <typename T>
+int halloc(T** p, size_t count)
+{
+   if (hipHostMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return set_err(FA_E_OOM, "Not enough memory to allocate");
+   }
+   return FA_OK;
+}
+
+// A batch whose frames are slots: every slot is sized for the largest of the references and
+// starts finished (out of the schedule) until a request is admitted into it.
+int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, int S,
+                     const fa_tr_params* prm, fa_batch** out)
+{
+   auto* b = new fa_batch();
+   b->ctx = ctx;
+   b->n_slots = S;
+   if (prm) b->prm = *prm;
+   else fa_tr_default(&b->prm);
+   int nmax = 0, ndmax = 0, fmax = 0, tr_need = 0;
+   int64_t cmax = 0;
+   bool has2 = false, has3 = false;
+   std::vector<RefDev> rdev;
+   int64_t widest = 0;
+   for (fa_reference* r : refs) {
+      rdev.push_back(r->dev);
+      nmax = std::max(nmax, r->nD * r->n_dim);
+      ndmax = std::max(ndmax, r->nD);
+      fmax = std::max(fmax, r->nD * r->n_dim * r->n_dim);
+      cmax = std::max<int64_t>(cmax, (int64_t)r->chunks.size());
+      (r->n_dim == 2 ? has2 : has3) = true;
+      tr_need = std::max(tr_need, 5 * r->nD * r->n_dim + 4 * r->nD * r->n_dim * r->n_dim);
+      const int64_t stride = ((r->nx + 2 * kPad) + 3) & ~3;
+      const int64_t rows = (r->chunk_px + r->nx - 1) / r->nx + 6;
+      widest = std::max(widest, rows * stride * (r->n_dim == 3 ? 3 : 1));
+   }
+   b->tune.lds_floats = (int)std::min<int64_t>(widest, kLdsCapFloats);
+   b->tune.blocks_per_cu = kBlocksPerCu;
+   b->tune.use_lds = 1;
+   b->tune.check_every = 8;
+   b->tune.use_graph = 0;
+   b->tr_lds = tr_need <= kTrLdsMaxDoubles ? tr_need : 0;
+   b->hframes.resize(S);
+   for (int j = 0; j < S; j++) {
+      BatchFrame& bf = b->hframes[j];
+      bf = BatchFrame{};
+      bf.work_base = (int64_t)j * cmax;
+      bf.x_base = (int64_t)j * nmax;
+      bf.vec_base = (int64_t)j * 4 * nmax;
+      bf.fac_base = (int64_t)j * 2 * fmax;
+      bf.d_base = (int64_t)j * 3 * ndmax;
+      b->user.push_back(j);
+   }
+   b->total_work = S * cmax;
+   b->total_x = (int64_t)S * nmax;
+   b->total_vec = 4 * (int64_t)S * nmax;
+   b->total_fac = 2 * (int64_t)S * fmax;
+   b->total_D = 3 * (int64_t)S * ndmax;
+   b->work2 = has2 ? b->total_work : 0;
+   b->work3 = has3 ? b->total_work : 0;
+   int rc;
+   auto fail = [&](int code) {
+      delete b;
+      return code;
+   };
+   if ((rc = dalloc(&b->d_refs, rdev.size())) || (rc = dalloc(&b->d_frames, S)) ||
+       (rc = dalloc(&b->d_state, S)) || (rc = dalloc(&b->d_x, b->total_x)) ||
+       (rc = dalloc(&b->d_vec, b->total_vec)) || (rc = dalloc(&b->d_fac, b->total_fac)) ||
+       (rc = dalloc(&b->d_D, b->total_D)) ||
+       (rc = dalloc(&b->d_partials, (size_t)b->total_work * kPassStride)) ||
+       (rc = dalloc(&b->d_active_frames, S)) || (rc = dalloc(&b->d_active_base, S + 1)) ||
+       (rc = dalloc(&b->d_counters, 1)) || (rc = halloc(&b->h_counters, 1)))
+      return fail(rc);
+   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      b->stream = nullptr;
+      return fail(set_err(FA_E_DEVICE, "stream creation failed"));
+   }
+   std::vector<TrState> st(S);
+   for (auto& t : st) {
+      t = TrState{};
+      t.done = 1;
+   }
+   hipStream_t s = b->stream;
+   if (hipMemcpyAsync(b->d_refs, rdev.data(), rdev.size() * sizeof(RefDev), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(b->d_frames, b->hframes.data(), S * sizeof(BatchFrame), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(b->d_state, st.data(), S * sizeof(TrState), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemsetAsync(b->d_counters, 0, sizeof(Counters), s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(set_err(FA_E_DEVICE, "server upload failed"));
+   }
+   set_grids(b);
+   *out = b;
+   return FA_OK;
+}
+
+int server_step(fa_server* sv, int rounds)
+{
+   fa_batch* b = sv->b;
+   hipStream_t s = b->stream;
+   const BatchDev B = b->dev();
+   // admit queued requests into free slots
+   int na = 0;
+   while (!sv->free_slots.empty() && !sv->pending.empty()) {
+      fa_server::Req r = sv->pending.front();
+      sv->pending.pop_front();
+      const int slot = sv->free_slots.back();
+      sv->free_slots.pop_back();
+      int ri = (int)(std::find(sv->refs.begin(), sv->refs.end(), r.f->ref) - sv->refs.begin());
+      BatchFrame bf = b->hframes[slot];
+      bf.f.data = r.f->d_data;
+      bf.f.dtype = r.f->dtype;
+      bf.f.offset = r.f->offset;
+      bf.ref = ri;
+      bf.n = r.f->ref->nD * r.f->ref->n_dim;
+      sv->h_bf[na] = bf;
+      std::memcpy(sv->h_x + (size_t)na * sv->nmax, r.x, bf.n * sizeof(double));
+      sv->h_slots[na] = slot;
+      sv->slot_req[slot] = r;
+      sv->busy[slot] = 1;
+      na++;
+   }
+   for (int i = 0; i < na; i++) {
+      const int slot = sv->h_slots[i];
+      HIPCHK(hipMemcpyAsync(b->d_frames + slot, &sv->h_bf[i], sizeof(BatchFrame), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(b->d_x + b->hframes[slot].x_base, sv->h_x + (size_t)i * sv->nmax,
+                            sv->h_bf[i].n * sizeof(double), hipMemcpyHostToDevice, s));
+   }
+   if (na) {
+      HIPCHK(hipMemcpyAsync(sv->d_slots, sv->h_slots, na * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      HIPCHK(launch_tr_init_slots(B, sv->d_slots, na, s));
+   }
+   bool any = false;
+   for (int i = 0; i < sv->S; i++) any = any || sv->busy[i];
+   if (!any) return FA_OK;
+   for (int r = 0; r < rounds; r++) CHK(run_round(b));
+   CHK(run_poll(b));
+   HIPCHK(hipMemcpyAsync(sv->h_state, b->d_state, sv->S * sizeof(TrState), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(sv->h_xall, b->d_x, b->total_x * sizeof(double), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   // harvest finished slots
+   for (int j = 0; j < sv->S; j++) {
+      if (!sv->busy[j] || !sv->h_state[j].done) continue;
+      const fa_server::Req& r = sv->slot_req[j];
+      const int n = r.f->ref->nD * r.f->ref->n_dim;
+      std::memcpy(r.x, sv->h_xall + b->hframes[j].x_base, n * sizeof(double));
+      fa_server_result res{};
+      res.tag = r.tag;
+      const TrState& t = sv->h_state[j];
+      res.stats.iterations = t.iterations;
+      res.stats.evaluations = t.evaluations;
+      res.stats.rejected = t.rejected;
+      res.stats.status = t.status;
+      res.stats.f_final = t.f;
+      res.stats.radius_final = t.radius;
+      sv->results.push_back(res);
+      sv->busy[j] = 0;
+      sv->free_slots.push_back(j);
+   }
+   return FA_OK;
+}
+
+}  // namespace
+
 // ---- C ABI ------------------------------------------------------------------------------------
 
 extern "C" {
@@ -861,7 +1067,7 @@ int fa_reference_release(fa_reference* r)
 {
    if (!r) return FA_OK;
    if (r->live_frames > 0)
-      return set_err(FA_E_STATE, "reference still has registered frames; deregister them first");
+      return set_err(FA_E_STATE, "reference still has registered frames or servers; release them first");
    DeviceGuard g(r->ctx->device);
    if (r->counted) r->ctx->live_refs--;
    dfree(r->d_rg4);
@@ -1184,6 +1390,97 @@ int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, 
    int rc = fa_batch_run(b, x_inout, stats, nullptr);
    fa_batch_destroy(b);
    return rc;
+}
+
+int fa_context_mem_info(fa_context* ctx, uint64_t* free_bytes, uint64_t* total_bytes)
+{
+   if (!ctx || !free_bytes || !total_bytes) return set_err(FA_E_ARG, "null argument");
+   DeviceGuard g(ctx->device);
+   size_t f = 0, t = 0;
+   HIPCHK(hipMemGetInfo(&f, &t));
+   *free_bytes = f;
+   *total_bytes = t;
+   return FA_OK;
+}
+
+int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs, int32_t slots,
+                     const fa_tr_params* prm, fa_server** out)
+{
+   if (!ctx || !refs || n_refs <= 0 || slots <= 0 || !out) return set_err(FA_E_ARG, "invalid server arguments");
+   *out = nullptr;
+   for (int i = 0; i < n_refs; i++)
+      if (!refs[i] || refs[i]->ctx != ctx) return set_err(FA_E_ARG, "reference of another context");
+   std::lock_guard<std::mutex> lk(ctx->mu);
+   DeviceGuard g(ctx->device);
+   auto* sv = new fa_server();
+   sv->ctx = ctx;
+   for (int i = 0; i < n_refs; i++)
+      if (std::find(sv->refs.begin(), sv->refs.end(), refs[i]) == sv->refs.end()) sv->refs.push_back(refs[i]);
+   sv->S = slots;
+   for (fa_reference* r : sv->refs) sv->nmax = std::max(sv->nmax, r->nD * r->n_dim);
+   int rc = slot_batch_build(ctx, sv->refs, slots, prm, &sv->b);
+   if (rc == FA_OK)
+      if ((rc = halloc(&sv->h_bf, slots)) || (rc = halloc(&sv->h_x, (size_t)slots * sv->nmax)) ||
+          (rc = halloc(&sv->h_slots, slots)) || (rc = halloc(&sv->h_state, slots)) ||
+          (rc = halloc(&sv->h_xall, (size_t)slots * sv->nmax)) || (rc = dalloc(&sv->d_slots, slots))) {
+      }
+   if (rc != FA_OK) {
+      delete sv;
+      return rc;
+   }
+   sv->slot_req.resize(slots);
+   sv->busy.assign(slots, 0);
+   for (int j = slots - 1; j >= 0; j--) sv->free_slots.push_back(j);
+   for (fa_reference* r : sv->refs) r->live_frames++;  // the references outlive the server
+   *out = sv;
+   return FA_OK;
+}
+
+int fa_server_destroy(fa_server* sv)
+{
+   if (!sv) return FA_OK;
+   std::lock_guard<std::mutex> lk(sv->ctx->mu);
+   DeviceGuard g(sv->ctx->device);
+   for (fa_reference* r : sv->refs) r->live_frames--;
+   delete sv;
+   return FA_OK;
+}
+
+int fa_server_submit(fa_server* sv, fa_frame* frame, double* x_inout, int64_t tag)
+{
+   if (!sv || !frame || !x_inout) return set_err(FA_E_ARG, "null argument");
+   if (std::find(sv->refs.begin(), sv->refs.end(), frame->ref) == sv->refs.end())
+      return set_err(FA_E_ARG, "frame of a reference the server does not serve");
+   std::lock_guard<std::mutex> lk(sv->ctx->mu);
+   sv->pending.push_back(fa_server::Req{frame, x_inout, tag});
+   return FA_OK;
+}
+
+int fa_server_step(fa_server* sv, int32_t rounds, int32_t* in_flight, int32_t* queued)
+{
+   if (!sv || rounds < 1) return set_err(FA_E_ARG, "invalid argument");
+   std::lock_guard<std::mutex> lk(sv->ctx->mu);
+   DeviceGuard g(sv->ctx->device);
+   CHK(server_step(sv, rounds));
+   if (in_flight) {
+      int n = 0;
+      for (char c : sv->busy) n += c;
+      *in_flight = n;
+   }
+   if (queued) *queued = (int32_t)sv->pending.size();
+   return FA_OK;
+}
+
+int fa_server_results(fa_server* sv, fa_server_result* out, int32_t max, int32_t* n_out)
+{
+   if (!sv || !n_out || (max > 0 && !out)) return set_err(FA_E_ARG, "null argument");
+   int n = 0;
+   while (n < max && !sv->results.empty()) {
+      out[n++] = sv->results.front();
+      sv->results.pop_front();
+   }
+   *n_out = n;
+   return FA_OK;
 }
 
 }  // extern "C"

@@ -901,62 +901,68 @@ __global__ __launch_bounds__(kBlock) void k_splat(RefDev R, FrameDev F, const do
 }
 
 // ---- scheduling: compact the active frames and their work items -----------------------
-// account = 0 only refreshes the counters the host polls.
+// The active list holds the 2-D frames first, then the 3-D frames (k_pass<2> and k_pass<3>
+// split the work there), each kind in frame order. account = 0 only refreshes the counters
+// the host polls.
 __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
 {
-   __shared__ int64_t s_work[1024];
-   __shared__ int64_t s_work2[1024];
-   __shared__ int32_t s_fr[1024];
-   __shared__ unsigned long long s_bytes[1024];
+   __shared__ int64_t s_w[2][1024];
+   __shared__ int32_t s_f[2][1024];
+   __shared__ unsigned long long s_b[1024];
    const int t = threadIdx.x, T = blockDim.x;
    const int n = B.n_frames;
    const int f0 = (int)((int64_t)n * t / T), f1 = (int)((int64_t)n * (t + 1) / T);
-   int64_t work = 0, work2 = 0;
-   int32_t fr = 0;
+   int64_t work[2] = {0, 0};
+   int32_t fr[2] = {0, 0};
    unsigned long long bytes = 0;
    for (int f = f0; f < f1; f++) {
       if (B.state[f].done) continue;
       const RefDev& R = B.refs[B.frames[f].ref];
-      work += R.n_chunks;
-      if (R.n_dim == 2) work2 += R.n_chunks;
-      fr++;
+      const int kind = R.n_dim == 3;
+      work[kind] += R.n_chunks;
+      fr[kind]++;
       bytes += (unsigned long long)R.n_px *
                (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + (R.n_dim == 3 ? 20 : 16));
    }
-   s_work[t] = work;
-   s_work2[t] = work2;
-   s_fr[t] = fr;
-   s_bytes[t] = bytes;
+   for (int k = 0; k < 2; k++) {
+      s_w[k][t] = work[k];
+      s_f[k][t] = fr[k];
+   }
+   s_b[t] = bytes;
    __syncthreads();
-   // inclusive scan (Hillis-Steele)
+   // inclusive scans (Hillis-Steele)
    for (int off = 1; off < T; off <<= 1) {
-      int64_t aw = t >= off ? s_work[t - off] : 0;
-      int64_t aw2 = t >= off ? s_work2[t - off] : 0;
-      int32_t af = t >= off ? s_fr[t - off] : 0;
-      unsigned long long ab = t >= off ? s_bytes[t - off] : 0;
+      int64_t aw0 = t >= off ? s_w[0][t - off] : 0, aw1 = t >= off ? s_w[1][t - off] : 0;
+      int32_t af0 = t >= off ? s_f[0][t - off] : 0, af1 = t >= off ? s_f[1][t - off] : 0;
+      unsigned long long ab = t >= off ? s_b[t - off] : 0;
       __syncthreads();
-      s_work[t] += aw;
-      s_work2[t] += aw2;
-      s_fr[t] += af;
-      s_bytes[t] += ab;
+      s_w[0][t] += aw0;
+      s_w[1][t] += aw1;
+      s_f[0][t] += af0;
+      s_f[1][t] += af1;
+      s_b[t] += ab;
       __syncthreads();
    }
-   int64_t wbase = s_work[t] - work;
-   int32_t fbase = s_fr[t] - fr;
+   const int32_t F2 = s_f[0][T - 1], F3 = s_f[1][T - 1];
+   const int64_t W2 = s_w[0][T - 1], W3 = s_w[1][T - 1];
+   int64_t wbase[2] = {s_w[0][t] - work[0], W2 + s_w[1][t] - work[1]};
+   int32_t fbase[2] = {s_f[0][t] - fr[0], F2 + s_f[1][t] - fr[1]};
    for (int f = f0; f < f1; f++) {
       if (B.state[f].done) continue;
-      B.active_frames[fbase] = f;
-      B.active_base[fbase] = wbase;
-      wbase += B.refs[B.frames[f].ref].n_chunks;
-      fbase++;
+      const RefDev& R = B.refs[B.frames[f].ref];
+      const int kind = R.n_dim == 3;
+      B.active_frames[fbase[kind]] = f;
+      B.active_base[fbase[kind]] = wbase[kind];
+      wbase[kind] += R.n_chunks;
+      fbase[kind]++;
    }
    if (t == T - 1) {
-      B.active_base[s_fr[t]] = s_work[t];
-      B.counters->n_active_frames = s_fr[t];
-      B.counters->n_active_work = s_work[t];
-      B.counters->n_active_work2 = s_work2[t];
+      B.active_base[F2 + F3] = W2 + W3;
+      B.counters->n_active_frames = F2 + F3;
+      B.counters->n_active_work = W2 + W3;
+      B.counters->n_active_work2 = W2;
       if (account) {
-         B.counters->bytes += s_bytes[t];
+         B.counters->bytes += s_b[t];
          B.counters->rounds += 1;
       }
    }
@@ -1289,18 +1295,17 @@ __global__ __launch_bounds__(64) void k_reduce_eval(BatchDev B)
    if (threadIdx.x == 0) B.state[j].f = f;
 }
 
-// Starts every frame: x (already uploaded) is the first evaluation point.
-__global__ void k_tr_init(BatchDev B, int eval_only)
+// Starts frame j: x (already uploaded) is the first evaluation point.
+__device__ __forceinline__ void init_frame(const BatchDev& B, const int j, const int eval_only)
 {
-   const int j = blockIdx.x;
    const BatchFrame& bf = B.frames[j];
    const RefDev& R = B.refs[bf.ref];
-   TrScratch S = scratch_of(B, bf);
+   const double* x = B.x + bf.x_base;
    double* Dt = B.Dtrial + bf.d_base;
    const int m = R.n_dim;
    if (!eval_only)
       for (int i = threadIdx.x; i < R.nD; i += blockDim.x)
-         for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < m ? S.x[i * m + c] : 0.0;
+         for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < m ? x[i * m + c] : 0.0;
    if (threadIdx.x == 0) {
       TrState st = {};
       st.radius = B.prm.radius;
@@ -1310,9 +1315,11 @@ __global__ void k_tr_init(BatchDev B, int eval_only)
    }
 }
 
-// One wave per frame. The frame's vectors and the reference's JtJ blocks and factor are
-// copied into LDS first (when they fit), because the triangular solves are a sequential chain
-// that would otherwise wait on a global-memory round trip at every block.
+__global__ void k_tr_init(BatchDev B, int eval_only) { init_frame(B, blockIdx.x, eval_only); }
+
+// Starts the listed slots of a serving batch.
+__global__ void k_tr_init_slots(BatchDev B, const int32_t* slots) { init_frame(B, slots[blockIdx.x], 0); }
+
 template <int M, bool LDS>
 __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
 {
@@ -1535,6 +1542,12 @@ hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s)
 hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s)
 {
    hipLaunchKernelGGL(k_tr_init, dim3(B.n_frames), dim3(64), 0, s, B, eval_only);
+   return hipGetLastError();
+}
+
+hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_tr_init_slots, dim3(n), dim3(64), 0, s, B, slots);
    return hipGetLastError();
 }
 

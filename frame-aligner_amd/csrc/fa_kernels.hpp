@@ -20,6 +20,7 @@ hipError_t launch_splat(const RefDev& R, const FrameDev& F, const double* D, flo
                         float offset, int grid, hipStream_t s);
 hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s);
 hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s);
+hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, hipStream_t s);
 hipError_t launch_tr_update(const BatchDev& B, hipStream_t s);
 hipError_t launch_reduce_eval(const BatchDev& B, hipStream_t s);
 

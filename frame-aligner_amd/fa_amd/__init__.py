@@ -55,6 +55,10 @@ class Tuning(C.Structure):
                 ("check_every", C.c_int32), ("use_graph", C.c_int32)]
 
 
+class ServerResult(C.Structure):
+    _fields_ = [("tag", C.c_int64), ("stats", AlignStats)]
+
+
 class BatchTiming(C.Structure):
     _fields_ = [("wall_ms", C.c_double), ("pass_ms", C.c_double), ("pass_launches", C.c_int64),
                 ("pass_bytes", C.c_double), ("tr_rounds", C.c_int64)]
@@ -67,7 +71,8 @@ EXPORTS = [
     "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_image", "fa_warp_point", "fa_align", "fa_batch_create",
     "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_get_tuning", "fa_batch_set_tuning",
     "fa_batch_run", "fa_batch_eval", "fa_align_batch",
-    "fa_context_stream",
+    "fa_context_stream", "fa_context_mem_info", "fa_server_create", "fa_server_destroy",
+    "fa_server_submit", "fa_server_step", "fa_server_results",
 ]
 
 _lib = None
@@ -111,6 +116,12 @@ def load_library(path=None):
     L.fa_batch_run.argtypes = [P, P, P, C.POINTER(BatchTiming)]
     L.fa_batch_eval.argtypes = [P, P, P, P]
     L.fa_align_batch.argtypes = [P, i32, C.POINTER(TrParams), P, P]
+    L.fa_context_mem_info.argtypes = [P, P, P]
+    L.fa_server_create.argtypes = [P, P, i32, i32, C.POINTER(TrParams), C.POINTER(P)]
+    L.fa_server_destroy.argtypes = [P]
+    L.fa_server_submit.argtypes = [P, P, P, C.c_int64]
+    L.fa_server_step.argtypes = [P, i32, P, P]
+    L.fa_server_results.argtypes = [P, P, i32, P]
     L.fa_context_stream.argtypes = [P]
     L.fa_context_stream.restype = P
     _lib = L
@@ -159,7 +170,7 @@ class Context:
         """Releases every batch, frame and reference made on this context, then the context."""
         if getattr(self, "h", None):
             kids = list(self._children)
-            for kind in (Batch, Frame, Reference):
+            for kind in (Server, Batch, Frame, Reference):
                 for k in kids:
                     if isinstance(k, kind):
                         k._release_handle()
@@ -175,6 +186,12 @@ class Context:
     @property
     def stream(self):
         return self.L.fa_context_stream(self.h)
+
+    def mem_info(self):
+        """(free, total) device bytes."""
+        f, t = C.c_uint64(), C.c_uint64()
+        _check(self.L.fa_context_mem_info(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
 
 
 class Reference:
@@ -204,8 +221,10 @@ class Reference:
         self.n_px = nz * ny * nx
 
     def _release_handle(self):
+        """Frees the device copy; raises FaError (and keeps it) while frames or servers of
+        this reference are alive."""
         if getattr(self, "h", None):
-            self.L.fa_reference_release(self.h)
+            _check(self.L.fa_reference_release(self.h))
             self.h = None
 
     release = _release_handle
@@ -388,6 +407,68 @@ class Batch:
         jac = np.zeros(self.offsets[-1])
         _check(self.L.fa_batch_eval(self.h, _ptr(D), _ptr(err), _ptr(jac)))
         return err, [jac[self.offsets[i]:self.offsets[i + 1]].copy() for i in range(len(self.frames))]
+
+
+class Server:
+    """Continuous batching: frames submitted at any time join the trust-region rounds of the
+    frames already in flight (a fixed number of slots) and leave when they finish."""
+
+    def __init__(self, refs, slots, **kw):
+        refs = list(refs)
+        self.L = refs[0].L
+        self.ctx = refs[0].ctx
+        arr = (C.c_void_p * len(refs))(*[r.h.value for r in refs])
+        p = tr_params(**kw)
+        h = C.c_void_p()
+        _check(self.L.fa_server_create(self.ctx.h, arr, len(refs), int(slots), C.byref(p), C.byref(h)))
+        self.h = h
+        self.slots = int(slots)
+        self._live = {}  # tag -> (frame, x): kept alive until the result is taken
+        self._buf = (ServerResult * max(self.slots, 64))()
+        self.ctx._adopt(self)
+
+    def _release_handle(self):
+        if getattr(self, "h", None):
+            self.L.fa_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
+
+    close = _release_handle
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, frame, tag, x0=None):
+        n = frame.ref.nD * frame.ref.n_dim
+        x = np.zeros(n) if x0 is None else np.array(x0, np.float64).reshape(n).copy()
+        if tag in self._live:
+            raise ValueError(f"tag {tag} already in flight")
+        self._live[tag] = (frame, x)
+        try:
+            _check(self.L.fa_server_submit(self.h, frame.h, _ptr(x), int(tag)))
+        except Exception:
+            del self._live[tag]
+            raise
+
+    def step(self, rounds=1):
+        """Admits queued frames, runs rounds trust-region rounds and collects what finished:
+        returns (in_flight, queued, [(tag, x, stats), ...])."""
+        a, q = C.c_int32(), C.c_int32()
+        _check(self.L.fa_server_step(self.h, int(rounds), C.byref(a), C.byref(q)))
+        out = []
+        while True:
+            n = C.c_int32()
+            _check(self.L.fa_server_results(self.h, self._buf, len(self._buf), C.byref(n)))
+            for i in range(n.value):
+                r = self._buf[i]
+                _, x = self._live.pop(r.tag)
+                out.append((r.tag, x, stats_dict(r.stats)))
+            if n.value < len(self._buf):
+                break
+        return a.value, q.value, out
 
 
 class HipFrameWarper:

@@ -1,0 +1,140 @@
+"""Serving loop: frames streaming from several acquisitions, aligned by continuous batching.
+
+Every acquisition keeps its reference (packed intensity + gradients, node map, Hessian
+factors) resident in HBM. Frames arrive with exponential inter-arrival times; on arrival a
+frame is uploaded (registered) when the device memory it needs is free, else it waits in the
+host queue; uploaded frames are submitted to a Server whose fixed slots run trust-region
+rounds together, so a frame that arrives while others are on the GPU joins their launches at
+the next round and leaves when it finishes. Latency is arrival -> result on the host.
+
+The reference aligner has no serving loop: its MEX front end queues addFrame calls on a
+thread pool (FrameAlignerMex/FrameAlignerMex.cpp:110-130), one frame per task.
+"""
+import time
+
+import numpy as np
+
+from . import synth
+
+# Acquisition sizes (z, y, x) of the serving workload, 256^2 x 1 up to 1024^2 x 8.
+SERVE_SHAPES = [(1, 256, 256), (1, 512, 512), (1, 1024, 1024), (4, 256, 256),
+                (4, 512, 512), (8, 256, 256), (8, 512, 512), (8, 1024, 1024)]
+
+
+def make_acquisitions(seed=2026, shapes=SERVE_SHAPES, pool=3, node_range=(10, 40), mean=50.0):
+    """One seeded acquisition per shape: uint16 Poisson blob phantom (blob count scaled with
+    the imaged area and depth), nodes drawn uniformly from node_range, `pool` distinct frames
+    that the arrival stream cycles through. Returns a list of dicts (synth.make_stack's keys)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, shape in enumerate(shapes):
+        nz, ny, nx = shape
+        nD = int(rng.integers(node_range[0], node_range[1] + 1))
+        nD = max(nD, nz + 2)
+        blobs = int(600 * ny * nx / 512 ** 2 * max(1.0, nz / 4))
+        st = synth.make_stack(shape, pool, nD, seed=seed * 100 + i, n_blobs=blobs,
+                              sigma=(2.0, 6.0) if nz == 1 else (1.5, 4.0), mean=mean,
+                              dtype="uint16")
+        out.append(st)
+    return out
+
+
+def arrivals(n_acq, rate, duration, seed):
+    """Merged Poisson stream at `rate` frames/s over `duration` s, each arrival from an
+    acquisition drawn uniformly (so each acquisition streams at rate / n_acq).
+    Returns (times, acquisition index) arrays."""
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rate * duration * 1.5) + 16)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    t = t[t < duration]
+    return t, rng.integers(0, n_acq, t.size)
+
+
+class ServeLoop:
+    """The request loop. refs: one fa_amd.Reference per acquisition."""
+
+    def __init__(self, server, refs, pools, mem_budget=None, reserve=1 << 30, rounds=2):
+        self.server = server
+        self.refs = refs
+        self.pools = pools  # per acquisition: array of raw uint16 frames
+        self.ctx = refs[0].ctx
+        self.mem_budget = mem_budget
+        self.reserve = reserve
+        self.rounds = rounds
+        self.frame_bytes = [p[0].nbytes for p in pools]
+
+    def run(self, times, acq, keep=None):
+        """Streams the arrivals in real time. Returns dict with per-frame latency (s), the
+        results of the frames whose index is in `keep`, and loop counters."""
+        S = self.server
+        n = len(times)
+        lat = np.full(n, np.nan)
+        kept = {}
+        waiting = []          # arrived, not yet uploaded (index)
+        uploaded = {}         # index -> Frame
+        used = 0
+        nxt = 0
+        done = 0
+        steps = 0
+        max_flight = 0
+        deferred = 0
+        t0 = time.perf_counter()
+        while done < n:
+            now = time.perf_counter() - t0
+            while nxt < n and times[nxt] <= now:
+                waiting.append(nxt)
+                nxt += 1
+            # admission by free memory: upload while the frame fits in what is free
+            while waiting:
+                i = waiting[0]
+                a = int(acq[i])
+                need = self.frame_bytes[a]
+                if self.mem_budget is not None and used + need > self.mem_budget:
+                    deferred += 1
+                    break
+                free, _ = self.ctx.mem_info()
+                if free < need + self.reserve:
+                    deferred += 1
+                    break
+                waiting.pop(0)
+                pool = self.pools[a]
+                fr = self.refs[a].register(pool[i % len(pool)])
+                uploaded[i] = fr
+                used += need
+                S.submit(fr, tag=i)
+            if not uploaded:
+                if waiting:
+                    raise MemoryError("a frame does not fit the device memory budget on its own")
+                if nxt < n:
+                    time.sleep(max(0.0, times[nxt] - (time.perf_counter() - t0)))
+                continue
+            in_flight, queued, res = S.step(self.rounds)
+            steps += 1
+            max_flight = max(max_flight, in_flight)
+            tnow = time.perf_counter() - t0
+            for tag, x, st in res:
+                lat[tag] = tnow - times[tag]
+                if keep is not None and tag in keep:
+                    kept[tag] = (x, st)
+                fr = uploaded.pop(tag)
+                used -= self.frame_bytes[int(acq[tag])]
+                fr.release()
+                done += 1
+        wall = time.perf_counter() - t0
+        return dict(latency=lat, kept=kept, wall=wall, steps=steps, max_in_flight=max_flight,
+                    deferred=deferred)
+
+
+def summarize(res, times, acq, pools):
+    lat = res["latency"] * 1e3
+    n = len(times)
+    px = sum(pools[int(a)][0].size for a in acq)
+    return {"frames": n, "wall_s": round(res["wall"], 3),
+            "frames_per_s": round(n / res["wall"], 1),
+            "offered_frames_per_s": round(n / max(times[-1], 1e-9), 1),
+            "mpx_per_s": round(px / res["wall"] / 1e6, 1),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3),
+                           "max": round(float(lat.max()), 3)},
+            "steps": res["steps"], "max_in_flight": res["max_in_flight"],
+            "admission_deferred": res["deferred"]}

@@ -41,6 +41,7 @@ typedef struct fa_context fa_context;
 typedef struct fa_reference fa_reference;
 typedef struct fa_frame fa_frame;
 typedef struct fa_batch fa_batch;
+typedef struct fa_server fa_server;
 
 /* Raw constructor arguments of ImageScanParameters (FrameAligner/ImageScanParameters.h:7-20);
    the library applies the constructor's fix-ups and processScanParameters
@@ -187,6 +188,29 @@ int fa_batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_t
 int fa_batch_eval(fa_batch* b, const double* D, double* err, double* jac);
 int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
                    double* x_inout, fa_align_stats* stats);
+
+/* Serving (continuous batching): a fixed set of slots, each sized for the largest of the
+   references the server serves. Submitted frames wait in a queue; every fa_server_step admits
+   queued frames into free slots, runs `rounds` trust-region rounds over all occupied slots
+   (frames admitted at different times share the launches), and harvests finished slots:
+   x_inout (caller memory, valid until the result is returned) receives the result and
+   fa_server_results lists the finished tags. Each result is the same as fa_align's for that
+   frame. Use one thread per server; frames stay registered until their result is taken.
+   There is no counterpart in the reference beyond its thread pool of addFrame tasks
+   (FrameAlignerMex/FrameAlignerMex.cpp:110-130). */
+typedef struct fa_server_result {
+   int64_t tag;
+   fa_align_stats stats;
+} fa_server_result;
+
+int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs, int32_t slots,
+                     const fa_tr_params* prm, fa_server** out);
+int fa_server_destroy(fa_server* server);
+int fa_server_submit(fa_server* server, fa_frame* frame, double* x_inout, int64_t tag);
+int fa_server_step(fa_server* server, int32_t rounds, int32_t* in_flight, int32_t* queued);
+int fa_server_results(fa_server* server, fa_server_result* out, int32_t max, int32_t* n_out);
+/* Free and total device memory (hipMemGetInfo), for admission by free memory. */
+int fa_context_mem_info(fa_context* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* The HIP stream the context launches on (hipStream_t), for callers that time it. */
 void* fa_context_stream(fa_context* ctx);

@@ -1,0 +1,113 @@
+"""Continuous batching: a frame's result is the same alone and when it joins a server's slots
+while other frames (other references, 2-D and 3-D, uint16 and float) are in flight."""
+import numpy as np
+import pytest
+
+import oracle as O
+import fa_amd as F
+from fa_amd import serve
+from helpers import small_stack
+
+pytestmark = pytest.mark.gpu
+
+D_TOL = 0.02
+KW = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = F.Context(0)
+    yield c
+    c.close()
+
+
+def _setup(ctx):
+    a = small_stack((1, 96, 80), n_frames=5, nD=9, seed=41)
+    b = small_stack((1, 64, 72), n_frames=4, nD=13, seed=42, dtype="uint16")
+    c = small_stack((6, 40, 36), n_frames=3, nD=19, seed=43, n_blobs=120, sigma=(1.5, 4.0))
+    ra = F.Reference(ctx, a["ref_w"], 9)
+    rb = F.Reference(ctx, b["ref_w"], 13, interline=1.3)
+    rc = F.Reference(ctx, c["ref_w"], 19)
+    frames = ([ra.register(f) for f in a["frames_w"]] + [rb.register(f) for f in b["frames"]] +
+              [rc.register(f) for f in c["frames_w"]])
+    return (a, b, c), (ra, rb, rc), frames
+
+
+def test_server_same_alone_and_in_mix(ctx):
+    stacks, refs, frames = _setup(ctx)
+    alone = [f.align(**KW) for f in frames]
+    srv = F.Server(refs, slots=4, **KW)  # fewer slots than frames: some wait in the queue
+    order = [11, 0, 5, 9, 1, 6, 10, 2, 7, 3, 8, 4]
+    got = {}
+    k = 0
+    steps = 0
+    while len(got) < len(frames):
+        # frames arrive two at a time while others are in flight
+        for _ in range(2):
+            if k < len(order):
+                i = order[k]
+                srv.submit(frames[i], tag=i)
+                k += 1
+        _, _, res = srv.step(rounds=1 + steps % 3)
+        steps += 1
+        for tag, x, st in res:
+            got[tag] = (x, st)
+        assert steps < 10000
+    for i, (x1, s1) in enumerate(alone):
+        x2, s2 = got[i]
+        assert np.array_equal(x1, x2), i
+        assert (s1["evaluations"], s1["iterations"], s1["status"]) == \
+            (s2["evaluations"], s2["iterations"], s2["status"]), i
+        assert s1["f_final"] == s2["f_final"]
+    # the 2-D float frames against the oracle
+    orc = O.Oracle().set_reference(stacks[0]["ref_w"], 9)
+    for i in range(2):
+        x_o, _ = orc.align(stacks[0]["frames_w"][i], **KW)
+        assert np.max(np.abs(got[i][0] - x_o)) <= D_TOL
+    # a started run is independent of later submissions: run everything again, one at a time
+    for i in (3, 10):
+        srv.submit(frames[i], tag=100 + i)
+        res = []
+        while not res:
+            _, _, res = srv.step(rounds=2)
+        assert np.array_equal(res[0][1], alone[i][0])
+    srv.close()
+
+
+def test_server_errors(ctx):
+    st = small_stack((1, 64, 64), n_frames=1, nD=9, seed=44)
+    r1 = F.Reference(ctx, st["ref_w"], 9)
+    r2 = F.Reference(ctx, st["ref_w"], 9)
+    srv = F.Server([r1], slots=2)
+    fr = r2.register(st["frames_w"][0])
+    with pytest.raises(F.FaError):
+        srv.submit(fr, tag=1)
+    fr.release()
+    with pytest.raises(F.FaError):
+        r1._release_handle()  # the server still serves it
+    srv.close()
+    r1._release_handle()
+    r2._release_handle()
+
+
+def test_serve_loop_small(ctx):
+    acqs = serve.make_acquisitions(seed=5, shapes=[(1, 64, 64), (1, 96, 80), (4, 40, 36)], pool=2,
+                                   node_range=(8, 12))
+    refs = [F.Reference(ctx, serve.synth.with_offset(a["reference"]), a["nD"]) for a in acqs]
+    srv = F.Server(refs, slots=8, **KW)
+    pools = [a["frames"] for a in acqs]
+    times, acq = serve.arrivals(len(acqs), rate=400.0, duration=0.1, seed=6)
+    loop = serve.ServeLoop(srv, refs, pools, mem_budget=4 * pools[0][0].nbytes, rounds=2)
+    keep = set(range(len(times)))
+    res = loop.run(times, acq, keep=keep)
+    assert np.all(np.isfinite(res["latency"])) and np.all(res["latency"] >= 0)
+    # each served result is the same as aligning that frame alone
+    for i in range(len(times)):
+        a = int(acq[i])
+        fr = refs[a].register(pools[a][i % len(pools[a])])
+        x, st = fr.align(**KW)
+        assert np.array_equal(x, res["kept"][i][0]), i
+        fr.release()
+    s = serve.summarize(res, times, acq, pools)
+    assert s["frames"] == len(times)
+    srv.close()
