@@ -77,6 +77,60 @@ def isolated_pass(F, frames, tr, reps=10):
     return ms / k, nbytes / k
 
 
+def serving(F, args, ctx):
+    """Serving form: 8 acquisitions (256^2 x 1 .. 1024^2 x 8, 10-40 nodes, uint16 blob
+    phantoms) streaming frames with exponential inter-arrival times at args.serve_rate
+    frames/s for args.serve_seconds, aligned by the continuous-batching server. Latency is
+    arrival -> result. A sample of served results is checked against the same frame aligned
+    alone (bit-identical) and, for the 256^2 2-D acquisition, against the CPU oracle."""
+    from fa_amd import serve
+    t0 = time.time()
+    acqs = serve.make_acquisitions(seed=args.seed, pool=2)
+    log(f"serving: generated {len(acqs)} acquisitions in {time.time() - t0:.1f}s")
+    refs = [F.Reference(ctx, serve.synth.with_offset(a["reference"]), a["nD"]) for a in acqs]
+    pools = [a["frames"] for a in acqs]
+    tr = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    srv = F.Server(refs, slots=args.serve_slots, **tr)
+    loop = serve.ServeLoop(srv, refs, pools, rounds=args.serve_rounds)
+    times, acq = serve.arrivals(len(acqs), args.serve_rate, args.serve_seconds, seed=args.seed + 1)
+    loop.run(times[:200], acq[:200])  # warm-up
+    keep = set(range(0, len(times), max(1, len(times) // 64)))
+    res = loop.run(times, acq, keep=keep)
+    out = serve.summarize(res, times, acq, pools)
+    out["workload"] = ("8 acquisitions " + ", ".join("x".join(map(str, a["shape"][::-1])) + f"/{a['nD']}n"
+                                                     for a in acqs) + f", Poisson arrivals {args.serve_rate:g}/s")
+    out["slots"] = args.serve_slots
+    out["rounds_per_step"] = args.serve_rounds
+    # served == alone, on a sample
+    same = True
+    for i in sorted(keep):
+        a = int(acq[i])
+        fr = refs[a].register(pools[a][i % len(pools[a])])
+        x, _ = fr.align(**tr)
+        same = same and bool(np.array_equal(x, res["kept"][i][0]))
+        fr.release()
+    out["same_as_alone"] = {"frames": len(keep), "all_identical": same}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        a0 = 0  # the 256^2 2-D acquisition
+        orc = O.Oracle().set_reference(serve.synth.with_offset(acqs[a0]["reference"]), acqs[a0]["nD"])
+        err = 0.0
+        n = 0
+        for i in sorted(keep):
+            if int(acq[i]) != a0:
+                continue
+            x_o, _ = orc.align(serve.synth.with_offset(pools[a0][i % len(pools[a0])]), **tr)
+            err = max(err, float(np.max(np.abs(res["kept"][i][0] - x_o))))
+            n += 1
+        out["max_disp_err_px_vs_oracle"] = {"frames": n, "acquisition": "256x256x1", "max": err}
+    loop.close()
+    srv.close()
+    for r in refs:
+        r.release()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +143,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-slots", type=int, default=128)
+    ap.add_argument("--serve-rounds", type=int, default=2)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied device-to-device once (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -191,6 +250,11 @@ def main():
     if args.copy_probe:
         out["copy_probe_mib"] = copy_probe(args.copy_probe)
 
+    if not args.no_serve:
+        out["serving"] = serving(F, args, ctx)
+        log(f"serving: {out['serving']['frames_per_s']} frames/s, p50 {out['serving']['latency_ms']['p50']} ms, "
+            f"p99 {out['serving']['latency_ms']['p99']} ms")
+
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -208,6 +272,8 @@ def main():
                                "latency_ms": {"p50": round(1e3 * float(np.percentile(secs, 50)), 1),
                                               "p99": round(1e3 * float(np.percentile(secs, 99)), 1)}}
         out["max_disp_err_px_vs_oracle"] = err
+        out["cpu_baseline"]["evaluations_per_frame"] = {"mean": float(np.mean([d["evaluations"] for d in s_o])),
+                                                        "max": int(np.max([d["evaluations"] for d in s_o]))}
     print(json.dumps(out), flush=True)
 
 

@@ -62,7 +62,7 @@ void dfree(T*& p)
 
 constexpr int kTrLdsMaxDoubles = 7680;  // 60 KB for the trust-region update
 constexpr int kLdsCapFloats = 15360;  // 60 KB of staged frame rows per workgroup at most
-constexpr int kBlocksPerCu = 5;
+constexpr int kBlocksPerCu = 8;
 
 // ImageScanParameters ctor (ImageScanParameters.h:7-20) then processScanParameters
 // (FrameWarper.cpp:223-236).
@@ -130,6 +130,7 @@ struct fa_batch {
    int tr_lds = 0;
    std::vector<int> user;  // internal index -> caller's index
    int n_slots = 0;        // serving batches: number of slots (frames are admitted into them)
+   int64_t lds_need[2] = {0, 0};  // LDS floats the 2-D / 3-D frames' work items stage
    RefDev* d_refs = nullptr;
    BatchFrame* d_frames = nullptr;
    TrState* d_state = nullptr;
@@ -139,7 +140,10 @@ struct fa_batch {
    double* d_D = nullptr;
    double* d_partials = nullptr;
    int32_t* d_active_frames = nullptr;
-   int64_t* d_active_base = nullptr;
+   int32_t* d_run_first = nullptr;
+   int32_t* d_run_count = nullptr;
+   int64_t* d_run_base = nullptr;
+   int n_refs = 0;
    Counters* d_counters = nullptr;
    Counters* h_counters = nullptr;  // pinned
    std::vector<hipEvent_t> ev;      // start/end pairs
@@ -159,7 +163,10 @@ struct fa_batch {
       B.Dtrial = d_D;
       B.partials = d_partials;
       B.active_frames = d_active_frames;
-      B.active_base = d_active_base;
+      B.run_first = d_run_first;
+      B.run_count = d_run_count;
+      B.run_base = d_run_base;
+      B.n_refs = n_refs;
       B.counters = d_counters;
       B.prm.radius = prm.radius;
       B.prm.min_delta = prm.min_delta;
@@ -182,7 +189,10 @@ struct fa_batch {
       a.Dtrial = d_D;
       a.partials = d_partials;
       a.active_frames = d_active_frames;
-      a.active_base = d_active_base;
+      a.run_first = d_run_first;
+      a.run_count = d_run_count;
+      a.run_base = d_run_base;
+      a.n_refs = n_refs;
       a.counters = d_counters;
       a.lds_cap = tune.lds_floats;
       a.use_lds = tune.use_lds;
@@ -210,7 +220,9 @@ struct fa_batch {
       dfree(d_D);
       dfree(d_partials);
       dfree(d_active_frames);
-      dfree(d_active_base);
+      dfree(d_run_first);
+      dfree(d_run_count);
+      dfree(d_run_base);
       dfree(d_counters);
       dfree(d_warp);
       dfree(d_cov);
@@ -221,17 +233,22 @@ struct fa_batch {
 
 namespace {
 
-// Default LDS budget: the rows one work item spans plus room for the rows (and, in 3-D, the
-// planes) its displacement range reaches, for the widest frame of the batch.
+// LDS a work item of reference r stages: its rows (one more when it starts mid-row) plus the
+// row below for the taps and one for a displacement range that crosses a row boundary; in
+// 3-D two planes (work items whose z range crosses a plane boundary need three and sample
+// from global memory instead).
+int64_t lds_floats_for(const fa_reference* r)
+{
+   const int64_t stride = ((r->nx + 2 * kPad) + 3) & ~3;
+   const int64_t rows = (r->chunk_px + r->nx - 1) / r->nx + 3;
+   return rows * stride * (r->n_dim == 3 ? 2 : 1);
+}
+
+// Default LDS budget: the largest over the batch's frames, capped.
 int default_lds_floats(const std::vector<fa_frame*>& frames)
 {
    int64_t best = 0;
-   for (fa_frame* f : frames) {
-      const fa_reference* r = f->ref;
-      const int64_t stride = ((r->nx + 2 * kPad) + 3) & ~3;
-      const int64_t rows = (r->chunk_px + r->nx - 1) / r->nx + 6;
-      best = std::max(best, rows * stride * (r->n_dim == 3 ? 3 : 1));
-   }
+   for (fa_frame* f : frames) best = std::max(best, lds_floats_for(f->ref));
    return (int)std::min<int64_t>(best, kLdsCapFloats);
 }
 
@@ -313,6 +330,8 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       bf.d_base = d_off[i];
       b->total_work += (int64_t)r->chunks.size();
       (r->n_dim == 2 ? b->work2 : b->work3) += (int64_t)r->chunks.size();
+      int64_t& lds = b->lds_need[r->n_dim == 3];
+      lds = std::max(lds, lds_floats_for(r));
       b->total_vec += 4 * (int64_t)bf.n;
       b->total_fac += 2 * (int64_t)r->nD * r->n_dim * r->n_dim;
       const int need = 5 * bf.n + 4 * r->nD * r->n_dim * r->n_dim;
@@ -320,6 +339,11 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
    }
    // the update works from LDS when every frame's vectors and JtJ blocks fit
    b->tr_lds = tr_need <= kTrLdsMaxDoubles ? tr_need : 0;
+   b->n_refs = (int)refs.size();
+   if (b->n_refs > kMaxRefs) {
+      delete b;
+      return set_err(FA_E_ARG, "too many distinct references in one batch (at most 4096)");
+   }
    int rc = FA_OK;
    auto fail = [&](int code) {
       delete b;
@@ -330,7 +354,8 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
        (rc = dalloc(&b->d_vec, b->total_vec)) || (rc = dalloc(&b->d_fac, b->total_fac)) ||
        (rc = dalloc(&b->d_D, b->total_D)) ||
        (rc = dalloc(&b->d_partials, (size_t)b->total_work * kPassStride)) ||
-       (rc = dalloc(&b->d_active_frames, n)) || (rc = dalloc(&b->d_active_base, n + 1)) ||
+       (rc = dalloc(&b->d_active_frames, n)) || (rc = dalloc(&b->d_run_first, refs.size())) ||
+       (rc = dalloc(&b->d_run_count, refs.size())) || (rc = dalloc(&b->d_run_base, refs.size() + 1)) ||
        (rc = dalloc(&b->d_counters, 1)))
       return fail(rc);
    if (hipHostMalloc((void**)&b->h_counters, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
@@ -360,11 +385,19 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
 }
 
 // Enqueues the fused pass over the active frames of every kind.
+// Each kind's pass reserves the LDS its own work items stage (within the tuned cap), so
+// 2-D frames keep their occupancy beside 3-D frames.
 int launch_passes(fa_batch* b, hipStream_t s)
 {
-   const PassArgs pa = b->pass_args();
-   if (b->grid2) HIPCHK(launch_pass(pa, 2, b->grid2, s));
-   if (b->grid3) HIPCHK(launch_pass(pa, 3, b->grid3, s));
+   PassArgs pa = b->pass_args();
+   if (b->grid2) {
+      pa.lds_cap = (int)std::min<int64_t>(b->tune.lds_floats, b->lds_need[0]);
+      HIPCHK(launch_pass(pa, 2, b->grid2, s));
+   }
+   if (b->grid3) {
+      pa.lds_cap = (int)std::min<int64_t>(b->tune.lds_floats, b->lds_need[1]);
+      HIPCHK(launch_pass(pa, 3, b->grid3, s));
+   }
    return FA_OK;
 }
 
@@ -650,11 +683,17 @@ int single_batch(fa_frame* f, fa_batch** out)
 // ---- serving: continuous batching over fixed slots ----------------------------------------
 
 struct fa_server {
+   // A request: a registered frame (f), or host data uploaded into the server's frame arena.
    struct Req {
-      fa_frame* f;
-      double* x;
-      int64_t tag;
+      fa_frame* f = nullptr;
+      fa_reference* ref = nullptr;
+      const void* host = nullptr;
+      int32_t dtype = FA_F32;
+      float offset = 0.f;
+      double* x = nullptr;
+      int64_t tag = 0;
    };
+   enum : char { kFree = 0, kUploading = 1, kRunning = 2 };
    fa_context* ctx = nullptr;
    fa_batch* b = nullptr;  // slot batch: n_slots frames' worth of state, sized for the largest reference
    std::vector<fa_reference*> refs;
@@ -662,18 +701,65 @@ struct fa_server {
    std::deque<Req> pending;
    std::vector<int> free_slots;
    std::vector<Req> slot_req;
-   std::vector<char> busy;
+   std::vector<char> sstate;
+   std::vector<size_t> blk_off, blk_size;  // arena block of each slot (size 0: none)
+   std::vector<hipEvent_t> up_ev;          // upload of each slot's frame done
    std::deque<fa_server_result> results;
-   BatchFrame* h_bf = nullptr;  // pinned staging
+   // frame arena: first-fit over free blocks (offset -> size), neighbours merged on release
+   uint8_t* d_arena = nullptr;
+   size_t arena_bytes = 0, arena_used = 0;
+   std::map<size_t, size_t> free_blocks;
+   hipStream_t cstream = nullptr;  // uploads (default priority)
+   BatchFrame* h_bf = nullptr;     // pinned staging
    double* h_x = nullptr;
    int32_t* h_slots = nullptr;
    TrState* h_state = nullptr;
    double* h_xall = nullptr;
    int32_t* d_slots = nullptr;
 
+   static size_t round_up(size_t n) { return (n + 255) & ~(size_t)255; }
+
+   bool arena_alloc(size_t n, size_t* off)
+   {
+      n = round_up(n);
+      for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+         if (it->second < n) continue;
+         *off = it->first;
+         const size_t rest = it->second - n;
+         free_blocks.erase(it);
+         if (rest) free_blocks[*off + n] = rest;
+         arena_used += n;
+         return true;
+      }
+      return false;
+   }
+
+   void arena_free(size_t off, size_t n)
+   {
+      n = round_up(n);
+      arena_used -= n;
+      auto it = free_blocks.emplace(off, n).first;
+      auto nx = std::next(it);
+      if (nx != free_blocks.end() && it->first + it->second == nx->first) {
+         it->second += nx->second;
+         free_blocks.erase(nx);
+      }
+      if (it != free_blocks.begin()) {
+         auto pv = std::prev(it);
+         if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_blocks.erase(it);
+         }
+      }
+   }
+
    ~fa_server()
    {
+      if (cstream) (void)hipStreamSynchronize(cstream);
       delete b;
+      for (auto e : up_ev) (void)hipEventDestroy(e);
+      if (cstream) (void)hipStreamDestroy(cstream);
+      dfree(d_arena);
       if (h_bf) (void)hipHostFree(h_bf);
       if (h_x) (void)hipHostFree(h_x);
       if (h_slots) (void)hipHostFree(h_slots);
@@ -696,6 +782,8 @@ int halloc(T** p, size_t count)
    return FA_OK;
 }
 
+size_t frame_bytes(const fa_reference* r, int dtype) { return (size_t)r->n_px * (dtype == FA_U16 ? 2 : 4); }
+
 // A batch whose frames are slots: every slot is sized for the largest of the references and
 // starts finished (out of the schedule) until a request is admitted into it.
 int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, int S,
@@ -709,8 +797,13 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
    int nmax = 0, ndmax = 0, fmax = 0, tr_need = 0;
    int64_t cmax = 0;
    bool has2 = false, has3 = false;
+   if ((int)refs.size() > kMaxRefs) {
+      delete b;
+      return set_err(FA_E_ARG, "too many distinct references in one server (at most 4096)");
+   }
    std::vector<RefDev> rdev;
    int64_t widest = 0;
+   b->n_refs = (int)refs.size();
    for (fa_reference* r : refs) {
       rdev.push_back(r->dev);
       nmax = std::max(nmax, r->nD * r->n_dim);
@@ -719,9 +812,8 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
       cmax = std::max<int64_t>(cmax, (int64_t)r->chunks.size());
       (r->n_dim == 2 ? has2 : has3) = true;
       tr_need = std::max(tr_need, 5 * r->nD * r->n_dim + 4 * r->nD * r->n_dim * r->n_dim);
-      const int64_t stride = ((r->nx + 2 * kPad) + 3) & ~3;
-      const int64_t rows = (r->chunk_px + r->nx - 1) / r->nx + 6;
-      widest = std::max(widest, rows * stride * (r->n_dim == 3 ? 3 : 1));
+      widest = std::max(widest, lds_floats_for(r));
+      b->lds_need[r->n_dim == 3] = std::max(b->lds_need[r->n_dim == 3], lds_floats_for(r));
    }
    b->tune.lds_floats = (int)std::min<int64_t>(widest, kLdsCapFloats);
    b->tune.blocks_per_cu = kBlocksPerCu;
@@ -757,7 +849,8 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
        (rc = dalloc(&b->d_vec, b->total_vec)) || (rc = dalloc(&b->d_fac, b->total_fac)) ||
        (rc = dalloc(&b->d_D, b->total_D)) ||
        (rc = dalloc(&b->d_partials, (size_t)b->total_work * kPassStride)) ||
-       (rc = dalloc(&b->d_active_frames, S)) || (rc = dalloc(&b->d_active_base, S + 1)) ||
+       (rc = dalloc(&b->d_active_frames, S)) || (rc = dalloc(&b->d_run_first, rdev.size())) ||
+       (rc = dalloc(&b->d_run_count, rdev.size())) || (rc = dalloc(&b->d_run_base, rdev.size() + 1)) ||
        (rc = dalloc(&b->d_counters, 1)) || (rc = halloc(&b->h_counters, 1)))
       return fail(rc);
    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -788,25 +881,58 @@ int server_step(fa_server* sv, int rounds)
    fa_batch* b = sv->b;
    hipStream_t s = b->stream;
    const BatchDev B = b->dev();
-   // admit queued requests into free slots
-   int na = 0;
+   // 1. take queued requests, in arrival order, while a slot and (for host data) arena memory
+   //    are free; host data is uploaded on the copy stream while the running frames compute
+   bool uploading = false;
    while (!sv->free_slots.empty() && !sv->pending.empty()) {
-      fa_server::Req r = sv->pending.front();
-      sv->pending.pop_front();
+      fa_server::Req& r = sv->pending.front();
       const int slot = sv->free_slots.back();
+      if (r.host) {
+         const size_t nb = frame_bytes(r.ref, r.dtype);
+         size_t off;
+         if (!sv->arena_alloc(nb, &off)) break;  // admission by free memory
+         sv->blk_off[slot] = off;
+         sv->blk_size[slot] = nb;
+         HIPCHK(hipMemcpyAsync(sv->d_arena + off, r.host, nb, hipMemcpyHostToDevice, sv->cstream));
+         HIPCHK(hipEventRecord(sv->up_ev[slot], sv->cstream));
+      } else {
+         sv->blk_size[slot] = 0;
+         HIPCHK(hipEventRecord(sv->up_ev[slot], sv->cstream));
+      }
       sv->free_slots.pop_back();
-      int ri = (int)(std::find(sv->refs.begin(), sv->refs.end(), r.f->ref) - sv->refs.begin());
+      sv->slot_req[slot] = r;
+      sv->sstate[slot] = fa_server::kUploading;
+      sv->pending.pop_front();
+      uploading = true;
+   }
+   bool running = false;
+   for (int j = 0; j < sv->S; j++) running = running || sv->sstate[j] == fa_server::kRunning;
+   if (uploading && !running) HIPCHK(hipStreamSynchronize(sv->cstream));  // nothing to overlap with
+   // 2. start the frames whose data is on the device
+   int na = 0;
+   for (int slot = 0; slot < sv->S; slot++) {
+      if (sv->sstate[slot] != fa_server::kUploading) continue;
+      const hipError_t q = hipEventQuery(sv->up_ev[slot]);
+      if (q == hipErrorNotReady) continue;
+      HIPCHK(q);
+      const fa_server::Req& r = sv->slot_req[slot];
+      fa_reference* ref = r.f ? r.f->ref : r.ref;
       BatchFrame bf = b->hframes[slot];
-      bf.f.data = r.f->d_data;
-      bf.f.dtype = r.f->dtype;
-      bf.f.offset = r.f->offset;
-      bf.ref = ri;
-      bf.n = r.f->ref->nD * r.f->ref->n_dim;
+      if (r.f) {
+         bf.f.data = r.f->d_data;
+         bf.f.dtype = r.f->dtype;
+         bf.f.offset = r.f->offset;
+      } else {
+         bf.f.data = sv->d_arena + sv->blk_off[slot];
+         bf.f.dtype = r.dtype;
+         bf.f.offset = r.offset;
+      }
+      bf.ref = (int)(std::find(sv->refs.begin(), sv->refs.end(), ref) - sv->refs.begin());
+      bf.n = ref->nD * ref->n_dim;
       sv->h_bf[na] = bf;
       std::memcpy(sv->h_x + (size_t)na * sv->nmax, r.x, bf.n * sizeof(double));
       sv->h_slots[na] = slot;
-      sv->slot_req[slot] = r;
-      sv->busy[slot] = 1;
+      sv->sstate[slot] = fa_server::kRunning;
       na++;
    }
    for (int i = 0; i < na; i++) {
@@ -819,19 +945,19 @@ int server_step(fa_server* sv, int rounds)
       HIPCHK(hipMemcpyAsync(sv->d_slots, sv->h_slots, na * sizeof(int32_t), hipMemcpyHostToDevice, s));
       HIPCHK(launch_tr_init_slots(B, sv->d_slots, na, s));
    }
-   bool any = false;
-   for (int i = 0; i < sv->S; i++) any = any || sv->busy[i];
-   if (!any) return FA_OK;
+   running = running || na > 0;
+   if (!running) return FA_OK;
+   // 3. rounds over every running frame, then collect the finished ones
    for (int r = 0; r < rounds; r++) CHK(run_round(b));
    CHK(run_poll(b));
    HIPCHK(hipMemcpyAsync(sv->h_state, b->d_state, sv->S * sizeof(TrState), hipMemcpyDeviceToHost, s));
    HIPCHK(hipMemcpyAsync(sv->h_xall, b->d_x, b->total_x * sizeof(double), hipMemcpyDeviceToHost, s));
    HIPCHK(hipStreamSynchronize(s));
-   // harvest finished slots
    for (int j = 0; j < sv->S; j++) {
-      if (!sv->busy[j] || !sv->h_state[j].done) continue;
+      if (sv->sstate[j] != fa_server::kRunning || !sv->h_state[j].done) continue;
       const fa_server::Req& r = sv->slot_req[j];
-      const int n = r.f->ref->nD * r.f->ref->n_dim;
+      const fa_reference* ref = r.f ? r.f->ref : r.ref;
+      const int n = ref->nD * ref->n_dim;
       std::memcpy(r.x, sv->h_xall + b->hframes[j].x_base, n * sizeof(double));
       fa_server_result res{};
       res.tag = r.tag;
@@ -843,7 +969,9 @@ int server_step(fa_server* sv, int rounds)
       res.stats.f_final = t.f;
       res.stats.radius_final = t.radius;
       sv->results.push_back(res);
-      sv->busy[j] = 0;
+      if (sv->blk_size[j]) sv->arena_free(sv->blk_off[j], sv->blk_size[j]);
+      sv->blk_size[j] = 0;
+      sv->sstate[j] = fa_server::kFree;
       sv->free_slots.push_back(j);
    }
    return FA_OK;
@@ -931,7 +1059,11 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    r->n_dim = nz > 1 ? 3 : 2;
    r->n_px = n_px;
    // default work item: about 8 frame rows, 2048..8192 pixels
-   r->chunk_px = chunk_px > 0 ? chunk_px : (int)std::min<int64_t>(8192, std::max<int64_t>(2048, 8 * (int64_t)nx));
+   // default work item: 8 rows (2-D), 2..8 rows of about 4096 pixels (3-D, which stages two
+   // planes), within 2048..8192 pixels
+   const int64_t item_rows = nz > 1 ? std::min<int64_t>(8, std::max<int64_t>(2, 4096 / nx)) : 8;
+   r->chunk_px = chunk_px > 0 ? chunk_px
+                              : (int)std::min<int64_t>(8192, std::max<int64_t>(2048, item_rows * (int64_t)nx));
    derive_scan(*scan, nz, ny, nx, nD, r->scan_host, &r->scan);
    const int m = r->n_dim;
 
@@ -1404,7 +1536,7 @@ int fa_context_mem_info(fa_context* ctx, uint64_t* free_bytes, uint64_t* total_b
 }
 
 int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs, int32_t slots,
-                     const fa_tr_params* prm, fa_server** out)
+                     uint64_t arena_bytes, const fa_tr_params* prm, fa_server** out)
 {
    if (!ctx || !refs || n_refs <= 0 || slots <= 0 || !out) return set_err(FA_E_ARG, "invalid server arguments");
    *out = nullptr;
@@ -1414,22 +1546,49 @@ int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs,
    DeviceGuard g(ctx->device);
    auto* sv = new fa_server();
    sv->ctx = ctx;
-   for (int i = 0; i < n_refs; i++)
-      if (std::find(sv->refs.begin(), sv->refs.end(), refs[i]) == sv->refs.end()) sv->refs.push_back(refs[i]);
+   size_t largest = 0;
+   // references of 2-D frames first: the schedule orders the pass's work by reference index
+   for (int kind = 2; kind <= 3; kind++)
+      for (int i = 0; i < n_refs; i++)
+         if (refs[i]->n_dim == kind &&
+             std::find(sv->refs.begin(), sv->refs.end(), refs[i]) == sv->refs.end())
+            sv->refs.push_back(refs[i]);
+   for (fa_reference* r : sv->refs) {
+      sv->nmax = std::max(sv->nmax, r->nD * r->n_dim);
+      largest = std::max(largest, fa_server::round_up(frame_bytes(r, FA_F32)));
+   }
    sv->S = slots;
-   for (fa_reference* r : sv->refs) sv->nmax = std::max(sv->nmax, r->nD * r->n_dim);
+   sv->arena_bytes = arena_bytes ? fa_server::round_up(arena_bytes) : largest * (size_t)slots;
    int rc = slot_batch_build(ctx, sv->refs, slots, prm, &sv->b);
    if (rc == FA_OK)
       if ((rc = halloc(&sv->h_bf, slots)) || (rc = halloc(&sv->h_x, (size_t)slots * sv->nmax)) ||
           (rc = halloc(&sv->h_slots, slots)) || (rc = halloc(&sv->h_state, slots)) ||
-          (rc = halloc(&sv->h_xall, (size_t)slots * sv->nmax)) || (rc = dalloc(&sv->d_slots, slots))) {
+          (rc = halloc(&sv->h_xall, (size_t)slots * sv->nmax)) || (rc = dalloc(&sv->d_slots, slots)) ||
+          (rc = dalloc(&sv->d_arena, sv->arena_bytes))) {
       }
+   if (rc == FA_OK && hipStreamCreateWithFlags(&sv->cstream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      sv->cstream = nullptr;
+      rc = set_err(FA_E_DEVICE, "stream creation failed");
+   }
+   for (int j = 0; rc == FA_OK && j < slots; j++) {
+      hipEvent_t e;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+         (void)hipGetLastError();
+         rc = set_err(FA_E_DEVICE, "event creation failed");
+         break;
+      }
+      sv->up_ev.push_back(e);
+   }
    if (rc != FA_OK) {
       delete sv;
       return rc;
    }
+   sv->free_blocks[0] = sv->arena_bytes;
    sv->slot_req.resize(slots);
-   sv->busy.assign(slots, 0);
+   sv->sstate.assign(slots, fa_server::kFree);
+   sv->blk_off.assign(slots, 0);
+   sv->blk_size.assign(slots, 0);
    for (int j = slots - 1; j >= 0; j--) sv->free_slots.push_back(j);
    for (fa_reference* r : sv->refs) r->live_frames++;  // the references outlive the server
    *out = sv;
@@ -1452,7 +1611,32 @@ int fa_server_submit(fa_server* sv, fa_frame* frame, double* x_inout, int64_t ta
    if (std::find(sv->refs.begin(), sv->refs.end(), frame->ref) == sv->refs.end())
       return set_err(FA_E_ARG, "frame of a reference the server does not serve");
    std::lock_guard<std::mutex> lk(sv->ctx->mu);
-   sv->pending.push_back(fa_server::Req{frame, x_inout, tag});
+   fa_server::Req r;
+   r.f = frame;
+   r.x = x_inout;
+   r.tag = tag;
+   sv->pending.push_back(r);
+   return FA_OK;
+}
+
+int fa_server_submit_host(fa_server* sv, fa_reference* ref, const void* data, int32_t dtype,
+                          float add_offset, double* x_inout, int64_t tag)
+{
+   if (!sv || !ref || !data || !x_inout) return set_err(FA_E_ARG, "null argument");
+   if (dtype != FA_F32 && dtype != FA_U16) return set_err(FA_E_ARG, "unsupported frame type");
+   if (std::find(sv->refs.begin(), sv->refs.end(), ref) == sv->refs.end())
+      return set_err(FA_E_ARG, "frame of a reference the server does not serve");
+   if (fa_server::round_up(frame_bytes(ref, dtype)) > sv->arena_bytes)
+      return set_err(FA_E_OOM, "frame larger than the server's frame memory");
+   std::lock_guard<std::mutex> lk(sv->ctx->mu);
+   fa_server::Req r;
+   r.ref = ref;
+   r.host = data;
+   r.dtype = dtype;
+   r.offset = add_offset;
+   r.x = x_inout;
+   r.tag = tag;
+   sv->pending.push_back(r);
    return FA_OK;
 }
 
@@ -1464,10 +1648,36 @@ int fa_server_step(fa_server* sv, int32_t rounds, int32_t* in_flight, int32_t* q
    CHK(server_step(sv, rounds));
    if (in_flight) {
       int n = 0;
-      for (char c : sv->busy) n += c;
+      for (char c : sv->sstate) n += c != fa_server::kFree;
       *in_flight = n;
    }
    if (queued) *queued = (int32_t)sv->pending.size();
+   return FA_OK;
+}
+
+int fa_server_memory(const fa_server* sv, uint64_t* used, uint64_t* total)
+{
+   if (!sv || !used || !total) return set_err(FA_E_ARG, "null argument");
+   *used = sv->arena_used;
+   *total = sv->arena_bytes;
+   return FA_OK;
+}
+
+int fa_host_alloc(uint64_t bytes, void** out)
+{
+   if (!out) return set_err(FA_E_ARG, "null argument");
+   *out = nullptr;
+   if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      *out = nullptr;
+      return set_err(FA_E_OOM, "Not enough memory to allocate");
+   }
+   return FA_OK;
+}
+
+int fa_host_free(void* p)
+{
+   if (p) HIPCHK(hipHostFree(p));
    return FA_OK;
 }
 

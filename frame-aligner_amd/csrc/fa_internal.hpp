@@ -39,6 +39,7 @@ constexpr int kBlock = 256;  // threads per fused-pass workgroup (4 wave64)
 // Zero columns staged on each side of a frame row in LDS: taps need no bounds checks while
 // |dx| stays below kPad - 2 over a work item.
 constexpr int kPad = 16;
+constexpr int kMaxRefs = 4096;  // distinct references in one batch
 
 // Per-pixel reference record read by every pass. 2-D: (R, gx, gy, Df) in one float4;
 // 3-D: (R, gx, gy, gz) plus Df in a separate float array. Df is the node fraction of
@@ -93,7 +94,7 @@ struct Counters {
    int32_t n_active_frames;
    int32_t pad;
    int64_t n_active_work;
-   int64_t n_active_work2;    // work items of active 2-D frames (they come first)
+   int64_t n_active_work2;    // work items of active 2-D frames (their references come first)
    unsigned long long bytes;  // algorithmic bytes scheduled so far
    unsigned long long rounds;
 };
@@ -103,8 +104,14 @@ struct PassArgs {
    const BatchFrame* frames;
    const double* Dtrial;
    double* partials;
+   // Active frames grouped by reference: reference r's active frames are
+   // active_frames[run_first[r] .. + run_count[r]) and its work items
+   // [run_base[r], run_base[r + 1]), chunk-major (item = chunk * run_count + frame).
    const int32_t* active_frames;
-   const int64_t* active_base;
+   const int32_t* run_first;
+   const int32_t* run_count;
+   const int64_t* run_base;
+   int32_t n_refs;
    const Counters* counters;
    int32_t lds_cap;   // floats of dynamic LDS available for staging
    int32_t use_lds;
@@ -120,10 +127,13 @@ struct BatchDev {
    double* Dtrial;
    const double* partials;
    int32_t* active_frames;
-   int64_t* active_base;
+   int32_t* run_first;
+   int32_t* run_count;
+   int64_t* run_base;
    Counters* counters;
    TrParamsDev prm;
    int32_t n_frames;
+   int32_t n_refs;    // references, those of 2-D frames first
    int32_t tr_lds;    // doubles of dynamic LDS for the trust-region update (0: global memory)
 };
 

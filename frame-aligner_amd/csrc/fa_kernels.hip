@@ -599,10 +599,14 @@ __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, 
 // For every pixel of a work item: displacement d = f D[k+1] + (1-f) D[k] (warpPoint,
 // FrameWarper.cpp:397-419), zero-padded bilinear/trilinear pull-sample of the frame at p + d
 // (warpImageInterpolated, FrameWarper.cpp:31-69), residual against the reference and SSD over
-// samples that exist (computeErrorImage, FrameWarper.cpp:4-28), and the two Jacobian rows
-// (computeJacobian, FrameWarper.cpp:147-167). The sample position is formed in FP64 as the
-// reference does, so the tap indices and the "no sample" decision match it; the interpolation
-// and the sums are FP32 with compensated accumulation.
+// samples that exist (computeErrorImage, FrameWarper.cpp:4-28), and the Jacobian rows
+// (computeJacobian, FrameWarper.cpp:147-167).
+//
+// Everything per pixel is FP32. d is formed relative to the pixel as D[k] + f (D[k+1] - D[k])
+// (one fma per component) and split into an integer tap offset and a fraction, both exact in
+// float, so the sample position carries the error of d alone (about 1e-7 px for displacements
+// of a few px) instead of the rounding of an absolute coordinate; D[k+1] = D[k] gives D[k]
+// exactly (integer shifts sample exactly). The sums are compensated.
 template <int M, typename FT, int MODE>
 __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
                            const double* __restrict__ D, float* lds, int lds_cap, bool want_lds,
@@ -612,36 +616,44 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    const int nx = R.nx, ny = R.ny, nz = R.nz;
    const int64_t plane_sz = (int64_t)nx * ny;
    const int k = ch.k;
-   double D0[3], D1[3];
+   float Da[3], Dd[3];
+   bool finite = true;
 #pragma unroll
    for (int c = 0; c < 3; c++) {
-      D0[c] = (c < M) ? D[3 * k + c] : 0.0;
-      D1[c] = (c < M) ? D[3 * (k + 1) + c] : 0.0;
+      const double d0 = (c < M) ? D[3 * k + c] : 0.0, d1 = (c < M) ? D[3 * (k + 1) + c] : 0.0;
+      finite = finite && fabs(d0) < 1e6 && fabs(d1) < 1e6;
+      Da[c] = (float)d0;
+      Dd[c] = (float)d1 - Da[c];
    }
    const int plane = (int)(ch.p0 / plane_sz);
    const int q0 = (int)(ch.p0 - plane * plane_sz);
    const int r0 = q0 / nx, r1 = (int)((ch.p1 - 1 - plane * plane_sz) / nx);
 
-   // d is a convex combination of D[k] and D[k+1]: bound the reachable samples.
+   // f rises with the pixel's scan time, so over the work item it lies in [f(p0), f(p1-1)]
+   // and d between its values there (exact in double); the margin covers the rounding of the
+   // per-pixel fma (half an ulp of |d|; none when D[k+1] = D[k]).
    Stage S{0, 1, 0, 0, 0};
    bool use = false;
-   if (want_lds) {
+   if (want_lds && finite) {
       const double lim = kPad - 3;
+      const double fa = (double)((M == 3) ? R.df[ch.p0] : R.rg4[ch.p0].w);
+      const double fb = (double)((M == 3) ? R.df[ch.p1 - 1] : R.rg4[ch.p1 - 1].w);
       double lo[3], hi[3];
-      bool ok = true;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-         lo[c] = fmin(D0[c], D1[c]);
-         hi[c] = fmax(D0[c], D1[c]);
-         ok = ok && lo[c] > -1e6 && hi[c] < 1e6;
+         const double a = Da[c], dd = Dd[c];
+         const double va = a + fa * dd, vb = a + fb * dd;
+         const double e = dd == 0.0 ? 0.0 : 0x1p-21 * (fabs(a) + fabs(dd));
+         lo[c] = fmin(va, vb) - e;
+         hi[c] = fmax(va, vb) + e;
       }
-      ok = ok && lo[0] > -lim && hi[0] < lim;
-      if (ok) {
-         S.ya = (int)floor(r0 + lo[1]) - 1;
-         S.nrows = (int)floor(r1 + hi[1]) + 2 - S.ya + 1;
+      if (lo[0] > -lim && hi[0] < lim) {
+         // taps y0, y0 + 1 with y0 = y + floor(d_y); likewise in z
+         S.ya = r0 + (int)floor(lo[1]);
+         S.nrows = r1 + (int)floor(hi[1]) + 1 - S.ya + 1;
          if (M == 3) {
-            S.za = (int)floor(plane + lo[2]) - 1;
-            S.nzs = (int)floor(plane + hi[2]) + 2 - S.za + 1;
+            S.za = plane + (int)floor(lo[2]);
+            S.nzs = plane + (int)floor(hi[2]) + 1 - S.za + 1;
          }
          S.stride = ((nx + 2 * kPad) + 3) & ~3;
          use = (int64_t)S.nzs * S.nrows * S.stride <= lds_cap;
@@ -685,50 +697,40 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
       if (p >= ch.p1) break;
       const float4 rec = recs[u];
       const float f = fs[u];
-      const double fd = (double)f, omf = 1.0 - fd;
-      double sp[3];
+      int ti[3];
+      float wt[3];
 #pragma unroll
-      for (int c = 0; c < M; c++) sp[c] = fd * D1[c] + omf * D0[c];
-      sp[0] += (double)w.x;
-      sp[1] += (double)w.y;
-      if (M == 3) sp[2] += (double)plane;
+      for (int c = 0; c < M; c++) {
+         const float d = fmaf(f, Dd[c], Da[c]);
+         const float fl = floorf(d);
+         ti[c] = (int)fl;
+         wt[c] = d - fl;
+      }
+      const int x0 = w.x + ti[0], y0 = w.y + ti[1];
+      const float wx1 = wt[0], wx0 = 1.f - wt[0];
+      const float wy1 = wt[1], wy0 = 1.f - wt[1];
       float v = 0.f;
       if (use) {
-         const double fx = floor(sp[0]), fy = floor(sp[1]);
-         const int x0 = (int)fx, y0 = (int)fy;
-         const double ax = sp[0] - fx, ay = sp[1] - fy;
-         const float wx1 = (float)ax, wx0 = (float)(1.0 - ax);
-         const float wy1 = (float)ay, wy0 = (float)(1.0 - ay);
          int idx = (y0 - S.ya) * S.stride + x0 + kPad;
-         if (M == 3) idx += ((int)floor(sp[2]) - S.za) * S.nrows * S.stride;
+         if (M == 3) idx += (plane + ti[2] - S.za) * S.nrows * S.stride;
          idx = min(max(idx, 0), lds_lim);
          const float* L = lds + idx;
-         float v0 = wy0 * (wx0 * L[0] + wx1 * L[1]) + wy1 * (wx0 * L[S.stride] + wx1 * L[S.stride + 1]);
+         const float v0 = wy0 * (wx0 * L[0] + wx1 * L[1]) + wy1 * (wx0 * L[S.stride] + wx1 * L[S.stride + 1]);
          if (M == 3) {
-            const double fz = floor(sp[2]);
-            const double az = sp[2] - fz;
-            const float wz1 = (float)az, wz0 = (float)(1.0 - az);
             const float* L2 = L + S.nrows * S.stride;
-            float v1 = wy0 * (wx0 * L2[0] + wx1 * L2[1]) + wy1 * (wx0 * L2[S.stride] + wx1 * L2[S.stride + 1]);
-            v = wz0 * v0 + wz1 * v1;
+            const float v1 = wy0 * (wx0 * L2[0] + wx1 * L2[1]) + wy1 * (wx0 * L2[S.stride] + wx1 * L2[S.stride + 1]);
+            v = (1.f - wt[2]) * v0 + wt[2] * v1;
          } else {
             v = v0;
          }
-      } else if (fabs(sp[0]) < 1e9 && fabs(sp[1]) < 1e9 && (M == 2 || fabs(sp[2]) < 1e9)) {
-         const double fx = floor(sp[0]), fy = floor(sp[1]);
-         const int x0 = (int)fx, y0 = (int)fy;
-         const double ax = sp[0] - fx, ay = sp[1] - fy;
-         const float wx1 = (float)ax, wx0 = (float)(1.0 - ax);
-         const float wy1 = (float)ay, wy0 = (float)(1.0 - ay);
-         const int z0 = M == 3 ? (int)floor(sp[2]) : 0;
-         float v0 = wy0 * (wx0 * tap_g(z0, y0, x0) + wx1 * tap_g(z0, y0, x0 + 1)) +
-                    wy1 * (wx0 * tap_g(z0, y0 + 1, x0) + wx1 * tap_g(z0, y0 + 1, x0 + 1));
+      } else if (finite) {
+         const int z0 = M == 3 ? plane + ti[2] : 0;
+         const float v0 = wy0 * (wx0 * tap_g(z0, y0, x0) + wx1 * tap_g(z0, y0, x0 + 1)) +
+                          wy1 * (wx0 * tap_g(z0, y0 + 1, x0) + wx1 * tap_g(z0, y0 + 1, x0 + 1));
          if (M == 3) {
-            const double az = sp[2] - floor(sp[2]);
-            const float wz1 = (float)az, wz0 = (float)(1.0 - az);
-            float v1 = wy0 * (wx0 * tap_g(z0 + 1, y0, x0) + wx1 * tap_g(z0 + 1, y0, x0 + 1)) +
-                       wy1 * (wx0 * tap_g(z0 + 1, y0 + 1, x0) + wx1 * tap_g(z0 + 1, y0 + 1, x0 + 1));
-            v = wz0 * v0 + wz1 * v1;
+            const float v1 = wy0 * (wx0 * tap_g(z0 + 1, y0, x0) + wx1 * tap_g(z0 + 1, y0, x0 + 1)) +
+                             wy1 * (wx0 * tap_g(z0 + 1, y0 + 1, x0) + wx1 * tap_g(z0 + 1, y0 + 1, x0 + 1));
+            v = (1.f - wt[2]) * v0 + wt[2] * v1;
          } else {
             v = v0;
          }
@@ -741,7 +743,7 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          const float e = v - rec.x;
          part[0] += e * e;
          const float ej = (p != ch.end_p) ? e : 0.f;  // the interval's last pixel has no row
-         const float omff = (float)omf;
+         const float omff = 1.f - f;
          const float g[3] = {rec.y, rec.z, rec.w};
 #pragma unroll
          for (int c = 0; c < M; c++) {
@@ -779,9 +781,11 @@ __device__ __forceinline__ void dispatch_chunk(const RefDev& R, const FrameDev& 
 
 // Persistent over the active work list: workgroup b takes a contiguous share of the items,
 // with workgroups renumbered so that each XCD (workgroups are dealt to the 8 XCDs round
-// robin) owns one contiguous range and the frame rows it stages stay in its L2. The 2-D
-// frames of a batch come first; k_pass<2> takes their items, k_pass<3> the rest, so each
-// variant is compiled for its own register budget.
+// robin) owns one contiguous range, so the reference chunks and frame rows it reads stay in
+// its L2: within a reference the items are chunk-major, so consecutive items read the same
+// reference chunk for different frames. The 2-D frames' references come first; k_pass<2>
+// takes their items, k_pass<3> the rest, so each variant is compiled for its own register
+// budget.
 template <int M>
 __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a)
 {
@@ -790,29 +794,34 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a)
    const int64_t W2 = a.counters->n_active_work2;
    const int64_t Wb = M == 2 ? 0 : W2;
    const int64_t W = (M == 2 ? W2 : a.counters->n_active_work) - Wb;
-   const int nfa = a.counters->n_active_frames;
-   if (W <= 0 || nfa <= 0) return;
+   if (W <= 0 || a.counters->n_active_frames <= 0) return;
    const int G = gridDim.x;
    int lb = blockIdx.x;
    if (G % 8 == 0) lb = (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
    const int64_t w0 = Wb + W * lb / G, w1 = Wb + W * (lb + 1) / G;
-   int j = 0;
-   int64_t jb = 0, je = -1;
+   int r = 0, gn = 1, gf = 0;
+   int64_t rb = 0, re = -1;
    for (int64_t wi = w0; wi < w1; wi++) {
-      if (wi >= je || wi < jb) {
-         int lo = 0, hi = nfa - 1;
+      if (wi >= re || wi < rb) {
+         // the last reference whose items start at or before wi (empty groups share the
+         // start of the next one and are skipped)
+         int lo = 0, hi = a.n_refs - 1;
          while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (a.active_base[mid] <= wi) lo = mid;
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.run_base[mid] <= wi) lo = mid;
             else hi = mid - 1;
          }
-         j = lo;
-         jb = a.active_base[j];
-         je = a.active_base[j + 1];
+         r = lo;
+         rb = a.run_base[r];
+         re = a.run_base[r + 1];
+         gn = a.run_count[r];
+         gf = a.run_first[r];
       }
+      const int64_t local = wi - rb;
+      const int c = (int)(local / gn);
+      const int j = gf + (int)(local - (int64_t)c * gn);
       const BatchFrame& bf = a.frames[a.active_frames[j]];
       const RefDev& R = a.refs[bf.ref];
-      const int c = (int)(wi - jb);
       const Chunk ch = R.chunks[c];
       double* out = a.partials + (bf.work_base + c) * kPassStride;
       if (bf.f.dtype == FA_U16)
@@ -901,70 +910,88 @@ __global__ __launch_bounds__(kBlock) void k_splat(RefDev R, FrameDev F, const do
 }
 
 // ---- scheduling: compact the active frames and their work items -----------------------
-// The active list holds the 2-D frames first, then the 3-D frames (k_pass<2> and k_pass<3>
-// split the work there), each kind in frame order. account = 0 only refreshes the counters
-// the host polls.
+// The active frames are grouped by reference (a counting sort; references of 2-D frames have
+// the lower indices, so k_pass<2> takes the first items and k_pass<3> the rest). A reference's
+// work items are chunk-major: the active frames of one reference process a chunk one after
+// the other, so its reference record is read from HBM once per group rather than per frame.
+// The order within a group does not affect any result (each item's sums go to the partials
+// of its own frame and chunk). account = 0 only refreshes the counters the host polls.
 __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
 {
-   __shared__ int64_t s_w[2][1024];
-   __shared__ int32_t s_f[2][1024];
-   __shared__ unsigned long long s_b[1024];
+   __shared__ int32_t s_cnt[kMaxRefs];
+   __shared__ int32_t s_fill[kMaxRefs];
+   __shared__ int64_t s_w[1024];
+   __shared__ int64_t s_w2[1024];
+   __shared__ int32_t s_f[1024];
+   __shared__ unsigned long long s_bytes;
    const int t = threadIdx.x, T = blockDim.x;
-   const int n = B.n_frames;
-   const int f0 = (int)((int64_t)n * t / T), f1 = (int)((int64_t)n * (t + 1) / T);
-   int64_t work[2] = {0, 0};
-   int32_t fr[2] = {0, 0};
-   unsigned long long bytes = 0;
-   for (int f = f0; f < f1; f++) {
-      if (B.state[f].done) continue;
-      const RefDev& R = B.refs[B.frames[f].ref];
-      const int kind = R.n_dim == 3;
-      work[kind] += R.n_chunks;
-      fr[kind]++;
-      bytes += (unsigned long long)R.n_px *
-               (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + (R.n_dim == 3 ? 20 : 16));
+   const int n = B.n_frames, R = B.n_refs;
+   for (int r = t; r < R; r += T) {
+      s_cnt[r] = 0;
+      s_fill[r] = 0;
    }
-   for (int k = 0; k < 2; k++) {
-      s_w[k][t] = work[k];
-      s_f[k][t] = fr[k];
-   }
-   s_b[t] = bytes;
+   if (t == 0) s_bytes = 0;
    __syncthreads();
-   // inclusive scans (Hillis-Steele)
+   unsigned long long bytes = 0;
+   for (int f = t; f < n; f += T) {
+      if (B.state[f].done) continue;
+      const int r = B.frames[f].ref;
+      atomicAdd(&s_cnt[r], 1);
+      const RefDev& RD = B.refs[r];
+      bytes += (unsigned long long)RD.n_px *
+               (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + (RD.n_dim == 3 ? 20 : 16));
+   }
+   if (bytes) atomicAdd(&s_bytes, bytes);
+   __syncthreads();
+   // exclusive scans over the references (each thread a contiguous range of them)
+   const int r0 = (int)((int64_t)R * t / T), r1 = (int)((int64_t)R * (t + 1) / T);
+   int32_t fs = 0;
+   int64_t ws = 0, w2s = 0;
+   for (int r = r0; r < r1; r++) {
+      const int64_t w = (int64_t)s_cnt[r] * B.refs[r].n_chunks;
+      fs += s_cnt[r];
+      ws += w;
+      if (B.refs[r].n_dim == 2) w2s += w;
+   }
+   s_f[t] = fs;
+   s_w[t] = ws;
+   s_w2[t] = w2s;
+   __syncthreads();
    for (int off = 1; off < T; off <<= 1) {
-      int64_t aw0 = t >= off ? s_w[0][t - off] : 0, aw1 = t >= off ? s_w[1][t - off] : 0;
-      int32_t af0 = t >= off ? s_f[0][t - off] : 0, af1 = t >= off ? s_f[1][t - off] : 0;
-      unsigned long long ab = t >= off ? s_b[t - off] : 0;
+      const int32_t af = t >= off ? s_f[t - off] : 0;
+      const int64_t aw = t >= off ? s_w[t - off] : 0, aw2 = t >= off ? s_w2[t - off] : 0;
       __syncthreads();
-      s_w[0][t] += aw0;
-      s_w[1][t] += aw1;
-      s_f[0][t] += af0;
-      s_f[1][t] += af1;
-      s_b[t] += ab;
+      s_f[t] += af;
+      s_w[t] += aw;
+      s_w2[t] += aw2;
       __syncthreads();
    }
-   const int32_t F2 = s_f[0][T - 1], F3 = s_f[1][T - 1];
-   const int64_t W2 = s_w[0][T - 1], W3 = s_w[1][T - 1];
-   int64_t wbase[2] = {s_w[0][t] - work[0], W2 + s_w[1][t] - work[1]};
-   int32_t fbase[2] = {s_f[0][t] - fr[0], F2 + s_f[1][t] - fr[1]};
-   for (int f = f0; f < f1; f++) {
-      if (B.state[f].done) continue;
-      const RefDev& R = B.refs[B.frames[f].ref];
-      const int kind = R.n_dim == 3;
-      B.active_frames[fbase[kind]] = f;
-      B.active_base[fbase[kind]] = wbase[kind];
-      wbase[kind] += R.n_chunks;
-      fbase[kind]++;
+   int32_t fb = s_f[t] - fs;
+   int64_t wb = s_w[t] - ws;
+   for (int r = r0; r < r1; r++) {
+      const int32_t c = s_cnt[r];
+      B.run_first[r] = fb;
+      B.run_count[r] = c;
+      B.run_base[r] = wb;
+      s_cnt[r] = fb;  // now the group's first position
+      fb += c;
+      wb += (int64_t)c * B.refs[r].n_chunks;
    }
    if (t == T - 1) {
-      B.active_base[F2 + F3] = W2 + W3;
-      B.counters->n_active_frames = F2 + F3;
-      B.counters->n_active_work = W2 + W3;
-      B.counters->n_active_work2 = W2;
+      B.run_base[R] = s_w[t];
+      B.counters->n_active_frames = s_f[t];
+      B.counters->n_active_work = s_w[t];
+      B.counters->n_active_work2 = s_w2[t];
       if (account) {
-         B.counters->bytes += s_b[t];
+         B.counters->bytes += s_bytes;
          B.counters->rounds += 1;
       }
+   }
+   __syncthreads();
+   for (int f = t; f < n; f += T) {
+      if (B.state[f].done) continue;
+      const int r = B.frames[f].ref;
+      B.active_frames[s_cnt[r] + atomicAdd(&s_fill[r], 1)] = f;
    }
 }
 

@@ -72,7 +72,8 @@ EXPORTS = [
     "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_get_tuning", "fa_batch_set_tuning",
     "fa_batch_run", "fa_batch_eval", "fa_align_batch",
     "fa_context_stream", "fa_context_mem_info", "fa_server_create", "fa_server_destroy",
-    "fa_server_submit", "fa_server_step", "fa_server_results",
+    "fa_server_submit", "fa_server_submit_host", "fa_server_step", "fa_server_results",
+    "fa_server_memory", "fa_host_alloc", "fa_host_free",
 ]
 
 _lib = None
@@ -117,7 +118,11 @@ def load_library(path=None):
     L.fa_batch_eval.argtypes = [P, P, P, P]
     L.fa_align_batch.argtypes = [P, i32, C.POINTER(TrParams), P, P]
     L.fa_context_mem_info.argtypes = [P, P, P]
-    L.fa_server_create.argtypes = [P, P, i32, i32, C.POINTER(TrParams), C.POINTER(P)]
+    L.fa_server_create.argtypes = [P, P, i32, i32, C.c_uint64, C.POINTER(TrParams), C.POINTER(P)]
+    L.fa_server_submit_host.argtypes = [P, P, P, i32, C.c_float, P, C.c_int64]
+    L.fa_server_memory.argtypes = [P, P, P]
+    L.fa_host_alloc.argtypes = [C.c_uint64, C.POINTER(P)]
+    L.fa_host_free.argtypes = [P]
     L.fa_server_destroy.argtypes = [P]
     L.fa_server_submit.argtypes = [P, P, P, C.c_int64]
     L.fa_server_step.argtypes = [P, i32, P, P]
@@ -409,18 +414,45 @@ class Batch:
         return err, [jac[self.offsets[i]:self.offsets[i + 1]].copy() for i in range(len(self.frames))]
 
 
+class PinnedArray:
+    """A numpy array in pinned host memory (fa_host_alloc), freed with the object."""
+
+    def __init__(self, shape, dtype):
+        self.L = load_library()
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        p = C.c_void_p()
+        _check(self.L.fa_host_alloc(n, C.byref(p)))
+        self.p = p
+        self.array = np.ctypeslib.as_array((C.c_uint8 * max(n, 1)).from_address(p.value))[:n] \
+            .view(dtype).reshape(shape)
+
+    def close(self):
+        if getattr(self, "p", None):
+            self.array = None
+            self.L.fa_host_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     """Continuous batching: frames submitted at any time join the trust-region rounds of the
     frames already in flight (a fixed number of slots) and leave when they finish."""
 
-    def __init__(self, refs, slots, **kw):
+    def __init__(self, refs, slots, arena_bytes=0, **kw):
         refs = list(refs)
         self.L = refs[0].L
         self.ctx = refs[0].ctx
         arr = (C.c_void_p * len(refs))(*[r.h.value for r in refs])
         p = tr_params(**kw)
         h = C.c_void_p()
-        _check(self.L.fa_server_create(self.ctx.h, arr, len(refs), int(slots), C.byref(p), C.byref(h)))
+        _check(self.L.fa_server_create(self.ctx.h, arr, len(refs), int(slots), int(arena_bytes),
+                                       C.byref(p), C.byref(h)))
         self.h = h
         self.slots = int(slots)
         self._live = {}  # tag -> (frame, x): kept alive until the result is taken
@@ -452,6 +484,33 @@ class Server:
         except Exception:
             del self._live[tag]
             raise
+
+    def submit_host(self, ref, data, tag, x0=None, offset=None):
+        """A frame in host memory (ideally a PinnedArray), uploaded into the server's frame
+        arena when a slot and arena room are free. data must stay unchanged until the result."""
+        a = data if isinstance(data, np.ndarray) else np.asarray(data)
+        if not a.flags.c_contiguous or a.dtype not in (np.uint16, np.float32):
+            raise ValueError("host frames are C-contiguous uint16 or float32")
+        if a.size != ref.n_px:
+            raise FaError(-1, "Frame is incorrect size")
+        dt = FA_U16 if a.dtype == np.uint16 else FA_F32
+        off = (1.0 if dt == FA_U16 else 0.0) if offset is None else offset
+        n = ref.nD * ref.n_dim
+        x = np.zeros(n) if x0 is None else np.array(x0, np.float64).reshape(n).copy()
+        if tag in self._live:
+            raise ValueError(f"tag {tag} already in flight")
+        self._live[tag] = (a, x)
+        try:
+            _check(self.L.fa_server_submit_host(self.h, ref.h, _ptr(a), dt, float(off), _ptr(x), int(tag)))
+        except Exception:
+            del self._live[tag]
+            raise
+
+    def memory(self):
+        """(used, total) bytes of the frame arena."""
+        u, t = C.c_uint64(), C.c_uint64()
+        _check(self.L.fa_server_memory(self.h, C.byref(u), C.byref(t)))
+        return u.value, t.value
 
     def step(self, rounds=1):
         """Admits queued frames, runs rounds trust-region rounds and collects what finished:

@@ -1,11 +1,12 @@
 """Serving loop: frames streaming from several acquisitions, aligned by continuous batching.
 
 Every acquisition keeps its reference (packed intensity + gradients, node map, Hessian
-factors) resident in HBM. Frames arrive with exponential inter-arrival times; on arrival a
-frame is uploaded (registered) when the device memory it needs is free, else it waits in the
-host queue; uploaded frames are submitted to a Server whose fixed slots run trust-region
-rounds together, so a frame that arrives while others are on the GPU joins their launches at
-the next round and leaves when it finishes. Latency is arrival -> result on the host.
+factors) resident in HBM. Frames arrive with exponential inter-arrival times and are handed
+to a Server, which admits a frame when one of its slots and room for the frame in its device
+frame arena are free (admission by free memory), uploads it on a copy stream while the
+frames already admitted compute, and runs the trust-region rounds of all admitted frames
+together: a frame that arrives while others are on the GPU joins their launches and leaves
+when it finishes. Latency is arrival -> result on the host.
 
 The reference aligner has no serving loop: its MEX front end queues addFrame calls on a
 thread pool (FrameAlignerMex/FrameAlignerMex.cpp:110-130), one frame per task.
@@ -51,17 +52,22 @@ def arrivals(n_acq, rate, duration, seed):
 
 
 class ServeLoop:
-    """The request loop. refs: one fa_amd.Reference per acquisition."""
+    """The request loop. refs: one fa_amd.Reference per acquisition; pools: per acquisition,
+    the raw uint16 frames its stream cycles through. The pools are copied once into pinned
+    host memory (where a frame grabber would DMA them), so uploads are asynchronous."""
 
-    def __init__(self, server, refs, pools, mem_budget=None, reserve=1 << 30, rounds=2):
+    def __init__(self, server, refs, pools, rounds=2):
+        from . import PinnedArray
         self.server = server
         self.refs = refs
-        self.pools = pools  # per acquisition: array of raw uint16 frames
-        self.ctx = refs[0].ctx
-        self.mem_budget = mem_budget
-        self.reserve = reserve
+        self.pinned = [PinnedArray(p.shape, p.dtype) for p in pools]
+        for dst, src in zip(self.pinned, pools):
+            dst.array[...] = src
         self.rounds = rounds
-        self.frame_bytes = [p[0].nbytes for p in pools]
+
+    def close(self):
+        for p in self.pinned:
+            p.close()
 
     def run(self, times, acq, keep=None):
         """Streams the arrivals in real time. Returns dict with per-frame latency (s), the
@@ -70,59 +76,38 @@ class ServeLoop:
         n = len(times)
         lat = np.full(n, np.nan)
         kept = {}
-        waiting = []          # arrived, not yet uploaded (index)
-        uploaded = {}         # index -> Frame
-        used = 0
         nxt = 0
         done = 0
         steps = 0
         max_flight = 0
-        deferred = 0
+        max_queued = 0
+        busy = 0
         t0 = time.perf_counter()
         while done < n:
             now = time.perf_counter() - t0
             while nxt < n and times[nxt] <= now:
-                waiting.append(nxt)
+                a = int(acq[nxt])
+                pool = self.pinned[a].array
+                S.submit_host(self.refs[a], pool[nxt % len(pool)], tag=nxt)
                 nxt += 1
-            # admission by free memory: upload while the frame fits in what is free
-            while waiting:
-                i = waiting[0]
-                a = int(acq[i])
-                need = self.frame_bytes[a]
-                if self.mem_budget is not None and used + need > self.mem_budget:
-                    deferred += 1
-                    break
-                free, _ = self.ctx.mem_info()
-                if free < need + self.reserve:
-                    deferred += 1
-                    break
-                waiting.pop(0)
-                pool = self.pools[a]
-                fr = self.refs[a].register(pool[i % len(pool)])
-                uploaded[i] = fr
-                used += need
-                S.submit(fr, tag=i)
-            if not uploaded:
-                if waiting:
-                    raise MemoryError("a frame does not fit the device memory budget on its own")
-                if nxt < n:
-                    time.sleep(max(0.0, times[nxt] - (time.perf_counter() - t0)))
+                busy += 1
+            if busy == 0:
+                time.sleep(max(0.0, times[nxt] - (time.perf_counter() - t0)))
                 continue
             in_flight, queued, res = S.step(self.rounds)
             steps += 1
             max_flight = max(max_flight, in_flight)
+            max_queued = max(max_queued, queued)
             tnow = time.perf_counter() - t0
             for tag, x, st in res:
                 lat[tag] = tnow - times[tag]
                 if keep is not None and tag in keep:
                     kept[tag] = (x, st)
-                fr = uploaded.pop(tag)
-                used -= self.frame_bytes[int(acq[tag])]
-                fr.release()
                 done += 1
+                busy -= 1
         wall = time.perf_counter() - t0
         return dict(latency=lat, kept=kept, wall=wall, steps=steps, max_in_flight=max_flight,
-                    deferred=deferred)
+                    max_queued=max_queued)
 
 
 def summarize(res, times, acq, pools):
@@ -137,4 +122,4 @@ def summarize(res, times, acq, pools):
                            "p99": round(float(np.percentile(lat, 99)), 3),
                            "max": round(float(lat.max()), 3)},
             "steps": res["steps"], "max_in_flight": res["max_in_flight"],
-            "admission_deferred": res["deferred"]}
+            "max_queued": res["max_queued"]}

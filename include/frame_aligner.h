@@ -190,12 +190,17 @@ int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
                    double* x_inout, fa_align_stats* stats);
 
 /* Serving (continuous batching): a fixed set of slots, each sized for the largest of the
-   references the server serves. Submitted frames wait in a queue; every fa_server_step admits
-   queued frames into free slots, runs `rounds` trust-region rounds over all occupied slots
-   (frames admitted at different times share the launches), and harvests finished slots:
-   x_inout (caller memory, valid until the result is returned) receives the result and
-   fa_server_results lists the finished tags. Each result is the same as fa_align's for that
-   frame. Use one thread per server; frames stay registered until their result is taken.
+   references the server serves, and a device frame arena of arena_bytes (0: slots times the
+   largest float frame). Submitted frames wait in a queue, in arrival order. Every
+   fa_server_step takes queued frames while a slot is free and (for host frames) the arena
+   has room for the frame (admission by free memory), uploads host frames on a copy stream
+   that overlaps the running rounds, starts the frames whose data is on the device, runs
+   `rounds` trust-region rounds over every running slot (frames admitted at different times
+   share the launches), and harvests the finished slots: x_inout (caller memory, valid until
+   the result is returned) receives the result and fa_server_results lists the finished tags.
+   Each result is the same as fa_align's for that frame. Host frames should be pinned
+   (fa_host_alloc) for the upload to be asynchronous, and stay untouched until their result is
+   returned; registered frames stay registered until then. Use one thread per server.
    There is no counterpart in the reference beyond its thread pool of addFrame tasks
    (FrameAlignerMex/FrameAlignerMex.cpp:110-130). */
 typedef struct fa_server_result {
@@ -204,11 +209,21 @@ typedef struct fa_server_result {
 } fa_server_result;
 
 int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs, int32_t slots,
-                     const fa_tr_params* prm, fa_server** out);
+                     uint64_t arena_bytes, const fa_tr_params* prm, fa_server** out);
 int fa_server_destroy(fa_server* server);
+/* A registered frame (fa_register_frame). */
 int fa_server_submit(fa_server* server, fa_frame* frame, double* x_inout, int64_t tag);
+/* A frame in host memory, nz*ny*nx values of dtype, sampled as value + add_offset (as
+   fa_register_frame), uploaded into the arena when admitted. */
+int fa_server_submit_host(fa_server* server, fa_reference* ref, const void* data, int32_t dtype,
+                          float add_offset, double* x_inout, int64_t tag);
 int fa_server_step(fa_server* server, int32_t rounds, int32_t* in_flight, int32_t* queued);
 int fa_server_results(fa_server* server, fa_server_result* out, int32_t max, int32_t* n_out);
+/* Bytes of the frame arena in use and in total. */
+int fa_server_memory(const fa_server* server, uint64_t* used, uint64_t* total);
+/* Pinned host memory for frames handed to fa_server_submit_host. */
+int fa_host_alloc(uint64_t bytes, void** out);
+int fa_host_free(void* p);
 /* Free and total device memory (hipMemGetInfo), for admission by free memory. */
 int fa_context_mem_info(fa_context* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 

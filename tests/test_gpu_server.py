@@ -90,14 +90,49 @@ def test_server_errors(ctx):
     r2._release_handle()
 
 
+def test_server_host_frames_and_arena(ctx):
+    """Host frames uploaded into the arena give the same results as registered frames; an
+    arena that holds two frames at a time still serves a queue of many."""
+    stacks, refs, frames = _setup(ctx)
+    alone = [f.align(**KW) for f in frames]
+    hosts = ([(refs[0], f) for f in stacks[0]["frames_w"]] + [(refs[1], f) for f in stacks[1]["frames"]] +
+             [(refs[2], f) for f in stacks[2]["frames_w"]])
+    big = max(f.nbytes for _, f in hosts)
+    srv = F.Server(refs, slots=6, arena_bytes=2 * big + 512, **KW)
+    pinned = [F.PinnedArray(f.shape, f.dtype) for _, f in hosts]
+    for p, (_, f) in zip(pinned, hosts):
+        p.array[...] = f
+    for i, ((r, _), p) in enumerate(zip(hosts, pinned)):
+        # float frames are already offset, uint16 counts get +1 on the device
+        srv.submit_host(r, p.array, tag=i, offset=0.0 if p.array.dtype == np.float32 else None)
+    got = {}
+    steps = 0
+    while len(got) < len(hosts):
+        _, _, res = srv.step(rounds=2)
+        used, total = srv.memory()
+        assert used <= total
+        for tag, x, st in res:
+            got[tag] = x
+        steps += 1
+        assert steps < 10000
+    for i, (x1, _) in enumerate(alone):
+        assert np.array_equal(x1, got[i]), i
+    assert srv.memory()[0] == 0
+    with pytest.raises(F.FaError):  # never fits
+        F.Server(refs[:1], slots=1, arena_bytes=256).submit_host(refs[0], pinned[0].array, tag=0)
+    srv.close()
+    for p in pinned:
+        p.close()
+
+
 def test_serve_loop_small(ctx):
     acqs = serve.make_acquisitions(seed=5, shapes=[(1, 64, 64), (1, 96, 80), (4, 40, 36)], pool=2,
                                    node_range=(8, 12))
     refs = [F.Reference(ctx, serve.synth.with_offset(a["reference"]), a["nD"]) for a in acqs]
-    srv = F.Server(refs, slots=8, **KW)
     pools = [a["frames"] for a in acqs]
+    srv = F.Server(refs, slots=8, arena_bytes=4 * max(p[0].nbytes for p in pools), **KW)
     times, acq = serve.arrivals(len(acqs), rate=400.0, duration=0.1, seed=6)
-    loop = serve.ServeLoop(srv, refs, pools, mem_budget=4 * pools[0][0].nbytes, rounds=2)
+    loop = serve.ServeLoop(srv, refs, pools, rounds=2)
     keep = set(range(len(times)))
     res = loop.run(times, acq, keep=keep)
     assert np.all(np.isfinite(res["latency"])) and np.all(res["latency"] >= 0)
@@ -110,4 +145,5 @@ def test_serve_loop_small(ctx):
         fr.release()
     s = serve.summarize(res, times, acq, pools)
     assert s["frames"] == len(times)
+    loop.close()
     srv.close()
