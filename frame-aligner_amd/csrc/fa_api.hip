@@ -60,7 +60,6 @@ void dfree(T*& p)
    p = nullptr;
 }
 
-constexpr int kTrLdsMaxDoubles = 7680;  // 60 KB for the trust-region update
 constexpr int kLdsCapFloats = 15360;  // 60 KB of staged frame rows per workgroup at most
 constexpr int kBlocksPerCu = 8;
 
@@ -127,7 +126,7 @@ struct fa_batch {
    int64_t work2 = 0, work3 = 0;
    fa_tuning tune{};
    int timing = 0;
-   int tr_lds = 0;
+   int tr_lds = 0;  // doubles of LDS the update's factor blocks need when they fit, else 0
    std::vector<int> user;  // internal index -> caller's index
    int n_slots = 0;        // serving batches: number of slots (frames are admitted into them)
    int64_t lds_need[2] = {0, 0};  // LDS floats the 2-D / 3-D frames' work items stage
@@ -177,7 +176,7 @@ struct fa_batch {
       B.prm.sub_max_iter = prm.sub_max_iter;
       B.prm.skip_repeat = prm.skip_repeat;
       B.n_frames = n_slots > 0 ? n_slots : (int32_t)frames.size();
-      B.tr_lds = tr_lds;
+      B.tr_lds = tune.tr_lds ? tr_lds : 0;
       return B;
    }
 
@@ -244,6 +243,20 @@ int64_t lds_floats_for(const fa_reference* r)
    return rows * stride * (r->n_dim == 3 ? 2 : 1);
 }
 
+// Doubles of LDS the trust-region update of a frame of reference r uses: the cyclic-reduction
+// factor blocks (3 nD n_dim^2) and the solve vector (nD n_dim).
+int tr_lds_need(const fa_reference* r)
+{
+   return 3 * r->nD * r->n_dim * r->n_dim + r->nD * r->n_dim;
+}
+
+// The update runs from LDS when the largest frame's need fits one workgroup's LDS, else from
+// global memory (0).
+int tr_lds_for(const fa_context* ctx, int need)
+{
+   return (int64_t)need * (int64_t)sizeof(double) <= ctx->lds_per_block ? need : 0;
+}
+
 // Default LDS budget: the largest over the batch's frames, capped.
 int default_lds_floats(const std::vector<fa_frame*>& frames)
 {
@@ -259,6 +272,7 @@ void default_tuning(fa_tuning* t, const std::vector<fa_frame*>& frames)
    t->use_lds = 1;
    t->check_every = 8;
    t->use_graph = 1;
+   t->tr_lds = 1;
 }
 
 void set_grids(fa_batch* b)
@@ -332,13 +346,11 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       (r->n_dim == 2 ? b->work2 : b->work3) += (int64_t)r->chunks.size();
       int64_t& lds = b->lds_need[r->n_dim == 3];
       lds = std::max(lds, lds_floats_for(r));
-      b->total_vec += 4 * (int64_t)bf.n;
-      b->total_fac += 2 * (int64_t)r->nD * r->n_dim * r->n_dim;
-      const int need = 5 * bf.n + 4 * r->nD * r->n_dim * r->n_dim;
-      tr_need = std::max(tr_need, need);
+      b->total_vec += 3 * (int64_t)bf.n;
+      b->total_fac += 3 * (int64_t)r->nD * r->n_dim * r->n_dim;
+      tr_need = std::max(tr_need, tr_lds_need(r));
    }
-   // the update works from LDS when every frame's vectors and JtJ blocks fit
-   b->tr_lds = tr_need <= kTrLdsMaxDoubles ? tr_need : 0;
+   b->tr_lds = tr_lds_for(ctx, tr_need);
    b->n_refs = (int)refs.size();
    if (b->n_refs > kMaxRefs) {
       delete b;
@@ -811,7 +823,7 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
       fmax = std::max(fmax, r->nD * r->n_dim * r->n_dim);
       cmax = std::max<int64_t>(cmax, (int64_t)r->chunks.size());
       (r->n_dim == 2 ? has2 : has3) = true;
-      tr_need = std::max(tr_need, 5 * r->nD * r->n_dim + 4 * r->nD * r->n_dim * r->n_dim);
+      tr_need = std::max(tr_need, tr_lds_need(r));
       widest = std::max(widest, lds_floats_for(r));
       b->lds_need[r->n_dim == 3] = std::max(b->lds_need[r->n_dim == 3], lds_floats_for(r));
    }
@@ -820,22 +832,23 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
    b->tune.use_lds = 1;
    b->tune.check_every = 8;
    b->tune.use_graph = 0;
-   b->tr_lds = tr_need <= kTrLdsMaxDoubles ? tr_need : 0;
+   b->tune.tr_lds = 1;
+   b->tr_lds = tr_lds_for(ctx, tr_need);
    b->hframes.resize(S);
    for (int j = 0; j < S; j++) {
       BatchFrame& bf = b->hframes[j];
       bf = BatchFrame{};
       bf.work_base = (int64_t)j * cmax;
       bf.x_base = (int64_t)j * nmax;
-      bf.vec_base = (int64_t)j * 4 * nmax;
-      bf.fac_base = (int64_t)j * 2 * fmax;
+      bf.vec_base = (int64_t)j * 3 * nmax;
+      bf.fac_base = (int64_t)j * 3 * fmax;
       bf.d_base = (int64_t)j * 3 * ndmax;
       b->user.push_back(j);
    }
    b->total_work = S * cmax;
    b->total_x = (int64_t)S * nmax;
-   b->total_vec = 4 * (int64_t)S * nmax;
-   b->total_fac = 2 * (int64_t)S * fmax;
+   b->total_vec = 3 * (int64_t)S * nmax;
+   b->total_fac = 3 * (int64_t)S * fmax;
    b->total_D = 3 * (int64_t)S * ndmax;
    b->work2 = has2 ? b->total_work : 0;
    b->work3 = has3 ? b->total_work : 0;
@@ -1014,6 +1027,11 @@ int fa_create(int device, fa_context** out)
    auto* c = new fa_context();
    c->device = device;
    c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+   c->lds_per_block = (int)std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
+   // raises the dynamic-LDS limit of the kernels that use more than 64 KB where the runtime
+   // keeps one; launches report their own errors
+   c->lds_config = (int)configure_kernels(c->lds_per_block);
+   (void)hipGetLastError();
    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
       delete c;
       return set_err(FA_E_DEVICE, "stream creation failed");
@@ -1144,7 +1162,7 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    const int mm = m * m;
    if ((rc = dalloc(&r->d_chunks, nc)) || (rc = dalloc(&r->d_ichunk, nD)) ||
        (rc = dalloc(&r->d_Hdiag, (size_t)nD * mm)) || (rc = dalloc(&r->d_Hsub, (size_t)nD * mm)) ||
-       (rc = dalloc(&r->d_L0d, (size_t)nD * mm)) || (rc = dalloc(&r->d_L0s, (size_t)nD * mm)) ||
+       (rc = dalloc(&r->d_L0, (size_t)3 * nD * mm)) ||
        (rc = dalloc(&d_hpart, (size_t)nc * kHessStride)))
       return fail(rc);
 
@@ -1155,8 +1173,7 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    R.ichunk = r->d_ichunk;
    R.Hdiag = r->d_Hdiag;
    R.Hsub = r->d_Hsub;
-   R.L0d = r->d_L0d;
-   R.L0s = r->d_L0s;
+   R.L0 = r->d_L0;
    R.scan = r->scan;
    R.nx = nx;
    R.ny = ny;
@@ -1174,9 +1191,9 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    if (hipMemcpyAsync(r->d_chunks, r->chunks.data(), nc * sizeof(Chunk), hipMemcpyHostToDevice, s) != hipSuccess ||
        hipMemcpyAsync(r->d_ichunk, r->ichunk.data(), nD * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
        hipMemsetAsync(r->d_Hsub, 0, (size_t)nD * mm * sizeof(double), s) != hipSuccess ||
-       hipMemsetAsync(r->d_L0s, 0, (size_t)nD * mm * sizeof(double), s) != hipSuccess ||
+       hipMemsetAsync(r->d_L0, 0, (size_t)3 * nD * mm * sizeof(double), s) != hipSuccess ||
        launch_reference_record(d_T, nz, ny, nx, m, r->scan, r->d_rg4, r->d_df, s) != hipSuccess ||
-       launch_hessian(R, d_hpart, r->d_Hdiag, r->d_Hsub, r->d_L0d, r->d_L0s, d_info, s) != hipSuccess ||
+       launch_hessian(R, d_hpart, r->d_Hdiag, r->d_Hsub, r->d_L0, d_info, s) != hipSuccess ||
        hipMemcpyAsync(info, d_info, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
        hipMemcpyAsync(r->Hdiag.data(), r->d_Hdiag, (size_t)nD * mm * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
        hipMemcpyAsync(r->Hsub.data(), r->d_Hsub, (size_t)nD * mm * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -1208,8 +1225,7 @@ int fa_reference_release(fa_reference* r)
    dfree(r->d_ichunk);
    dfree(r->d_Hdiag);
    dfree(r->d_Hsub);
-   dfree(r->d_L0d);
-   dfree(r->d_L0s);
+   dfree(r->d_L0);
    delete r;
    return FA_OK;
 }
@@ -1475,6 +1491,8 @@ int fa_batch_get_tuning(const fa_batch* b, fa_tuning* t)
 {
    if (!b || !t) return set_err(FA_E_ARG, "null argument");
    *t = b->tune;
+   const fa_batch* leaf = b->parts.empty() ? b : b->parts[0];
+   t->tr_lds = b->tune.tr_lds && leaf->tr_lds > 0;
    return FA_OK;
 }
 
@@ -1482,7 +1500,7 @@ int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t)
 {
    if (!b || !t) return set_err(FA_E_ARG, "null argument");
    if (t->lds_floats < 0 || t->lds_floats > 15360 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
-       t->check_every < 1 || t->check_every > 64)
+       t->check_every < 1 || t->check_every > 64 || t->tr_lds < 0 || t->tr_lds > 1)
       return set_err(FA_E_ARG, "tuning out of range (lds_floats <= 15360, 1 <= blocks_per_cu <= 64)");
    std::lock_guard<std::mutex> lk(b->ctx->mu);
    b->tune = *t;
@@ -1522,6 +1540,17 @@ int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, 
    int rc = fa_batch_run(b, x_inout, stats, nullptr);
    fa_batch_destroy(b);
    return rc;
+}
+
+int fa_context_info(const fa_context* ctx, int32_t* o)
+{
+   if (!ctx || !o) return set_err(FA_E_ARG, "null argument");
+   o[0] = ctx->device;
+   o[1] = ctx->n_cu;
+   o[2] = ctx->lds_per_block;
+   o[3] = 64;
+   o[4] = ctx->lds_config;
+   return FA_OK;
 }
 
 int fa_context_mem_info(fa_context* ctx, uint64_t* free_bytes, uint64_t* total_bytes)

@@ -52,8 +52,7 @@ struct RefDev {
    const int32_t* ichunk;  // first chunk of interval k (k = 0..nD-2), ichunk[nD-1] = n_chunks
    const double* Hdiag;    // nD blocks n_dim x n_dim
    const double* Hsub;     // nD-1 blocks, block (k+1, k)
-   const double* L0d;      // Cholesky factor of H (valid when h_pd)
-   const double* L0s;
+   const double* L0;       // cyclic-reduction Cholesky factor of H (A, C, W blocks; valid when h_pd)
    ScanDev scan;
    int32_t nx, ny, nz, nD, n_dim, n_chunks, h_pd, pad;
    double eig_lo;          // Gershgorin lower bound of the spectrum of H
@@ -123,7 +122,7 @@ struct BatchDev {
    TrState* state;
    double* x;         // parameter vectors, concatenated in batch order
    double* vec;       // per frame: p, g, t1, t2 (4 n doubles)
-   double* fac;       // per frame: Ld, Ls (2 nD n_dim^2 doubles)
+   double* fac;       // per frame: factor blocks A, C, W of H + lambda I (3 nD n_dim^2 doubles)
    double* Dtrial;
    const double* partials;
    int32_t* active_frames;
@@ -144,6 +143,8 @@ struct BatchDev {
 struct fa_context {
    int device = 0;
    int n_cu = 256;
+   int lds_per_block = 65536;  // bytes of LDS one workgroup may allocate
+   int lds_config = 0;         // hipError_t of raising the kernels' dynamic-LDS limit
    hipStream_t stream = nullptr;
    std::mutex mu;
    int live_refs = 0;  // references not yet released; fa_destroy refuses while > 0
@@ -168,8 +169,7 @@ struct fa_reference {
    int32_t* d_ichunk = nullptr;
    double* d_Hdiag = nullptr;
    double* d_Hsub = nullptr;
-   double* d_L0d = nullptr;
-   double* d_L0s = nullptr;
+   double* d_L0 = nullptr;  // 3 nD n_dim^2: factor blocks A, C, W of H
    fa::RefDev dev{};
    int live_frames = 0;
    bool counted = false;  // included in ctx->live_refs

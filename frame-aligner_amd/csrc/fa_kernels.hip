@@ -283,138 +283,290 @@ __device__ __forceinline__ bool chol_block(double (&A)[M][M])
    return true;
 }
 
-// L L^T = H + lambda I with H block-tridiagonal (diag blocks, sub-diagonal blocks (k+1,k)).
+// ---- block cyclic reduction ---------------------------------------------------------------
+// Cholesky factorisation of T = H + lambda I (block-tridiagonal: nb diagonal blocks, blocks
+// (k+1, k) below) in odd-even elimination order. At level s = 1, 2, 4, ... the blocks
+// i = s, 3s, 5s, ... < nb are eliminated; no two of them are neighbours in the remaining chain,
+// so a whole level runs in parallel, and the Schur complement on the blocks 0, 2s, 4s, ... is
+// block-tridiagonal again. Block 0 is eliminated last. This is the Cholesky factor of a
+// symmetric permutation of T: it exists exactly when T is positive definite, and
+// ||L^-1 v||^2 = v^T T^-1 v as with the natural order. Depth ceil(log2 nb) + 1 levels.
+//
+// Storage, nb M x M blocks each (row-major):
+//   A[i] = L_i, the Cholesky factor of block i's Schur complement when it is eliminated,
+//          with the reciprocals of its diagonal in place of the diagonal;
+//   W[i] = L_i^-1 T(i, i-s), the link to its left neighbour at its level s;
+//   C[i] = L_i^-1 T(i+s, i)^T, the link to its right neighbour (when i + s < nb).
+// While the factorisation runs, C[j] of a block j still in the chain holds T(j+s, j) at the
+// current level. NT threads share the work and all of them call these functions; each level
+// ends with a workgroup barrier.
+
 template <int M>
-__device__ __forceinline__ bool bt_chol(int nb, const double* diag, const double* sub, double lambda, double* Ld,
-                        double* Ls)
+__device__ __forceinline__ void ldb(const double* p, double (&X)[M][M])
 {
-   constexpr int MM = M * M;
-   double C[M][M];
 #pragma unroll
    for (int i = 0; i < M; i++)
 #pragma unroll
-      for (int j = 0; j < M; j++) C[i][j] = 0;
-   for (int k = 0; k < nb; k++) {
-      double A[M][M];
-#pragma unroll
-      for (int i = 0; i < M; i++)
-#pragma unroll
-         for (int j = 0; j < M; j++) A[i][j] = diag[k * MM + i * M + j];
-#pragma unroll
-      for (int i = 0; i < M; i++) A[i][i] += lambda;
-      if (k > 0) {
-#pragma unroll
-         for (int i = 0; i < M; i++)
-#pragma unroll
-            for (int j = 0; j < M; j++) {
-               double s = 0;
-#pragma unroll
-               for (int t = 0; t < M; t++) s += C[i][t] * C[j][t];
-               A[i][j] -= s;
-            }
-      }
-      if (!chol_block<M>(A)) return false;
-#pragma unroll
-      for (int i = 0; i < M; i++)
-#pragma unroll
-         for (int j = 0; j < M; j++) Ld[k * MM + i * M + j] = A[i][j];
-      if (k + 1 < nb) {
-#pragma unroll
-         for (int i = 0; i < M; i++)
-#pragma unroll
-            for (int j = 0; j < M; j++) {
-               double t = sub[k * MM + i * M + j];
-#pragma unroll
-               for (int q = 0; q < j; q++) t -= C[i][q] * A[j][q];
-               C[i][j] = t * (1.0 / A[j][j]);
-            }
-#pragma unroll
-         for (int i = 0; i < M; i++)
-#pragma unroll
-            for (int j = 0; j < M; j++) Ls[k * MM + i * M + j] = C[i][j];
-      }
-   }
-   return true;
+      for (int j = 0; j < M; j++) X[i][j] = p[i * M + j];
 }
 
-// y = L^-1 b.
 template <int M>
-__device__ __forceinline__ void bt_fwd(int nb, const double* Ld, const double* Ls, const double* b, double* y)
+__device__ __forceinline__ void stb(double* p, const double (&X)[M][M])
+{
+#pragma unroll
+   for (int i = 0; i < M; i++)
+#pragma unroll
+      for (int j = 0; j < M; j++) p[i * M + j] = X[i][j];
+}
+
+// X := L^-1 X for a lower-triangular L with reciprocal diagonal il.
+template <int M>
+__device__ __forceinline__ void trsm_lower(const double (&L)[M][M], const double (&il)[M], double (&X)[M][M])
+{
+#pragma unroll
+   for (int c = 0; c < M; c++)
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+         double t = X[i][c];
+#pragma unroll
+         for (int q = 0; q < i; q++) t -= L[i][q] * X[q][c];
+         X[i][c] = t * il[i];
+      }
+}
+
+// v := L^-1 v and v := L^-T v for one diagonal factor block (reciprocal diagonal stored).
+template <int M>
+__device__ __forceinline__ void fwd_block(const double* Lp, double (&v)[M])
+{
+#pragma unroll
+   for (int i = 0; i < M; i++) {
+      double t = v[i];
+#pragma unroll
+      for (int q = 0; q < i; q++) t -= Lp[i * M + q] * v[q];
+      v[i] = t * Lp[i * M + i];
+   }
+}
+
+template <int M>
+__device__ __forceinline__ void bwd_block(const double* Lp, double (&v)[M])
+{
+#pragma unroll
+   for (int i = M - 1; i >= 0; i--) {
+      double t = v[i];
+#pragma unroll
+      for (int q = i + 1; q < M; q++) t -= Lp[q * M + i] * v[q];
+      v[i] = t * Lp[i * M + i];
+   }
+}
+
+// Stores a factor block with its diagonal replaced by the reciprocals il.
+template <int M>
+__device__ __forceinline__ void stb_factor(double* p, const double (&L)[M][M], const double (&il)[M])
+{
+#pragma unroll
+   for (int i = 0; i < M; i++)
+#pragma unroll
+      for (int j = 0; j < M; j++) p[i * M + j] = i == j ? il[i] : L[i][j];
+}
+
+// Factor of Hd/Hs + lambda I into (A, C, W); false (on every thread) when not positive definite.
+template <int M, int NT>
+__device__ bool cr_factor(int nb, const double* __restrict__ Hd, const double* __restrict__ Hs, double lambda,
+                          double* A, double* C, double* W)
 {
    constexpr int MM = M * M;
-   double yp[M];
-#pragma unroll
-   for (int i = 0; i < M; i++) yp[i] = 0;
-   for (int k = 0; k < nb; k++) {
-      double r[M], yk[M];
-#pragma unroll
-      for (int i = 0; i < M; i++) {
-         r[i] = b[k * M + i];
-         if (k > 0)
-#pragma unroll
-            for (int t = 0; t < M; t++) r[i] -= Ls[(k - 1) * MM + i * M + t] * yp[t];
-      }
-      const double* L = &Ld[k * MM];
-      double il[M];
-#pragma unroll
-      for (int i = 0; i < M; i++) il[i] = 1.0 / L[i * M + i];
-#pragma unroll
-      for (int i = 0; i < M; i++) {
-         double t = r[i];
-#pragma unroll
-         for (int q = 0; q < i; q++) t -= L[i * M + q] * yk[q];
-         yk[i] = t * il[i];
-      }
-#pragma unroll
-      for (int i = 0; i < M; i++) {
-         y[k * M + i] = yk[i];
-         yp[i] = yk[i];
-      }
+   const int t = threadIdx.x;
+   for (int i = t; i < nb * MM; i += NT) {
+      const int e = i % MM;
+      A[i] = Hd[i] + (e % (M + 1) == 0 ? lambda : 0.0);
+      C[i] = i < (nb - 1) * MM ? Hs[i] : 0.0;
    }
+   __syncthreads();
+   for (int s = 1; s < nb; s <<= 1) {
+      const int ne = (nb + s - 1) / (2 * s);  // i = s, 3s, ... < nb
+      int fail = 0;
+      for (int q = t; q < ne; q += NT) {
+         const int i = s * (2 * q + 1);
+         double L[M][M], X[M][M], il[M];
+         ldb<M>(A + i * MM, L);
+         if (!chol_block<M>(L)) {
+            fail = 1;
+            continue;
+         }
+#pragma unroll
+         for (int r = 0; r < M; r++) il[r] = 1.0 / L[r][r];
+         stb_factor<M>(A + i * MM, L, il);
+         ldb<M>(C + (i - s) * MM, X);
+         trsm_lower<M>(L, il, X);
+         stb<M>(W + i * MM, X);
+         if (i + s < nb) {
+#pragma unroll
+            for (int a = 0; a < M; a++)
+#pragma unroll
+               for (int b = 0; b < M; b++) X[a][b] = C[i * MM + b * M + a];
+            trsm_lower<M>(L, il, X);
+            stb<M>(C + i * MM, X);
+         }
+      }
+      if (__syncthreads_or(fail)) return false;
+      const int nr = (nb + 2 * s - 1) / (2 * s);  // j = 0, 2s, 4s, ... < nb
+      for (int q = t; q < nr; q += NT) {
+         const int j = 2 * s * q;
+         double Aj[M][M], X[M][M], V[M][M];
+         ldb<M>(A + j * MM, Aj);
+         if (j + s < nb) {
+            ldb<M>(W + (j + s) * MM, X);
+#pragma unroll
+            for (int a = 0; a < M; a++)
+#pragma unroll
+               for (int b = 0; b < M; b++) {
+                  double u = 0;
+#pragma unroll
+                  for (int r = 0; r < M; r++) u += X[r][a] * X[r][b];
+                  Aj[a][b] -= u;
+               }
+            if (j + 2 * s < nb) {
+               // new coupling T'(j+2s, j) = -T(j+2s, j+s) T(j+s, j+s)^-1 T(j+s, j) = -V^T W
+               ldb<M>(C + (j + s) * MM, V);
+               double N[M][M];
+#pragma unroll
+               for (int a = 0; a < M; a++)
+#pragma unroll
+                  for (int b = 0; b < M; b++) {
+                     double u = 0;
+#pragma unroll
+                     for (int r = 0; r < M; r++) u += V[r][a] * X[r][b];
+                     N[a][b] = -u;
+                  }
+               stb<M>(C + j * MM, N);
+            }
+         }
+         if (j >= s) {
+            ldb<M>(C + (j - s) * MM, V);
+#pragma unroll
+            for (int a = 0; a < M; a++)
+#pragma unroll
+               for (int b = 0; b < M; b++) {
+                  double u = 0;
+#pragma unroll
+                  for (int r = 0; r < M; r++) u += V[r][a] * V[r][b];
+                  Aj[a][b] -= u;
+               }
+         }
+         stb<M>(A + j * MM, Aj);
+      }
+      __syncthreads();
+   }
+   int fail = 0;
+   if (t == 0) {
+      double L[M][M], il[M];
+      ldb<M>(A, L);
+      fail = chol_block<M>(L) ? 0 : 1;
+#pragma unroll
+      for (int r = 0; r < M; r++) il[r] = 1.0 / L[r][r];
+      stb_factor<M>(A, L, il);
+   }
+   return __syncthreads_or(fail) == 0;
 }
 
-// x = L^-T y.
-template <int M>
-__device__ __forceinline__ void bt_bwd(int nb, const double* Ld, const double* Ls, const double* y, double* x)
+// y := L^-1 y with the factor of cr_factor (y: nb blocks of M).
+template <int M, int NT>
+__device__ void cr_fwd(int nb, const double* A, const double* C, const double* W, double* y)
 {
    constexpr int MM = M * M;
-   double xn[M];
+   const int t = threadIdx.x;
+   for (int s = 1; s < nb; s <<= 1) {
+      const int ne = (nb + s - 1) / (2 * s);
+      for (int q = t; q < ne; q += NT) {
+         const int i = s * (2 * q + 1);
+         double v[M];
 #pragma unroll
-   for (int i = 0; i < M; i++) xn[i] = 0;
-   for (int k = nb - 1; k >= 0; k--) {
-      double r[M], xk[M];
+         for (int a = 0; a < M; a++) v[a] = y[i * M + a];
+         fwd_block<M>(A + i * MM, v);
 #pragma unroll
-      for (int i = 0; i < M; i++) {
-         r[i] = y[k * M + i];
-         if (k + 1 < nb)
-#pragma unroll
-            for (int t = 0; t < M; t++) r[i] -= Ls[k * MM + t * M + i] * xn[t];
+         for (int a = 0; a < M; a++) y[i * M + a] = v[a];
       }
-      const double* L = &Ld[k * MM];
-      double il[M];
+      __syncthreads();
+      const int nr = (nb + 2 * s - 1) / (2 * s);
+      for (int q = t; q < nr; q += NT) {
+         const int j = 2 * s * q;
+         double v[M];
 #pragma unroll
-      for (int i = 0; i < M; i++) il[i] = 1.0 / L[i * M + i];
+         for (int a = 0; a < M; a++) v[a] = y[j * M + a];
+         if (j + s < nb) {
+            const double* X = W + (j + s) * MM;
 #pragma unroll
-      for (int i = M - 1; i >= 0; i--) {
-         double t = r[i];
+            for (int a = 0; a < M; a++)
 #pragma unroll
-         for (int q = i + 1; q < M; q++) t -= L[q * M + i] * xk[q];
-         xk[i] = t * il[i];
+               for (int r = 0; r < M; r++) v[a] -= X[r * M + a] * y[(j + s) * M + r];
+         }
+         if (j >= s) {
+            const double* V = C + (j - s) * MM;
+#pragma unroll
+            for (int a = 0; a < M; a++)
+#pragma unroll
+               for (int r = 0; r < M; r++) v[a] -= V[r * M + a] * y[(j - s) * M + r];
+         }
+#pragma unroll
+         for (int a = 0; a < M; a++) y[j * M + a] = v[a];
       }
+      __syncthreads();
+   }
+   if (t == 0) {
+      double v[M];
 #pragma unroll
-      for (int i = 0; i < M; i++) {
-         x[k * M + i] = xk[i];
-         xn[i] = xk[i];
+      for (int a = 0; a < M; a++) v[a] = y[a];
+      fwd_block<M>(A, v);
+#pragma unroll
+      for (int a = 0; a < M; a++) y[a] = v[a];
+   }
+   __syncthreads();
+}
+
+// y := L^-T y with the factor of cr_factor.
+template <int M, int NT>
+__device__ void cr_bwd(int nb, const double* A, const double* C, const double* W, double* y)
+{
+   constexpr int MM = M * M;
+   const int t = threadIdx.x;
+   if (t == 0) {
+      double v[M];
+#pragma unroll
+      for (int a = 0; a < M; a++) v[a] = y[a];
+      bwd_block<M>(A, v);
+#pragma unroll
+      for (int a = 0; a < M; a++) y[a] = v[a];
+   }
+   __syncthreads();
+   int top = 1;
+   while (2 * top < nb) top <<= 1;
+   for (int s = nb > 1 ? top : 0; s >= 1; s >>= 1) {
+      const int ne = (nb + s - 1) / (2 * s);
+      for (int q = t; q < ne; q += NT) {
+         const int i = s * (2 * q + 1);
+         double v[M];
+         const double* X = W + i * MM;
+#pragma unroll
+         for (int a = 0; a < M; a++) {
+            double u = y[i * M + a];
+#pragma unroll
+            for (int r = 0; r < M; r++) u -= X[a * M + r] * y[(i - s) * M + r];
+            v[a] = u;
+         }
+         if (i + s < nb) {
+            const double* V = C + i * MM;
+#pragma unroll
+            for (int a = 0; a < M; a++)
+#pragma unroll
+               for (int r = 0; r < M; r++) v[a] -= V[a * M + r] * y[(i + s) * M + r];
+         }
+         bwd_block<M>(A + i * MM, v);
+#pragma unroll
+         for (int a = 0; a < M; a++) y[i * M + a] = v[a];
       }
+      __syncthreads();
    }
 }
-
-__device__ bool d_bt_chol(int nb, int m, const double* diag, const double* sub, double lambda,
-                          double* Ld, double* Ls)
-{
-   return m == 2 ? bt_chol<2>(nb, diag, sub, lambda, Ld, Ls) : bt_chol<3>(nb, diag, sub, lambda, Ld, Ls);
-}
-
 __device__ __forceinline__ double d_norm(int n, const double* v)
 {
    double s = 0;
@@ -434,8 +586,10 @@ __device__ double h_get(const RefDev& R, int i, int j)
 }
 
 // ---- JtJ assembly, factorisation and spectrum bound (one workgroup) --------------------
-__global__ void k_hess_assemble(RefDev R, const double* partials, double* Hdiag, double* Hsub,
-                                double* L0d, double* L0s, double* info)
+constexpr int kHessThreads = 256;
+
+__global__ __launch_bounds__(kHessThreads) void k_hess_assemble(RefDev R, const double* partials, double* Hdiag, double* Hsub,
+                                double* L0, double* info)
 {
    const int m = R.n_dim, mm = m * m;
    const int NA = m * (m + 1) / 2;
@@ -468,22 +622,33 @@ __global__ void k_hess_assemble(RefDev R, const double* partials, double* Hdiag,
       }
    }
    __syncthreads();
+   R.Hdiag = Hdiag;
+   R.Hsub = Hsub;
+   // Gershgorin lower bound of the spectrum: min over rows of diagonal - sum |off-diagonal|
+   __shared__ double s_min[kHessThreads / 64];
+   const int n = R.nD * m;
+   double lo = INFINITY;
+   for (int i = threadIdx.x; i < n; i += kHessThreads) {
+      double c = h_get(R, i, i), r = 0;
+      int j0 = (i / m - 1) * m, j1 = (i / m + 2) * m;
+      if (j0 < 0) j0 = 0;
+      if (j1 > n) j1 = n;
+      for (int j = j0; j < j1; j++)
+         if (j != i) r += fabs(h_get(R, i, j));
+      lo = fmin(lo, c - r);
+   }
+#pragma unroll
+   for (int off = 32; off >= 1; off >>= 1) lo = fmin(lo, __shfl_xor(lo, off, 64));
+   if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = lo;
+   __syncthreads();
+   const int64_t nbmm = (int64_t)R.nD * mm;
+   const bool pd = m == 2 ? cr_factor<2, kHessThreads>(R.nD, Hdiag, Hsub, 0.0, L0, L0 + nbmm, L0 + 2 * nbmm)
+                          : cr_factor<3, kHessThreads>(R.nD, Hdiag, Hsub, 0.0, L0, L0 + nbmm, L0 + 2 * nbmm);
    if (threadIdx.x == 0) {
-      R.Hdiag = Hdiag;
-      R.Hsub = Hsub;
-      const int n = R.nD * m;
-      double eig_lo = INFINITY;
-      for (int i = 0; i < n; i++) {
-         double c = h_get(R, i, i), r = 0;
-         int j0 = (i / m - 1) * m, j1 = (i / m + 2) * m;
-         if (j0 < 0) j0 = 0;
-         if (j1 > n) j1 = n;
-         for (int j = j0; j < j1; j++)
-            if (j != i) r += fabs(h_get(R, i, j));
-         if (c - r < eig_lo) eig_lo = c - r;
-      }
-      info[0] = d_bt_chol(R.nD, m, Hdiag, Hsub, 0.0, L0d, L0s) ? 1.0 : 0.0;
-      info[1] = eig_lo;
+      double e = s_min[0];
+      for (int w = 1; w < kHessThreads / 64; w++) e = fmin(e, s_min[w]);
+      info[0] = pd ? 1.0 : 0.0;
+      info[1] = e;
    }
 }
 
@@ -1001,14 +1166,14 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
 // The subproblem is Nocedal & Wright Alg. 4.3 on the block-tridiagonal H.
 //
 // One wave (64 lanes) per frame. Every lane runs the same scalar logic on wave-reduced values,
-// so the control flow is uniform; vector work is spread over the lanes. The block-triangular
-// solves are recurrences y_k = A_k y_(k-1) + c_k and are evaluated as a parallel prefix scan
-// of affine maps (composition is associative): each lane composes the maps of its segment of
-// blocks, the lanes scan the segment maps, and each lane replays its segment from the value
-// entering it. Only the block Cholesky of H + lambda I (lambda > 0) is sequential.
+// so the control flow is uniform; vector work is spread over the lanes. The factorisations of
+// H + lambda I and the triangular solves use block cyclic reduction (cr_factor, cr_fwd,
+// cr_bwd): ceil(log2 nD) + 1 parallel levels instead of a sequential sweep over the nodes.
 
 struct TrScratch {
-   double *x, *p, *g, *t1, *t2, *Ld, *Ls;
+   double *x, *p, *g;
+   double *y;             // solve vector of the subproblem
+   double *fa, *fc, *fw;  // factor blocks A, C, W of H + lambda I
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -1023,163 +1188,6 @@ __device__ __forceinline__ double wave_dot(int n, const double* a, const double*
    double s = 0;
    for (int i = threadIdx.x; i < n; i += 64) s += a[i] * b[i];
    return wave_sum(s);
-}
-
-template <int M>
-struct Aff {
-   double A[M][M];
-   double c[M];
-};
-
-template <int M>
-__device__ __forceinline__ Aff<M> aff_identity()
-{
-   Aff<M> F;
-#pragma unroll
-   for (int i = 0; i < M; i++) {
-      F.c[i] = 0;
-#pragma unroll
-      for (int j = 0; j < M; j++) F.A[i][j] = i == j ? 1.0 : 0.0;
-   }
-   return F;
-}
-
-// G after F: y -> G.A (F.A y + F.c) + G.c
-template <int M>
-__device__ __forceinline__ Aff<M> aff_then(const Aff<M>& F, const Aff<M>& G)
-{
-   Aff<M> H;
-#pragma unroll
-   for (int i = 0; i < M; i++) {
-      double c = G.c[i];
-#pragma unroll
-      for (int q = 0; q < M; q++) c += G.A[i][q] * F.c[q];
-      H.c[i] = c;
-#pragma unroll
-      for (int j = 0; j < M; j++) {
-         double a = 0;
-#pragma unroll
-         for (int q = 0; q < M; q++) a += G.A[i][q] * F.A[q][j];
-         H.A[i][j] = a;
-      }
-   }
-   return H;
-}
-
-template <int M>
-__device__ __forceinline__ Aff<M> aff_shfl_up(const Aff<M>& F, int d)
-{
-   Aff<M> O;
-#pragma unroll
-   for (int i = 0; i < M; i++) {
-      O.c[i] = __shfl_up(F.c[i], d, 64);
-#pragma unroll
-      for (int j = 0; j < M; j++) O.A[i][j] = __shfl_up(F.A[i][j], d, 64);
-   }
-   return O;
-}
-
-// Inverse of a lower-triangular block.
-template <int M>
-__device__ __forceinline__ void tri_inv(const double* L, double (&Li)[M][M])
-{
-#pragma unroll
-   for (int i = 0; i < M; i++) {
-      const double il = 1.0 / L[i * M + i];
-#pragma unroll
-      for (int j = 0; j < M; j++) {
-         double t = i == j ? 1.0 : 0.0;
-#pragma unroll
-         for (int q = 0; q < i; q++) t -= L[i * M + q] * Li[q][j];
-         Li[i][j] = j <= i ? t * il : 0.0;
-      }
-   }
-}
-
-// Forward (L y = b) or backward (L^T x = b) solve with the block factor (Ld, Ls) of a
-// block-tridiagonal matrix, across the wave. The caller synchronises before using out.
-template <int M, bool BWD>
-__device__ void bt_solve_scan(int nb, const double* Ld, const double* Ls, const double* b,
-                              double* out)
-{
-   constexpr int MM = M * M;
-   const int lane = threadIdx.x;
-   const int per = (nb + 63) / 64;
-   const int s0 = lane * per, s1 = min(nb, s0 + per);
-   // scan position s visits block k = s (forward) or nb-1-s (backward)
-   auto term = [&](int s, Aff<M>& F) {
-      const int k = BWD ? nb - 1 - s : s;
-      double Li[M][M];
-      tri_inv<M>(&Ld[k * MM], Li);
-      if (!BWD) {
-         // y_k = Li (b_k - L(k,k-1) y_(k-1))
-#pragma unroll
-         for (int i = 0; i < M; i++) {
-            double c = 0;
-#pragma unroll
-            for (int q = 0; q < M; q++) c += Li[i][q] * b[k * M + q];
-            F.c[i] = c;
-#pragma unroll
-            for (int j = 0; j < M; j++) {
-               double a = 0;
-               if (k > 0)
-#pragma unroll
-                  for (int q = 0; q < M; q++) a -= Li[i][q] * Ls[(k - 1) * MM + q * M + j];
-               F.A[i][j] = a;
-            }
-         }
-      } else {
-         // x_k = Li^T (b_k - L(k+1,k)^T x_(k+1))
-#pragma unroll
-         for (int i = 0; i < M; i++) {
-            double c = 0;
-#pragma unroll
-            for (int q = 0; q < M; q++) c += Li[q][i] * b[k * M + q];
-            F.c[i] = c;
-#pragma unroll
-            for (int j = 0; j < M; j++) {
-               double a = 0;
-               if (k + 1 < nb)
-#pragma unroll
-                  for (int q = 0; q < M; q++) a -= Li[q][i] * Ls[k * MM + j * M + q];
-               F.A[i][j] = a;
-            }
-         }
-      }
-   };
-   Aff<M> S = aff_identity<M>();
-   for (int sp = s0; sp < s1; sp++) {
-      Aff<M> F;
-      term(sp, F);
-      S = aff_then<M>(S, F);
-   }
-#pragma unroll
-   for (int off = 1; off < 64; off <<= 1) {
-      Aff<M> O = aff_shfl_up<M>(S, off);
-      if (lane >= off) S = aff_then<M>(O, S);
-   }
-   Aff<M> P = aff_shfl_up<M>(S, 1);
-   double y[M];
-#pragma unroll
-   for (int i = 0; i < M; i++) y[i] = lane > 0 ? P.c[i] : 0.0;
-   for (int sp = s0; sp < s1; sp++) {
-      Aff<M> F;
-      term(sp, F);
-      const int k = BWD ? nb - 1 - sp : sp;
-      double yn[M];
-#pragma unroll
-      for (int i = 0; i < M; i++) {
-         double t = F.c[i];
-#pragma unroll
-         for (int q = 0; q < M; q++) t += F.A[i][q] * y[q];
-         yn[i] = t;
-      }
-#pragma unroll
-      for (int i = 0; i < M; i++) {
-         out[k * M + i] = yn[i];
-         y[i] = yn[i];
-      }
-   }
 }
 
 // p' H p over the lanes.
@@ -1208,6 +1216,7 @@ template <int M>
 __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* g, double radius,
                              double eps, int max_iter, double* p, double* lam_out)
 {
+   constexpr int MM = M * M;
    const int nb = R.nD, n = nb * M;
    const double gnorm = sqrt(wave_dot(n, g, g));
    for (int i = threadIdx.x; i < n; i += 64) p[i] = 0;
@@ -1218,36 +1227,35 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
    double lam = 0, lam_lo = 0, lam_hi = gnorm / radius + (eig_lo < 0 ? -eig_lo : 0);
    int it;
    for (it = 0; it < max_iter; it++) {
-      const double *Ld, *Ls;
+      const double *A = nullptr, *C = nullptr, *W = nullptr;
       bool ok;
       if (lam == 0 && R.h_pd) {
-         Ld = R.L0d;
-         Ls = R.L0s;
+         A = R.L0;
+         C = R.L0 + (int64_t)nb * MM;
+         W = R.L0 + 2 * (int64_t)nb * MM;
          ok = true;
       } else if (lam == 0) {
-         Ld = Ls = nullptr;
          ok = false;
       } else {
-         int okl = 0;
-         if (threadIdx.x == 0) okl = bt_chol<M>(nb, R.Hdiag, R.Hsub, lam, S.Ld, S.Ls) ? 1 : 0;
-         ok = __shfl(okl, 0, 64) != 0;
-         __syncthreads();
-         Ld = S.Ld;
-         Ls = S.Ls;
+         ok = cr_factor<M, 64>(nb, R.Hdiag, R.Hsub, lam, S.fa, S.fc, S.fw);
+         A = S.fa;
+         C = S.fc;
+         W = S.fw;
       }
       if (!ok) {
          lam_lo = lam;
          lam = 0.5 * (lam_lo + lam_hi);
          continue;
       }
-      for (int i = threadIdx.x; i < n; i += 64) S.t2[i] = -g[i];
+      // (H + lam I) p = -g
+      double* y = S.y;
+      for (int i = threadIdx.x; i < n; i += 64) y[i] = -g[i];
       __syncthreads();
-      bt_solve_scan<M, false>(nb, Ld, Ls, S.t2, S.t1);
-      __syncthreads();
-      bt_solve_scan<M, true>(nb, Ld, Ls, S.t1, p);
-      __syncthreads();
+      cr_fwd<M, 64>(nb, A, C, W, y);
+      cr_bwd<M, 64>(nb, A, C, W, y);
+      for (int i = threadIdx.x; i < n; i += 64) p[i] = y[i];
       *lam_out = lam;
-      const double pn = sqrt(wave_dot(n, p, p));
+      const double pn = sqrt(wave_dot(n, y, y));
       if (lam == 0 && pn <= radius) {
          it++;
          break;
@@ -1258,13 +1266,14 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
       }
       if (pn > radius) lam_lo = lam;
       else lam_hi = lam;
-      bt_solve_scan<M, false>(nb, Ld, Ls, p, S.t1);
-      __syncthreads();
-      const double qn = sqrt(wave_dot(n, S.t1, S.t1));
+      // q = L^-1 p (y holds p)
+      cr_fwd<M, 64>(nb, A, C, W, y);
+      const double qn = sqrt(wave_dot(n, y, y));
       double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
       if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
       lam = lam_new;
    }
+   __syncthreads();
    return it;
 }
 
@@ -1276,11 +1285,12 @@ __device__ __forceinline__ TrScratch scratch_of(const BatchDev& B, const BatchFr
    S.x = B.x + bf.x_base;
    S.p = v;
    S.g = v + n;
-   S.t1 = v + 2 * n;
-   S.t2 = v + 3 * n;
-   S.Ld = B.fac + bf.fac_base;
+   S.y = v + 2 * n;
    const RefDev& R = B.refs[bf.ref];
-   S.Ls = S.Ld + (int64_t)R.nD * R.n_dim * R.n_dim;
+   const int64_t nbmm = (int64_t)R.nD * R.n_dim * R.n_dim;
+   S.fa = B.fac + bf.fac_base;
+   S.fc = S.fa + nbmm;
+   S.fw = S.fa + 2 * nbmm;
    return S;
 }
 
@@ -1354,34 +1364,16 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
    const BatchFrame& bf = B.frames[j];
    RefDev R = B.refs[bf.ref];
    const TrParamsDev& prm = B.prm;
-   const TrScratch G = scratch_of(B, bf);
-   TrScratch S = G;
+   TrScratch S = scratch_of(B, bf);
    const double* P = B.partials + bf.work_base * kPassStride;
    const int n = bf.n, nb = R.nD;
    if (LDS) {
-      // vectors, JtJ blocks and the factor scratch of lambda > 0 in LDS
-      double* q = smem;
-      S.x = q;
-      S.p = q + n;
-      S.g = q + 2 * n;
-      S.t1 = q + 3 * n;
-      S.t2 = q + 4 * n;
-      q += 5 * n;
-      double *hd = q, *hs = q + nb * MM;
-      S.Ld = q + 2 * nb * MM;
-      S.Ls = q + 3 * nb * MM;
-      for (int i = threadIdx.x; i < n; i += 64) {
-         S.x[i] = G.x[i];
-         S.p[i] = G.p[i];
-         S.g[i] = G.g[i];
-      }
-      for (int i = threadIdx.x; i < nb * MM; i += 64) {
-         hd[i] = R.Hdiag[i];
-         hs[i] = R.Hsub[i];
-      }
-      R.Hdiag = hd;
-      R.Hsub = hs;
-      __syncthreads();
+      // the factor blocks and the solve vector, which every level of the cyclic reduction
+      // reads and writes, in LDS; H, x, p, g stay in global memory
+      S.fa = smem;
+      S.fc = smem + nb * MM;
+      S.fw = smem + 2 * nb * MM;
+      S.y = smem + 3 * nb * MM;
    }
    const double f_new = reduce_ssd(R, P);
    TrState st = B.state[j];
@@ -1476,14 +1468,6 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
          for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < M ? S.x[i * M + c] + S.p[i * M + c] : 0.0;
    }
    if (threadIdx.x == 0) B.state[j] = st;
-   if (LDS) {
-      __syncthreads();
-      for (int i = threadIdx.x; i < n; i += 64) {
-         G.x[i] = S.x[i];
-         G.p[i] = S.p[i];
-         G.g[i] = S.g[i];
-      }
-   }
 }
 
 template <bool LDS>
@@ -1525,14 +1509,14 @@ hipError_t launch_reference_record(const float* T, int nz, int ny, int nx, int n
 }
 
 hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, double* Hsub,
-                          double* L0d, double* L0s, double* info, hipStream_t s)
+                          double* L0, double* info, hipStream_t s)
 {
    unsigned g = (unsigned)std::min<int64_t>(R.n_chunks, 65535);
    hipLaunchKernelGGL(k_hess_partials, dim3(g), dim3(kBlock), 0, s, R, partials);
    hipError_t e = hipGetLastError();
    if (e != hipSuccess) return e;
-   hipLaunchKernelGGL(k_hess_assemble, dim3(1), dim3(256), 0, s, R, (const double*)partials, Hdiag,
-                      Hsub, L0d, L0s, info);
+   hipLaunchKernelGGL(k_hess_assemble, dim3(1), dim3(kHessThreads), 0, s, R, (const double*)partials,
+                      Hdiag, Hsub, L0, info);
    return hipGetLastError();
 }
 
@@ -1576,6 +1560,13 @@ hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, 
 {
    hipLaunchKernelGGL(k_tr_init_slots, dim3(n), dim3(64), 0, s, B, slots);
    return hipGetLastError();
+}
+
+// Allows the LDS variant of the trust-region update up to max_lds_bytes of dynamic LDS.
+hipError_t configure_kernels(int max_lds_bytes)
+{
+   return hipFuncSetAttribute((const void*)k_tr_update<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              max_lds_bytes);
 }
 
 hipError_t launch_tr_update(const BatchDev& B, hipStream_t s)

@@ -12,7 +12,7 @@ hipError_t launch_interval_begin(const uint16_t* Di, int64_t n, int nD, int32_t*
 hipError_t launch_reference_record(const float* T, int nz, int ny, int nx, int n_dim,
                                    const ScanDev& sc, float4* rg4, float* df, hipStream_t s);
 hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, double* Hsub,
-                          double* L0d, double* L0s, double* info, hipStream_t s);
+                          double* L0, double* info, hipStream_t s);
 hipError_t launch_pass(const PassArgs& a, int n_dim, int grid, hipStream_t s);
 hipError_t launch_warp(const RefDev& R, const FrameDev& F, const double* D, float* out,
                        float invalid, int lds_cap, int use_lds, int grid, hipStream_t s);
@@ -22,6 +22,7 @@ hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s);
 hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s);
 hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, hipStream_t s);
 hipError_t launch_tr_update(const BatchDev& B, hipStream_t s);
+hipError_t configure_kernels(int max_lds_bytes);
 hipError_t launch_reduce_eval(const BatchDev& B, hipStream_t s);
 
 }  // namespace fa

@@ -52,7 +52,7 @@ class AlignStats(C.Structure):
 class Tuning(C.Structure):
     """Launch shape of the fused pass (fa_tuning in include/frame_aligner.h)."""
     _fields_ = [("lds_floats", C.c_int32), ("blocks_per_cu", C.c_int32), ("use_lds", C.c_int32),
-                ("check_every", C.c_int32), ("use_graph", C.c_int32)]
+                ("check_every", C.c_int32), ("use_graph", C.c_int32), ("tr_lds", C.c_int32)]
 
 
 class ServerResult(C.Structure):
@@ -71,7 +71,7 @@ EXPORTS = [
     "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_image", "fa_warp_point", "fa_align", "fa_batch_create",
     "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_get_tuning", "fa_batch_set_tuning",
     "fa_batch_run", "fa_batch_eval", "fa_align_batch",
-    "fa_context_stream", "fa_context_mem_info", "fa_server_create", "fa_server_destroy",
+    "fa_context_stream", "fa_context_mem_info", "fa_context_info", "fa_server_create", "fa_server_destroy",
     "fa_server_submit", "fa_server_submit_host", "fa_server_step", "fa_server_results",
     "fa_server_memory", "fa_host_alloc", "fa_host_free",
 ]
@@ -118,6 +118,7 @@ def load_library(path=None):
     L.fa_batch_eval.argtypes = [P, P, P, P]
     L.fa_align_batch.argtypes = [P, i32, C.POINTER(TrParams), P, P]
     L.fa_context_mem_info.argtypes = [P, P, P]
+    L.fa_context_info.argtypes = [P, P]
     L.fa_server_create.argtypes = [P, P, i32, i32, C.c_uint64, C.POINTER(TrParams), C.POINTER(P)]
     L.fa_server_submit_host.argtypes = [P, P, P, i32, C.c_float, P, C.c_int64]
     L.fa_server_memory.argtypes = [P, P, P]
@@ -197,6 +198,13 @@ class Context:
         f, t = C.c_uint64(), C.c_uint64()
         _check(self.L.fa_context_mem_info(self.h, C.byref(f), C.byref(t)))
         return f.value, t.value
+
+    def info(self):
+        """dict(device, n_cu, lds_per_block, wave_size, lds_config)."""
+        o = np.zeros(5, np.int32)
+        _check(self.L.fa_context_info(self.h, _ptr(o)))
+        return dict(device=int(o[0]), n_cu=int(o[1]), lds_per_block=int(o[2]), wave_size=int(o[3]),
+                    lds_config=int(o[4]))
 
 
 class Reference:

@@ -105,13 +105,17 @@ typedef struct fa_batch_timing {
    blocks_per_cu - persistent workgroups per compute unit;
    use_lds - 0 forces global loads; check_every - trust-region rounds between host polls;
    use_graph - 1 replays the check_every rounds as one captured hipGraph (runs with
-               fa_batch_set_timing on launch eagerly, to time each pass with HIP events). */
+               fa_batch_set_timing on launch eagerly, to time each pass with HIP events);
+   tr_lds - 1 runs the trust-region update with its factor blocks in LDS when the largest
+            frame's fit one workgroup's LDS, 0 keeps them in global memory; reading the
+            tuning back gives the effective choice. */
 typedef struct fa_tuning {
    int32_t lds_floats;
    int32_t blocks_per_cu;
    int32_t use_lds;
    int32_t check_every;
    int32_t use_graph;
+   int32_t tr_lds;
 } fa_tuning;
 
 const char* fa_last_error(void);
@@ -226,6 +230,10 @@ int fa_host_alloc(uint64_t bytes, void** out);
 int fa_host_free(void* p);
 /* Free and total device memory (hipMemGetInfo), for admission by free memory. */
 int fa_context_mem_info(fa_context* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* out[0..4] = device ordinal, compute units, bytes of LDS one workgroup may allocate,
+   wavefront size, hipError_t of raising the kernels' dynamic-LDS limit (0 = success). */
+int fa_context_info(const fa_context* ctx, int32_t* out5);
 
 /* The HIP stream the context launches on (hipStream_t), for callers that time it. */
 void* fa_context_stream(fa_context* ctx);

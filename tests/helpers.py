@@ -26,3 +26,20 @@ PARITY_CASES = [
     ("2d_interline", (1, 64, 72), 13, dict(line=1.0, interline=1.3), {}),
     ("3d", (6, 40, 36), 6 * 3 + 1, dict(line=1.0, interline=1.0), dict(n_blobs=120, sigma=(1.5, 4.0))),
 ]
+
+
+# Cases with many nodes: more node blocks than the 64 lanes of a trust-region wave take in one
+# level of the cyclic-reduction factorisation (nD = 150 in 2-D, 161 in 3-D). The sparse blob
+# phantom leaves some node intervals nearly featureless, so these cases start the trust region
+# at the generator's displacement, from where they are well posed (tests/test_parity_cases.py).
+MANY_NODE_CASES = [
+    ("2d_many_nodes", (1, 160, 64), 150, dict(line=1.0, interline=1.0),
+     dict(n_blobs=80, walk_sigma=0.05, sin_amp=1.5)),
+    ("3d_many_nodes", (4, 64, 64), 4 * 40 + 1, dict(line=1.0, interline=1.0),
+     dict(n_blobs=200, sigma=(1.5, 4.0), walk_sigma=0.05, sin_amp=1.5, z_amp=0.5)),
+]
+
+
+def truth_start(st, f, n_dim):
+    """Parameter vector (D2col layout) of frame f's generator displacement."""
+    return st["D_true"][f][:, :n_dim].reshape(-1).copy()

@@ -10,7 +10,7 @@ import pytest
 
 import oracle as O
 import fa_amd as F
-from helpers import PARITY_CASES, rel_err, small_stack
+from helpers import MANY_NODE_CASES, PARITY_CASES, rel_err, small_stack, truth_start
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +27,7 @@ def ctx():
 
 
 CASES = [c[:4] for c in PARITY_CASES]
-GEN = {c[0]: c[4] for c in PARITY_CASES}
+GEN = {c[0]: c[4] for c in PARITY_CASES + MANY_NODE_CASES}
 
 
 def _pair(ctx, shape, nD, scan, seed=3, name=None, **kw):
@@ -139,6 +139,23 @@ def test_align_matches_oracle(ctx, name, shape, nD, scan):
         assert abs(s_g["f_final"] - e_o) <= ERR_TOL * e_o
         assert abs(s_g["f_final"] - s_o["f_final"]) <= 10 * kw["rel_delta"] * s_o["f_final"]
         fr.release()
+
+
+@pytest.mark.parametrize("name,shape,nD,scan", [c[:4] for c in MANY_NODE_CASES])
+def test_align_many_nodes_matches_oracle(ctx, name, shape, nD, scan):
+    """More node blocks than lanes per level of the cyclic reduction (150 and 161 nodes)."""
+    st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
+    assert rel_err(ref.H, orc.hessian()) < JAC_TOL
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    frames = [ref.register(fw) for fw in st["frames_w"]]
+    x0 = [truth_start(st, f, ref.n_dim) for f in range(len(frames))]
+    batch = F.Batch(frames, **kw)
+    xs, stats, _ = batch.run(x0)
+    for f_i, fw in enumerate(st["frames_w"]):
+        x_o, s_o = orc.align(fw, x0=x0[f_i], **kw)
+        assert np.max(np.abs(xs[f_i] - x_o)) <= D_TOL, (np.max(np.abs(xs[f_i] - x_o)), s_o, stats[f_i])
+        x_g, s_g = frames[f_i].align(x0=x0[f_i], **kw)
+        assert np.array_equal(x_g, xs[f_i])
 
 
 def test_batch_same_alone_and_in_mix(ctx):
