@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=128)
     ap.add_argument("--serve-rounds", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape, no autotuning")
+    ap.add_argument("--tune-frames", type=int, default=0, help="frames the chunk-size tuning uses (0: all)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied device-to-device once (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -157,7 +159,7 @@ def main():
         sys.exit(2)
 
     import fa_amd as F
-    from fa_amd import synth
+    from fa_amd import synth, tune
 
     nD, shape = 31, (1, 512, 512)
     t0 = time.time()
@@ -168,32 +170,54 @@ def main():
     tr = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
 
     ctx = F.Context(0)
-    ref = F.Reference(ctx, ref_w, nD, chunk_px=args.chunk)
+    chunk = args.chunk
+    chunk_trials = None
+    if not chunk and not args.no_tune:
+        # work-item size (pixels per workgroup tile), then the launch shape, by the autotuner
+        nt = args.tune_frames or args.frames
+        chunk, chunk_trials = tune.tune_chunk(ctx, ref_w, nD, st["frames"][:nt], tr=tr, reps=5)
+        log(f"autotuner: chunk_px {chunk} ({ {k: round(v * 1e3, 3) for k, v in chunk_trials.items()} } ms)")
+    ref = F.Reference(ctx, ref_w, nD, chunk_px=chunk)
     frames = [ref.register(f) for f in st["frames"]]  # uint16 counts, +1 on the device
     batch = F.Batch(frames, **tr)
-    batch.set_timing(True)
+    if not args.no_tune:
+        best, best_t, trials = tune.tune_batch(batch, reps=5, sweeps=1)
+        log(f"autotuner: {best} ({best_t * 1e3:.3f} ms per run, {len(trials)} shapes tried)")
+    tuning = batch.tuning
     for _ in range(args.warmup):
         batch.run()
+    # timed region of `value`: K untimed steps (hipGraph replay when the tuner chose it)
     tot_wall = 0.0
+    for k in range(args.steps):
+        t = time.perf_counter()
+        xs, stats, _ = batch.run()
+        tot_wall += time.perf_counter() - t
+        log(f"step {k}: {1e3 * (time.perf_counter() - t):.2f} ms")
+    # the same steps again with HIP events around every fused pass (eager launches, since
+    # events inside a replayed graph cannot be read): per-pass kernel time and bytes
+    batch.set_timing(True)
     pass_ms = 0.0
     pass_bytes = 0.0
     launches = 0
     rounds = 0
+    timed_wall = 0.0
     for k in range(args.steps):
         t = time.perf_counter()
-        xs, stats, tm = batch.run()
-        tot_wall += time.perf_counter() - t
+        xs_t, _, tm = batch.run()
+        timed_wall += time.perf_counter() - t
         pass_ms += tm["pass_ms"]
         pass_bytes += tm["pass_bytes"]
         launches += tm["pass_launches"]
         rounds += tm["tr_rounds"]
-        log(f"step {k}: {tm['wall_ms']:.1f} ms, {tm['pass_launches']} passes, "
+        log(f"timed step {k}: {tm['wall_ms']:.1f} ms, {tm['pass_launches']} passes, "
             f"pass {tm['pass_ms']:.2f} ms")
+    batch.set_timing(False)
     n = args.frames
     fps = n * args.steps / tot_wall
     ms_per_step = 1e3 * tot_wall / args.steps
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     evals = [s["evaluations"] for s in stats]
+    same_timed = all(np.array_equal(a, b) for a, b in zip(xs, xs_t))
 
     # plain path: one frame per launch sequence
     lat = []
@@ -229,7 +253,10 @@ def main():
         "data": "synthetic (seeded blob phantom, Poisson counts; no dataset)",
         "config": {"workload": "512x512 uint16, 200 frames, 30 resampling points (nD=31), "
                                "TR to 1e-4 rel or 50 iterations, batched",
-                   "frames": n, "nD": nD, "shape": list(shape), "chunk_px": ref.n_chunks and args.chunk},
+                   "frames": n, "nD": nD, "shape": list(shape), "chunk_px": chunk,
+                   "tuning": tuning, "tuned": not args.no_tune},
+        "timed_ms_per_step": round(1e3 * timed_wall / args.steps, 3),
+        "timed_same_result": same_timed,
         "mpx_per_s": round(fps * 512 * 512 / 1e6, 1),
         "evaluations_per_frame": {"mean": float(np.mean(evals)), "max": int(np.max(evals))},
         "plain": {"frames_per_s": round(plain_fps, 2), "frames": npl,
@@ -241,6 +268,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(),
                      "kernel": "k_pass (fused warp + SSD + Jtr)",
+                     "measured": f"HIP events around every fused pass of {args.steps} more steps of "
+                                 f"the same workload (eager launches)",
                      "launches": launches, "avg_launch_us": round(1e3 * pass_ms / max(launches, 1), 2),
                      "bytes_per_launch": pass_bytes / max(launches, 1),
                      "isolated": {"avg_launch_us": round(1e3 * iso_ms, 2), "bytes_per_launch": iso_bytes,

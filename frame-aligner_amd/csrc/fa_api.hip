@@ -61,7 +61,7 @@ void dfree(T*& p)
 }
 
 constexpr int kLdsCapFloats = 15360;  // 60 KB of staged frame rows per workgroup at most
-constexpr int kBlocksPerCu = 8;
+constexpr int kBlocksPerCu = 64;  // pass grid cap per compute unit: about one workgroup per work item
 
 // ImageScanParameters ctor (ImageScanParameters.h:7-20) then processScanParameters
 // (FrameWarper.cpp:223-236).

@@ -308,6 +308,7 @@ class Frame:
         h = C.c_void_p()
         _check(self.L.fa_register_frame(ref.h, _ptr(a), dt, float(off), C.byref(h)))
         self.h = h
+        self.dtype = dt
         ref.ctx._adopt(self)
 
     def _release_handle(self):
@@ -391,6 +392,7 @@ class Batch:
 
     def set_timing(self, on=True):
         _check(self.L.fa_batch_set_timing(self.h, 1 if on else 0))
+        self._timing = bool(on)
 
     @property
     def tuning(self):
