@@ -201,6 +201,15 @@ struct PixelWalk {
       x += kBlock;
       while (x >= nx) { x -= nx; y++; }
    }
+   // The same step without a loop: kBlock = sdy rows + sdx columns with sdx < nx.
+   __device__ __forceinline__ void step(int nx, int sdx, int sdy)
+   {
+      x += sdx;
+      y += sdy;
+      const bool wrap = x >= nx;
+      x = wrap ? x - nx : x;
+      y = wrap ? y + 1 : y;
+   }
 };
 
 // ---- JtJ partial sums (FrameWarper.cpp:296-395) -----------------------------------------
@@ -838,8 +847,11 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    for (int v = 0; v < NV; v++) acc[v] = KF{0.f, 0.f};
 
    // Each thread takes kUnroll pixels per step (kBlock apart) and issues their reference
-   // records before using any, so several loads are in flight per wave.
+   // records before using any, so several loads are in flight per wave. Pixels past the end
+   // of the work item are computed on clamped data and masked, so the loop body has no
+   // divergent branches.
    constexpr int kUnroll = 4;
+   const int sdy = kBlock / nx, sdx = kBlock - sdy * nx;
    PixelWalk w;
    w.init(q0 + threadIdx.x, nx);
    for (int pb = ch.p0 + threadIdx.x; pb < ch.p1; pb += kUnroll * kBlock) {
@@ -857,65 +869,66 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          fs[u] = (M == 3) ? R.df[pu] : recs[u].w;
       }
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++, w.step(nx)) {
-      const int p = pb + u * kBlock;
-      if (p >= ch.p1) break;
-      const float4 rec = recs[u];
-      const float f = fs[u];
-      int ti[3];
-      float wt[3];
+      for (int u = 0; u < kUnroll; u++, w.step(nx, sdx, sdy)) {
+         const int p = pb + u * kBlock;
+         const bool in = p < ch.p1;
+         const float4 rec = recs[u];
+         const float f = fs[u];
+         int ti[3];
+         float wt[3];
 #pragma unroll
-      for (int c = 0; c < M; c++) {
-         const float d = fmaf(f, Dd[c], Da[c]);
-         const float fl = floorf(d);
-         ti[c] = (int)fl;
-         wt[c] = d - fl;
-      }
-      const int x0 = w.x + ti[0], y0 = w.y + ti[1];
-      const float wx1 = wt[0], wx0 = 1.f - wt[0];
-      const float wy1 = wt[1], wy0 = 1.f - wt[1];
-      float v = 0.f;
-      if (use) {
-         int idx = (y0 - S.ya) * S.stride + x0 + kPad;
-         if (M == 3) idx += (plane + ti[2] - S.za) * S.nrows * S.stride;
-         idx = min(max(idx, 0), lds_lim);
-         const float* L = lds + idx;
-         const float v0 = wy0 * (wx0 * L[0] + wx1 * L[1]) + wy1 * (wx0 * L[S.stride] + wx1 * L[S.stride + 1]);
-         if (M == 3) {
-            const float* L2 = L + S.nrows * S.stride;
-            const float v1 = wy0 * (wx0 * L2[0] + wx1 * L2[1]) + wy1 * (wx0 * L2[S.stride] + wx1 * L2[S.stride + 1]);
-            v = (1.f - wt[2]) * v0 + wt[2] * v1;
-         } else {
-            v = v0;
+         for (int c = 0; c < M; c++) {
+            const float d = fmaf(f, Dd[c], Da[c]);
+            const float fl = floorf(d);
+            ti[c] = (int)fl;
+            wt[c] = d - fl;
          }
-      } else if (finite) {
-         const int z0 = M == 3 ? plane + ti[2] : 0;
-         const float v0 = wy0 * (wx0 * tap_g(z0, y0, x0) + wx1 * tap_g(z0, y0, x0 + 1)) +
-                          wy1 * (wx0 * tap_g(z0, y0 + 1, x0) + wx1 * tap_g(z0, y0 + 1, x0 + 1));
-         if (M == 3) {
-            const float v1 = wy0 * (wx0 * tap_g(z0 + 1, y0, x0) + wx1 * tap_g(z0 + 1, y0, x0 + 1)) +
-                             wy1 * (wx0 * tap_g(z0 + 1, y0 + 1, x0) + wx1 * tap_g(z0 + 1, y0 + 1, x0 + 1));
-            v = (1.f - wt[2]) * v0 + wt[2] * v1;
-         } else {
-            v = v0;
+         const int x0 = w.x + ti[0], y0 = w.y + ti[1];
+         const float wx1 = wt[0], wx0 = 1.f - wt[0];
+         const float wy1 = wt[1], wy0 = 1.f - wt[1];
+         float v = 0.f;
+         if (use) {
+            int idx = (y0 - S.ya) * S.stride + x0 + kPad;
+            if (M == 3) idx += (plane + ti[2] - S.za) * S.nrows * S.stride;
+            idx = min(max(idx, 0), lds_lim);
+            const float* L = lds + idx;
+            const float v0 = wy0 * (wx0 * L[0] + wx1 * L[1]) + wy1 * (wx0 * L[S.stride] + wx1 * L[S.stride + 1]);
+            if (M == 3) {
+               const float* L2 = L + S.nrows * S.stride;
+               const float v1 = wy0 * (wx0 * L2[0] + wx1 * L2[1]) + wy1 * (wx0 * L2[S.stride] + wx1 * L2[S.stride + 1]);
+               v = (1.f - wt[2]) * v0 + wt[2] * v1;
+            } else {
+               v = v0;
+            }
+         } else if (finite) {
+            const int z0 = M == 3 ? plane + ti[2] : 0;
+            const float v0 = wy0 * (wx0 * tap_g(z0, y0, x0) + wx1 * tap_g(z0, y0, x0 + 1)) +
+                             wy1 * (wx0 * tap_g(z0, y0 + 1, x0) + wx1 * tap_g(z0, y0 + 1, x0 + 1));
+            if (M == 3) {
+               const float v1 = wy0 * (wx0 * tap_g(z0 + 1, y0, x0) + wx1 * tap_g(z0 + 1, y0, x0 + 1)) +
+                                wy1 * (wx0 * tap_g(z0 + 1, y0 + 1, x0) + wx1 * tap_g(z0 + 1, y0 + 1, x0 + 1));
+               v = (1.f - wt[2]) * v0 + wt[2] * v1;
+            } else {
+               v = v0;
+            }
          }
-      }
-      if (MODE == MODE_WARP) {
-         wout[p] = (v == 0.f) ? invalid : v;
-         continue;
-      }
-      if (v != 0.f) {  // no sample: residual 0, outside the SSD
-         const float e = v - rec.x;
-         part[0] += e * e;
-         const float ej = (p != ch.end_p) ? e : 0.f;  // the interval's last pixel has no row
+         if (MODE == MODE_WARP) {
+            if (in) wout[p] = (v == 0.f) ? invalid : v;
+            continue;
+         }
+         // no sample (v == 0): residual 0, outside the SSD; the interval's last pixel has no
+         // Jacobian row
+         const float e = (in && v != 0.f) ? v - rec.x : 0.f;
+         const float ej = (p != ch.end_p) ? e : 0.f;
+         part[0] = fmaf(e, e, part[0]);
          const float omff = 1.f - f;
          const float g[3] = {rec.y, rec.z, rec.w};
 #pragma unroll
          for (int c = 0; c < M; c++) {
-            part[1 + c] += (g[c] * omff) * ej;
-            part[1 + M + c] += (g[c] * f) * ej;
+            const float ge = g[c] * ej;
+            part[1 + c] = fmaf(ge, omff, part[1 + c]);
+            part[1 + M + c] = fmaf(ge, f, part[1 + M + c]);
          }
-      }
       }
       if (MODE == MODE_PASS)
 #pragma unroll
