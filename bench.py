@@ -131,34 +131,148 @@ def serving(F, args, ctx):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--frames", type=int, default=200)
-    ap.add_argument("--plain-frames", type=int, default=40)
-    ap.add_argument("--cpu-frames", type=int, default=200)
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--seed", type=int, default=20260)
-    ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--serve-seconds", type=float, default=2.0)
-    ap.add_argument("--serve-slots", type=int, default=128)
-    ap.add_argument("--serve-rounds", type=int, default=2)
-    ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="default launch shape, no autotuning")
-    ap.add_argument("--tune-frames", type=int, default=0, help="frames the chunk-size tuning uses (0: all)")
-    ap.add_argument("--copy-probe", type=int, default=0,
-                    help="MiB copied device-to-device once (calibrates rocprof byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames are "
-              f"independent, see the sharding note in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def timed_steps(batch, steps):
+    """`steps` untimed runs (graph replay where tuned), then `steps` runs with HIP events around
+    every fused pass. Returns (results, stats, wall s per run, pass ms, pass bytes, launches)."""
+    t = time.perf_counter()
+    for _ in range(steps):
+        xs, stats, _ = batch.run()
+    wall = (time.perf_counter() - t) / steps
+    batch.set_timing(True)
+    pms = pb = nl = 0
+    for _ in range(steps):
+        _, _, tm = batch.run()
+        pms += tm["pass_ms"]
+        pb += tm["pass_bytes"]
+        nl += tm["pass_launches"]
+    batch.set_timing(False)
+    return xs, stats, wall, pms, pb, nl
 
-    import fa_amd as F
+
+def roofline_of(pass_ms, pass_bytes, launches, kernel):
+    gbs = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": kernel, "launches": int(launches),
+            "avg_launch_us": round(1e3 * pass_ms / max(launches, 1), 2)}
+
+
+def precision_3d(F, args, ctx):
+    """Precision form (3-D): 256x256x32 float32 volumes (1500-blob phantom, sigma 1.5-4 voxels,
+    Poisson noise), 16 resampling points per plane (nD = 16*32+1 = 513), 3 components (z within
+    2 voxels); FP32 compensated pass sums, FP64 cross-block reductions and block-tridiagonal
+    solves. Frames are resampled with trilinear interpolation (order 1) to bound the generator's
+    time. Reported: volumes/s of the batched run, the 3-D pass roofline, and per result the
+    largest displacement difference to the FP64 oracle and to the float-storage oracle on a
+    sample of volumes (the oracle's time on that sample is the CPU baseline)."""
+    from fa_amd import synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    cfg = synth.CONFIGS["c3"]
+    t0 = time.time()
+    st = synth.make_stack(cfg["shape"], args.c3_frames, cfg["nD"], seed=args.seed + 3, n_blobs=cfg["n_blobs"],
+                          sigma=cfg["sigma"], dtype="float32", order=1)
+    log(f"precision_3d: generated {args.c3_frames} volumes in {time.time() - t0:.1f}s")
+    ref_w = synth.with_offset(st["reference"])
+    tr = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    ref = F.Reference(ctx, ref_w, cfg["nD"])
+    frames = [ref.register(synth.with_offset(f)) for f in st["frames"]]
+    b = F.Batch(frames, **tr)
+    b.run()
+    xs, stats, wall, pms, pb, nl = timed_steps(b, args.steps)
+    n = len(frames)
+    out = {"workload": f"256x256x32 float32, {n} volumes, nD=513 x 3 components, TR to 1e-4 rel or 50 "
+                       f"iterations, batched", "volumes": n,
+           "volumes_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 3),
+           "mvox_per_s": round(n * ref.n_px / wall / 1e6, 1),
+           "evaluations_per_volume": {"mean": float(np.mean([s_["evaluations"] for s_ in stats])),
+                                      "max": int(np.max([s_["evaluations"] for s_ in stats]))},
+           "roofline": roofline_of(pms, pb, nl, "k_pass<3> (fused trilinear warp + SSD + Jtr)")}
+    if not args.no_cpu:
+        k = min(args.c3_oracle, n)
+        fw = np.stack([synth.with_offset(f) for f in st["frames"][:k]])
+        per = []
+        for fp64 in (True, False):
+            orc = O.Oracle(fp64=fp64).set_reference(ref_w, cfg["nD"])
+            t = time.perf_counter()
+            x_o, s_o, secs = orc.align_many(fw, threads=min(k, args.cpu_threads), **tr)
+            wall_o = time.perf_counter() - t
+            errs = [float(np.max(np.abs(xs[i] - x_o[i]))) for i in range(k)]
+            per.append(errs)
+            if fp64:
+                out["cpu_baseline"] = {"value": round(k / wall_o, 3), "unit": "volumes/s", "cores": min(k, args.cpu_threads),
+                                       "kind": "port", "oracle": "fp64",
+                                       "sample": f"first {k} of the {n} volumes, one volume per thread, "
+                                                 f"{wall_o:.1f}s wall, {secs.sum():.1f}s CPU"}
+        out["max_disp_err_px_per_result"] = {"vs_fp64_oracle": [round(e, 6) for e in per[0]],
+                                             "vs_float_oracle": [round(e, 6) for e in per[1]]}
+    b.close()
+    for f in frames:
+        f.release()
+    ref.release()
+    return out
+
+
+def uneven_batch(F, args, ctx):
+    """Batched form: 64 independent stacks of uneven size in one batch (hipGraph-replayed
+    trust-region rounds, launch shape from the autotuner). Sizes uniform over {128^2, 256^2,
+    512^2, 1024^2} px x {1, 4, 16} planes, 10-40 resampling points per plane (nD = n * planes + 1),
+    20-100 frames per stack, float32 blob phantoms with a seeded per-stack blob density. Each
+    frame is the stack's noise-free phantom pulled through its own node displacement (the
+    generator's motion model), computed with the library's interpolated warp on the device."""
+    from fa_amd import synth, tune
+    rng = np.random.default_rng(args.seed + 4)
+    t0 = time.time()
+    stacks, refs, frames = [], [], []
+    tr = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    for i in range(args.c4_stacks):
+        side = int(rng.choice([128, 256, 512, 1024]))
+        nz = int(rng.choice([1, 4, 16]))
+        n_res = int(rng.integers(10, 41))
+        nD = n_res * nz + 1
+        n_frames = int(rng.integers(20, 101))
+        shape = (nz, side, side)
+        density = float(rng.uniform(300, 900))  # blobs per 512^2 px area per plane group
+        n_blobs = max(20, int(density * side * side / 512 ** 2 * max(1.0, nz / 4)))
+        srng = np.random.default_rng(rng.integers(1 << 62))
+        ph = synth.blob_phantom(shape, n_blobs, (2.0, 6.0) if nz == 1 else (1.5, 4.0), rng=srng)
+        ph32 = synth.with_offset(ph)
+        ref = F.Reference(ctx, ph32, nD)
+        src = ref.register(ph32)  # the phantom as a frame, to pull-sample it through -D
+        bg = float(np.median(ph32))
+        n_dim = 3 if nz > 1 else 2
+        for _ in range(n_frames):
+            D = synth.node_displacements(nD, n_dim, srng)
+            frames.append(ref.register(src.warp_interpolated(-D, invalid_value=bg)))
+        src.release()
+        refs.append(ref)
+        stacks.append(f"{side}x{side}x{nz}/{nD}n/{n_frames}f")
+    n = len(frames)
+    vox = sum(f.ref.n_px for f in frames)
+    log(f"uneven_batch: {len(stacks)} stacks, {n} frames, {vox / 1e9:.2f} Gvoxel, generated in {time.time() - t0:.1f}s")
+    b = F.Batch(frames, **tr)
+    best = None
+    if not args.no_tune:
+        best, _, trials = tune.tune_batch(b, reps=2, sweeps=1)
+        log(f"uneven_batch: autotuner {best} ({len(trials)} shapes)")
+    b.run()
+    xs, stats, wall, pms, pb, nl = timed_steps(b, max(1, args.steps - 1))
+    out = {"workload": f"{len(stacks)} stacks, {n} frames, {vox / 1e9:.2f} Gvoxel per pass over all frames",
+           "stacks": stacks, "frames": n, "frames_per_s": round(n / wall, 1), "ms_per_step": round(1e3 * wall, 2),
+           "mvox_per_s": round(vox / wall / 1e6, 1), "tuning": b.tuning,
+           "evaluations_per_frame": {"mean": float(np.mean([s_["evaluations"] for s_ in stats])),
+                                     "max": int(np.max([s_["evaluations"] for s_ in stats]))},
+           "roofline": roofline_of(pms, pb, nl, "k_pass<2>, k_pass<3>")}
+    b.close()
+    for f in frames:
+        f.release()
+    for r in refs:
+        r.release()
+    return out
+
+
+def headline(F, args, ctx):
+    """The 512x512 uint16, 200-frame, nD = 31 workload: batched frames/s (the `value`), the pass
+    roofline, the plain one-frame-per-launch path and the CPU oracle beside it."""
     from fa_amd import synth, tune
 
     nD, shape = 31, (1, 512, 512)
@@ -169,7 +283,6 @@ def main():
     ref_w = synth.with_offset(st["reference"])
     tr = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
 
-    ctx = F.Context(0)
     chunk = args.chunk
     chunk_trials = None
     if not chunk and not args.no_tune:
@@ -276,14 +389,6 @@ def main():
                                   "achieved": round(iso_bytes / (iso_ms * 1e-3) / 1e9, 1)}},
     }
 
-    if args.copy_probe:
-        out["copy_probe_mib"] = copy_probe(args.copy_probe)
-
-    if not args.no_serve:
-        out["serving"] = serving(F, args, ctx)
-        log(f"serving: {out['serving']['frames_per_s']} frames/s, p50 {out['serving']['latency_ms']['p50']} ms, "
-            f"p99 {out['serving']['latency_ms']['p99']} ms")
-
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -303,6 +408,65 @@ def main():
         out["max_disp_err_px_vs_oracle"] = err
         out["cpu_baseline"]["evaluations_per_frame"] = {"mean": float(np.mean([d["evaluations"] for d in s_o])),
                                                         "max": int(np.max([d["evaluations"] for d in s_o]))}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=200)
+    ap.add_argument("--plain-frames", type=int, default=40)
+    ap.add_argument("--cpu-frames", type=int, default=200)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-slots", type=int, default=128)
+    ap.add_argument("--serve-rounds", type=int, default=2)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape, no autotuning")
+    ap.add_argument("--sections", default="batch,serve,precision,uneven",
+                    help="comma list of: batch (the headline 512^2 workload), serve, precision (3-D), "
+                         "uneven (64 stacks)")
+    ap.add_argument("--c3-frames", type=int, default=50)
+    ap.add_argument("--c3-oracle", type=int, default=4, help="volumes checked against the oracles")
+    ap.add_argument("--c4-stacks", type=int, default=64)
+    ap.add_argument("--tune-frames", type=int, default=0, help="frames the chunk-size tuning uses (0: all)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB copied device-to-device once (calibrates rocprof byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames are "
+              f"independent, see the sharding note in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    secs = set(args.sections.split(","))
+    import fa_amd as F
+    ctx = F.Context(0)
+    if "batch" in secs:
+        out = headline(F, args, ctx)
+    else:
+        out = {"metric": METRIC, "value": None, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f32", "data": "synthetic", "config": {"workload": "sections " + args.sections}}
+
+    if args.copy_probe:
+        out["copy_probe_mib"] = copy_probe(args.copy_probe)
+
+    if "serve" in secs and not args.no_serve:
+        out["serving"] = serving(F, args, ctx)
+        log(f"serving: {out['serving']['frames_per_s']} frames/s, p50 {out['serving']['latency_ms']['p50']} ms, "
+            f"p99 {out['serving']['latency_ms']['p99']} ms")
+    if "precision" in secs:
+        out["precision_3d"] = precision_3d(F, args, ctx)
+        log(f"precision_3d: {out['precision_3d']['volumes_per_s']} volumes/s")
+    if "uneven" in secs:
+        out["batched_uneven"] = uneven_batch(F, args, ctx)
+        log(f"batched_uneven: {out['batched_uneven']['frames_per_s']} frames/s")
     print(json.dumps(out), flush=True)
 
 

@@ -122,14 +122,23 @@ struct fa_batch {
    std::vector<BatchFrame> hframes;
    std::vector<int64_t> x_off;  // offset of each frame's parameter vector in x_inout
    int64_t total_work = 0, total_x = 0, total_vec = 0, total_fac = 0, total_D = 0;
-   int grid2 = 0, grid3 = 0;  // pass grids for the 2-D and 3-D frames (0: none of that kind)
-   int64_t work2 = 0, work3 = 0;
+   // Launch classes of the fused pass: references of one kind with a similar LDS need (the
+   // batch's references are ordered by kind, then need), each launched with its own LDS size.
+   struct PassClass {
+      int n_dim = 2, ref_lo = 0, ref_hi = 0;
+      int64_t lds_first = 0, lds = 0;  // LDS floats of the class's first and largest reference
+      int64_t work = 0;                // work items with every frame active
+      int grid = 0;
+   };
+   std::vector<PassClass> pcls;
+   // Launch classes of the trust-region update: LDS doubles of each (0: global memory) and
+   // the class of each reference.
+   std::vector<int> trc_lds;
+   std::vector<int> ref_trc;
    fa_tuning tune{};
    int timing = 0;
-   int tr_lds = 0;  // doubles of LDS the update's factor blocks need when they fit, else 0
    std::vector<int> user;  // internal index -> caller's index
    int n_slots = 0;        // serving batches: number of slots (frames are admitted into them)
-   int64_t lds_need[2] = {0, 0};  // LDS floats the 2-D / 3-D frames' work items stage
    RefDev* d_refs = nullptr;
    BatchFrame* d_frames = nullptr;
    TrState* d_state = nullptr;
@@ -176,7 +185,6 @@ struct fa_batch {
       B.prm.sub_max_iter = prm.sub_max_iter;
       B.prm.skip_repeat = prm.skip_repeat;
       B.n_frames = n_slots > 0 ? n_slots : (int32_t)frames.size();
-      B.tr_lds = tune.tr_lds ? tr_lds : 0;
       return B;
    }
 
@@ -195,6 +203,8 @@ struct fa_batch {
       a.counters = d_counters;
       a.lds_cap = tune.lds_floats;
       a.use_lds = tune.use_lds;
+      a.ref_lo = 0;
+      a.ref_hi = n_refs;
       return a;
    }
 
@@ -275,11 +285,62 @@ void default_tuning(fa_tuning* t, const std::vector<fa_frame*>& frames)
    t->tr_lds = 1;
 }
 
+// Launch order of references: 2-D before 3-D, then by the LDS their work items stage.
+void order_refs(std::vector<fa_reference*>& rs)
+{
+   std::stable_sort(rs.begin(), rs.end(), [](const fa_reference* a, const fa_reference* c) {
+      if (a->n_dim != c->n_dim) return a->n_dim < c->n_dim;
+      return lds_floats_for(a) < lds_floats_for(c);
+   });
+}
+
+// Launch classes over the ordered references rs. Pass: a class ends at a change of kind or
+// before a reference that stages more than 1.5x the LDS of the class's first. Update: the
+// references' factor sizes, a class starting above 2x its first. ref_items: each reference's
+// work items with every frame of the batch active.
+void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::vector<int64_t>& ref_items)
+{
+   b->pcls.clear();
+   for (int i = 0; i < (int)rs.size(); i++) {
+      const int64_t need = lds_floats_for(rs[i]);
+      if (b->pcls.empty() || b->pcls.back().n_dim != rs[i]->n_dim || 2 * need > 3 * b->pcls.back().lds_first) {
+         fa_batch::PassClass c;
+         c.n_dim = rs[i]->n_dim;
+         c.ref_lo = i;
+         c.lds_first = need;
+         b->pcls.push_back(c);
+      }
+      fa_batch::PassClass& c = b->pcls.back();
+      c.ref_hi = i + 1;
+      c.lds = std::max(c.lds, need);
+      c.work += ref_items[i];
+   }
+   std::vector<int> order(rs.size());
+   for (size_t i = 0; i < rs.size(); i++) order[i] = (int)i;
+   std::stable_sort(order.begin(), order.end(),
+                    [&](int a, int c) { return tr_lds_need(rs[a]) < tr_lds_need(rs[c]); });
+   b->ref_trc.assign(rs.size(), 0);
+   b->trc_lds.clear();
+   int first = 0, largest = 0;
+   for (int i : order) {
+      const int need = tr_lds_need(rs[i]);
+      if (b->trc_lds.empty() || need > 2 * first) {
+         b->trc_lds.push_back(0);
+         first = need;
+         largest = 0;
+      }
+      largest = std::max(largest, need);
+      b->trc_lds.back() = tr_lds_for(b->ctx, largest);
+      b->ref_trc[i] = (int)b->trc_lds.size() - 1;
+   }
+}
+
+// Grids of the pass classes: about one workgroup per work item, at most blocks_per_cu per
+// compute unit.
 void set_grids(fa_batch* b)
 {
    const int64_t cap = (int64_t)b->ctx->n_cu * std::max(1, b->tune.blocks_per_cu);
-   b->grid2 = b->work2 ? (int)std::max<int64_t>(1, std::min(b->work2, cap)) : 0;
-   b->grid3 = b->work3 ? (int)std::max<int64_t>(1, std::min(b->work3, cap)) : 0;
+   for (auto& c : b->pcls) c.grid = c.work ? (int)std::max<int64_t>(1, std::min(c.work, cap)) : 0;
 }
 
 int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out)
@@ -315,22 +376,30 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
    for (int i = 0; i < n; i++)
       if (frames[i]->ref->n_dim == 3) b->user.push_back(i);
 
+   std::vector<fa_reference*> rs;
+   for (int j = 0; j < n; j++) {
+      fa_reference* r = frames[b->user[j]]->ref;
+      if (std::find(rs.begin(), rs.end(), r) == rs.end()) rs.push_back(r);
+   }
+   if ((int)rs.size() > kMaxRefs) {
+      delete b;
+      return set_err(FA_E_ARG, "too many distinct references in one batch (at most 4096)");
+   }
+   order_refs(rs);
    std::map<fa_reference*, int> ref_idx;
    std::vector<RefDev> refs;
+   for (size_t ri = 0; ri < rs.size(); ri++) {
+      ref_idx[rs[ri]] = (int)ri;
+      refs.push_back(rs[ri]->dev);
+   }
+   std::vector<int64_t> ref_items(rs.size(), 0);
+   for (int j = 0; j < n; j++) ref_items[ref_idx[frames[b->user[j]]->ref]] += (int64_t)frames[b->user[j]]->ref->chunks.size();
+   make_classes(b, rs, ref_items);
    b->hframes.resize(n);
-   int tr_need = 0;
    for (int j = 0; j < n; j++) {
       const int i = b->user[j];
       fa_reference* r = frames[i]->ref;
-      auto it = ref_idx.find(r);
-      int ri;
-      if (it == ref_idx.end()) {
-         ri = (int)refs.size();
-         ref_idx[r] = ri;
-         refs.push_back(r->dev);
-      } else {
-         ri = it->second;
-      }
+      const int ri = ref_idx[r];
       BatchFrame& bf = b->hframes[j];
       bf.f.data = frames[i]->d_data;
       bf.f.dtype = frames[i]->dtype;
@@ -342,20 +411,12 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       bf.vec_base = b->total_vec;
       bf.fac_base = b->total_fac;
       bf.d_base = d_off[i];
+      bf.trc = b->ref_trc[ri];
       b->total_work += (int64_t)r->chunks.size();
-      (r->n_dim == 2 ? b->work2 : b->work3) += (int64_t)r->chunks.size();
-      int64_t& lds = b->lds_need[r->n_dim == 3];
-      lds = std::max(lds, lds_floats_for(r));
       b->total_vec += 3 * (int64_t)bf.n;
       b->total_fac += 3 * (int64_t)r->nD * r->n_dim * r->n_dim;
-      tr_need = std::max(tr_need, tr_lds_need(r));
    }
-   b->tr_lds = tr_lds_for(ctx, tr_need);
    b->n_refs = (int)refs.size();
-   if (b->n_refs > kMaxRefs) {
-      delete b;
-      return set_err(FA_E_ARG, "too many distinct references in one batch (at most 4096)");
-   }
    int rc = FA_OK;
    auto fail = [&](int code) {
       delete b;
@@ -402,14 +463,22 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
 int launch_passes(fa_batch* b, hipStream_t s)
 {
    PassArgs pa = b->pass_args();
-   if (b->grid2) {
-      pa.lds_cap = (int)std::min<int64_t>(b->tune.lds_floats, b->lds_need[0]);
-      HIPCHK(launch_pass(pa, 2, b->grid2, s));
+   for (const auto& c : b->pcls) {
+      if (!c.grid) continue;
+      pa.ref_lo = c.ref_lo;
+      pa.ref_hi = c.ref_hi;
+      pa.lds_cap = (int)std::min<int64_t>(b->tune.lds_floats, c.lds);
+      HIPCHK(launch_pass(pa, c.n_dim, c.grid, s));
    }
-   if (b->grid3) {
-      pa.lds_cap = (int)std::min<int64_t>(b->tune.lds_floats, b->lds_need[1]);
-      HIPCHK(launch_pass(pa, 3, b->grid3, s));
-   }
+   return FA_OK;
+}
+
+// Enqueues the trust-region update, one launch per update class.
+int launch_updates(fa_batch* b, hipStream_t s)
+{
+   const BatchDev B = b->dev();
+   for (size_t c = 0; c < b->trc_lds.size(); c++)
+      HIPCHK(launch_tr_update(B, (int)c, b->tune.tr_lds ? b->trc_lds[c] : 0, s));
    return FA_OK;
 }
 
@@ -457,7 +526,7 @@ int run_round(fa_batch* b)
       HIPCHK(hipEventRecord(ee, s));
       b->timed++;
    }
-   HIPCHK(launch_tr_update(B, s));
+   CHK(launch_updates(b, s));
    return FA_OK;
 }
 
@@ -517,7 +586,7 @@ int ensure_graph(fa_batch* b, int K)
    for (int r = 0; r < K && rc == FA_OK; r++) {
       if (launch_schedule(B, 1, s) != hipSuccess) rc = set_err(FA_E_DEVICE, "graph capture failed");
       if (rc == FA_OK) rc = launch_passes(b, s);
-      if (rc == FA_OK && launch_tr_update(B, s) != hipSuccess) rc = set_err(FA_E_DEVICE, "graph capture failed");
+      if (rc == FA_OK) rc = launch_updates(b, s);
    }
    hipGraph_t graph = nullptr;
    hipError_t e = hipStreamEndCapture(s, &graph);
@@ -806,9 +875,8 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
    b->n_slots = S;
    if (prm) b->prm = *prm;
    else fa_tr_default(&b->prm);
-   int nmax = 0, ndmax = 0, fmax = 0, tr_need = 0;
+   int nmax = 0, ndmax = 0, fmax = 0;
    int64_t cmax = 0;
-   bool has2 = false, has3 = false;
    if ((int)refs.size() > kMaxRefs) {
       delete b;
       return set_err(FA_E_ARG, "too many distinct references in one server (at most 4096)");
@@ -822,10 +890,17 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
       ndmax = std::max(ndmax, r->nD);
       fmax = std::max(fmax, r->nD * r->n_dim * r->n_dim);
       cmax = std::max<int64_t>(cmax, (int64_t)r->chunks.size());
-      (r->n_dim == 2 ? has2 : has3) = true;
-      tr_need = std::max(tr_need, tr_lds_need(r));
       widest = std::max(widest, lds_floats_for(r));
-      b->lds_need[r->n_dim == 3] = std::max(b->lds_need[r->n_dim == 3], lds_floats_for(r));
+   }
+   // a slot may hold a frame of any reference: a class's work is bounded by every slot
+   // holding its largest reference
+   std::vector<int64_t> ref_items;
+   for (fa_reference* r : refs) ref_items.push_back((int64_t)S * (int64_t)r->chunks.size());
+   make_classes(b, refs, ref_items);
+   for (auto& c : b->pcls) {
+      int64_t m = 0;
+      for (int i = c.ref_lo; i < c.ref_hi; i++) m = std::max(m, ref_items[i]);
+      c.work = m;
    }
    b->tune.lds_floats = (int)std::min<int64_t>(widest, kLdsCapFloats);
    b->tune.blocks_per_cu = kBlocksPerCu;
@@ -833,7 +908,6 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
    b->tune.check_every = 8;
    b->tune.use_graph = 0;
    b->tune.tr_lds = 1;
-   b->tr_lds = tr_lds_for(ctx, tr_need);
    b->hframes.resize(S);
    for (int j = 0; j < S; j++) {
       BatchFrame& bf = b->hframes[j];
@@ -850,8 +924,6 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
    b->total_vec = 3 * (int64_t)S * nmax;
    b->total_fac = 3 * (int64_t)S * fmax;
    b->total_D = 3 * (int64_t)S * ndmax;
-   b->work2 = has2 ? b->total_work : 0;
-   b->work3 = has3 ? b->total_work : 0;
    int rc;
    auto fail = [&](int code) {
       delete b;
@@ -941,6 +1013,7 @@ int server_step(fa_server* sv, int rounds)
          bf.f.offset = r.offset;
       }
       bf.ref = (int)(std::find(sv->refs.begin(), sv->refs.end(), ref) - sv->refs.begin());
+      bf.trc = b->ref_trc[bf.ref];
       bf.n = ref->nD * ref->n_dim;
       sv->h_bf[na] = bf;
       std::memcpy(sv->h_x + (size_t)na * sv->nmax, r.x, bf.n * sizeof(double));
@@ -1076,12 +1149,11 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    r->nD = nD;
    r->n_dim = nz > 1 ? 3 : 2;
    r->n_px = n_px;
-   // default work item: about 8 frame rows, 2048..8192 pixels
-   // default work item: 8 rows (2-D), 2..8 rows of about 4096 pixels (3-D, which stages two
-   // planes), within 2048..8192 pixels
-   const int64_t item_rows = nz > 1 ? std::min<int64_t>(8, std::max<int64_t>(2, 4096 / nx)) : 8;
+   // default work item: 8 rows within 2048..8192 pixels (2-D); 2048 pixels, or one row of a
+   // wider frame (3-D, which stages two planes per item; measured best for 256..1024 wide)
    r->chunk_px = chunk_px > 0 ? chunk_px
-                              : (int)std::min<int64_t>(8192, std::max<int64_t>(2048, item_rows * (int64_t)nx));
+                 : nz > 1     ? std::max(2048, nx)
+                              : (int)std::min<int64_t>(8192, std::max<int64_t>(2048, 8 * (int64_t)nx));
    derive_scan(*scan, nz, ny, nx, nD, r->scan_host, &r->scan);
    const int m = r->n_dim;
 
@@ -1492,7 +1564,9 @@ int fa_batch_get_tuning(const fa_batch* b, fa_tuning* t)
    if (!b || !t) return set_err(FA_E_ARG, "null argument");
    *t = b->tune;
    const fa_batch* leaf = b->parts.empty() ? b : b->parts[0];
-   t->tr_lds = b->tune.tr_lds && leaf->tr_lds > 0;
+   bool any = false;
+   for (int l : leaf->trc_lds) any = any || l > 0;
+   t->tr_lds = b->tune.tr_lds && any;
    return FA_OK;
 }
 
@@ -1576,12 +1650,10 @@ int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs,
    auto* sv = new fa_server();
    sv->ctx = ctx;
    size_t largest = 0;
-   // references of 2-D frames first: the schedule orders the pass's work by reference index
-   for (int kind = 2; kind <= 3; kind++)
-      for (int i = 0; i < n_refs; i++)
-         if (refs[i]->n_dim == kind &&
-             std::find(sv->refs.begin(), sv->refs.end(), refs[i]) == sv->refs.end())
-            sv->refs.push_back(refs[i]);
+   // the server's references in launch order (kind, then LDS need); slots index them
+   for (int i = 0; i < n_refs; i++)
+      if (std::find(sv->refs.begin(), sv->refs.end(), refs[i]) == sv->refs.end()) sv->refs.push_back(refs[i]);
+   order_refs(sv->refs);  // the slot batch's launch order
    for (fa_reference* r : sv->refs) {
       sv->nmax = std::max(sv->nmax, r->nD * r->n_dim);
       largest = std::max(largest, fa_server::round_up(frame_bytes(r, FA_F32)));

@@ -75,6 +75,8 @@ struct BatchFrame {
    int64_t vec_base;   // offset of this frame's work vectors (p, g, t1, t2) in the pool
    int64_t fac_base;   // offset of this frame's factor scratch
    int64_t d_base;     // offset of this frame's nD x 3 trial displacement
+   int32_t trc;        // launch class of its trust-region update (LDS size)
+   int32_t pad;
 };
 
 struct TrParamsDev {
@@ -114,6 +116,7 @@ struct PassArgs {
    const Counters* counters;
    int32_t lds_cap;   // floats of dynamic LDS available for staging
    int32_t use_lds;
+   int32_t ref_lo, ref_hi;  // the launch takes the work items of references [ref_lo, ref_hi)
 };
 
 struct BatchDev {
@@ -132,8 +135,7 @@ struct BatchDev {
    Counters* counters;
    TrParamsDev prm;
    int32_t n_frames;
-   int32_t n_refs;    // references, those of 2-D frames first
-   int32_t tr_lds;    // doubles of dynamic LDS for the trust-region update (0: global memory)
+   int32_t n_refs;    // references: 2-D before 3-D, by LDS need within a kind
 };
 
 }  // namespace fa

@@ -957,22 +957,23 @@ __device__ __forceinline__ void dispatch_chunk(const RefDev& R, const FrameDev& 
    }
 }
 
-// Persistent over the active work list: workgroup b takes a contiguous share of the items,
-// with workgroups renumbered so that each XCD (workgroups are dealt to the 8 XCDs round
-// robin) owns one contiguous range, so the reference chunks and frame rows it reads stay in
-// its L2: within a reference the items are chunk-major, so consecutive items read the same
-// reference chunk for different frames. The 2-D frames' references come first; k_pass<2>
-// takes their items, k_pass<3> the rest, so each variant is compiled for its own register
-// budget.
+// Over the active work items of the references [ref_lo, ref_hi) (one kind, one LDS class):
+// workgroup b takes a contiguous share of the items, with workgroups renumbered so that each
+// XCD (workgroups are dealt to the 8 XCDs round robin) owns one contiguous range, so the
+// reference chunks and frame rows it reads stay in its L2: within a reference the items are
+// chunk-major, so consecutive items read the same reference chunk for different frames. The
+// grid is about one workgroup per item when every frame is active, and the hardware
+// dispatcher balances them. k_pass<2> and k_pass<3> are compiled for their own register
+// budgets.
 template <int M>
 __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a)
 {
    extern __shared__ float lds[];
    __shared__ KF sred[kBlock / 64][7];
-   const int64_t W2 = a.counters->n_active_work2;
-   const int64_t Wb = M == 2 ? 0 : W2;
-   const int64_t W = (M == 2 ? W2 : a.counters->n_active_work) - Wb;
-   if (W <= 0 || a.counters->n_active_frames <= 0) return;
+   if (a.counters->n_active_frames <= 0) return;
+   const int64_t Wb = a.run_base[a.ref_lo];
+   const int64_t W = a.run_base[a.ref_hi] - Wb;
+   if (W <= 0) return;
    const int G = gridDim.x;
    int lb = blockIdx.x;
    if (G % 8 == 0) lb = (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
@@ -1088,8 +1089,8 @@ __global__ __launch_bounds__(kBlock) void k_splat(RefDev R, FrameDev F, const do
 }
 
 // ---- scheduling: compact the active frames and their work items -----------------------
-// The active frames are grouped by reference (a counting sort; references of 2-D frames have
-// the lower indices, so k_pass<2> takes the first items and k_pass<3> the rest). A reference's
+// The active frames are grouped by reference (a counting sort), so the items of a range of
+// references are contiguous and each pass launch takes those of its class. A reference's
 // work items are chunk-major: the active frames of one reference process a chunk one after
 // the other, so its reference record is read from HBM once per group rather than per frame.
 // The order within a group does not affect any result (each item's sums go to the partials
@@ -1483,12 +1484,14 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
    if (threadIdx.x == 0) B.state[j] = st;
 }
 
+// One launch per update class (frames whose factor blocks need a similar amount of LDS):
+// the frames of other classes return at once.
 template <bool LDS>
-__global__ __launch_bounds__(64) void k_tr_update(BatchDev B)
+__global__ __launch_bounds__(64) void k_tr_update(BatchDev B, int cls)
 {
    extern __shared__ double smem[];
    const int j = blockIdx.x;
-   if (B.state[j].done) return;
+   if (B.state[j].done || B.frames[j].trc != cls) return;
    if (B.refs[B.frames[j].ref].n_dim == 2) tr_update_frame<2, LDS>(B, j, smem);
    else tr_update_frame<3, LDS>(B, j, smem);
 }
@@ -1582,12 +1585,13 @@ hipError_t configure_kernels(int max_lds_bytes)
                               max_lds_bytes);
 }
 
-hipError_t launch_tr_update(const BatchDev& B, hipStream_t s)
+hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, hipStream_t s)
 {
-   if (B.tr_lds > 0)
-      hipLaunchKernelGGL(k_tr_update<true>, dim3(B.n_frames), dim3(64), (size_t)B.tr_lds * sizeof(double), s, B);
+   if (lds_doubles > 0)
+      hipLaunchKernelGGL(k_tr_update<true>, dim3(B.n_frames), dim3(64), (size_t)lds_doubles * sizeof(double), s,
+                         B, cls);
    else
-      hipLaunchKernelGGL(k_tr_update<false>, dim3(B.n_frames), dim3(64), 0, s, B);
+      hipLaunchKernelGGL(k_tr_update<false>, dim3(B.n_frames), dim3(64), 0, s, B, cls);
    return hipGetLastError();
 }
 

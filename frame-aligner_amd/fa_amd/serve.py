@@ -24,14 +24,15 @@ SERVE_SHAPES = [(1, 256, 256), (1, 512, 512), (1, 1024, 1024), (4, 256, 256),
 
 def make_acquisitions(seed=2026, shapes=SERVE_SHAPES, pool=3, node_range=(10, 40), mean=50.0):
     """One seeded acquisition per shape: uint16 Poisson blob phantom (blob count scaled with
-    the imaged area and depth), nodes drawn uniformly from node_range, `pool` distinct frames
-    that the arrival stream cycles through. Returns a list of dicts (synth.make_stack's keys)."""
+    the imaged area and depth), resampling points per plane drawn uniformly from node_range
+    (nD = points * planes + 1, as the aligner sets it, FrameWarpAligner.cpp:100), `pool`
+    distinct frames that the arrival stream cycles through. Returns a list of dicts
+    (synth.make_stack's keys)."""
     rng = np.random.default_rng(seed)
     out = []
     for i, shape in enumerate(shapes):
         nz, ny, nx = shape
-        nD = int(rng.integers(node_range[0], node_range[1] + 1))
-        nD = max(nD, nz + 2)
+        nD = int(rng.integers(node_range[0], node_range[1] + 1)) * nz + 1
         blobs = int(600 * ny * nx / 512 ** 2 * max(1.0, nz / 4))
         st = synth.make_stack(shape, pool, nD, seed=seed * 100 + i, n_blobs=blobs,
                               sigma=(2.0, 6.0) if nz == 1 else (1.5, 4.0), mean=mean,
