@@ -101,6 +101,18 @@ def serving(F, args, ctx):
                                                      for a in acqs) + f", Poisson arrivals {args.serve_rate:g}/s")
     out["slots"] = args.serve_slots
     out["rounds_per_step"] = args.serve_rounds
+    # the same stream at lower offered rates: the highest the GPU sustains is where the
+    # latency stops growing with the run's length
+    out["at_lower_rates"] = []
+    for rate in [float(r) for r in args.serve_lower.split(",") if r]:
+        t2, a2 = serve.arrivals(len(acqs), rate, args.serve_seconds, seed=args.seed + 2)
+        r2 = loop.run(t2, a2)
+        s2 = serve.summarize(r2, t2, a2, pools)
+        out["at_lower_rates"].append({"offered_frames_per_s": s2["offered_frames_per_s"],
+                                      "frames_per_s": s2["frames_per_s"], "latency_ms": s2["latency_ms"],
+                                      "max_queued": s2["max_queued"]})
+        log(f"serving at {rate:g}/s: {s2['frames_per_s']} frames/s, p50 {s2['latency_ms']['p50']} ms, "
+            f"p99 {s2['latency_ms']['p99']} ms")
     # served == alone, on a sample
     same = True
     for i in sorted(keep):
@@ -428,6 +440,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=128)
     ap.add_argument("--serve-rounds", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-lower", default="1500,1000",
+                    help="lower offered rates (frames/s) the serving section also runs")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape, no autotuning")
     ap.add_argument("--sections", default="batch,serve,precision,uneven",
                     help="comma list of: batch (the headline 512^2 workload), serve, precision (3-D), "

@@ -298,7 +298,8 @@ void order_refs(std::vector<fa_reference*>& rs)
 // before a reference that stages more than 1.5x the LDS of the class's first. Update: the
 // references' factor sizes, a class starting above 2x its first. ref_items: each reference's
 // work items with every frame of the batch active.
-void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::vector<int64_t>& ref_items)
+void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::vector<int64_t>& ref_items,
+                  int n_frames)
 {
    b->pcls.clear();
    for (int i = 0; i < (int)rs.size(); i++) {
@@ -314,6 +315,16 @@ void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::
       c.ref_hi = i + 1;
       c.lds = std::max(c.lds, need);
       c.work += ref_items[i];
+   }
+   // The update classes run one after another on the batch's stream. When every frame's
+   // workgroup is resident at once even with the largest LDS size, one launch is quicker.
+   int most = 0;
+   for (fa_reference* r : rs) most = std::max(most, tr_lds_need(r));
+   const int64_t per_cu = most ? b->ctx->lds_per_block / ((int64_t)most * (int64_t)sizeof(double)) : 1;
+   if ((int64_t)n_frames <= (int64_t)b->ctx->n_cu * std::max<int64_t>(per_cu, 1)) {
+      b->ref_trc.assign(rs.size(), 0);
+      b->trc_lds.assign(1, tr_lds_for(b->ctx, most));
+      return;
    }
    std::vector<int> order(rs.size());
    for (size_t i = 0; i < rs.size(); i++) order[i] = (int)i;
@@ -394,7 +405,7 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
    }
    std::vector<int64_t> ref_items(rs.size(), 0);
    for (int j = 0; j < n; j++) ref_items[ref_idx[frames[b->user[j]]->ref]] += (int64_t)frames[b->user[j]]->ref->chunks.size();
-   make_classes(b, rs, ref_items);
+   make_classes(b, rs, ref_items, n);
    b->hframes.resize(n);
    for (int j = 0; j < n; j++) {
       const int i = b->user[j];
@@ -896,7 +907,7 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
    // holding its largest reference
    std::vector<int64_t> ref_items;
    for (fa_reference* r : refs) ref_items.push_back((int64_t)S * (int64_t)r->chunks.size());
-   make_classes(b, refs, ref_items);
+   make_classes(b, refs, ref_items, S);
    for (auto& c : b->pcls) {
       int64_t m = 0;
       for (int i = c.ref_lo; i < c.ref_hi; i++) m = std::max(m, ref_items[i]);
@@ -906,7 +917,7 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
    b->tune.blocks_per_cu = kBlocksPerCu;
    b->tune.use_lds = 1;
    b->tune.check_every = 8;
-   b->tune.use_graph = 0;
+   b->tune.use_graph = 1;
    b->tune.tr_lds = 1;
    b->hframes.resize(S);
    for (int j = 0; j < S; j++) {
@@ -1033,8 +1044,13 @@ int server_step(fa_server* sv, int rounds)
    }
    running = running || na > 0;
    if (!running) return FA_OK;
-   // 3. rounds over every running frame, then collect the finished ones
-   for (int r = 0; r < rounds; r++) CHK(run_round(b));
+   // 3. rounds over every running frame (one graph replay), then collect the finished ones
+   if (b->tune.use_graph) {
+      CHK(ensure_graph(b, rounds));
+      HIPCHK(hipGraphLaunch(b->gexec, s));
+   } else {
+      for (int r = 0; r < rounds; r++) CHK(run_round(b));
+   }
    CHK(run_poll(b));
    HIPCHK(hipMemcpyAsync(sv->h_state, b->d_state, sv->S * sizeof(TrState), hipMemcpyDeviceToHost, s));
    HIPCHK(hipMemcpyAsync(sv->h_xall, b->d_x, b->total_x * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -76,6 +76,7 @@ class ServeLoop:
         S = self.server
         n = len(times)
         lat = np.full(n, np.nan)
+        evals = np.zeros(n, np.int64)
         kept = {}
         nxt = 0
         done = 0
@@ -102,20 +103,32 @@ class ServeLoop:
             tnow = time.perf_counter() - t0
             for tag, x, st in res:
                 lat[tag] = tnow - times[tag]
+                evals[tag] = st["evaluations"]
                 if keep is not None and tag in keep:
                     kept[tag] = (x, st)
                 done += 1
                 busy -= 1
         wall = time.perf_counter() - t0
         return dict(latency=lat, kept=kept, wall=wall, steps=steps, max_in_flight=max_flight,
-                    max_queued=max_queued)
+                    max_queued=max_queued, evaluations=evals)
 
 
 def summarize(res, times, acq, pools):
     lat = res["latency"] * 1e3
     n = len(times)
     px = sum(pools[int(a)][0].size for a in acq)
-    return {"frames": n, "wall_s": round(res["wall"], 3),
+    ev = res.get("evaluations")
+    extra = {}
+    if ev is not None:
+        extra["evaluations_per_frame"] = {"mean": round(float(np.mean(ev)), 2), "max": int(np.max(ev))}
+        per = {}
+        for a, e, l in zip(acq, ev, lat):
+            per.setdefault(int(a), []).append((e, l))
+        extra["per_acquisition"] = {int(a): {"frames": len(v), "evaluations_mean": round(float(np.mean([e for e, _ in v])), 1),
+                                             "latency_p50_ms": round(float(np.percentile([l for _, l in v], 50)), 2),
+                                             "latency_p99_ms": round(float(np.percentile([l for _, l in v], 99)), 2)}
+                                    for a, v in sorted(per.items())}
+    return {"frames": n, "wall_s": round(res["wall"], 3), **extra,
             "frames_per_s": round(n / res["wall"], 1),
             "offered_frames_per_s": round(n / max(times[-1], 1e-9), 1),
             "mpx_per_s": round(px / res["wall"] / 1e6, 1),
