@@ -346,7 +346,7 @@ def headline(F, args, ctx):
 
     # plain path: one frame per launch sequence
     lat = []
-    npl = min(args.plain_frames, n)
+    npl = max(1, min(args.plain_frames, n))
     for i in range(npl):  # creates each frame's one-frame batch outside the timed loop
         frames[i].align(**tr)
     x_plain = []
@@ -361,7 +361,7 @@ def headline(F, args, ctx):
 
     # the pass alone, all frames active, no other work on the GPU: one fused pass over the
     # whole batch per launch, timed with HIP events
-    iso_ms, iso_bytes = isolated_pass(F, frames, tr)
+    iso_ms, iso_bytes = isolated_pass(F, frames, tr) if not args.no_isolated else (0.0, 0.0)
 
     out = {
         "metric": METRIC,
@@ -398,7 +398,7 @@ def headline(F, args, ctx):
                      "launches": launches, "avg_launch_us": round(1e3 * pass_ms / max(launches, 1), 2),
                      "bytes_per_launch": pass_bytes / max(launches, 1),
                      "isolated": {"avg_launch_us": round(1e3 * iso_ms, 2), "bytes_per_launch": iso_bytes,
-                                  "achieved": round(iso_bytes / (iso_ms * 1e-3) / 1e9, 1)}},
+                                  "achieved": round(iso_bytes / (iso_ms * 1e-3) / 1e9, 1) if iso_ms else None}},
     }
 
     if not args.no_cpu:
@@ -443,6 +443,7 @@ def main():
     ap.add_argument("--serve-lower", default="1500,1000",
                     help="lower offered rates (frames/s) the serving section also runs")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape, no autotuning")
+    ap.add_argument("--no-isolated", action="store_true", help="skip the isolated one-pass timing")
     ap.add_argument("--sections", default="batch,serve,precision,uneven",
                     help="comma list of: batch (the headline 512^2 workload), serve, precision (3-D), "
                          "uneven (64 stacks)")

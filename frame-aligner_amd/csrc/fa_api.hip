@@ -134,6 +134,7 @@ struct fa_batch {
    // Launch classes of the trust-region update: LDS doubles of each (0: global memory) and
    // the class of each reference.
    std::vector<int> trc_lds;
+   std::vector<int> trc_full;  // 1: the class stages everything in LDS
    std::vector<int> ref_trc;
    fa_tuning tune{};
    int timing = 0;
@@ -254,11 +255,23 @@ int64_t lds_floats_for(const fa_reference* r)
 }
 
 // Doubles of LDS the trust-region update of a frame of reference r uses: the cyclic-reduction
-// factor blocks (3 nD n_dim^2) and the solve vector (nD n_dim).
+// factor blocks (3 nD n_dim^2) and the solve vector (nD n_dim); staging everything
+// (tr_lds_full) adds the pass partials, H, x, p, g and the interval table.
 int tr_lds_need(const fa_reference* r)
 {
    return 3 * r->nD * r->n_dim * r->n_dim + r->nD * r->n_dim;
 }
+
+int tr_lds_full(const fa_reference* r)
+{
+   const int n = r->nD * r->n_dim;
+   return tr_lds_need(r) + (int)r->chunks.size() * kPassStride + 2 * r->nD * r->n_dim * r->n_dim + 3 * n +
+          (r->nD + 1) / 2;
+}
+
+// Updates whose full staging fits this many doubles stage everything (several such waves
+// still share a compute unit's LDS).
+constexpr int kTrFullDoubles = 8192;
 
 // The update runs from LDS when the largest frame's need fits one workgroup's LDS, else from
 // global memory (0).
@@ -321,9 +334,17 @@ void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::
    int most = 0;
    for (fa_reference* r : rs) most = std::max(most, tr_lds_need(r));
    const int64_t per_cu = most ? b->ctx->lds_per_block / ((int64_t)most * (int64_t)sizeof(double)) : 1;
-   if ((int64_t)n_frames <= (int64_t)b->ctx->n_cu * std::max<int64_t>(per_cu, 1)) {
+   int most_full = 0;
+   for (fa_reference* r : rs) most_full = std::max(most_full, tr_lds_full(r));
+   if ((int64_t)n_frames <= (int64_t)b->ctx->n_cu * std::max<int64_t>(per_cu, 1) || most_full <= kTrFullDoubles) {
       b->ref_trc.assign(rs.size(), 0);
-      b->trc_lds.assign(1, tr_lds_for(b->ctx, most));
+      if (most_full <= kTrFullDoubles) {
+         b->trc_lds.assign(1, tr_lds_for(b->ctx, most_full));
+         b->trc_full.assign(1, b->trc_lds[0] > 0 ? 1 : 0);
+      } else {
+         b->trc_lds.assign(1, tr_lds_for(b->ctx, most));
+         b->trc_full.assign(1, 0);
+      }
       return;
    }
    std::vector<int> order(rs.size());
@@ -332,16 +353,21 @@ void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::
                     [&](int a, int c) { return tr_lds_need(rs[a]) < tr_lds_need(rs[c]); });
    b->ref_trc.assign(rs.size(), 0);
    b->trc_lds.clear();
-   int first = 0, largest = 0;
+   b->trc_full.clear();
+   int first = 0, largest = 0, largest_full = 0;
    for (int i : order) {
       const int need = tr_lds_need(rs[i]);
       if (b->trc_lds.empty() || need > 2 * first) {
          b->trc_lds.push_back(0);
+         b->trc_full.push_back(0);
          first = need;
-         largest = 0;
+         largest = largest_full = 0;
       }
       largest = std::max(largest, need);
-      b->trc_lds.back() = tr_lds_for(b->ctx, largest);
+      largest_full = std::max(largest_full, tr_lds_full(rs[i]));
+      const bool full = largest_full <= kTrFullDoubles;
+      b->trc_lds.back() = tr_lds_for(b->ctx, full ? largest_full : largest);
+      b->trc_full.back() = full && b->trc_lds.back() > 0 ? 1 : 0;
       b->ref_trc[i] = (int)b->trc_lds.size() - 1;
    }
 }
@@ -489,7 +515,7 @@ int launch_updates(fa_batch* b, hipStream_t s)
 {
    const BatchDev B = b->dev();
    for (size_t c = 0; c < b->trc_lds.size(); c++)
-      HIPCHK(launch_tr_update(B, (int)c, b->tune.tr_lds ? b->trc_lds[c] : 0, s));
+      HIPCHK(launch_tr_update(B, (int)c, b->tune.tr_lds ? b->trc_lds[c] : 0, b->trc_full[c], s));
    return FA_OK;
 }
 
@@ -1293,6 +1319,23 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    r->eig_lo = info[1];
    R.h_pd = r->h_pd;
    R.eig_lo = r->eig_lo;
+   // small problems: eigendecomposition of H for the trust-region subproblem
+   R.eig = 0;
+   R.ev_min = 0;
+   const int nn = nD * m, mp = nn + (nn & 1);
+   if (nn <= kEigMax && (int64_t)(2 * mp * mp + mp) * (int64_t)sizeof(double) <= ctx->lds_per_block) {
+      if ((rc = dalloc(&r->d_Q, (size_t)nn * nn)) || (rc = dalloc(&r->d_ev, nn))) return fail(rc);
+      R.Q = r->d_Q;
+      R.ev = r->d_ev;
+      if (launch_eigen(R, r->d_Q, r->d_ev, d_info, s) != hipSuccess ||
+          hipMemcpyAsync(info, d_info, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+         (void)hipGetLastError();
+         return fail(set_err(FA_E_DEVICE, "reference set-up failed"));
+      }
+      R.eig = info[0] != 0.0 ? 1 : 0;
+      R.ev_min = info[1];
+   }
    cleanup();
    ctx->live_refs++;
    r->counted = true;
@@ -1314,6 +1357,8 @@ int fa_reference_release(fa_reference* r)
    dfree(r->d_Hdiag);
    dfree(r->d_Hsub);
    dfree(r->d_L0);
+   dfree(r->d_Q);
+   dfree(r->d_ev);
    delete r;
    return FA_OK;
 }

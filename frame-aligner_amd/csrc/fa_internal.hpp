@@ -40,6 +40,7 @@ constexpr int kBlock = 256;  // threads per fused-pass workgroup (4 wave64)
 // |dx| stays below kPad - 2 over a work item.
 constexpr int kPad = 16;
 constexpr int kMaxRefs = 4096;  // distinct references in one batch
+constexpr int kEigMax = 90;     // largest nD * n_dim whose subproblem uses the eigendecomposition of H
 
 // Per-pixel reference record read by every pass. 2-D: (R, gx, gy, Df) in one float4;
 // 3-D: (R, gx, gy, gz) plus Df in a separate float array. Df is the node fraction of
@@ -53,9 +54,12 @@ struct RefDev {
    const double* Hdiag;    // nD blocks n_dim x n_dim
    const double* Hsub;     // nD-1 blocks, block (k+1, k)
    const double* L0;       // cyclic-reduction Cholesky factor of H (A, C, W blocks; valid when h_pd)
+   const double* Q;        // eigenvectors of H, n x n row-major, column i for ev[i] (valid when eig)
+   const double* ev;       // eigenvalues of H
    ScanDev scan;
-   int32_t nx, ny, nz, nD, n_dim, n_chunks, h_pd, pad;
+   int32_t nx, ny, nz, nD, n_dim, n_chunks, h_pd, eig;
    double eig_lo;          // Gershgorin lower bound of the spectrum of H
+   double ev_min;          // smallest eigenvalue (valid when eig)
    int64_t n_px;
 };
 
@@ -172,6 +176,8 @@ struct fa_reference {
    double* d_Hdiag = nullptr;
    double* d_Hsub = nullptr;
    double* d_L0 = nullptr;  // 3 nD n_dim^2: factor blocks A, C, W of H
+   double* d_Q = nullptr;   // eigenvectors and eigenvalues of H (nD n_dim <= kEigMax)
+   double* d_ev = nullptr;
    fa::RefDev dev{};
    int live_frames = 0;
    bool counted = false;  // included in ctx->live_refs

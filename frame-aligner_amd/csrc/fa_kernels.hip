@@ -661,6 +661,128 @@ __global__ __launch_bounds__(kHessThreads) void k_hess_assemble(RefDev R, const 
    }
 }
 
+// ---- eigendecomposition of H (small problems) --------------------------------------------
+// Parallel cyclic Jacobi (Golub & Van Loan, 8.5): every round applies m/2 disjoint rotations
+// chosen by the circle method, so that each sweep of m - 1 rounds visits every pair (p, q)
+// once; A <- J^T A J, V <- V J. A and V live in LDS (m <= kEigMax + 1). Sweeps stop when the
+// off-diagonal Frobenius norm is below 1e-15 of ||H||_F (or after 40 sweeps, reported as not
+// converged). info[0] = 1 if converged, info[1] = smallest eigenvalue.
+constexpr int kEigThreads = 256;
+
+__device__ __forceinline__ void round_pair(int m, int r, int k, int& p, int& q)
+{
+   if (k == 0) {
+      p = r % (m - 1);
+      q = m - 1;
+   } else {
+      p = (r + k) % (m - 1);
+      q = (r - k + (m - 1)) % (m - 1);
+   }
+   if (p > q) {
+      const int t = p;
+      p = q;
+      q = t;
+   }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+#pragma unroll
+   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+   __syncthreads();
+   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+   __syncthreads();
+   double t = 0;
+   for (int w = 0; w < kEigThreads / 64; w++) t += red[w];
+   return t;
+}
+
+__global__ __launch_bounds__(kEigThreads) void k_eigen(RefDev R, double* Q, double* ev, double* info)
+{
+   extern __shared__ double sm[];
+   __shared__ double red[kEigThreads / 64];
+   const int n0 = R.nD * R.n_dim, m = n0 + (n0 & 1), h = m / 2;
+   double* A = sm;
+   double* V = sm + m * m;
+   double* cs = V + m * m;
+   const int t = threadIdx.x;
+   for (int i = t; i < m * m; i += kEigThreads) {
+      const int r = i / m, c = i - r * m;
+      A[i] = (r < n0 && c < n0) ? h_get(R, r, c) : 0.0;
+      V[i] = r == c ? 1.0 : 0.0;
+   }
+   __syncthreads();
+   double fro = 0;
+   for (int i = t; i < m * m; i += kEigThreads) fro += A[i] * A[i];
+   fro = sqrt(block_sum(fro, red));
+   int sweep = 0;
+   for (; sweep < 40; sweep++) {
+      double off = 0;
+      for (int i = t; i < m * m; i += kEigThreads) {
+         const int r = i / m, c = i - r * m;
+         if (r != c) off += A[i] * A[i];
+      }
+      if (!(sqrt(block_sum(off, red)) > 1e-15 * fro)) break;
+      for (int rnd = 0; rnd < m - 1; rnd++) {
+         for (int k = t; k < h; k += kEigThreads) {
+            int p, q;
+            round_pair(m, rnd, k, p, q);
+            double c = 1.0, s = 0.0;
+            const double apq = A[p * m + q];
+            if (q < n0 && apq != 0.0) {
+               const double th = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
+               const double tt = th == 0.0 ? 1.0 : copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
+               c = 1.0 / sqrt(tt * tt + 1.0);
+               s = tt * c;
+            }
+            cs[2 * k] = c;
+            cs[2 * k + 1] = s;
+         }
+         __syncthreads();
+         for (int e = t; e < h * m; e += kEigThreads) {
+            const int k = e / m, j = e - k * m;
+            int p, q;
+            round_pair(m, rnd, k, p, q);
+            const double c = cs[2 * k], s = cs[2 * k + 1];
+            const double ap = A[p * m + j], aq = A[q * m + j];
+            A[p * m + j] = c * ap - s * aq;
+            A[q * m + j] = s * ap + c * aq;
+         }
+         __syncthreads();
+         for (int e = t; e < h * m; e += kEigThreads) {
+            const int k = e / m, i = e - k * m;
+            int p, q;
+            round_pair(m, rnd, k, p, q);
+            const double c = cs[2 * k], s = cs[2 * k + 1];
+            const double ap = A[i * m + p], aq = A[i * m + q];
+            A[i * m + p] = c * ap - s * aq;
+            A[i * m + q] = s * ap + c * aq;
+            const double vp = V[i * m + p], vq = V[i * m + q];
+            V[i * m + p] = c * vp - s * vq;
+            V[i * m + q] = s * vp + c * vq;
+         }
+         __syncthreads();
+      }
+   }
+   for (int i = t; i < n0 * n0; i += kEigThreads) Q[i] = V[(i / n0) * m + (i % n0)];
+   double lo = INFINITY;
+   for (int i = t; i < n0; i += kEigThreads) {
+      ev[i] = A[i * m + i];
+      lo = fmin(lo, A[i * m + i]);
+   }
+#pragma unroll
+   for (int off = 32; off >= 1; off >>= 1) lo = fmin(lo, __shfl_xor(lo, off, 64));
+   __syncthreads();
+   if ((t & 63) == 0) red[t >> 6] = lo;
+   __syncthreads();
+   if (t == 0) {
+      double e = red[0];
+      for (int w = 1; w < kEigThreads / 64; w++) e = fmin(e, red[w]);
+      info[0] = sweep < 40 ? 1.0 : 0.0;
+      info[1] = e;
+   }
+}
+
 // ---- the fused pass ------------------------------------------------------------------------
 // For every pixel of a work item: node fraction, displacement d = f D[k+1] + (1-f) D[k]
 // (warpPoint, FrameWarper.cpp:397-419), zero-padded bilinear/trilinear pull-sample of the
@@ -1291,6 +1413,78 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
    return it;
 }
 
+// The same subproblem iteration through H = Q diag(ev) Q^T: with qv = Q^T g,
+// ||p(lam)||^2 = sum qv_i^2 / (ev_i + lam)^2 and ||L^-1 p||^2 = sum qv_i^2 / (ev_i + lam)^3 for
+// the Cholesky factor L of H + lam I, which exists when ev_min + lam > 0 (lam = 0: when H's own
+// factorisation succeeded). Each iteration is two wave reductions instead of a factorisation
+// and three block-triangular solves; p = -Q (qv / (ev + lam)) is formed once, for the last
+// lambda at which the factor exists.
+__device__ int tr_subproblem_eig(const RefDev& R, const TrScratch& S, const double* g, double radius,
+                                 double eps, int max_iter, double* p, double* lam_out)
+{
+   const int n = R.nD * R.n_dim;
+   double* qv = S.fa;  // the factor scratch is free on this path
+   const double gnorm = sqrt(wave_dot(n, g, g));
+   for (int i = threadIdx.x; i < n; i += 64) p[i] = 0;
+   *lam_out = 0;
+   __syncthreads();
+   if (gnorm == 0) return 1;
+   for (int i = threadIdx.x; i < n; i += 64) {
+      double s = 0;
+      for (int r = 0; r < n; r++) s += R.Q[(int64_t)r * n + i] * g[r];
+      qv[i] = s;
+   }
+   __syncthreads();
+   const double eig_lo = R.eig_lo;
+   double lam = 0, lam_lo = 0, lam_hi = gnorm / radius + (eig_lo < 0 ? -eig_lo : 0);
+   double lam_sol = -1;  // lambda of the last solution (none yet)
+   int it;
+   for (it = 0; it < max_iter; it++) {
+      const bool ok = lam == 0 ? R.h_pd != 0 : R.ev_min + lam > 0;
+      if (!ok) {
+         lam_lo = lam;
+         lam = 0.5 * (lam_lo + lam_hi);
+         continue;
+      }
+      double s2 = 0, s3 = 0;
+      for (int i = threadIdx.x; i < n; i += 64) {
+         const double d = 1.0 / (R.ev[i] + lam);
+         const double a = qv[i] * d;
+         s2 += a * a;
+         s3 += a * a * d;
+      }
+      const double pn = sqrt(wave_sum(s2));
+      lam_sol = lam;
+      if (lam == 0 && pn <= radius) {
+         it++;
+         break;
+      }
+      if (fabs(pn - radius) <= eps * radius) {
+         it++;
+         break;
+      }
+      if (pn > radius) lam_lo = lam;
+      else lam_hi = lam;
+      const double qn = sqrt(wave_sum(s3));
+      double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
+      if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
+      lam = lam_new;
+   }
+   if (lam_sol >= 0) {
+      double* w = S.fa + n;
+      for (int i = threadIdx.x; i < n; i += 64) w[i] = qv[i] / (R.ev[i] + lam_sol);
+      __syncthreads();
+      for (int r = threadIdx.x; r < n; r += 64) {
+         double s = 0;
+         for (int i = 0; i < n; i++) s += R.Q[(int64_t)r * n + i] * w[i];
+         p[r] = -s;
+      }
+      *lam_out = lam_sol;
+   }
+   __syncthreads();
+   return it;
+}
+
 __device__ __forceinline__ TrScratch scratch_of(const BatchDev& B, const BatchFrame& bf)
 {
    TrScratch S;
@@ -1371,23 +1565,55 @@ __global__ void k_tr_init(BatchDev B, int eval_only) { init_frame(B, blockIdx.x,
 // Starts the listed slots of a serving batch.
 __global__ void k_tr_init_slots(BatchDev B, const int32_t* slots) { init_frame(B, slots[blockIdx.x], 0); }
 
+// LDS use of the update (full = 1): the frame's pass partials, H, ichunk, x, p, g, the factor
+// blocks and the solve vector are all staged with one burst of loads, so the update's many
+// small dependent steps run at LDS latency; x, p, g are written back at the end. full = 0:
+// only the factor blocks and the solve vector (read and written at every level of the cyclic
+// reduction) are in LDS.
 template <int M, bool LDS>
-__device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
+__device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, int full)
 {
    constexpr int MM = M * M;
    const BatchFrame& bf = B.frames[j];
    RefDev R = B.refs[bf.ref];
    const TrParamsDev& prm = B.prm;
-   TrScratch S = scratch_of(B, bf);
+   const TrScratch G = scratch_of(B, bf);
+   TrScratch S = G;
    const double* P = B.partials + bf.work_base * kPassStride;
    const int n = bf.n, nb = R.nD;
    if (LDS) {
-      // the factor blocks and the solve vector, which every level of the cyclic reduction
-      // reads and writes, in LDS; H, x, p, g stay in global memory
-      S.fa = smem;
-      S.fc = smem + nb * MM;
-      S.fw = smem + 2 * nb * MM;
-      S.y = smem + 3 * nb * MM;
+      double* q = smem;
+      S.fa = q;
+      S.fc = q + nb * MM;
+      S.fw = q + 2 * nb * MM;
+      S.y = q + 3 * nb * MM;
+      q += 3 * nb * MM + n;
+      if (full) {
+         const int np = R.n_chunks * kPassStride;
+         double* pl = q;
+         double* hd = pl + np;
+         double* hs = hd + nb * MM;
+         S.x = hs + nb * MM;
+         S.p = S.x + n;
+         S.g = S.p + n;
+         int32_t* ich = (int32_t*)(S.g + n);
+         for (int i = threadIdx.x; i < np; i += 64) pl[i] = P[i];
+         for (int i = threadIdx.x; i < nb * MM; i += 64) {
+            hd[i] = R.Hdiag[i];
+            hs[i] = R.Hsub[i];
+         }
+         for (int i = threadIdx.x; i < n; i += 64) {
+            S.x[i] = G.x[i];
+            S.p[i] = G.p[i];
+            S.g[i] = G.g[i];
+         }
+         for (int i = threadIdx.x; i < nb; i += 64) ich[i] = R.ichunk[i];
+         __syncthreads();
+         P = pl;
+         R.Hdiag = hd;
+         R.Hsub = hs;
+         R.ichunk = ich;
+      }
    }
    const double f_new = reduce_ssd(R, P);
    TrState st = B.state[j];
@@ -1473,7 +1699,8 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
    }
    if (!st.done) {
       double lam;
-      st.sub_iters = tr_subproblem<M>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
+      st.sub_iters = R.eig ? tr_subproblem_eig(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam)
+                           : tr_subproblem<M>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
       st.lambda = lam;
       st.phase = 1;
       double* Dt = B.Dtrial + bf.d_base;
@@ -1482,18 +1709,26 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
          for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < M ? S.x[i * M + c] + S.p[i * M + c] : 0.0;
    }
    if (threadIdx.x == 0) B.state[j] = st;
+   if (LDS && full) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < n; i += 64) {
+         G.x[i] = S.x[i];
+         G.p[i] = S.p[i];
+         G.g[i] = S.g[i];
+      }
+   }
 }
 
 // One launch per update class (frames whose factor blocks need a similar amount of LDS):
 // the frames of other classes return at once.
 template <bool LDS>
-__global__ __launch_bounds__(64) void k_tr_update(BatchDev B, int cls)
+__global__ __launch_bounds__(64) void k_tr_update(BatchDev B, int cls, int full)
 {
    extern __shared__ double smem[];
    const int j = blockIdx.x;
    if (B.state[j].done || B.frames[j].trc != cls) return;
-   if (B.refs[B.frames[j].ref].n_dim == 2) tr_update_frame<2, LDS>(B, j, smem);
-   else tr_update_frame<3, LDS>(B, j, smem);
+   if (B.refs[B.frames[j].ref].n_dim == 2) tr_update_frame<2, LDS>(B, j, smem, full);
+   else tr_update_frame<3, LDS>(B, j, smem, full);
 }
 
 // ---- launchers ----------------------------------------------------------------------------
@@ -1533,6 +1768,14 @@ hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, doub
    if (e != hipSuccess) return e;
    hipLaunchKernelGGL(k_hess_assemble, dim3(1), dim3(kHessThreads), 0, s, R, (const double*)partials,
                       Hdiag, Hsub, L0, info);
+   return hipGetLastError();
+}
+
+hipError_t launch_eigen(const RefDev& R, double* Q, double* ev, double* info, hipStream_t s)
+{
+   const int n0 = R.nD * R.n_dim, m = n0 + (n0 & 1);
+   const size_t bytes = (2 * (size_t)m * m + m) * sizeof(double);
+   hipLaunchKernelGGL(k_eigen, dim3(1), dim3(kEigThreads), bytes, s, R, Q, ev, info);
    return hipGetLastError();
 }
 
@@ -1581,17 +1824,18 @@ hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, 
 // Allows the LDS variant of the trust-region update up to max_lds_bytes of dynamic LDS.
 hipError_t configure_kernels(int max_lds_bytes)
 {
+   (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes);
    return hipFuncSetAttribute((const void*)k_tr_update<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               max_lds_bytes);
 }
 
-hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, hipStream_t s)
+hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, int full, hipStream_t s)
 {
    if (lds_doubles > 0)
       hipLaunchKernelGGL(k_tr_update<true>, dim3(B.n_frames), dim3(64), (size_t)lds_doubles * sizeof(double), s,
-                         B, cls);
+                         B, cls, full);
    else
-      hipLaunchKernelGGL(k_tr_update<false>, dim3(B.n_frames), dim3(64), 0, s, B, cls);
+      hipLaunchKernelGGL(k_tr_update<false>, dim3(B.n_frames), dim3(64), 0, s, B, cls, 0);
    return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@ hipError_t launch_reference_record(const float* T, int nz, int ny, int nx, int n
                                    const ScanDev& sc, float4* rg4, float* df, hipStream_t s);
 hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, double* Hsub,
                           double* L0, double* info, hipStream_t s);
+hipError_t launch_eigen(const RefDev& R, double* Q, double* ev, double* info, hipStream_t s);
 hipError_t launch_pass(const PassArgs& a, int n_dim, int grid, hipStream_t s);
 hipError_t launch_warp(const RefDev& R, const FrameDev& F, const double* D, float* out,
                        float invalid, int lds_cap, int use_lds, int grid, hipStream_t s);
@@ -21,7 +22,7 @@ hipError_t launch_splat(const RefDev& R, const FrameDev& F, const double* D, flo
 hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s);
 hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s);
 hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, hipStream_t s);
-hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, hipStream_t s);
+hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, int full, hipStream_t s);
 hipError_t configure_kernels(int max_lds_bytes);
 hipError_t launch_reduce_eval(const BatchDev& B, hipStream_t s);
 
