@@ -10,7 +10,8 @@ OUT="$HERE/lib/libfa_hip.so"
 "$HIPCC" --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
   -Xclang -target-feature -Xclang -packed-fp32-ops \
   -I"$ROOT/include" -I"$HERE/csrc" \
-  "$HERE/csrc/fa_kernels.hip" "$HERE/csrc/fa_api.hip" \
+  "$HERE/csrc/fa_kernels.hip" "$HERE/csrc/fa_api.hip" "$HERE/csrc/fa_frontend.hip" \
+  -L/opt/rocm/lib -lrocfft -Wl,-rpath,/opt/rocm/lib \
   -o "$OUT.tmp"
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"

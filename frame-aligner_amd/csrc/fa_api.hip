@@ -1107,6 +1107,27 @@ int server_step(fa_server* sv, int rounds)
 
 }  // namespace
 
+namespace fa {
+
+int set_error(int code, const std::string& msg) { return set_err(code, msg); }
+
+// A frame whose samples are already in device memory (d_data, nz*ny*nx values of dtype), owned
+// by the frame from now on (freed on deregistration).
+int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offset, fa_frame** out)
+{
+   std::lock_guard<std::mutex> lk(r->ctx->mu);
+   auto* f = new fa_frame();
+   f->ref = r;
+   f->dtype = dtype;
+   f->offset = offset;
+   f->d_data = d_data;
+   r->live_frames++;
+   *out = f;
+   return FA_OK;
+}
+
+}  // namespace fa
+
 // ---- C ABI ------------------------------------------------------------------------------------
 
 extern "C" {

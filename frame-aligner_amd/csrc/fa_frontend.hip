@@ -1,0 +1,613 @@
+// Front end of the warp aligner on the device: FrameWarpAligner::setReference and addFrame
+// (FrameAligner/FrameWarpAligner.cpp:46-223) around the warper of fa_api.hip.
+//
+//  * smoothStack: per z-slice Gaussian blur, sigma x = smoothing and sigma y = 1 (the
+//    reference passes 1 as cv::GaussianBlur's sigmaY), border reflect-101, row pass then column
+//    pass in float; then + 1.0f.
+//  * rigid start: VolumePhaseCorrelator (FrameAligner/VolumePhaseCorrelator.cpp): the raw
+//    frame binned in x and y (AbstractFrameAligner.cpp:37-56), times the square-root Hann
+//    window in y and x, real-to-complex 3-D FFT in FP64, cross-power spectrum with the
+//    reference's conjugate spectrum normalised by |.| + DBL_MIN, inverse FFT, first maximum,
+//    indices above half the size wrapped negative, x and y scaled by the binning.
+//  * starts {0, 0, rigid}: the first with the smallest SSD (FrameWarpAligner.cpp:150-198),
+//    then the trust region of all frames as one batch (type Warp).
+// The FFTs are rocFFT plans (double precision) on the aligner's own stream.
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "fa_internal.hpp"
+
+using namespace fa;
+
+namespace {
+
+#define FCHK(expr)                                                                                \
+   do {                                                                                           \
+      hipError_t e_ = (expr);                                                                     \
+      if (e_ != hipSuccess) return set_error(FA_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+   } while (0)
+
+#define RCHK(expr)                                                                                \
+   do {                                                                                           \
+      if ((expr) != rocfft_status_success) return set_error(FA_E_DEVICE, std::string("rocFFT: ") + #expr); \
+   } while (0)
+
+#define OK(expr)                                                                                  \
+   do {                                                                                           \
+      int r_ = (expr);                                                                            \
+      if (r_ != FA_OK) return r_;                                                                 \
+   } while (0)
+
+constexpr int kFB = 256;
+
+template <typename T>
+int falloc(T** p, size_t n)
+{
+   *p = nullptr;
+   if (hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return set_error(FA_E_OOM, "Not enough memory to allocate");
+   }
+   return FA_OK;
+}
+
+template <typename T>
+void ffree(T*& p)
+{
+   if (p) (void)hipFree((void*)p);
+   p = nullptr;
+}
+
+// ---- kernels ---------------------------------------------------------------------------------
+
+__global__ void k_to_float(const void* raw, int dtype, int64_t n, float* out)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) return;
+   out[i] = dtype == FA_U16 ? (float)((const uint16_t*)raw)[i] : ((const float*)raw)[i];
+}
+
+__device__ __forceinline__ int reflect101(int i, int n)
+{
+   if (n == 1) return 0;
+   const int period = 2 * n - 2;
+   i = abs(i) % period;
+   return i >= n ? period - i : i;
+}
+
+// Symmetric filter along x (axis = 0) or y (axis = 1) of every slice:
+// out = k[r] v[i] + sum_{j=1..r} k[r+j] (v[i-j] + v[i+j]), j ascending, in float; add is
+// added to the result (the aligner's +1 after the last pass).
+__global__ void k_blur(const float* in, float* out, int nz, int ny, int nx, const float* k, int r, int axis,
+                       float add)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t n = (int64_t)nz * ny * nx;
+   if (i >= n) return;
+   const int x = (int)(i % nx);
+   const int64_t row = i / nx;
+   const int y = (int)(row % ny);
+   const int64_t base = row - y;  // first row of the slice
+   float s;
+   if (axis == 0) {
+      const float* v = in + row * nx;
+      s = k[r] * v[x];
+      for (int j = 1; j <= r; j++) s = s + k[r + j] * (v[reflect101(x - j, nx)] + v[reflect101(x + j, nx)]);
+   } else {
+      const float* v = in + base * nx + x;
+      s = k[r] * v[(int64_t)y * nx];
+      for (int j = 1; j <= r; j++)
+         s = s + k[r + j] * (v[(int64_t)reflect101(y - j, ny) * nx] + v[(int64_t)reflect101(y + j, ny) * nx]);
+   }
+   out[i] = add != 0.f ? s + add : s;
+}
+
+__global__ void k_add(float* v, int64_t n, float add)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) v[i] = v[i] + add;
+}
+
+// Binned (x, y summed in bins of f, y outer and x inner, in float) volume times the window.
+__global__ void k_bin_window(const float* v, int nz, int ny, int nx, int f, int ph, int pw, const double* win,
+                             double* out)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t n = (int64_t)nz * ph * pw;
+   if (i >= n) return;
+   const int X = (int)(i % pw);
+   const int Y = (int)((i / pw) % ph);
+   const int z = (int)(i / ((int64_t)pw * ph));
+   float s = 0.f;
+   for (int y = Y * f; y < min(ny, Y * f + f); y++)
+      for (int x = X * f; x < min(nx, X * f + f); x++) s += v[((int64_t)z * ny + y) * nx + x];
+   out[i] = (double)s * win[i];
+}
+
+__global__ void k_conj(const double2* in, double2* out, int64_t n)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) out[i] = make_double2(in[i].x, -in[i].y);
+}
+
+__global__ void k_cross_power(double2* s, const double2* ref, int64_t n)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) return;
+   const double2 a = s[i], b = ref[i];
+   const double re = a.x * b.x - a.y * b.y, im = a.x * b.y + a.y * b.x;
+   const double nrm = hypot(re, im) + DBL_MIN;
+   s[i] = make_double2(re / nrm, im / nrm);
+}
+
+// First maximum: per workgroup (value, index), ties to the lower index; the last stage is one
+// workgroup over the partials.
+__device__ __forceinline__ void better(double& v, int64_t& i, double v2, int64_t i2)
+{
+   if (v2 > v || (v2 == v && i2 < i)) {
+      v = v2;
+      i = i2;
+   }
+}
+
+__global__ __launch_bounds__(kFB) void k_argmax(const double* v, int64_t n, double* pv, int64_t* pi)
+{
+   __shared__ double sv[kFB];
+   __shared__ int64_t si[kFB];
+   double bv = -DBL_MAX;
+   int64_t bi = INT64_MAX;
+   for (int64_t i = (int64_t)blockIdx.x * kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB)
+      better(bv, bi, v[i], i);
+   sv[threadIdx.x] = bv;
+   si[threadIdx.x] = bi;
+   __syncthreads();
+   for (int off = kFB / 2; off >= 1; off >>= 1) {
+      if (threadIdx.x < off) better(sv[threadIdx.x], si[threadIdx.x], sv[threadIdx.x + off], si[threadIdx.x + off]);
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) {
+      pv[blockIdx.x] = sv[0];
+      pi[blockIdx.x] = si[0];
+   }
+}
+
+__global__ __launch_bounds__(kFB) void k_argmax_final(const double* pv, const int64_t* pi, int m, int64_t* out)
+{
+   __shared__ double sv[kFB];
+   __shared__ int64_t si[kFB];
+   double bv = -DBL_MAX;
+   int64_t bi = INT64_MAX;
+   for (int i = threadIdx.x; i < m; i += kFB) better(bv, bi, pv[i], pi[i]);
+   sv[threadIdx.x] = bv;
+   si[threadIdx.x] = bi;
+   __syncthreads();
+   for (int off = kFB / 2; off >= 1; off >>= 1) {
+      if (threadIdx.x < off) better(sv[threadIdx.x], si[threadIdx.x], sv[threadIdx.x + off], si[threadIdx.x + off]);
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) *out = si[0];
+}
+
+unsigned grid_of(int64_t n) { return (unsigned)((n + kFB - 1) / kFB); }
+
+// Float Gaussian weights of cv::GaussianBlur(.., Size(0, 0), sigma) on a float image:
+// ksize = round(8 sigma + 1) | 1; exp(-(i - c)^2 / (2 sigma^2)) rounded to float, then scaled
+// by 1 / (their double sum) and rounded again.
+std::vector<float> gaussian_weights(double sigma)
+{
+   const int n = (int)std::nearbyint(sigma * 8 + 1) | 1;
+   std::vector<float> k(n);
+   const double c = (n - 1) * 0.5, s2 = -0.5 / (sigma * sigma);
+   double sum = 0;
+   for (int i = 0; i < n; i++) {
+      k[i] = (float)std::exp(s2 * (i - c) * (i - c));
+      sum += k[i];
+   }
+   const double inv = 1.0 / sum;
+   for (int i = 0; i < n; i++) k[i] = (float)(k[i] * inv);
+   return k;
+}
+
+std::vector<double> hann(int n)
+{
+   std::vector<double> w(n, 1.0);
+   if (n > 1) {
+      const double f = 2.0 * 3.14159265358979323846 / (n - 1);
+      for (int i = 0; i < n; i++) w[i] = std::sqrt(0.5 * (1 - std::cos(f * i)));
+   }
+   return w;
+}
+
+std::once_flag g_fft_setup;
+
+}  // namespace
+
+struct fa_aligner {
+   fa_context* ctx = nullptr;
+   fa_realign_params rp{};
+   fa_scan_params scan{};
+   int nz = 0, ny = 0, nx = 0, nD = 0, n_dim = 0;
+   int64_t n_px = 0;
+   fa_reference* ref = nullptr;
+   hipStream_t stream = nullptr;
+   std::mutex mu;
+   // phase correlation over the binned volume (pz, ph, pw)
+   int pz = 0, ph = 0, pw = 0;
+   int64_t n_el = 0, n_cplx = 0;
+   rocfft_plan fwd = nullptr, inv = nullptr;
+   rocfft_execution_info info = nullptr;
+   void* work = nullptr;
+   double* d_win = nullptr;
+   double* d_real = nullptr;
+   double2* d_spec = nullptr;
+   double2* d_refspec = nullptr;
+   double* d_pv = nullptr;
+   int64_t* d_pi = nullptr;
+   int64_t* d_idx = nullptr;
+   int n_part = 0;
+   // blur
+   float* d_kx = nullptr;
+   float* d_ky = nullptr;
+   int rx = 0, ry = 0;
+   float* d_raw = nullptr;  // raw frame as float
+   float* d_tmp = nullptr;
+   void* d_up = nullptr;    // raw upload
+
+   ~fa_aligner()
+   {
+      if (stream) (void)hipStreamSynchronize(stream);
+      if (fwd) rocfft_plan_destroy(fwd);
+      if (inv) rocfft_plan_destroy(inv);
+      if (info) rocfft_execution_info_destroy(info);
+      if (work) (void)hipFree(work);
+      ffree(d_win);
+      ffree(d_real);
+      ffree(d_spec);
+      ffree(d_refspec);
+      ffree(d_pv);
+      ffree(d_pi);
+      ffree(d_idx);
+      ffree(d_kx);
+      ffree(d_ky);
+      ffree(d_raw);
+      ffree(d_tmp);
+      if (d_up) (void)hipFree(d_up);
+      if (ref) fa_reference_release(ref);
+      if (stream) (void)hipStreamDestroy(stream);
+   }
+
+   size_t raw_bytes(int dtype) const { return (size_t)n_px * (dtype == FA_U16 ? 2 : 4); }
+
+   // d_raw = float(raw)
+   int upload(const void* raw, int dtype)
+   {
+      FCHK(hipMemcpyAsync(d_up, raw, raw_bytes(dtype), hipMemcpyHostToDevice, stream));
+      hipLaunchKernelGGL(k_to_float, dim3(grid_of(n_px)), dim3(kFB), 0, stream, (const void*)d_up, dtype, n_px, d_raw);
+      FCHK(hipGetLastError());
+      return FA_OK;
+   }
+
+   // out = smoothStack(d_raw) + 1
+   int smooth_plus_one(float* out)
+   {
+      if (rp.smoothing > 0) {
+         hipLaunchKernelGGL(k_blur, dim3(grid_of(n_px)), dim3(kFB), 0, stream, (const float*)d_raw, d_tmp, nz, ny, nx,
+                            (const float*)d_kx, rx, 0, 0.f);
+         hipLaunchKernelGGL(k_blur, dim3(grid_of(n_px)), dim3(kFB), 0, stream, (const float*)d_tmp, out, nz, ny, nx,
+                            (const float*)d_ky, ry, 1, 1.f);
+      } else {
+         if (out != d_raw) FCHK(hipMemcpyAsync(out, d_raw, n_px * sizeof(float), hipMemcpyDeviceToDevice, stream));
+         hipLaunchKernelGGL(k_add, dim3(grid_of(n_px)), dim3(kFB), 0, stream, out, n_px, 1.f);
+      }
+      FCHK(hipGetLastError());
+      return FA_OK;
+   }
+
+   // spectrum of the binned, windowed d_raw into d_spec
+   int spectrum()
+   {
+      const int f = std::max(1, rp.spatial_binning);
+      hipLaunchKernelGGL(k_bin_window, dim3(grid_of(n_el)), dim3(kFB), 0, stream, (const float*)d_raw, nz, ny, nx, f,
+                         ph, pw, (const double*)d_win, d_real);
+      FCHK(hipGetLastError());
+      void* in[1] = {d_real};
+      void* out[1] = {d_spec};
+      RCHK(rocfft_execute(fwd, in, out, info));
+      return FA_OK;
+   }
+
+   // rigid shift of d_raw
+   int shift(double* s3)
+   {
+      OK(spectrum());
+      hipLaunchKernelGGL(k_cross_power, dim3(grid_of(n_cplx)), dim3(kFB), 0, stream, d_spec,
+                         (const double2*)d_refspec, n_cplx);
+      FCHK(hipGetLastError());
+      void* in[1] = {d_spec};
+      void* out[1] = {d_real};
+      RCHK(rocfft_execute(inv, in, out, info));
+      hipLaunchKernelGGL(k_argmax, dim3(n_part), dim3(kFB), 0, stream, (const double*)d_real, n_el, d_pv, d_pi);
+      hipLaunchKernelGGL(k_argmax_final, dim3(1), dim3(kFB), 0, stream, (const double*)d_pv, (const int64_t*)d_pi,
+                         n_part, d_idx);
+      FCHK(hipGetLastError());
+      int64_t idx = 0;
+      FCHK(hipMemcpyAsync(&idx, d_idx, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+      FCHK(hipStreamSynchronize(stream));
+      int x = (int)(idx % pw), y = (int)((idx / pw) % ph), z = (int)(idx / ((int64_t)pw * ph));
+      if (x > pw / 2) x -= pw;
+      if (y > ph / 2) y -= ph;
+      if (z > pz / 2) z -= pz;
+      const int f = std::max(1, rp.spatial_binning);
+      s3[0] = (double)x * f;
+      s3[1] = (double)y * f;
+      s3[2] = (double)z;
+      return FA_OK;
+   }
+};
+
+namespace {
+
+int aligner_init(fa_aligner* a, const void* reference, int dtype)
+{
+   const int f = std::max(1, a->rp.spatial_binning);
+   a->pz = a->nz;
+   a->ph = (a->ny + f - 1) / f;
+   a->pw = (a->nx + f - 1) / f;
+   a->n_el = (int64_t)a->pz * a->ph * a->pw;
+   a->n_cplx = (int64_t)a->pz * a->ph * (a->pw / 2 + 1);
+   a->n_part = (int)std::min<int64_t>(1024, (a->n_el + kFB - 1) / kFB);
+   FCHK(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
+   OK(falloc(&a->d_win, a->n_el));
+   OK(falloc(&a->d_real, a->n_el));
+   OK(falloc(&a->d_spec, a->n_cplx));
+   OK(falloc(&a->d_refspec, a->n_cplx));
+   OK(falloc(&a->d_pv, a->n_part));
+   OK(falloc(&a->d_pi, a->n_part));
+   OK(falloc(&a->d_idx, 1));
+   OK(falloc(&a->d_raw, a->n_px));
+   OK(falloc(&a->d_tmp, a->n_px));
+   if (hipMalloc(&a->d_up, a->n_px * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      a->d_up = nullptr;
+      return set_error(FA_E_OOM, "Not enough memory to allocate");
+   }
+   // window: 1 in z, square-root Hann in y and x
+   {
+      const std::vector<double> wy = hann(a->ph), wx = hann(a->pw);
+      std::vector<double> w((size_t)a->n_el);
+      size_t i = 0;
+      for (int z = 0; z < a->pz; z++)
+         for (int y = 0; y < a->ph; y++)
+            for (int x = 0; x < a->pw; x++) w[i++] = wy[y] * wx[x];
+      FCHK(hipMemcpy(a->d_win, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+   }
+   if (a->rp.smoothing > 0) {
+      const std::vector<float> kx = gaussian_weights(a->rp.smoothing), ky = gaussian_weights(1.0);
+      a->rx = (int)kx.size() / 2;
+      a->ry = (int)ky.size() / 2;
+      OK(falloc(&a->d_kx, kx.size()));
+      OK(falloc(&a->d_ky, ky.size()));
+      FCHK(hipMemcpy(a->d_kx, kx.data(), kx.size() * sizeof(float), hipMemcpyHostToDevice));
+      FCHK(hipMemcpy(a->d_ky, ky.data(), ky.size() * sizeof(float), hipMemcpyHostToDevice));
+   }
+   // FP64 real-to-complex and complex-to-real plans (lengths fastest first)
+   std::call_once(g_fft_setup, [] { rocfft_setup(); });
+   const size_t dims = a->pz > 1 ? 3 : 2;
+   const size_t len[3] = {(size_t)a->pw, (size_t)a->ph, (size_t)a->pz};
+   RCHK(rocfft_plan_create(&a->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                           rocfft_precision_double, dims, len, 1, nullptr));
+   RCHK(rocfft_plan_create(&a->inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                           rocfft_precision_double, dims, len, 1, nullptr));
+   size_t w1 = 0, w2 = 0;
+   RCHK(rocfft_plan_get_work_buffer_size(a->fwd, &w1));
+   RCHK(rocfft_plan_get_work_buffer_size(a->inv, &w2));
+   RCHK(rocfft_execution_info_create(&a->info));
+   if (std::max(w1, w2) > 0) {
+      if (hipMalloc(&a->work, std::max(w1, w2)) != hipSuccess) {
+         (void)hipGetLastError();
+         a->work = nullptr;
+         return set_error(FA_E_OOM, "Not enough memory to allocate");
+      }
+      RCHK(rocfft_execution_info_set_work_buffer(a->info, a->work, std::max(w1, w2)));
+   }
+   RCHK(rocfft_execution_info_set_stream(a->info, a->stream));
+   // reference: conjugate spectrum of the raw reference, and the warper's reference
+   OK(a->upload(reference, dtype));
+   OK(a->spectrum());
+   hipLaunchKernelGGL(k_conj, dim3(grid_of(a->n_cplx)), dim3(kFB), 0, a->stream, (const double2*)a->d_spec,
+                      a->d_refspec, a->n_cplx);
+   FCHK(hipGetLastError());
+   float* d_sm = nullptr;
+   OK(falloc(&d_sm, a->n_px));
+   std::vector<float> sm((size_t)a->n_px);
+   int rc = a->smooth_plus_one(d_sm);
+   if (rc == FA_OK && (hipMemcpyAsync(sm.data(), d_sm, a->n_px * sizeof(float), hipMemcpyDeviceToHost, a->stream) != hipSuccess ||
+                       hipStreamSynchronize(a->stream) != hipSuccess)) {
+      (void)hipGetLastError();
+      rc = set_error(FA_E_DEVICE, "reference preprocessing failed");
+   }
+   ffree(d_sm);
+   OK(rc);
+   return fa_set_reference(a->ctx, sm.data(), a->nz, a->ny, a->nx, a->nD, &a->scan, 0, &a->ref);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fa_aligner_create(fa_context* ctx, const fa_realign_params* rp, const fa_scan_params* scan,
+                      const void* reference, int32_t dtype, fa_aligner** out)
+{
+   if (!ctx || !rp || !scan || !reference || !out) return set_error(FA_E_ARG, "null argument");
+   *out = nullptr;
+   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+   if (rp->type < FA_REALIGN_NONE || rp->type > FA_REALIGN_WARP || rp->spatial_binning < 1 ||
+       rp->n_resampling_points < 1 || !(rp->smoothing >= 0))
+      return set_error(FA_E_ARG, "invalid realignment parameters");
+   if (scan->n_x < 1 || scan->n_y < 1 || scan->n_z < 1) return set_error(FA_E_ARG, "Reference must be three dimensional");
+   if (hipSetDevice(ctx->device) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(FA_E_DEVICE, "No supported GPU");
+   }
+   auto* a = new fa_aligner();
+   a->ctx = ctx;
+   a->rp = *rp;
+   a->scan = *scan;
+   a->nz = scan->n_z;
+   a->ny = scan->n_y;
+   a->nx = scan->n_x;
+   a->n_px = (int64_t)a->nz * a->ny * a->nx;
+   a->n_dim = a->nz > 1 ? 3 : 2;
+   a->nD = rp->n_resampling_points * a->nz + 1;
+   const int rc = aligner_init(a, reference, dtype);
+   if (rc != FA_OK) {
+      delete a;
+      return rc;
+   }
+   *out = a;
+   return FA_OK;
+}
+
+int fa_aligner_destroy(fa_aligner* a)
+{
+   if (!a) return FA_OK;
+   (void)hipSetDevice(a->ctx->device);
+   delete a;
+   return FA_OK;
+}
+
+int fa_aligner_reference(fa_aligner* a, fa_reference** ref, int32_t* nD)
+{
+   if (!a) return set_error(FA_E_ARG, "null argument");
+   if (ref) *ref = a->ref;
+   if (nD) *nD = a->nD;
+   return FA_OK;
+}
+
+int fa_aligner_preprocess(fa_aligner* a, const void* raw, int32_t dtype, float* out)
+{
+   if (!a || !raw || !out) return set_error(FA_E_ARG, "null argument");
+   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+   std::lock_guard<std::mutex> lk(a->mu);
+   FCHK(hipSetDevice(a->ctx->device));
+   OK(a->upload(raw, dtype));
+   OK(a->smooth_plus_one(a->d_raw));  // in place (the blur passes through d_tmp)
+   FCHK(hipMemcpyAsync(out, a->d_raw, a->n_px * sizeof(float), hipMemcpyDeviceToHost, a->stream));
+   FCHK(hipStreamSynchronize(a->stream));
+   return FA_OK;
+}
+
+int fa_aligner_rigid_shift(fa_aligner* a, const void* raw, int32_t dtype, double* shift3)
+{
+   if (!a || !raw || !shift3) return set_error(FA_E_ARG, "null argument");
+   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+   std::lock_guard<std::mutex> lk(a->mu);
+   FCHK(hipSetDevice(a->ctx->device));
+   OK(a->upload(raw, dtype));
+   return a->shift(shift3);
+}
+
+int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, int32_t dtype,
+                          const fa_tr_params* prm, double* D_out, float* realigned, fa_add_result* results)
+{
+   if (!a || !frames || n <= 0 || !D_out) return set_error(FA_E_ARG, "null argument");
+   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+   for (int i = 0; i < n; i++)
+      if (!frames[i]) return set_error(FA_E_ARG, "null frame");
+   std::lock_guard<std::mutex> lk(a->mu);
+   FCHK(hipSetDevice(a->ctx->device));
+   const int nD = a->nD, m = a->n_dim, np = nD * m;
+   std::vector<fa_frame*> fr(n, nullptr);
+   std::vector<double> shifts(3 * (size_t)n, 0.0);
+   fa_batch* b = nullptr;
+   auto release = [&]() {
+      if (b) fa_batch_destroy(b);
+      for (fa_frame* f : fr)
+         if (f) fa_deregister_frame(f);
+   };
+   int rc = FA_OK;
+   // 1. per frame: rigid shift of the raw frame, then the warper's frame (blur + 1) on the device
+   for (int i = 0; i < n && rc == FA_OK; i++) {
+      float* d_frame = nullptr;
+      if ((rc = a->upload(frames[i], dtype)) != FA_OK || (rc = a->shift(&shifts[3 * (size_t)i])) != FA_OK ||
+          (rc = falloc(&d_frame, a->n_px)) != FA_OK)
+         break;
+      if ((rc = a->smooth_plus_one(d_frame)) != FA_OK || hipStreamSynchronize(a->stream) != hipSuccess) {
+         (void)hipGetLastError();
+         ffree(d_frame);
+         if (rc == FA_OK) rc = set_error(FA_E_DEVICE, "frame preprocessing failed");
+         break;
+      }
+      rc = adopt_device_frame(a->ref, d_frame, FA_F32, 0.f, &fr[i]);
+   }
+   if (rc != FA_OK) {
+      release();
+      return rc;
+   }
+   fa_tr_params p;
+   if (prm) p = *prm;
+   else fa_tr_default(&p);
+   if ((rc = fa_batch_create(fr.data(), n, &p, &b)) != FA_OK) {
+      release();
+      return rc;
+   }
+   // 2. SSD at the zero start (twice in the reference) and at the rigid start
+   std::vector<double> D0(3 * (size_t)nD * n, 0.0), Dr(3 * (size_t)nD * n), e0(n), er(n);
+   for (int i = 0; i < n; i++)
+      for (int k = 0; k < nD; k++)
+         for (int c = 0; c < 3; c++) Dr[((size_t)i * nD + k) * 3 + c] = shifts[3 * (size_t)i + c];
+   if ((rc = fa_batch_eval(b, D0.data(), e0.data(), nullptr)) != FA_OK ||
+       (rc = fa_batch_eval(b, Dr.data(), er.data(), nullptr)) != FA_OK) {
+      release();
+      return rc;
+   }
+   // 3. start: the first smallest of {e0, e0, er}; x in D2col order
+   std::vector<double> x((size_t)np * n, 0.0);
+   std::vector<int> start(n, 0);
+   for (int i = 0; i < n; i++) {
+      start[i] = er[i] < e0[i] ? 2 : 0;
+      if (start[i] == 2)
+         for (int k = 0; k < nD; k++)
+            for (int c = 0; c < m; c++) x[(size_t)i * np + k * m + c] = shifts[3 * (size_t)i + c];
+   }
+   // 4. trust region (Warp); None keeps D = 0 (NullFrameAligner), Translation the start
+   std::vector<fa_align_stats> st(n);
+   std::memset(st.data(), 0, st.size() * sizeof(fa_align_stats));
+   if (a->rp.type == FA_REALIGN_WARP) {
+      if ((rc = fa_batch_run(b, x.data(), st.data(), nullptr)) != FA_OK) {
+         release();
+         return rc;
+      }
+   } else if (a->rp.type == FA_REALIGN_NONE) {
+      std::fill(x.begin(), x.end(), 0.0);
+   }
+   for (int i = 0; i < n; i++) {
+      double* D = D_out + (size_t)i * nD * 3;
+      for (int k = 0; k < nD; k++)
+         for (int c = 0; c < 3; c++) D[k * 3 + c] = c < m ? x[(size_t)i * np + k * m + c] : 0.0;
+      if (results) {
+         fa_add_result& r = results[i];
+         std::memset(&r, 0, sizeof(r));
+         r.start = start[i];
+         for (int c = 0; c < 3; c++) r.rigid_shift[c] = shifts[3 * (size_t)i + c];
+         r.start_error[0] = r.start_error[1] = e0[i];
+         r.start_error[2] = er[i];
+         r.stats = st[i];
+      }
+      if (realigned && (rc = fa_warp_interpolated(fr[i], D, realigned + (size_t)i * a->n_px, 0.f)) != FA_OK) {
+         release();
+         return rc;
+      }
+   }
+   release();
+   return FA_OK;
+}
+
+}  // extern "C"

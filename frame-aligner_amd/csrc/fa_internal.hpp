@@ -190,3 +190,9 @@ struct fa_frame {
    void* d_data = nullptr;
    fa_batch* single = nullptr;  // cached one-frame batch for fa_align and single evaluations
 };
+
+namespace fa {
+// Sets the calling thread's fa_last_error() message; returns code.
+int set_error(int code, const std::string& msg);
+int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offset, fa_frame** out);
+}  // namespace fa

@@ -55,6 +55,20 @@ class Tuning(C.Structure):
                 ("check_every", C.c_int32), ("use_graph", C.c_int32), ("tr_lds", C.c_int32)]
 
 
+class RealignParams(C.Structure):
+    """RealignmentParameters (AbstractFrameAligner.h:31-48), the fields addFrame reads."""
+    _fields_ = [("type", C.c_int32), ("spatial_binning", C.c_int32), ("n_resampling_points", C.c_int32),
+                ("pad", C.c_int32), ("smoothing", C.c_double)]
+
+
+class AddResult(C.Structure):
+    _fields_ = [("start", C.c_int32), ("pad", C.c_int32), ("rigid_shift", C.c_double * 3),
+                ("start_error", C.c_double * 3), ("stats", AlignStats)]
+
+
+REALIGN_TYPES = {"none": 0, "translation": 1, "warp": 2}
+
+
 class ServerResult(C.Structure):
     _fields_ = [("tag", C.c_int64), ("stats", AlignStats)]
 
@@ -74,6 +88,8 @@ EXPORTS = [
     "fa_context_stream", "fa_context_mem_info", "fa_context_info", "fa_server_create", "fa_server_destroy",
     "fa_server_submit", "fa_server_submit_host", "fa_server_step", "fa_server_results",
     "fa_server_memory", "fa_host_alloc", "fa_host_free",
+    "fa_aligner_create", "fa_aligner_destroy", "fa_aligner_reference", "fa_aligner_preprocess",
+    "fa_aligner_rigid_shift", "fa_aligner_add_frames",
 ]
 
 _lib = None
@@ -119,6 +135,12 @@ def load_library(path=None):
     L.fa_align_batch.argtypes = [P, i32, C.POINTER(TrParams), P, P]
     L.fa_context_mem_info.argtypes = [P, P, P]
     L.fa_context_info.argtypes = [P, P]
+    L.fa_aligner_create.argtypes = [P, C.POINTER(RealignParams), C.POINTER(ScanParams), P, i32, C.POINTER(P)]
+    L.fa_aligner_destroy.argtypes = [P]
+    L.fa_aligner_reference.argtypes = [P, C.POINTER(P), C.POINTER(i32)]
+    L.fa_aligner_preprocess.argtypes = [P, P, i32, P]
+    L.fa_aligner_rigid_shift.argtypes = [P, P, i32, P]
+    L.fa_aligner_add_frames.argtypes = [P, P, i32, i32, C.POINTER(TrParams), P, P, P]
     L.fa_server_create.argtypes = [P, P, i32, i32, C.c_uint64, C.POINTER(TrParams), C.POINTER(P)]
     L.fa_server_submit_host.argtypes = [P, P, P, i32, C.c_float, P, C.c_int64]
     L.fa_server_memory.argtypes = [P, P, P]
@@ -176,7 +198,7 @@ class Context:
         """Releases every batch, frame and reference made on this context, then the context."""
         if getattr(self, "h", None):
             kids = list(self._children)
-            for kind in (Server, Batch, Frame, Reference):
+            for kind in (Aligner, Server, Batch, Frame, Reference):
                 for k in kids:
                     if isinstance(k, kind):
                         k._release_handle()
@@ -538,6 +560,89 @@ class Server:
             if n.value < len(self._buf):
                 break
         return a.value, q.value, out
+
+
+class Aligner:
+    """FrameWarpAligner on the device (FrameWarpAligner.cpp:46-223): raw reference and frames,
+    optional blur, +1, rigid phase-correlation start, best of the starts, trust region."""
+
+    def __init__(self, ctx, reference, type="warp", n_resampling_points=30, smoothing=0.0,
+                 spatial_binning=1, line=1.0, interline=1.0, interframe=None):
+        self.ctx = ctx
+        self.L = ctx.L
+        ref = np.ascontiguousarray(reference)
+        if ref.ndim == 2:
+            ref = ref[None]
+        self.dtype = FA_U16 if ref.dtype == np.uint16 else FA_F32
+        if self.dtype == FA_F32:
+            ref = np.ascontiguousarray(ref, dtype=np.float32)
+        nz, ny, nx = ref.shape
+        self.shape = (nz, ny, nx)
+        if interframe is None:
+            interframe = ny * max(interline, line)
+        sp = ScanParams(line, interline, interframe, nx, ny, nz, 0)
+        rp = RealignParams(REALIGN_TYPES[type] if isinstance(type, str) else int(type), spatial_binning,
+                           n_resampling_points, 0, smoothing)
+        h = C.c_void_p()
+        _check(self.L.fa_aligner_create(ctx.h, C.byref(rp), C.byref(sp), _ptr(ref), self.dtype, C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+        nD = C.c_int32()
+        _check(self.L.fa_aligner_reference(self.h, None, C.byref(nD)))
+        self.nD = nD.value
+        self.n_dim = 3 if nz > 1 else 2
+
+    def _release_handle(self):
+        if getattr(self, "h", None):
+            self.L.fa_aligner_destroy(self.h)
+            self.h = None
+
+    close = _release_handle
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _raw(self, frame):
+        a = np.ascontiguousarray(frame)
+        a = a if a.dtype == np.uint16 else np.ascontiguousarray(a, dtype=np.float32)
+        if a.size != int(np.prod(self.shape)):
+            raise FaError(-1, "Frame is incorrect size")
+        return a, (FA_U16 if a.dtype == np.uint16 else FA_F32)
+
+    def preprocess(self, frame):
+        """smoothStack(frame) + 1, as the warper receives it."""
+        a, dt = self._raw(frame)
+        out = np.zeros(self.shape, np.float32)
+        _check(self.L.fa_aligner_preprocess(self.h, _ptr(a), dt, _ptr(out)))
+        return out
+
+    def rigid_shift(self, frame):
+        a, dt = self._raw(frame)
+        o = np.zeros(3)
+        _check(self.L.fa_aligner_rigid_shift(self.h, _ptr(a), dt, _ptr(o)))
+        return tuple(float(v) for v in o)
+
+    def add_frames(self, frames, realigned=False, **kw):
+        """addFrame for each raw frame (one dtype): returns (D list (nD x 3), result dicts,
+        realigned frames or None)."""
+        arrs = [self._raw(f) for f in frames]
+        dts = {dt for _, dt in arrs}
+        if len(dts) != 1:
+            raise FaError(-1, "frames of one dtype expected")
+        n = len(arrs)
+        ptrs = (C.c_void_p * n)(*[a.ctypes.data for a, _ in arrs])
+        D = np.zeros((n, self.nD, 3))
+        res = (AddResult * n)()
+        out = np.zeros((n,) + self.shape, np.float32) if realigned else None
+        p = tr_params(**kw)
+        _check(self.L.fa_aligner_add_frames(self.h, ptrs, n, dts.pop(), C.byref(p), _ptr(D),
+                                           _ptr(out) if realigned else None, res))
+        results = [dict(start=r.start, rigid_shift=tuple(r.rigid_shift), start_error=tuple(r.start_error),
+                        stats=stats_dict(r.stats)) for r in res]
+        return [D[i] for i in range(n)], results, out
 
 
 class HipFrameWarper:

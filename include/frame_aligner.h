@@ -231,6 +231,57 @@ int fa_host_free(void* p);
 /* Free and total device memory (hipMemGetInfo), for admission by free memory. */
 int fa_context_mem_info(fa_context* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 
+/* ---- aligner front end: FrameWarpAligner::setReference / addFrame ------------------------
+   RealignmentParameters (FrameAligner/AbstractFrameAligner.h:31-48), the fields the warp
+   aligner reads. */
+enum fa_realign_type { FA_REALIGN_NONE = 0, FA_REALIGN_TRANSLATION = 1, FA_REALIGN_WARP = 2 };
+
+typedef struct fa_realign_params {
+   int32_t type;                /* fa_realign_type (reference default None; Warp runs the TR) */
+   int32_t spatial_binning;     /* phase-correlation downsampling in x and y (1) */
+   int32_t n_resampling_points; /* nodes per plane (30): nD = n * nz + 1 (FrameWarpAligner.cpp:100) */
+   int32_t pad;
+   double smoothing;            /* Gaussian sigma in x of the per-slice blur, 0 = none; sigma in y
+                                   is 1 (FrameWarpAligner.cpp:46-64) */
+} fa_realign_params;
+
+typedef struct fa_aligner fa_aligner;
+
+/* Per frame result of fa_aligner_add_frames: the three starts of addFrame
+   (FrameWarpAligner.cpp:150-198) are {0, 0, rigid}; start is the index of the first with the
+   smallest SSD. */
+typedef struct fa_add_result {
+   int32_t start;
+   int32_t pad;
+   double rigid_shift[3];  /* phase-correlation shift: x, y (scaled by the binning), z */
+   double start_error[3];  /* SSD at the three starts */
+   fa_align_stats stats;   /* trust region (Warp only) */
+} fa_add_result;
+
+/* FrameWarpAligner(params) + setReference(0, reference) (FrameWarpAligner.cpp:68-105):
+   reference is the raw nz*ny*nx volume of dtype (FA_U16 counts or FA_F32). Builds the blurred
+   reference + 1 for the warper (as fa_set_reference) and the phase correlator's reference
+   spectrum (windowed, binned, FP64 3-D FFT). */
+int fa_aligner_create(fa_context* ctx, const fa_realign_params* rp, const fa_scan_params* scan,
+                      const void* reference, int32_t dtype, fa_aligner** out);
+int fa_aligner_destroy(fa_aligner* a);
+/* The warper's reference (read-only; owned by the aligner) and nD. */
+int fa_aligner_reference(fa_aligner* a, fa_reference** ref, int32_t* nD);
+/* smoothStack + 1 of one raw frame (FrameWarpAligner.cpp:46-64,141-142): out gets the
+   nz*ny*nx floats the warper aligns. */
+int fa_aligner_preprocess(fa_aligner* a, const void* raw, int32_t dtype, float* out);
+/* Rigid start of one raw frame (VolumePhaseCorrelator::computeShift of the binned frame, x and
+   y scaled by the binning, FrameWarpAligner.cpp:186-189). */
+int fa_aligner_rigid_shift(fa_aligner* a, const void* raw, int32_t dtype, double* shift3);
+/* FrameWarpAligner::addFrame (FrameWarpAligner.cpp:117-223) for n raw frames of dtype: blur
+   and +1, the rigid start, the start with the smallest SSD, then (type Warp) the trust region
+   of all n frames as one batch. D_out: n x nD x 3; realigned (may be NULL): n x nz*ny*nx floats
+   of warpImageInterpolated(frame, D) with invalid value 0 (FrameWarpAligner.cpp:222);
+   results: n entries (may be NULL). prm NULL = the reference's stop rule. */
+int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, int32_t dtype,
+                          const fa_tr_params* prm, double* D_out, float* realigned,
+                          fa_add_result* results);
+
 /* out[0..4] = device ordinal, compute units, bytes of LDS one workgroup may allocate,
    wavefront size, hipError_t of raising the kernels' dynamic-LDS limit (0 = success). */
 int fa_context_info(const fa_context* ctx, int32_t* out5);

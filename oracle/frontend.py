@@ -1,0 +1,144 @@
+"""CPU restatement of FrameWarpAligner::addFrame's front end (numpy, float64 FFTs).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the device front end.
+
+* downsample            - AbstractFrameAligner.cpp:37-56 (x, y summed in bins, z kept)
+* hann / window         - VolumePhaseCorrelator.cpp:134-158 (square root of the Hann window in
+                          y and x, 1 in z)
+* PhaseCorrelator       - VolumePhaseCorrelator.cpp:10-131: reference spectrum conj(FFT(ref * w)),
+                          cross-power spectrum normalised by |.| + DBL_MIN, inverse FFT, first
+                          maximum, indices above half the size wrapped negative; the sub-pixel
+                          centroid is computed by the reference and discarded, so the shift is
+                          integer
+* gaussian_kernel       - OpenCV's getGaussianKernel for cv::GaussianBlur(.., Size(0, 0), sx, sy)
+                          on float images: ksize = round(8 sigma + 1) | 1, weights
+                          exp(-(i - (ksize-1)/2)^2 / (2 sigma^2)) normalised to sum 1 (float
+                          weights). OpenCV is not in this image and its version is not pinned:
+                          the blur is parity unpinned against OpenCV itself at the level of
+                          rounding (its SIMD row/column filters may sum in another order).
+* smooth_stack          - FrameWarpAligner.cpp:46-64: per z-slice GaussianBlur with sigma x =
+                          smoothing and sigma y = 1 (the reference's second argument), border
+                          BORDER_REFLECT_101, row pass then column pass in float32
+* rigid start, starts   - FrameWarpAligner.cpp:150-198: starts {0, 0, rigid}, the first with the
+                          smallest SSD
+"""
+import numpy as np
+
+DBL_MIN = np.finfo(np.float64).tiny
+
+
+def downsample(vol, factor):
+    """im2[z, y // f, x // f] += im1[z, y, x] in float32, y outer and x inner."""
+    vol = np.asarray(vol, np.float32)
+    nz, h, w = vol.shape
+    if factor == 1:
+        return vol.copy()
+    nh, nw = -(-h // factor), -(-w // factor)
+    out = np.zeros((nz, nh, nw), np.float32)
+    for y in range(h):
+        for x in range(w):
+            out[:, y // factor, x // factor] += vol[:, y, x]
+    return out
+
+
+def hann(n):
+    if n <= 1:
+        return np.ones(max(n, 1))
+    f = 2.0 * np.pi / (n - 1)
+    i = np.arange(n, dtype=np.float64)
+    return np.sqrt(0.5 * (1.0 - np.cos(f * i)))
+
+
+def window(dims):
+    nz, ny, nx = dims
+    return np.ones(nz)[:, None, None] * hann(ny)[None, :, None] * hann(nx)[None, None, :]
+
+
+class PhaseCorrelator:
+    def __init__(self, dims):
+        self.dims = tuple(int(d) for d in dims)
+        self.w = window(self.dims)
+        self.ref = None
+
+    def _spec(self, vol):
+        v = np.asarray(vol, np.float32).astype(np.float64) * self.w
+        return np.fft.rfftn(v, axes=(0, 1, 2))
+
+    def set_reference(self, vol):
+        self.ref = np.conj(self._spec(vol))
+        return self
+
+    def correlation(self, vol):
+        p = self._spec(vol) * self.ref
+        p = p / (np.abs(p) + DBL_MIN)
+        return np.fft.irfftn(p, s=self.dims, axes=(0, 1, 2)) * np.prod(self.dims)  # unnormalised
+
+    def compute_shift(self, vol):
+        c = self.correlation(vol)
+        idx = int(np.argmax(c.ravel()))  # first maximum
+        nz, ny, nx = self.dims
+        x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+        if x > nx // 2:
+            x -= nx
+        if y > ny // 2:
+            y -= ny
+        if z > nz // 2:
+            z -= nz
+        return float(x), float(y), float(z)
+
+
+def rigid_shift(correlator, raw_frame, binning):
+    """(x, y, z) of the rigid start: phase correlation of the binned raw frame, x and y scaled
+    by the binning (FrameWarpAligner.cpp:186-189)."""
+    x, y, z = correlator.compute_shift(downsample(raw_frame, binning))
+    return x * binning, y * binning, z
+
+
+def gaussian_kernel(sigma):
+    """Float weights: each exp() rounded to float, normalised by 1 / (their double sum) and
+    rounded again, as getGaussianKernel does for a float kernel."""
+    ksize = int(np.rint(sigma * 8 + 1)) | 1
+    c = (ksize - 1) * 0.5
+    i = np.arange(ksize, dtype=np.float64)
+    t = np.exp(-0.5 / (sigma * sigma) * (i - c) ** 2).astype(np.float32)
+    inv = 1.0 / float(np.sum(t.astype(np.float64)))
+    return (t.astype(np.float64) * inv).astype(np.float32)
+
+
+def _reflect101(i, n):
+    if n == 1:
+        return 0
+    period = 2 * n - 2
+    i = abs(i) % period
+    return period - i if i >= n else i
+
+
+def _filter_axis(img, k, axis):
+    """Symmetric filter along axis in float32: k[r] x[i] + sum_{j=1..r} k[r+j] (x[i-j] + x[i+j]),
+    j ascending, rows outside the image reflected (BORDER_REFLECT_101)."""
+    img = np.asarray(img, np.float32)
+    n = img.shape[axis]
+    r = len(k) // 2
+    take = lambda d: np.take(img, np.array([_reflect101(i + d, n) for i in range(n)]), axis=axis)
+    out = np.float32(k[r]) * img
+    for j in range(1, r + 1):
+        out = (out + np.float32(k[r + j]) * (take(-j) + take(j))).astype(np.float32)
+    return out.astype(np.float32)
+
+
+def smooth_stack(vol, smoothing):
+    """Per-slice blur (sigma x = smoothing, sigma y = 1) of a (nz, ny, nx) float volume; the
+    volume converted to float when smoothing is 0."""
+    vol = np.asarray(vol, np.float32)
+    if not smoothing > 0:
+        return vol.copy()
+    kx, ky = gaussian_kernel(smoothing), gaussian_kernel(1.0)
+    out = np.empty_like(vol)
+    for z in range(vol.shape[0]):
+        out[z] = _filter_axis(_filter_axis(vol[z], kx, 1), ky, 0)
+    return out
+
+
+def choose_start(errors):
+    """Index of the start addFrame takes: the first smallest SSD (std::min_element)."""
+    return int(np.argmin(np.asarray(errors)))

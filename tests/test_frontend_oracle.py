@@ -1,0 +1,124 @@
+"""Pins of the front-end oracle (oracle/frontend.py): rigid phase-correlation start, binning,
+window, Gaussian blur and start selection of FrameWarpAligner::addFrame
+(FrameWarpAligner.cpp:46-64,117-198; VolumePhaseCorrelator.cpp; AbstractFrameAligner.cpp:37-56).
+Each check is chosen so that a plausible mistake (sign of the shift, swapped axes, wrap
+off-by-one, binning scale, blur axis or border rule) fails it."""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import oracle as O
+import frontend as FE
+from fa_amd import synth
+
+
+def test_hann_window_shape():
+    assert not FE.hann(2).any()   # both samples at the window's zeros (the reference's formula)
+    for n in (5, 8, 64):
+        w = FE.hann(n)
+        assert w[0] == 0.0 and abs(w[-1]) < 1e-12          # sqrt(0.5 (1 - cos 0)) = 0 at both ends
+        assert np.allclose(w, w[::-1])
+        assert np.isclose(w.max(), 1.0, atol=0.1 if n % 2 == 0 else 1e-12)
+    assert np.array_equal(FE.hann(1), [1.0])
+    W = FE.window((3, 8, 6))
+    assert np.array_equal(W[0], W[2])                        # z is not windowed
+
+
+def test_downsample_sums_bins():
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=(2, 6, 8)).astype(np.float32)
+    d = FE.downsample(v, 2)
+    assert d.shape == (2, 3, 4)
+    assert np.allclose(d, v.reshape(2, 3, 2, 4, 2).sum(axis=(2, 4)), rtol=1e-6)
+    r = FE.downsample(rng.normal(size=(1, 7, 5)).astype(np.float32), 3)   # ragged: ceil sizes
+    assert r.shape == (1, 3, 2)
+
+
+def _phantom_pair(shape, shift, seed=3):
+    """A blob phantom and the same blobs moved by shift (x, y, z), evaluated analytically, so
+    frame(q) = reference(q - shift)."""
+    nz, ny, nx = shape
+    rng = np.random.default_rng(seed)
+    n = 60
+    c = np.stack([rng.uniform(12, nx - 12, n), rng.uniform(12, ny - 12, n),
+                  rng.uniform(1, max(nz - 1, 1), n) if nz > 1 else np.zeros(n)], 1)
+    s = rng.uniform(2, 4, n)
+    a = rng.uniform(20, 200, n)
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+
+    def render(off):
+        img = np.full(shape, 5.0)
+        for i in range(n):
+            r2 = (xx - c[i, 0] - off[0]) ** 2 + (yy - c[i, 1] - off[1]) ** 2
+            if nz > 1:
+                r2 = r2 + (zz - c[i, 2] - off[2]) ** 2
+            img += a[i] * np.exp(-0.5 * r2 / s[i] ** 2)
+        return img.astype(np.float32)
+
+    return render((0, 0, 0)), render(shift)
+
+
+@pytest.mark.parametrize("shape,shift", [((1, 64, 80), (5, -3, 0)), ((1, 64, 64), (-7, 4, 0)),
+                                         ((6, 48, 40), (3, -2, 1))])
+def test_rigid_shift_sign_and_axes(shape, shift):
+    ref, frame = _phantom_pair(shape, shift)
+    pc = FE.PhaseCorrelator(shape).set_reference(ref)
+    s = FE.rigid_shift(pc, frame, 1)
+    n_dim = 3 if shape[0] > 1 else 2
+    assert s[:n_dim] == tuple(float(v) for v in shift[:n_dim])
+    # the shift is the displacement the warper needs: frame(p + D) = reference(p)
+    nD = 5 * shape[0] + 1
+    o = O.Oracle().set_reference(ref + 1, nD)
+    D = np.tile(np.array(s, float), (nD, 1))
+    assert o.get_error(frame + 1, D) < 0.05 * o.get_error(frame + 1, np.zeros((nD, 3)))
+
+
+def test_rigid_shift_binning_scales_x_y():
+    ref, frame = _phantom_pair((1, 64, 64), (6, -4, 0))
+    pc = FE.PhaseCorrelator((1, 32, 32)).set_reference(FE.downsample(ref, 2))
+    assert FE.rigid_shift(pc, frame, 2) == (6.0, -4.0, 0.0)
+
+
+def test_wrap_rule():
+    pc = FE.PhaseCorrelator((4, 8, 10))
+    for (x, y, z), expect in [((5, 0, 0), (5, 0, 0)), ((6, 0, 0), (-4, 0, 0)), ((0, 4, 0), (0, 4, 0)),
+                              ((0, 5, 0), (0, -3, 0)), ((0, 0, 2), (0, 0, 2)), ((0, 0, 3), (0, 0, -1))]:
+        c = np.zeros((4, 8, 10))
+        c[z, y, x] = 1.0
+        c[3, 7, 9] = 1.0  # an equal maximum later in scan order is not taken
+        pc.correlation = lambda v, c=c: c
+        assert pc.compute_shift(None) == expect
+
+
+def test_gaussian_kernel_sizes_and_sum():
+    for sigma, n in ((1.0, 9), (2.0, 17), (0.5, 5), (1.3, 11)):
+        k = FE.gaussian_kernel(sigma)
+        assert len(k) == n and k.dtype == np.float32
+        assert abs(float(k.astype(np.float64).sum()) - 1.0) < 1e-6
+        assert np.array_equal(k, k[::-1]) and np.argmax(k) == n // 2
+
+
+def test_blur_axes_and_border():
+    # a delta blurs into outer(ky, kx): sigma x on x, sigma y = 1 on y
+    v = np.zeros((2, 31, 33), np.float32)
+    v[1, 15, 16] = 1.0
+    out = FE.smooth_stack(v, 2.0)
+    kx, ky = FE.gaussian_kernel(2.0), FE.gaussian_kernel(1.0)
+    expect = np.zeros((31, 33))
+    expect[15 - 4:15 + 5, 16 - 8:16 + 9] = np.outer(ky, kx)
+    assert np.allclose(out[1], expect, atol=1e-7) and not out[0].any()
+    # constants stay constant; reflect-101 border equals scipy's 'mirror' correlation
+    assert np.allclose(FE.smooth_stack(np.full((1, 12, 9), 7.0, np.float32), 1.5), 7.0, rtol=1e-6)
+    rng = np.random.default_rng(2)
+    img = rng.uniform(0, 100, (1, 20, 17)).astype(np.float32)
+    ref = ndimage.correlate1d(img[0].astype(np.float64), FE.gaussian_kernel(1.5).astype(np.float64), axis=1,
+                              mode="mirror")
+    ref = ndimage.correlate1d(ref, FE.gaussian_kernel(1.0).astype(np.float64), axis=0, mode="mirror")
+    assert np.allclose(FE.smooth_stack(img, 1.5)[0], ref, rtol=1e-5)
+    assert np.array_equal(FE.smooth_stack(img, 0.0), img)
+
+
+def test_choose_start_first_minimum():
+    assert FE.choose_start([3.0, 3.0, 2.0]) == 2
+    assert FE.choose_start([2.0, 2.0, 2.0]) == 0   # ties keep the zero start
+    assert FE.choose_start([1.0, 1.0, 2.0]) == 0

@@ -1,0 +1,128 @@
+"""Device front end of the aligner (fa_aligner_*, FrameWarpAligner::setReference/addFrame)
+against the front-end oracle (oracle/frontend.py) and the warper oracle: rigid phase-correlation
+start, blur + 1, start selection and the trust region from the chosen start."""
+import numpy as np
+import pytest
+
+import oracle as O
+import frontend as FE
+import fa_amd as F
+from fa_amd import synth
+from test_frontend_oracle import _phantom_pair
+
+pytestmark = pytest.mark.gpu
+
+D_TOL = 0.02
+ERR_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = F.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("shape,shift,binning", [((1, 64, 80), (5, -3, 0), 1), ((1, 64, 64), (-7, 4, 0), 1),
+                                                 ((6, 48, 40), (3, -2, 1), 1), ((1, 64, 64), (6, -4, 0), 2)])
+def test_rigid_shift_matches_oracle(ctx, shape, shift, binning):
+    ref, frame = _phantom_pair(shape, shift)
+    n_dim = 3 if shape[0] > 1 else 2
+    al = F.Aligner(ctx, ref, n_resampling_points=4, spatial_binning=binning)
+    pc = FE.PhaseCorrelator(FE.downsample(ref, binning).shape).set_reference(FE.downsample(ref, binning))
+    s_o = FE.rigid_shift(pc, frame, binning)
+    s_g = al.rigid_shift(frame)
+    assert s_g == s_o
+    assert s_g[:n_dim] == tuple(float(v) for v in shift[:n_dim])
+    # uint16 input gives the same shift as its float conversion
+    q = np.clip(np.rint(frame), 0, 65535).astype(np.uint16)
+    assert al.rigid_shift(q) == FE.rigid_shift(pc, q.astype(np.float32), binning)
+    al.close()
+
+
+@pytest.mark.parametrize("smoothing", [0.0, 1.5, 3.0])
+@pytest.mark.parametrize("dtype", ["float32", "uint16"])
+def test_preprocess_matches_oracle(ctx, smoothing, dtype):
+    st = synth.make_stack((3, 40, 52), 1, 7, seed=4, n_blobs=50, sigma=(1.5, 4.0), dtype=dtype)
+    al = F.Aligner(ctx, st["reference"], n_resampling_points=2, smoothing=smoothing)
+    raw = st["frames"][0]
+    out = al.preprocess(raw)
+    expect = FE.smooth_stack(raw.astype(np.float32), smoothing) + np.float32(1.0)
+    assert np.max(np.abs(out - expect)) <= 1e-5 * np.max(np.abs(expect))
+    if smoothing == 0:
+        assert np.array_equal(out, expect)
+    al.close()
+
+
+def _offset_frames(shape, nD, n, offset, seed, n_blobs=160, scale=4.0):
+    """Frames of the generator's motion model (gentle: random walk 0.1 px per node, 0.5 px
+    sinusoid) plus a rigid offset (x, y) on every node, so that the rigid start is the better
+    one for the offset frames; raw float photon counts (phantom scaled by `scale`)."""
+    rng = np.random.default_rng(seed)
+    n_dim = 3 if shape[0] > 1 else 2
+    ph = scale * synth.blob_phantom(shape, n_blobs, (2.0, 5.0), rng=rng)
+    u = synth.node_coordinate(shape, nD)
+    frames, Ds = [], []
+    for i in range(n):
+        D = synth.node_displacements(nD, n_dim, rng, walk_sigma=0.1, sin_amp=0.5)
+        D[:, :2] += offset[i]
+        clean = np.maximum(synth.displaced(ph, synth.displacement_field(u, D)), 0)
+        frames.append(rng.poisson(clean).astype(np.float32))
+        Ds.append(D)
+    return rng.poisson(ph).astype(np.float32), frames, Ds
+
+
+@pytest.mark.parametrize("smoothing", [0.0, 1.0])
+def test_add_frames_matches_oracle(ctx, smoothing):
+    shape, n_res = (1, 128, 112), 8
+    nD = n_res + 1
+    ref, frames, _ = _offset_frames(shape, nD, 4, [(0, 0), (3, -2), (-2, 3), (0.4, 0.3)], seed=11)
+    al = F.Aligner(ctx, ref, n_resampling_points=n_res, smoothing=smoothing)
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    D_g, res, realigned = al.add_frames(frames, realigned=True, **kw)
+    ref_w = FE.smooth_stack(ref, smoothing) + np.float32(1.0)
+    orc = O.Oracle().set_reference(ref_w, nD)
+    pc = FE.PhaseCorrelator(shape).set_reference(ref)
+    rng = np.random.default_rng(0)
+    starts, compared = [], []
+    for i, raw in enumerate(frames):
+        fw = FE.smooth_stack(raw, smoothing) + np.float32(1.0)
+        s = FE.rigid_shift(pc, raw, 1)
+        assert res[i]["rigid_shift"] == s
+        Dr = np.tile(np.array(s), (nD, 1))
+        e0, er = orc.get_error(fw, np.zeros((nD, 3))), orc.get_error(fw, Dr)
+        assert abs(res[i]["start_error"][0] - e0) <= ERR_TOL * e0
+        assert abs(res[i]["start_error"][2] - er) <= ERR_TOL * er
+        k = FE.choose_start([e0, e0, er])
+        assert res[i]["start"] == k
+        starts.append(k)
+        w_o = orc.warp_interpolated(fw, D_g[i], 0.0)
+        assert np.max(np.abs(realigned[i] - w_o)) <= 1e-5 * np.max(np.abs(w_o))
+        # the displacement is compared where the trust region from this start is well posed:
+        # a 1e-5 px perturbation of the start moves the oracle's result by < 0.005 px
+        x0 = Dr[:, :2].reshape(-1) if k == 2 else np.zeros(2 * nD)
+        x_o, _ = orc.align(fw, x0=x0, **kw)
+        moves = [np.max(np.abs(orc.align(fw, x0=x0 + rng.normal(0, 1e-5, x0.size), **kw)[0] - x_o))
+                 for _ in range(3)]
+        if max(moves) < 0.005:
+            assert np.max(np.abs(D_g[i][:, :2].reshape(-1) - x_o)) <= D_TOL
+            compared.append(k)
+    assert 2 in starts and 0 in starts   # both kinds of start occur in this set
+    # From the integer rigid start the reference's trust region (radius 40) is seldom well posed
+    # on this workload (1e-5 px start perturbations move the oracle's result by 0.1-1 px), so
+    # the displacement parity is certified on the zero-start frames here; the rigid start itself
+    # is compared bit for bit above, and the trust region from a given start is pinned by
+    # test_gpu_parity.test_align_many_nodes_matches_oracle.
+    assert 0 in compared
+
+
+def test_translation_type_returns_the_start(ctx):
+    shape = (1, 64, 64)
+    ref, frames, _ = _offset_frames(shape, 9, 2, [(5, -3), (0, 0)], seed=12)
+    al = F.Aligner(ctx, ref, type="translation", n_resampling_points=8)
+    D, res, _ = al.add_frames(frames)
+    for Di, r in zip(D, res):
+        expect = np.tile(np.array(r["rigid_shift"])[:2], (al.nD, 1)) if r["start"] == 2 else np.zeros((al.nD, 2))
+        assert np.array_equal(Di[:, :2], expect) and not Di[:, 2].any()
+        assert r["stats"]["evaluations"] == 0
+    al.close()
