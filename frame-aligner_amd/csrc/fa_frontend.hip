@@ -22,6 +22,9 @@
 #include <vector>
 
 #include "fa_internal.hpp"
+#include "fa_kernels.hpp"
+
+#include <cstdio>
 
 using namespace fa;
 
@@ -195,6 +198,84 @@ __global__ __launch_bounds__(kFB) void k_argmax_final(const double* pv, const in
    if (threadIdx.x == 0) *out = si[0];
 }
 
+// float -> uint16 as cv::Mat::convertTo(CV_16U) does: round to nearest even, saturate to
+// [0, 65535]; NaN, infinities and values beyond the int range convert to 0 (the x86 integer
+// conversion's out-of-range value, then saturated).
+__device__ __forceinline__ uint16_t to_u16(float v)
+{
+   if (!(fabsf(v) < 2147483648.f)) return 0;
+   const float r = rintf(v);
+   return (uint16_t)(r < 0.f ? 0.f : (r > 65535.f ? 65535.f : r));
+}
+
+__global__ void k_to_u16(const float* in, int64_t n, uint16_t* out)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) out[i] = to_u16(in[i]);
+}
+
+// getWarpedFrame: the splat divided by its coverage (float division), to uint16.
+__global__ void k_preserving(const float* wimg, const float* cov, int64_t n, uint16_t* out)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) out[i] = to_u16(wimg[i] / cov[i]);
+}
+
+// Masked moments for correlation() (Cv3dUtils.cpp:64-88): over pixels with cov != 0,
+// stage 1: count, sum x, sum x^2, sum y, sum y^2 in double (y = the record's reference value);
+// stage 2 (fm >= 0): sum of float((x - mx) * (y - my)) with mx, my the float means.
+// Per-workgroup partials, summed in a fixed order by k_sum_partials.
+constexpr int kMoments = 5;
+
+__global__ __launch_bounds__(kFB) void k_moments(const float* x, const float4* rec, const float* cov, int64_t n,
+                                                 int stage, float mx, float my, double* part)
+{
+   __shared__ double sm[kFB][kMoments];
+   double a[kMoments] = {0, 0, 0, 0, 0};
+   for (int64_t i = (int64_t)blockIdx.x * kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB) {
+      if (cov[i] == 0.f) continue;
+      const float xv = x[i], yv = rec[i].x;
+      if (stage == 1) {
+         a[0] += 1.0;
+         a[1] += xv;
+         a[2] += (double)xv * xv;
+         a[3] += yv;
+         a[4] += (double)yv * yv;
+      } else {
+         const float dx = xv - mx, dy = yv - my;
+         a[0] += (double)(dx * dy);
+      }
+   }
+   for (int k = 0; k < kMoments; k++) sm[threadIdx.x][k] = a[k];
+   __syncthreads();
+   for (int off = kFB / 2; off >= 1; off >>= 1) {
+      if (threadIdx.x < off)
+         for (int k = 0; k < kMoments; k++) sm[threadIdx.x][k] += sm[threadIdx.x + off][k];
+      __syncthreads();
+   }
+   if (threadIdx.x < kMoments) part[(int64_t)blockIdx.x * kMoments + threadIdx.x] = sm[0][threadIdx.x];
+}
+
+__global__ void k_sum_partials(const double* part, int m, double* out)
+{
+   if (threadIdx.x >= kMoments) return;
+   double s = 0;
+   for (int b = 0; b < m; b++) s += part[(int64_t)b * kMoments + threadIdx.x];
+   out[threadIdx.x] = s;
+}
+
+__global__ void k_count_nonzero(const float* v, int64_t n, unsigned long long* cnt)
+{
+   __shared__ unsigned long long sc;
+   if (threadIdx.x == 0) sc = 0;
+   __syncthreads();
+   unsigned long long c = 0;
+   for (int64_t i = (int64_t)blockIdx.x * kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB) c += v[i] != 0.f;
+   atomicAdd(&sc, c);
+   __syncthreads();
+   if (threadIdx.x == 0) atomicAdd(cnt, sc);
+}
+
 unsigned grid_of(int64_t n) { return (unsigned)((n + kFB - 1) / kFB); }
 
 // Float Gaussian weights of cv::GaussianBlur(.., Size(0, 0), sigma) on a float image:
@@ -349,6 +430,118 @@ struct fa_aligner {
       s3[1] = (double)y * f;
       s3[2] = (double)z;
       return FA_OK;
+   }
+
+   // correlation() of Cv3dUtils.cpp:64-88 between x (device floats) and the warper's reference
+   // (the record's first channel: blurred reference + 1) over the pixels where cov != 0
+   int correlation(const float* x, const float* cov, double* part, double* sum, int parts, double* out)
+   {
+      const RefDev& R = ref->dev;
+      double h[kMoments];
+      hipLaunchKernelGGL(k_moments, dim3(parts), dim3(kFB), 0, stream, x, R.rg4, cov, n_px, 1, 0.f, 0.f, part);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, stream, (const double*)part, parts, sum);
+      FCHK(hipGetLastError());
+      FCHK(hipMemcpyAsync(h, sum, sizeof(h), hipMemcpyDeviceToHost, stream));
+      FCHK(hipStreamSynchronize(stream));
+      const double cnt = h[0];
+      if (cnt == 0) {
+         *out = NAN;
+         return FA_OK;
+      }
+      const double m1 = h[1] / cnt, m2 = h[3] / cnt;
+      const double s1 = std::sqrt(std::max(h[2] / cnt - m1 * m1, 0.0)), s2 = std::sqrt(std::max(h[4] / cnt - m2 * m2, 0.0));
+      hipLaunchKernelGGL(k_moments, dim3(parts), dim3(kFB), 0, stream, x, R.rg4, cov, n_px, 2, (float)m1, (float)m2,
+                         part);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, stream, (const double*)part, parts, sum);
+      FCHK(hipGetLastError());
+      FCHK(hipMemcpyAsync(h, sum, sizeof(h), hipMemcpyDeviceToHost, stream));
+      FCHK(hipStreamSynchronize(stream));
+      *out = (h[0] / cnt) / (s1 * s2);
+      return FA_OK;
+   }
+
+   // Output images and metrics of n frames at D (n x nD x 3): the raw frame's splat (mask,
+   // preserving), its interpolated warp (realigned), the warp of the blurred frame + 1 (warped)
+   // and the correlations and coverage (FrameWarpAligner.cpp:218-240,317-346).
+   int products(const void* const* frames, int n, int dtype, const double* D_all, const std::vector<fa_frame*>& fr,
+                const fa_add_outputs* o, fa_add_result* res)
+   {
+      const RefDev& R = ref->dev;
+      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(R.n_chunks, (int64_t)ctx->n_cu * 8));
+      const int parts = (int)std::min<int64_t>(1024, (n_px + kFB - 1) / kFB);
+      double *d_D = nullptr, *d_part = nullptr, *d_sum = nullptr;
+      float *d_w = nullptr, *d_wimg = nullptr, *d_cov = nullptr;
+      uint16_t* d_u16 = nullptr;
+      unsigned long long* d_cnt = nullptr;
+      auto done = [&](int rc) {
+         ffree(d_D);
+         ffree(d_part);
+         ffree(d_sum);
+         ffree(d_w);
+         ffree(d_wimg);
+         ffree(d_cov);
+         ffree(d_u16);
+         ffree(d_cnt);
+         return rc;
+      };
+      int rc;
+      if ((rc = falloc(&d_D, 3 * (size_t)nD)) || (rc = falloc(&d_part, (size_t)parts * kMoments)) ||
+          (rc = falloc(&d_sum, kMoments)) || (rc = falloc(&d_w, n_px)) || (rc = falloc(&d_wimg, n_px)) ||
+          (rc = falloc(&d_cov, n_px)) || (rc = falloc(&d_u16, n_px)) || (rc = falloc(&d_cnt, 1)))
+         return done(rc);
+      for (int i = 0; i < n; i++) {
+         const double* D = D_all + (size_t)i * nD * 3;
+         const size_t off = (size_t)i * n_px;
+         if ((rc = upload(frames[i], dtype)) != FA_OK) return done(rc);
+         const FrameDev Fraw{d_raw, FA_F32, 0.f};
+         const FrameDev Fsm{fr[i]->d_data, FA_F32, 0.f};
+         if (hipMemcpyAsync(d_D, D, 3 * (size_t)nD * sizeof(double), hipMemcpyHostToDevice, stream) != hipSuccess ||
+             hipMemsetAsync(d_wimg, 0, n_px * sizeof(float), stream) != hipSuccess ||
+             hipMemsetAsync(d_cov, 0, n_px * sizeof(float), stream) != hipSuccess ||
+             launch_splat(R, Fraw, d_D, d_wimg, d_cov, 0.f, grid, stream) != hipSuccess)
+            return done(set_error(FA_E_DEVICE, "output warp failed"));
+         if (o && o->mask &&
+             hipMemcpyAsync(o->mask + off, d_cov, n_px * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
+            return done(set_error(FA_E_DEVICE, "output copy failed"));
+         if (o && o->preserving) {
+            hipLaunchKernelGGL(k_preserving, dim3(grid_of(n_px)), dim3(kFB), 0, stream, (const float*)d_wimg,
+                               (const float*)d_cov, n_px, d_u16);
+            if (hipMemcpyAsync(o->preserving + off, d_u16, n_px * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(stream) != hipSuccess)
+               return done(set_error(FA_E_DEVICE, "output copy failed"));
+         }
+         if (o && o->realigned) {
+            if (launch_warp(R, Fraw, d_D, d_w, 0.f, 0, 0, grid, stream) != hipSuccess)
+               return done(set_error(FA_E_DEVICE, "output warp failed"));
+            hipLaunchKernelGGL(k_to_u16, dim3(grid_of(n_px)), dim3(kFB), 0, stream, (const float*)d_w, n_px, d_u16);
+            if (hipMemcpyAsync(o->realigned + off, d_u16, n_px * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(stream) != hipSuccess)
+               return done(set_error(FA_E_DEVICE, "output copy failed"));
+         }
+         if (launch_warp(R, Fsm, d_D, d_w, 0.f, 0, 0, grid, stream) != hipSuccess)
+            return done(set_error(FA_E_DEVICE, "output warp failed"));
+         if (o && o->warped &&
+             hipMemcpyAsync(o->warped + off, d_w, n_px * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
+            return done(set_error(FA_E_DEVICE, "output copy failed"));
+         if (res) {
+            unsigned long long cnt = 0;
+            if (hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), stream) != hipSuccess)
+               return done(set_error(FA_E_DEVICE, "metrics failed"));
+            hipLaunchKernelGGL(k_count_nonzero, dim3(parts), dim3(kFB), 0, stream, (const float*)d_cov, n_px, d_cnt);
+            if (hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+                hipStreamSynchronize(stream) != hipSuccess)
+               return done(set_error(FA_E_DEVICE, "metrics failed"));
+            res[i].coverage = (double)cnt / (double)n_px;
+            if ((rc = correlation(d_w, d_cov, d_part, d_sum, parts, &res[i].correlation)) != FA_OK ||
+                (rc = correlation((const float*)fr[i]->d_data, d_cov, d_part, d_sum, parts, &res[i].unaligned_correlation)) !=
+                    FA_OK)
+               return done(rc);
+         }
+         if (hipStreamSynchronize(stream) != hipSuccess) return done(set_error(FA_E_DEVICE, "output copy failed"));
+      }
+      return done(FA_OK);
    }
 };
 
@@ -515,7 +708,8 @@ int fa_aligner_rigid_shift(fa_aligner* a, const void* raw, int32_t dtype, double
 }
 
 int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, int32_t dtype,
-                          const fa_tr_params* prm, double* D_out, float* realigned, fa_add_result* results)
+                          const fa_tr_params* prm, double* D_out, const fa_add_outputs* outputs,
+                          fa_add_result* results)
 {
    if (!a || !frames || n <= 0 || !D_out) return set_error(FA_E_ARG, "null argument");
    if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
@@ -601,13 +795,57 @@ int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, i
          r.start_error[2] = er[i];
          r.stats = st[i];
       }
-      if (realigned && (rc = fa_warp_interpolated(fr[i], D, realigned + (size_t)i * a->n_px, 0.f)) != FA_OK) {
-         release();
-         return rc;
-      }
    }
+   // 5. output images and metrics from the raw frame at the final D
+   if (outputs || results) rc = a->products(frames, n, dtype, D_out, fr, outputs, results);
    release();
+   return rc;
+}
+
+int fa_shift_pixel(const fa_reference* ref, const double* D, double* xyz)
+{
+   if (!ref || !D || !xyz) return set_error(FA_E_ARG, "null argument");
+   double loc[3];
+   OK(fa_warp_point(ref, D, (int32_t)xyz[0], (int32_t)xyz[1], (int32_t)xyz[2], 1, loc));
+   for (int c = 0; c < 3; c++) xyz[c] -= loc[c];
    return FA_OK;
+}
+
+int fa_write_realignment_info(const char* filename, const fa_scan_params* scan, int32_t n, int32_t nD,
+                              const double* D, const double* unaligned_correlation, const double* correlation,
+                              const double* coverage)
+{
+   if (!filename || !scan || n < 0 || nD < 1 || (n > 0 && (!D || !unaligned_correlation || !correlation || !coverage)))
+      return set_error(FA_E_ARG, "invalid argument");
+   if (n == 0) return FA_OK;  // nothing stored: the reference writes no file
+   FILE* fh = std::fopen(filename, "w");
+   if (!fh) return set_error(FA_E_ARG, std::string("cannot write ") + filename);
+   // ImageScanParameters constructor (ImageScanParameters.h:7-20)
+   const int ny = std::max(scan->n_y, 1);
+   double interline = scan->interline_duration;
+   if (interline < scan->line_duration) interline = 1.01 * scan->line_duration;
+   const double frame = ny * interline;
+   // std::ostream's default formatting of doubles is printf's %g
+   std::fprintf(fh, "Realignment Information Version, 2\n");
+   std::fprintf(fh, "Line Duration,%g\n", scan->line_duration);
+   std::fprintf(fh, "Interline Duration,%g\n", interline);
+   std::fprintf(fh, "Frame Duration,%g\n", frame);
+   std::fprintf(fh, "Interframe Duration,%g\n", scan->interframe_duration);
+   std::fprintf(fh, "Lines,%d\n", ny);
+   std::fprintf(fh, "Frames Per Stack,%d\n", scan->n_z);
+   std::fprintf(fh, "Frame, UnalignedCorrelation, Correlation, Coverage");
+   for (int j = 0; j < nD; j++) std::fprintf(fh, ", dx_%d, dy_%d, dz_%d", j, j, j);
+   std::fprintf(fh, "\n");
+   for (int i = 0; i < n; i++) {
+      std::fprintf(fh, "%d,%g,%g, %g", i, unaligned_correlation[i], correlation[i], coverage[i]);
+      for (int j = 0; j < nD; j++) {
+         const double* d = D + ((size_t)i * nD + j) * 3;
+         std::fprintf(fh, ", %g, %g, %g", d[0], d[1], d[2]);
+      }
+      std::fprintf(fh, "\n");
+   }
+   const bool ok = std::fclose(fh) == 0;
+   return ok ? FA_OK : set_error(FA_E_ARG, std::string("cannot write ") + filename);
 }
 
 }  // extern "C"

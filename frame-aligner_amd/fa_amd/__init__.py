@@ -63,7 +63,13 @@ class RealignParams(C.Structure):
 
 class AddResult(C.Structure):
     _fields_ = [("start", C.c_int32), ("pad", C.c_int32), ("rigid_shift", C.c_double * 3),
-                ("start_error", C.c_double * 3), ("stats", AlignStats)]
+                ("start_error", C.c_double * 3), ("stats", AlignStats), ("correlation", C.c_double),
+                ("unaligned_correlation", C.c_double), ("coverage", C.c_double)]
+
+
+class AddOutputs(C.Structure):
+    _fields_ = [("realigned", C.c_void_p), ("preserving", C.c_void_p), ("mask", C.c_void_p),
+                ("warped", C.c_void_p)]
 
 
 REALIGN_TYPES = {"none": 0, "translation": 1, "warp": 2}
@@ -89,7 +95,7 @@ EXPORTS = [
     "fa_server_submit", "fa_server_submit_host", "fa_server_step", "fa_server_results",
     "fa_server_memory", "fa_host_alloc", "fa_host_free",
     "fa_aligner_create", "fa_aligner_destroy", "fa_aligner_reference", "fa_aligner_preprocess",
-    "fa_aligner_rigid_shift", "fa_aligner_add_frames",
+    "fa_aligner_rigid_shift", "fa_aligner_add_frames", "fa_shift_pixel", "fa_write_realignment_info",
 ]
 
 _lib = None
@@ -140,7 +146,9 @@ def load_library(path=None):
     L.fa_aligner_reference.argtypes = [P, C.POINTER(P), C.POINTER(i32)]
     L.fa_aligner_preprocess.argtypes = [P, P, i32, P]
     L.fa_aligner_rigid_shift.argtypes = [P, P, i32, P]
-    L.fa_aligner_add_frames.argtypes = [P, P, i32, i32, C.POINTER(TrParams), P, P, P]
+    L.fa_aligner_add_frames.argtypes = [P, P, i32, i32, C.POINTER(TrParams), P, C.POINTER(AddOutputs), P]
+    L.fa_shift_pixel.argtypes = [P, P, P]
+    L.fa_write_realignment_info.argtypes = [C.c_char_p, C.POINTER(ScanParams), i32, i32, P, P, P, P]
     L.fa_server_create.argtypes = [P, P, i32, i32, C.c_uint64, C.POINTER(TrParams), C.POINTER(P)]
     L.fa_server_submit_host.argtypes = [P, P, P, i32, C.c_float, P, C.c_int64]
     L.fa_server_memory.argtypes = [P, P, P]
@@ -625,9 +633,10 @@ class Aligner:
         _check(self.L.fa_aligner_rigid_shift(self.h, _ptr(a), dt, _ptr(o)))
         return tuple(float(v) for v in o)
 
-    def add_frames(self, frames, realigned=False, **kw):
-        """addFrame for each raw frame (one dtype): returns (D list (nD x 3), result dicts,
-        realigned frames or None)."""
+    def add_frames(self, frames, outputs=(), **kw):
+        """addFrame for each raw frame (one dtype): returns (D list (nD x 3), result dicts, dict
+        of the requested output images: any of 'realigned', 'preserving' (uint16), 'mask',
+        'warped' (float), each n x nz x ny x nx)."""
         arrs = [self._raw(f) for f in frames]
         dts = {dt for _, dt in arrs}
         if len(dts) != 1:
@@ -636,13 +645,39 @@ class Aligner:
         ptrs = (C.c_void_p * n)(*[a.ctypes.data for a, _ in arrs])
         D = np.zeros((n, self.nD, 3))
         res = (AddResult * n)()
-        out = np.zeros((n,) + self.shape, np.float32) if realigned else None
+        imgs = {}
+        for k in outputs:
+            imgs[k] = np.zeros((n,) + self.shape, np.uint16 if k in ("realigned", "preserving") else np.float32)
+        o = AddOutputs(*[imgs[k].ctypes.data if k in imgs else None
+                         for k in ("realigned", "preserving", "mask", "warped")])
         p = tr_params(**kw)
-        _check(self.L.fa_aligner_add_frames(self.h, ptrs, n, dts.pop(), C.byref(p), _ptr(D),
-                                           _ptr(out) if realigned else None, res))
+        _check(self.L.fa_aligner_add_frames(self.h, ptrs, n, dts.pop(), C.byref(p), _ptr(D), C.byref(o), res))
         results = [dict(start=r.start, rigid_shift=tuple(r.rigid_shift), start_error=tuple(r.start_error),
-                        stats=stats_dict(r.stats)) for r in res]
-        return [D[i] for i in range(n)], results, out
+                        stats=stats_dict(r.stats), correlation=r.correlation,
+                        unaligned_correlation=r.unaligned_correlation, coverage=r.coverage) for r in res]
+        return [D[i] for i in range(n)], results, imgs
+
+    def shift_pixel(self, D, x, y, z):
+        """FrameWarpAligner::shiftPixel for a frame aligned to D."""
+        ref = C.c_void_p()
+        _check(self.L.fa_aligner_reference(self.h, C.byref(ref), None))
+        p = np.array([x, y, z], np.float64)
+        _check(self.L.fa_shift_pixel(ref, _ptr(as_D(D, self.nD, self.n_dim)), _ptr(p)))
+        return tuple(float(v) for v in p)
+
+
+def write_realignment_info(filename, D, unaligned_correlation, correlation, coverage, n_x, n_y, n_z=1,
+                           line=1.0, interline=1.0, interframe=None):
+    """FrameWarpAligner::writeRealignmentInfo: the version-2 CSV of the scan timing and, per
+    frame, the correlations, coverage and node displacements (D: n x nD x 3)."""
+    L = load_library()
+    D = np.ascontiguousarray(D, np.float64)
+    n, nD = D.shape[0], D.shape[1]
+    if interframe is None:
+        interframe = n_y * max(interline, line)
+    sp = ScanParams(line, interline, interframe, n_x, n_y, n_z, 0)
+    arr = [np.ascontiguousarray(v, np.float64) for v in (unaligned_correlation, correlation, coverage)]
+    _check(L.fa_write_realignment_info(os.fsencode(filename), C.byref(sp), n, nD, _ptr(D), *[_ptr(a) for a in arr]))
 
 
 class HipFrameWarper:

@@ -256,7 +256,25 @@ typedef struct fa_add_result {
    double rigid_shift[3];  /* phase-correlation shift: x, y (scaled by the binning), z */
    double start_error[3];  /* SSD at the three starts */
    fa_align_stats stats;   /* trust region (Warp only) */
+   /* RealignmentResult metrics (FrameWarpAligner.cpp:238-240; Cv3dUtils.cpp:64-88): Pearson
+      correlation with the blurred reference + 1 over the pixels the raw frame's splat covers,
+      of the warped and of the unwarped blurred frame + 1, and the covered fraction. */
+   double correlation;
+   double unaligned_correlation;
+   double coverage;
 } fa_add_result;
+
+/* Output images of addFrame (each n x nz*ny*nx, any pointer may be NULL), from the raw frame
+   at the final D: realigned - getInterpolatedWarpedFrame (warpImageInterpolated, invalid 0, to
+   uint16 as cv::Mat::convertTo rounds and saturates); preserving - getWarpedFrame (forward
+   splat divided by its coverage, to uint16); mask - getMask (the splat's coverage); warped -
+   the blurred frame + 1 warped by warpImageInterpolated (FrameWarpAligner.cpp:218-226,317-346). */
+typedef struct fa_add_outputs {
+   uint16_t* realigned;
+   uint16_t* preserving;
+   float* mask;
+   float* warped;
+} fa_add_outputs;
 
 /* FrameWarpAligner(params) + setReference(0, reference) (FrameWarpAligner.cpp:68-105):
    reference is the raw nz*ny*nx volume of dtype (FA_U16 counts or FA_F32). Builds the blurred
@@ -273,14 +291,22 @@ int fa_aligner_preprocess(fa_aligner* a, const void* raw, int32_t dtype, float* 
 /* Rigid start of one raw frame (VolumePhaseCorrelator::computeShift of the binned frame, x and
    y scaled by the binning, FrameWarpAligner.cpp:186-189). */
 int fa_aligner_rigid_shift(fa_aligner* a, const void* raw, int32_t dtype, double* shift3);
-/* FrameWarpAligner::addFrame (FrameWarpAligner.cpp:117-223) for n raw frames of dtype: blur
+/* FrameWarpAligner::addFrame (FrameWarpAligner.cpp:117-250) for n raw frames of dtype: blur
    and +1, the rigid start, the start with the smallest SSD, then (type Warp) the trust region
-   of all n frames as one batch. D_out: n x nD x 3; realigned (may be NULL): n x nz*ny*nx floats
-   of warpImageInterpolated(frame, D) with invalid value 0 (FrameWarpAligner.cpp:222);
-   results: n entries (may be NULL). prm NULL = the reference's stop rule. */
+   of all n frames as one batch, then the output images and metrics. D_out: n x nD x 3;
+   outputs and results (n entries) may be NULL. prm NULL = the reference's stop rule. */
 int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, int32_t dtype,
-                          const fa_tr_params* prm, double* D_out, float* realigned,
+                          const fa_tr_params* prm, double* D_out, const fa_add_outputs* outputs,
                           fa_add_result* results);
+/* FrameWarpAligner::shiftPixel (FrameWarpAligner.cpp:253-266): the point (x, y, z), truncated
+   to whole pixels, minus warpPoint(D) there; xyz is updated in place. */
+int fa_shift_pixel(const fa_reference* ref, const double* D, double* xyz);
+/* FrameWarpAligner::writeRealignmentInfo (FrameWarpAligner.cpp:288-314): the CSV (version 2)
+   of the scan timing (ImageScanParameters after its constructor's fix-ups) and, per frame,
+   the unaligned correlation, correlation, coverage and the nD displacements. D: n x nD x 3. */
+int fa_write_realignment_info(const char* filename, const fa_scan_params* scan, int32_t n, int32_t nD,
+                              const double* D, const double* unaligned_correlation,
+                              const double* correlation, const double* coverage);
 
 /* out[0..4] = device ordinal, compute units, bytes of LDS one workgroup may allocate,
    wavefront size, hipError_t of raising the kernels' dynamic-LDS limit (0 = success). */

@@ -142,3 +142,57 @@ def smooth_stack(vol, smoothing):
 def choose_start(errors):
     """Index of the start addFrame takes: the first smallest SSD (std::min_element)."""
     return int(np.argmin(np.asarray(errors)))
+
+
+def to_u16(v):
+    """cv::Mat::convertTo(CV_16U) of float values: round half to even, saturate to [0, 65535];
+    NaN, infinities and values beyond the int range give 0 (x86's out-of-range conversion)."""
+    v = np.asarray(v, np.float32)
+    out = np.zeros(v.shape, np.uint16)
+    ok = np.abs(v) < np.float32(2147483648.0)
+    out[ok] = np.clip(np.rint(v[ok]), 0, 65535).astype(np.uint16)
+    return out
+
+
+def preserving(wimg, cov):
+    """getWarpedFrame (FrameWarpAligner.cpp:317-325): intensity_preserving /= mask, to uint16."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return to_u16(np.asarray(wimg, np.float32) / np.asarray(cov, np.float32))
+
+
+def correlation(a, b, mask):
+    """correlation() of Cv3dUtils.cpp:64-88: mean and standard deviation (population, double
+    sums) of each image over the mask, then the mean over the mask of the float product of the
+    float-centred images, over the product of the deviations."""
+    a = np.asarray(a, np.float32).ravel()
+    b = np.asarray(b, np.float32).ravel()
+    m = np.asarray(mask).ravel() != 0
+    n = int(m.sum())
+    if n == 0:
+        return float("nan")
+    av, bv = a[m].astype(np.float64), b[m].astype(np.float64)
+    ma, mb = av.sum() / n, bv.sum() / n
+    sa = np.sqrt(max((av * av).sum() / n - ma * ma, 0.0))
+    sb = np.sqrt(max((bv * bv).sum() / n - mb * mb, 0.0))
+    prod = (a[m] - np.float32(ma)) * (b[m] - np.float32(mb))
+    return float(prod.astype(np.float64).sum() / n / (sa * sb))
+
+
+def realignment_csv(scan, rows):
+    """Text of writeRealignmentInfo (FrameWarpAligner.cpp:288-314). scan: dict of the
+    ImageScanParameters constructor arguments; rows: per frame (unaligned correlation,
+    correlation, coverage, D (nD x 3)). Doubles print as std::ostream does by default (%g)."""
+    line, interline = scan["line"], scan["interline"]
+    ny = max(scan["n_y"], 1)
+    if interline < line:
+        interline = 1.01 * line
+    out = ["Realignment Information Version, 2",
+           "Line Duration,%g" % line, "Interline Duration,%g" % interline,
+           "Frame Duration,%g" % (ny * interline), "Interframe Duration,%g" % scan["interframe"],
+           "Lines,%d" % ny, "Frames Per Stack,%d" % scan["n_z"]]
+    nD = len(rows[0][3])
+    out.append("Frame, UnalignedCorrelation, Correlation, Coverage" +
+               "".join(", dx_%d, dy_%d, dz_%d" % (j, j, j) for j in range(nD)))
+    for i, (u, c, cov, D) in enumerate(rows):
+        out.append("%d,%g,%g, %g" % (i, u, c, cov) + "".join(", %g, %g, %g" % tuple(d) for d in D))
+    return "\n".join(out) + "\n"

@@ -122,3 +122,40 @@ def test_choose_start_first_minimum():
     assert FE.choose_start([3.0, 3.0, 2.0]) == 2
     assert FE.choose_start([2.0, 2.0, 2.0]) == 0   # ties keep the zero start
     assert FE.choose_start([1.0, 1.0, 2.0]) == 0
+
+
+def test_to_u16_rounding_and_saturation():
+    v = np.array([0.5, 1.5, 2.5, -0.4, -3.0, 65535.4, 65536.0, 7e4, np.nan, np.inf, -np.inf, 3e9, 12.49],
+                 np.float32)
+    assert FE.to_u16(v).tolist() == [0, 2, 2, 0, 0, 65535, 65535, 65535, 0, 0, 0, 0, 12]
+    assert FE.preserving(np.array([3.0, 0.0, 5.0], np.float32), np.array([2.0, 0.0, 0.5], np.float32)).tolist() == [2, 0, 10]
+
+
+def test_correlation_pins():
+    rng = np.random.default_rng(6)
+    a = rng.normal(10, 3, (3, 20, 30)).astype(np.float32)
+    mask = rng.random(a.shape) > 0.3
+    assert abs(FE.correlation(a, a, mask) - 1.0) < 1e-6
+    assert abs(FE.correlation(a, 20 - a, mask) + 1.0) < 1e-6
+    b = (0.5 * a + rng.normal(0, 2, a.shape)).astype(np.float32)
+    expect = np.corrcoef(a[mask].astype(np.float64), b[mask].astype(np.float64))[0, 1]
+    assert abs(FE.correlation(a, b, mask) - expect) < 1e-5
+    assert np.isnan(FE.correlation(a, b, np.zeros(a.shape)))
+
+
+def test_realignment_csv_matches_library_writer(tmp_path):
+    import fa_amd as F
+    rng = np.random.default_rng(7)
+    n, nD = 3, 4
+    D = rng.normal(0, 2, (n, nD, 3))
+    u, c, cov = rng.random(n), rng.random(n), rng.random(n)
+    path = tmp_path / "realign.csv"
+    F.write_realignment_info(str(path), D, u, c, cov, n_x=64, n_y=48, n_z=2, line=1.0, interline=0.5,
+                             interframe=60.0)
+    text = path.read_text()
+    rows = [(u[i], c[i], cov[i], D[i]) for i in range(n)]
+    assert text == FE.realignment_csv(dict(line=1.0, interline=0.5, interframe=60.0, n_y=48, n_z=2), rows)
+    lines = text.splitlines()
+    assert lines[0] == "Realignment Information Version, 2" and lines[2] == "Interline Duration,1.01"
+    assert lines[3] == "Frame Duration,48.48" and lines[7].startswith("Frame, UnalignedCorrelation")
+    assert len(lines) == 8 + n and lines[8].startswith("0,")

@@ -79,7 +79,8 @@ def test_add_frames_matches_oracle(ctx, smoothing):
     ref, frames, _ = _offset_frames(shape, nD, 4, [(0, 0), (3, -2), (-2, 3), (0.4, 0.3)], seed=11)
     al = F.Aligner(ctx, ref, n_resampling_points=n_res, smoothing=smoothing)
     kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
-    D_g, res, realigned = al.add_frames(frames, realigned=True, **kw)
+    D_g, res, imgs = al.add_frames(frames, outputs=("warped",), **kw)
+    realigned = imgs["warped"]
     ref_w = FE.smooth_stack(ref, smoothing) + np.float32(1.0)
     orc = O.Oracle().set_reference(ref_w, nD)
     pc = FE.PhaseCorrelator(shape).set_reference(ref)
@@ -125,4 +126,46 @@ def test_translation_type_returns_the_start(ctx):
         expect = np.tile(np.array(r["rigid_shift"])[:2], (al.nD, 1)) if r["start"] == 2 else np.zeros((al.nD, 2))
         assert np.array_equal(Di[:, :2], expect) and not Di[:, 2].any()
         assert r["stats"]["evaluations"] == 0
+    al.close()
+
+
+@pytest.mark.parametrize("shape,n_res", [((1, 96, 80), 8), ((4, 40, 36), 5)])
+def test_add_frames_outputs_and_metrics(ctx, shape, n_res):
+    """RealignmentResult's images and metrics (FrameWarpAligner.cpp:218-240,317-346) at the
+    device's final D, against the oracle's warps of the same raw frame at that D."""
+    nD = n_res * shape[0] + 1
+    st = synth.make_stack(shape, 3, nD, seed=8, n_blobs=120, sigma=(1.5, 4.0), dtype="uint16")
+    al = F.Aligner(ctx, st["reference"], n_resampling_points=n_res, smoothing=1.0)
+    D_g, res, imgs = al.add_frames(list(st["frames"]), outputs=("realigned", "preserving", "mask", "warped"),
+                                   rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    ref_w = FE.smooth_stack(st["reference"], 1.0) + np.float32(1.0)
+    orc = O.Oracle().set_reference(ref_w, nD)
+    for i, raw in enumerate(st["frames"]):
+        rawf = raw.astype(np.float32)
+        wimg, cov = orc.warp_image(rawf, D_g[i])
+        assert np.max(np.abs(imgs["mask"][i] - cov)) <= 1e-5
+        assert res[i]["coverage"] == np.count_nonzero(cov) / cov.size
+        pres = FE.preserving(wimg, cov).astype(np.int64)
+        assert np.max(np.abs(imgs["preserving"][i].astype(np.int64) - pres)) <= 1
+        realg = FE.to_u16(orc.warp_interpolated(rawf, D_g[i], 0.0)).astype(np.int64)
+        diff = np.abs(imgs["realigned"][i].astype(np.int64) - realg)
+        assert diff.max() <= 1 and np.count_nonzero(diff) <= 1e-3 * diff.size   # only at .5 ties
+        fw = FE.smooth_stack(raw, 1.0) + np.float32(1.0)
+        warped_o = orc.warp_interpolated(fw, D_g[i], 0.0)
+        assert np.max(np.abs(imgs["warped"][i] - warped_o)) <= 1e-5 * np.max(np.abs(warped_o))
+        assert abs(res[i]["correlation"] - FE.correlation(warped_o, ref_w, cov)) <= 1e-5
+        assert abs(res[i]["unaligned_correlation"] - FE.correlation(fw, ref_w, cov)) <= 1e-5
+    al.close()
+
+
+def test_shift_pixel(ctx):
+    shape, n_res = (3, 40, 36), 4
+    nD = n_res * shape[0] + 1
+    st = synth.make_stack(shape, 1, nD, seed=9, n_blobs=60, sigma=(1.5, 4.0))
+    al = F.Aligner(ctx, st["reference"], n_resampling_points=n_res)
+    orc = O.Oracle().set_reference(synth.with_offset(st["reference"]), nD)
+    D = np.random.default_rng(3).normal(0, 2, (nD, 3))
+    for (x, y, z) in [(3.7, 5.2, 0.0), (20.0, 30.9, 2.0), (35.5, 39.0, 1.0), (-1.0, 4.0, 0.0), (10.0, 10.0, 5.0)]:
+        loc = orc.warp_point(D, int(x), int(y), int(z))
+        assert np.allclose(al.shift_pixel(D, x, y, z), (x - loc[0], y - loc[1], z - loc[2]), rtol=0, atol=1e-12)
     al.close()
