@@ -143,6 +143,45 @@ def serving(F, args, ctx):
     return out
 
 
+def serving_512(F, args, ctx, plain_fps):
+    """The serving form on the headline workload: one acquisition of 512x512 uint16 frames,
+    nD = 31, streamed with exponential inter-arrival times through the continuous-batching
+    server at 3x the plain path's frames/s (the north-star operating point) and at
+    args.serve512_high; frames/s and p50/p99 arrival -> result latency, and served == alone on a
+    sample."""
+    from fa_amd import serve, synth
+    st = synth.make_stack((1, 512, 512), 64, 31, seed=args.seed + 5, n_blobs=600, mean=50.0, dtype="uint16")
+    ref = F.Reference(ctx, synth.with_offset(st["reference"]), 31)
+    tr = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    srv = F.Server([ref], slots=args.serve_slots, **tr)
+    loop = serve.ServeLoop(srv, [ref], [st["frames"]], rounds=args.serve_rounds)
+    out = {"workload": "512x512 uint16, nD=31, one acquisition, 64 distinct frames cycled, Poisson arrivals",
+           "plain_frames_per_s": round(plain_fps, 1), "runs": []}
+    w, wa = serve.arrivals(1, 2000.0, 0.2, seed=args.seed + 6)
+    loop.run(w, wa)  # warm-up
+    for rate in (3.0 * plain_fps, args.serve512_high):
+        t, a = serve.arrivals(1, rate, args.serve_seconds, seed=args.seed + 7)
+        keep = set(range(0, len(t), max(1, len(t) // 16)))
+        r = loop.run(t, a, keep=keep)
+        s = serve.summarize(r, t, a, [st["frames"]])
+        same = True
+        for i in sorted(keep):
+            fr = ref.register(st["frames"][i % 64])
+            x, _ = fr.align(**tr)
+            same = same and bool(np.array_equal(x, r["kept"][i][0]))
+            fr.release()
+        out["runs"].append({"offered_frames_per_s": s["offered_frames_per_s"], "frames_per_s": s["frames_per_s"],
+                            "vs_plain": round(s["frames_per_s"] / plain_fps, 2), "latency_ms": s["latency_ms"],
+                            "max_in_flight": s["max_in_flight"], "max_queued": s["max_queued"],
+                            "same_as_alone": same})
+        log(f"serving 512^2 at {rate:.0f}/s: {s['frames_per_s']} frames/s, p50 {s['latency_ms']['p50']} ms, "
+            f"p99 {s['latency_ms']['p99']} ms")
+    loop.close()
+    srv.close()
+    ref.release()
+    return out
+
+
 def timed_steps(batch, steps):
     """`steps` untimed runs (graph replay where tuned), then `steps` runs with HIP events around
     every fused pass. Returns (results, stats, wall s per run, pass ms, pass bytes, launches)."""
@@ -440,6 +479,10 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=128)
     ap.add_argument("--serve-rounds", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve512-high", type=float, default=20000.0,
+                    help="second offered rate (frames/s) of the 512^2 serving run")
+    ap.add_argument("--plain-fps", type=float, default=1400.0,
+                    help="plain-path frames/s used for the 512^2 serving rate when the batch section is skipped")
     ap.add_argument("--serve-lower", default="1500,1000",
                     help="lower offered rates (frames/s) the serving section also runs")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape, no autotuning")
@@ -473,6 +516,8 @@ def main():
         out["copy_probe_mib"] = copy_probe(args.copy_probe)
 
     if "serve" in secs and not args.no_serve:
+        plain = out.get("plain", {}).get("frames_per_s") or args.plain_fps
+        out["serving_512"] = serving_512(F, args, ctx, plain)
         out["serving"] = serving(F, args, ctx)
         log(f"serving: {out['serving']['frames_per_s']} frames/s, p50 {out['serving']['latency_ms']['p50']} ms, "
             f"p99 {out['serving']['latency_ms']['p99']} ms")
