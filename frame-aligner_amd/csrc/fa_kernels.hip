@@ -827,30 +827,66 @@ struct Stage {
    int za, nzs, ya, nrows, stride;
 };
 
-template <typename FT>
-__device__ __forceinline__ float4 load4(const FrameDev& F, int64_t i);
+// Staged rows are visited as one flat index over (plane, row, column group), kBlock apart per
+// thread; (zi, yr, c) advance without a division, and each thread issues NB loads before it
+// stores any, so a wave has that many row segments in flight instead of waiting out one
+// global-load latency per row. NB is 6 for 2-D items (the pass then fits 7 waves per SIMD)
+// and 8 for 3-D items (two planes: twice the rows).
 
+struct StageWalk {
+   int zi, yr, c;
+   __device__ __forceinline__ void init(int i, int ncol, int nrows)
+   {
+      const int r = i / ncol;
+      c = i - r * ncol;
+      zi = r / nrows;
+      yr = r - zi * nrows;
+   }
+   __device__ __forceinline__ void step(int ncol, int nrows, int dr, int dc)
+   {
+      c += dc;
+      yr += dr;
+      if (c >= ncol) {
+         c -= ncol;
+         yr++;
+      }
+      while (yr >= nrows) {
+         yr -= nrows;
+         zi++;
+      }
+   }
+};
+
+// Four consecutive samples as stored (uint2 for uint16, float4 for float) and their float
+// values plus the frame's offset.
+template <typename FT>
+struct Raw4 {
+   using type = float4;
+};
 template <>
-__device__ __forceinline__ float4 load4<uint16_t>(const FrameDev& F, int64_t i)
+struct Raw4<uint16_t> {
+   using type = uint2;
+};
+
+__device__ __forceinline__ float4 cvt4(uint2 w, float off)
 {
-   const uint2 w = *(const uint2*)((const uint16_t*)F.data + i);
-   return make_float4((float)(w.x & 0xffffu) + F.offset, (float)(w.x >> 16) + F.offset,
-                      (float)(w.y & 0xffffu) + F.offset, (float)(w.y >> 16) + F.offset);
+   return make_float4((float)(w.x & 0xffffu) + off, (float)(w.x >> 16) + off, (float)(w.y & 0xffffu) + off,
+                      (float)(w.y >> 16) + off);
 }
 
-template <>
-__device__ __forceinline__ float4 load4<float>(const FrameDev& F, int64_t i)
+__device__ __forceinline__ float4 cvt4(float4 w, float off)
 {
-   const float4 w = *(const float4*)((const float*)F.data + i);
-   return make_float4(w.x + F.offset, w.y + F.offset, w.z + F.offset, w.w + F.offset);
+   return make_float4(w.x + off, w.y + off, w.z + off, w.w + off);
 }
 
-template <typename FT>
+template <int NB, typename FT>
 __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, float* lds)
 {
    const int nx = R.nx, ny = R.ny, nz = R.nz;
    const int rows = S.nzs * S.nrows;
    const int tid = threadIdx.x;
+   const FT* __restrict__ src = (const FT*)F.data;
+   const float off = F.offset;
    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
    if ((nx & 3) == 0) {
       constexpr int P4 = kPad / 4;
@@ -860,33 +896,53 @@ __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, 
          *(float4*)&lds[r * S.stride + col] = zero4;
       }
       const int nx4 = nx >> 2;
-      auto row4 = [&](int r, int c) {
-         const int zi = r / S.nrows;
-         const int yy = S.ya + (r - zi * S.nrows), zz = S.za + zi;
-         float4 v = zero4;
-         if ((unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz)
-            v = load4<FT>(F, ((int64_t)zz * ny + yy) * nx + 4 * c);
-         *(float4*)&lds[r * S.stride + kPad + 4 * c] = v;
-      };
-      if (nx4 <= kBlock) {
-         const int rpi = kBlock / nx4, tr = tid / nx4, tc = tid - tr * nx4;
-         if (tr < rpi)
-            for (int r = tr; r < rows; r += rpi) row4(r, tc);
-      } else {
-         for (int r = 0; r < rows; r++)
-            for (int c = tid; c < nx4; c += kBlock) row4(r, c);
+      const int total = rows * nx4;
+      const int dr = kBlock / nx4, dc = kBlock - dr * nx4;
+      StageWalk w;
+      w.init(tid, nx4, S.nrows);
+      using RT = typename Raw4<FT>::type;
+      for (int i0 = tid; i0 < total; i0 += NB * kBlock) {
+         // every lane loads (rows outside the frame from a clamped, in-bounds row) so that the
+         // loads are not under branches and all NB are in flight together
+         RT raw[NB];
+         int dst[NB];
+         bool ok[NB];
+#pragma unroll
+         for (int u = 0; u < NB; u++) {
+            const int yy = S.ya + w.yr, zz = S.za + w.zi;
+            const bool live = i0 + u * kBlock < total;
+            dst[u] = live ? (w.zi * S.nrows + w.yr) * S.stride + kPad + 4 * w.c : -1;
+            ok[u] = live && (unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz;
+            const int yc = min(max(yy, 0), ny - 1), zc = min(max(zz, 0), nz - 1);
+            raw[u] = *(const RT*)(src + ((int64_t)zc * ny + yc) * nx + 4 * w.c);
+            w.step(nx4, S.nrows, dr, dc);
+         }
+#pragma unroll
+         for (int u = 0; u < NB; u++)
+            if (dst[u] >= 0) *(float4*)&lds[dst[u]] = ok[u] ? cvt4(raw[u], off) : zero4;
       }
    } else {
-      for (int r = 0; r < rows; r++) {
-         const int zi = r / S.nrows;
-         const int yy = S.ya + (r - zi * S.nrows), zz = S.za + zi;
-         const bool in = (unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz;
-         for (int c = tid; c < S.stride; c += kBlock) {
-            const int xx = c - kPad;
-            float v = 0.f;
-            if (in && (unsigned)xx < (unsigned)nx) v = fload<FT>(F, ((int64_t)zz * ny + yy) * nx + xx);
-            lds[r * S.stride + c] = v;
+      const int total = rows * S.stride;
+      const int dr = kBlock / S.stride, dc = kBlock - dr * S.stride;
+      StageWalk w;
+      w.init(tid, S.stride, S.nrows);
+      for (int i0 = tid; i0 < total; i0 += NB * kBlock) {
+         FT raw[NB];
+         int dst[NB];
+         bool ok[NB];
+#pragma unroll
+         for (int u = 0; u < NB; u++) {
+            const int yy = S.ya + w.yr, zz = S.za + w.zi, xx = w.c - kPad;
+            const bool live = i0 + u * kBlock < total;
+            dst[u] = live ? (w.zi * S.nrows + w.yr) * S.stride + w.c : -1;
+            ok[u] = live && (unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz && (unsigned)xx < (unsigned)nx;
+            const int yc = min(max(yy, 0), ny - 1), zc = min(max(zz, 0), nz - 1), xc = min(max(xx, 0), nx - 1);
+            raw[u] = src[((int64_t)zc * ny + yc) * nx + xc];
+            w.step(S.stride, S.nrows, dr, dc);
          }
+#pragma unroll
+         for (int u = 0; u < NB; u++)
+            if (dst[u] >= 0) lds[dst[u]] = ok[u] ? (float)raw[u] + off : 0.f;
       }
    }
    __syncthreads();
@@ -955,7 +1011,7 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          use = (int64_t)S.nzs * S.nrows * S.stride <= lds_cap;
       }
    }
-   if (use) stage_frame<FT>(R, F, S, lds);
+   if (use) stage_frame<M == 3 ? 8 : 6, FT>(R, F, S, lds);
    const int lds_lim = S.nzs * S.nrows * S.stride - (M == 3 ? S.nrows * S.stride : 0) - S.stride - 2;
 
    auto tap_g = [&](int zz, int yy, int xx) -> float {
