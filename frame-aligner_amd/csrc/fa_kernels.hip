@@ -1014,11 +1014,8 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    if (use) stage_frame<M == 3 ? 8 : 6, FT>(R, F, S, lds);
    const int lds_lim = S.nzs * S.nrows * S.stride - (M == 3 ? S.nrows * S.stride : 0) - S.stride - 2;
 
-   auto tap_g = [&](int zz, int yy, int xx) -> float {
-      if ((unsigned)xx >= (unsigned)nx || (unsigned)yy >= (unsigned)ny) return 0.f;
-      if (M == 3 && (unsigned)zz >= (unsigned)nz) return 0.f;
-      return fload<FT>(F, ((int64_t)zz * ny + yy) * nx + xx);
-   };
+   const FT* __restrict__ src = (const FT*)F.data;
+   const float off = F.offset;
 
    KF acc[NV];
 #pragma unroll
@@ -1079,12 +1076,31 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
                v = v0;
             }
          } else if (finite) {
+            // the taps are read from clamped, in-bounds addresses and the ones outside the
+            // frame replaced by 0 afterwards, so all of them are loads without branches that
+            // issue together
             const int z0 = M == 3 ? plane + ti[2] : 0;
-            const float v0 = wy0 * (wx0 * tap_g(z0, y0, x0) + wx1 * tap_g(z0, y0, x0 + 1)) +
-                             wy1 * (wx0 * tap_g(z0, y0 + 1, x0) + wx1 * tap_g(z0, y0 + 1, x0 + 1));
+            const int xa = min(max(x0, 0), nx - 1), xb = min(max(x0 + 1, 0), nx - 1);
+            const int ya = min(max(y0, 0), ny - 1), yb = min(max(y0 + 1, 0), ny - 1);
+            const bool vxa = (unsigned)x0 < (unsigned)nx, vxb = (unsigned)(x0 + 1) < (unsigned)nx;
+            const bool vya = (unsigned)y0 < (unsigned)ny, vyb = (unsigned)(y0 + 1) < (unsigned)ny;
+            auto taps = [&](int zz, bool vz, float& t00, float& t01, float& t10, float& t11) {
+               const int zc = M == 3 ? min(max(zz, 0), nz - 1) : 0;
+               const FT* ra = src + ((int64_t)zc * ny + ya) * nx;
+               const FT* rb = src + ((int64_t)zc * ny + yb) * nx;
+               const float a0 = (float)ra[xa], a1 = (float)ra[xb], b0 = (float)rb[xa], b1 = (float)rb[xb];
+               t00 = vz && vya && vxa ? a0 + off : 0.f;
+               t01 = vz && vya && vxb ? a1 + off : 0.f;
+               t10 = vz && vyb && vxa ? b0 + off : 0.f;
+               t11 = vz && vyb && vxb ? b1 + off : 0.f;
+            };
+            float t00, t01, t10, t11;
+            taps(z0, M != 3 || (unsigned)z0 < (unsigned)nz, t00, t01, t10, t11);
+            const float v0 = wy0 * (wx0 * t00 + wx1 * t01) + wy1 * (wx0 * t10 + wx1 * t11);
             if (M == 3) {
-               const float v1 = wy0 * (wx0 * tap_g(z0 + 1, y0, x0) + wx1 * tap_g(z0 + 1, y0, x0 + 1)) +
-                                wy1 * (wx0 * tap_g(z0 + 1, y0 + 1, x0) + wx1 * tap_g(z0 + 1, y0 + 1, x0 + 1));
+               float u00, u01, u10, u11;
+               taps(z0 + 1, (unsigned)(z0 + 1) < (unsigned)nz, u00, u01, u10, u11);
+               const float v1 = wy0 * (wx0 * u00 + wx1 * u01) + wy1 * (wx0 * u10 + wx1 * u11);
                v = (1.f - wt[2]) * v0 + wt[2] * v1;
             } else {
                v = v0;
