@@ -135,6 +135,7 @@ struct fa_batch {
    // the class of each reference.
    std::vector<int> trc_lds;
    std::vector<int> trc_full;  // 1: the class stages everything in LDS
+   std::vector<int> trc_threads;  // workgroup size of the class's update (64 or 256)
    std::vector<int> ref_trc;
    fa_tuning tune{};
    int timing = 0;
@@ -273,6 +274,13 @@ int tr_lds_full(const fa_reference* r)
 // still share a compute unit's LDS).
 constexpr int kTrFullDoubles = 8192;
 
+// Frames of references with at least this many nodes run their update with 256 threads (the
+// cyclic reduction's first levels then have a block per thread), others with one wave. The
+// width depends on the reference alone, so a frame's result is the same in any batch.
+constexpr int kWideUpdateNodes = 128;
+
+int tr_threads(const fa_reference* r) { return r->nD >= kWideUpdateNodes ? 256 : 64; }
+
 // The update runs from LDS when the largest frame's need fits one workgroup's LDS, else from
 // global memory (0).
 int tr_lds_for(const fa_context* ctx, int need)
@@ -329,37 +337,36 @@ void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::
       c.lds = std::max(c.lds, need);
       c.work += ref_items[i];
    }
-   // The update classes run one after another on the batch's stream. When every frame's
-   // workgroup is resident at once even with the largest LDS size, one launch is quicker.
-   int most = 0;
-   for (fa_reference* r : rs) most = std::max(most, tr_lds_need(r));
-   const int64_t per_cu = most ? b->ctx->lds_per_block / ((int64_t)most * (int64_t)sizeof(double)) : 1;
-   int most_full = 0;
-   for (fa_reference* r : rs) most_full = std::max(most_full, tr_lds_full(r));
-   if ((int64_t)n_frames <= (int64_t)b->ctx->n_cu * std::max<int64_t>(per_cu, 1) || most_full <= kTrFullDoubles) {
-      b->ref_trc.assign(rs.size(), 0);
-      if (most_full <= kTrFullDoubles) {
-         b->trc_lds.assign(1, tr_lds_for(b->ctx, most_full));
-         b->trc_full.assign(1, b->trc_lds[0] > 0 ? 1 : 0);
-      } else {
-         b->trc_lds.assign(1, tr_lds_for(b->ctx, most));
-         b->trc_full.assign(1, 0);
-      }
-      return;
+   // The update classes run one after another on the batch's stream: one per workgroup width,
+   // split further by factor size (a class starting above 2x its first) unless every frame's
+   // workgroup is resident at once even with the largest LDS size, when fewer launches are
+   // quicker.
+   int most = 0, most_full = 0;
+   for (fa_reference* r : rs) {
+      most = std::max(most, tr_lds_need(r));
+      most_full = std::max(most_full, tr_lds_full(r));
    }
+   const int64_t per_cu = most ? b->ctx->lds_per_block / ((int64_t)most * (int64_t)sizeof(double)) : 1;
+   const bool by_width_only =
+      (int64_t)n_frames <= (int64_t)b->ctx->n_cu * std::max<int64_t>(per_cu, 1) || most_full <= kTrFullDoubles;
    std::vector<int> order(rs.size());
    for (size_t i = 0; i < rs.size(); i++) order[i] = (int)i;
-   std::stable_sort(order.begin(), order.end(),
-                    [&](int a, int c) { return tr_lds_need(rs[a]) < tr_lds_need(rs[c]); });
+   std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+      if (tr_threads(rs[a]) != tr_threads(rs[c])) return tr_threads(rs[a]) < tr_threads(rs[c]);
+      return tr_lds_need(rs[a]) < tr_lds_need(rs[c]);
+   });
    b->ref_trc.assign(rs.size(), 0);
    b->trc_lds.clear();
    b->trc_full.clear();
+   b->trc_threads.clear();
    int first = 0, largest = 0, largest_full = 0;
    for (int i : order) {
       const int need = tr_lds_need(rs[i]);
-      if (b->trc_lds.empty() || need > 2 * first) {
+      const int nt = tr_threads(rs[i]);
+      if (b->trc_lds.empty() || nt != b->trc_threads.back() || (!by_width_only && need > 2 * first)) {
          b->trc_lds.push_back(0);
          b->trc_full.push_back(0);
+         b->trc_threads.push_back(nt);
          first = need;
          largest = largest_full = 0;
       }
@@ -515,7 +522,7 @@ int launch_updates(fa_batch* b, hipStream_t s)
 {
    const BatchDev B = b->dev();
    for (size_t c = 0; c < b->trc_lds.size(); c++)
-      HIPCHK(launch_tr_update(B, (int)c, b->tune.tr_lds ? b->trc_lds[c] : 0, b->trc_full[c], s));
+      HIPCHK(launch_tr_update(B, (int)c, b->tune.tr_lds ? b->trc_lds[c] : 0, b->trc_full[c], b->trc_threads[c], s));
    return FA_OK;
 }
 

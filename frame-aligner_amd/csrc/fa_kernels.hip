@@ -1373,8 +1373,10 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
 // (FrameWarpAligner.cpp:205-209), over f = SSD, g = Jtr, B = H (OptimisationModel.cpp:45-63).
 // The subproblem is Nocedal & Wright Alg. 4.3 on the block-tridiagonal H.
 //
-// One wave (64 lanes) per frame. Every lane runs the same scalar logic on wave-reduced values,
-// so the control flow is uniform; vector work is spread over the lanes. The factorisations of
+// One workgroup per frame: one wave (64 lanes), or 256 threads for long node chains. Every
+// thread runs the same scalar logic on workgroup-reduced values, so the control flow is
+// uniform; vector work is spread over the threads, with a barrier wherever one thread reads
+// what another wrote (implicit within one wave). The factorisations of
 // H + lambda I and the triangular solves use block cyclic reduction (cr_factor, cr_fwd,
 // cr_bwd): ceil(log2 nD) + 1 parallel levels instead of a sequential sweep over the nodes.
 
@@ -1391,21 +1393,42 @@ __device__ __forceinline__ double wave_sum(double v)
    return __shfl(v, 0, 64);
 }
 
-__device__ __forceinline__ double wave_dot(int n, const double* a, const double* b)
+// Sum over the NT threads of the workgroup, the same value in every thread: the waves' sums in
+// wave order. Every thread calls it at the same point (the update's control flow is uniform).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v)
 {
-   double s = 0;
-   for (int i = threadIdx.x; i < n; i += 64) s += a[i] * b[i];
-   return wave_sum(s);
+   v = wave_sum(v);
+   if constexpr (NT == 64) {
+      return v;
+   } else {
+      __shared__ double red[NT / 64];
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+      __syncthreads();
+      double s = red[0];
+#pragma unroll
+      for (int w = 1; w < NT / 64; w++) s += red[w];
+      __syncthreads();
+      return s;
+   }
 }
 
-// p' H p over the lanes.
-template <int M>
-__device__ __forceinline__ double wave_quad(const RefDev& R, const double* p)
+template <int NT>
+__device__ __forceinline__ double block_dot(int n, const double* a, const double* b)
+{
+   double s = 0;
+   for (int i = threadIdx.x; i < n; i += NT) s += a[i] * b[i];
+   return block_sum<NT>(s);
+}
+
+// p' H p over the workgroup.
+template <int M, int NT>
+__device__ __forceinline__ double block_quad(const RefDev& R, const double* p)
 {
    constexpr int MM = M * M;
    const int nb = R.nD;
    double s = 0;
-   for (int k = threadIdx.x; k < nb; k += 64) {
+   for (int k = threadIdx.x; k < nb; k += NT) {
 #pragma unroll
       for (int a = 0; a < M; a++)
 #pragma unroll
@@ -1417,17 +1440,17 @@ __device__ __forceinline__ double wave_quad(const RefDev& R, const double* p)
             for (int b = 0; b < M; b++)
                s += 2.0 * p[(k + 1) * M + a] * R.Hsub[k * MM + a * M + b] * p[k * M + b];
    }
-   return wave_sum(s);
+   return block_sum<NT>(s);
 }
 
-template <int M>
+template <int M, int NT>
 __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* g, double radius,
                              double eps, int max_iter, double* p, double* lam_out)
 {
    constexpr int MM = M * M;
    const int nb = R.nD, n = nb * M;
-   const double gnorm = sqrt(wave_dot(n, g, g));
-   for (int i = threadIdx.x; i < n; i += 64) p[i] = 0;
+   const double gnorm = sqrt(block_dot<NT>(n, g, g));
+   for (int i = threadIdx.x; i < n; i += NT) p[i] = 0;
    *lam_out = 0;
    __syncthreads();
    if (gnorm == 0) return 1;
@@ -1445,7 +1468,7 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
       } else if (lam == 0) {
          ok = false;
       } else {
-         ok = cr_factor<M, 64>(nb, R.Hdiag, R.Hsub, lam, S.fa, S.fc, S.fw);
+         ok = cr_factor<M, NT>(nb, R.Hdiag, R.Hsub, lam, S.fa, S.fc, S.fw);
          A = S.fa;
          C = S.fc;
          W = S.fw;
@@ -1457,13 +1480,13 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
       }
       // (H + lam I) p = -g
       double* y = S.y;
-      for (int i = threadIdx.x; i < n; i += 64) y[i] = -g[i];
+      for (int i = threadIdx.x; i < n; i += NT) y[i] = -g[i];
       __syncthreads();
-      cr_fwd<M, 64>(nb, A, C, W, y);
-      cr_bwd<M, 64>(nb, A, C, W, y);
-      for (int i = threadIdx.x; i < n; i += 64) p[i] = y[i];
+      cr_fwd<M, NT>(nb, A, C, W, y);
+      cr_bwd<M, NT>(nb, A, C, W, y);
+      for (int i = threadIdx.x; i < n; i += NT) p[i] = y[i];
       *lam_out = lam;
-      const double pn = sqrt(wave_dot(n, y, y));
+      const double pn = sqrt(block_dot<NT>(n, y, y));
       if (lam == 0 && pn <= radius) {
          it++;
          break;
@@ -1475,8 +1498,8 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
       if (pn > radius) lam_lo = lam;
       else lam_hi = lam;
       // q = L^-1 p (y holds p)
-      cr_fwd<M, 64>(nb, A, C, W, y);
-      const double qn = sqrt(wave_dot(n, y, y));
+      cr_fwd<M, NT>(nb, A, C, W, y);
+      const double qn = sqrt(block_dot<NT>(n, y, y));
       double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
       if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
       lam = lam_new;
@@ -1491,17 +1514,18 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
 // factorisation succeeded). Each iteration is two wave reductions instead of a factorisation
 // and three block-triangular solves; p = -Q (qv / (ev + lam)) is formed once, for the last
 // lambda at which the factor exists.
+template <int NT>
 __device__ int tr_subproblem_eig(const RefDev& R, const TrScratch& S, const double* g, double radius,
                                  double eps, int max_iter, double* p, double* lam_out)
 {
    const int n = R.nD * R.n_dim;
    double* qv = S.fa;  // the factor scratch is free on this path
-   const double gnorm = sqrt(wave_dot(n, g, g));
-   for (int i = threadIdx.x; i < n; i += 64) p[i] = 0;
+   const double gnorm = sqrt(block_dot<NT>(n, g, g));
+   for (int i = threadIdx.x; i < n; i += NT) p[i] = 0;
    *lam_out = 0;
    __syncthreads();
    if (gnorm == 0) return 1;
-   for (int i = threadIdx.x; i < n; i += 64) {
+   for (int i = threadIdx.x; i < n; i += NT) {
       double s = 0;
       for (int r = 0; r < n; r++) s += R.Q[(int64_t)r * n + i] * g[r];
       qv[i] = s;
@@ -1519,13 +1543,13 @@ __device__ int tr_subproblem_eig(const RefDev& R, const TrScratch& S, const doub
          continue;
       }
       double s2 = 0, s3 = 0;
-      for (int i = threadIdx.x; i < n; i += 64) {
+      for (int i = threadIdx.x; i < n; i += NT) {
          const double d = 1.0 / (R.ev[i] + lam);
          const double a = qv[i] * d;
          s2 += a * a;
          s3 += a * a * d;
       }
-      const double pn = sqrt(wave_sum(s2));
+      const double pn = sqrt(block_sum<NT>(s2));
       lam_sol = lam;
       if (lam == 0 && pn <= radius) {
          it++;
@@ -1537,16 +1561,16 @@ __device__ int tr_subproblem_eig(const RefDev& R, const TrScratch& S, const doub
       }
       if (pn > radius) lam_lo = lam;
       else lam_hi = lam;
-      const double qn = sqrt(wave_sum(s3));
+      const double qn = sqrt(block_sum<NT>(s3));
       double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
       if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
       lam = lam_new;
    }
    if (lam_sol >= 0) {
       double* w = S.fa + n;
-      for (int i = threadIdx.x; i < n; i += 64) w[i] = qv[i] / (R.ev[i] + lam_sol);
+      for (int i = threadIdx.x; i < n; i += NT) w[i] = qv[i] / (R.ev[i] + lam_sol);
       __syncthreads();
-      for (int r = threadIdx.x; r < n; r += 64) {
+      for (int r = threadIdx.x; r < n; r += NT) {
          double s = 0;
          for (int i = 0; i < n; i++) s += R.Q[(int64_t)r * n + i] * w[i];
          p[r] = -s;
@@ -1578,17 +1602,19 @@ __device__ __forceinline__ TrScratch scratch_of(const BatchDev& B, const BatchFr
 // SSD: lanes take chunks strided, then a fixed butterfly; Jtr: one node per lane, chunks of
 // interval k-1 (weight f) and interval k (weight 1-f) in order. Fixed order, so the result
 // does not depend on which other frames share the launch.
+template <int NT>
 __device__ __forceinline__ double reduce_ssd(const RefDev& R, const double* P)
 {
    double s = 0;
-   for (int c = threadIdx.x; c < R.n_chunks; c += 64) s += P[(int64_t)c * kPassStride];
-   return wave_sum(s);
+   for (int c = threadIdx.x; c < R.n_chunks; c += NT) s += P[(int64_t)c * kPassStride];
+   return block_sum<NT>(s);
 }
 
+template <int NT>
 __device__ __forceinline__ void reduce_jtr(const RefDev& R, const double* P, double* g)
 {
    const int m = R.n_dim;
-   for (int k = threadIdx.x; k < R.nD; k += 64) {
+   for (int k = threadIdx.x; k < R.nD; k += NT) {
       double acc[3] = {0, 0, 0};
       if (k >= 1)
          for (int c = R.ichunk[k - 1]; c < R.ichunk[k]; c++)
@@ -1606,9 +1632,9 @@ __global__ __launch_bounds__(64) void k_reduce_eval(BatchDev B)
    const BatchFrame& bf = B.frames[j];
    const RefDev& R = B.refs[bf.ref];
    const double* P = B.partials + bf.work_base * kPassStride;
-   const double f = reduce_ssd(R, P);
+   const double f = reduce_ssd<64>(R, P);
    double* g = B.vec + bf.vec_base + (int64_t)bf.n;
-   reduce_jtr(R, P, g);
+   reduce_jtr<64>(R, P, g);
    if (threadIdx.x == 0) B.state[j].f = f;
 }
 
@@ -1642,7 +1668,7 @@ __global__ void k_tr_init_slots(BatchDev B, const int32_t* slots) { init_frame(B
 // small dependent steps run at LDS latency; x, p, g are written back at the end. full = 0:
 // only the factor blocks and the solve vector (read and written at every level of the cyclic
 // reduction) are in LDS.
-template <int M, bool LDS>
+template <int M, bool LDS, int NT>
 __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, int full)
 {
    constexpr int MM = M * M;
@@ -1669,17 +1695,17 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
          S.p = S.x + n;
          S.g = S.p + n;
          int32_t* ich = (int32_t*)(S.g + n);
-         for (int i = threadIdx.x; i < np; i += 64) pl[i] = P[i];
-         for (int i = threadIdx.x; i < nb * MM; i += 64) {
+         for (int i = threadIdx.x; i < np; i += NT) pl[i] = P[i];
+         for (int i = threadIdx.x; i < nb * MM; i += NT) {
             hd[i] = R.Hdiag[i];
             hs[i] = R.Hsub[i];
          }
-         for (int i = threadIdx.x; i < n; i += 64) {
+         for (int i = threadIdx.x; i < n; i += NT) {
             S.x[i] = G.x[i];
             S.p[i] = G.p[i];
             S.g[i] = G.g[i];
          }
-         for (int i = threadIdx.x; i < nb; i += 64) ich[i] = R.ichunk[i];
+         for (int i = threadIdx.x; i < nb; i += NT) ich[i] = R.ichunk[i];
          __syncthreads();
          P = pl;
          R.Hdiag = hd;
@@ -1687,7 +1713,7 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
          R.ichunk = ich;
       }
    }
-   const double f_new = reduce_ssd(R, P);
+   const double f_new = reduce_ssd<NT>(R, P);
    TrState st = B.state[j];
    int accept = 0;
    if (st.phase == 0) {
@@ -1697,8 +1723,8 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
       accept = 1;  // first point: take f and g, no step bookkeeping
    } else {
       st.evaluations++;
-      const double gp = wave_dot(n, S.g, S.p);
-      const double pred = -(0.5 * wave_quad<M>(R, S.p) + gp);
+      const double gp = block_dot<NT>(n, S.g, S.p);
+      const double pred = -(0.5 * block_quad<M, NT>(R, S.p) + gp);
       const double meas = st.f - f_new;
       if (fabs(pred) <= fabs(meas) * DBL_EPSILON) {
          st.status = FA_STOP_NO_PROGRESS;
@@ -1717,7 +1743,7 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
                st.radius = 2 * st.radius < 1000 ? 2 * st.radius : 1000;
             }
             if (rho > 0) {
-               for (int i = threadIdx.x; i < n; i += 64) S.x[i] = S.x[i] + S.p[i];
+               for (int i = threadIdx.x; i < n; i += NT) S.x[i] = S.x[i] + S.p[i];
                st.f = f_new;
                st.stale = 0;
                accept = 1;
@@ -1727,7 +1753,7 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
                if (prm.skip_repeat && st.lambda == 0 && !collapse) {
                   // the unchanged interior step would be proposed, evaluated and rejected
                   // again while it fits the shrunken region
-                  const double pn = sqrt(wave_dot(n, S.p, S.p));
+                  const double pn = sqrt(block_dot<NT>(n, S.p, S.p));
                   while (pn <= st.radius) {
                      st.radius *= 0.25;
                      st.rejected++;
@@ -1745,7 +1771,7 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
          }
       }
    }
-   if (accept) reduce_jtr(R, P, S.g);
+   if (accept) reduce_jtr<NT>(R, P, S.g);
    __syncthreads();
    if (!st.done && !st.stale) {
       st.iterations++;
@@ -1771,19 +1797,19 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
    }
    if (!st.done) {
       double lam;
-      st.sub_iters = R.eig ? tr_subproblem_eig(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam)
-                           : tr_subproblem<M>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
+      st.sub_iters = R.eig ? tr_subproblem_eig<NT>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam)
+                           : tr_subproblem<M, NT>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
       st.lambda = lam;
       st.phase = 1;
       double* Dt = B.Dtrial + bf.d_base;
-      for (int i = threadIdx.x; i < nb; i += 64)
+      for (int i = threadIdx.x; i < nb; i += NT)
 #pragma unroll
          for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < M ? S.x[i * M + c] + S.p[i * M + c] : 0.0;
    }
    if (threadIdx.x == 0) B.state[j] = st;
    if (LDS && full) {
       __syncthreads();
-      for (int i = threadIdx.x; i < n; i += 64) {
+      for (int i = threadIdx.x; i < n; i += NT) {
          G.x[i] = S.x[i];
          G.p[i] = S.p[i];
          G.g[i] = S.g[i];
@@ -1791,16 +1817,19 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
    }
 }
 
-// One launch per update class (frames whose factor blocks need a similar amount of LDS):
-// the frames of other classes return at once.
-template <bool LDS>
-__global__ __launch_bounds__(64) void k_tr_update(BatchDev B, int cls, int full)
+// One launch per update class (frames of one workgroup width whose factor blocks need a
+// similar amount of LDS): the frames of other classes return at once. Long node chains use 256
+// threads per frame (the first levels of the cyclic reduction have nD/2 independent blocks),
+// short ones one wave; the width is a property of the reference, so a frame's result does
+// not depend on the batch it runs in.
+template <bool LDS, int NT>
+__global__ __launch_bounds__(NT) void k_tr_update(BatchDev B, int cls, int full)
 {
    extern __shared__ double smem[];
    const int j = blockIdx.x;
    if (B.state[j].done || B.frames[j].trc != cls) return;
-   if (B.refs[B.frames[j].ref].n_dim == 2) tr_update_frame<2, LDS>(B, j, smem, full);
-   else tr_update_frame<3, LDS>(B, j, smem, full);
+   if (B.refs[B.frames[j].ref].n_dim == 2) tr_update_frame<2, LDS, NT>(B, j, smem, full);
+   else tr_update_frame<3, LDS, NT>(B, j, smem, full);
 }
 
 // ---- launchers ----------------------------------------------------------------------------
@@ -1897,17 +1926,26 @@ hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, 
 hipError_t configure_kernels(int max_lds_bytes)
 {
    (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes);
-   return hipFuncSetAttribute((const void*)k_tr_update<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+   (void)hipFuncSetAttribute((const void*)k_tr_update<true, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             max_lds_bytes);
+   return hipFuncSetAttribute((const void*)k_tr_update<true, 64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               max_lds_bytes);
 }
 
-hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, int full, hipStream_t s)
+hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, int full, int threads, hipStream_t s)
 {
-   if (lds_doubles > 0)
-      hipLaunchKernelGGL(k_tr_update<true>, dim3(B.n_frames), dim3(64), (size_t)lds_doubles * sizeof(double), s,
-                         B, cls, full);
-   else
-      hipLaunchKernelGGL(k_tr_update<false>, dim3(B.n_frames), dim3(64), 0, s, B, cls, 0);
+   const size_t lds = (size_t)lds_doubles * sizeof(double);
+   if (threads == 256) {
+      if (lds_doubles > 0)
+         hipLaunchKernelGGL((k_tr_update<true, 256>), dim3(B.n_frames), dim3(256), lds, s, B, cls, full);
+      else
+         hipLaunchKernelGGL((k_tr_update<false, 256>), dim3(B.n_frames), dim3(256), 0, s, B, cls, 0);
+   } else {
+      if (lds_doubles > 0)
+         hipLaunchKernelGGL((k_tr_update<true, 64>), dim3(B.n_frames), dim3(64), lds, s, B, cls, full);
+      else
+         hipLaunchKernelGGL((k_tr_update<false, 64>), dim3(B.n_frames), dim3(64), 0, s, B, cls, 0);
+   }
    return hipGetLastError();
 }
 

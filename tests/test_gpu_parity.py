@@ -158,6 +158,28 @@ def test_align_many_nodes_matches_oracle(ctx, name, shape, nD, scan):
         assert np.array_equal(x_g, xs[f_i])
 
 
+def test_batch_same_alone_and_in_mix_update_widths(ctx):
+    """Frames whose update runs with 256 threads (150 nodes) beside frames whose update runs
+    with one wave (9 nodes) in one batch: each result is bit-identical to the frame alone, and
+    the long chain's result agrees with the oracle."""
+    name, shape, nD, scan, _ = MANY_NODE_CASES[0]
+    a, orc_a, ref_a = _pair(ctx, shape, nD, scan, name=name)
+    b, orc_b, ref_b = _pair(ctx, (1, 96, 80), 9, dict(line=1.0, interline=1.0), seed=21)
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    fa_ = [ref_a.register(f) for f in a["frames_w"]]
+    fb_ = [ref_b.register(f) for f in b["frames_w"]]
+    x0 = [truth_start(a, i, 2) for i in range(len(fa_))] + [None] * len(fb_)
+    frames = fa_ + fb_
+    alone = [f.align(x0=x, **kw) for f, x in zip(frames, x0)]
+    xs, stats, _ = F.Batch(frames, **kw).run([x if x is not None else np.zeros(f.ref.nD * f.ref.n_dim)
+                                              for f, x in zip(frames, x0)])
+    for (x1, s1), x2, s2 in zip(alone, xs, stats):
+        assert np.array_equal(x1, x2)
+        assert s1["evaluations"] == s2["evaluations"]
+    x_o, _ = orc_a.align(a["frames_w"][0], x0=x0[0], **kw)
+    assert np.max(np.abs(xs[0] - x_o)) <= D_TOL
+
+
 def test_batch_same_alone_and_in_mix(ctx):
     """Each frame's result is bit-identical alone and inside a mixed batch (two references of
     different size, uint16 and float frames)."""
