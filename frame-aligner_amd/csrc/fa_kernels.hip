@@ -1454,31 +1454,12 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
    *lam_out = 0;
    __syncthreads();
    if (gnorm == 0) return 1;
-   const double eig_lo = R.eig_lo;
-   double lam = 0, lam_lo = 0, lam_hi = gnorm / radius + (eig_lo < 0 ? -eig_lo : 0);
-   int it;
-   for (it = 0; it < max_iter; it++) {
-      const double *A = nullptr, *C = nullptr, *W = nullptr;
-      bool ok;
-      if (lam == 0 && R.h_pd) {
-         A = R.L0;
-         C = R.L0 + (int64_t)nb * MM;
-         W = R.L0 + 2 * (int64_t)nb * MM;
-         ok = true;
-      } else if (lam == 0) {
-         ok = false;
-      } else {
-         ok = cr_factor<M, NT>(nb, R.Hdiag, R.Hsub, lam, S.fa, S.fc, S.fw);
-         A = S.fa;
-         C = S.fc;
-         W = S.fw;
-      }
-      if (!ok) {
-         lam_lo = lam;
-         lam = 0.5 * (lam_lo + lam_hi);
-         continue;
-      }
-      // (H + lam I) p = -g
+   // (H + lam I) p = -g through the factor (A, C, W); ||p|| and, unless the iteration stops
+   // here, ||L^-1 p|| and the new bracket. It is called with the factor of H itself (global
+   // memory) or the one just formed (LDS or global scratch); each call site then sees one
+   // memory space for the factor blocks.
+   double lam = 0, lam_lo = 0, lam_hi;
+   auto solve = [&](const double* A, const double* C, const double* W, double& pn, double& qn) -> bool {
       double* y = S.y;
       for (int i = threadIdx.x; i < n; i += NT) y[i] = -g[i];
       __syncthreads();
@@ -1486,20 +1467,33 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
       cr_bwd<M, NT>(nb, A, C, W, y);
       for (int i = threadIdx.x; i < n; i += NT) p[i] = y[i];
       *lam_out = lam;
-      const double pn = sqrt(block_dot<NT>(n, y, y));
-      if (lam == 0 && pn <= radius) {
-         it++;
-         break;
-      }
-      if (fabs(pn - radius) <= eps * radius) {
-         it++;
-         break;
-      }
+      pn = sqrt(block_dot<NT>(n, y, y));
+      if (lam == 0 && pn <= radius) return true;
+      if (fabs(pn - radius) <= eps * radius) return true;
       if (pn > radius) lam_lo = lam;
       else lam_hi = lam;
       // q = L^-1 p (y holds p)
       cr_fwd<M, NT>(nb, A, C, W, y);
-      const double qn = sqrt(block_dot<NT>(n, y, y));
+      qn = sqrt(block_dot<NT>(n, y, y));
+      return false;
+   };
+   const double eig_lo = R.eig_lo;
+   lam_hi = gnorm / radius + (eig_lo < 0 ? -eig_lo : 0);
+   int it;
+   for (it = 0; it < max_iter; it++) {
+      const bool use_h = lam == 0 && R.h_pd;
+      if (!use_h && !(lam != 0 && cr_factor<M, NT>(nb, R.Hdiag, R.Hsub, lam, S.fa, S.fc, S.fw))) {
+         lam_lo = lam;
+         lam = 0.5 * (lam_lo + lam_hi);
+         continue;
+      }
+      double pn = 0, qn = 0;
+      const bool stop = use_h ? solve(R.L0, R.L0 + (int64_t)nb * MM, R.L0 + 2 * (int64_t)nb * MM, pn, qn)
+                              : solve(S.fa, S.fc, S.fw, pn, qn);
+      if (stop) {
+         it++;
+         break;
+      }
       double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
       if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
       lam = lam_new;
@@ -1668,8 +1662,8 @@ __global__ void k_tr_init_slots(BatchDev B, const int32_t* slots) { init_frame(B
 // small dependent steps run at LDS latency; x, p, g are written back at the end. full = 0:
 // only the factor blocks and the solve vector (read and written at every level of the cyclic
 // reduction) are in LDS.
-template <int M, bool LDS, int NT>
-__device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, int full)
+template <int M, bool LDS, int NT, bool FULL>
+__device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
 {
    constexpr int MM = M * M;
    const BatchFrame& bf = B.frames[j];
@@ -1679,14 +1673,14 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
    TrScratch S = G;
    const double* P = B.partials + bf.work_base * kPassStride;
    const int n = bf.n, nb = R.nD;
-   if (LDS) {
+   if constexpr (LDS) {
       double* q = smem;
       S.fa = q;
       S.fc = q + nb * MM;
       S.fw = q + 2 * nb * MM;
       S.y = q + 3 * nb * MM;
       q += 3 * nb * MM + n;
-      if (full) {
+      if constexpr (FULL) {
          const int np = R.n_chunks * kPassStride;
          double* pl = q;
          double* hd = pl + np;
@@ -1807,7 +1801,7 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
          for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < M ? S.x[i * M + c] + S.p[i * M + c] : 0.0;
    }
    if (threadIdx.x == 0) B.state[j] = st;
-   if (LDS && full) {
+   if constexpr (LDS && FULL) {
       __syncthreads();
       for (int i = threadIdx.x; i < n; i += NT) {
          G.x[i] = S.x[i];
@@ -1822,14 +1816,14 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem, in
 // threads per frame (the first levels of the cyclic reduction have nD/2 independent blocks),
 // short ones one wave; the width is a property of the reference, so a frame's result does
 // not depend on the batch it runs in.
-template <bool LDS, int NT>
-__global__ __launch_bounds__(NT) void k_tr_update(BatchDev B, int cls, int full)
+template <bool LDS, int NT, bool FULL>
+__global__ __launch_bounds__(NT) void k_tr_update(BatchDev B, int cls)
 {
    extern __shared__ double smem[];
    const int j = blockIdx.x;
    if (B.state[j].done || B.frames[j].trc != cls) return;
-   if (B.refs[B.frames[j].ref].n_dim == 2) tr_update_frame<2, LDS, NT>(B, j, smem, full);
-   else tr_update_frame<3, LDS, NT>(B, j, smem, full);
+   if (B.refs[B.frames[j].ref].n_dim == 2) tr_update_frame<2, LDS, NT, FULL>(B, j, smem);
+   else tr_update_frame<3, LDS, NT, FULL>(B, j, smem);
 }
 
 // ---- launchers ----------------------------------------------------------------------------
@@ -1926,25 +1920,30 @@ hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, 
 hipError_t configure_kernels(int max_lds_bytes)
 {
    (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes);
-   (void)hipFuncSetAttribute((const void*)k_tr_update<true, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             max_lds_bytes);
-   return hipFuncSetAttribute((const void*)k_tr_update<true, 64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              max_lds_bytes);
+   const void* fns[] = {(const void*)k_tr_update<true, 64, false>, (const void*)k_tr_update<true, 64, true>,
+                        (const void*)k_tr_update<true, 256, false>, (const void*)k_tr_update<true, 256, true>};
+   hipError_t e = hipSuccess;
+   for (const void* f : fns) {
+      const hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes);
+      if (r != hipSuccess) e = r;
+   }
+   return e;
 }
 
+// The update's variants: factor blocks in global memory (lds_doubles = 0), in LDS, or the whole
+// working set in LDS (full); 64 or 256 threads per frame.
 hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, int full, int threads, hipStream_t s)
 {
+   const dim3 grid(B.n_frames);
    const size_t lds = (size_t)lds_doubles * sizeof(double);
    if (threads == 256) {
-      if (lds_doubles > 0)
-         hipLaunchKernelGGL((k_tr_update<true, 256>), dim3(B.n_frames), dim3(256), lds, s, B, cls, full);
-      else
-         hipLaunchKernelGGL((k_tr_update<false, 256>), dim3(B.n_frames), dim3(256), 0, s, B, cls, 0);
+      if (lds_doubles <= 0) hipLaunchKernelGGL((k_tr_update<false, 256, false>), grid, dim3(256), 0, s, B, cls);
+      else if (full) hipLaunchKernelGGL((k_tr_update<true, 256, true>), grid, dim3(256), lds, s, B, cls);
+      else hipLaunchKernelGGL((k_tr_update<true, 256, false>), grid, dim3(256), lds, s, B, cls);
    } else {
-      if (lds_doubles > 0)
-         hipLaunchKernelGGL((k_tr_update<true, 64>), dim3(B.n_frames), dim3(64), lds, s, B, cls, full);
-      else
-         hipLaunchKernelGGL((k_tr_update<false, 64>), dim3(B.n_frames), dim3(64), 0, s, B, cls, 0);
+      if (lds_doubles <= 0) hipLaunchKernelGGL((k_tr_update<false, 64, false>), grid, dim3(64), 0, s, B, cls);
+      else if (full) hipLaunchKernelGGL((k_tr_update<true, 64, true>), grid, dim3(64), lds, s, B, cls);
+      else hipLaunchKernelGGL((k_tr_update<true, 64, false>), grid, dim3(64), lds, s, B, cls);
    }
    return hipGetLastError();
 }
