@@ -827,11 +827,11 @@ struct Stage {
    int za, nzs, ya, nrows, stride;
 };
 
-// Staged rows are visited as one flat index over (plane, row, column group), kBlock apart per
-// thread; (zi, yr, c) advance without a division, and each thread issues NB loads before it
-// stores any, so a wave has that many row segments in flight instead of waiting out one
-// global-load latency per row. NB is 6 for 2-D items (the pass then fits 7 waves per SIMD)
-// and 8 for 3-D items (two planes: twice the rows).
+// Each thread issues NB loads of staged rows before it stores any, so a wave has that many row
+// segments in flight instead of waiting out one global-load latency per row. NB is 6 for 2-D
+// items (the pass then fits 7 waves per SIMD) and 8 for 3-D items (two planes: twice the rows).
+// Frames whose width is not a multiple of 4 are staged sample by sample over one flat index of
+// (plane, row, column), kBlock apart per thread, with (zi, yr, c) advanced without a division.
 
 struct StageWalk {
    int zi, yr, c;
@@ -896,30 +896,47 @@ __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, 
          *(float4*)&lds[r * S.stride + col] = zero4;
       }
       const int nx4 = nx >> 2;
-      const int total = rows * nx4;
-      const int dr = kBlock / nx4, dc = kBlock - dr * nx4;
-      StageWalk w;
-      w.init(tid, nx4, S.nrows);
       using RT = typename Raw4<FT>::type;
-      for (int i0 = tid; i0 < total; i0 += NB * kBlock) {
-         // every lane loads (rows outside the frame from a clamped, in-bounds row) so that the
-         // loads are not under branches and all NB are in flight together
-         RT raw[NB];
-         int dst[NB];
-         bool ok[NB];
+      // A thread keeps one column group c (four samples) and takes every rpi-th staged row,
+      // rpi = rows per sweep of the workgroup; rows wider than the workgroup take several
+      // column groups per thread. Row r is plane zi, row yr of the tile, tracked
+      // incrementally; indices into the frame are 32-bit (frames hold fewer than 2^31
+      // samples).
+      const int rpi = nx4 <= kBlock ? kBlock / nx4 : 1;
+      const int tr = nx4 <= kBlock ? tid / nx4 : 0;
+      const int cb = nx4 <= kBlock ? tid - tr * nx4 : tid;
+      if (tr < rpi) {
+         for (int c = cb; c < nx4; c += kBlock) {
+            int zi = 0, yr = tr;
+            while (yr >= S.nrows) {
+               yr -= S.nrows;
+               zi++;
+            }
+            for (int r0 = tr; r0 < rows; r0 += NB * rpi) {
+               // every lane loads (rows outside the frame from a clamped, in-bounds row) so
+               // that the loads are not under branches and all NB are in flight together
+               RT raw[NB];
+               int dst[NB];
+               bool ok[NB];
 #pragma unroll
-         for (int u = 0; u < NB; u++) {
-            const int yy = S.ya + w.yr, zz = S.za + w.zi;
-            const bool live = i0 + u * kBlock < total;
-            dst[u] = live ? (w.zi * S.nrows + w.yr) * S.stride + kPad + 4 * w.c : -1;
-            ok[u] = live && (unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz;
-            const int yc = min(max(yy, 0), ny - 1), zc = min(max(zz, 0), nz - 1);
-            raw[u] = *(const RT*)(src + ((int64_t)zc * ny + yc) * nx + 4 * w.c);
-            w.step(nx4, S.nrows, dr, dc);
+               for (int u = 0; u < NB; u++) {
+                  const int yy = S.ya + yr, zz = S.za + zi;
+                  const bool live = r0 + u * rpi < rows;
+                  dst[u] = live ? (r0 + u * rpi) * S.stride + kPad + 4 * c : -1;
+                  ok[u] = live && (unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz;
+                  const unsigned yc = min(max(yy, 0), ny - 1), zc = min(max(zz, 0), nz - 1);
+                  raw[u] = *(const RT*)(src + ((zc * (unsigned)ny + yc) * (unsigned)nx + 4u * c));
+                  yr += rpi;
+                  while (yr >= S.nrows) {
+                     yr -= S.nrows;
+                     zi++;
+                  }
+               }
+#pragma unroll
+               for (int u = 0; u < NB; u++)
+                  if (dst[u] >= 0) *(float4*)&lds[dst[u]] = ok[u] ? cvt4(raw[u], off) : zero4;
+            }
          }
-#pragma unroll
-         for (int u = 0; u < NB; u++)
-            if (dst[u] >= 0) *(float4*)&lds[dst[u]] = ok[u] ? cvt4(raw[u], off) : zero4;
       }
    } else {
       const int total = rows * S.stride;
