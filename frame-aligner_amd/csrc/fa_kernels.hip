@@ -792,28 +792,49 @@ __global__ __launch_bounds__(kEigThreads) void k_eigen(RefDev R, double* Q, doub
 // reach are staged in LDS when they fit.
 enum { MODE_PASS = 0, MODE_WARP = 1 };
 
+// Sum over the 64 lanes of a wave with DPP lane moves (no LDS traffic): pairs and quads
+// (quad_perm), the two halves of each row of 8 and of 16 (row_half_mirror, row_mirror), then
+// row 0 into row 1 and row 2 into row 3 (row_bcast15), and rows 0-1 into rows 2-3
+// (row_bcast31). The total is in lane 63; the order of the additions is fixed.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v)
+{
+   const int t = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
+   return v + __int_as_float(t);
+}
+
+__device__ __forceinline__ float wave_total(float v)
+{
+   v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+   v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+   v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
+   v = dpp_add<0x140, 0xf>(v);  // row_mirror
+   v = dpp_add<0x142, 0xa>(v);  // row_bcast15 into rows 1 and 3
+   v = dpp_add<0x143, 0xc>(v);  // row_bcast31 into rows 2 and 3
+   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Block reduction used by the fused pass: the compensated pairs of each thread are summed as
-// two plain float trees (leading parts and compensations) and combined once per wave.
+// two plain float sums per wave (leading parts and compensations), combined in double over
+// the waves.
 template <int NV>
 __device__ __forceinline__ void block_reduce_fast(KF (&acc)[NV], KF (*sred)[NV], double* out)
 {
+   KF w[NV];
 #pragma unroll
-   for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-      for (int v = 0; v < NV; v++) {
-         acc[v].s += __shfl_xor(acc[v].s, off, 64);
-         acc[v].c += __shfl_xor(acc[v].c, off, 64);
-      }
+   for (int v = 0; v < NV; v++) {
+      w[v].s = wave_total(acc[v].s);
+      w[v].c = wave_total(acc[v].c);
    }
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    if (lane == 0)
 #pragma unroll
-      for (int v = 0; v < NV; v++) sred[wave][v] = acc[v];
+      for (int v = 0; v < NV; v++) sred[wave][v] = w[v];
    __syncthreads();
    if (threadIdx.x < NV) {
       const int v = threadIdx.x;
       double t = 0;
-      for (int w = 0; w < kBlock / 64; w++) t += (double)sred[w][v].s - (double)sred[w][v].c;
+      for (int q = 0; q < kBlock / 64; q++) t += (double)sred[q][v].s - (double)sred[q][v].c;
       out[v] = t;
    }
    __syncthreads();
