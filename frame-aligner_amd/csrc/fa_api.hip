@@ -244,14 +244,14 @@ struct fa_batch {
 
 namespace {
 
-// LDS a work item of reference r stages: its rows (one more when it starts mid-row) plus the
-// row below for the taps and one for a displacement range that crosses a row boundary; in
-// 3-D two planes (work items whose z range crosses a plane boundary need three and sample
-// from global memory instead).
+// LDS a work item of reference r stages: the rows the longest item spans plus the row below
+// for the taps and one for a displacement range that crosses a row boundary; in 3-D two
+// planes (work items whose z range crosses a plane boundary need three and sample from
+// global memory instead).
 int64_t lds_floats_for(const fa_reference* r)
 {
    const int64_t stride = ((r->nx + 2 * kPad) + 3) & ~3;
-   const int64_t rows = (r->chunk_px + r->nx - 1) / r->nx + 3;
+   const int64_t rows = r->item_rows + 2;
    return rows * stride * (r->n_dim == 3 ? 2 : 1);
 }
 
@@ -1285,18 +1285,27 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    }
    r->D_range[2 * last + 1] = (int32_t)(n_px - 1);
 
-   // work chunks: within one interval and one plane, at most chunk_px pixels
+   // work chunks: the part of an interval in one plane is cut into the fewest items of at
+   // most chunk_px pixels, of equal length (to one pixel), so no item is a short remainder
    const int64_t plane = (int64_t)nx * ny;
    r->ichunk.assign(nD, 0);
+   r->item_rows = 1;
    for (int i = 0; i <= nD - 2; i++) {
       r->ichunk[i] = (int32_t)r->chunks.size();
       if (begin[i] == INT32_MAX) continue;
       int64_t b0 = r->D_range[2 * i], e0 = r->D_range[2 * i + 1];
-      for (int64_t p = b0; p <= e0;) {
-         int64_t plane_end = (p / plane + 1) * plane - 1;
-         int64_t q = std::min<int64_t>({e0, plane_end, p + r->chunk_px - 1});
-         r->chunks.push_back(Chunk{(int32_t)p, (int32_t)(q + 1), i, (int32_t)e0});
-         p = q + 1;
+      for (int64_t s0 = b0; s0 <= e0;) {
+         const int64_t s1 = std::min<int64_t>(e0, (s0 / plane + 1) * plane - 1);  // segment end
+         const int64_t len = s1 - s0 + 1, m = (len + r->chunk_px - 1) / r->chunk_px;
+         const int64_t base = len / m, extra = len % m;
+         int64_t p = s0;
+         for (int64_t t = 0; t < m; t++) {
+            const int64_t q = p + base + (t < extra ? 1 : 0) - 1;
+            r->chunks.push_back(Chunk{(int32_t)p, (int32_t)(q + 1), i, (int32_t)e0});
+            r->item_rows = std::max<int>(r->item_rows, (int)(q / nx - p / nx + 1));
+            p = q + 1;
+         }
+         s0 = s1 + 1;
       }
    }
    r->ichunk[nD - 1] = (int32_t)r->chunks.size();

@@ -159,6 +159,7 @@ struct fa_context {
 struct fa_reference {
    fa_context* ctx = nullptr;
    int nz = 0, ny = 0, nx = 0, nD = 0, n_dim = 0, chunk_px = 0;
+   int item_rows = 1;  // most frame rows one work item spans
    int64_t n_px = 0;
    fa::ScanDev scan{};
    double scan_host[6] = {0};

@@ -129,8 +129,9 @@ int fa_destroy(fa_context* ctx);
    (FrameAligner/FrameWarper.cpp:186-205; GPU set-up GpuFrameWarper.cpp:75-152).
    ref: nz*ny*nx floats, already smoothed and offset by the caller, as the reference's
    aligner passes it (FrameWarpAligner.cpp:86-102). Builds the node map, Sobel gradient,
-   block-tridiagonal JtJ and its Cholesky factor on the device. chunk_px = 0 picks the
-   default work-item size (pixels per workgroup task). */
+   block-tridiagonal JtJ and its Cholesky factor on the device. Work items (pixels per
+   workgroup task) cut the part of each node interval in one plane into the fewest pieces of
+   at most chunk_px pixels, of equal length; chunk_px = 0 picks the default. */
 int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, int32_t nx,
                      int32_t nD, const fa_scan_params* scan, int32_t chunk_px,
                      fa_reference** out);
