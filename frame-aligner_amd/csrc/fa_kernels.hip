@@ -1081,8 +1081,12 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          recs[u] = R.rg4[pu];
          fs[u] = (M == 3) ? R.df[pu] : recs[u].w;
       }
+      // the wave's first pixel of this step: once a slot starts past the item's end, this and
+      // the later slots of the step hold no pixel of the wave (the last step of an item)
+      const int wave_first = __builtin_amdgcn_readfirstlane(pb - (int)(threadIdx.x & 63));
 #pragma unroll
       for (int u = 0; u < kUnroll; u++, w.step(nx, sdx, sdy)) {
+         if (wave_first + u * kBlock >= ch.p1) break;
          const int p = pb + u * kBlock;
          const bool in = p < ch.p1;
          const float4 rec = recs[u];
@@ -1101,8 +1105,10 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          const float wy1 = wt[1], wy0 = 1.f - wt[1];
          float v = 0.f;
          if (use) {
-            int idx = (y0 - S.ya) * S.stride + x0 + kPad;
-            if (M == 3) idx += (plane + ti[2] - S.za) * S.nrows * S.stride;
+            // 24-bit products (full rate): in-tile offsets are far below 2^23; masked pixels
+            // past the item may give anything, which the clamp keeps inside the tile
+            int idx = __mul24(y0 - S.ya, S.stride) + x0 + kPad;
+            if (M == 3) idx += __mul24(plane + ti[2] - S.za, S.nrows * S.stride);
             idx = min(max(idx, 0), lds_lim);
             const float* L = lds + idx;
             const float v0 = wy0 * (wx0 * L[0] + wx1 * L[1]) + wy1 * (wx0 * L[S.stride] + wx1 * L[S.stride + 1]);
