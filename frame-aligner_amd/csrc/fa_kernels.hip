@@ -1337,9 +1337,8 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
 {
    __shared__ int32_t s_cnt[kMaxRefs];
    __shared__ int32_t s_fill[kMaxRefs];
-   __shared__ int64_t s_w[1024];
-   __shared__ int64_t s_w2[1024];
-   __shared__ int32_t s_f[1024];
+   __shared__ int32_t s_wf[1024 / 64];
+   __shared__ int64_t s_ww[1024 / 64], s_ww2[1024 / 64];
    __shared__ unsigned long long s_bytes;
    const int t = threadIdx.x, T = blockDim.x;
    const int n = B.n_frames, R = B.n_refs;
@@ -1370,21 +1369,34 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
       ws += w;
       if (B.refs[r].n_dim == 2) w2s += w;
    }
-   s_f[t] = fs;
-   s_w[t] = ws;
-   s_w2[t] = w2s;
-   __syncthreads();
-   for (int off = 1; off < T; off <<= 1) {
-      const int32_t af = t >= off ? s_f[t - off] : 0;
-      const int64_t aw = t >= off ? s_w[t - off] : 0, aw2 = t >= off ? s_w2[t - off] : 0;
-      __syncthreads();
-      s_f[t] += af;
-      s_w[t] += aw;
-      s_w2[t] += aw2;
-      __syncthreads();
+   // inclusive scans of the threads' sums within each wave (lane shuffles), then the waves'
+   // totals before this wave
+   const int lane = t & 63, wv = t >> 6;
+   int32_t f_in = fs;
+   int64_t w_in = ws, w2_in = w2s;
+#pragma unroll
+   for (int off = 1; off < 64; off <<= 1) {
+      const int32_t a = __shfl_up(f_in, off, 64);
+      const int64_t b = __shfl_up(w_in, off, 64), c = __shfl_up(w2_in, off, 64);
+      if (lane >= off) {
+         f_in += a;
+         w_in += b;
+         w2_in += c;
+      }
    }
-   int32_t fb = s_f[t] - fs;
-   int64_t wb = s_w[t] - ws;
+   if (lane == 63) {
+      s_wf[wv] = f_in;
+      s_ww[wv] = w_in;
+      s_ww2[wv] = w2_in;
+   }
+   __syncthreads();
+   for (int q = 0; q < wv; q++) {
+      f_in += s_wf[q];
+      w_in += s_ww[q];
+      w2_in += s_ww2[q];
+   }
+   int32_t fb = f_in - fs;
+   int64_t wb = w_in - ws;
    for (int r = r0; r < r1; r++) {
       const int32_t c = s_cnt[r];
       B.run_first[r] = fb;
@@ -1395,10 +1407,10 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
       wb += (int64_t)c * B.refs[r].n_chunks;
    }
    if (t == T - 1) {
-      B.run_base[R] = s_w[t];
-      B.counters->n_active_frames = s_f[t];
-      B.counters->n_active_work = s_w[t];
-      B.counters->n_active_work2 = s_w2[t];
+      B.run_base[R] = w_in;
+      B.counters->n_active_frames = f_in;
+      B.counters->n_active_work = w_in;
+      B.counters->n_active_work2 = w2_in;
       if (account) {
          B.counters->bytes += s_bytes;
          B.counters->rounds += 1;
