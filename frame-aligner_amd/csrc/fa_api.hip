@@ -1204,7 +1204,8 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    *out = nullptr;
    if (nz < 1 || ny < 2 || nx < 2) return set_err(FA_E_ARG, "Reference must be three dimensional");
    const int64_t n_px = (int64_t)nz * ny * nx;
-   if (n_px >= INT32_MAX) return set_err(FA_E_ARG, "volume too large");
+   // pixel indices are 32-bit, with room for the pass to step a few blocks past an item's end
+   if (n_px > (int64_t)INT32_MAX - 4 * kBlock) return set_err(FA_E_ARG, "volume too large");
    if (nD < 2 || nD - 1 > 65535 || nD <= nz) return set_err(FA_E_ARG, "invalid number of nodes");
    if (scan->bidirectional) return set_err(FA_E_ARG, "bidirectional scans are not supported");
    std::lock_guard<std::mutex> lk(ctx->mu);

@@ -97,6 +97,50 @@ def test_error_and_jacobian(ctx, name, shape, nD, scan, dtype):
         fr.release()
 
 
+# Widths that are not a multiple of 4 stage their rows sample by sample (the other widths in
+# groups of four); 3-D with an odd width.
+ODD_WIDTH_CASES = [
+    ("2d_width70", (1, 64, 70), 9, dict(line=1.0, interline=1.0), {}),
+    ("3d_width37", (4, 40, 37), 4 * 4 + 1, dict(line=1.0, interline=1.0), dict(n_blobs=90, sigma=(1.5, 4.0))),
+]
+
+
+@pytest.mark.parametrize("name,shape,nD,scan,gen", ODD_WIDTH_CASES)
+@pytest.mark.parametrize("dtype", ["float32", "uint16"])
+def test_error_and_jacobian_odd_width(ctx, name, shape, nD, scan, gen, dtype):
+    st, orc, ref = _pair(ctx, shape, nD, scan, dtype=dtype, **gen)
+    rng = np.random.default_rng(12)
+    for f_i in range(len(st["frames"])):
+        fr = ref.register(st["frames"][f_i], offset=1.0)
+        fw = st["frames_w"][f_i]
+        for D in _displacements(st, nD, rng):
+            if ref.n_dim == 2:
+                D = D.copy()
+                D[:, 2] = 0
+            e_o, j_o = orc.get_jacobian(fw, D)
+            e_g, j_g = fr.jacobian(D)
+            assert abs(e_g - e_o) <= ERR_TOL * abs(e_o), (e_g, e_o)
+            assert rel_err(j_g, j_o) < JAC_TOL, rel_err(j_g, j_o)
+        fr.release()
+
+
+@pytest.mark.parametrize("name,shape,nD,scan,gen", [c[:4] + (GEN.get(c[0], {}),) for c in CASES] + ODD_WIDTH_CASES)
+def test_staged_and_global_sampling_agree(ctx, name, shape, nD, scan, gen):
+    """Work items sample the frame from their LDS tile or, when it does not fit, from global
+    memory with the same arithmetic: a batch run with staging switched off gives the same bits."""
+    st, orc, ref = _pair(ctx, shape, nD, scan, dtype="uint16", **gen)
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    frames = [ref.register(f, offset=1.0) for f in st["frames"]]
+    batch = F.Batch(frames, **kw)
+    x1, s1, _ = batch.run()
+    batch.set_tuning(use_lds=0)
+    x0, s0, _ = batch.run()
+    for a, b, sa, sb in zip(x1, x0, s1, s0):
+        assert np.array_equal(a, b)
+        assert sa["evaluations"] == sb["evaluations"]
+    batch.close()
+
+
 @pytest.mark.parametrize("name,shape,nD,scan", CASES)
 def test_warp_interpolated(ctx, name, shape, nD, scan):
     st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
