@@ -379,9 +379,11 @@ __device__ __forceinline__ void stb_factor(double* p, const double (&L)[M][M], c
 }
 
 // Factor of Hd/Hs + lambda I into (A, C, W); false (on every thread) when not positive definite.
+// With y (nb blocks of M), also y := L^-1 y within the same levels: the steps of cr_fwd, in its
+// order, after each level's elimination and Schur update (y is left undefined on failure).
 template <int M, int NT>
 __device__ bool cr_factor(int nb, const double* __restrict__ Hd, const double* __restrict__ Hs, double lambda,
-                          double* A, double* C, double* W)
+                          double* A, double* C, double* W, double* y = nullptr)
 {
    constexpr int MM = M * M;
    const int t = threadIdx.x;
@@ -405,6 +407,14 @@ __device__ bool cr_factor(int nb, const double* __restrict__ Hd, const double* _
 #pragma unroll
          for (int r = 0; r < M; r++) il[r] = 1.0 / L[r][r];
          stb_factor<M>(A + i * MM, L, il);
+         if (y) {
+            double v[M];
+#pragma unroll
+            for (int a = 0; a < M; a++) v[a] = y[i * M + a];
+            fwd_block<M>(A + i * MM, v);
+#pragma unroll
+            for (int a = 0; a < M; a++) y[i * M + a] = v[a];
+         }
          ldb<M>(C + (i - s) * MM, X);
          trsm_lower<M>(L, il, X);
          stb<M>(W + i * MM, X);
@@ -463,6 +473,27 @@ __device__ bool cr_factor(int nb, const double* __restrict__ Hd, const double* _
                }
          }
          stb<M>(A + j * MM, Aj);
+         if (y) {
+            double v[M];
+#pragma unroll
+            for (int a = 0; a < M; a++) v[a] = y[j * M + a];
+            if (j + s < nb) {
+               const double* X = W + (j + s) * MM;
+#pragma unroll
+               for (int a = 0; a < M; a++)
+#pragma unroll
+                  for (int r = 0; r < M; r++) v[a] -= X[r * M + a] * y[(j + s) * M + r];
+            }
+            if (j >= s) {
+               const double* V = C + (j - s) * MM;
+#pragma unroll
+               for (int a = 0; a < M; a++)
+#pragma unroll
+                  for (int r = 0; r < M; r++) v[a] -= V[r * M + a] * y[(j - s) * M + r];
+            }
+#pragma unroll
+            for (int a = 0; a < M; a++) y[j * M + a] = v[a];
+         }
       }
       __syncthreads();
    }
@@ -474,6 +505,14 @@ __device__ bool cr_factor(int nb, const double* __restrict__ Hd, const double* _
 #pragma unroll
       for (int r = 0; r < M; r++) il[r] = 1.0 / L[r][r];
       stb_factor<M>(A, L, il);
+      if (y && !fail) {
+         double v[M];
+#pragma unroll
+         for (int a = 0; a < M; a++) v[a] = y[a];
+         fwd_block<M>(A, v);
+#pragma unroll
+         for (int a = 0; a < M; a++) y[a] = v[a];
+      }
    }
    return __syncthreads_or(fail) == 0;
 }
@@ -1515,11 +1554,15 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
    // memory) or the one just formed (LDS or global scratch); each call site then sees one
    // memory space for the factor blocks.
    double lam = 0, lam_lo = 0, lam_hi;
-   auto solve = [&](const double* A, const double* C, const double* W, double& pn, double& qn) -> bool {
+   // fwd_done: y already holds L^-1 (-g) (formed with the factor)
+   auto solve = [&](const double* A, const double* C, const double* W, bool fwd_done, double& pn,
+                    double& qn) -> bool {
       double* y = S.y;
-      for (int i = threadIdx.x; i < n; i += NT) y[i] = -g[i];
-      __syncthreads();
-      cr_fwd<M, NT>(nb, A, C, W, y);
+      if (!fwd_done) {
+         for (int i = threadIdx.x; i < n; i += NT) y[i] = -g[i];
+         __syncthreads();
+         cr_fwd<M, NT>(nb, A, C, W, y);
+      }
       cr_bwd<M, NT>(nb, A, C, W, y);
       for (int i = threadIdx.x; i < n; i += NT) p[i] = y[i];
       *lam_out = lam;
@@ -1538,14 +1581,23 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
    int it;
    for (it = 0; it < max_iter; it++) {
       const bool use_h = lam == 0 && R.h_pd;
-      if (!use_h && !(lam != 0 && cr_factor<M, NT>(nb, R.Hdiag, R.Hsub, lam, S.fa, S.fc, S.fw))) {
-         lam_lo = lam;
-         lam = 0.5 * (lam_lo + lam_hi);
-         continue;
+      if (!use_h) {
+         // factor H + lam I and, in the same levels, the forward solve of -g
+         bool ok = lam != 0;
+         if (ok) {
+            for (int i = threadIdx.x; i < n; i += NT) S.y[i] = -g[i];
+            __syncthreads();
+            ok = cr_factor<M, NT>(nb, R.Hdiag, R.Hsub, lam, S.fa, S.fc, S.fw, S.y);
+         }
+         if (!ok) {
+            lam_lo = lam;
+            lam = 0.5 * (lam_lo + lam_hi);
+            continue;
+         }
       }
       double pn = 0, qn = 0;
-      const bool stop = use_h ? solve(R.L0, R.L0 + (int64_t)nb * MM, R.L0 + 2 * (int64_t)nb * MM, pn, qn)
-                              : solve(S.fa, S.fc, S.fw, pn, qn);
+      const bool stop = use_h ? solve(R.L0, R.L0 + (int64_t)nb * MM, R.L0 + 2 * (int64_t)nb * MM, false, pn, qn)
+                              : solve(S.fa, S.fc, S.fw, true, pn, qn);
       if (stop) {
          it++;
          break;
