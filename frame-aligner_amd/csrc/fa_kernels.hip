@@ -387,10 +387,26 @@ __device__ bool cr_factor(int nb, const double* __restrict__ Hd, const double* _
 {
    constexpr int MM = M * M;
    const int t = threadIdx.x;
-   for (int i = t; i < nb * MM; i += NT) {
-      const int e = i % MM;
-      A[i] = Hd[i] + (e % (M + 1) == 0 ? lambda : 0.0);
-      C[i] = i < (nb - 1) * MM ? Hs[i] : 0.0;
+   // H + lambda I into (A, C): each thread loads kCopy entries of both before it stores any,
+   // so the loads (global memory or LDS) overlap instead of completing one by one
+   constexpr int kCopy = 8;
+   const int ne = nb * MM, nc = (nb - 1) * MM;
+   for (int i0 = t; i0 < ne; i0 += kCopy * NT) {
+      double hd[kCopy], hs[kCopy];
+#pragma unroll
+      for (int u = 0; u < kCopy; u++) {
+         const int i = min(i0 + u * NT, ne - 1);
+         hd[u] = Hd[i];
+         hs[u] = Hs[min(i, nc > 0 ? nc - 1 : 0)];
+      }
+#pragma unroll
+      for (int u = 0; u < kCopy; u++) {
+         const int i = i0 + u * NT;
+         if (i < ne) {
+            A[i] = hd[u] + ((i % MM) % (M + 1) == 0 ? lambda : 0.0);
+            C[i] = i < nc ? hs[u] : 0.0;
+         }
+      }
    }
    __syncthreads();
    for (int s = 1; s < nb; s <<= 1) {
