@@ -201,6 +201,8 @@ def timed_steps(batch, steps):
 
 
 def roofline_of(pass_ms, pass_bytes, launches, kernel):
+    """Pass roofline from HIP events around every fused pass (eager runs; a batch split into two
+    parts runs them one after the other in those runs, so each pass is timed alone)."""
     gbs = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": kernel, "launches": int(launches),
@@ -433,7 +435,8 @@ def headline(F, args, ctx):
                      "traffic": measured_traffic(),
                      "kernel": "k_pass (fused warp + SSD + Jtr)",
                      "measured": f"HIP events around every fused pass of {args.steps} more steps of "
-                                 f"the same workload (eager launches)",
+                                 f"the same workload (eager launches; the batch's two parts run one "
+                                 f"after the other, so each pass is timed alone)",
                      "launches": launches, "avg_launch_us": round(1e3 * pass_ms / max(launches, 1), 2),
                      "bytes_per_launch": pass_bytes / max(launches, 1),
                      "isolated": {"avg_launch_us": round(1e3 * iso_ms, 2), "bytes_per_launch": iso_bytes,

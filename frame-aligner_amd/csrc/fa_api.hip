@@ -680,36 +680,47 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
    if (graphs)
       for (fa_batch* q : leaves) CHK(ensure_graph(q, K));
    for (size_t i = 0; i < leaves.size(); i++) CHK(run_begin(leaves[i], x_inout + xoff[i]));
-   int64_t round = 0;
    const int64_t limit = max_rounds(b);
-   for (;;) {
-      bool any = false;
-      if (graphs) {
-         for (fa_batch* q : leaves)
-            if (!q->finished) {
-               HIPCHK(hipGraphLaunch(q->gexec, q->stream));
-               any = true;
-            }
-         round += K;
-      } else {
-         for (int r = 0; r < K && round < limit; r++, round++)
-            for (fa_batch* q : leaves)
+   // runs the leaves of `set` to completion, their rounds interleaved
+   auto run_set = [&](const std::vector<fa_batch*>& set) -> int {
+      int64_t round = 0;
+      for (;;) {
+         bool any = false;
+         if (graphs) {
+            for (fa_batch* q : set)
                if (!q->finished) {
-                  CHK(run_round(q));
+                  HIPCHK(hipGraphLaunch(q->gexec, q->stream));
                   any = true;
                }
-      }
-      if (!any) break;
-      for (fa_batch* q : leaves)
-         if (!q->finished) CHK(run_poll(q));
-      bool all = true;
-      for (fa_batch* q : leaves)
-         if (!q->finished) {
-            HIPCHK(hipStreamSynchronize(q->stream));
-            q->finished = q->h_counters->n_active_frames == 0;
-            all = all && q->finished;
+            round += K;
+         } else {
+            for (int r = 0; r < K && round < limit; r++, round++)
+               for (fa_batch* q : set)
+                  if (!q->finished) {
+                     CHK(run_round(q));
+                     any = true;
+                  }
          }
-      if (all || round >= limit) break;
+         if (!any) break;
+         for (fa_batch* q : set)
+            if (!q->finished) CHK(run_poll(q));
+         bool all = true;
+         for (fa_batch* q : set)
+            if (!q->finished) {
+               HIPCHK(hipStreamSynchronize(q->stream));
+               q->finished = q->h_counters->n_active_frames == 0;
+               all = all && q->finished;
+            }
+         if (all || round >= limit) break;
+      }
+      return FA_OK;
+   };
+   if (b->timing) {
+      // timed runs take the parts one after the other, so each pass's events bracket that
+      // pass alone rather than time it shares with the other part's passes
+      for (fa_batch* q : leaves) CHK(run_set(std::vector<fa_batch*>{q}));
+   } else {
+      CHK(run_set(leaves));
    }
    for (size_t i = 0; i < leaves.size(); i++) CHK(run_finish(leaves[i], x_inout + xoff[i], stats, first[i]));
    if (timing) {

@@ -183,6 +183,9 @@ int fa_align(fa_frame* frame, const fa_tr_params* prm, double* x_inout, fa_align
 int fa_batch_create(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
                     fa_batch** out);
 int fa_batch_destroy(fa_batch* b);
+/* enable = 1: the following runs launch eagerly and time every fused pass with HIP events
+   (fa_batch_timing); a batch large enough to run as two interleaved parts then runs its parts
+   one after the other, so each pass is timed alone. Results are the same either way. */
 int fa_batch_set_timing(fa_batch* b, int32_t enable);
 int fa_batch_get_tuning(const fa_batch* b, fa_tuning* t);
 int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t);
