@@ -250,7 +250,7 @@ namespace {
 // global memory instead).
 int64_t lds_floats_for(const fa_reference* r)
 {
-   const int64_t stride = ((r->nx + 2 * kPad) + 3) & ~3;
+   const int64_t stride = ((r->item_w + 2 * kPad) + 3) & ~3;
    const int64_t rows = r->item_rows + 2;
    return rows * stride * (r->n_dim == 3 ? 2 : 1);
 }
@@ -1231,9 +1231,15 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    r->nD = nD;
    r->n_dim = nz > 1 ? 3 : 2;
    r->n_px = n_px;
-   // default work item: 8 rows within 2048..8192 pixels (2-D); 2048 pixels, or one row of a
-   // wider frame (3-D, which stages two planes per item; measured best for 256..1024 wide)
+   // Frames at least 768 wide whose width is a multiple of kStripWidth are cut into strips of
+   // kStripWidth columns: a work item then stages a narrow tile of many rows instead of a few
+   // whole rows (less staging per pixel, more pixels per item for the same LDS).
+   const bool strips = nx >= 3 * kStripWidth && nx % kStripWidth == 0;
+   r->item_w = strips ? kStripWidth : nx;
+   // default work item: strips 4096 pixels (16 rows); else 8 rows within 2048..8192 pixels
+   // (2-D), 2048 pixels or one row of a wider frame (3-D, which stages two planes per item)
    r->chunk_px = chunk_px > 0 ? chunk_px
+                 : strips     ? 16 * kStripWidth
                  : nz > 1     ? std::max(2048, nx)
                               : (int)std::min<int64_t>(8192, std::max<int64_t>(2048, 8 * (int64_t)nx));
    derive_scan(*scan, nz, ny, nx, nD, r->scan_host, &r->scan);
@@ -1308,14 +1314,41 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
       int64_t b0 = r->D_range[2 * i], e0 = r->D_range[2 * i + 1];
       for (int64_t s0 = b0; s0 <= e0;) {
          const int64_t s1 = std::min<int64_t>(e0, (s0 / plane + 1) * plane - 1);  // segment end
-         const int64_t len = s1 - s0 + 1, m = (len + r->chunk_px - 1) / r->chunk_px;
-         const int64_t base = len / m, extra = len % m;
-         int64_t p = s0;
-         for (int64_t t = 0; t < m; t++) {
-            const int64_t q = p + base + (t < extra ? 1 : 0) - 1;
-            r->chunks.push_back(Chunk{(int32_t)p, (int32_t)(q + 1), i, (int32_t)e0});
-            r->item_rows = std::max<int>(r->item_rows, (int)(q / nx - p / nx + 1));
-            p = q + 1;
+         if (!strips) {
+            const int64_t len = s1 - s0 + 1, m = (len + r->chunk_px - 1) / r->chunk_px;
+            const int64_t base = len / m, extra = len % m;
+            int64_t p = s0;
+            for (int64_t t = 0; t < m; t++) {
+               const int64_t q = p + base + (t < extra ? 1 : 0) - 1;
+               r->chunks.push_back(Chunk{(int32_t)p, (int32_t)(q + 1), i, (int32_t)e0, 0, nx});
+               r->item_rows = std::max<int>(r->item_rows, (int)(q / nx - p / nx + 1));
+               p = q + 1;
+            }
+         } else {
+            // the segment's rows in the fewest equal groups of at most chunk_px / kStripWidth
+            // rows, each group cut into strips; an item runs from the strip's first pixel in
+            // the segment to its last
+            const int64_t ra = s0 / nx, rb = s1 / nx, nrow = rb - ra + 1;
+            const int64_t rmax = std::max<int64_t>(1, r->chunk_px / kStripWidth);
+            const int64_t m = (nrow + rmax - 1) / rmax, base = nrow / m, extra = nrow % m;
+            int64_t ga = ra;
+            for (int64_t t = 0; t < m; t++) {
+               const int64_t gb = ga + base + (t < extra ? 1 : 0) - 1;
+               for (int cx = 0; cx < nx; cx += kStripWidth) {
+                  // first pixel: in row ga unless the segment starts past the strip there
+                  int64_t p0 = ga * nx + cx;
+                  if (p0 + kStripWidth - 1 < s0) p0 = (ga + 1) * nx + cx;
+                  p0 = std::max(p0, s0);
+                  // last pixel: in row gb unless the segment ends before the strip there
+                  int64_t pl = gb * nx + cx + kStripWidth - 1;
+                  if (gb * nx + cx > s1) pl = (gb - 1) * nx + cx + kStripWidth - 1;
+                  pl = std::min(pl, s1);
+                  if (pl < p0) continue;
+                  r->chunks.push_back(Chunk{(int32_t)p0, (int32_t)(pl + 1), i, (int32_t)e0, cx, kStripWidth});
+                  r->item_rows = std::max<int>(r->item_rows, (int)(pl / nx - p0 / nx + 1));
+               }
+               ga = gb + 1;
+            }
          }
          s0 = s1 + 1;
       }

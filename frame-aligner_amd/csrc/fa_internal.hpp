@@ -25,9 +25,16 @@ struct ScanDev {
 // plane. end_p is the last pixel of interval k, which the reference leaves out of the
 // steepest-descent images, JtJ and Jtr (its consumers iterate [begin, end),
 // FrameWarper.cpp:309-344) but keeps in the SSD.
+//
+// A work item is either a run of whole-width rows (x0 = 0, w = nx: the pixels p0 .. p1-1) or,
+// in frames whose width is a multiple of kStripWidth, a strip of kStripWidth columns
+// [x0, x0 + w) over a run of rows: the pixels of those rows and columns between p0 (its
+// first) and p1 - 1 (its last) in scan order.
 struct Chunk {
-   int32_t p0, p1, k, end_p;
+   int32_t p0, p1, k, end_p, x0, w;
 };
+// Column width of strip work items (one column per pass thread).
+constexpr int kStripWidth = 256;
 
 // Doubles written per work item by the fused pass: SSD, Jtr of node k (n_dim values,
 // weight 1-f), Jtr of node k+1 (n_dim values, weight f); padded to 8.
@@ -160,6 +167,7 @@ struct fa_reference {
    fa_context* ctx = nullptr;
    int nz = 0, ny = 0, nx = 0, nD = 0, n_dim = 0, chunk_px = 0;
    int item_rows = 1;  // most frame rows one work item spans
+   int item_w = 0;     // columns of a work item: nx, or kStripWidth for strip items
    int64_t n_px = 0;
    fa::ScanDev scan{};
    double scan_host[6] = {0};

@@ -224,8 +224,13 @@ __device__ void hess_chunk(const RefDev& R, const Chunk ch, double* out, KF (*sr
    KF acc[NV];
 #pragma unroll
    for (int v = 0; v < NV; v++) acc[v] = KF{0.f, 0.f};
+   const int64_t plane_sz = (int64_t)R.nx * R.ny;
    for (int p = ch.p0 + threadIdx.x; p < ch.p1; p += kBlock) {
       if (p == ch.end_p) continue;
+      if (ch.w != R.nx) {  // strip item: only its columns
+         const int x = (int)((int64_t)p % plane_sz % R.nx);
+         if (x < ch.x0 || x >= ch.x0 + ch.w) continue;
+      }
       const float4 rec = R.rg4[p];
       const double fd = (double)(M == 3 ? R.df[p] : rec.w);
       const float g[3] = {rec.y, rec.z, rec.w};
@@ -955,8 +960,10 @@ __device__ __forceinline__ float4 cvt4(float4 w, float off)
    return make_float4(w.x + off, w.y + off, w.z + off, w.w + off);
 }
 
+// The tile's columns: frame columns [cx - kPad, cx + cw + kPad) at tile columns 0, 1, ...;
+// cw = nx (cx = 0) for whole-width items, whose padding columns lie outside the frame.
 template <int NB, typename FT>
-__device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, float* lds)
+__device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, float* lds, int cx, int cw)
 {
    const int nx = R.nx, ny = R.ny, nz = R.nz;
    const int rows = S.nzs * S.nrows;
@@ -964,33 +971,41 @@ __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, 
    const FT* __restrict__ src = (const FT*)F.data;
    const float off = F.offset;
    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-   if ((nx & 3) == 0) {
+   if ((nx & 3) == 0 && cw == nx) {
       constexpr int P4 = kPad / 4;
       for (int i = tid; i < rows * 2 * P4; i += kBlock) {
          const int r = i / (2 * P4), j = i - r * (2 * P4);
          const int col = j < P4 ? 4 * j : kPad + nx + 4 * (j - P4);
          *(float4*)&lds[r * S.stride + col] = zero4;
       }
-      const int nx4 = nx >> 2;
+   }
+   if ((nx & 3) == 0) {
+      // column groups of four samples: the frame's own (whole width; padding zeroed above)
+      // or the strip's with its padding, which holds frame samples where the frame has them
+      const int ncol4 = cw == nx ? nx >> 2 : (cw + 2 * kPad) >> 2;
+      const int x_org = cw == nx ? 0 : cx - kPad;  // frame column of group 0
+      const int t_org = cw == nx ? kPad : 0;       // tile column of group 0
       using RT = typename Raw4<FT>::type;
-      // A thread keeps one column group c (four samples) and takes every rpi-th staged row,
-      // rpi = rows per sweep of the workgroup; rows wider than the workgroup take several
-      // column groups per thread. Row r is plane zi, row yr of the tile, tracked
-      // incrementally; indices into the frame are 32-bit (frames hold fewer than 2^31
-      // samples).
-      const int rpi = nx4 <= kBlock ? kBlock / nx4 : 1;
-      const int tr = nx4 <= kBlock ? tid / nx4 : 0;
-      const int cb = nx4 <= kBlock ? tid - tr * nx4 : tid;
+      // A thread keeps one column group c and takes every rpi-th staged row, rpi = rows per
+      // sweep of the workgroup; rows wider than the workgroup take several column groups per
+      // thread. Row r is plane zi, row yr of the tile, tracked incrementally; indices into the
+      // frame are 32-bit (frames hold fewer than 2^31 samples).
+      const int rpi = ncol4 <= kBlock ? kBlock / ncol4 : 1;
+      const int tr = ncol4 <= kBlock ? tid / ncol4 : 0;
+      const int cb = ncol4 <= kBlock ? tid - tr * ncol4 : tid;
       if (tr < rpi) {
-         for (int c = cb; c < nx4; c += kBlock) {
+         for (int c = cb; c < ncol4; c += kBlock) {
+            const int xg = x_org + 4 * c;
+            const bool col_ok = (unsigned)xg < (unsigned)nx;
+            const unsigned xs = col_ok ? (unsigned)xg : 0u;
             int zi = 0, yr = tr;
             while (yr >= S.nrows) {
                yr -= S.nrows;
                zi++;
             }
             for (int r0 = tr; r0 < rows; r0 += NB * rpi) {
-               // every lane loads (rows outside the frame from a clamped, in-bounds row) so
-               // that the loads are not under branches and all NB are in flight together
+               // every lane loads (rows and columns outside the frame from a clamped, in-bounds
+               // address) so that the loads are not under branches and all NB are in flight
                RT raw[NB];
                int dst[NB];
                bool ok[NB];
@@ -998,10 +1013,10 @@ __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, 
                for (int u = 0; u < NB; u++) {
                   const int yy = S.ya + yr, zz = S.za + zi;
                   const bool live = r0 + u * rpi < rows;
-                  dst[u] = live ? (r0 + u * rpi) * S.stride + kPad + 4 * c : -1;
-                  ok[u] = live && (unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz;
+                  dst[u] = live ? (r0 + u * rpi) * S.stride + t_org + 4 * c : -1;
+                  ok[u] = live && col_ok && (unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz;
                   const unsigned yc = min(max(yy, 0), ny - 1), zc = min(max(zz, 0), nz - 1);
-                  raw[u] = *(const RT*)(src + ((zc * (unsigned)ny + yc) * (unsigned)nx + 4u * c));
+                  raw[u] = *(const RT*)(src + ((zc * (unsigned)ny + yc) * (unsigned)nx + xs));
                   yr += rpi;
                   while (yr >= S.nrows) {
                      yr -= S.nrows;
@@ -1100,11 +1115,12 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
             S.za = plane + (int)floor(lo[2]);
             S.nzs = plane + (int)floor(hi[2]) + 1 - S.za + 1;
          }
-         S.stride = ((nx + 2 * kPad) + 3) & ~3;
+         S.stride = ((ch.w + 2 * kPad) + 3) & ~3;
          use = (int64_t)S.nzs * S.nrows * S.stride <= lds_cap;
       }
    }
-   if (use) stage_frame<M == 3 ? 8 : 6, FT>(R, F, S, lds);
+   const int cx = ch.x0;  // first column of the item (0 for whole-width items)
+   if (use) stage_frame<M == 3 ? 8 : 6, FT>(R, F, S, lds, cx, ch.w);
    const int lds_lim = S.nzs * S.nrows * S.stride - (M == 3 ? S.nrows * S.stride : 0) - S.stride - 2;
 
    const FT* __restrict__ src = (const FT*)F.data;
@@ -1118,11 +1134,19 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    // records before using any, so several loads are in flight per wave. Pixels past the end
    // of the work item are computed on clamped data and masked, so the loop body has no
    // divergent branches.
+   //
+   // The walk runs over the item's own width vw: the frame width for whole-width items (a
+   // thread's pixels are kBlock apart in scan order), kBlock for strips (a thread keeps its
+   // column and its pixels are one row apart). w.x is the column within the item.
    constexpr int kUnroll = 4;
-   const int sdy = kBlock / nx, sdx = kBlock - sdy * nx;
+   const bool whole = ch.w == nx;
+   const int vw = whole ? nx : kBlock;
+   const int sdy = kBlock / vw, sdx = kBlock - sdy * vw;
+   const int pstep = whole ? kBlock : nx;
    PixelWalk w;
-   w.init(q0 + threadIdx.x, nx);
-   for (int pb = ch.p0 + threadIdx.x; pb < ch.p1; pb += kUnroll * kBlock) {
+   w.init((whole ? q0 : (q0 / nx) * kBlock + (q0 % nx - cx)) + threadIdx.x, vw);
+   const int pfirst = plane * (int)plane_sz + w.y * nx + w.x + cx;
+   for (int pb = pfirst; pb < ch.p1; pb += kUnroll * pstep) {
       float4 recs[kUnroll];
       float fs[kUnroll];
       // plain FP32 sums over the kUnroll pixels of this step, folded into the compensated
@@ -1132,17 +1156,17 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
       for (int v = 0; v < NV; v++) part[v] = 0.f;
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
-         const int pu = min(pb + u * kBlock, ch.p1 - 1);
+         const int pu = min(pb + u * pstep, ch.p1 - 1);
          recs[u] = R.rg4[pu];
          fs[u] = (M == 3) ? R.df[pu] : recs[u].w;
       }
       // the wave's first pixel of this step: once a slot starts past the item's end, this and
       // the later slots of the step hold no pixel of the wave (the last step of an item)
-      const int wave_first = __builtin_amdgcn_readfirstlane(pb - (int)(threadIdx.x & 63));
+      const int wave_first = __builtin_amdgcn_readfirstlane(pb);
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++, w.step(nx, sdx, sdy)) {
-         if (wave_first + u * kBlock >= ch.p1) break;
-         const int p = pb + u * kBlock;
+      for (int u = 0; u < kUnroll; u++, w.step(vw, sdx, sdy)) {
+         if (wave_first + u * pstep >= ch.p1) break;
+         const int p = pb + u * pstep;
          const bool in = p < ch.p1;
          const float4 rec = recs[u];
          const float f = fs[u];
@@ -1155,14 +1179,14 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
             ti[c] = (int)fl;
             wt[c] = d - fl;
          }
-         const int x0 = w.x + ti[0], y0 = w.y + ti[1];
+         const int x0 = w.x + cx + ti[0], y0 = w.y + ti[1];  // frame coordinates
          const float wx1 = wt[0], wx0 = 1.f - wt[0];
          const float wy1 = wt[1], wy0 = 1.f - wt[1];
          float v = 0.f;
          if (use) {
             // 24-bit products (full rate): in-tile offsets are far below 2^23; masked pixels
             // past the item may give anything, which the clamp keeps inside the tile
-            int idx = __mul24(y0 - S.ya, S.stride) + x0 + kPad;
+            int idx = __mul24(y0 - S.ya, S.stride) + x0 - cx + kPad;
             if (M == 3) idx += __mul24(plane + ti[2] - S.za, S.nrows * S.stride);
             idx = min(max(idx, 0), lds_lim);
             const float* L = lds + idx;
@@ -1336,6 +1360,7 @@ __device__ void splat_chunk(const RefDev& R, const FrameDev& F, const Chunk ch, 
    PixelWalk w;
    w.init(q0 + threadIdx.x, nx);
    for (int p = ch.p0 + threadIdx.x; p < ch.p1; p += kBlock, w.step(nx)) {
+      if (w.x < ch.x0 || w.x >= ch.x0 + ch.w) continue;  // strip item: only its columns
       const double fd = (double)(M == 3 ? R.df[p] : R.rg4[p].w);
       double l[3];
       l[0] = (double)w.x - (fd * D[3 * (k + 1) + 0] + (1.0 - fd) * D[3 * k + 0]);

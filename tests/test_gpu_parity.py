@@ -29,6 +29,19 @@ def ctx():
 CASES = [c[:4] for c in PARITY_CASES]
 GEN = {c[0]: c[4] for c in PARITY_CASES + MANY_NODE_CASES}
 
+# Widths that are not a multiple of 4 stage their rows sample by sample (the other widths in
+# groups of four); 3-D with an odd width. Frames 768 or more wide (a multiple of 256) are cut
+# into strip work items of 256 columns.
+ODD_WIDTH_CASES = [
+    ("2d_width70", (1, 64, 70), 9, dict(line=1.0, interline=1.0), {}),
+    ("3d_width37", (4, 40, 37), 4 * 4 + 1, dict(line=1.0, interline=1.0), dict(n_blobs=90, sigma=(1.5, 4.0))),
+]
+STRIP_CASES = [
+    ("2d_strips", (1, 48, 768), 9, dict(line=1.0, interline=1.0), dict(n_blobs=220)),
+    ("3d_strips", (3, 20, 768), 3 * 4 + 1, dict(line=1.0, interline=1.0), dict(n_blobs=260, sigma=(1.5, 4.0))),
+]
+GEN.update({c[0]: c[4] for c in STRIP_CASES})
+
 
 def _pair(ctx, shape, nD, scan, seed=3, name=None, **kw):
     kw = {**GEN.get(name, {}), **kw}
@@ -57,7 +70,7 @@ def test_sobel_bit_exact(ctx, name, shape, nD, scan):
         assert np.array_equal(go[c].astype(np.float32), gg[c]), f"component {c}"
 
 
-@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+@pytest.mark.parametrize("name,shape,nD,scan", CASES + [c[:4] for c in STRIP_CASES])
 def test_hessian(ctx, name, shape, nD, scan):
     st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
     Ho, Hg = orc.hessian(), ref.H
@@ -97,15 +110,9 @@ def test_error_and_jacobian(ctx, name, shape, nD, scan, dtype):
         fr.release()
 
 
-# Widths that are not a multiple of 4 stage their rows sample by sample (the other widths in
-# groups of four); 3-D with an odd width.
-ODD_WIDTH_CASES = [
-    ("2d_width70", (1, 64, 70), 9, dict(line=1.0, interline=1.0), {}),
-    ("3d_width37", (4, 40, 37), 4 * 4 + 1, dict(line=1.0, interline=1.0), dict(n_blobs=90, sigma=(1.5, 4.0))),
-]
 
 
-@pytest.mark.parametrize("name,shape,nD,scan,gen", ODD_WIDTH_CASES)
+@pytest.mark.parametrize("name,shape,nD,scan,gen", ODD_WIDTH_CASES + STRIP_CASES)
 @pytest.mark.parametrize("dtype", ["float32", "uint16"])
 def test_error_and_jacobian_odd_width(ctx, name, shape, nD, scan, gen, dtype):
     st, orc, ref = _pair(ctx, shape, nD, scan, dtype=dtype, **gen)
@@ -124,7 +131,8 @@ def test_error_and_jacobian_odd_width(ctx, name, shape, nD, scan, gen, dtype):
         fr.release()
 
 
-@pytest.mark.parametrize("name,shape,nD,scan,gen", [c[:4] + (GEN.get(c[0], {}),) for c in CASES] + ODD_WIDTH_CASES)
+@pytest.mark.parametrize("name,shape,nD,scan,gen",
+                         [c[:4] + (GEN.get(c[0], {}),) for c in CASES] + ODD_WIDTH_CASES + STRIP_CASES)
 def test_staged_and_global_sampling_agree(ctx, name, shape, nD, scan, gen):
     """Work items sample the frame from their LDS tile or, when it does not fit, from global
     memory with the same arithmetic: a batch run with staging switched off gives the same bits."""
@@ -141,7 +149,7 @@ def test_staged_and_global_sampling_agree(ctx, name, shape, nD, scan, gen):
     batch.close()
 
 
-@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+@pytest.mark.parametrize("name,shape,nD,scan", CASES + [c[:4] for c in STRIP_CASES])
 def test_warp_interpolated(ctx, name, shape, nD, scan):
     st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
     fr = ref.register(st["frames_w"][0])
@@ -167,7 +175,7 @@ def test_zero_and_integer_shift_exact(ctx):
     assert np.array_equal(w, expect)
 
 
-@pytest.mark.parametrize("name,shape,nD,scan", CASES)
+@pytest.mark.parametrize("name,shape,nD,scan", CASES + [c[:4] for c in STRIP_CASES])
 def test_align_matches_oracle(ctx, name, shape, nD, scan):
     st, orc, ref = _pair(ctx, shape, nD, scan, name=name)
     kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
