@@ -315,17 +315,25 @@ void order_refs(std::vector<fa_reference*>& rs)
    });
 }
 
+// Fused-pass workgroups whose LDS tiles fit one compute unit's LDS at once (at most 8).
+int64_t resident_blocks(const fa_context* ctx, int64_t lds_floats)
+{
+   return std::min<int64_t>(8, (int64_t)ctx->lds_per_block / (lds_floats * (int64_t)sizeof(float) + 1024));
+}
+
 // Launch classes over the ordered references rs. Pass: a class ends at a change of kind or
-// before a reference that stages more than 1.5x the LDS of the class's first. Update: the
-// references' factor sizes, a class starting above 2x its first. ref_items: each reference's
-// work items with every frame of the batch active.
+// before a reference whose tile lets fewer workgroups share a compute unit than the class's
+// first (the class launches with its largest tile, so a larger one would cost the others
+// occupancy). Update: the references' factor sizes, a class starting above 2x its first.
+// ref_items: each reference's work items with every frame of the batch active.
 void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::vector<int64_t>& ref_items,
                   int n_frames)
 {
    b->pcls.clear();
    for (int i = 0; i < (int)rs.size(); i++) {
       const int64_t need = lds_floats_for(rs[i]);
-      if (b->pcls.empty() || b->pcls.back().n_dim != rs[i]->n_dim || 2 * need > 3 * b->pcls.back().lds_first) {
+      if (b->pcls.empty() || b->pcls.back().n_dim != rs[i]->n_dim ||
+          resident_blocks(b->ctx, need) < resident_blocks(b->ctx, b->pcls.back().lds_first)) {
          fa_batch::PassClass c;
          c.n_dim = rs[i]->n_dim;
          c.ref_lo = i;
