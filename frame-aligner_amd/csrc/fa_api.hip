@@ -860,6 +860,7 @@ struct fa_server {
    TrState* h_state = nullptr;
    double* h_xall = nullptr;
    int32_t* d_slots = nullptr;
+   bool launched = false;  // rounds launched by the last step, not yet harvested
 
    static size_t round_up(size_t n) { return (n + 255) & ~(size_t)255; }
 
@@ -900,6 +901,7 @@ struct fa_server {
    ~fa_server()
    {
       if (cstream) (void)hipStreamSynchronize(cstream);
+      if (b && b->stream) (void)hipStreamSynchronize(b->stream);
       delete b;
       for (auto e : up_ev) (void)hipEventDestroy(e);
       if (cstream) (void)hipStreamDestroy(cstream);
@@ -1024,8 +1026,41 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
    return FA_OK;
 }
 
+// Waits for the rounds the previous step launched and returns its finished slots' results.
+int server_harvest(fa_server* sv)
+{
+   if (!sv->launched) return FA_OK;
+   fa_batch* b = sv->b;
+   HIPCHK(hipStreamSynchronize(b->stream));
+   sv->launched = false;
+   for (int j = 0; j < sv->S; j++) {
+      if (sv->sstate[j] != fa_server::kRunning || !sv->h_state[j].done) continue;
+      const fa_server::Req& r = sv->slot_req[j];
+      const fa_reference* ref = r.f ? r.f->ref : r.ref;
+      const int n = ref->nD * ref->n_dim;
+      std::memcpy(r.x, sv->h_xall + b->hframes[j].x_base, n * sizeof(double));
+      fa_server_result res{};
+      res.tag = r.tag;
+      const TrState& t = sv->h_state[j];
+      res.stats.iterations = t.iterations;
+      res.stats.evaluations = t.evaluations;
+      res.stats.rejected = t.rejected;
+      res.stats.status = t.status;
+      res.stats.f_final = t.f;
+      res.stats.radius_final = t.radius;
+      sv->results.push_back(res);
+      if (sv->blk_size[j]) sv->arena_free(sv->blk_off[j], sv->blk_size[j]);
+      sv->blk_size[j] = 0;
+      sv->sstate[j] = fa_server::kFree;
+      sv->free_slots.push_back(j);
+   }
+   return FA_OK;
+}
+
 int server_step(fa_server* sv, int rounds)
 {
+   // 0. results of the rounds the previous step launched
+   CHK(server_harvest(sv));
    fa_batch* b = sv->b;
    hipStream_t s = b->stream;
    const BatchDev B = b->dev();
@@ -1106,31 +1141,9 @@ int server_step(fa_server* sv, int rounds)
    CHK(run_poll(b));
    HIPCHK(hipMemcpyAsync(sv->h_state, b->d_state, sv->S * sizeof(TrState), hipMemcpyDeviceToHost, s));
    HIPCHK(hipMemcpyAsync(sv->h_xall, b->d_x, b->total_x * sizeof(double), hipMemcpyDeviceToHost, s));
-   HIPCHK(hipStreamSynchronize(s));
-   for (int j = 0; j < sv->S; j++) {
-      if (sv->sstate[j] != fa_server::kRunning || !sv->h_state[j].done) continue;
-      const fa_server::Req& r = sv->slot_req[j];
-      const fa_reference* ref = r.f ? r.f->ref : r.ref;
-      const int n = ref->nD * ref->n_dim;
-      std::memcpy(r.x, sv->h_xall + b->hframes[j].x_base, n * sizeof(double));
-      fa_server_result res{};
-      res.tag = r.tag;
-      const TrState& t = sv->h_state[j];
-      res.stats.iterations = t.iterations;
-      res.stats.evaluations = t.evaluations;
-      res.stats.rejected = t.rejected;
-      res.stats.status = t.status;
-      res.stats.f_final = t.f;
-      res.stats.radius_final = t.radius;
-      sv->results.push_back(res);
-      if (sv->blk_size[j]) sv->arena_free(sv->blk_off[j], sv->blk_size[j]);
-      sv->blk_size[j] = 0;
-      sv->sstate[j] = fa_server::kFree;
-      sv->free_slots.push_back(j);
-   }
+   sv->launched = true;
    return FA_OK;
 }
-
 }  // namespace
 
 namespace fa {

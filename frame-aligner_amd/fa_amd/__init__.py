@@ -553,8 +553,9 @@ class Server:
         return u.value, t.value
 
     def step(self, rounds=1):
-        """Admits queued frames, runs rounds trust-region rounds and collects what finished:
-        returns (in_flight, queued, [(tag, x, stats), ...])."""
+        """Collects what the previous step's rounds finished, admits queued frames and launches
+        rounds trust-region rounds without waiting for them: returns (in_flight, queued,
+        [(tag, x, stats), ...])."""
         a, q = C.c_int32(), C.c_int32()
         _check(self.L.fa_server_step(self.h, int(rounds), C.byref(a), C.byref(q)))
         out = []

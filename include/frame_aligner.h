@@ -200,12 +200,15 @@ int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
 /* Serving (continuous batching): a fixed set of slots, each sized for the largest of the
    references the server serves, and a device frame arena of arena_bytes (0: slots times the
    largest float frame). Submitted frames wait in a queue, in arrival order. Every
-   fa_server_step takes queued frames while a slot is free and (for host frames) the arena
-   has room for the frame (admission by free memory), uploads host frames on a copy stream
-   that overlaps the running rounds, starts the frames whose data is on the device, runs
-   `rounds` trust-region rounds over every running slot (frames admitted at different times
-   share the launches), and harvests the finished slots: x_inout (caller memory, valid until
-   the result is returned) receives the result and fa_server_results lists the finished tags.
+   fa_server_step first waits for the rounds the previous step launched and harvests the
+   slots they finished: x_inout (caller memory, valid until the result is returned) receives
+   the result and fa_server_results lists the finished tags. It then takes queued frames
+   while a slot is free and (for host frames) the arena has room for the frame (admission by
+   free memory), uploads host frames on a copy stream that overlaps the running rounds,
+   starts the frames whose data is on the device, and launches `rounds` trust-region rounds
+   over every running slot (frames admitted at different times share the launches) without
+   waiting for them, so the device computes while the caller submits and reads results; a
+   frame's result is returned by the step after the one whose rounds finished it.
    Each result is the same as fa_align's for that frame. Host frames should be pinned
    (fa_host_alloc) for the upload to be asynchronous, and stay untouched until their result is
    returned; registered frames stay registered until then. Use one thread per server.
