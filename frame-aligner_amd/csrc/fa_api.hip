@@ -547,6 +547,7 @@ int event_pair(fa_batch* b, size_t i, hipEvent_t* s, hipEvent_t* e)
 }
 
 constexpr size_t kMaxTimedRounds = 4096;
+constexpr int kServerSplitSlots = 16;  // servers with at least this many slots run two slot batches
 constexpr int kSplitFrames = 64;  // batches of at least this many frames run as two parts
 
 int64_t max_rounds(const fa_batch* b) { return b->prm.max_evals > 0 ? (int64_t)b->prm.max_evals + 2 : 100000; }
@@ -839,9 +840,15 @@ struct fa_server {
    };
    enum : char { kFree = 0, kUploading = 1, kRunning = 2 };
    fa_context* ctx = nullptr;
-   fa_batch* b = nullptr;  // slot batch: n_slots frames' worth of state, sized for the largest reference
+   // slot batches (one, or two on their own streams whose rounds interleave): each holds its
+   // slots' worth of state, sized for the largest reference; global slot j is local slot
+   // j - first[p] of part p
+   std::vector<fa_batch*> parts;
+   std::vector<int> first;
+   std::vector<int> d_slots_off;  // each part's region of d_slots / h_slots
    std::vector<fa_reference*> refs;
    int S = 0, nmax = 0;
+   int part_of(int j) const { return (int)parts.size() > 1 && j >= first[1] ? 1 : 0; }
    std::deque<Req> pending;
    std::vector<int> free_slots;
    std::vector<Req> slot_req;
@@ -901,8 +908,10 @@ struct fa_server {
    ~fa_server()
    {
       if (cstream) (void)hipStreamSynchronize(cstream);
-      if (b && b->stream) (void)hipStreamSynchronize(b->stream);
-      delete b;
+      for (fa_batch* q : parts) {
+         if (q->stream) (void)hipStreamSynchronize(q->stream);
+         delete q;
+      }
       for (auto e : up_ev) (void)hipEventDestroy(e);
       if (cstream) (void)hipStreamDestroy(cstream);
       dfree(d_arena);
@@ -1030,15 +1039,14 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
 int server_harvest(fa_server* sv)
 {
    if (!sv->launched) return FA_OK;
-   fa_batch* b = sv->b;
-   HIPCHK(hipStreamSynchronize(b->stream));
+   for (fa_batch* q : sv->parts) HIPCHK(hipStreamSynchronize(q->stream));
    sv->launched = false;
    for (int j = 0; j < sv->S; j++) {
       if (sv->sstate[j] != fa_server::kRunning || !sv->h_state[j].done) continue;
       const fa_server::Req& r = sv->slot_req[j];
       const fa_reference* ref = r.f ? r.f->ref : r.ref;
       const int n = ref->nD * ref->n_dim;
-      std::memcpy(r.x, sv->h_xall + b->hframes[j].x_base, n * sizeof(double));
+      std::memcpy(r.x, sv->h_xall + (size_t)j * sv->nmax, n * sizeof(double));
       fa_server_result res{};
       res.tag = r.tag;
       const TrState& t = sv->h_state[j];
@@ -1061,9 +1069,6 @@ int server_step(fa_server* sv, int rounds)
 {
    // 0. results of the rounds the previous step launched
    CHK(server_harvest(sv));
-   fa_batch* b = sv->b;
-   hipStream_t s = b->stream;
-   const BatchDev B = b->dev();
    // 1. take queued requests, in arrival order, while a slot and (for host data) arena memory
    //    are free; host data is uploaded on the copy stream while the running frames compute
    bool uploading = false;
@@ -1091,56 +1096,73 @@ int server_step(fa_server* sv, int rounds)
    bool running = false;
    for (int j = 0; j < sv->S; j++) running = running || sv->sstate[j] == fa_server::kRunning;
    if (uploading && !running) HIPCHK(hipStreamSynchronize(sv->cstream));  // nothing to overlap with
-   // 2. start the frames whose data is on the device
-   int na = 0;
-   for (int slot = 0; slot < sv->S; slot++) {
-      if (sv->sstate[slot] != fa_server::kUploading) continue;
-      const hipError_t q = hipEventQuery(sv->up_ev[slot]);
-      if (q == hipErrorNotReady) continue;
-      HIPCHK(q);
-      const fa_server::Req& r = sv->slot_req[slot];
-      fa_reference* ref = r.f ? r.f->ref : r.ref;
-      BatchFrame bf = b->hframes[slot];
-      if (r.f) {
-         bf.f.data = r.f->d_data;
-         bf.f.dtype = r.f->dtype;
-         bf.f.offset = r.f->offset;
-      } else {
-         bf.f.data = sv->d_arena + sv->blk_off[slot];
-         bf.f.dtype = r.dtype;
-         bf.f.offset = r.offset;
+   // 2. start the frames whose data is on the device, part by part
+   const int P = (int)sv->parts.size();
+   for (int pi = 0; pi < P; pi++) {
+      fa_batch* b = sv->parts[pi];
+      const int j0 = sv->first[pi], j1 = pi + 1 < P ? sv->first[pi + 1] : sv->S;
+      const int base = sv->d_slots_off[pi];
+      int na = 0;
+      for (int slot = j0; slot < j1; slot++) {
+         if (sv->sstate[slot] != fa_server::kUploading) continue;
+         const hipError_t q = hipEventQuery(sv->up_ev[slot]);
+         if (q == hipErrorNotReady) continue;
+         HIPCHK(q);
+         const int l = slot - j0;
+         const fa_server::Req& r = sv->slot_req[slot];
+         fa_reference* ref = r.f ? r.f->ref : r.ref;
+         BatchFrame bf = b->hframes[l];
+         if (r.f) {
+            bf.f.data = r.f->d_data;
+            bf.f.dtype = r.f->dtype;
+            bf.f.offset = r.f->offset;
+         } else {
+            bf.f.data = sv->d_arena + sv->blk_off[slot];
+            bf.f.dtype = r.dtype;
+            bf.f.offset = r.offset;
+         }
+         bf.ref = (int)(std::find(sv->refs.begin(), sv->refs.end(), ref) - sv->refs.begin());
+         bf.trc = b->ref_trc[bf.ref];
+         bf.n = ref->nD * ref->n_dim;
+         const int i = base + na;
+         sv->h_bf[i] = bf;
+         std::memcpy(sv->h_x + (size_t)i * sv->nmax, r.x, bf.n * sizeof(double));
+         sv->h_slots[i] = l;
+         sv->sstate[slot] = fa_server::kRunning;
+         na++;
       }
-      bf.ref = (int)(std::find(sv->refs.begin(), sv->refs.end(), ref) - sv->refs.begin());
-      bf.trc = b->ref_trc[bf.ref];
-      bf.n = ref->nD * ref->n_dim;
-      sv->h_bf[na] = bf;
-      std::memcpy(sv->h_x + (size_t)na * sv->nmax, r.x, bf.n * sizeof(double));
-      sv->h_slots[na] = slot;
-      sv->sstate[slot] = fa_server::kRunning;
-      na++;
+      hipStream_t s = b->stream;
+      for (int i = base; i < base + na; i++) {
+         const int l = sv->h_slots[i];
+         HIPCHK(hipMemcpyAsync(b->d_frames + l, &sv->h_bf[i], sizeof(BatchFrame), hipMemcpyHostToDevice, s));
+         HIPCHK(hipMemcpyAsync(b->d_x + b->hframes[l].x_base, sv->h_x + (size_t)i * sv->nmax,
+                               sv->h_bf[i].n * sizeof(double), hipMemcpyHostToDevice, s));
+      }
+      if (na) {
+         HIPCHK(hipMemcpyAsync(sv->d_slots + base, sv->h_slots + base, na * sizeof(int32_t), hipMemcpyHostToDevice, s));
+         HIPCHK(launch_tr_init_slots(b->dev(), sv->d_slots + base, na, s));
+      }
+      running = running || na > 0;
    }
-   for (int i = 0; i < na; i++) {
-      const int slot = sv->h_slots[i];
-      HIPCHK(hipMemcpyAsync(b->d_frames + slot, &sv->h_bf[i], sizeof(BatchFrame), hipMemcpyHostToDevice, s));
-      HIPCHK(hipMemcpyAsync(b->d_x + b->hframes[slot].x_base, sv->h_x + (size_t)i * sv->nmax,
-                            sv->h_bf[i].n * sizeof(double), hipMemcpyHostToDevice, s));
-   }
-   if (na) {
-      HIPCHK(hipMemcpyAsync(sv->d_slots, sv->h_slots, na * sizeof(int32_t), hipMemcpyHostToDevice, s));
-      HIPCHK(launch_tr_init_slots(B, sv->d_slots, na, s));
-   }
-   running = running || na > 0;
    if (!running) return FA_OK;
-   // 3. rounds over every running frame (one graph replay), then collect the finished ones
-   if (b->tune.use_graph) {
-      CHK(ensure_graph(b, rounds));
-      HIPCHK(hipGraphLaunch(b->gexec, s));
-   } else {
-      for (int r = 0; r < rounds; r++) CHK(run_round(b));
+   // 3. rounds over every running frame (one graph replay per part; the parts' streams
+   //    interleave, so one part's update overlaps the other's pass), then the states and
+   //    parameter vectors back to the host for the next step's harvest
+   for (int pi = 0; pi < P; pi++) {
+      fa_batch* b = sv->parts[pi];
+      hipStream_t s = b->stream;
+      const int j0 = sv->first[pi], nj = (pi + 1 < P ? sv->first[pi + 1] : sv->S) - j0;
+      if (b->tune.use_graph) {
+         CHK(ensure_graph(b, rounds));
+         HIPCHK(hipGraphLaunch(b->gexec, s));
+      } else {
+         for (int r = 0; r < rounds; r++) CHK(run_round(b));
+      }
+      CHK(run_poll(b));
+      HIPCHK(hipMemcpyAsync(sv->h_state + j0, b->d_state, nj * sizeof(TrState), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(sv->h_xall + (size_t)j0 * sv->nmax, b->d_x, b->total_x * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
    }
-   CHK(run_poll(b));
-   HIPCHK(hipMemcpyAsync(sv->h_state, b->d_state, sv->S * sizeof(TrState), hipMemcpyDeviceToHost, s));
-   HIPCHK(hipMemcpyAsync(sv->h_xall, b->d_x, b->total_x * sizeof(double), hipMemcpyDeviceToHost, s));
    sv->launched = true;
    return FA_OK;
 }
@@ -1824,7 +1846,19 @@ int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs,
    }
    sv->S = slots;
    sv->arena_bytes = arena_bytes ? fa_server::round_up(arena_bytes) : largest * (size_t)slots;
-   int rc = slot_batch_build(ctx, sv->refs, slots, prm, &sv->b);
+   // two slot batches when there are enough slots for both to fill their launches
+   const int P = slots >= kServerSplitSlots ? 2 : 1;
+   int rc = FA_OK;
+   for (int pi = 0, j = 0; pi < P && rc == FA_OK; pi++) {
+      const int sp = slots / P + (pi < slots % P ? 1 : 0);
+      fa_batch* q = nullptr;
+      rc = slot_batch_build(ctx, sv->refs, sp, prm, &q);
+      if (rc != FA_OK) break;
+      sv->parts.push_back(q);
+      sv->first.push_back(j);
+      sv->d_slots_off.push_back(j);
+      j += sp;
+   }
    if (rc == FA_OK)
       if ((rc = halloc(&sv->h_bf, slots)) || (rc = halloc(&sv->h_x, (size_t)slots * sv->nmax)) ||
           (rc = halloc(&sv->h_slots, slots)) || (rc = halloc(&sv->h_state, slots)) ||

@@ -33,10 +33,13 @@ def _setup(ctx):
     return (a, b, c), (ra, rb, rc), frames
 
 
-def test_server_same_alone_and_in_mix(ctx):
+@pytest.mark.parametrize("slots", [4, 16])
+def test_server_same_alone_and_in_mix(ctx, slots):
+    """4 slots: fewer than the frames, some wait in the queue; 16 slots: two slot batches on
+    their own streams, both holding frames."""
     stacks, refs, frames = _setup(ctx)
     alone = [f.align(**KW) for f in frames]
-    srv = F.Server(refs, slots=4, **KW)  # fewer slots than frames: some wait in the queue
+    srv = F.Server(refs, slots=slots, **KW)
     order = [11, 0, 5, 9, 1, 6, 10, 2, 7, 3, 8, 4]
     got = {}
     k = 0
