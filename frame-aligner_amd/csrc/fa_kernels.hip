@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_hess_partials(RefDev R, double* part
 // is the same as the oracle's (oracle/fa_oracle.c).
 
 template <int M>
-__device__ __forceinline__ bool chol_block(double (&A)[M][M])
+__device__ __forceinline__ bool chol_block(double (&A)[M][M], double (&il)[M])
 {
 #pragma unroll
    for (int j = 0; j < M; j++) {
@@ -282,14 +282,14 @@ __device__ __forceinline__ bool chol_block(double (&A)[M][M])
       for (int k = 0; k < j; k++) s -= A[j][k] * A[j][k];
       if (!(s > 0) || !isfinite(s)) return false;
       const double l = sqrt(s);
-      const double il = 1.0 / l;
+      il[j] = 1.0 / l;
       A[j][j] = l;
 #pragma unroll
       for (int i = j + 1; i < M; i++) {
          double t = A[i][j];
 #pragma unroll
          for (int k = 0; k < j; k++) t -= A[i][k] * A[j][k];
-         A[i][j] = t * il;
+         A[i][j] = t * il[j];
       }
 #pragma unroll
       for (int i = 0; i < j; i++) A[i][j] = 0;
@@ -421,12 +421,10 @@ __device__ bool cr_factor(int nb, const double* __restrict__ Hd, const double* _
          const int i = s * (2 * q + 1);
          double L[M][M], X[M][M], il[M];
          ldb<M>(A + i * MM, L);
-         if (!chol_block<M>(L)) {
+         if (!chol_block<M>(L, il)) {  // il: reciprocals of L's diagonal
             fail = 1;
             continue;
          }
-#pragma unroll
-         for (int r = 0; r < M; r++) il[r] = 1.0 / L[r][r];
          stb_factor<M>(A + i * MM, L, il);
          if (y) {
             double v[M];
@@ -520,11 +518,9 @@ __device__ bool cr_factor(int nb, const double* __restrict__ Hd, const double* _
    }
    int fail = 0;
    if (t == 0) {
-      double L[M][M], il[M];
+      double L[M][M], il[M] = {};
       ldb<M>(A, L);
-      fail = chol_block<M>(L) ? 0 : 1;
-#pragma unroll
-      for (int r = 0; r < M; r++) il[r] = 1.0 / L[r][r];
+      fail = chol_block<M>(L, il) ? 0 : 1;
       stb_factor<M>(A, L, il);
       if (y && !fail) {
          double v[M];
