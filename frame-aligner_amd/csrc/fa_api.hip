@@ -850,7 +850,16 @@ struct fa_server {
    int S = 0, nmax = 0;
    int part_of(int j) const { return (int)parts.size() > 1 && j >= first[1] ? 1 : 0; }
    std::deque<Req> pending;
-   std::vector<int> free_slots;
+   std::vector<std::vector<int>> free_slots;  // per part
+   // the part with the most free slots (frames spread evenly over the parts, so both streams
+   // carry work), or -1
+   int free_part() const
+   {
+      int best = -1;
+      for (int p = 0; p < (int)free_slots.size(); p++)
+         if (!free_slots[p].empty() && (best < 0 || free_slots[p].size() > free_slots[best].size())) best = p;
+      return best;
+   }
    std::vector<Req> slot_req;
    std::vector<char> sstate;
    std::vector<size_t> blk_off, blk_size;  // arena block of each slot (size 0: none)
@@ -1060,7 +1069,7 @@ int server_harvest(fa_server* sv)
       if (sv->blk_size[j]) sv->arena_free(sv->blk_off[j], sv->blk_size[j]);
       sv->blk_size[j] = 0;
       sv->sstate[j] = fa_server::kFree;
-      sv->free_slots.push_back(j);
+      sv->free_slots[sv->part_of(j)].push_back(j);
    }
    return FA_OK;
 }
@@ -1072,9 +1081,10 @@ int server_step(fa_server* sv, int rounds)
    // 1. take queued requests, in arrival order, while a slot and (for host data) arena memory
    //    are free; host data is uploaded on the copy stream while the running frames compute
    bool uploading = false;
-   while (!sv->free_slots.empty() && !sv->pending.empty()) {
+   while (sv->free_part() >= 0 && !sv->pending.empty()) {
       fa_server::Req& r = sv->pending.front();
-      const int slot = sv->free_slots.back();
+      const int fp = sv->free_part();
+      const int slot = sv->free_slots[fp].back();
       if (r.host) {
          const size_t nb = frame_bytes(r.ref, r.dtype);
          size_t off;
@@ -1087,7 +1097,7 @@ int server_step(fa_server* sv, int rounds)
          sv->blk_size[slot] = 0;
          HIPCHK(hipEventRecord(sv->up_ev[slot], sv->cstream));
       }
-      sv->free_slots.pop_back();
+      sv->free_slots[fp].pop_back();
       sv->slot_req[slot] = r;
       sv->sstate[slot] = fa_server::kUploading;
       sv->pending.pop_front();
@@ -1152,6 +1162,9 @@ int server_step(fa_server* sv, int rounds)
       fa_batch* b = sv->parts[pi];
       hipStream_t s = b->stream;
       const int j0 = sv->first[pi], nj = (pi + 1 < P ? sv->first[pi + 1] : sv->S) - j0;
+      bool busy = false;
+      for (int j = j0; j < j0 + nj; j++) busy = busy || sv->sstate[j] == fa_server::kRunning;
+      if (!busy) continue;  // its states on the host are still current (nothing ran)
       if (b->tune.use_graph) {
          CHK(ensure_graph(b, rounds));
          HIPCHK(hipGraphLaunch(b->gexec, s));
@@ -1888,7 +1901,8 @@ int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs,
    sv->sstate.assign(slots, fa_server::kFree);
    sv->blk_off.assign(slots, 0);
    sv->blk_size.assign(slots, 0);
-   for (int j = slots - 1; j >= 0; j--) sv->free_slots.push_back(j);
+   sv->free_slots.assign(sv->parts.size(), {});
+   for (int j = slots - 1; j >= 0; j--) sv->free_slots[sv->part_of(j)].push_back(j);
    for (fa_reference* r : sv->refs) r->live_frames++;  // the references outlive the server
    *out = sv;
    return FA_OK;

@@ -44,9 +44,10 @@ def test_server_same_alone_and_in_mix(ctx, slots):
     got = {}
     k = 0
     steps = 0
+    per_step = 2 if slots < 16 else len(order)  # 16 slots: all at once, over both slot batches
     while len(got) < len(frames):
-        # frames arrive two at a time while others are in flight
-        for _ in range(2):
+        # frames arrive per_step at a time while others are in flight
+        for _ in range(per_step):
             if k < len(order):
                 i = order[k]
                 srv.submit(frames[i], tag=i)
@@ -93,15 +94,18 @@ def test_server_errors(ctx):
     r2._release_handle()
 
 
-def test_server_host_frames_and_arena(ctx):
+@pytest.mark.parametrize("slots", [6, 16])
+def test_server_host_frames_and_arena(ctx, slots):
     """Host frames uploaded into the arena give the same results as registered frames; an
-    arena that holds two frames at a time still serves a queue of many."""
+    arena that holds two frames at a time still serves a queue of many (one slot batch); with
+    16 slots every frame is admitted at once, spread over both slot batches."""
     stacks, refs, frames = _setup(ctx)
     alone = [f.align(**KW) for f in frames]
     hosts = ([(refs[0], f) for f in stacks[0]["frames_w"]] + [(refs[1], f) for f in stacks[1]["frames"]] +
              [(refs[2], f) for f in stacks[2]["frames_w"]])
     big = max(f.nbytes for _, f in hosts)
-    srv = F.Server(refs, slots=6, arena_bytes=2 * big + 512, **KW)
+    # 6 slots: an arena of two frames; 16 slots: all frames admitted at once, over both parts
+    srv = F.Server(refs, slots=slots, arena_bytes=(2 if slots < 16 else len(hosts)) * (big + 512), **KW)
     pinned = [F.PinnedArray(f.shape, f.dtype) for _, f in hosts]
     for p, (_, f) in zip(pinned, hosts):
         p.array[...] = f
