@@ -35,6 +35,7 @@ struct Chunk {
 };
 // Column width of strip work items (one column per pass thread).
 constexpr int kStripWidth = 256;
+static_assert(kStripWidth == kBlock, "a strip item has one column per fused-pass thread");
 
 // Doubles written per work item by the fused pass: SSD, Jtr of node k (n_dim values,
 // weight 1-f), Jtr of node k+1 (n_dim values, weight f); padded to 8.
