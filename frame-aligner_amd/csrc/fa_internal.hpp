@@ -35,7 +35,6 @@ struct Chunk {
 };
 // Column width of strip work items (one column per pass thread).
 constexpr int kStripWidth = 256;
-static_assert(kStripWidth == kBlock, "a strip item has one column per fused-pass thread");
 
 // Doubles written per work item by the fused pass: SSD, Jtr of node k (n_dim values,
 // weight 1-f), Jtr of node k+1 (n_dim values, weight f); padded to 8.
@@ -44,6 +43,7 @@ constexpr int kPassStride = 8;
 // 21 in 3-D, padded to 24.
 constexpr int kHessStride = 24;
 constexpr int kBlock = 256;  // threads per fused-pass workgroup (4 wave64)
+static_assert(kStripWidth == kBlock, "a strip item has one column per fused-pass thread");
 // Zero columns staged on each side of a frame row in LDS: taps need no bounds checks while
 // |dx| stays below kPad - 2 over a work item.
 constexpr int kPad = 16;
