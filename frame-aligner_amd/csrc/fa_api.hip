@@ -842,10 +842,10 @@ struct fa_server {
    fa_context* ctx = nullptr;
    // slot batches (one, or two on their own streams whose rounds interleave): each holds its
    // slots' worth of state, sized for the largest reference; global slot j is local slot
-   // j - first[p] of part p
+   // j - first[p] of part p, and the part's frames use the staging entries (h_bf, h_x,
+   // h_slots, d_slots) from first[p] on
    std::vector<fa_batch*> parts;
    std::vector<int> first;
-   std::vector<int> d_slots_off;  // each part's region of d_slots / h_slots
    std::vector<fa_reference*> refs;
    int S = 0, nmax = 0;
    int part_of(int j) const { return (int)parts.size() > 1 && j >= first[1] ? 1 : 0; }
@@ -1111,7 +1111,7 @@ int server_step(fa_server* sv, int rounds)
    for (int pi = 0; pi < P; pi++) {
       fa_batch* b = sv->parts[pi];
       const int j0 = sv->first[pi], j1 = pi + 1 < P ? sv->first[pi + 1] : sv->S;
-      const int base = sv->d_slots_off[pi];
+      const int base = j0;  // the part's region of the staging arrays and d_slots
       int na = 0;
       for (int slot = j0; slot < j1; slot++) {
          if (sv->sstate[slot] != fa_server::kUploading) continue;
@@ -1869,7 +1869,6 @@ int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs,
       if (rc != FA_OK) break;
       sv->parts.push_back(q);
       sv->first.push_back(j);
-      sv->d_slots_off.push_back(j);
       j += sp;
    }
    if (rc == FA_OK)
