@@ -281,11 +281,15 @@ constexpr int kWideUpdateNodes = 128;
 
 int tr_threads(const fa_reference* r) { return r->nD >= kWideUpdateNodes ? 256 : 64; }
 
+// LDS the update kernels declare statically (their workgroup reductions), kept free beside the
+// dynamic LDS.
+constexpr int kTrStaticLds = 1024;
+
 // The update runs from LDS when the largest frame's need fits one workgroup's LDS, else from
 // global memory (0).
 int tr_lds_for(const fa_context* ctx, int need)
 {
-   return (int64_t)need * (int64_t)sizeof(double) <= ctx->lds_per_block ? need : 0;
+   return (int64_t)need * (int64_t)sizeof(double) + kTrStaticLds <= ctx->lds_per_block ? need : 0;
 }
 
 // Default LDS budget: the largest over the batch's frames, capped.
@@ -345,10 +349,9 @@ void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::
       c.lds = std::max(c.lds, need);
       c.work += ref_items[i];
    }
-   // The update classes run one after another on the batch's stream: one per workgroup width,
-   // split further by factor size (a class starting above 2x its first) unless every frame's
-   // workgroup is resident at once even with the largest LDS size, when fewer launches are
-   // quicker.
+   // Update classes: one per workgroup width, split further by factor size (a class starting
+   // above 2x its first) unless every frame's workgroup is resident at once even with the
+   // largest LDS size; launch_updates then runs the classes in one launch.
    int most = 0, most_full = 0;
    for (fa_reference* r : rs) {
       most = std::max(most, tr_lds_need(r));
@@ -525,11 +528,33 @@ int launch_passes(fa_batch* b, hipStream_t s)
    return FA_OK;
 }
 
-// Enqueues the trust-region update, one launch per update class.
+// Enqueues the trust-region update: one launch per update class or, when there are several
+// and every frame's workgroup is resident at once even with the largest class's LDS size, one
+// launch of all of them (the update is latency-bound, so class launches one after another
+// would add their latencies).
 int launch_updates(fa_batch* b, hipStream_t s)
 {
    const BatchDev B = b->dev();
-   for (size_t c = 0; c < b->trc_lds.size(); c++)
+   const size_t nc = b->trc_lds.size();
+   if (nc > 1 && nc <= 32) {
+      int most = 0;
+      bool fit = true;
+      uint32_t wide = 0, full = 0;
+      for (size_t c = 0; c < nc; c++) {
+         most = std::max(most, b->trc_lds[c]);
+         fit = fit && b->trc_lds[c] > 0;
+         if (b->trc_threads[c] == 256) wide |= 1u << c;
+         if (b->trc_full[c]) full |= 1u << c;
+      }
+      const int lds = b->tune.tr_lds ? most : 0;
+      const int64_t per_cu =
+          lds ? b->ctx->lds_per_block / ((int64_t)lds * (int64_t)sizeof(double)) : 8;
+      if ((fit || !b->tune.tr_lds) && (int64_t)B.n_frames <= (int64_t)b->ctx->n_cu * std::max<int64_t>(per_cu, 1)) {
+         HIPCHK(launch_tr_update_all(B, lds, wide, lds ? full : 0u, s));
+         return FA_OK;
+      }
+   }
+   for (size_t c = 0; c < nc; c++)
       HIPCHK(launch_tr_update(B, (int)c, b->tune.tr_lds ? b->trc_lds[c] : 0, b->trc_full[c], b->trc_threads[c], s));
    return FA_OK;
 }

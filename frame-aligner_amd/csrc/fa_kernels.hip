@@ -1971,6 +1971,44 @@ __global__ __launch_bounds__(NT) void k_tr_update(BatchDev B, int cls)
    else tr_update_frame<3, LDS, NT, FULL>(B, j, smem);
 }
 
+// One frame's update in the variant its class launches (NT threads, FULL staging).
+template <bool LDS, int NT>
+__device__ __forceinline__ void tr_update_variant(const BatchDev& B, const int j, const bool full, double* smem)
+{
+   const bool three = B.refs[B.frames[j].ref].n_dim == 3;
+   if constexpr (LDS) {
+      if (full) {
+         if (three) tr_update_frame<3, true, NT, true>(B, j, smem);
+         else tr_update_frame<2, true, NT, true>(B, j, smem);
+         return;
+      }
+   }
+   if (three) tr_update_frame<3, LDS, NT, false>(B, j, smem);
+   else tr_update_frame<2, LDS, NT, false>(B, j, smem);
+}
+
+// Every update class in one launch, for batches whose workgroups are all resident at once (the
+// update is latency-bound, so class launches one after another would add their latencies).
+// Bit c of wide / full: class c runs 256 threads / stages its whole working set. A frame of a
+// one-wave class runs in the first wave and the other three waves end at once (a workgroup
+// barrier waits only for the waves that have not ended), so every frame runs the same code as
+// in its class's own launch.
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_tr_update_all(BatchDev B, uint32_t wide, uint32_t full)
+{
+   extern __shared__ double smem[];
+   const int j = blockIdx.x;
+   if (B.state[j].done) return;
+   const int c = B.frames[j].trc;
+   const bool f = (full >> c) & 1;
+   if ((wide >> c) & 1) {
+      tr_update_variant<LDS, 256>(B, j, f, smem);
+   } else {
+      if (__builtin_amdgcn_readfirstlane(threadIdx.x) >= 64) return;
+      tr_update_variant<LDS, 64>(B, j, f, smem);
+   }
+}
+
 // ---- launchers ----------------------------------------------------------------------------
 
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -2066,7 +2104,8 @@ hipError_t configure_kernels(int max_lds_bytes)
 {
    (void)hipFuncSetAttribute((const void*)k_eigen, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes);
    const void* fns[] = {(const void*)k_tr_update<true, 64, false>, (const void*)k_tr_update<true, 64, true>,
-                        (const void*)k_tr_update<true, 256, false>, (const void*)k_tr_update<true, 256, true>};
+                        (const void*)k_tr_update<true, 256, false>, (const void*)k_tr_update<true, 256, true>,
+                        (const void*)k_tr_update_all<true>};
    hipError_t e = hipSuccess;
    for (const void* f : fns) {
       const hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes);
@@ -2090,6 +2129,18 @@ hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, int ful
       else if (full) hipLaunchKernelGGL((k_tr_update<true, 64, true>), grid, dim3(64), lds, s, B, cls);
       else hipLaunchKernelGGL((k_tr_update<true, 64, false>), grid, dim3(64), lds, s, B, cls);
    }
+   return hipGetLastError();
+}
+
+// All classes in one launch (k_tr_update_all): LDS of the largest class, or global (0).
+hipError_t launch_tr_update_all(const BatchDev& B, int lds_doubles, uint32_t wide, uint32_t full, hipStream_t s)
+{
+   const dim3 grid(B.n_frames);
+   if (lds_doubles <= 0)
+      hipLaunchKernelGGL((k_tr_update_all<false>), grid, dim3(256), 0, s, B, wide, 0u);
+   else
+      hipLaunchKernelGGL((k_tr_update_all<true>), grid, dim3(256), (size_t)lds_doubles * sizeof(double), s, B,
+                         wide, full);
    return hipGetLastError();
 }
 
