@@ -480,7 +480,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-slots", type=int, default=128)
-    ap.add_argument("--serve-rounds", type=int, default=2)
+    ap.add_argument("--serve-rounds", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve512-high", type=float, default=20000.0,
                     help="second offered rate (frames/s) of the 512^2 serving run")

@@ -57,7 +57,7 @@ class ServeLoop:
     the raw uint16 frames its stream cycles through. The pools are copied once into pinned
     host memory (where a frame grabber would DMA them), so uploads are asynchronous."""
 
-    def __init__(self, server, refs, pools, rounds=2):
+    def __init__(self, server, refs, pools, rounds=3):
         from . import PinnedArray
         self.server = server
         self.refs = refs
