@@ -572,7 +572,8 @@ int event_pair(fa_batch* b, size_t i, hipEvent_t* s, hipEvent_t* e)
 }
 
 constexpr size_t kMaxTimedRounds = 4096;
-constexpr int kServerSplitSlots = 16;  // servers with at least this many slots run two slot batches
+constexpr int kServerSplitSlots = 16;  // servers with at least this many slots run kServerParts
+constexpr int kServerParts = 2;        // slot batches (at least 8 slots each)
 constexpr int kSplitFrames = 64;  // batches of at least this many frames run as two parts
 
 int64_t max_rounds(const fa_batch* b) { return b->prm.max_evals > 0 ? (int64_t)b->prm.max_evals + 2 : 100000; }
@@ -873,7 +874,12 @@ struct fa_server {
    std::vector<int> first;
    std::vector<fa_reference*> refs;
    int S = 0, nmax = 0;
-   int part_of(int j) const { return (int)parts.size() > 1 && j >= first[1] ? 1 : 0; }
+   int part_of(int j) const
+   {
+      int p = 0;
+      while (p + 1 < (int)first.size() && j >= first[p + 1]) p++;
+      return p;
+   }
    std::deque<Req> pending;
    std::vector<std::vector<int>> free_slots;  // per part
    // the part with the most free slots (frames spread evenly over the parts, so both streams
@@ -1885,7 +1891,7 @@ int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs,
    sv->S = slots;
    sv->arena_bytes = arena_bytes ? fa_server::round_up(arena_bytes) : largest * (size_t)slots;
    // two slot batches when there are enough slots for both to fill their launches
-   const int P = slots >= kServerSplitSlots ? 2 : 1;
+   const int P = slots >= kServerSplitSlots ? std::min(kServerParts, slots / 8) : 1;
    int rc = FA_OK;
    for (int pi = 0, j = 0; pi < P && rc == FA_OK; pi++) {
       const int sp = slots / P + (pi < slots % P ? 1 : 0);
