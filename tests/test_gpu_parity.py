@@ -232,6 +232,39 @@ def test_batch_same_alone_and_in_mix_update_widths(ctx):
     assert np.max(np.abs(xs[0] - x_o)) <= D_TOL
 
 
+def test_batch_update_classes_launched_apart(ctx):
+    """A batch with more frames than can hold their update workgroups at once runs its update
+    classes one launch after another (a batch small enough runs them in one launch): 256-thread
+    3-D frames (161 nodes) beside a thousand-odd one-wave 2-D frames, each result bit-identical
+    to its frame aligned alone."""
+    name, shape, nD, scan, _ = MANY_NODE_CASES[1]
+    a, _, ref_a = _pair(ctx, shape, nD, scan, name=name)
+    b, _, ref_b = _pair(ctx, (1, 96, 80), 9, dict(line=1.0, interline=1.0), seed=21)
+    info = ctx.info()
+    # the 3-D frames' update needs at least its factor blocks and solve vector in LDS
+    # (3 nD 9 + 3 nD doubles), so at most this many workgroups are resident at once; the batch
+    # runs as two parts of about half its pixels each, so the part with the 3-D frames holds
+    # more than that when the 2-D frames number twice as many and a margin
+    resident = info["n_cu"] * max(1, info["lds_per_block"] // (8 * 30 * nD))
+    reps = (2 * resident + 64) // len(b["frames_w"]) + 1
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    fa_ = [ref_a.register(f) for f in a["frames_w"]]
+    fb_ = [ref_b.register(b["frames_w"][i % len(b["frames_w"])]) for i in range(reps * len(b["frames_w"]))]
+    assert len(fb_) > 2 * resident + 32
+    alone_a = [f.align(**kw) for f in fa_]
+    alone_b = [f.align(**kw) for f in fb_[:len(b["frames_w"])]]
+    xs, stats, _ = F.Batch(fa_ + fb_, **kw).run()
+    for i, (x1, s1) in enumerate(alone_a):
+        assert np.array_equal(x1, xs[i])
+        assert s1["evaluations"] == stats[i]["evaluations"]
+    for j in range(len(fb_)):
+        x1, s1 = alone_b[j % len(b["frames_w"])]
+        assert np.array_equal(x1, xs[len(fa_) + j])
+        assert s1["evaluations"] == stats[len(fa_) + j]["evaluations"]
+    for f in fa_ + fb_:
+        f.release()
+
+
 def test_batch_same_alone_and_in_mix(ctx):
     """Each frame's result is bit-identical alone and inside a mixed batch (two references of
     different size, uint16 and float frames)."""
