@@ -547,9 +547,7 @@ int launch_updates(fa_batch* b, hipStream_t s)
          if (b->trc_full[c]) full |= 1u << c;
       }
       const int lds = b->tune.tr_lds ? most : 0;
-      const int64_t per_cu =
-          lds ? b->ctx->lds_per_block / ((int64_t)lds * (int64_t)sizeof(double)) : 8;
-      if ((fit || !b->tune.tr_lds) && (int64_t)B.n_frames <= (int64_t)b->ctx->n_cu * std::max<int64_t>(per_cu, 1)) {
+      if ((fit || !b->tune.tr_lds) && (int64_t)B.n_frames <= (int64_t)b->ctx->n_cu * tr_update_all_per_cu(lds)) {
          HIPCHK(launch_tr_update_all(B, lds, wide, lds ? full : 0u, s));
          return FA_OK;
       }

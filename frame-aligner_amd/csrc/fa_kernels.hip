@@ -2132,6 +2132,20 @@ hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, int ful
    return hipGetLastError();
 }
 
+// Workgroups of k_tr_update_all one compute unit holds at once with this much dynamic LDS
+// (registers and LDS both count), at least 1.
+int tr_update_all_per_cu(int lds_doubles)
+{
+   int n = 0;
+   const void* f = lds_doubles > 0 ? (const void*)k_tr_update_all<true> : (const void*)k_tr_update_all<false>;
+   const size_t lds = lds_doubles > 0 ? (size_t)lds_doubles * sizeof(double) : 0;
+   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 256, lds) != hipSuccess) {
+      (void)hipGetLastError();
+      return 1;
+   }
+   return n > 1 ? n : 1;
+}
+
 // All classes in one launch (k_tr_update_all): LDS of the largest class, or global (0).
 hipError_t launch_tr_update_all(const BatchDev& B, int lds_doubles, uint32_t wide, uint32_t full, hipStream_t s)
 {

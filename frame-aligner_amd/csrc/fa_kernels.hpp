@@ -24,6 +24,7 @@ hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s);
 hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, hipStream_t s);
 hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, int full, int threads, hipStream_t s);
 hipError_t launch_tr_update_all(const BatchDev& B, int lds_doubles, uint32_t wide, uint32_t full, hipStream_t s);
+int tr_update_all_per_cu(int lds_doubles);
 hipError_t configure_kernels(int max_lds_bytes);
 hipError_t launch_reduce_eval(const BatchDev& B, hipStream_t s);
 
