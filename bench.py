@@ -243,9 +243,10 @@ def precision_3d(F, args, ctx):
     if not args.no_cpu:
         k = min(args.c3_oracle, n)
         fw = np.stack([synth.with_offset(f) for f in st["frames"][:k]])
-        per, xo, so = [], [], []
+        per, xo, so, orcs = [], [], [], []
         for fp64 in (True, False):
             orc = O.Oracle(fp64=fp64).set_reference(ref_w, cfg["nD"])
+            orcs.append(orc)
             t = time.perf_counter()
             x_o, s_o, secs = orc.align_many(fw, threads=min(k, args.cpu_threads), **tr)
             wall_o = time.perf_counter() - t
@@ -268,6 +269,20 @@ def precision_3d(F, args, ctx):
         out["final_ssd_rel_to_fp64_oracle"] = {
             "gpu": [round(stats[i]["f_final"] / so[0][i]["f_final"] - 1, 8) for i in range(k)],
             "float_oracle": [round(so[1][i]["f_final"] / so[0][i]["f_final"] - 1, 8) for i in range(k)]}
+        # one evaluation at the generator's displacements: the GPU's SSD and Jtr (FP32 warp and
+        # compensated sums, FP64 across work items) against the FP64 oracle's, and JtJ
+        ssd_e, jtr_e = [], []
+        for i in range(k):
+            D = st["D_true"][i]
+            e_g, j_g = frames[i].jacobian(D)
+            e_o, j_o = orcs[0].get_jacobian(fw[i], D)
+            ssd_e.append(abs(e_g - e_o) / abs(e_o))
+            jtr_e.append(float(np.linalg.norm(j_g - j_o) / np.linalg.norm(j_o)))
+        H_g, H_o = ref.H, orcs[0].hessian()
+        out["at_true_D_vs_fp64_oracle"] = {
+            "ssd_rel_err": [float(f"{e:.3g}") for e in ssd_e], "jtr_rel_err": [float(f"{e:.3g}") for e in jtr_e],
+            "jtj_rel_err": float(f"{np.linalg.norm(H_g - H_o) / np.linalg.norm(H_o):.3g}"),
+            "tolerance": {"ssd": 1e-4, "jtr_jtj": 1e-3}}
     b.close()
     for f in frames:
         f.release()
