@@ -1,6 +1,6 @@
 """Benchmark: aligned frames/s on the 512x512, 30-node workload on one MI355X.
 
-Workload (BASELINE configs[1]): 2-D 512x512 uint16 photon-count frames (600-blob
+Workload: 2-D 512x512 uint16 photon-count frames (600-blob
 phantom, mean ~50 photons/px, Poisson noise), 200 frames, 30 resampling points
 (nD = 31 nodes, 2 components), trust region to 1e-4 relative change of the SSD
 or 50 iterations. A step aligns the whole 200-frame batch from the zero start,
@@ -146,7 +146,7 @@ def serving(F, args, ctx):
 def serving_512(F, args, ctx, plain_fps):
     """The serving form on the headline workload: one acquisition of 512x512 uint16 frames,
     nD = 31, streamed with exponential inter-arrival times through the continuous-batching
-    server at 3x the plain path's frames/s (the north-star operating point) and at
+    server at 3x the plain path's frames/s (the batched-serving target's operating point) and at
     args.serve512_high; frames/s and p50/p99 arrival -> result latency, and served == alone on a
     sample."""
     from fa_amd import serve, synth
@@ -527,7 +527,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames are "
-              f"independent, see the sharding note in DESIGN.md)", file=sys.stderr)
+              f"independent given the reference: several GPUs would each take whole frames)", file=sys.stderr)
         sys.exit(2)
 
     secs = set(args.sections.split(","))

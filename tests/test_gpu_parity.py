@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C ABI) against the CPU oracle on seeded inputs.
 
-Tolerances: SSD within 1e-4 relative, JtJ / Jtr within 1e-3
+Tolerances (the path's parity contract): SSD within 1e-4 relative, JtJ / Jtr within 1e-3
 relative, converged node displacements within 0.02 px. Index work (node map,
 interval ranges) and the Sobel gradient, which the reference computes in plain
 float with a fixed operation order, are compared bit for bit.
