@@ -33,6 +33,41 @@ def _setup(ctx):
     return (a, b, c), (ra, rb, rc), frames
 
 
+def test_server_update_widths_in_one_launch(ctx):
+    """References whose updates differ in width and subproblem form share the slot batches:
+    9 nodes (eigendecomposition, one wave), 60 nodes (cyclic reduction, one wave) and 150 nodes
+    (cyclic reduction, 256 threads), whose updates then run as one launch. Every result is the
+    frame's alone."""
+    from helpers import MANY_NODE_CASES
+    name, shape, nD, scan, gen = MANY_NODE_CASES[0]
+    a = small_stack((1, 96, 80), n_frames=3, nD=9, seed=45)
+    d = small_stack((1, 96, 80), n_frames=3, nD=60, seed=46, walk_sigma=0.1, sin_amp=1.5)
+    e = small_stack(shape, n_frames=2, nD=nD, seed=47, **gen)
+    refs = [F.Reference(ctx, a["ref_w"], 9), F.Reference(ctx, d["ref_w"], 60),
+            F.Reference(ctx, e["ref_w"], nD, **scan)]
+    frames = ([refs[0].register(f) for f in a["frames_w"]] + [refs[1].register(f) for f in d["frames_w"]] +
+              [refs[2].register(f) for f in e["frames_w"]])
+    alone = [f.align(**KW) for f in frames]
+    srv = F.Server(refs, slots=16, **KW)
+    got = {}
+    for i in (6, 0, 3, 7, 1, 4, 2, 5):
+        srv.submit(frames[i], tag=i)
+    steps = 0
+    while len(got) < len(frames):
+        _, _, res = srv.step(rounds=1 + steps % 3)
+        steps += 1
+        for tag, x, st in res:
+            got[tag] = (x, st)
+        assert steps < 10000
+    for i, (x1, s1) in enumerate(alone):
+        x2, s2 = got[i]
+        assert np.array_equal(x1, x2), i
+        assert (s1["evaluations"], s1["status"], s1["f_final"]) == (s2["evaluations"], s2["status"], s2["f_final"]), i
+    srv.close()
+    for f in frames:
+        f.release()
+
+
 @pytest.mark.parametrize("slots", [4, 16])
 def test_server_same_alone_and_in_mix(ctx, slots):
     """4 slots: fewer than the frames, some wait in the queue; 16 slots: two slot batches on
