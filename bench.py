@@ -220,7 +220,7 @@ def precision_3d(F, args, ctx):
     from fa_amd import synth
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    cfg = synth.CONFIGS["c3"]
+    cfg = synth.CONFIGS["3d_256x256x32_f32"]
     t0 = time.time()
     st = synth.make_stack(cfg["shape"], args.c3_frames, cfg["nD"], seed=args.seed + 3, n_blobs=cfg["n_blobs"],
                           sigma=cfg["sigma"], dtype="float32", order=1)
@@ -519,7 +519,7 @@ def main():
                     help="comma list of: batch (the headline 512^2 workload), serve, precision (3-D), "
                          "uneven (64 stacks)")
     ap.add_argument("--c3-frames", type=int, default=50)
-    ap.add_argument("--c3-oracle", type=int, default=4, help="volumes checked against the oracles")
+    ap.add_argument("--c3-oracle", type=int, default=8, help="volumes checked against the oracles")
     ap.add_argument("--c4-stacks", type=int, default=64)
     ap.add_argument("--tune-frames", type=int, default=0, help="frames the chunk-size tuning uses (0: all)")
     ap.add_argument("--copy-probe", type=int, default=0,

@@ -160,6 +160,8 @@ struct fa_batch {
    float* d_warp = nullptr;         // output buffers of warps (one-frame batches)
    float* d_cov = nullptr;
    int64_t warp_px = 0;
+   double* d_trace = nullptr;       // trust-region trace (fa_batch_set_trace), or null
+   int trace_cap = 0, trace_rec = 0;
 
    BatchDev dev() const
    {
@@ -187,6 +189,9 @@ struct fa_batch {
       B.prm.sub_max_iter = prm.sub_max_iter;
       B.prm.skip_repeat = prm.skip_repeat;
       B.n_frames = n_slots > 0 ? n_slots : (int32_t)frames.size();
+      B.trace = d_trace;
+      B.trace_cap = trace_cap;
+      B.trace_rec = trace_rec;
       return B;
    }
 
@@ -237,6 +242,7 @@ struct fa_batch {
       dfree(d_counters);
       dfree(d_warp);
       dfree(d_cov);
+      dfree(d_trace);
       if (h_counters) (void)hipHostFree(h_counters);
       if (stream) (void)hipStreamDestroy(stream);
    }
@@ -582,6 +588,8 @@ int run_begin(fa_batch* b, const double* x)
    hipStream_t s = b->stream;
    HIPCHK(hipMemcpyAsync(b->d_x, x, b->total_x * sizeof(double), hipMemcpyHostToDevice, s));
    HIPCHK(hipMemsetAsync(b->d_counters, 0, sizeof(Counters), s));
+   if (b->d_trace)
+      HIPCHK(hipMemsetAsync(b->d_trace, 0, (size_t)b->frames.size() * b->trace_cap * b->trace_rec * sizeof(double), s));
    HIPCHK(launch_tr_init(b->dev(), 0, s));
    b->finished = false;
    b->timed = 0;
@@ -837,6 +845,41 @@ int batch_create_split(fa_frame* const* frames, int32_t n, const fa_tr_params* p
    b->total_x = a->total_x + c->total_x;
    b->total_D = a->total_D + c->total_D;
    *out = b;
+   return FA_OK;
+}
+
+// Largest parameter count nD * n_dim of the batch's frames.
+int max_params(const fa_batch* b)
+{
+   int m = 0;
+   for (fa_frame* f : b->frames) m = std::max(m, f->ref->nD * f->ref->n_dim);
+   return m;
+}
+
+int leaf_set_trace(fa_batch* b, int cap, int rec)
+{
+   HIPCHK(hipStreamSynchronize(b->stream));
+   dfree(b->d_trace);
+   b->trace_cap = 0;
+   b->trace_rec = 0;
+   b->drop_graph();  // graphs carry the trace pointer
+   if (cap <= 0) return FA_OK;
+   CHK(dalloc(&b->d_trace, b->frames.size() * (size_t)cap * rec));
+   HIPCHK(hipMemset(b->d_trace, 0, b->frames.size() * (size_t)cap * rec * sizeof(double)));
+   b->trace_cap = cap;
+   b->trace_rec = rec;
+   return FA_OK;
+}
+
+int leaf_get_trace(fa_batch* b, double* out, int first_user, int cap, int rec)
+{
+   const size_t per = (size_t)cap * rec;
+   if (!b->d_trace || b->trace_cap != cap || b->trace_rec != rec) return set_err(FA_E_STATE, "tracing is off");
+   std::vector<double> h(b->frames.size() * per);
+   HIPCHK(hipMemcpyAsync(h.data(), b->d_trace, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+   HIPCHK(hipStreamSynchronize(b->stream));
+   for (size_t j = 0; j < b->frames.size(); j++)
+      std::memcpy(out + (size_t)(first_user + b->user[j]) * per, h.data() + j * per, per * sizeof(double));
    return FA_OK;
 }
 
@@ -1784,6 +1827,30 @@ int fa_batch_set_timing(fa_batch* b, int32_t enable)
    if (!b) return set_err(FA_E_ARG, "null batch");
    b->timing = enable ? 1 : 0;
    for (fa_batch* q : b->parts) q->timing = b->timing;
+   return FA_OK;
+}
+
+int fa_batch_set_trace(fa_batch* b, int32_t max_records)
+{
+   if (!b) return set_err(FA_E_ARG, "null batch");
+   if (max_records < 0 || max_records > 100000) return set_err(FA_E_ARG, "max_records out of range");
+   DeviceGuard g(b->ctx->device);
+   const int rec = kTraceFields + 3 * max_params(b);
+   if (b->parts.empty()) return leaf_set_trace(b, max_records, rec);
+   for (fa_batch* q : b->parts) CHK(leaf_set_trace(q, max_records, rec));
+   b->trace_cap = max_records;
+   b->trace_rec = max_records > 0 ? rec : 0;
+   return FA_OK;
+}
+
+int fa_batch_get_trace(fa_batch* b, double* out)
+{
+   if (!b || !out) return set_err(FA_E_ARG, "null argument");
+   if (b->trace_cap <= 0) return set_err(FA_E_STATE, "tracing is off");
+   DeviceGuard g(b->ctx->device);
+   if (b->parts.empty()) return leaf_get_trace(b, out, 0, b->trace_cap, b->trace_rec);
+   for (size_t i = 0; i < b->parts.size(); i++)
+      CHK(leaf_get_trace(b->parts[i], out, b->part_first[i], b->trace_cap, b->trace_rec));
    return FA_OK;
 }
 

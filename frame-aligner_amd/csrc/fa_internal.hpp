@@ -131,6 +131,13 @@ struct PassArgs {
    int32_t ref_lo, ref_hi;  // the launch takes the work items of references [ref_lo, ref_hi)
 };
 
+// Fields of one trust-region trace record (one per objective evaluation of a frame), in this
+// order: evaluations so far, f of this evaluation, f at x after the decision, predicted and
+// measured decrease of the step tried, rho, accepted (1/0), radius after the decision,
+// stop-strategy iterations, stop status (0 while running), lambda and iterations of the next
+// subproblem, |p| of the next step, |g| at x, g'p, rejected steps so far. x, g and p follow.
+constexpr int kTraceFields = FA_TRACE_FIELDS;
+
 struct BatchDev {
    const RefDev* refs;
    BatchFrame* frames;
@@ -148,6 +155,9 @@ struct BatchDev {
    TrParamsDev prm;
    int32_t n_frames;
    int32_t n_refs;    // references: 2-D before 3-D, by LDS need within a kind
+   double* trace;     // null, or trace_cap records of trace_rec doubles per frame
+   int32_t trace_cap;
+   int32_t trace_rec; // kTraceFields + 3 * (largest nD n_dim of the batch)
 };
 
 }  // namespace fa

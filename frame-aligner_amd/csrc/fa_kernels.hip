@@ -848,18 +848,21 @@ __global__ __launch_bounds__(kEigThreads) void k_eigen(RefDev R, double* Q, doub
 // reach are staged in LDS when they fit.
 enum { MODE_PASS = 0, MODE_WARP = 1 };
 
-// Sum over the 64 lanes of a wave with DPP lane moves (no LDS traffic): pairs and quads
+// Sum over the 64 lanes of a wave with DPP lane moves (no LDS traffic), in FP64: each step
+// moves the two 32-bit halves of the partner lane's value and adds. Pairs and quads
 // (quad_perm), the two halves of each row of 8 and of 16 (row_half_mirror, row_mirror), then
 // row 0 into row 1 and row 2 into row 3 (row_bcast15), and rows 0-1 into rows 2-3
 // (row_bcast31). The total is in lane 63; the order of the additions is fixed.
 template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_add(float v)
+__device__ __forceinline__ double dpp_add(double v)
 {
-   const int t = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
-   return v + __int_as_float(t);
+   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+   const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, false);
+   const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+   return v + __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-__device__ __forceinline__ float wave_total(float v)
+__device__ __forceinline__ double wave_total(double v)
 {
    v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
    v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
@@ -867,21 +870,20 @@ __device__ __forceinline__ float wave_total(float v)
    v = dpp_add<0x140, 0xf>(v);  // row_mirror
    v = dpp_add<0x142, 0xa>(v);  // row_bcast15 into rows 1 and 3
    v = dpp_add<0x143, 0xc>(v);  // row_bcast31 into rows 2 and 3
-   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Block reduction used by the fused pass: the compensated pairs of each thread are summed as
-// two plain float sums per wave (leading parts and compensations), combined in double over
-// the waves.
+// Block reduction of the fused pass: every thread's FP64 sums, over the lanes of each wave
+// (wave_total), then over the waves in wave order.
 template <int NV>
-__device__ __forceinline__ void block_reduce_fast(KF (&acc)[NV], KF (*sred)[NV], double* out)
+__device__ __forceinline__ void block_reduce_fast(double (&acc)[NV], double (*sred)[NV], double* out)
 {
-   KF w[NV];
+   double w[NV];
 #pragma unroll
-   for (int v = 0; v < NV; v++) {
-      w[v].s = wave_total(acc[v].s);
-      w[v].c = wave_total(acc[v].c);
-   }
+   for (int v = 0; v < NV; v++) w[v] = wave_total(acc[v]);
    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    if (lane == 0)
 #pragma unroll
@@ -890,7 +892,7 @@ __device__ __forceinline__ void block_reduce_fast(KF (&acc)[NV], KF (*sred)[NV],
    if (threadIdx.x < NV) {
       const int v = threadIdx.x;
       double t = 0;
-      for (int q = 0; q < kBlock / 64; q++) t += (double)sred[q][v].s - (double)sred[q][v].c;
+      for (int q = 0; q < kBlock / 64; q++) t += sred[q][v];
       out[v] = t;
    }
    __syncthreads();
@@ -1058,36 +1060,43 @@ __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, 
 // samples that exist (computeErrorImage, FrameWarper.cpp:4-28), and the Jacobian rows
 // (computeJacobian, FrameWarper.cpp:147-167).
 //
-// Everything per pixel is FP32. d is formed relative to the pixel as D[k] + f (D[k+1] - D[k])
-// (one fma per component) and split into an integer tap offset and a fraction, both exact in
-// float, so the sample position carries the error of d alone (about 1e-7 px for displacements
-// of a few px) instead of the rounding of an absolute coordinate; D[k+1] = D[k] gives D[k]
-// exactly (integer shifts sample exactly). The sums are compensated.
+// Precision. The displacement is FP64, as the reference forms it: d = D[k] + f (D[k+1] - D[k])
+// relative to the pixel (one FP64 fma per component from the float node fraction f), split
+// into an integer tap offset and a fraction; D[k+1] = D[k] gives D[k] exactly (integer shifts
+// sample exactly). The interpolation, the residual e and the products with the gradient are
+// FP32. Each thread sums its pixels' terms in FP64 (e*e, and (g e)(1-f), (g e) f with g e a
+// float product, formed exactly in FP64 by the fma), the waves and the workgroup in FP64 in a
+// fixed order. On gfx950 an FP64 fma issues at the rate of an unpacked FP32 one, so these
+// sums cost about what compensated FP32 sums of 4-pixel FP32 partials did, and they are as
+// accurate as the reference's double sums. (Displacements rounded to float move the sample
+// point by ~1e-7 px per node, and the FP32 partial sums lose ~1e-8 of Jtr; both are small
+// against the 1e-3 tolerance, but the trust region amplifies them: on 256^2 x 32 volumes with
+// 513 nodes it took a different path from the reference's arithmetic.)
 template <int M, typename FT, int MODE>
 __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
                            const double* __restrict__ D, float* lds, int lds_cap, bool want_lds,
-                           double* out, float* wout, float invalid, KF (*sred)[7])
+                           double* out, float* wout, float invalid, double (*sred)[7])
 {
    constexpr int NV = 1 + 2 * M;
    const int nx = R.nx, ny = R.ny, nz = R.nz;
    const int64_t plane_sz = (int64_t)nx * ny;
    const int k = ch.k;
-   float Da[3], Dd[3];
+   double Da[3], Dd[3];
    bool finite = true;
 #pragma unroll
    for (int c = 0; c < 3; c++) {
       const double d0 = (c < M) ? D[3 * k + c] : 0.0, d1 = (c < M) ? D[3 * (k + 1) + c] : 0.0;
       finite = finite && fabs(d0) < 1e6 && fabs(d1) < 1e6;
-      Da[c] = (float)d0;
-      Dd[c] = (float)d1 - Da[c];
+      Da[c] = d0;
+      Dd[c] = d1 - d0;
    }
    const int plane = (int)(ch.p0 / plane_sz);
    const int q0 = (int)(ch.p0 - plane * plane_sz);
    const int r0 = q0 / nx, r1 = (int)((ch.p1 - 1 - plane * plane_sz) / nx);
 
    // f rises with the pixel's scan time, so over the work item it lies in [f(p0), f(p1-1)]
-   // and d between its values there (exact in double); the margin covers the rounding of the
-   // per-pixel fma (half an ulp of |d|; none when D[k+1] = D[k]).
+   // and d between its values there; the margin covers the rounding of the per-pixel fma
+   // (half an FP64 ulp of |d|; none when D[k+1] = D[k]).
    Stage S{0, 1, 0, 0, 0};
    bool use = false;
    if (want_lds && finite) {
@@ -1099,7 +1108,7 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
       for (int c = 0; c < 3; c++) {
          const double a = Da[c], dd = Dd[c];
          const double va = a + fa * dd, vb = a + fb * dd;
-         const double e = dd == 0.0 ? 0.0 : 0x1p-21 * (fabs(a) + fabs(dd));
+         const double e = dd == 0.0 ? 0.0 : 0x1p-40 * (fabs(a) + fabs(dd));
          lo[c] = fmin(va, vb) - e;
          hi[c] = fmax(va, vb) + e;
       }
@@ -1122,9 +1131,9 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    const FT* __restrict__ src = (const FT*)F.data;
    const float off = F.offset;
 
-   KF acc[NV];
+   double acc[NV];
 #pragma unroll
-   for (int v = 0; v < NV; v++) acc[v] = KF{0.f, 0.f};
+   for (int v = 0; v < NV; v++) acc[v] = 0.0;
 
    // Each thread takes kUnroll pixels per step (kBlock apart) and issues their reference
    // records before using any, so several loads are in flight per wave. Pixels past the end
@@ -1145,11 +1154,6 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    for (int pb = pfirst; pb < ch.p1; pb += kUnroll * pstep) {
       float4 recs[kUnroll];
       float fs[kUnroll];
-      // plain FP32 sums over the kUnroll pixels of this step, folded into the compensated
-      // accumulators once per step
-      float part[NV];
-#pragma unroll
-      for (int v = 0; v < NV; v++) part[v] = 0.f;
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
          const int pu = min(pb + u * pstep, ch.p1 - 1);
@@ -1170,10 +1174,10 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          float wt[3];
 #pragma unroll
          for (int c = 0; c < M; c++) {
-            const float d = fmaf(f, Dd[c], Da[c]);
-            const float fl = floorf(d);
+            const double d = fma((double)f, Dd[c], Da[c]);
+            const double fl = floor(d);
             ti[c] = (int)fl;
-            wt[c] = d - fl;
+            wt[c] = (float)(d - fl);
          }
          const int x0 = w.x + cx + ti[0], y0 = w.y + ti[1];  // frame coordinates
          const float wx1 = wt[0], wx0 = 1.f - wt[0];
@@ -1233,21 +1237,19 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          // Jacobian row
          const float e = (in && v != 0.f) ? v - rec.x : 0.f;
          const float ej = (p != ch.end_p) ? e : 0.f;
-         part[0] = fmaf(e, e, part[0]);
-         const float omff = 1.f - f;
+         const double ed = (double)e;
+         acc[0] = fma(ed, ed, acc[0]);
+         const double fd = (double)f, omfd = 1.0 - fd;
          const float g[3] = {rec.y, rec.z, rec.w};
 #pragma unroll
          for (int c = 0; c < M; c++) {
-            const float ge = g[c] * ej;
-            part[1 + c] = fmaf(ge, omff, part[1 + c]);
-            part[1 + M + c] = fmaf(ge, f, part[1 + M + c]);
+            const double ge = (double)(g[c] * ej);
+            acc[1 + c] = fma(ge, omfd, acc[1 + c]);
+            acc[1 + M + c] = fma(ge, fd, acc[1 + M + c]);
          }
       }
-      if (MODE == MODE_PASS)
-#pragma unroll
-         for (int v = 0; v < NV; v++) kadd(acc[v], part[v]);
    }
-   if (MODE == MODE_PASS) block_reduce_fast<NV>(acc, (KF(*)[NV])sred, out);
+   if (MODE == MODE_PASS) block_reduce_fast<NV>(acc, (double(*)[NV])sred, out);
    else __syncthreads();
 }
 
@@ -1255,7 +1257,7 @@ template <int MODE>
 __device__ __forceinline__ void dispatch_chunk(const RefDev& R, const FrameDev& F,
                                                const Chunk ch, const double* D, float* lds,
                                                int lds_cap, bool use_lds, double* out,
-                                               float* wout, float invalid, KF (*sred)[7])
+                                               float* wout, float invalid, double (*sred)[7])
 {
    if (R.n_dim == 2) {
       if (F.dtype == FA_U16)
@@ -1282,7 +1284,7 @@ template <int M>
 __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a)
 {
    extern __shared__ float lds[];
-   __shared__ KF sred[kBlock / 64][7];
+   __shared__ double sred[kBlock / 64][7];
    if (a.counters->n_active_frames <= 0) return;
    const int64_t Wb = a.run_base[a.ref_lo];
    const int64_t W = a.run_base[a.ref_hi] - Wb;
@@ -1330,7 +1332,7 @@ __global__ __launch_bounds__(kBlock) void k_warp(RefDev R, FrameDev F, const dou
                                                  int use_lds)
 {
    extern __shared__ float lds[];
-   __shared__ KF sred[kBlock / 64][7];
+   __shared__ double sred[kBlock / 64][7];
    for (int c = blockIdx.x; c < R.n_chunks; c += gridDim.x) {
       dispatch_chunk<MODE_WARP>(R, F, R.chunks[c], D, lds, lds_cap, use_lds != 0, nullptr, out,
                                 invalid, sred);
@@ -1855,6 +1857,7 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
    const double f_new = reduce_ssd<NT>(R, P);
    TrState st = B.state[j];
    int accept = 0;
+   double pred = 0, meas = 0, rho = 0;  // of the step this evaluation tried
    if (st.phase == 0) {
       st.f = f_new;
       st.evaluations = 1;
@@ -1863,16 +1866,17 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
    } else {
       st.evaluations++;
       const double gp = block_dot<NT>(n, S.g, S.p);
-      const double pred = -(0.5 * block_quad<M, NT>(R, S.p) + gp);
-      const double meas = st.f - f_new;
+      pred = -(0.5 * block_quad<M, NT>(R, S.p) + gp);
+      meas = st.f - f_new;
       if (fabs(pred) <= fabs(meas) * DBL_EPSILON) {
          st.status = FA_STOP_NO_PROGRESS;
          st.done = 1;
       } else {
-         const double rho = meas / fabs(pred);
+         rho = meas / fabs(pred);
          if (!isfinite(rho)) {
             st.status = FA_STOP_NONFINITE;
             st.done = 1;
+            rho = 0;
          } else {
             int collapse = 0;
             if (rho < 0.25) {
@@ -1881,7 +1885,9 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
             } else if (rho > 0.75 && st.sub_iters > 1) {
                st.radius = 2 * st.radius < 1000 ? 2 * st.radius : 1000;
             }
-            if (rho > 0) {
+            if (collapse) {
+               // the radius has collapsed: stop where x is, without taking the step
+            } else if (rho > 0) {
                for (int i = threadIdx.x; i < n; i += NT) S.x[i] = S.x[i] + S.p[i];
                st.f = f_new;
                st.stale = 0;
@@ -1944,6 +1950,27 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
       for (int i = threadIdx.x; i < nb; i += NT)
 #pragma unroll
          for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < M ? S.x[i * M + c] + S.p[i * M + c] : 0.0;
+   }
+   if (B.trace && st.evaluations - 1 < B.trace_cap) {
+      // record of this evaluation (layout of kTraceFields), then x, g, p
+      __syncthreads();
+      const bool ran = !st.done;
+      const double pn = ran ? sqrt(block_dot<NT>(n, S.p, S.p)) : 0.0;
+      const double gn = sqrt(block_dot<NT>(n, S.g, S.g));
+      const double gp = ran ? block_dot<NT>(n, S.g, S.p) : 0.0;
+      double* o = B.trace + (int64_t)j * B.trace_cap * B.trace_rec + (int64_t)(st.evaluations - 1) * B.trace_rec;
+      if (threadIdx.x == 0) {
+         const double v[kTraceFields] = {(double)st.evaluations, f_new, st.f, pred, meas, rho, (double)accept,
+                                         st.radius, (double)st.iterations, st.done ? (double)st.status : 0.0,
+                                         ran ? st.lambda : 0.0, ran ? (double)st.sub_iters : 0.0, pn, gn, gp,
+                                         (double)st.rejected};
+         for (int i = 0; i < kTraceFields; i++) o[i] = v[i];
+      }
+      for (int i = threadIdx.x; i < n; i += NT) {
+         o[kTraceFields + i] = S.x[i];
+         o[kTraceFields + n + i] = S.g[i];
+         o[kTraceFields + 2 * n + i] = S.p[i];
+      }
    }
    if (threadIdx.x == 0) B.state[j] = st;
    if constexpr (LDS && FULL) {

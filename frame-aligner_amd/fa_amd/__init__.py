@@ -25,6 +25,11 @@ STOP_NAMES = {1: "abs_delta", 2: "rel_delta", 3: "max_iter", 4: "no_progress", 5
               6: "radius", 7: "max_evals"}
 
 
+# Fields of a trust-region trace record (fa_batch_set_trace), then x, g and p.
+TRACE_FIELDS = ["evals", "f_new", "f", "pred", "meas", "rho", "accept", "radius", "iterations",
+                "status", "lambda", "sub_iters", "pnorm", "gnorm", "gp", "rejected"]
+
+
 class FaError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{msg} (code {code})")
@@ -89,7 +94,8 @@ EXPORTS = [
     "fa_reference_release", "fa_reference_info", "fa_get_hessian", "fa_get_node_map",
     "fa_get_gradient", "fa_register_frame", "fa_deregister_frame", "fa_get_error",
     "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_image", "fa_warp_point", "fa_align", "fa_batch_create",
-    "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_get_tuning", "fa_batch_set_tuning",
+    "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_set_trace", "fa_batch_get_trace",
+    "fa_batch_get_tuning", "fa_batch_set_tuning",
     "fa_batch_run", "fa_batch_eval", "fa_align_batch",
     "fa_context_stream", "fa_context_mem_info", "fa_context_info", "fa_server_create", "fa_server_destroy",
     "fa_server_submit", "fa_server_submit_host", "fa_server_step", "fa_server_results",
@@ -134,6 +140,8 @@ def load_library(path=None):
     L.fa_batch_create.argtypes = [P, i32, C.POINTER(TrParams), C.POINTER(P)]
     L.fa_batch_destroy.argtypes = [P]
     L.fa_batch_set_timing.argtypes = [P, i32]
+    L.fa_batch_set_trace.argtypes = [P, i32]
+    L.fa_batch_get_trace.argtypes = [P, P]
     L.fa_batch_get_tuning.argtypes = [P, C.POINTER(Tuning)]
     L.fa_batch_set_tuning.argtypes = [P, C.POINTER(Tuning)]
     L.fa_batch_run.argtypes = [P, P, P, C.POINTER(BatchTiming)]
@@ -423,6 +431,27 @@ class Batch:
     def set_timing(self, on=True):
         _check(self.L.fa_batch_set_timing(self.h, 1 if on else 0))
         self._timing = bool(on)
+
+    def set_trace(self, max_records):
+        """Records every trust-region evaluation of the following runs (TRACE_FIELDS, then
+        x, g and p), at most max_records per frame; 0 turns tracing off."""
+        _check(self.L.fa_batch_set_trace(self.h, int(max_records)))
+        self._trace_cap = int(max_records)
+
+    def trace(self):
+        """Records of the last run: one (evaluations, TRACE_FIELDS + 3 n) array per frame."""
+        cap = getattr(self, "_trace_cap", 0)
+        nmax = max(self.sizes)
+        rec = len(TRACE_FIELDS) + 3 * nmax
+        out = np.zeros((len(self.frames), cap, rec))
+        _check(self.L.fa_batch_get_trace(self.h, _ptr(out)))
+        res = []
+        for i, n in enumerate(self.sizes):
+            k = int(np.count_nonzero(out[i, :, 0]))
+            r = out[i, :k]
+            res.append(np.concatenate([r[:, :len(TRACE_FIELDS)], r[:, len(TRACE_FIELDS):len(TRACE_FIELDS) + 3 * n]],
+                                      axis=1))
+        return res
 
     @property
     def tuning(self):

@@ -150,8 +150,11 @@ def with_offset(a):
 # Workloads of the benchmark configurations (nodes = n_resampling_points * n_z + 1,
 # FrameWarpAligner.cpp:100).
 CONFIGS = {
-    "c1": dict(shape=(1, 256, 256), n_frames=20, nD=11, n_blobs=200, dtype="float32"),
-    "c2": dict(shape=(1, 512, 512), n_frames=200, nD=31, n_blobs=600, mean=50.0, dtype="uint16"),
-    "c3": dict(shape=(32, 256, 256), n_frames=50, nD=16 * 32 + 1, n_blobs=1500,
-               sigma=(1.5, 4.0), dtype="float32"),
+    # CPU-warper workload: 256x256 float32, 20 frames, 10 resampling points
+    "2d_256_f32": dict(shape=(1, 256, 256), n_frames=20, nD=11, n_blobs=200, dtype="float32"),
+    # headline workload: 512x512 uint16 counts (~50 photons/px), 200 frames, 30 points
+    "2d_512_u16": dict(shape=(1, 512, 512), n_frames=200, nD=31, n_blobs=600, mean=50.0, dtype="uint16"),
+    # 3-D precision workload: 256x256x32 float32, 50 volumes, 16 points per plane
+    "3d_256x256x32_f32": dict(shape=(32, 256, 256), n_frames=50, nD=16 * 32 + 1, n_blobs=1500,
+                              sigma=(1.5, 4.0), dtype="float32"),
 }

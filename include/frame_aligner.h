@@ -187,6 +187,19 @@ int fa_batch_destroy(fa_batch* b);
    (fa_batch_timing); a batch large enough to run as two interleaved parts then runs its parts
    one after the other, so each pass is timed alone. Results are the same either way. */
 int fa_batch_set_timing(fa_batch* b, int32_t enable);
+/* Diagnostics, no counterpart in the reference: records every objective evaluation of the
+   trust region of each frame in the following runs (at most max_records per frame; 0 turns
+   tracing off). A record is FA_TRACE_FIELDS doubles - evaluations so far, f of this
+   evaluation, f at x after the decision, predicted and measured decrease of the step tried,
+   rho, accepted, radius after the decision, stop-strategy iterations, stop status (0 while
+   running), lambda and iterations of the next subproblem, |p| of the next step, |g| at x, g'p,
+   rejected steps so far - followed by x, g and p (nD*n_dim each) after the record's update,
+   padded to FA_TRACE_FIELDS + 3 * n_max doubles (n_max: the batch's largest nD*n_dim). */
+#define FA_TRACE_FIELDS 16
+int fa_batch_set_trace(fa_batch* b, int32_t max_records);
+/* out: n_frames x max_records records (caller's frame order) of the last run; records past a
+   frame's last evaluation are zero. */
+int fa_batch_get_trace(fa_batch* b, double* out);
 int fa_batch_get_tuning(const fa_batch* b, fa_tuning* t);
 int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t);
 /* x_inout: concatenated per-frame parameter vectors; stats: n entries (may be NULL). */

@@ -848,18 +848,17 @@ int orc_tr_subproblem_dense(int n, const double* B, const double* g, double radi
    return it;
 }
 
-/* p'Bp for the block-tridiagonal H. */
-static double quad_bt(const orc_warper* w, const double* p)
+/* p'Bp for a block-tridiagonal B (nb diagonal blocks of m x m, sub = blocks (k+1, k)). */
+static double quad_bt(int nb, int m, const double* diag, const double* sub, const double* p)
 {
-   const int m = w->n_dim, mm = m * m, nb = w->nD;
+   const int mm = m * m;
    double s = 0;
    for (int k = 0; k < nb; k++) {
       for (int a = 0; a < m; a++)
-         for (int b = 0; b < m; b++) s += p[k * m + a] * w->Hdiag[k * mm + a * m + b] * p[k * m + b];
+         for (int b = 0; b < m; b++) s += p[k * m + a] * diag[k * mm + a * m + b] * p[k * m + b];
       if (k + 1 < nb)
          for (int a = 0; a < m; a++)
-            for (int b = 0; b < m; b++)
-               s += 2.0 * p[(k + 1) * m + a] * w->Hsub[k * mm + a * m + b] * p[k * m + b];
+            for (int b = 0; b < m; b++) s += 2.0 * p[(k + 1) * m + a] * sub[k * mm + a * m + b] * p[k * m + b];
    }
    return s;
 }
@@ -876,6 +875,170 @@ void orc_tr_default(orc_tr_params* p)
    p->skip_repeat = 0;
 }
 
+/* One trace record per objective evaluation (layout in fa_oracle.h), then x, g, p. */
+static void put_rec(double* trace, int cap, int n, int r, const double* v, const double* x,
+                    const double* g, const double* p)
+{
+   if (!trace || r < 0 || r >= cap) return;
+   double* o = trace + (size_t)r * (ORC_TRACE_FIELDS + 3 * (size_t)n);
+   memcpy(o, v, sizeof(double) * ORC_TRACE_FIELDS);
+   memcpy(o + ORC_TRACE_FIELDS, x, sizeof(double) * (size_t)n);
+   memcpy(o + ORC_TRACE_FIELDS + n, g, sizeof(double) * (size_t)n);
+   memcpy(o + ORC_TRACE_FIELDS + 2 * (size_t)n, p, sizeof(double) * (size_t)n);
+}
+
+/* dlib find_min_trust_region with objective_delta_stop_strategy, as called at
+   FrameWarpAligner.cpp:205-209 with the OptimisationModel concept (OptimisationModel.h:6-27):
+   a model gives f(x) and the derivative at the last evaluated point, and a constant B.
+   Restated from dlib's published algorithm (Nocedal & Wright Alg. 4.1; dlib is not part of
+   the reference tree):
+     - the stop strategy runs at the top of every iteration whose x is new: iteration count
+       +1; from the second call on, stop when the count exceeds max_iter (if set) or
+       |f - f_prev| < min_delta; then f_prev = f;
+     - p from the subproblem at the current radius, f_new = f(x + p);
+     - stop when |pred| <= |meas| * DBL_EPSILON (no progress) or rho = meas / |pred| is not
+       finite;
+     - rho < 0.25: radius /= 4, and stop without moving x once the radius has collapsed;
+       otherwise rho > 0.75 with a subproblem that iterated more than once (the step was
+       held by the radius): radius = min(2 radius, 1000);
+     - rho > 0: x += p, new f and g; else x is stale (no stop check next time).
+   Deliberate differences, none of which changes the returned x beyond 1e-13 of the radius:
+   the radius counts as collapsed at radius0 * DBL_EPSILON (dlib: DBL_MIN, which costs up to
+   ~500 more rejected evaluations before the same stop), and the relative stop, evaluation
+   cap and skip_repeat are additions for the benchmark configurations. */
+int orc_tr_run(const orc_tr_model* M, const orc_tr_params* prm, double* x, orc_tr_stats* st,
+               double* trace, int cap)
+{
+   const int n = M->nb * M->m;
+   orc_tr_stats s = {0};
+   double* g = (double*)malloc(sizeof(double) * (size_t)n);
+   double* p = (double*)calloc((size_t)n, sizeof(double));
+   double* xt = (double*)malloc(sizeof(double) * (size_t)n);
+   mat_t A = {0};
+   A.n = n;
+   A.nb = M->nb;
+   A.m = M->m;
+   A.diag = M->diag;
+   A.sub = M->sub;
+   A.Ld = (double*)malloc(sizeof(double) * (size_t)n * M->m);
+   A.Ls = (double*)malloc(sizeof(double) * (size_t)n * M->m);
+
+   double f = M->value(M->user, x);
+   s.evaluations = 1;
+   M->gradient(M->user, g);
+   double radius = prm->radius;
+   const double r0 = radius;
+   int stale = 0, been_used = 0;
+   double prev_f = 0;
+   /* the evaluation the next trace record describes */
+   double rv[ORC_TRACE_FIELDS] = {0};
+   rv[ORC_T_FNEW] = f;
+   rv[ORC_T_ACCEPT] = 1;
+   for (;;) {
+      int status = 0;
+      if (!stale) {
+         s.iterations++;
+         if (been_used) {
+            double df = fabs(f - prev_f);
+            if (prm->max_iter && s.iterations > prm->max_iter) status = ORC_STOP_MAX_ITER;
+            else if (df < prm->min_delta) status = ORC_STOP_ABS_DELTA;
+            else if (prm->rel_delta > 0 && df < prm->rel_delta * fabs(prev_f)) status = ORC_STOP_REL_DELTA;
+         }
+         been_used = 1;
+         prev_f = f;
+      }
+      if (!status && prm->max_evals && s.evaluations >= prm->max_evals) status = ORC_STOP_MAX_EVALS;
+      double lam = 0;
+      int it = 0;
+      if (!status) it = tr_subproblem(&A, g, radius, prm->sub_eps, prm->sub_max_iter, p, &lam);
+      rv[ORC_T_EVALS] = s.evaluations;
+      rv[ORC_T_F] = f;
+      rv[ORC_T_RADIUS] = radius;
+      rv[ORC_T_ITER] = s.iterations;
+      rv[ORC_T_STATUS] = status;
+      rv[ORC_T_LAMBDA] = lam;
+      rv[ORC_T_SUBIT] = it;
+      rv[ORC_T_PNORM] = status ? 0 : norm2(n, p);
+      rv[ORC_T_GNORM] = norm2(n, g);
+      double gp = 0;
+      for (int i = 0; i < n; i++) gp += g[i] * p[i];
+      rv[ORC_T_GP] = status ? 0 : gp;
+      rv[ORC_T_REJ] = s.rejected;
+      put_rec(trace, cap, n, s.evaluations - 1, rv, x, g, p);
+      if (status) {
+         s.status = status;
+         break;
+      }
+      for (int i = 0; i < n; i++) xt[i] = x[i] + p[i];
+      double fn = M->value(M->user, xt);
+      s.evaluations++;
+      double pred = -(0.5 * quad_bt(M->nb, M->m, M->diag, M->sub, p) + gp);
+      double meas = f - fn;
+      memset(rv, 0, sizeof rv);
+      rv[ORC_T_FNEW] = fn;
+      rv[ORC_T_PRED] = pred;
+      rv[ORC_T_MEAS] = meas;
+      if (fabs(pred) <= fabs(meas) * DBL_EPSILON) status = ORC_STOP_NO_PROGRESS;
+      double rho = status ? 0 : meas / fabs(pred);
+      if (!status && !isfinite(rho)) status = ORC_STOP_NONFINITE;
+      if (!status) {
+         rv[ORC_T_RHO] = rho;
+         if (rho < 0.25) {
+            radius *= 0.25;
+            if (radius <= r0 * DBL_EPSILON) status = ORC_STOP_RADIUS; /* x stays */
+         } else if (rho > 0.75 && it > 1) {
+            radius = 2 * radius < 1000 ? 2 * radius : 1000;
+         }
+      }
+      if (!status) {
+         if (rho > 0) {
+            memcpy(x, xt, sizeof(double) * (size_t)n);
+            f = fn;
+            M->gradient(M->user, g);
+            stale = 0;
+            rv[ORC_T_ACCEPT] = 1;
+         } else {
+            stale = 1;
+            s.rejected++;
+            if (prm->skip_repeat && lam == 0) {
+               /* The unchanged interior step would be proposed, evaluated and rejected again
+                  while it fits the shrunken region. */
+               double pn = norm2(n, p);
+               while (pn <= radius) {
+                  radius *= 0.25;
+                  s.rejected++;
+                  if (radius <= r0 * DBL_EPSILON) {
+                     status = ORC_STOP_RADIUS;
+                     break;
+                  }
+               }
+            }
+         }
+      }
+      if (status) {
+         rv[ORC_T_EVALS] = s.evaluations;
+         rv[ORC_T_F] = f;
+         rv[ORC_T_RADIUS] = radius;
+         rv[ORC_T_ITER] = s.iterations;
+         rv[ORC_T_STATUS] = status;
+         rv[ORC_T_GNORM] = norm2(n, g);
+         rv[ORC_T_REJ] = s.rejected;
+         put_rec(trace, cap, n, s.evaluations - 1, rv, x, g, p);
+         s.status = status;
+         break;
+      }
+   }
+   s.f_final = f;
+   s.radius_final = radius;
+   if (st) *st = s;
+   free(g);
+   free(p);
+   free(xt);
+   free(A.Ld);
+   free(A.Ls);
+   return 0;
+}
+
 /* col2D (OptimisationModel.cpp:29-42): x = [x0, y0, (z0), x1, ...] -> D (nD x 3). */
 static void col2D(const orc_warper* w, const double* x, double* D)
 {
@@ -886,107 +1049,55 @@ static void col2D(const orc_warper* w, const double* x, double* D)
    }
 }
 
-/* dlib find_min_trust_region with objective_delta_stop_strategy, as called at
-   FrameWarpAligner.cpp:205-209, over OptimisationModel (OptimisationModel.cpp:45-63):
-   f = SSD (getError), g = Jtr (getJacobian at the same point), B = H (constant).
-   Restated from dlib's published algorithm (dlib is not part of the reference tree). */
+/* OptimisationModel over one frame (OptimisationModel.cpp:45-63): operator() = getError at
+   col2D(x), keeping the error image; get_derivative_and_hessian = getJacobian from that error
+   image (the last getError on the frame) and the warper's H. */
+typedef struct {
+   const orc_warper* w;
+   const float* frame;
+   double* D;
+   double* err;
+} warp_model_t;
+
+static double warp_model_value(void* u, const double* x)
+{
+   warp_model_t* m = (warp_model_t*)u;
+   col2D(m->w, x, m->D);
+   return orc_get_error(m->w, m->frame, m->D, m->err, NULL);
+}
+
+static void warp_model_gradient(void* u, double* g)
+{
+   warp_model_t* m = (warp_model_t*)u;
+   jacobian(m->w, m->err, g);
+}
+
+int orc_align_trace(const orc_warper* w, const float* frame, const orc_tr_params* prm, double* x,
+                    orc_tr_stats* st, double* trace, int cap)
+{
+   warp_model_t wm;
+   wm.w = w;
+   wm.frame = frame;
+   wm.D = (double*)malloc(sizeof(double) * 3 * (size_t)w->nD);
+   wm.err = (double*)malloc(sizeof(double) * (size_t)w->n_px);
+   orc_tr_model M;
+   M.nb = w->nD;
+   M.m = w->n_dim;
+   M.diag = w->Hdiag;
+   M.sub = w->Hsub;
+   M.value = warp_model_value;
+   M.gradient = warp_model_gradient;
+   M.user = &wm;
+   int rc = orc_tr_run(&M, prm, x, st, trace, cap);
+   free(wm.D);
+   free(wm.err);
+   return rc;
+}
+
 int orc_align(const orc_warper* w, const float* frame, const orc_tr_params* prm, double* x,
               orc_tr_stats* st)
 {
-   const int n = w->nD * w->n_dim;
-   const size_t npx = (size_t)w->n_px;
-   orc_tr_stats s = {0};
-   double* D = (double*)malloc(sizeof(double) * 3 * (size_t)w->nD);
-   double* err = (double*)malloc(sizeof(double) * npx);
-   double* g = (double*)malloc(sizeof(double) * (size_t)n);
-   double* p = (double*)malloc(sizeof(double) * (size_t)n);
-   double* xt = (double*)malloc(sizeof(double) * (size_t)n);
-   mat_t A = {0};
-   A.n = n;
-   A.nb = w->nD;
-   A.m = w->n_dim;
-   A.diag = w->Hdiag;
-   A.sub = w->Hsub;
-   A.Ld = (double*)malloc(sizeof(double) * (size_t)n * w->n_dim);
-   A.Ls = (double*)malloc(sizeof(double) * (size_t)n * w->n_dim);
-
-   col2D(w, x, D);
-   double f = orc_get_error(w, frame, D, err, NULL);
-   s.evaluations = 1;
-   jacobian(w, err, g);
-   double radius = prm->radius;
-   const double r0 = radius;
-   int stale = 0, been_used = 0;
-   double prev_f = 0;
-   for (;;) {
-      if (!stale) {
-         s.iterations++;
-         if (been_used) {
-            double df = fabs(f - prev_f);
-            if (prm->max_iter && s.iterations > prm->max_iter) { s.status = ORC_STOP_MAX_ITER; break; }
-            if (df < prm->min_delta) { s.status = ORC_STOP_ABS_DELTA; break; }
-            if (prm->rel_delta > 0 && df < prm->rel_delta * fabs(prev_f)) { s.status = ORC_STOP_REL_DELTA; break; }
-         }
-         been_used = 1;
-         prev_f = f;
-      }
-      double lam;
-      int it = tr_subproblem(&A, g, radius, prm->sub_eps, prm->sub_max_iter, p, &lam);
-      if (prm->max_evals && s.evaluations >= prm->max_evals) { s.status = ORC_STOP_MAX_EVALS; break; }
-      for (int i = 0; i < n; i++) xt[i] = x[i] + p[i];
-      col2D(w, xt, D);
-      double* err_t = (double*)malloc(sizeof(double) * npx);
-      double fn = orc_get_error(w, frame, D, err_t, NULL);
-      s.evaluations++;
-      double gp = 0;
-      for (int i = 0; i < n; i++) gp += g[i] * p[i];
-      double pred = -(0.5 * quad_bt(w, p) + gp);
-      double meas = f - fn;
-      if (fabs(pred) <= fabs(meas) * DBL_EPSILON) { free(err_t); s.status = ORC_STOP_NO_PROGRESS; break; }
-      double rho = meas / fabs(pred);
-      if (!isfinite(rho)) { free(err_t); s.status = ORC_STOP_NONFINITE; break; }
-      int collapse = 0;
-      if (rho < 0.25) {
-         radius *= 0.25;
-         if (radius <= r0 * DBL_EPSILON) collapse = 1;
-      } else if (rho > 0.75 && it > 1) {
-         radius = 2 * radius < 1000 ? 2 * radius : 1000;
-      }
-      if (rho > 0) {
-         memcpy(x, xt, sizeof(double) * (size_t)n);
-         f = fn;
-         free(err);
-         err = err_t;
-         jacobian(w, err, g);
-         stale = 0;
-      } else {
-         free(err_t);
-         stale = 1;
-         s.rejected++;
-         if (prm->skip_repeat && lam == 0 && !collapse) {
-            /* The unchanged interior step would be proposed, evaluated and rejected again
-               while it fits the shrunken region. */
-            double pn = norm2(n, p);
-            while (pn <= radius) {
-               radius *= 0.25;
-               s.rejected++;
-               if (radius <= r0 * DBL_EPSILON) { collapse = 1; break; }
-            }
-         }
-      }
-      if (collapse) { s.status = ORC_STOP_RADIUS; break; }
-   }
-   s.f_final = f;
-   s.radius_final = radius;
-   if (st) *st = s;
-   free(D);
-   free(err);
-   free(g);
-   free(p);
-   free(xt);
-   free(A.Ld);
-   free(A.Ls);
-   return 0;
+   return orc_align_trace(w, frame, prm, x, st, NULL, 0);
 }
 
 typedef struct {

@@ -107,6 +107,49 @@ void orc_tr_default(orc_tr_params* p);
 /* Trust-region alignment of one frame from x_inout (nD*n_dim, D2col layout). */
 int orc_align(const orc_warper* w, const float* frame, const orc_tr_params* prm,
               double* x_inout, orc_tr_stats* stats);
+
+/* Per-evaluation trace of the trust region. Record r describes objective evaluation r + 1
+   (r = 0: the start point) and the state after it was processed: the decision on it, the stop
+   check that followed and the next subproblem. Each record is ORC_TRACE_FIELDS doubles, then
+   x, g and p (n = nD*n_dim each) after the record's update. */
+enum {
+   ORC_T_EVALS = 0,   /* evaluations so far (r + 1) */
+   ORC_T_FNEW = 1,    /* f of this evaluation */
+   ORC_T_F = 2,       /* f at the current x after the decision */
+   ORC_T_PRED = 3,    /* predicted decrease of the step evaluated (0 for the start) */
+   ORC_T_MEAS = 4,    /* measured decrease */
+   ORC_T_RHO = 5,     /* meas / |pred| */
+   ORC_T_ACCEPT = 6,  /* 1 if the evaluated point became x */
+   ORC_T_RADIUS = 7,  /* radius after the decision */
+   ORC_T_ITER = 8,    /* stop-strategy iterations after the stop check */
+   ORC_T_STATUS = 9,  /* ORC_STOP_* if the loop ended here, else 0 */
+   ORC_T_LAMBDA = 10, /* lambda of the next subproblem (0 if none ran) */
+   ORC_T_SUBIT = 11,  /* iterations of the next subproblem (0 if none ran) */
+   ORC_T_PNORM = 12,  /* |p| of the next step (0 if none) */
+   ORC_T_GNORM = 13,  /* |g| at x */
+   ORC_T_GP = 14,     /* g'p of the next step */
+   ORC_T_REJ = 15,    /* rejected steps so far */
+   ORC_TRACE_FIELDS = 16
+};
+/* orc_align with a trace of at most cap records (trace: cap x (ORC_TRACE_FIELDS + 3n)). */
+int orc_align_trace(const orc_warper* w, const float* frame, const orc_tr_params* prm,
+                    double* x_inout, orc_tr_stats* stats, double* trace, int cap);
+
+/* The trust-region loop over any model of the OptimisationModel concept
+   (OptimisationModel.h:6-27) with a constant block-tridiagonal B, for tests that drive the
+   loop's control flow with closed-form objectives. value(x) returns f(x); gradient(g) gives
+   the derivative at the point of the last value() call whose point was accepted (the loop
+   calls it right after such a value()). */
+typedef struct orc_tr_model {
+   int nb, m;          /* n = nb*m parameters; B: nb diagonal blocks of m x m */
+   const double* diag; /* nb x m*m, row-major */
+   const double* sub;  /* nb-1 blocks (k+1, k), m*m each */
+   double (*value)(void* user, const double* x);
+   void (*gradient)(void* user, double* g);
+   void* user;
+} orc_tr_model;
+int orc_tr_run(const orc_tr_model* model, const orc_tr_params* prm, double* x_inout,
+               orc_tr_stats* stats, double* trace, int cap);
 /* Aligns n frames of n_px each with n_threads workers (one frame per task). */
 int orc_align_many(const orc_warper* w, const float* frames, int n_frames,
                    const orc_tr_params* prm, double* x_inout, orc_tr_stats* stats,

@@ -31,6 +31,20 @@ class TrStats(C.Structure):
                 ("status", C.c_int), ("f_final", C.c_double), ("radius_final", C.c_double)]
 
 
+VALUE_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.POINTER(C.c_double))
+GRADIENT_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double))
+
+
+class TrModel(C.Structure):
+    _fields_ = [("nb", C.c_int), ("m", C.c_int), ("diag", C.c_void_p), ("sub", C.c_void_p),
+                ("value", VALUE_FN), ("gradient", GRADIENT_FN), ("user", C.c_void_p)]
+
+
+# Per-evaluation trace record of the trust region (fa_oracle.h ORC_T_*), then x, g, p.
+TRACE_FIELDS = ["evals", "f_new", "f", "pred", "meas", "rho", "accept", "radius", "iterations",
+                "status", "lambda", "sub_iters", "pnorm", "gnorm", "gp", "rejected"]
+N_TRACE = len(TRACE_FIELDS)
+
 _lib = None
 
 
@@ -63,6 +77,8 @@ def lib():
         L.orc_warp_point.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, P]
         L.orc_tr_default.argtypes = [C.POINTER(TrParams)]
         L.orc_align.argtypes = [P, P, C.POINTER(TrParams), P, C.POINTER(TrStats)]
+        L.orc_align_trace.argtypes = [P, P, C.POINTER(TrParams), P, C.POINTER(TrStats), P, C.c_int]
+        L.orc_tr_run.argtypes = [C.POINTER(TrModel), C.POINTER(TrParams), P, C.POINTER(TrStats), P, C.c_int]
         L.orc_align_many.argtypes = [P, P, C.c_int, C.POINTER(TrParams), P, C.POINTER(TrStats),
                                      P, C.c_int]
         L.orc_bt_cholesky.argtypes = [C.c_int, C.c_int, P, P, C.c_double, P, P]
@@ -217,6 +233,18 @@ class Oracle:
         self.L.orc_align(self.h, _ptr(f), C.byref(p), _ptr(x), C.byref(s))
         return x, stats_dict(s)
 
+    def align_trace(self, frame, cap=64, x0=None, **kw):
+        """align() with the per-evaluation trace: (x, stats, records), records of shape
+        (evaluations, N_TRACE + 3n) (fields TRACE_FIELDS, then x, g, p after each record)."""
+        f = self._frame(frame)
+        n = self.nD * self.n_dim
+        x = np.zeros(n) if x0 is None else np.array(x0, np.float64).copy()
+        p = tr_params(**kw)
+        s = TrStats()
+        tr = np.zeros((cap, N_TRACE + 3 * n))
+        self.L.orc_align_trace(self.h, _ptr(f), C.byref(p), _ptr(x), C.byref(s), _ptr(tr), cap)
+        return x, stats_dict(s), tr[:min(cap, s.evaluations)]
+
     def align_many(self, frames, x0=None, threads=None, **kw):
         frames = np.ascontiguousarray(frames, dtype=np.float32)
         nf = frames.shape[0]
@@ -266,3 +294,32 @@ def tr_subproblem_dense(B, g, radius, eps=0.1, max_iter=20):
                                        _ptr(np.ascontiguousarray(g, np.float64)), radius, eps,
                                        max_iter, _ptr(p), C.byref(lam))
     return p, lam.value, it
+
+
+def tr_run(diag, sub, value, gradient, x0, cap=64, **kw):
+    """The oracle's trust-region loop over a closed-form model: value(x) -> f, gradient(x) -> g
+    (numpy), constant block-tridiagonal B (diag: nb x m x m, sub: nb-1 x m x m). Returns
+    (x, stats, records) as Oracle.align_trace."""
+    diag = np.ascontiguousarray(diag, np.float64)
+    nb, m, _ = diag.shape
+    sub = np.ascontiguousarray(sub, np.float64).reshape(max(nb - 1, 0), m, m)
+    subp = sub if sub.size else np.zeros((1, m, m))
+    n = nb * m
+    last = {}
+
+    def _value(_u, xp):
+        xv = np.ctypeslib.as_array(xp, shape=(n,)).copy()
+        last["x"] = xv
+        return float(value(xv))
+
+    def _gradient(_u, gp):
+        np.ctypeslib.as_array(gp, shape=(n,))[:] = gradient(last["x"])
+
+    vf, gf = VALUE_FN(_value), GRADIENT_FN(_gradient)
+    M = TrModel(nb, m, _ptr(diag), _ptr(subp), vf, gf, None)
+    x = np.array(x0, np.float64).reshape(n).copy()
+    p = tr_params(**kw)
+    s = TrStats()
+    tr = np.zeros((cap, N_TRACE + 3 * n))
+    lib().orc_tr_run(C.byref(M), C.byref(p), _ptr(x), C.byref(s), _ptr(tr), cap)
+    return x, stats_dict(s), tr[:min(cap, s.evaluations)]
