@@ -19,12 +19,15 @@
  *              rounding to float is exact for + - * /, since 53 >= 2*24+2.)
  *   fp64 == 1  the same algorithm with no float rounding anywhere.
  *
- * Parity pins (see tests/test_oracle_*.py): closed forms for the scan timing,
- * node map and Sobel ramps; brute-force per-pixel JtJ / Jtr sums; exact
- * warp identities (zero, integer and half-pixel shifts); SSD zero at the
- * truth; trust-region solutions of quadratics with known minimisers; dense vs
- * block-tridiagonal solver agreement.
- * Parity unpinned: the dlib trust-region trajectory itself (dlib is not
+ * Parity pins (tests/test_oracle_pins.py, tests/test_tr_loop_pins.py): closed
+ * forms for the scan timing, node map and Sobel ramps; brute-force per-pixel
+ * JtJ / Jtr sums; exact warp identities (zero, integer and half-pixel shifts);
+ * SSD zero at the truth; trust-region subproblem solutions against the
+ * secular equation; the trust-region loop's control flow (thresholds, radius
+ * factors and cap, expansion rule, stale steps, collapse, stops) on
+ * closed-form models; dense vs block-tridiagonal solver agreement.
+ * Parity unpinned: the lambda safeguarding inside dlib's subproblem solver in
+ * several dimensions and the radius at which dlib gives up (dlib is not
  * available here and its version is not pinned by the reference), and the
  * -ffast-math reorderings of the reference's Release build.
  */
