@@ -65,6 +65,7 @@ def isolated_pass(F, frames, tr, reps=10):
     """Mean duration of one fused pass over all frames (a one-part batch, every frame active,
     max_evals = 1 so each run is exactly one pass), from HIP events on the batch's stream."""
     b = F.Batch(frames[:63], **dict(tr, max_evals=1))  # < 64 frames: one stream, no overlap
+    b.set_tuning(check_every=1)  # one round per poll: every timed launch is the full pass
     b.set_timing(True)
     b.run()
     ms, nbytes, k = 0.0, 0.0, 0

@@ -326,9 +326,13 @@ void order_refs(std::vector<fa_reference*>& rs)
 }
 
 // Fused-pass workgroups whose LDS tiles fit one compute unit's LDS at once (at most 8).
-int64_t resident_blocks(const fa_context* ctx, int64_t lds_floats)
+// Pass workgroups of a class resident per compute unit by LDS: the staging tile, at least the
+// work item's reduction buffer (1 + 2 n_dim doubles per thread, aliasing the tile).
+int64_t resident_blocks(const fa_context* ctx, int64_t lds_floats, int n_dim)
 {
-   return std::min<int64_t>(8, (int64_t)ctx->lds_per_block / (lds_floats * (int64_t)sizeof(float) + 1024));
+   const int64_t bytes = std::max<int64_t>(lds_floats * (int64_t)sizeof(float),
+                                           (int64_t)(1 + 2 * n_dim) * kBlock * (int64_t)sizeof(double));
+   return std::min<int64_t>(8, (int64_t)ctx->lds_per_block / (bytes + 1024));
 }
 
 // Launch classes over the ordered references rs. Pass: a class ends at a change of kind or
@@ -343,7 +347,8 @@ void make_classes(fa_batch* b, const std::vector<fa_reference*>& rs, const std::
    for (int i = 0; i < (int)rs.size(); i++) {
       const int64_t need = lds_floats_for(rs[i]);
       if (b->pcls.empty() || b->pcls.back().n_dim != rs[i]->n_dim ||
-          resident_blocks(b->ctx, need) < resident_blocks(b->ctx, b->pcls.back().lds_first)) {
+          resident_blocks(b->ctx, need, rs[i]->n_dim) <
+              resident_blocks(b->ctx, b->pcls.back().lds_first, rs[i]->n_dim)) {
          fa_batch::PassClass c;
          c.n_dim = rs[i]->n_dim;
          c.ref_lo = i;

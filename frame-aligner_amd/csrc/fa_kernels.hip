@@ -8,8 +8,10 @@
 // the trust-region solve are FP64.
 #include "fa_kernels.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 namespace fa {
 
@@ -185,6 +187,19 @@ __device__ __forceinline__ float fload(const FrameDev& F, int64_t i)
    return (float)((const FT*)F.data)[i] + F.offset;
 }
 
+// p[i] through a global-address-space pointer with a 32-bit index: the compiler then issues a
+// global load with a scalar base and a 32-bit offset instead of a flat load, whose completion
+// is also counted against LDS operations.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p, unsigned i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+   return ((const __attribute__((address_space(1))) T*)p)[i];
+#else
+   return p[i];  // host pass: never executed
+#endif
+}
+
 // Pixel coordinates of the pixels a thread visits in [p0, p1) of one plane: start at
 // p0 + tid and advance by kBlock.
 struct PixelWalk {
@@ -200,15 +215,6 @@ struct PixelWalk {
    {
       x += kBlock;
       while (x >= nx) { x -= nx; y++; }
-   }
-   // The same step without a loop: kBlock = sdy rows + sdx columns with sdx < nx.
-   __device__ __forceinline__ void step(int nx, int sdx, int sdy)
-   {
-      x += sdx;
-      y += sdy;
-      const bool wrap = x >= nx;
-      x = wrap ? x - nx : x;
-      y = wrap ? y + 1 : y;
    }
 };
 
@@ -848,11 +854,8 @@ __global__ __launch_bounds__(kEigThreads) void k_eigen(RefDev R, double* Q, doub
 // reach are staged in LDS when they fit.
 enum { MODE_PASS = 0, MODE_WARP = 1 };
 
-// Sum over the 64 lanes of a wave with DPP lane moves (no LDS traffic), in FP64: each step
-// moves the two 32-bit halves of the partner lane's value and adds. Pairs and quads
-// (quad_perm), the two halves of each row of 8 and of 16 (row_half_mirror, row_mirror), then
-// row 0 into row 1 and row 2 into row 3 (row_bcast15), and rows 0-1 into rows 2-3
-// (row_bcast31). The total is in lane 63; the order of the additions is fixed.
+// v plus the value of the lane that DPP control CTRL selects, in FP64 (the two 32-bit halves
+// are moved separately).
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ double dpp_add(double v)
 {
@@ -860,42 +863,6 @@ __device__ __forceinline__ double dpp_add(double v)
    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, false);
    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xf, false);
    return v + __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-__device__ __forceinline__ double wave_total(double v)
-{
-   v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
-   v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
-   v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
-   v = dpp_add<0x140, 0xf>(v);  // row_mirror
-   v = dpp_add<0x142, 0xa>(v);  // row_bcast15 into rows 1 and 3
-   v = dpp_add<0x143, 0xc>(v);  // row_bcast31 into rows 2 and 3
-   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
-   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
-   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// Block reduction of the fused pass: every thread's FP64 sums, over the lanes of each wave
-// (wave_total), then over the waves in wave order.
-template <int NV>
-__device__ __forceinline__ void block_reduce_fast(double (&acc)[NV], double (*sred)[NV], double* out)
-{
-   double w[NV];
-#pragma unroll
-   for (int v = 0; v < NV; v++) w[v] = wave_total(acc[v]);
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   if (lane == 0)
-#pragma unroll
-      for (int v = 0; v < NV; v++) sred[wave][v] = w[v];
-   __syncthreads();
-   if (threadIdx.x < NV) {
-      const int v = threadIdx.x;
-      double t = 0;
-      for (int q = 0; q < kBlock / 64; q++) t += sred[q][v];
-      out[v] = t;
-   }
-   __syncthreads();
 }
 
 // Frame samples a work item can reach, staged in LDS: planes [za, za+nzs) and rows
@@ -1054,6 +1021,40 @@ __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, 
    __syncthreads();
 }
 
+// The work item's sums (SSD, then per component sum(g e) and sum((g e) f)) over the workgroup,
+// through LDS, written to out as SSD, Jtr of node k (per component), Jtr of node k + 1. Every
+// thread stores its values (red: NV x kBlock doubles), then lanes 8v .. 8v + 7 of the first
+// wave each add a fixed eighth of value v's column and the eight partial sums are combined
+// with DPP moves within each group of 8 lanes (lane pairs, then quad pairs, then the two quads
+// by row_half_mirror), so every value is summed in a fixed order and a frame's sums do not
+// depend on which work items or frames share the launch. Only the first wave does
+// floating-point work; the other waves store and go on. red may alias the staging tile: the
+// first barrier waits for every wave to finish sampling, and the caller's next staging
+// follows the second.
+template <int M>
+__device__ __forceinline__ void block_reduce_lds(const double (&acc)[1 + 2 * M], double* red, double* out)
+{
+   constexpr int NV = 1 + 2 * M;
+   __syncthreads();
+#pragma unroll
+   for (int v = 0; v < NV; v++) red[v * kBlock + threadIdx.x] = acc[v];
+   __syncthreads();
+   if (threadIdx.x < 64) {
+      const int v = threadIdx.x >> 3, s = threadIdx.x & 7;
+      double t = 0;
+      if (v < NV)
+#pragma unroll 8
+         for (int i = 0; i < kBlock / 8; i++) t += red[v * kBlock + s + 8 * i];
+      t = dpp_add<0xb1, 0xf>(t);   // quad_perm [1,0,3,2]
+      t = dpp_add<0x4e, 0xf>(t);   // quad_perm [2,3,0,1]
+      t = dpp_add<0x141, 0xf>(t);  // row_half_mirror: the other quad of the 8
+      // node k's Jtr rows are sum(g e) - sum((g e) f), the latter held 8 M lanes up
+      const double tm = __shfl(t, (int)((threadIdx.x + 8 * M) & 63), 64);
+      if (s == 0 && v < NV) out[v] = (v >= 1 && v <= M) ? t - tm : t;
+   }
+   __syncthreads();
+}
+
 // For every pixel of a work item: displacement d = f D[k+1] + (1-f) D[k] (warpPoint,
 // FrameWarper.cpp:397-419), zero-padded bilinear/trilinear pull-sample of the frame at p + d
 // (warpImageInterpolated, FrameWarper.cpp:31-69), residual against the reference and SSD over
@@ -1064,18 +1065,26 @@ __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, 
 // relative to the pixel (one FP64 fma per component from the float node fraction f), split
 // into an integer tap offset and a fraction; D[k+1] = D[k] gives D[k] exactly (integer shifts
 // sample exactly). The interpolation, the residual e and the products with the gradient are
-// FP32. Each thread sums its pixels' terms in FP64 (e*e, and (g e)(1-f), (g e) f with g e a
-// float product, formed exactly in FP64 by the fma), the waves and the workgroup in FP64 in a
-// fixed order. On gfx950 an FP64 fma issues at the rate of an unpacked FP32 one, so these
-// sums cost about what compensated FP32 sums of 4-pixel FP32 partials did, and they are as
-// accurate as the reference's double sums. (Displacements rounded to float move the sample
-// point by ~1e-7 px per node, and the FP32 partial sums lose ~1e-8 of Jtr; both are small
-// against the 1e-3 tolerance, but the trust region amplifies them: on 256^2 x 32 volumes with
-// 513 nodes it took a different path from the reference's arithmetic.)
+// FP32. Each thread sums its pixels' terms in FP64 (e*e, g e and (g e) f with g e a float
+// product, formed exactly in FP64 by the fma; the Jtr row of node k is sum(g e) - sum((g e) f)),
+// the workgroup in FP64 in a fixed order. On gfx950 an FP64 fma issues at the rate of an
+// unpacked FP32 one, so these sums cost about what compensated FP32 sums of 4-pixel FP32
+// partials did, and they are as accurate as the reference's double sums. (Displacements
+// rounded to float move the sample point by ~1e-7 px per node, and FP32 partial sums lose ~1e-8
+// of Jtr; both are small against the 1e-3 tolerance, but the trust region amplifies them: on
+// 256^2 x 32 volumes with 513 nodes it took a different path from the reference's arithmetic.)
+//
+// Thread layout. A work item is the pixels of rows r0..r1 of one plane and columns
+// [x0, x0 + w) between p0 and p1 - 1 in scan order. Each thread keeps one column (w >= kBlock:
+// columns kBlock apart; narrower items run kBlock / w rows side by side) and walks down its
+// rows, so a pixel's record, frame row and tile row advance by constants; the column's first and
+// last rows follow from p0 and p1 (an item may start and end mid-row). kUnroll rows per step:
+// their records are loaded before any is used; rows past the column's last are computed on
+// its last row and masked.
 template <int M, typename FT, int MODE>
 __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
                            const double* __restrict__ D, float* lds, int lds_cap, bool want_lds,
-                           double* out, float* wout, float invalid, double (*sred)[7])
+                           double* out, float* wout, float invalid)
 {
    constexpr int NV = 1 + 2 * M;
    const int nx = R.nx, ny = R.ny, nz = R.nz;
@@ -1091,8 +1100,8 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
       Dd[c] = d1 - d0;
    }
    const int plane = (int)(ch.p0 / plane_sz);
-   const int q0 = (int)(ch.p0 - plane * plane_sz);
-   const int r0 = q0 / nx, r1 = (int)((ch.p1 - 1 - plane * plane_sz) / nx);
+   const int pb = plane * (int)plane_sz;
+   const int r0 = (ch.p0 - pb) / nx, r1 = (ch.p1 - 1 - pb) / nx;
 
    // f rises with the pixel's scan time, so over the work item it lies in [f(p0), f(p1-1)]
    // and d between its values there; the margin covers the rounding of the per-pixel fma
@@ -1124,132 +1133,137 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          use = (int64_t)S.nzs * S.nrows * S.stride <= lds_cap;
       }
    }
-   const int cx = ch.x0;  // first column of the item (0 for whole-width items)
-   if (use) stage_frame<M == 3 ? 8 : 6, FT>(R, F, S, lds, cx, ch.w);
-   const int lds_lim = S.nzs * S.nrows * S.stride - (M == 3 ? S.nrows * S.stride : 0) - S.stride - 2;
+   const int cx = ch.x0, cw = ch.w;  // columns of the item (cx = 0, cw = nx: whole rows)
+   if (use) stage_frame<M == 3 ? 8 : 6, FT>(R, F, S, lds, cx, cw);
+   const int plane_step = S.nrows * S.stride;  // tile floats between staged planes
 
    const FT* __restrict__ src = (const FT*)F.data;
    const float off = F.offset;
 
+   // SSD, then per component sum(g e) and sum((g e) f)
    double acc[NV];
 #pragma unroll
    for (int v = 0; v < NV; v++) acc[v] = 0.0;
 
-   // Each thread takes kUnroll pixels per step (kBlock apart) and issues their reference
-   // records before using any, so several loads are in flight per wave. Pixels past the end
-   // of the work item are computed on clamped data and masked, so the loop body has no
-   // divergent branches.
-   //
-   // The walk runs over the item's own width vw: the frame width for whole-width items (a
-   // thread's pixels are kBlock apart in scan order), kBlock for strips (a thread keeps its
-   // column and its pixels are one row apart). w.x is the column within the item.
    constexpr int kUnroll = 4;
-   const bool whole = ch.w == nx;
-   const int vw = whole ? nx : kBlock;
-   const int sdy = kBlock / vw, sdx = kBlock - sdy * vw;
-   const int pstep = whole ? kBlock : nx;
-   PixelWalk w;
-   w.init((whole ? q0 : (q0 / nx) * kBlock + (q0 % nx - cx)) + threadIdx.x, vw);
-   const int pfirst = plane * (int)plane_sz + w.y * nx + w.x + cx;
-   for (int pb = pfirst; pb < ch.p1; pb += kUnroll * pstep) {
-      float4 recs[kUnroll];
-      float fs[kUnroll];
+   const int rpar = cw >= kBlock ? 1 : kBlock / cw;
+   const int ro = cw >= kBlock ? 0 : (int)threadIdx.x / cw;
+   // the loop is compiled once per sampler (staged tile or global loads), so the choice, which
+   // is uniform over the work item, costs no instructions per pixel
+   auto rows_loop = [&](auto use_tile) {
+      constexpr bool TILE = decltype(use_tile)::value;
+      for (int c = cx + (int)threadIdx.x - ro * cw; c < cx + cw; c += kBlock) {
+         const int rf = (pb + r0 * nx + c >= ch.p0 ? r0 : r0 + 1) + ro;
+         const int rl = pb + r1 * nx + c < ch.p1 ? r1 : r1 - 1;
+         const int colk = c - cx + kPad;  // tile column of the pixel
+         // tile sampling takes kUnroll rows per step; global sampling (4 or 8 loads per pixel
+         // already in flight) one, which keeps it within the register budget
+         constexpr int kU = TILE ? kUnroll : 1;
+         for (int r = rf; r <= rl; r += kU * rpar) {
+            float4 recs[kU];
+            float fs[kU];
+            int rows[kU];
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-         const int pu = min(pb + u * pstep, ch.p1 - 1);
-         recs[u] = R.rg4[pu];
-         fs[u] = (M == 3) ? R.df[pu] : recs[u].w;
-      }
-      // the wave's first pixel of this step: once a slot starts past the item's end, this and
-      // the later slots of the step hold no pixel of the wave (the last step of an item)
-      const int wave_first = __builtin_amdgcn_readfirstlane(pb);
-#pragma unroll
-      for (int u = 0; u < kUnroll; u++, w.step(vw, sdx, sdy)) {
-         if (wave_first + u * pstep >= ch.p1) break;
-         const int p = pb + u * pstep;
-         const bool in = p < ch.p1;
-         const float4 rec = recs[u];
-         const float f = fs[u];
-         int ti[3];
-         float wt[3];
-#pragma unroll
-         for (int c = 0; c < M; c++) {
-            const double d = fma((double)f, Dd[c], Da[c]);
-            const double fl = floor(d);
-            ti[c] = (int)fl;
-            wt[c] = (float)(d - fl);
-         }
-         const int x0 = w.x + cx + ti[0], y0 = w.y + ti[1];  // frame coordinates
-         const float wx1 = wt[0], wx0 = 1.f - wt[0];
-         const float wy1 = wt[1], wy0 = 1.f - wt[1];
-         float v = 0.f;
-         if (use) {
-            // 24-bit products (full rate): in-tile offsets are far below 2^23; masked pixels
-            // past the item may give anything, which the clamp keeps inside the tile
-            int idx = __mul24(y0 - S.ya, S.stride) + x0 - cx + kPad;
-            if (M == 3) idx += __mul24(plane + ti[2] - S.za, S.nrows * S.stride);
-            idx = min(max(idx, 0), lds_lim);
-            const float* L = lds + idx;
-            const float v0 = wy0 * (wx0 * L[0] + wx1 * L[1]) + wy1 * (wx0 * L[S.stride] + wx1 * L[S.stride + 1]);
-            if (M == 3) {
-               const float* L2 = L + S.nrows * S.stride;
-               const float v1 = wy0 * (wx0 * L2[0] + wx1 * L2[1]) + wy1 * (wx0 * L2[S.stride] + wx1 * L2[S.stride + 1]);
-               v = (1.f - wt[2]) * v0 + wt[2] * v1;
-            } else {
-               v = v0;
+            for (int u = 0; u < kU; u++) {
+               rows[u] = min(r + u * rpar, rl);
+               const unsigned pu = (unsigned)(pb + rows[u] * nx + c);
+               recs[u] = gld(R.rg4, pu);
+               fs[u] = (M == 3) ? gld(R.df, pu) : recs[u].w;
             }
-         } else if (finite) {
-            // the taps are read from clamped, in-bounds addresses and the ones outside the
-            // frame replaced by 0 afterwards, so all of them are loads without branches that
-            // issue together
-            const int z0 = M == 3 ? plane + ti[2] : 0;
-            const int xa = min(max(x0, 0), nx - 1), xb = min(max(x0 + 1, 0), nx - 1);
-            const int ya = min(max(y0, 0), ny - 1), yb = min(max(y0 + 1, 0), ny - 1);
-            const bool vxa = (unsigned)x0 < (unsigned)nx, vxb = (unsigned)(x0 + 1) < (unsigned)nx;
-            const bool vya = (unsigned)y0 < (unsigned)ny, vyb = (unsigned)(y0 + 1) < (unsigned)ny;
-            auto taps = [&](int zz, bool vz, float& t00, float& t01, float& t10, float& t11) {
-               const int zc = M == 3 ? min(max(zz, 0), nz - 1) : 0;
-               const FT* ra = src + ((int64_t)zc * ny + ya) * nx;
-               const FT* rb = src + ((int64_t)zc * ny + yb) * nx;
-               const float a0 = (float)ra[xa], a1 = (float)ra[xb], b0 = (float)rb[xa], b1 = (float)rb[xb];
-               t00 = vz && vya && vxa ? a0 + off : 0.f;
-               t01 = vz && vya && vxb ? a1 + off : 0.f;
-               t10 = vz && vyb && vxa ? b0 + off : 0.f;
-               t11 = vz && vyb && vxb ? b1 + off : 0.f;
-            };
-            float t00, t01, t10, t11;
-            taps(z0, M != 3 || (unsigned)z0 < (unsigned)nz, t00, t01, t10, t11);
-            const float v0 = wy0 * (wx0 * t00 + wx1 * t01) + wy1 * (wx0 * t10 + wx1 * t11);
-            if (M == 3) {
-               float u00, u01, u10, u11;
-               taps(z0 + 1, (unsigned)(z0 + 1) < (unsigned)nz, u00, u01, u10, u11);
-               const float v1 = wy0 * (wx0 * u00 + wx1 * u01) + wy1 * (wx0 * u10 + wx1 * u11);
-               v = (1.f - wt[2]) * v0 + wt[2] * v1;
-            } else {
-               v = v0;
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+               const bool in = r + u * rpar <= rl;
+               const int y = rows[u];
+               const float4 rec = recs[u];
+               const double fd = (double)fs[u];
+               int ti[3];
+               float wt[3];
+#pragma unroll
+               for (int q = 0; q < M; q++) {
+                  const double d = fma(fd, Dd[q], Da[q]);
+                  const double fl = floor(d);
+                  ti[q] = (int)fl;
+                  // below 1 after the rounding to float too, so a weight of 1 never hides the
+                  // other tap
+                  wt[q] = fminf((float)(d - fl), 0x1.fffffep-1f);
+               }
+               float v = 0.f;
+               if constexpr (TILE) {
+                  // 24-bit products (full rate): tile offsets are far below 2^23
+                  int idx = __mul24(y + ti[1] - S.ya, S.stride) + colk + ti[0];
+                  if (M == 3) idx += __mul24(plane + ti[2] - S.za, plane_step);
+                  const float* L = lds + idx;
+                  const float top = fmaf(wt[0], L[1] - L[0], L[0]);
+                  const float bot = fmaf(wt[0], L[S.stride + 1] - L[S.stride], L[S.stride]);
+                  const float v0 = fmaf(wt[1], bot - top, top);
+                  if (M == 3) {
+                     const float* L2 = L + plane_step;
+                     const float top2 = fmaf(wt[0], L2[1] - L2[0], L2[0]);
+                     const float bot2 = fmaf(wt[0], L2[S.stride + 1] - L2[S.stride], L2[S.stride]);
+                     const float v1 = fmaf(wt[1], bot2 - top2, top2);
+                     v = fmaf(wt[2], v1 - v0, v0);
+                  } else {
+                     v = v0;
+                  }
+               } else {
+                  // the taps are read from clamped, in-bounds addresses and the ones outside the
+                  // frame replaced by 0 afterwards, so all of them are loads without branches that
+                  // issue together (non-finite displacements sample nothing)
+                  const int x0 = c + ti[0], y0 = y + ti[1];
+                  const int z0 = M == 3 ? plane + ti[2] : 0;
+                  const int xa = min(max(x0, 0), nx - 1), xb = min(max(x0 + 1, 0), nx - 1);
+                  const int ya = min(max(y0, 0), ny - 1), yb = min(max(y0 + 1, 0), ny - 1);
+                  const bool vxa = (unsigned)x0 < (unsigned)nx, vxb = (unsigned)(x0 + 1) < (unsigned)nx;
+                  const bool vya = (unsigned)y0 < (unsigned)ny && finite;
+                  const bool vyb = (unsigned)(y0 + 1) < (unsigned)ny && finite;
+                  auto bilinear = [&](int zz, bool vz) -> float {
+                     const int zc = M == 3 ? min(max(zz, 0), nz - 1) : 0;
+                     const unsigned ra = (unsigned)((zc * ny + ya) * nx), rb = (unsigned)((zc * ny + yb) * nx);
+                     const float a0 = (float)gld(src, ra + xa), a1 = (float)gld(src, ra + xb);
+                     const float b0 = (float)gld(src, rb + xa), b1 = (float)gld(src, rb + xb);
+                     const float t00 = vz && vya && vxa ? a0 + off : 0.f;
+                     const float t01 = vz && vya && vxb ? a1 + off : 0.f;
+                     const float t10 = vz && vyb && vxa ? b0 + off : 0.f;
+                     const float t11 = vz && vyb && vxb ? b1 + off : 0.f;
+                     const float top = fmaf(wt[0], t01 - t00, t00);
+                     const float bot = fmaf(wt[0], t11 - t10, t10);
+                     return fmaf(wt[1], bot - top, top);
+                  };
+                  const float v0 = bilinear(z0, M != 3 || (unsigned)z0 < (unsigned)nz);
+                  if (M == 3) {
+                     const float v1 = bilinear(z0 + 1, (unsigned)(z0 + 1) < (unsigned)nz);
+                     v = fmaf(wt[2], v1 - v0, v0);
+                  } else {
+                     v = v0;
+                  }
+               }
+               const int p = pb + y * nx + c;
+               if (MODE == MODE_WARP) {
+                  if (in) wout[p] = (v == 0.f) ? invalid : v;
+                  continue;
+               }
+               // no sample (v == 0): residual 0, outside the SSD; the interval's last pixel has no
+               // Jacobian row
+               const float e = (in && v != 0.f) ? v - rec.x : 0.f;
+               const float ej = (p != ch.end_p) ? e : 0.f;
+               const double ed = (double)e;
+               acc[0] = fma(ed, ed, acc[0]);
+               const float g[3] = {rec.y, rec.z, rec.w};
+#pragma unroll
+               for (int q = 0; q < M; q++) {
+                  const double ge = (double)(g[q] * ej);
+                  acc[1 + q] += ge;
+                  acc[1 + M + q] = fma(ge, fd, acc[1 + M + q]);
+               }
             }
          }
-         if (MODE == MODE_WARP) {
-            if (in) wout[p] = (v == 0.f) ? invalid : v;
-            continue;
-         }
-         // no sample (v == 0): residual 0, outside the SSD; the interval's last pixel has no
-         // Jacobian row
-         const float e = (in && v != 0.f) ? v - rec.x : 0.f;
-         const float ej = (p != ch.end_p) ? e : 0.f;
-         const double ed = (double)e;
-         acc[0] = fma(ed, ed, acc[0]);
-         const double fd = (double)f, omfd = 1.0 - fd;
-         const float g[3] = {rec.y, rec.z, rec.w};
-#pragma unroll
-         for (int c = 0; c < M; c++) {
-            const double ge = (double)(g[c] * ej);
-            acc[1 + c] = fma(ge, omfd, acc[1 + c]);
-            acc[1 + M + c] = fma(ge, fd, acc[1 + M + c]);
-         }
       }
+   };
+   if (ro < rpar) {
+      if (use) rows_loop(std::true_type{});
+      else rows_loop(std::false_type{});
    }
-   if (MODE == MODE_PASS) block_reduce_fast<NV>(acc, (double(*)[NV])sred, out);
+   if (MODE == MODE_PASS) block_reduce_lds<M>(acc, (double*)lds, out);
    else __syncthreads();
 }
 
@@ -1257,18 +1271,18 @@ template <int MODE>
 __device__ __forceinline__ void dispatch_chunk(const RefDev& R, const FrameDev& F,
                                                const Chunk ch, const double* D, float* lds,
                                                int lds_cap, bool use_lds, double* out,
-                                               float* wout, float invalid, double (*sred)[7])
+                                               float* wout, float invalid)
 {
    if (R.n_dim == 2) {
       if (F.dtype == FA_U16)
-         pass_chunk<2, uint16_t, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid, sred);
+         pass_chunk<2, uint16_t, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid);
       else
-         pass_chunk<2, float, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid, sred);
+         pass_chunk<2, float, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid);
    } else {
       if (F.dtype == FA_U16)
-         pass_chunk<3, uint16_t, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid, sred);
+         pass_chunk<3, uint16_t, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid);
       else
-         pass_chunk<3, float, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid, sred);
+         pass_chunk<3, float, MODE>(R, F, ch, D, lds, lds_cap, use_lds, out, wout, invalid);
    }
 }
 
@@ -1279,12 +1293,13 @@ __device__ __forceinline__ void dispatch_chunk(const RefDev& R, const FrameDev& 
 // chunk-major, so consecutive items read the same reference chunk for different frames. The
 // grid is about one workgroup per item when every frame is active, and the hardware
 // dispatcher balances them. k_pass<2> and k_pass<3> are compiled for their own register
-// budgets.
+// budgets: 6 waves per SIMD in 2-D (80 VGPRs), 5 in 3-D (96 VGPRs); left to itself the compiler
+// keeps every unrolled pixel's FP64 temporaries live at once and halves the occupancy.
 template <int M>
-__global__ __launch_bounds__(kBlock) void k_pass(PassArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(M == 2 ? 6 : 5, 8))) void k_pass(PassArgs a)
 {
+   // staging tile, and the work item's reduction (aliasing it; at least 7 x kBlock doubles)
    extern __shared__ float lds[];
-   __shared__ double sred[kBlock / 64][7];
    if (a.counters->n_active_frames <= 0) return;
    const int64_t Wb = a.run_base[a.ref_lo];
    const int64_t W = a.run_base[a.ref_hi] - Wb;
@@ -1320,10 +1335,10 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a)
       double* out = a.partials + (bf.work_base + c) * kPassStride;
       if (bf.f.dtype == FA_U16)
          pass_chunk<M, uint16_t, MODE_PASS>(R, bf.f, ch, a.Dtrial + bf.d_base, lds, a.lds_cap,
-                                            a.use_lds != 0, out, nullptr, 0.f, sred);
+                                            a.use_lds != 0, out, nullptr, 0.f);
       else
          pass_chunk<M, float, MODE_PASS>(R, bf.f, ch, a.Dtrial + bf.d_base, lds, a.lds_cap,
-                                         a.use_lds != 0, out, nullptr, 0.f, sred);
+                                         a.use_lds != 0, out, nullptr, 0.f);
    }
 }
 
@@ -1332,10 +1347,9 @@ __global__ __launch_bounds__(kBlock) void k_warp(RefDev R, FrameDev F, const dou
                                                  int use_lds)
 {
    extern __shared__ float lds[];
-   __shared__ double sred[kBlock / 64][7];
    for (int c = blockIdx.x; c < R.n_chunks; c += gridDim.x) {
       dispatch_chunk<MODE_WARP>(R, F, R.chunks[c], D, lds, lds_cap, use_lds != 0, nullptr, out,
-                                invalid, sred);
+                                invalid);
    }
 }
 
@@ -2084,12 +2098,18 @@ hipError_t launch_eigen(const RefDev& R, double* Q, double* ev, double* info, hi
    return hipGetLastError();
 }
 
+size_t pass_lds_bytes(int lds_cap, int n_dim)
+{
+   return std::max((size_t)lds_cap * sizeof(float), (size_t)(1 + 2 * n_dim) * kBlock * sizeof(double));
+}
+
 hipError_t launch_pass(const PassArgs& a, int n_dim, int grid, hipStream_t s)
 {
+   const size_t lds = pass_lds_bytes(a.lds_cap, n_dim);
    if (n_dim == 2)
-      hipLaunchKernelGGL(k_pass<2>, dim3(grid), dim3(kBlock), (size_t)a.lds_cap * sizeof(float), s, a);
+      hipLaunchKernelGGL(k_pass<2>, dim3(grid), dim3(kBlock), lds, s, a);
    else
-      hipLaunchKernelGGL(k_pass<3>, dim3(grid), dim3(kBlock), (size_t)a.lds_cap * sizeof(float), s, a);
+      hipLaunchKernelGGL(k_pass<3>, dim3(grid), dim3(kBlock), lds, s, a);
    return hipGetLastError();
 }
 
