@@ -190,6 +190,30 @@ __device__ __forceinline__ float fload(const FrameDev& F, int64_t i)
 // p[i] through a global-address-space pointer with a 32-bit index: the compiler then issues a
 // global load with a scalar base and a 32-bit offset instead of a flat load, whose completion
 // is also counted against LDS operations.
+// A pointer the whole wave holds the same value of, in scalar registers (the compiler cannot
+// prove that for pointers read from a table indexed by the work item), so loads through it take
+// a scalar base and a 32-bit vector offset.
+template <typename T>
+__device__ __forceinline__ const T* uniform_ptr(const T* p)
+{
+   const unsigned long long v = (unsigned long long)p;
+   const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+   const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+   return (const T*)(((unsigned long long)hi << 32) | lo);
+}
+
+// *(p + byte offset off), off 32-bit: a global load with a scalar base and a 32-bit vector
+// offset when p is uniform (uniform_ptr).
+template <typename T>
+__device__ __forceinline__ T gld_bytes(const T* p, unsigned off)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+   return *(const __attribute__((address_space(1))) T*)((const __attribute__((address_space(1))) char*)p + off);
+#else
+   return *(const T*)((const char*)p + off);  // host pass: never executed
+#endif
+}
+
 template <typename T>
 __device__ __forceinline__ T gld(const T* p, unsigned i)
 {
@@ -926,14 +950,16 @@ __device__ __forceinline__ float4 cvt4(float4 w, float off)
 }
 
 // The tile's columns: frame columns [cx - kPad, cx + cw + kPad) at tile columns 0, 1, ...;
-// cw = nx (cx = 0) for whole-width items, whose padding columns lie outside the frame.
-template <int NB, typename FT>
+// cw = nx (cx = 0) for whole-width items, whose padding columns lie outside the frame. Staged
+// row R (0 .. nzs * nrows - 1) is frame plane za + R / nrows, row ya + R % nrows, at tile
+// floats R * stride.
+template <int M, int NB, typename FT>
 __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, float* lds, int cx, int cw)
 {
    const int nx = R.nx, ny = R.ny, nz = R.nz;
    const int rows = S.nzs * S.nrows;
    const int tid = threadIdx.x;
-   const FT* __restrict__ src = (const FT*)F.data;
+   const FT* __restrict__ src = uniform_ptr((const FT*)F.data);
    const float off = F.offset;
    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
    if ((nx & 3) == 0 && cw == nx) {
@@ -951,46 +977,50 @@ __device__ void stage_frame(const RefDev& R, const FrameDev& F, const Stage& S, 
       const int x_org = cw == nx ? 0 : cx - kPad;  // frame column of group 0
       const int t_org = cw == nx ? kPad : 0;       // tile column of group 0
       using RT = typename Raw4<FT>::type;
-      // A thread keeps one column group c and takes every rpi-th staged row, rpi = rows per
-      // sweep of the workgroup; rows wider than the workgroup take several column groups per
-      // thread. Row r is plane zi, row yr of the tile, tracked incrementally; indices into the
-      // frame are 32-bit (frames hold fewer than 2^31 samples).
+      // A thread keeps one column group cg and takes staged rows tr, tr + rpi, ... (rpi = rows
+      // per sweep of the workgroup), NB of them loaded before any is stored, so a wave has NB
+      // row segments in flight; rows wider than the workgroup take several column groups per
+      // thread. In 3-D the plane of staged row R is R / nrows, formed as a float product with
+      // a half-row margin (exact for the few thousand rows a tile has). Rows and planes
+      // outside the frame load from a clamped row (in bounds) and store zeros, so the loads
+      // are not under branches. Byte offsets are 32-bit (frames hold fewer than 2^28 samples).
       const int rpi = ncol4 <= kBlock ? kBlock / ncol4 : 1;
       const int tr = ncol4 <= kBlock ? tid / ncol4 : 0;
-      const int cb = ncol4 <= kBlock ? tid - tr * ncol4 : tid;
+      const float inv_nrows = 1.0f / (float)S.nrows;
       if (tr < rpi) {
-         for (int c = cb; c < ncol4; c += kBlock) {
-            const int xg = x_org + 4 * c;
+         for (int cg = ncol4 <= kBlock ? tid - tr * ncol4 : tid; cg < ncol4; cg += kBlock) {
+            const int xg = x_org + 4 * cg;
             const bool col_ok = (unsigned)xg < (unsigned)nx;
             const unsigned xs = col_ok ? (unsigned)xg : 0u;
-            int zi = 0, yr = tr;
-            while (yr >= S.nrows) {
-               yr -= S.nrows;
-               zi++;
-            }
-            for (int r0 = tr; r0 < rows; r0 += NB * rpi) {
-               // every lane loads (rows and columns outside the frame from a clamped, in-bounds
-               // address) so that the loads are not under branches and all NB are in flight
+            const int tcol = t_org + 4 * cg;
+            for (int R0 = tr; R0 < rows; R0 += NB * rpi) {
                RT raw[NB];
-               int dst[NB];
                bool ok[NB];
 #pragma unroll
                for (int u = 0; u < NB; u++) {
-                  const int yy = S.ya + yr, zz = S.za + zi;
-                  const bool live = r0 + u * rpi < rows;
-                  dst[u] = live ? (r0 + u * rpi) * S.stride + t_org + 4 * c : -1;
-                  ok[u] = live && col_ok && (unsigned)yy < (unsigned)ny && (unsigned)zz < (unsigned)nz;
-                  const unsigned yc = min(max(yy, 0), ny - 1), zc = min(max(zz, 0), nz - 1);
-                  raw[u] = *(const RT*)(src + ((zc * (unsigned)ny + yc) * (unsigned)nx + xs));
-                  yr += rpi;
-                  while (yr >= S.nrows) {
-                     yr -= S.nrows;
-                     zi++;
+                  const int Rr = R0 + u * rpi;
+                  int zi = 0, yr = Rr;
+                  if (M == 3) {
+                     zi = (int)(((float)Rr + 0.5f) * inv_nrows);
+                     yr = Rr - __mul24(zi, S.nrows);
                   }
+                  const int yy = S.ya + yr, zz = S.za + zi;
+                  ok[u] = Rr < rows && col_ok && (unsigned)yy < (unsigned)ny && (M == 2 || (unsigned)zz < (unsigned)nz);
+                  // rows and planes outside the frame read some row of the frame (discarded)
+                  const unsigned yc = min((unsigned)yy, (unsigned)(ny - 1));
+                  const unsigned zc = M == 3 ? min((unsigned)zz, (unsigned)(nz - 1)) : 0u;
+                  raw[u] = gld_bytes((const RT*)src, (unsigned)sizeof(FT) * (__umul24(zc * (unsigned)ny + yc, (unsigned)nx) + xs));
                }
 #pragma unroll
-               for (int u = 0; u < NB; u++)
-                  if (dst[u] >= 0) *(float4*)&lds[dst[u]] = ok[u] ? cvt4(raw[u], off) : zero4;
+               for (int u = 0; u < NB; u++) {
+                  const int Rr = R0 + u * rpi;
+                  // samples outside the frame are zero: masked with a bitwise and, not a branch
+                  const float4 v = cvt4(raw[u], off);
+                  const unsigned mk = ok[u] ? 0xffffffffu : 0u;
+                  const float4 w = make_float4(__uint_as_float(__float_as_uint(v.x) & mk), __uint_as_float(__float_as_uint(v.y) & mk),
+                                               __uint_as_float(__float_as_uint(v.z) & mk), __uint_as_float(__float_as_uint(v.w) & mk));
+                  if (Rr < rows) *(float4*)&lds[__mul24(Rr, S.stride) + tcol] = w;
+               }
             }
          }
       }
@@ -1108,6 +1138,11 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    // (half an FP64 ulp of |d|; none when D[k+1] = D[k]).
    Stage S{0, 1, 0, 0, 0};
    bool use = false;
+   // tile sampling: Dsh = D[k] + K with integers K = -floor(lo) that make every pixel's
+   // displacement d + K >= 0 over the work item, so its integer part is a truncation and its
+   // fraction one v_fract (d + K rounds at ~1e-15 px)
+   double Dsh[3] = {0.0, 0.0, 0.0};
+   int Ksh[3] = {0, 0, 0};
    if (want_lds && finite) {
       const double lim = kPad - 3;
       const double fa = (double)((M == 3) ? R.df[ch.p0] : R.rg4[ch.p0].w);
@@ -1131,13 +1166,22 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          }
          S.stride = ((ch.w + 2 * kPad) + 3) & ~3;
          use = (int64_t)S.nzs * S.nrows * S.stride <= lds_cap;
+#pragma unroll
+         for (int c = 0; c < M; c++) {
+            Ksh[c] = -(int)floor(lo[c]);
+            Dsh[c] = Da[c] + (double)Ksh[c];
+         }
       }
    }
    const int cx = ch.x0, cw = ch.w;  // columns of the item (cx = 0, cw = nx: whole rows)
-   if (use) stage_frame<M == 3 ? 8 : 6, FT>(R, F, S, lds, cx, cw);
+   if (use) stage_frame<M, 6, FT>(R, F, S, lds, cx, cw);
    const int plane_step = S.nrows * S.stride;  // tile floats between staged planes
+   const int stride4 = 4 * S.stride;            // tile row in bytes
+   const unsigned nx16 = 16u * (unsigned)nx, nx4 = 4u * (unsigned)nx;  // a frame row of records, of node fractions
+   const float4* __restrict__ rg4 = uniform_ptr(R.rg4);
+   const float* __restrict__ dfp = uniform_ptr(R.df);
 
-   const FT* __restrict__ src = (const FT*)F.data;
+   const FT* __restrict__ src = uniform_ptr((const FT*)F.data);
    const float off = F.offset;
 
    // SSD, then per component sum(g e) and sum((g e) f)
@@ -1155,7 +1199,10 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
       for (int c = cx + (int)threadIdx.x - ro * cw; c < cx + cw; c += kBlock) {
          const int rf = (pb + r0 * nx + c >= ch.p0 ? r0 : r0 + 1) + ro;
          const int rl = pb + r1 * nx + c < ch.p1 ? r1 : r1 - 1;
-         const int colk = c - cx + kPad;  // tile column of the pixel
+         // byte offset in the tile of the pixel's column shifted by -K0 (row and plane offsets
+         // are relative to row r0 and the first staged plane, where K1 and K2 put them)
+         const int colb = 4 * (c - cx + kPad - Ksh[0]);
+         const unsigned pb16 = 16u * (unsigned)(pb + c), pb4 = 4u * (unsigned)(pb + c);
          // tile sampling takes kUnroll rows per step; global sampling (4 or 8 loads per pixel
          // already in flight) one, which keeps it within the register budget
          constexpr int kU = TILE ? kUnroll : 1;
@@ -1166,9 +1213,8 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
 #pragma unroll
             for (int u = 0; u < kU; u++) {
                rows[u] = min(r + u * rpar, rl);
-               const unsigned pu = (unsigned)(pb + rows[u] * nx + c);
-               recs[u] = gld(R.rg4, pu);
-               fs[u] = (M == 3) ? gld(R.df, pu) : recs[u].w;
+               recs[u] = gld_bytes(rg4, __umul24((unsigned)rows[u], nx16) + pb16);
+               fs[u] = (M == 3) ? gld_bytes(dfp, __umul24((unsigned)rows[u], nx4) + pb4) : recs[u].w;
             }
 #pragma unroll
             for (int u = 0; u < kU; u++) {
@@ -1180,19 +1226,31 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
                float wt[3];
 #pragma unroll
                for (int q = 0; q < M; q++) {
-                  const double d = fma(fd, Dd[q], Da[q]);
-                  const double fl = floor(d);
-                  ti[q] = (int)fl;
+                  double fr;
+                  if constexpr (TILE) {
+                     // ti = floor(d) + K (d + K >= 0: truncation is the floor)
+                     const double d = fma(fd, Dd[q], Dsh[q]);
+                     fr = __builtin_amdgcn_fract(d);
+                     ti[q] = (int)d;
+                  } else {
+                     const double d = fma(fd, Dd[q], Da[q]);
+                     const double fl = floor(d);
+                     fr = d - fl;
+                     ti[q] = (int)fl;
+                  }
                   // below 1 after the rounding to float too, so a weight of 1 never hides the
                   // other tap
-                  wt[q] = fminf((float)(d - fl), 0x1.fffffep-1f);
+                  wt[q] = fminf((float)fr, 0x1.fffffep-1f);
                }
                float v = 0.f;
                if constexpr (TILE) {
-                  // 24-bit products (full rate): tile offsets are far below 2^23
-                  int idx = __mul24(y + ti[1] - S.ya, S.stride) + colk + ti[0];
-                  if (M == 3) idx += __mul24(plane + ti[2] - S.za, plane_step);
-                  const float* L = lds + idx;
+                  // byte offset in the tile with 24-bit products (full rate; tile offsets are far
+                  // below 2^23): tile row y - r0 + floor(d1) + K1, column + floor(d0), plane
+                  // floor(d2) + K2
+                  int off = __mul24(y - r0 + ti[1], stride4) + colb;
+                  off += ti[0] << 2;
+                  if (M == 3) off += __mul24(ti[2], 4 * plane_step);
+                  const float* L = (const float*)((const char*)lds + off);
                   const float top = fmaf(wt[0], L[1] - L[0], L[0]);
                   const float bot = fmaf(wt[0], L[S.stride + 1] - L[S.stride], L[S.stride]);
                   const float v0 = fmaf(wt[1], bot - top, top);

@@ -30,7 +30,7 @@ SPACE = {
     "use_lds": [1, 0],
     "tr_lds": [1, 0],
 }
-CHUNKS = [4096, 2048, 8192]  # the library's default first
+CHUNKS = [4096, 2048, 8192, 16384]  # the library's default first
 
 
 def signature(frames, ctx=None):
