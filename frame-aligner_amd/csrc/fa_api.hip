@@ -1279,6 +1279,186 @@ int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offse
 
 }  // namespace fa
 
+// ---- allocation-free alignment --------------------------------------------------------------
+// fa_align_noalloc runs fa_align's kernels with every device buffer it needs (the frame's
+// trust-region vectors, factor scratch, trial displacement, pass partials, state and counters)
+// carved from one file-scope device array, so it calls no allocator. Each GPU the library runs
+// on holds its own copy of the array until the process ends; its device address is looked up
+// once per device when a context is created on it. Calls on one GPU share the array: calls
+// from different contexts that overlap on one GPU race (calls through one context are
+// serialised by its mutex).
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = 4u << 20;  // 4 MiB per GPU
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// With the device current: the device address of this GPU's scratch array (once per device).
+void scratch_init(int device)
+{
+   if (device < 0 || device >= kMaxDevices) return;
+   std::lock_guard<std::mutex> lk(g_scratch_mu);
+   if (g_scratch[device]) return;
+   void* ptr = nullptr;
+   const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+   if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+      return;
+   }
+   g_scratch[device] = ptr;
+}
+
+// Carves aligned pieces off the scratch array; overflow leaves ok false.
+struct Carve {
+   unsigned char* base;
+   size_t used = 0;
+   bool ok = true;
+   template <typename T>
+   T* take(size_t count)
+   {
+      const size_t at = (used + 255) & ~(size_t)255;
+      const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+      if (at + bytes > kNoAllocScratchBytes) {
+         ok = false;
+         return nullptr;
+      }
+      used = at + bytes;
+      return (T*)(base + at);
+   }
+};
+
+int align_noalloc(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_align_stats* stats)
+{
+   fa_reference* r = f->ref;
+   fa_context* ctx = r->ctx;
+   const int dev = ctx->device;
+   void* base = (dev >= 0 && dev < kMaxDevices) ? g_scratch[dev] : nullptr;
+   if (!base)
+      return set_err(FA_E_DEVICE, std::string("allocation-free scratch unavailable on this device: ") +
+                                      hipGetErrorString(g_scratch_err[dev >= 0 && dev < kMaxDevices ? dev : 0]));
+   const int n = r->nD * r->n_dim;
+   const int64_t n_chunks = (int64_t)r->chunks.size();
+   Carve cv{(unsigned char*)base};
+   RefDev* d_ref = cv.take<RefDev>(1);
+   BatchFrame* d_frame = cv.take<BatchFrame>(1);
+   TrState* d_state = cv.take<TrState>(1);
+   Counters* d_counters = cv.take<Counters>(1);
+   int32_t* d_active = cv.take<int32_t>(1);
+   int32_t* d_run_first = cv.take<int32_t>(1);
+   int32_t* d_run_count = cv.take<int32_t>(1);
+   int64_t* d_run_base = cv.take<int64_t>(2);
+   double* d_x = cv.take<double>(n);
+   double* d_vec = cv.take<double>(3 * (size_t)n);
+   double* d_fac = cv.take<double>(3 * (size_t)r->nD * r->n_dim * r->n_dim);
+   double* d_D = cv.take<double>(3 * (size_t)r->nD);
+   double* d_partials = cv.take<double>((size_t)n_chunks * kPassStride);
+   if (!cv.ok)
+      return set_err(FA_E_ARG, "problem too large for the allocation-free entry point (its scratch is " +
+                                   std::to_string(kNoAllocScratchBytes >> 20) + " MiB per GPU)");
+   fa_tr_params P;
+   if (prm) P = *prm;
+   else fa_tr_default(&P);
+
+   BatchFrame bf{};
+   bf.f.data = f->d_data;
+   bf.f.dtype = f->dtype;
+   bf.f.offset = f->offset;
+   bf.ref = 0;
+   bf.n = n;
+   bf.trc = 0;
+   BatchDev B{};
+   B.refs = d_ref;
+   B.frames = d_frame;
+   B.state = d_state;
+   B.x = d_x;
+   B.vec = d_vec;
+   B.fac = d_fac;
+   B.Dtrial = d_D;
+   B.partials = d_partials;
+   B.active_frames = d_active;
+   B.run_first = d_run_first;
+   B.run_count = d_run_count;
+   B.run_base = d_run_base;
+   B.counters = d_counters;
+   B.prm.radius = P.radius;
+   B.prm.min_delta = P.min_delta;
+   B.prm.rel_delta = P.rel_delta;
+   B.prm.sub_eps = P.sub_eps;
+   B.prm.max_iter = P.max_iter;
+   B.prm.max_evals = P.max_evals;
+   B.prm.sub_max_iter = P.sub_max_iter;
+   B.prm.skip_repeat = P.skip_repeat;
+   B.n_frames = 1;
+   B.n_refs = 1;
+   B.trace = nullptr;
+   // the launch shape and update variant a one-frame batch of this reference takes
+   const int lds_floats = (int)std::min<int64_t>(lds_floats_for(r), kLdsCapFloats);
+   PassArgs pa{};
+   pa.refs = d_ref;
+   pa.frames = d_frame;
+   pa.Dtrial = d_D;
+   pa.partials = d_partials;
+   pa.active_frames = d_active;
+   pa.run_first = d_run_first;
+   pa.run_count = d_run_count;
+   pa.run_base = d_run_base;
+   pa.n_refs = 1;
+   pa.counters = d_counters;
+   pa.lds_cap = lds_floats;
+   pa.use_lds = 1;
+   pa.ref_lo = 0;
+   pa.ref_hi = 1;
+   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(n_chunks, (int64_t)ctx->n_cu * kBlocksPerCu));
+   const int need = tr_lds_need(r), need_full = tr_lds_full(r);
+   const bool full = need_full <= kTrFullDoubles;
+   const int tr_lds = tr_lds_for(ctx, full ? need_full : need);
+   const int tr_full = full && tr_lds > 0 ? 1 : 0;
+   const int tr_nt = tr_threads(r);
+
+   hipStream_t s = ctx->stream;
+   const RefDev rd = r->dev;
+   const Counters zero{};
+   HIPCHK(hipMemcpyAsync(d_ref, &rd, sizeof rd, hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemcpyAsync(d_frame, &bf, sizeof bf, hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemcpyAsync(d_counters, &zero, sizeof zero, hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemcpyAsync(d_x, x_inout, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+   HIPCHK(launch_tr_init(B, 0, s));
+   constexpr int K = 8;  // rounds between polls of the active count
+   const int64_t limit = P.max_evals > 0 ? (int64_t)P.max_evals + 2 : 100000;
+   Counters hc{};
+   for (int64_t round = 0; round < limit;) {
+      for (int k = 0; k < K && round < limit; k++, round++) {
+         HIPCHK(launch_schedule(B, 1, s));
+         HIPCHK(launch_pass(pa, r->n_dim, grid, s));
+         HIPCHK(launch_tr_update(B, 0, tr_lds, tr_full, tr_nt, s));
+      }
+      HIPCHK(launch_schedule(B, 0, s));
+      HIPCHK(hipMemcpyAsync(&hc, d_counters, sizeof hc, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (hc.n_active_frames == 0) break;
+   }
+   TrState st{};
+   HIPCHK(hipMemcpyAsync(x_inout, d_x, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(&st, d_state, sizeof st, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   if (stats) {
+      stats->iterations = st.iterations;
+      stats->evaluations = st.evaluations;
+      stats->rejected = st.rejected;
+      stats->status = st.done ? st.status : FA_STOP_MAX_EVALS;
+      stats->f_final = st.f;
+      stats->radius_final = st.radius;
+   }
+   return FA_OK;
+}
+
+}  // namespace
+
 // ---- C ABI ------------------------------------------------------------------------------------
 
 extern "C" {
@@ -1311,6 +1491,7 @@ int fa_create(int device, fa_context** out)
    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
       return set_err(FA_E_DEVICE, std::string("No supported GPU (found ") + prop.gcnArchName + ")");
    HIPCHK(hipSetDevice(device));
+   scratch_init(device);  // the allocation-free entry point's array on this GPU
    auto* c = new fa_context();
    c->device = device;
    c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -1807,6 +1988,14 @@ int fa_align(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_align_sta
    if (prm) b->prm = *prm;
    else fa_tr_default(&b->prm);
    return batch_run(b, x_inout, stats, nullptr);
+}
+
+int fa_align_noalloc(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_align_stats* stats)
+{
+   if (!f || !x_inout) return set_err(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   DeviceGuard g(f->ref->ctx->device);
+   return align_noalloc(f, prm, x_inout, stats);
 }
 
 int fa_batch_create(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out)

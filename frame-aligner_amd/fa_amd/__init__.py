@@ -93,7 +93,8 @@ EXPORTS = [
     "fa_last_error", "fa_tr_default", "fa_create", "fa_destroy", "fa_set_reference",
     "fa_reference_release", "fa_reference_info", "fa_get_hessian", "fa_get_node_map",
     "fa_get_gradient", "fa_register_frame", "fa_deregister_frame", "fa_get_error",
-    "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_image", "fa_warp_point", "fa_align", "fa_batch_create",
+    "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_image", "fa_warp_point", "fa_align", "fa_align_noalloc",
+    "fa_batch_create",
     "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_set_trace", "fa_batch_get_trace",
     "fa_batch_get_tuning", "fa_batch_set_tuning",
     "fa_batch_run", "fa_batch_eval", "fa_align_batch",
@@ -137,6 +138,7 @@ def load_library(path=None):
     L.fa_warp_point.argtypes = [P, P, i32, i32, i32, i32, P]
     L.fa_warp_image.argtypes = [P, P, P, P, i32]
     L.fa_align.argtypes = [P, C.POINTER(TrParams), P, C.POINTER(AlignStats)]
+    L.fa_align_noalloc.argtypes = [P, C.POINTER(TrParams), P, C.POINTER(AlignStats)]
     L.fa_batch_create.argtypes = [P, i32, C.POINTER(TrParams), C.POINTER(P)]
     L.fa_batch_destroy.argtypes = [P]
     L.fa_batch_set_timing.argtypes = [P, i32]
@@ -395,6 +397,16 @@ class Frame:
         p = tr_params(**kw)
         s = AlignStats()
         _check(self.L.fa_align(self.h, C.byref(p), _ptr(x), C.byref(s)))
+        return x, stats_dict(s)
+
+    def align_noalloc(self, x0=None, **kw):
+        """align() through fa_align_noalloc (no device allocation; the library's per-GPU
+        scratch array)."""
+        n = self.ref.nD * self.ref.n_dim
+        x = np.zeros(n) if x0 is None else np.array(x0, np.float64).reshape(n).copy()
+        p = tr_params(**kw)
+        s = AlignStats()
+        _check(self.L.fa_align_noalloc(self.h, C.byref(p), _ptr(x), C.byref(s)))
         return x, stats_dict(s)
 
 

@@ -178,6 +178,15 @@ int fa_warp_point(const fa_reference* ref, const double* D, int32_t x, int32_t y
    on the device for one frame: x_inout (nD*n_dim) is the start and receives the result. */
 int fa_align(fa_frame* frame, const fa_tr_params* prm, double* x_inout, fa_align_stats* stats);
 
+/* fa_align without allocating: no counterpart in the reference; for callers that cannot
+   allocate device memory (code under graph capture, serving loops, harnesses that forbid it).
+   Same kernels and results as fa_align, with the frame's trust-region buffers in a 4 MiB
+   device array the library holds per GPU for the life of the process (references whose
+   buffers exceed it are rejected with FA_E_ARG). It launches eagerly on the context's stream
+   and waits for the result. Calls on one GPU share the array: calls from different contexts
+   that overlap on one GPU race. */
+int fa_align_noalloc(fa_frame* frame, const fa_tr_params* prm, double* x_inout, fa_align_stats* stats);
+
 /* Batched form: many frames (any references, sizes and dtypes) aligned together, each
    trust-region round one fused pass over every active frame plus one device update. */
 int fa_batch_create(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
