@@ -8,9 +8,11 @@ with frames already resident in HBM.
 
 Prints one JSON line: value = frames/s of the batched form; roofline of the fused
 warp+SSD+Jtr pass (algorithmic bytes / HIP-event-timed kernel time); the plain
-one-frame-per-launch path and per-frame latency beside it; the CPU oracle timed
-on a bounded sample of the same frames on this host (also the parity check of the
-measured results, <= 0.02 px).
+one-frame-per-launch path and per-frame latency beside it; the CPU baseline (the
+oracle built with the reference's Release flags, dense subproblem solves, one frame
+per thread) timed on the same frames on this host, and the 256x256, 20-frame
+workload on both; the checker build of the oracle gives the parity check of the
+measured results (<= 0.02 px).
 """
 import argparse
 import json
@@ -42,6 +44,47 @@ def measured_traffic():
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+# a single-GPU job's share of the host's cores: the CPU baseline uses at most this many threads
+HOST_THREAD_SHARE = 16
+
+
+def host_cpu(requested=0):
+    """The host's CPU model and hardware_concurrency, and the worker threads the CPU baseline
+    uses: hardware_concurrency (the reference's MEX pool, FrameAlignerMex.cpp:24-25), limited
+    to the cores this process may run on and to a single-GPU job's share of the host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    hc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = hc
+    threads = requested or min(hc, allowed, HOST_THREAD_SHARE)
+    return {"cpu_model": model, "hardware_concurrency": hc, "affinity_cores": allowed, "threads": threads,
+            "thread_rule": f"min(hardware_concurrency, affinity, {HOST_THREAD_SHARE}: one GPU's share of the host)"
+            if not requested else "--cpu-threads"}
+
+
+# CPU baseline build and solve: the oracle compiled with the reference's Release flags,
+# factorising the dense n x n B at every subproblem iteration as dlib does
+CPU_BASELINE_BUILD = "oracle/build/libfa_oracle_fast.so: gcc -O3 -ffast-math -ftree-vectorize -msse3 (CMakeLists.txt:18-19)"
+CPU_BASELINE_SOLVE = "dense Cholesky of the n x n B per subproblem iteration (dlib general_matrix, OptimisationModel.cpp:55-63)"
+
+
+def cpu_time(O, ref_w, nD, frames, threads, tr):
+    """The Release-flag oracle (dense subproblem solves) on frames, one frame per thread."""
+    orc = O.Oracle(fast=True).set_reference(ref_w, nD)
+    t = time.perf_counter()
+    x, s, secs = orc.align_many(frames, threads=threads, dense=1, **tr)
+    return x, s, secs, time.perf_counter() - t
 
 
 def copy_probe(mib):
@@ -213,11 +256,12 @@ def roofline_of(pass_ms, pass_bytes, launches, kernel):
 def precision_3d(F, args, ctx):
     """Precision form (3-D): 256x256x32 float32 volumes (1500-blob phantom, sigma 1.5-4 voxels,
     Poisson noise), 16 resampling points per plane (nD = 16*32+1 = 513), 3 components (z within
-    2 voxels); FP32 compensated pass sums, FP64 cross-block reductions and block-tridiagonal
-    solves. Frames are resampled with trilinear interpolation (order 1) to bound the generator's
-    time. Reported: volumes/s of the batched run, the 3-D pass roofline, and per result the
-    largest displacement difference to the FP64 oracle and to the float-storage oracle on a
-    sample of volumes (the oracle's time on that sample is the CPU baseline)."""
+    2 voxels); FP64 sample positions and per-lane sums in the pass, FP64 cross-block reductions
+    and block-tridiagonal solves. Frames are resampled with trilinear interpolation (order 1) to
+    bound the generator's time. Reported: volumes/s of the batched run, the 3-D pass roofline,
+    and per result the largest displacement difference to the FP64 oracle and to the
+    float-storage oracle on a sample of volumes; the CPU baseline times the Release-flag oracle
+    build with dense solves on that sample."""
     from fa_amd import synth
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -243,23 +287,25 @@ def precision_3d(F, args, ctx):
            "roofline": roofline_of(pms, pb, nl, "k_pass<3> (fused trilinear warp + SSD + Jtr)")}
     if not args.no_cpu:
         k = min(args.c3_oracle, n)
+        host = host_cpu(args.cpu_threads)
+        threads = min(k, host["threads"])
         fw = np.stack([synth.with_offset(f) for f in st["frames"][:k]])
         per, xo, so, orcs = [], [], [], []
         for fp64 in (True, False):
             orc = O.Oracle(fp64=fp64).set_reference(ref_w, cfg["nD"])
             orcs.append(orc)
-            t = time.perf_counter()
-            x_o, s_o, secs = orc.align_many(fw, threads=min(k, args.cpu_threads), **tr)
-            wall_o = time.perf_counter() - t
-            errs = [float(np.max(np.abs(xs[i] - x_o[i]))) for i in range(k)]
-            per.append(errs)
+            x_o, s_o, _ = orc.align_many(fw, threads=threads, **tr)
+            per.append([float(np.max(np.abs(xs[i] - x_o[i]))) for i in range(k)])
             xo.append(x_o)
             so.append(s_o)
-            if fp64:
-                out["cpu_baseline"] = {"value": round(k / wall_o, 3), "unit": "volumes/s", "cores": min(k, args.cpu_threads),
-                                       "kind": "port", "oracle": "fp64",
-                                       "sample": f"first {k} of the {n} volumes, one volume per thread, "
-                                                 f"{wall_o:.1f}s wall, {secs.sum():.1f}s CPU"}
+        # baseline: the Release-flag build with the reference's dense solves (n = 1539)
+        _, s_f, secs, wall_o = cpu_time(O, ref_w, cfg["nD"], fw, threads, tr)
+        out["cpu_baseline"] = {"value": round(k / wall_o, 3), "unit": "volumes/s", "cores": threads,
+                               "kind": "port", "host": host, "build": CPU_BASELINE_BUILD,
+                               "solve": CPU_BASELINE_SOLVE,
+                               "sample": f"first {k} of the {n} volumes, one volume per thread, "
+                                         f"{wall_o:.1f}s wall, {secs.sum():.1f}s CPU",
+                               "evaluations_per_volume": float(np.mean([d["evaluations"] for d in s_f]))}
         out["max_disp_err_px_per_result"] = {"vs_fp64_oracle": [round(e, 6) for e in per[0]],
                                              "vs_float_oracle": [round(e, 6) for e in per[1]]}
         # how far the oracle's own two builds end apart on the same volumes, and the final SSD
@@ -472,23 +518,66 @@ def headline(F, args, ctx):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
+        host = host_cpu(args.cpu_threads)
         ncpu = min(args.cpu_frames, n)
-        orc = O.Oracle().set_reference(ref_w, nD)
         fw = np.stack([synth.with_offset(f) for f in st["frames"][:ncpu]])
-        t = time.perf_counter()
-        x_o, s_o, secs = orc.align_many(fw, threads=args.cpu_threads, **tr)
-        wall = time.perf_counter() - t
+        # parity: the checker build (no fast-math, block-tridiagonal solves) on the same frames
+        x_o, s_o, _ = O.Oracle().set_reference(ref_w, nD).align_many(fw, threads=host["threads"], **tr)
         err = max(float(np.max(np.abs(xs[i] - x_o[i]))) for i in range(ncpu))
+        # baseline: the Release-flag build with the reference's dense solves
+        x_f, s_f, secs, wall = cpu_time(O, ref_w, nD, fw, host["threads"], tr)
         out["cpu_baseline"] = {"value": round(ncpu / wall, 3), "unit": "frames/s",
-                               "cores": args.cpu_threads, "kind": "port",
+                               "cores": host["threads"], "kind": "port",
                                "sample": f"first {ncpu} of the {n} frames, one frame per thread, "
                                          f"{wall:.1f}s wall, {secs.sum():.1f}s CPU",
+                               "host": host, "build": CPU_BASELINE_BUILD, "solve": CPU_BASELINE_SOLVE,
+                               "mpx_per_s": round(ncpu * 512 * 512 / wall / 1e6, 2),
                                "latency_ms": {"p50": round(1e3 * float(np.percentile(secs, 50)), 1),
-                                              "p99": round(1e3 * float(np.percentile(secs, 99)), 1)}}
+                                              "p99": round(1e3 * float(np.percentile(secs, 99)), 1)},
+                               "evaluations_per_frame": {"mean": float(np.mean([d["evaluations"] for d in s_f])),
+                                                         "max": int(np.max([d["evaluations"] for d in s_f]))},
+                               "max_disp_px_vs_checker_build": round(max(float(np.max(np.abs(x_f[i] - x_o[i])))
+                                                                         for i in range(ncpu)), 6)}
         out["max_disp_err_px_vs_oracle"] = err
-        out["cpu_baseline"]["evaluations_per_frame"] = {"mean": float(np.mean([d["evaluations"] for d in s_o])),
-                                                        "max": int(np.max([d["evaluations"] for d in s_o]))}
+        out["c1_beside"] = c1_beside(F, args, ctx, O, host)
     return out
+
+
+def c1_beside(F, args, ctx, O, host):
+    """The 256x256 float32, 20-frame, 10-resampling-point workload (nD = 11), the reference's
+    CPU-warper case: the GPU's batched frames/s and the CPU baseline on the same frames."""
+    from fa_amd import synth
+    cfg = synth.CONFIGS["2d_256_f32"]
+    st = synth.make_stack(cfg["shape"], cfg["n_frames"], cfg["nD"], seed=args.seed + 1, n_blobs=cfg["n_blobs"],
+                          dtype="float32")
+    ref_w = synth.with_offset(st["reference"])
+    fw = np.stack([synth.with_offset(f) for f in st["frames"]])
+    tr = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    ref = F.Reference(ctx, ref_w, cfg["nD"])
+    frames = [ref.register(f) for f in fw]
+    b = F.Batch(frames, **tr)
+    for _ in range(max(1, args.warmup)):
+        b.run()
+    reps = 20
+    t = time.perf_counter()
+    for _ in range(reps):
+        xs, _, _ = b.run()
+    wall = (time.perf_counter() - t) / reps
+    b.close()
+    for f in frames:
+        f.release()
+    ref.release()
+    x_o, _, _ = O.Oracle().set_reference(ref_w, cfg["nD"]).align_many(fw, threads=host["threads"], **tr)
+    _, _, secs, wall_c = cpu_time(O, ref_w, cfg["nD"], fw, host["threads"], tr)
+    n = len(fw)
+    return {"workload": "256x256 float32, 20 frames, 10 resampling points (nD=11), TR to 1e-4 rel or 50 "
+                        "iterations, batched",
+            "gpu": {"frames_per_s": round(n / wall, 1), "ms_per_step": round(1e3 * wall, 3), "steps": reps},
+            "cpu_baseline": {"value": round(n / wall_c, 2), "unit": "frames/s", "cores": host["threads"],
+                             "kind": "port", "sample": f"all {n} frames, one frame per thread, {wall_c:.2f}s wall",
+                             "latency_ms": {"p50": round(1e3 * float(np.percentile(secs, 50)), 2),
+                                            "p99": round(1e3 * float(np.percentile(secs, 99)), 2)}},
+            "max_disp_err_px_vs_oracle": round(max(float(np.max(np.abs(xs[i] - x_o[i]))) for i in range(n)), 6)}
 
 
 def main():
@@ -499,7 +588,8 @@ def main():
     ap.add_argument("--frames", type=int, default=200)
     ap.add_argument("--plain-frames", type=int, default=40)
     ap.add_argument("--cpu-frames", type=int, default=200)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: hardware_concurrency within one GPU's share of the host)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--seed", type=int, default=20260)

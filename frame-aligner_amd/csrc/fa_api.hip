@@ -1273,6 +1273,7 @@ int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offse
    f->offset = offset;
    f->d_data = d_data;
    r->live_frames++;
+   r->frames.push_back(f);
    *out = f;
    return FA_OK;
 }
@@ -1861,6 +1862,7 @@ int fa_register_frame(fa_reference* r, const void* data, int32_t dtype, float ad
       return set_err(FA_E_DEVICE, "frame upload failed");
    }
    r->live_frames++;
+   r->frames.push_back(f);
    *out = f;
    return FA_OK;
 }
@@ -1874,8 +1876,22 @@ int fa_deregister_frame(fa_frame* f)
    f->single = nullptr;
    void* d = f->d_data;
    dfree(d);
-   f->ref->live_frames--;
+   fa_reference* r = f->ref;
+   r->live_frames--;
+   r->frames.erase(std::remove(r->frames.begin(), r->frames.end(), f), r->frames.end());
    delete f;
+   return FA_OK;
+}
+
+int fa_clear_temp(fa_reference* r)
+{
+   if (!r) return set_err(FA_E_ARG, "null reference");
+   std::lock_guard<std::mutex> lk(r->ctx->mu);
+   DeviceGuard g(r->ctx->device);
+   for (fa_frame* f : r->frames) {
+      delete f->single;  // rebuilt at the frame's next fa_align or single evaluation
+      f->single = nullptr;
+   }
    return FA_OK;
 }
 

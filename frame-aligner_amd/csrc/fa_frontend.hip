@@ -214,6 +214,20 @@ __global__ void k_to_u16(const float* in, int64_t n, uint16_t* out)
    if (i < n) out[i] = to_u16(in[i]);
 }
 
+// float -> uint8 as cv::Mat::convertTo(CV_8U) does (the same rounding, saturated to [0, 255]).
+__device__ __forceinline__ uint8_t to_u8(float v)
+{
+   if (!(fabsf(v) < 2147483648.f)) return 0;
+   const float r = rintf(v);
+   return (uint8_t)(r < 0.f ? 0.f : (r > 255.f ? 255.f : r));
+}
+
+__global__ void k_to_u8(const float* in, int64_t n, uint8_t* out)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) out[i] = to_u8(in[i]);
+}
+
 // getWarpedFrame: the splat divided by its coverage (float division), to uint16.
 __global__ void k_preserving(const float* wimg, const float* cov, int64_t n, uint16_t* out)
 {
@@ -800,6 +814,68 @@ int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, i
    if (outputs || results) rc = a->products(frames, n, dtype, D_out, fr, outputs, results);
    release();
    return rc;
+}
+
+int fa_aligner_realign_as_frame(fa_aligner* a, const double* D, const void* frame, int32_t dtype,
+                                int32_t interpolate_missing, int32_t cv_type, void* out)
+{
+   if (!a || !D || !frame || !out) return set_error(FA_E_ARG, "null argument");
+   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+   if (!interpolate_missing && cv_type != 0 && cv_type != 2 && cv_type != 5)
+      return set_error(FA_E_ARG, "cv_type must be 0 (CV_8U), 2 (CV_16U) or 5 (CV_32F)");
+   std::lock_guard<std::mutex> lk(a->mu);
+   FCHK(hipSetDevice(a->ctx->device));
+   const RefDev& R = a->ref->dev;
+   const int64_t n_px = a->n_px;
+   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(R.n_chunks, (int64_t)a->ctx->n_cu * 8));
+   double* d_D = nullptr;
+   float *d_w = nullptr, *d_cov = nullptr;
+   void* d_out = nullptr;
+   auto done = [&](int rc) {
+      ffree(d_D);
+      ffree(d_w);
+      ffree(d_cov);
+      if (d_out) (void)hipFree(d_out);
+      return rc;
+   };
+   int rc;
+   if ((rc = falloc(&d_D, 3 * (size_t)a->nD)) || (rc = falloc(&d_w, n_px)) || (rc = falloc(&d_cov, n_px)))
+      return done(rc);
+   OK(a->upload(frame, dtype));  // d_raw = the frame as float, as given
+   const FrameDev Fraw{a->d_raw, FA_F32, 0.f};
+   if (hipMemcpyAsync(d_D, D, 3 * (size_t)a->nD * sizeof(double), hipMemcpyHostToDevice, a->stream) != hipSuccess)
+      return done(set_error(FA_E_DEVICE, "upload failed"));
+   size_t out_bytes = n_px * sizeof(float);
+   const void* src = d_w;
+   if (interpolate_missing) {
+      // warpImageInterpolated, invalid value 0
+      if (launch_warp(R, Fraw, d_D, d_w, 0.f, 0, 0, grid, a->stream) != hipSuccess)
+         return done(set_error(FA_E_DEVICE, "warp failed"));
+   } else {
+      // warpImage splat; the weighted values, not divided by the coverage, converted
+      if (hipMemsetAsync(d_w, 0, n_px * sizeof(float), a->stream) != hipSuccess ||
+          hipMemsetAsync(d_cov, 0, n_px * sizeof(float), a->stream) != hipSuccess ||
+          launch_splat(R, Fraw, d_D, d_w, d_cov, 0.f, grid, a->stream) != hipSuccess)
+         return done(set_error(FA_E_DEVICE, "warp failed"));
+      if (cv_type != 5) {
+         out_bytes = n_px * (cv_type == 0 ? 1 : 2);
+         if (hipMalloc(&d_out, out_bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return done(set_error(FA_E_DEVICE, "device allocation failed"));
+         }
+         if (cv_type == 0)
+            hipLaunchKernelGGL(k_to_u8, dim3(grid_of(n_px)), dim3(kFB), 0, a->stream, (const float*)d_w, n_px,
+                               (uint8_t*)d_out);
+         else
+            hipLaunchKernelGGL(k_to_u16, dim3(grid_of(n_px)), dim3(kFB), 0, a->stream, (const float*)d_w, n_px,
+                               (uint16_t*)d_out);
+         src = d_out;
+      }
+   }
+   if (hipGetLastError() != hipSuccess || hipMemcpyAsync(out, src, out_bytes, hipMemcpyDeviceToHost, a->stream) != hipSuccess ||
+       hipStreamSynchronize(a->stream) != hipSuccess)
+      return done(set_error(FA_E_DEVICE, "realignment failed"));
+   return done(FA_OK);
 }
 
 int fa_shift_pixel(const fa_reference* ref, const double* D, double* xyz)

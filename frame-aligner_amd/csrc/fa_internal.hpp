@@ -200,6 +200,7 @@ struct fa_reference {
    double* d_ev = nullptr;
    fa::RefDev dev{};
    int live_frames = 0;
+   std::vector<fa_frame*> frames;  // registered frames (fa_clear_temp drops their cached batches)
    bool counted = false;  // included in ctx->live_refs
 };
 

@@ -92,7 +92,7 @@ class BatchTiming(C.Structure):
 EXPORTS = [
     "fa_last_error", "fa_tr_default", "fa_create", "fa_destroy", "fa_set_reference",
     "fa_reference_release", "fa_reference_info", "fa_get_hessian", "fa_get_node_map",
-    "fa_get_gradient", "fa_register_frame", "fa_deregister_frame", "fa_get_error",
+    "fa_get_gradient", "fa_register_frame", "fa_deregister_frame", "fa_clear_temp", "fa_get_error",
     "fa_get_jacobian", "fa_warp_interpolated", "fa_warp_image", "fa_warp_point", "fa_align", "fa_align_noalloc",
     "fa_batch_create",
     "fa_batch_destroy", "fa_batch_set_timing", "fa_batch_set_trace", "fa_batch_get_trace",
@@ -102,7 +102,8 @@ EXPORTS = [
     "fa_server_submit", "fa_server_submit_host", "fa_server_step", "fa_server_results",
     "fa_server_memory", "fa_host_alloc", "fa_host_free",
     "fa_aligner_create", "fa_aligner_destroy", "fa_aligner_reference", "fa_aligner_preprocess",
-    "fa_aligner_rigid_shift", "fa_aligner_add_frames", "fa_shift_pixel", "fa_write_realignment_info",
+    "fa_aligner_rigid_shift", "fa_aligner_add_frames", "fa_aligner_realign_as_frame", "fa_shift_pixel",
+    "fa_write_realignment_info",
 ]
 
 _lib = None
@@ -132,6 +133,7 @@ def load_library(path=None):
     L.fa_get_gradient.argtypes = [P, P, P, P]
     L.fa_register_frame.argtypes = [P, P, i32, C.c_float, C.POINTER(P)]
     L.fa_deregister_frame.argtypes = [P]
+    L.fa_clear_temp.argtypes = [P]
     L.fa_get_error.argtypes = [P, P, P]
     L.fa_get_jacobian.argtypes = [P, P, P, P]
     L.fa_warp_interpolated.argtypes = [P, P, P, C.c_float]
@@ -157,6 +159,7 @@ def load_library(path=None):
     L.fa_aligner_preprocess.argtypes = [P, P, i32, P]
     L.fa_aligner_rigid_shift.argtypes = [P, P, i32, P]
     L.fa_aligner_add_frames.argtypes = [P, P, i32, i32, C.POINTER(TrParams), P, C.POINTER(AddOutputs), P]
+    L.fa_aligner_realign_as_frame.argtypes = [P, P, P, i32, i32, i32, P]
     L.fa_shift_pixel.argtypes = [P, P, P]
     L.fa_write_realignment_info.argtypes = [C.c_char_p, C.POINTER(ScanParams), i32, i32, P, P, P, P]
     L.fa_server_create.argtypes = [P, P, i32, i32, C.c_uint64, C.POINTER(TrParams), C.POINTER(P)]
@@ -287,6 +290,11 @@ class Reference:
             self.release()
         except Exception:
             pass
+
+    def clear_temp(self):
+        """GpuFrameWarper::clearTemp: frees the working spaces cached for this reference's
+        frames (rebuilt on next use); the frames stay registered."""
+        _check(self.L.fa_clear_temp(self.h))
 
     @property
     def H(self):
@@ -699,6 +707,22 @@ class Aligner:
                         unaligned_correlation=r.unaligned_correlation, coverage=r.coverage) for r in res]
         return [D[i] for i in range(n)], results, imgs
 
+    def realign_as_frame(self, D, frame, interpolate_missing=False, cv_type=0):
+        """FrameWarpAligner::realignAsFrame: frame (as given, no blur or offset) realigned with D
+        (nD x 3). interpolate_missing: the interpolated warp (float32, invalid samples 0); else
+        the splat's weighted values converted to cv_type (0 = CV_8U, 2 = CV_16U, 5 = CV_32F)."""
+        a, dt = self._raw(frame)
+        if interpolate_missing:
+            out_dtype = np.float32
+        else:
+            out_dtype = {0: np.uint8, 2: np.uint16, 5: np.float32}.get(int(cv_type))
+            if out_dtype is None:
+                raise FaError(-1, "cv_type must be 0 (CV_8U), 2 (CV_16U) or 5 (CV_32F)")
+        out = np.zeros(self.shape, out_dtype)
+        _check(self.L.fa_aligner_realign_as_frame(self.h, _ptr(as_D(D, self.nD, self.n_dim)), _ptr(a), dt,
+                                                  int(bool(interpolate_missing)), int(cv_type), _ptr(out)))
+        return out
+
     def shift_pixel(self, D, x, y, z):
         """FrameWarpAligner::shiftPixel for a frame aligned to D."""
         ref = C.c_void_p()
@@ -777,4 +801,5 @@ class HipFrameWarper:
         return self.ref.warp_point(D, x, y, z, spatial_binning)
 
     def clearTemp(self):
-        pass
+        if self.ref is not None:
+            self.ref.clear_temp()

@@ -64,7 +64,7 @@ class HipFrameWarper {
    HipFrameWarper& operator=(const HipFrameWarper&) = delete;
    ~HipFrameWarper()
    {
-      clearTemp();
+      release_frames();
       if (ref_) fa_reference_release(ref_);
       fa_destroy(ctx_);
    }
@@ -74,7 +74,7 @@ class HipFrameWarper {
                      const ImageScanParameters& p)
    {
       if (dims_zyx.size() != 3) throw std::runtime_error("Reference must be three dimensional");
-      clearTemp();
+      release_frames();
       if (ref_) check(fa_reference_release(ref_));
       ref_ = nullptr;
       fa_scan_params s{p.line_duration, p.interline_duration, p.interframe_duration, p.n_x, p.n_y, p.n_z, 0};
@@ -146,11 +146,12 @@ class HipFrameWarper {
       return h;
    }
 
+   // GpuFrameWarper::clearTemp (GpuFrameWarper.cpp:154-158): frees the cached working spaces;
+   // registered frames stay registered.
    void clearTemp()
    {
       std::lock_guard<std::mutex> lk(mu_);
-      for (auto& kv : frames_) fa_deregister_frame(kv.second);
-      frames_.clear();
+      if (ref_) check(fa_clear_temp(ref_));
    }
 
    int n_dim = 2;
@@ -158,6 +159,13 @@ class HipFrameWarper {
    std::vector<int> dims;
 
  private:
+   void release_frames()
+   {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (auto& kv : frames_) fa_deregister_frame(kv.second);
+      frames_.clear();
+   }
+
    void add(const void* frame, int dtype, float off)
    {
       if (!ref_) throw std::runtime_error("Reference imaging not yet set");

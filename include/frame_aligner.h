@@ -153,6 +153,11 @@ int fa_get_gradient(const fa_reference* ref, float* gx, float* gy, float* gz);
 int fa_register_frame(fa_reference* ref, const void* data, int32_t dtype, float add_offset,
                       fa_frame** out);
 int fa_deregister_frame(fa_frame* frame);
+/* GpuFrameWarper::clearTemp (FrameAligner/GpuFrameWarper.cpp:154-158; FrameWarper.h:118): drops
+   the device working spaces the library caches for the reference's registered frames (the
+   one-frame batch behind fa_align and the single evaluations); they are rebuilt on next use,
+   and the frames stay registered. */
+int fa_clear_temp(fa_reference* ref);
 
 /* getError (FrameWarper.h:103; CPU FrameWarper.cpp:169-176, GPU GpuFrameWarper.cpp:182-190):
    SSD of the warped frame against the reference over valid pixels. */
@@ -327,6 +332,17 @@ int fa_aligner_rigid_shift(fa_aligner* a, const void* raw, int32_t dtype, double
 int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, int32_t dtype,
                           const fa_tr_params* prm, double* D_out, const fa_add_outputs* outputs,
                           fa_add_result* results);
+/* FrameWarpAligner::realignAsFrame (FrameWarpAligner.cpp:270-285): frame (nz*ny*nx values of
+   dtype, warped as given: no blur, no offset) realigned with the displacements D (nD x 3) that
+   were found for some frame. interpolate_missing = 1: warpImageInterpolated with invalid value
+   0 (FrameWarper.cpp:31-69), out gets float32. interpolate_missing = 0: warpImageNormalised
+   (FrameWarper.cpp:138-144), the forward splat's weighted values (not divided by the coverage)
+   converted as cv::Mat::convertTo does to cv_type: 0 = CV_8U (the reference's default), 2 =
+   CV_16U, 5 = CV_32F (round half to even, saturate). The reference's check of the frame's
+   usability (FrameWarpAligner.cpp:277-278) is followed by the warp, which replaces its result,
+   so it has no effect and is not reproduced. */
+int fa_aligner_realign_as_frame(fa_aligner* a, const double* D, const void* frame, int32_t dtype,
+                                int32_t interpolate_missing, int32_t cv_type, void* out);
 /* FrameWarpAligner::shiftPixel (FrameWarpAligner.cpp:253-266): the point (x, y, z), truncated
    to whole pixels, minus warpPoint(D) there; xyz is updated in place. */
 int fa_shift_pixel(const fa_reference* ref, const double* D, double* xyz);

@@ -633,7 +633,7 @@ static void bt_forward(int nb, int m, const double* Ld, const double* Ls, const 
 {
    const int mm = m * m;
    for (int k = 0; k < nb; k++) {
-      double r[3];
+      double r[3] = {0, 0, 0};
       for (int i = 0; i < m; i++) {
          r[i] = b[k * m + i];
          if (k > 0)
@@ -652,7 +652,7 @@ static void bt_backward(int nb, int m, const double* Ld, const double* Ls, const
 {
    const int mm = m * m;
    for (int k = nb - 1; k >= 0; k--) {
-      double r[3];
+      double r[3] = {0, 0, 0};
       for (int i = 0; i < m; i++) {
          r[i] = y[k * m + i];
          if (k + 1 < nb)
@@ -873,6 +873,7 @@ void orc_tr_default(orc_tr_params* p)
    p->sub_eps = 0.1;
    p->sub_max_iter = 20;
    p->skip_repeat = 0;
+   p->dense = 0;
 }
 
 /* One trace record per objective evaluation (layout in fa_oracle.h), then x, g, p. */
@@ -922,6 +923,23 @@ int orc_tr_run(const orc_tr_model* M, const orc_tr_params* prm, double* x, orc_t
    A.sub = M->sub;
    A.Ld = (double*)malloc(sizeof(double) * (size_t)n * M->m);
    A.Ls = (double*)malloc(sizeof(double) * (size_t)n * M->m);
+   double* Bd = NULL; /* dense B (prm->dense): the n x n matrix dlib receives */
+   if (prm->dense) {
+      const int m = M->m, mm = m * m;
+      Bd = (double*)calloc((size_t)n * n, sizeof(double));
+      A.Ldense = (double*)malloc(sizeof(double) * (size_t)n * n);
+      for (int k = 0; k < M->nb; k++)
+         for (int a = 0; a < m; a++)
+            for (int b = 0; b < m; b++) {
+               Bd[(size_t)(k * m + a) * n + k * m + b] = M->diag[k * mm + a * m + b];
+               if (k + 1 < M->nb) {
+                  const double v = M->sub[k * mm + a * m + b];
+                  Bd[(size_t)((k + 1) * m + a) * n + k * m + b] = v;
+                  Bd[(size_t)(k * m + b) * n + (k + 1) * m + a] = v;
+               }
+            }
+      A.dense = Bd;
+   }
 
    double f = M->value(M->user, x);
    s.evaluations = 1;
@@ -972,7 +990,17 @@ int orc_tr_run(const orc_tr_model* M, const orc_tr_params* prm, double* x, orc_t
       for (int i = 0; i < n; i++) xt[i] = x[i] + p[i];
       double fn = M->value(M->user, xt);
       s.evaluations++;
-      double pred = -(0.5 * quad_bt(M->nb, M->m, M->diag, M->sub, p) + gp);
+      double pBp = 0;
+      if (Bd) {
+         for (int i = 0; i < n; i++) {
+            double t = 0;
+            for (int j = 0; j < n; j++) t += Bd[(size_t)i * n + j] * p[j];
+            pBp += p[i] * t;
+         }
+      } else {
+         pBp = quad_bt(M->nb, M->m, M->diag, M->sub, p);
+      }
+      double pred = -(0.5 * pBp + gp);
       double meas = f - fn;
       memset(rv, 0, sizeof rv);
       rv[ORC_T_FNEW] = fn;
@@ -1036,6 +1064,8 @@ int orc_tr_run(const orc_tr_model* M, const orc_tr_params* prm, double* x, orc_t
    free(xt);
    free(A.Ld);
    free(A.Ls);
+   free(Bd);
+   free(A.Ldense);
    return 0;
 }
 

@@ -51,6 +51,10 @@ typedef struct {
    int sub_max_iter;    /* subproblem iteration cap (dlib: 20) */
    int skip_repeat;     /* 1: do not re-evaluate a rejected step that would be
                            proposed again unchanged (same trajectory, fewer evaluations) */
+   int dense;           /* 1: factorise B as a dense n x n matrix at every subproblem iteration,
+                           as dlib does with the general_matrix the reference passes
+                           (OptimisationModel.cpp:55-63); 0: block-tridiagonal (same result,
+                           O(n) instead of O(n^3)) */
 } orc_tr_params;
 
 typedef struct {

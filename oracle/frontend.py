@@ -154,6 +154,28 @@ def to_u16(v):
     return out
 
 
+def to_u8(v):
+    """cv::Mat::convertTo(CV_8U) of float values: the same rounding, saturated to [0, 255]."""
+    v = np.asarray(v, np.float32)
+    out = np.zeros(v.shape, np.uint8)
+    ok = np.abs(v) < np.float32(2147483648.0)
+    out[ok] = np.clip(np.rint(v[ok]), 0, 255).astype(np.uint8)
+    return out
+
+
+def realign_as_frame(orc, frame, D, interpolate_missing=False, cv_type=0):
+    """FrameWarpAligner::realignAsFrame (FrameWarpAligner.cpp:270-285) with the oracle's warper:
+    the interpolated warp (FrameWarper.cpp:31-69, invalid 0, float32) or warpImageNormalised
+    (FrameWarper.cpp:138-144: the splat's weighted values, not divided by the coverage,
+    converted to cv_type 0 = CV_8U, 2 = CV_16U, 5 = CV_32F)."""
+    f = np.asarray(frame, np.float32)
+    if interpolate_missing:
+        return orc.warp_interpolated(f, D, 0.0).astype(np.float32)
+    wimg, _ = orc.warp_image(f, D)
+    wimg = np.asarray(wimg, np.float32)
+    return {0: to_u8, 2: to_u16, 5: lambda v: v}[cv_type](wimg)
+
+
 def preserving(wimg, cov):
     """getWarpedFrame (FrameWarpAligner.cpp:317-325): intensity_preserving /= mask, to uint16."""
     with np.errstate(divide="ignore", invalid="ignore"):

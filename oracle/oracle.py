@@ -11,11 +11,13 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "libfa_oracle.so")
+# the same source built with the reference's Release flags; times the CPU baseline only
+FAST_LIB_PATH = os.path.join(HERE, "build", "libfa_oracle_fast.so")
 
 
 def build(force=False):
     """Compiles the oracle with its Makefile (gcc, no GPU needed)."""
-    if force or not os.path.exists(LIB_PATH):
+    if force or not (os.path.exists(LIB_PATH) and os.path.exists(FAST_LIB_PATH)):
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
     return LIB_PATH
 
@@ -23,7 +25,7 @@ def build(force=False):
 class TrParams(C.Structure):
     _fields_ = [("radius", C.c_double), ("min_delta", C.c_double), ("rel_delta", C.c_double),
                 ("max_iter", C.c_int), ("max_evals", C.c_int), ("sub_eps", C.c_double),
-                ("sub_max_iter", C.c_int), ("skip_repeat", C.c_int)]
+                ("sub_max_iter", C.c_int), ("skip_repeat", C.c_int), ("dense", C.c_int)]
 
 
 class TrStats(C.Structure):
@@ -45,15 +47,16 @@ TRACE_FIELDS = ["evals", "f_new", "f", "pred", "meas", "rho", "accept", "radius"
                 "status", "lambda", "sub_iters", "pnorm", "gnorm", "gp", "rejected"]
 N_TRACE = len(TRACE_FIELDS)
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(fast=False):
+    """The checker library, or with fast=True the Release-flag build used for timing."""
+    path = FAST_LIB_PATH if fast else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
         L.orc_create.restype = P
@@ -87,8 +90,8 @@ def lib():
                                            C.c_int, P, dp]
         L.orc_tr_subproblem_dense.argtypes = [C.c_int, P, P, C.c_double, C.c_double, C.c_int,
                                               P, dp]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _ptr(a):
@@ -111,8 +114,8 @@ class Oracle:
     """CpuFrameWarper restated (FrameWarper.h:95-159). fp64=False is the
     reference-faithful float-storage mode, fp64=True the all-double mode."""
 
-    def __init__(self, fp64=False):
-        self.L = lib()
+    def __init__(self, fp64=False, fast=False):
+        self.L = lib(fast)
         self.h = self.L.orc_create(1 if fp64 else 0)
         self.fp64 = fp64
 

@@ -131,6 +131,26 @@ def test_to_u16_rounding_and_saturation():
     assert FE.preserving(np.array([3.0, 0.0, 5.0], np.float32), np.array([2.0, 0.0, 0.5], np.float32)).tolist() == [2, 0, 10]
 
 
+def test_to_u8_rounding_and_saturation():
+    v = np.array([0.5, 1.5, 2.5, -0.4, -3.0, 254.5, 255.4, 255.6, 300.0, np.nan, np.inf, 3e9, 12.49], np.float32)
+    assert FE.to_u8(v).tolist() == [0, 2, 2, 0, 0, 254, 255, 255, 255, 0, 0, 0, 12]
+
+
+def test_realign_as_frame_oracle_identities():
+    """At D = 0 the interpolated warp is the frame (zeros stay 0), and the splat puts every
+    pixel on itself with weight 1, so warpImageNormalised is the frame converted."""
+    rng = np.random.default_rng(11)
+    frame = rng.uniform(0, 400, (1, 24, 20)).astype(np.float32)
+    frame[0, 3, 4] = 0.0
+    orc = O.Oracle().set_reference(frame + 1, 5)
+    D = np.zeros((5, 3))
+    assert np.array_equal(FE.realign_as_frame(orc, frame, D, True), frame)
+    assert np.array_equal(FE.realign_as_frame(orc, frame, D, False, 5), frame)
+    assert np.array_equal(FE.realign_as_frame(orc, frame, D, False, 2), FE.to_u16(frame))
+    assert np.array_equal(FE.realign_as_frame(orc, frame, D, False, 0), FE.to_u8(frame))
+    assert FE.realign_as_frame(orc, frame, D, False, 0).dtype == np.uint8
+
+
 def test_correlation_pins():
     rng = np.random.default_rng(6)
     a = rng.normal(10, 3, (3, 20, 30)).astype(np.float32)

@@ -158,6 +158,58 @@ def test_add_frames_outputs_and_metrics(ctx, shape, n_res):
     al.close()
 
 
+@pytest.mark.parametrize("shape,n_res,dtype", [((1, 96, 80), 8, "uint16"), ((4, 40, 36), 5, "float32")])
+def test_realign_as_frame(ctx, shape, n_res, dtype):
+    """realignAsFrame (FrameWarpAligner.cpp:270-285) with the displacements add_frames found,
+    on the raw frame as given, against the oracle's warps at the same D: the interpolated warp
+    (float) and warpImageNormalised converted to CV_8U, CV_16U and CV_32F."""
+    nD = n_res * shape[0] + 1
+    st = synth.make_stack(shape, 2, nD, seed=21, n_blobs=120, sigma=(1.5, 4.0), dtype=dtype)
+    al = F.Aligner(ctx, st["reference"], n_resampling_points=n_res, smoothing=1.0)
+    D_g, _, _ = al.add_frames(list(st["frames"]), rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    orc = O.Oracle().set_reference(synth.with_offset(np.asarray(st["reference"], np.float32)), nD)
+    for i, raw in enumerate(st["frames"]):
+        got = al.realign_as_frame(D_g[i], raw, interpolate_missing=True)
+        exp = FE.realign_as_frame(orc, raw, D_g[i], True)
+        assert got.dtype == np.float32
+        assert np.max(np.abs(got - exp)) <= 1e-5 * np.max(np.abs(exp))
+        w32 = al.realign_as_frame(D_g[i], raw, cv_type=5)
+        e32 = FE.realign_as_frame(orc, raw, D_g[i], False, 5)
+        assert np.max(np.abs(w32 - e32)) <= 1e-5 * np.max(np.abs(e32))
+        for cv_type, dt in ((0, np.uint8), (2, np.uint16)):
+            got = al.realign_as_frame(D_g[i], raw, cv_type=cv_type)
+            assert got.dtype == dt
+            # the device's own float splat converted exactly; the oracle's within one count
+            conv = FE.to_u8(w32) if cv_type == 0 else FE.to_u16(w32)
+            assert np.array_equal(got, conv)
+            diff = np.abs(got.astype(np.int64) - FE.realign_as_frame(orc, raw, D_g[i], False, cv_type).astype(np.int64))
+            assert diff.max() <= 1
+    with pytest.raises(F.FaError):
+        al.realign_as_frame(D_g[0], st["frames"][0], cv_type=3)
+    al.close()
+
+
+def test_clear_temp_keeps_frames(ctx):
+    """clearTemp (GpuFrameWarper.cpp:154-158) frees the cached working spaces; the registered
+    frames stay usable and give the same results."""
+    st = synth.make_stack((1, 64, 72), 2, 9, seed=22)
+    ref = F.Reference(ctx, synth.with_offset(st["reference"]), 9)
+    frs = [ref.register(synth.with_offset(f)) for f in st["frames"]]
+    before = [f.align(rel_delta=1e-4, max_iter=50) for f in frs]
+    D = np.random.default_rng(2).normal(0, 0.5, (9, 3))
+    e0 = frs[0].error(D)
+    ref.clear_temp()
+    assert frs[0].error(D) == e0
+    after = [f.align(rel_delta=1e-4, max_iter=50) for f in frs]
+    for (x1, s1), (x2, s2) in zip(before, after):
+        assert np.array_equal(x1, x2) and s1["evaluations"] == s2["evaluations"]
+    ref.clear_temp()
+    for f in frs:
+        f.release()
+    ref.clear_temp()  # no frames left
+    ref.release()
+
+
 def test_shift_pixel(ctx):
     shape, n_res = (3, 40, 36), 4
     nD = n_res * shape[0] + 1

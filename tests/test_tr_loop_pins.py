@@ -166,3 +166,28 @@ def test_skip_repeat_gives_the_same_path():
 def test_zero_gradient_stops_with_no_progress():
     x, st, tr = run1d(1.0, 1.0, 3.0, 1.0, 3.0)
     assert st["status"] == 4 and st["evaluations"] == 2 and x[0] == 3.0
+
+
+def test_dense_factorisation_gives_the_same_path():
+    """dense = 1 factorises the full n x n B as dlib does with the reference's general_matrix;
+    on a 3-block model with coupling it takes the same steps as the block-tridiagonal solve."""
+    rng = np.random.default_rng(7)
+    nb, m = 3, 2
+    M = rng.normal(0, 1, (nb * m, nb * m))
+    B = M @ M.T + 0.5 * np.eye(nb * m)
+    for i in range(nb * m):  # keep B block-tridiagonal
+        for j in range(nb * m):
+            if abs(i // m - j // m) > 1:
+                B[i, j] = 0
+    B += 3 * np.eye(nb * m)
+    diag = np.array([B[k * m:(k + 1) * m, k * m:(k + 1) * m] for k in range(nb)])
+    sub = np.array([B[(k + 1) * m:(k + 2) * m, k * m:(k + 1) * m] for k in range(nb - 1)])
+    c = rng.normal(0, 30, nb * m)
+    A = B * 0.7
+    f = lambda x: 0.5 * (x - c) @ A @ (x - c) + 0.01 * np.sum((x - c) ** 4)
+    g = lambda x: A @ (x - c) + 0.04 * (x - c) ** 3
+    x1, s1, t1 = O.tr_run(diag, sub, f, g, np.zeros(nb * m), cap=400)
+    x2, s2, t2 = O.tr_run(diag, sub, f, g, np.zeros(nb * m), cap=400, dense=1)
+    assert s1["evaluations"] == s2["evaluations"] and s1["status"] == s2["status"]
+    assert np.allclose(t1[:, :O.N_TRACE], t2[:, :O.N_TRACE], rtol=1e-9, atol=1e-9)
+    assert np.max(np.abs(x1 - x2)) <= 1e-9
