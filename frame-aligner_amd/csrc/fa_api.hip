@@ -105,6 +105,7 @@ struct DeviceGuard {
 
 struct fa_batch {
    fa_context* ctx = nullptr;
+   std::mutex mu;                  // serialises runs, evaluations and tuning of this batch
    hipStream_t stream = nullptr;   // the batch's own stream (default priority)
    // A batch of many frames runs as parts on their own streams, whose trust-region rounds
    // interleave so that one part's update overlaps another part's pass. Leaf batches have no
@@ -1871,6 +1872,7 @@ int fa_deregister_frame(fa_frame* f)
 {
    if (!f) return FA_OK;
    std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   std::unique_lock<std::mutex> lf(f->mu);  // waits for a call on the frame still running
    DeviceGuard g(f->ref->ctx->device);
    delete f->single;
    f->single = nullptr;
@@ -1879,6 +1881,7 @@ int fa_deregister_frame(fa_frame* f)
    fa_reference* r = f->ref;
    r->live_frames--;
    r->frames.erase(std::remove(r->frames.begin(), r->frames.end(), f), r->frames.end());
+   lf.unlock();
    delete f;
    return FA_OK;
 }
@@ -1889,6 +1892,7 @@ int fa_clear_temp(fa_reference* r)
    std::lock_guard<std::mutex> lk(r->ctx->mu);
    DeviceGuard g(r->ctx->device);
    for (fa_frame* f : r->frames) {
+      std::lock_guard<std::mutex> lf(f->mu);
       delete f->single;  // rebuilt at the frame's next fa_align or single evaluation
       f->single = nullptr;
    }
@@ -1898,7 +1902,7 @@ int fa_clear_temp(fa_reference* r)
 int fa_get_error(fa_frame* f, const double* D, double* err)
 {
    if (!f || !D || !err) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   std::lock_guard<std::mutex> lk(f->mu);
    DeviceGuard g(f->ref->ctx->device);
    fa_batch* b;
    CHK(single_batch(f, &b));
@@ -1908,7 +1912,7 @@ int fa_get_error(fa_frame* f, const double* D, double* err)
 int fa_get_jacobian(fa_frame* f, const double* D, double* jac, double* err)
 {
    if (!f || !D || !jac) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   std::lock_guard<std::mutex> lk(f->mu);
    DeviceGuard g(f->ref->ctx->device);
    fa_batch* b;
    CHK(single_batch(f, &b));
@@ -1921,7 +1925,7 @@ int fa_get_jacobian(fa_frame* f, const double* D, double* jac, double* err)
 int fa_warp_interpolated(fa_frame* f, const double* D, float* out, float invalid_value)
 {
    if (!f || !D || !out) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   std::lock_guard<std::mutex> lk(f->mu);
    DeviceGuard g(f->ref->ctx->device);
    fa_batch* b;
    CHK(single_batch(f, &b));
@@ -1946,7 +1950,7 @@ int fa_warp_interpolated(fa_frame* f, const double* D, float* out, float invalid
 int fa_warp_image(fa_frame* f, const double* D, float* wimg, float* coverage, int32_t apply_offset)
 {
    if (!f || !D || !wimg || !coverage) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   std::lock_guard<std::mutex> lk(f->mu);
    DeviceGuard g(f->ref->ctx->device);
    fa_batch* b;
    CHK(single_batch(f, &b));
@@ -1997,7 +2001,7 @@ int fa_warp_point(const fa_reference* r, const double* D, int32_t x, int32_t y, 
 int fa_align(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_align_stats* stats)
 {
    if (!f || !x_inout) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+   std::lock_guard<std::mutex> lk(f->mu);
    DeviceGuard g(f->ref->ctx->device);
    fa_batch* b;
    CHK(single_batch(f, &b));
@@ -2081,7 +2085,7 @@ int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t)
    if (t->lds_floats < 0 || t->lds_floats > 15360 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
        t->check_every < 1 || t->check_every > 64 || t->tr_lds < 0 || t->tr_lds > 1)
       return set_err(FA_E_ARG, "tuning out of range (lds_floats <= 15360, 1 <= blocks_per_cu <= 64)");
-   std::lock_guard<std::mutex> lk(b->ctx->mu);
+   std::lock_guard<std::mutex> lk(b->mu);
    b->tune = *t;
    if (b->parts.empty()) {
       set_grids(b);
@@ -2098,7 +2102,7 @@ int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t)
 int fa_batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timing* timing)
 {
    if (!b || !x_inout) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(b->ctx->mu);
+   std::lock_guard<std::mutex> lk(b->mu);
    DeviceGuard g(b->ctx->device);
    return batch_run(b, x_inout, stats, timing);
 }
@@ -2106,7 +2110,7 @@ int fa_batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_t
 int fa_batch_eval(fa_batch* b, const double* D, double* err, double* jac)
 {
    if (!b || !D) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(b->ctx->mu);
+   std::lock_guard<std::mutex> lk(b->mu);
    DeviceGuard g(b->ctx->device);
    return batch_eval(b, D, err, jac);
 }

@@ -210,6 +210,9 @@ struct fa_frame {
    float offset = 0;
    void* d_data = nullptr;
    fa_batch* single = nullptr;  // cached one-frame batch for fa_align and single evaluations
+   // Serialises the calls on this frame (fa_align, the single evaluations and warps, which use
+   // the frame's own batch and stream); calls on different frames run concurrently.
+   std::mutex mu;
 };
 
 namespace fa {

@@ -19,6 +19,12 @@
  *  - Every call returns FA_OK (0) or a negative FA_E_* code; fa_last_error()
  *    returns the calling thread's last message. Calls on distinct frames may
  *    run from different threads; a reference is read-only after set-up.
+ *  - Per-frame calls (fa_get_error, fa_get_jacobian, fa_warp_*, fa_align) lock
+ *    only their frame and run on the frame's own stream, so calls on different
+ *    frames from different threads run at the same time on the device; calls on
+ *    one frame take turns. Batch runs and evaluations lock only their batch.
+ *    fa_align_noalloc, servers and set-up calls (references, frame registration,
+ *    batch creation) take the context's lock.
  */
 #ifndef FRAME_ALIGNER_H
 #define FRAME_ALIGNER_H
