@@ -163,6 +163,9 @@ struct fa_batch {
    int64_t warp_px = 0;
    double* d_trace = nullptr;       // trust-region trace (fa_batch_set_trace), or null
    int trace_cap = 0, trace_rec = 0;
+   // 1: one reference, one update launch: the update's last workgroup compacts the active
+   // frames for the next round, so a round is a pass and an update (no schedule launch)
+   int fold = 0;
 
    BatchDev dev() const
    {
@@ -193,6 +196,7 @@ struct fa_batch {
       B.trace = d_trace;
       B.trace_cap = trace_cap;
       B.trace_rec = trace_rec;
+      B.fold = fold;
       return B;
    }
 
@@ -270,16 +274,31 @@ int tr_lds_need(const fa_reference* r)
    return 3 * r->nD * r->n_dim * r->n_dim + r->nD * r->n_dim;
 }
 
-int tr_lds_full(const fa_reference* r)
+// Updates whose full staging fits this many doubles stage everything (several such waves
+// still share a compute unit's LDS).
+constexpr int kTrFullDoubles = 8192;
+
+int tr_lds_full_base(const fa_reference* r)
 {
    const int n = r->nD * r->n_dim;
    return tr_lds_need(r) + (int)r->chunks.size() * kPassStride + 2 * r->nD * r->n_dim * r->n_dim + 3 * n +
           (r->nD + 1) / 2;
 }
 
-// Updates whose full staging fits this many doubles stage everything (several such waves
-// still share a compute unit's LDS).
-constexpr int kTrFullDoubles = 8192;
+// Q and the eigenvalues of the eigendecomposition path in LDS: n^2 and n doubles, each rounded
+// up to 16 bytes, one double of alignment slack before them.
+int tr_lds_q(const fa_reference* r)
+{
+   const int n = r->nD * r->n_dim;
+   return ((n * n + 1) & ~1) + 1 + ((n + 1) & ~1);  // Q, then the eigenvalues
+}
+
+// Full staging of the update: tr_lds_full_base, plus Q when the eigendecomposition path's Q
+// fits beside it within kTrFullDoubles (q_lds).
+int tr_lds_full(const fa_reference* r)
+{
+   return tr_lds_full_base(r) + (r->dev.q_lds ? tr_lds_q(r) : 0);
+}
 
 // Frames of references with at least this many nodes run their update with 256 threads (the
 // cyclic reduction's first levels then have a block per thread), others with one wave. The
@@ -472,6 +491,7 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       bf.f.dtype = frames[i]->dtype;
       bf.f.offset = frames[i]->offset;
       bf.ref = ri;
+      bf.r = r->dev;
       bf.n = r->nD * r->n_dim;
       bf.work_base = b->total_work;
       bf.x_base = b->x_off[i];
@@ -520,6 +540,7 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       return fail(set_err(FA_E_DEVICE, "batch upload failed"));
    }
    set_grids(b);
+   b->fold = (rs.size() == 1 && b->trc_lds.size() == 1) ? 1 : 0;
    *out = b;
    return FA_OK;
 }
@@ -597,6 +618,8 @@ int run_begin(fa_batch* b, const double* x)
    if (b->d_trace)
       HIPCHK(hipMemsetAsync(b->d_trace, 0, (size_t)b->frames.size() * b->trace_cap * b->trace_rec * sizeof(double), s));
    HIPCHK(launch_tr_init(b->dev(), 0, s));
+   // a folded schedule runs at the end of each update: the first round's runs here
+   if (b->fold) HIPCHK(launch_schedule(b->dev(), 1, s));
    b->finished = false;
    b->timed = 0;
    return FA_OK;
@@ -606,7 +629,7 @@ int run_round(fa_batch* b)
 {
    hipStream_t s = b->stream;
    const BatchDev B = b->dev();
-   HIPCHK(launch_schedule(B, 1, s));
+   if (!b->fold) HIPCHK(launch_schedule(B, 1, s));
    hipEvent_t es = nullptr, ee = nullptr;
    if (b->timing && b->timed < kMaxTimedRounds) {
       CHK(event_pair(b, b->timed, &es, &ee));
@@ -624,7 +647,8 @@ int run_round(fa_batch* b)
 // Refreshes the counters the host polls and copies them back (asynchronously).
 int run_poll(fa_batch* b)
 {
-   HIPCHK(launch_schedule(b->dev(), 0, b->stream));
+   // with the folded schedule the counters are current after every update
+   if (!b->fold) HIPCHK(launch_schedule(b->dev(), 0, b->stream));
    HIPCHK(hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(Counters), hipMemcpyDeviceToHost, b->stream));
    return FA_OK;
 }
@@ -675,7 +699,7 @@ int ensure_graph(fa_batch* b, int K)
    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
    int rc = FA_OK;
    for (int r = 0; r < K && rc == FA_OK; r++) {
-      if (launch_schedule(B, 1, s) != hipSuccess) rc = set_err(FA_E_DEVICE, "graph capture failed");
+      if (!b->fold && launch_schedule(B, 1, s) != hipSuccess) rc = set_err(FA_E_DEVICE, "graph capture failed");
       if (rc == FA_OK) rc = launch_passes(b, s);
       if (rc == FA_OK) rc = launch_updates(b, s);
    }
@@ -1210,6 +1234,7 @@ int server_step(fa_server* sv, int rounds)
             bf.f.offset = r.offset;
          }
          bf.ref = (int)(std::find(sv->refs.begin(), sv->refs.end(), ref) - sv->refs.begin());
+         bf.r = ref->dev;
          bf.trc = b->ref_trc[bf.ref];
          bf.n = ref->nD * ref->n_dim;
          const int i = base + na;
@@ -1371,6 +1396,7 @@ int align_noalloc(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_alig
    bf.f.dtype = f->dtype;
    bf.f.offset = f->offset;
    bf.ref = 0;
+   bf.r = f->ref->dev;
    bf.n = n;
    bf.trc = 0;
    BatchDev B{};
@@ -1719,10 +1745,11 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
    R.eig_lo = r->eig_lo;
    // small problems: eigendecomposition of H for the trust-region subproblem
    R.eig = 0;
+   R.q_lds = 0;
    R.ev_min = 0;
    const int nn = nD * m, mp = nn + (nn & 1);
    if (nn <= kEigMax && (int64_t)(2 * mp * mp + mp) * (int64_t)sizeof(double) <= ctx->lds_per_block) {
-      if ((rc = dalloc(&r->d_Q, (size_t)nn * nn)) || (rc = dalloc(&r->d_ev, nn))) return fail(rc);
+      if ((rc = dalloc(&r->d_Q, 2 * (size_t)nn * nn)) || (rc = dalloc(&r->d_ev, nn))) return fail(rc);  // Q, Q^T
       R.Q = r->d_Q;
       R.ev = r->d_ev;
       if (launch_eigen(R, r->d_Q, r->d_ev, d_info, s) != hipSuccess ||
@@ -1733,6 +1760,7 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
       }
       R.eig = info[0] != 0.0 ? 1 : 0;
       R.ev_min = info[1];
+      R.q_lds = R.eig && tr_lds_full_base(r) + tr_lds_q(r) <= kTrFullDoubles ? 1 : 0;
    }
    cleanup();
    ctx->live_refs++;

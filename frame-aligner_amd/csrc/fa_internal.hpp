@@ -62,10 +62,11 @@ struct RefDev {
    const double* Hdiag;    // nD blocks n_dim x n_dim
    const double* Hsub;     // nD-1 blocks, block (k+1, k)
    const double* L0;       // cyclic-reduction Cholesky factor of H (A, C, W blocks; valid when h_pd)
-   const double* Q;        // eigenvectors of H, n x n row-major, column i for ev[i] (valid when eig)
+   const double* Q;        // eigenvectors of H, n x n row-major, column i for ev[i], then Q^T (valid when eig)
    const double* ev;       // eigenvalues of H
    ScanDev scan;
    int32_t nx, ny, nz, nD, n_dim, n_chunks, h_pd, eig;
+   int32_t q_lds;          // 1: an update that stages its working set in LDS also stages Q
    double eig_lo;          // Gershgorin lower bound of the spectrum of H
    double ev_min;          // smallest eigenvalue (valid when eig)
    int64_t n_px;
@@ -89,6 +90,7 @@ struct BatchFrame {
    int64_t d_base;     // offset of this frame's nD x 3 trial displacement
    int32_t trc;        // launch class of its trust-region update (LDS size)
    int32_t pad;
+   RefDev r;           // its reference (a copy of refs[ref]: one load chain fewer per use)
 };
 
 struct TrParamsDev {
@@ -105,7 +107,7 @@ struct TrState {
 // Counters the schedule kernel maintains for the host and the pass kernel.
 struct Counters {
    int32_t n_active_frames;
-   int32_t pad;
+   int32_t upd_done;          // update workgroups finished in this round (folded schedule)
    int64_t n_active_work;
    int64_t n_active_work2;    // work items of active 2-D frames (their references come first)
    unsigned long long bytes;  // algorithmic bytes scheduled so far
@@ -158,6 +160,7 @@ struct BatchDev {
    double* trace;     // null, or trace_cap records of trace_rec doubles per frame
    int32_t trace_cap;
    int32_t trace_rec; // kTraceFields + 3 * (largest nD n_dim of the batch)
+   int32_t fold;      // 1: the update's last workgroup schedules the next round (one reference)
 };
 
 }  // namespace fa

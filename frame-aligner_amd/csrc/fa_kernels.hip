@@ -850,7 +850,13 @@ __global__ __launch_bounds__(kEigThreads) void k_eigen(RefDev R, double* Q, doub
          __syncthreads();
       }
    }
-   for (int i = t; i < n0 * n0; i += kEigThreads) Q[i] = V[(i / n0) * m + (i % n0)];
+   // Q, then its transpose (the subproblem reads Q column-wise for Q^T g and row-wise for
+   // Q w; with the transpose both reads are coalesced)
+   for (int i = t; i < n0 * n0; i += kEigThreads) {
+      const int r = i / n0, c = i - r * n0;
+      Q[i] = V[r * m + c];
+      Q[n0 * n0 + c * n0 + r] = V[r * m + c];
+   }
    double lo = INFINITY;
    for (int i = t; i < n0; i += kEigThreads) {
       ev[i] = A[i * m + i];
@@ -1388,7 +1394,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(M == 2 ?
       const int c = (int)(local / gn);
       const int j = gf + (int)(local - (int64_t)c * gn);
       const BatchFrame& bf = a.frames[a.active_frames[j]];
-      const RefDev& R = a.refs[bf.ref];
+      const RefDev& R = bf.r;
       const Chunk ch = R.chunks[c];
       double* out = a.partials + (bf.work_base + c) * kPassStride;
       if (bf.f.dtype == FA_U16)
@@ -1592,11 +1598,22 @@ struct TrScratch {
    double *fa, *fc, *fw;  // factor blocks A, C, W of H + lambda I
 };
 
+// Sum over the wave, the same value in every lane: lane pairs, quads, halves of rows and rows
+// of 16 by DPP moves, then rows 0+1 and 2+3 (row_bcast:15) and the two halves (row_bcast:31),
+// which leave the total in lane 63, read back as a scalar. A fixed order, and every step a
+// DPP move instead of an LDS permute.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-   return __shfl(v, 0, 64);
+   v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+   v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+   v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
+   v = dpp_add<0x140, 0xf>(v);  // row_mirror
+   v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+   v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Sum over the NT threads of the workgroup, the same value in every thread: the waves' sums in
@@ -1728,8 +1745,8 @@ __device__ int tr_subproblem(const RefDev& R, const TrScratch& S, const double* 
 // and three block-triangular solves; p = -Q (qv / (ev + lam)) is formed once, for the last
 // lambda at which the factor exists.
 template <int NT>
-__device__ int tr_subproblem_eig(const RefDev& R, const TrScratch& S, const double* g, double radius,
-                                 double eps, int max_iter, double* p, double* lam_out)
+__device__ int tr_subproblem_eig(const RefDev& R, const TrScratch& S, const double* qs, const double* g,
+                                 double radius, double eps, int max_iter, double* p, double* lam_out)
 {
    const int n = R.nD * R.n_dim;
    double* qv = S.fa;  // the factor scratch is free on this path
@@ -1738,9 +1755,40 @@ __device__ int tr_subproblem_eig(const RefDev& R, const TrScratch& S, const doub
    *lam_out = 0;
    __syncthreads();
    if (gnorm == 0) return 1;
+   // qv = Q^T g: lane i walks column i (lanes read consecutive doubles); eight loads are
+   // issued before their products, so the column's loads overlap
+   const double* __restrict__ Qm = R.Q;
+   const double* __restrict__ Qt = R.Q + (int64_t)n * n;
+   if (qs) {
+      // Q staged in LDS: lane i reads column i (consecutive addresses)
+      for (int i = threadIdx.x; i < n; i += NT) {
+         double s = 0;
+         int r = 0;
+         for (; r + 8 <= n; r += 8) {
+            double q[8], h[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+               q[u] = qs[(r + u) * n + i];
+               h[u] = g[r + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) s += q[u] * h[u];
+         }
+         for (; r < n; r++) s += qs[r * n + i] * g[r];
+         qv[i] = s;
+      }
+   } else
    for (int i = threadIdx.x; i < n; i += NT) {
       double s = 0;
-      for (int r = 0; r < n; r++) s += R.Q[(int64_t)r * n + i] * g[r];
+      int r = 0;
+      for (; r + 8 <= n; r += 8) {
+         double q[8];
+#pragma unroll
+         for (int u = 0; u < 8; u++) q[u] = Qm[(int64_t)(r + u) * n + i];
+#pragma unroll
+         for (int u = 0; u < 8; u++) s += q[u] * g[r + u];
+      }
+      for (; r < n; r++) s += Qm[(int64_t)r * n + i] * g[r];
       qv[i] = s;
    }
    __syncthreads();
@@ -1783,15 +1831,96 @@ __device__ int tr_subproblem_eig(const RefDev& R, const TrScratch& S, const doub
       double* w = S.fa + n;
       for (int i = threadIdx.x; i < n; i += NT) w[i] = qv[i] / (R.ev[i] + lam_sol);
       __syncthreads();
+      // p = -Q w: lane r walks row r of Q (in LDS), or column r of Q^T (global)
+      if (qs) {
+         for (int r = threadIdx.x; r < n; r += NT) {
+            double s = 0;
+            int i = 0;
+            for (; i + 8 <= n; i += 8) {
+               double q[8], h[8];
+#pragma unroll
+               for (int u = 0; u < 8; u++) {
+                  q[u] = qs[r * n + i + u];
+                  h[u] = w[i + u];
+               }
+#pragma unroll
+               for (int u = 0; u < 8; u++) s += q[u] * h[u];
+            }
+            for (; i < n; i++) s += qs[r * n + i] * w[i];
+            p[r] = -s;
+         }
+      } else
       for (int r = threadIdx.x; r < n; r += NT) {
          double s = 0;
-         for (int i = 0; i < n; i++) s += R.Q[(int64_t)r * n + i] * w[i];
+         int i = 0;
+         for (; i + 8 <= n; i += 8) {
+            double q[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) q[u] = Qt[(int64_t)(i + u) * n + r];
+#pragma unroll
+            for (int u = 0; u < 8; u++) s += q[u] * w[i + u];
+         }
+         for (; i < n; i++) s += Qt[(int64_t)i * n + r] * w[i];
          p[r] = -s;
       }
       *lam_out = lam_sol;
    }
    __syncthreads();
    return it;
+}
+
+// dst[0, n) = src[0, n) (doubles) by direct global-to-LDS loads: each wave instruction moves
+// 128 doubles (16 bytes per lane) into LDS without passing through registers, so all of them
+// are in flight at once. src and dst are 16-byte aligned; for odd n the last lane reads and
+// writes one double past n (the caller's buffers have it). The caller waits (vmcnt) and
+// synchronises before reading dst.
+template <int NT>
+__device__ __forceinline__ void dma_to_lds(double* dst, const double* __restrict__ src, int n)
+{
+   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+   for (int b = 128 * wv; b < n; b += 128 * (NT / 64))
+      if (b + 2 * ln < n)
+         __builtin_amdgcn_global_load_lds((const void*)(src + b + 2 * ln),
+                                          (__attribute__((address_space(3))) void*)(dst + b), 16, 0, 0);
+}
+
+// dst[0, n) = src[0, n) by the NT threads, eight loads per thread issued before their stores
+// (a loop of load-store pairs would wait out one load latency per element).
+template <int NT, typename T>
+__device__ __forceinline__ void copy_to_lds(T* dst, const T* __restrict__ src, int n)
+{
+   for (int i0 = threadIdx.x; i0 < n; i0 += 8 * NT) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = i0 + u * NT < n ? src[i0 + u * NT] : T(0);
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+         if (i0 + u * NT < n) dst[i0 + u * NT] = v[u];
+   }
+}
+
+// dst[0, n) = src[0, n) for doubles or 32-bit values: direct loads when both ends allow them
+// (16-byte aligned doubles: the even part by dma_to_lds and an odd last double by one thread;
+// 32-bit values one per lane), else through registers. The caller waits and synchronises.
+template <int NT>
+__device__ __forceinline__ void stage_to_lds(double* dst, const double* __restrict__ src, int n)
+{
+   if ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0) {
+      dma_to_lds<NT>(dst, src, n & ~1);
+      if ((n & 1) && threadIdx.x == 0) dst[n - 1] = src[n - 1];
+   } else {
+      copy_to_lds<NT>(dst, src, n);
+   }
+}
+
+template <int NT>
+__device__ __forceinline__ void stage_to_lds(int32_t* dst, const int32_t* __restrict__ src, int n)
+{
+   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+   for (int b = 64 * wv; b < n; b += 64 * (NT / 64))
+      if (b + ln < n)
+         __builtin_amdgcn_global_load_lds((const void*)(src + b + ln),
+                                          (__attribute__((address_space(3))) void*)(dst + b), 4, 0, 0);
 }
 
 __device__ __forceinline__ TrScratch scratch_of(const BatchDev& B, const BatchFrame& bf)
@@ -1803,7 +1932,7 @@ __device__ __forceinline__ TrScratch scratch_of(const BatchDev& B, const BatchFr
    S.p = v;
    S.g = v + n;
    S.y = v + 2 * n;
-   const RefDev& R = B.refs[bf.ref];
+   const RefDev& R = bf.r;
    const int64_t nbmm = (int64_t)R.nD * R.n_dim * R.n_dim;
    S.fa = B.fac + bf.fac_base;
    S.fc = S.fa + nbmm;
@@ -1843,7 +1972,7 @@ __global__ __launch_bounds__(64) void k_reduce_eval(BatchDev B)
 {
    const int j = blockIdx.x;
    const BatchFrame& bf = B.frames[j];
-   const RefDev& R = B.refs[bf.ref];
+   const RefDev& R = bf.r;
    const double* P = B.partials + bf.work_base * kPassStride;
    const double f = reduce_ssd<64>(R, P);
    double* g = B.vec + bf.vec_base + (int64_t)bf.n;
@@ -1855,7 +1984,7 @@ __global__ __launch_bounds__(64) void k_reduce_eval(BatchDev B)
 __device__ __forceinline__ void init_frame(const BatchDev& B, const int j, const int eval_only)
 {
    const BatchFrame& bf = B.frames[j];
-   const RefDev& R = B.refs[bf.ref];
+   const RefDev& R = bf.r;
    const double* x = B.x + bf.x_base;
    double* Dt = B.Dtrial + bf.d_base;
    const int m = R.n_dim;
@@ -1886,12 +2015,13 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
 {
    constexpr int MM = M * M;
    const BatchFrame& bf = B.frames[j];
-   RefDev R = B.refs[bf.ref];
+   RefDev R = bf.r;
    const TrParamsDev& prm = B.prm;
    const TrScratch G = scratch_of(B, bf);
    TrScratch S = G;
    const double* P = B.partials + bf.work_base * kPassStride;
    const int n = bf.n, nb = R.nD;
+   const double* qs = nullptr;  // Q in LDS (eigendecomposition path, full staging)
    if constexpr (LDS) {
       double* q = smem;
       S.fa = q;
@@ -1908,17 +2038,27 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
          S.p = S.x + n;
          S.g = S.p + n;
          int32_t* ich = (int32_t*)(S.g + n);
-         for (int i = threadIdx.x; i < np; i += NT) pl[i] = P[i];
-         for (int i = threadIdx.x; i < nb * MM; i += NT) {
-            hd[i] = R.Hdiag[i];
-            hs[i] = R.Hsub[i];
+         // every array is requested before any is waited for: direct global-to-LDS loads
+         // wherever the addresses allow them, so the whole working set arrives in about one
+         // memory latency
+         stage_to_lds<NT>(pl, P, np);
+         if (R.q_lds) {
+            // Q after the interval table, at the next 16-byte boundary
+            const uintptr_t a = (uintptr_t)((double*)ich + (nb + 1) / 2);
+            double* qd = (double*)((a + 15) & ~(uintptr_t)15);
+            stage_to_lds<NT>(qd, R.Q, n * n);
+            qs = qd;
+            double* evd = qd + ((n * n + 1) & ~1);
+            stage_to_lds<NT>(evd, R.ev, n);
+            R.ev = evd;
          }
-         for (int i = threadIdx.x; i < n; i += NT) {
-            S.x[i] = G.x[i];
-            S.p[i] = G.p[i];
-            S.g[i] = G.g[i];
-         }
-         for (int i = threadIdx.x; i < nb; i += NT) ich[i] = R.ichunk[i];
+         stage_to_lds<NT>(hd, R.Hdiag, nb * MM);
+         stage_to_lds<NT>(hs, R.Hsub, nb * MM);
+         stage_to_lds<NT>(S.x, G.x, n);
+         stage_to_lds<NT>(S.p, G.p, n);
+         stage_to_lds<NT>(S.g, G.g, n);
+         stage_to_lds<NT>(ich, R.ichunk, nb);
+         __builtin_amdgcn_s_waitcnt(0);  // the direct loads have landed in LDS
          __syncthreads();
          P = pl;
          R.Hdiag = hd;
@@ -2014,7 +2154,7 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
    }
    if (!st.done) {
       double lam;
-      st.sub_iters = R.eig ? tr_subproblem_eig<NT>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam)
+      st.sub_iters = R.eig ? tr_subproblem_eig<NT>(R, S, qs, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam)
                            : tr_subproblem<M, NT>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
       st.lambda = lam;
       st.phase = 1;
@@ -2060,21 +2200,76 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
 // threads per frame (the first levels of the cyclic reduction have nD/2 independent blocks),
 // short ones one wave; the width is a property of the reference, so a frame's result does
 // not depend on the batch it runs in.
+// The schedule of the next round for a batch of one reference, run by the update's last
+// workgroup (B.fold): every workgroup publishes its frame's done flag with a device-scope
+// atomic store, waits for it to complete and counts itself done with an atomic; the one that
+// completes the count reads the flags with device-scope atomic loads (no cache-wide fences)
+// and lists the frames still running in frame order, with the counters k_schedule keeps. The
+// first wave does it with ballots; the count is reset for the next round.
+__device__ void fold_schedule(const BatchDev& B)
+{
+   __shared__ int s_last;
+   __syncthreads();  // the frame's update (its state is written by thread 0) is complete
+   if (threadIdx.x == 0) {
+      int32_t* done = &B.state[blockIdx.x].done;
+      __hip_atomic_store(done, *done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_waitcnt(0);  // the flag is visible device-wide before the count
+      s_last = __hip_atomic_fetch_add(&B.counters->upd_done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+               (int)gridDim.x - 1;
+   }
+   __syncthreads();
+   if (!s_last || threadIdx.x >= 64) return;
+   const int lane = threadIdx.x, n = B.n_frames;
+   const RefDev& R = B.refs[0];
+   int total = 0;
+   unsigned long long bytes = 0;
+   for (int base = 0; base < n; base += 64) {
+      const int f = base + lane;
+      const bool act = f < n && !__hip_atomic_load(&B.state[f].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long m = __ballot(act);
+      if (act) {
+         B.active_frames[total + __popcll(m & ((1ull << lane) - 1))] = f;
+         bytes += (unsigned long long)R.n_px *
+                  (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + (R.n_dim == 3 ? 20 : 16));
+      }
+      total += __popcll(m);
+   }
+#pragma unroll
+   for (int off = 32; off >= 1; off >>= 1) bytes += __shfl_xor(bytes, off, 64);
+   if (lane == 0) {
+      const int64_t w = (int64_t)total * R.n_chunks;
+      B.run_first[0] = 0;
+      B.run_count[0] = total;
+      B.run_base[0] = 0;
+      B.run_base[1] = w;
+      B.counters->n_active_frames = total;
+      B.counters->n_active_work = w;
+      B.counters->n_active_work2 = R.n_dim == 2 ? w : 0;
+      if (total > 0) {
+         B.counters->bytes += bytes;
+         B.counters->rounds += 1;
+      }
+      __hip_atomic_store(&B.counters->upd_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   }
+}
+
 template <bool LDS, int NT, bool FULL>
 __global__ __launch_bounds__(NT) void k_tr_update(BatchDev B, int cls)
 {
    extern __shared__ double smem[];
    const int j = blockIdx.x;
-   if (B.state[j].done || B.frames[j].trc != cls) return;
-   if (B.refs[B.frames[j].ref].n_dim == 2) tr_update_frame<2, LDS, NT, FULL>(B, j, smem);
-   else tr_update_frame<3, LDS, NT, FULL>(B, j, smem);
+   if (!B.state[j].done && B.frames[j].trc == cls) {
+      if (B.frames[j].r.n_dim == 2) tr_update_frame<2, LDS, NT, FULL>(B, j, smem);
+      else tr_update_frame<3, LDS, NT, FULL>(B, j, smem);
+   }
+   if (B.fold) fold_schedule(B);
 }
 
 // One frame's update in the variant its class launches (NT threads, FULL staging).
 template <bool LDS, int NT>
 __device__ __forceinline__ void tr_update_variant(const BatchDev& B, const int j, const bool full, double* smem)
 {
-   const bool three = B.refs[B.frames[j].ref].n_dim == 3;
+   const bool three = B.frames[j].r.n_dim == 3;
    if constexpr (LDS) {
       if (full) {
          if (three) tr_update_frame<3, true, NT, true>(B, j, smem);
