@@ -1200,125 +1200,135 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    const int ro = cw >= kBlock ? 0 : (int)threadIdx.x / cw;
    // the loop is compiled once per sampler (staged tile or global loads), so the choice, which
    // is uniform over the work item, costs no instructions per pixel
+   // the interval's last pixel (no Jacobian row) as a row and column of this plane (its row
+   // is past the plane's rows when it lies in a later plane)
+   const int ep = ch.end_p - pb, erow = ep / nx, ecol = ep - erow * nx;
    auto rows_loop = [&](auto use_tile) {
       constexpr bool TILE = decltype(use_tile)::value;
       for (int c = cx + (int)threadIdx.x - ro * cw; c < cx + cw; c += kBlock) {
          const int rf = (pb + r0 * nx + c >= ch.p0 ? r0 : r0 + 1) + ro;
          const int rl = pb + r1 * nx + c < ch.p1 ? r1 : r1 - 1;
+         const int yend = c == ecol ? erow : -1;
          // byte offset in the tile of the pixel's column shifted by -K0 (row and plane offsets
          // are relative to row r0 and the first staged plane, where K1 and K2 put them)
          const int colb = 4 * (c - cx + kPad - Ksh[0]);
          const unsigned pb16 = 16u * (unsigned)(pb + c), pb4 = 4u * (unsigned)(pb + c);
-         // tile sampling takes kUnroll rows per step; global sampling (4 or 8 loads per pixel
-         // already in flight) one, which keeps it within the register budget
+         // one pixel of the column: row y, its reference record and node fraction
+         auto pixel = [&](const int y, const float4 rec, const float fsv) {
+            const double fd = (double)fsv;
+            int ti[3];
+            float wt[3];
+#pragma unroll
+            for (int q = 0; q < M; q++) {
+               double fr;
+               if constexpr (TILE) {
+                  // ti = floor(d) + K (d + K >= 0: truncation is the floor)
+                  const double d = fma(fd, Dd[q], Dsh[q]);
+                  fr = __builtin_amdgcn_fract(d);
+                  ti[q] = (int)d;
+               } else {
+                  const double d = fma(fd, Dd[q], Da[q]);
+                  const double fl = floor(d);
+                  fr = d - fl;
+                  ti[q] = (int)fl;
+               }
+               // below 1 after the rounding to float too, so a weight of 1 never hides the
+               // other tap
+               wt[q] = fminf((float)fr, 0x1.fffffep-1f);
+            }
+            float v = 0.f;
+            if constexpr (TILE) {
+               // byte offset in the tile with 24-bit products (full rate; tile offsets are far
+               // below 2^23): tile row y - r0 + floor(d1) + K1, column + floor(d0), plane
+               // floor(d2) + K2
+               int off = __mul24(y - r0 + ti[1], stride4) + colb;
+               off += ti[0] << 2;
+               if (M == 3) off += __mul24(ti[2], 4 * plane_step);
+               const float* L = (const float*)((const char*)lds + off);
+               const float top = fmaf(wt[0], L[1] - L[0], L[0]);
+               const float bot = fmaf(wt[0], L[S.stride + 1] - L[S.stride], L[S.stride]);
+               const float v0 = fmaf(wt[1], bot - top, top);
+               if (M == 3) {
+                  const float* L2 = L + plane_step;
+                  const float top2 = fmaf(wt[0], L2[1] - L2[0], L2[0]);
+                  const float bot2 = fmaf(wt[0], L2[S.stride + 1] - L2[S.stride], L2[S.stride]);
+                  const float v1 = fmaf(wt[1], bot2 - top2, top2);
+                  v = fmaf(wt[2], v1 - v0, v0);
+               } else {
+                  v = v0;
+               }
+            } else {
+               // the taps are read from clamped, in-bounds addresses and the ones outside the
+               // frame replaced by 0 afterwards, so all of them are loads without branches that
+               // issue together (non-finite displacements sample nothing)
+               const int x0 = c + ti[0], y0 = y + ti[1];
+               const int z0 = M == 3 ? plane + ti[2] : 0;
+               const int xa = min(max(x0, 0), nx - 1), xb = min(max(x0 + 1, 0), nx - 1);
+               const int ya = min(max(y0, 0), ny - 1), yb = min(max(y0 + 1, 0), ny - 1);
+               const bool vxa = (unsigned)x0 < (unsigned)nx, vxb = (unsigned)(x0 + 1) < (unsigned)nx;
+               const bool vya = (unsigned)y0 < (unsigned)ny && finite;
+               const bool vyb = (unsigned)(y0 + 1) < (unsigned)ny && finite;
+               auto bilinear = [&](int zz, bool vz) -> float {
+                  const int zc = M == 3 ? min(max(zz, 0), nz - 1) : 0;
+                  const unsigned ra = (unsigned)((zc * ny + ya) * nx), rb = (unsigned)((zc * ny + yb) * nx);
+                  const float a0 = (float)gld(src, ra + xa), a1 = (float)gld(src, ra + xb);
+                  const float b0 = (float)gld(src, rb + xa), b1 = (float)gld(src, rb + xb);
+                  const float t00 = vz && vya && vxa ? a0 + off : 0.f;
+                  const float t01 = vz && vya && vxb ? a1 + off : 0.f;
+                  const float t10 = vz && vyb && vxa ? b0 + off : 0.f;
+                  const float t11 = vz && vyb && vxb ? b1 + off : 0.f;
+                  const float top = fmaf(wt[0], t01 - t00, t00);
+                  const float bot = fmaf(wt[0], t11 - t10, t10);
+                  return fmaf(wt[1], bot - top, top);
+               };
+               const float v0 = bilinear(z0, M != 3 || (unsigned)z0 < (unsigned)nz);
+               if (M == 3) {
+                  const float v1 = bilinear(z0 + 1, (unsigned)(z0 + 1) < (unsigned)nz);
+                  v = fmaf(wt[2], v1 - v0, v0);
+               } else {
+                  v = v0;
+               }
+            }
+            if (MODE == MODE_WARP) {
+               wout[pb + y * nx + c] = (v == 0.f) ? invalid : v;
+               return;
+            }
+            // no sample (v == 0): residual 0, outside the SSD; the interval's last pixel has no
+            // Jacobian row
+            const float e = v != 0.f ? v - rec.x : 0.f;
+            const float ej = y != yend ? e : 0.f;
+            const double ed = (double)e;
+            acc[0] = fma(ed, ed, acc[0]);
+            const float g[3] = {rec.y, rec.z, rec.w};
+#pragma unroll
+            for (int q = 0; q < M; q++) {
+               const double ge = (double)(g[q] * ej);
+               acc[1 + q] += ge;
+               acc[1 + M + q] = fma(ge, fd, acc[1 + M + q]);
+            }
+         };
+         // tile sampling takes kUnroll rows per step (their records loaded before any is used),
+         // then the column's last rows one at a time; global sampling (4 or 8 loads per pixel
+         // already in flight) one row per step, which keeps it within the register budget
          constexpr int kU = TILE ? kUnroll : 1;
-         for (int r = rf; r <= rl; r += kU * rpar) {
+         int r = rf;
+         for (; r + (kU - 1) * rpar <= rl; r += kU * rpar) {
             float4 recs[kU];
             float fs[kU];
-            int rows[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-               rows[u] = min(r + u * rpar, rl);
-               recs[u] = gld_bytes(rg4, __umul24((unsigned)rows[u], nx16) + pb16);
-               fs[u] = (M == 3) ? gld_bytes(dfp, __umul24((unsigned)rows[u], nx4) + pb4) : recs[u].w;
+               const unsigned row = (unsigned)(r + u * rpar);
+               recs[u] = gld_bytes(rg4, __umul24(row, nx16) + pb16);
+               fs[u] = (M == 3) ? gld_bytes(dfp, __umul24(row, nx4) + pb4) : recs[u].w;
             }
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-               const bool in = r + u * rpar <= rl;
-               const int y = rows[u];
-               const float4 rec = recs[u];
-               const double fd = (double)fs[u];
-               int ti[3];
-               float wt[3];
-#pragma unroll
-               for (int q = 0; q < M; q++) {
-                  double fr;
-                  if constexpr (TILE) {
-                     // ti = floor(d) + K (d + K >= 0: truncation is the floor)
-                     const double d = fma(fd, Dd[q], Dsh[q]);
-                     fr = __builtin_amdgcn_fract(d);
-                     ti[q] = (int)d;
-                  } else {
-                     const double d = fma(fd, Dd[q], Da[q]);
-                     const double fl = floor(d);
-                     fr = d - fl;
-                     ti[q] = (int)fl;
-                  }
-                  // below 1 after the rounding to float too, so a weight of 1 never hides the
-                  // other tap
-                  wt[q] = fminf((float)fr, 0x1.fffffep-1f);
-               }
-               float v = 0.f;
-               if constexpr (TILE) {
-                  // byte offset in the tile with 24-bit products (full rate; tile offsets are far
-                  // below 2^23): tile row y - r0 + floor(d1) + K1, column + floor(d0), plane
-                  // floor(d2) + K2
-                  int off = __mul24(y - r0 + ti[1], stride4) + colb;
-                  off += ti[0] << 2;
-                  if (M == 3) off += __mul24(ti[2], 4 * plane_step);
-                  const float* L = (const float*)((const char*)lds + off);
-                  const float top = fmaf(wt[0], L[1] - L[0], L[0]);
-                  const float bot = fmaf(wt[0], L[S.stride + 1] - L[S.stride], L[S.stride]);
-                  const float v0 = fmaf(wt[1], bot - top, top);
-                  if (M == 3) {
-                     const float* L2 = L + plane_step;
-                     const float top2 = fmaf(wt[0], L2[1] - L2[0], L2[0]);
-                     const float bot2 = fmaf(wt[0], L2[S.stride + 1] - L2[S.stride], L2[S.stride]);
-                     const float v1 = fmaf(wt[1], bot2 - top2, top2);
-                     v = fmaf(wt[2], v1 - v0, v0);
-                  } else {
-                     v = v0;
-                  }
-               } else {
-                  // the taps are read from clamped, in-bounds addresses and the ones outside the
-                  // frame replaced by 0 afterwards, so all of them are loads without branches that
-                  // issue together (non-finite displacements sample nothing)
-                  const int x0 = c + ti[0], y0 = y + ti[1];
-                  const int z0 = M == 3 ? plane + ti[2] : 0;
-                  const int xa = min(max(x0, 0), nx - 1), xb = min(max(x0 + 1, 0), nx - 1);
-                  const int ya = min(max(y0, 0), ny - 1), yb = min(max(y0 + 1, 0), ny - 1);
-                  const bool vxa = (unsigned)x0 < (unsigned)nx, vxb = (unsigned)(x0 + 1) < (unsigned)nx;
-                  const bool vya = (unsigned)y0 < (unsigned)ny && finite;
-                  const bool vyb = (unsigned)(y0 + 1) < (unsigned)ny && finite;
-                  auto bilinear = [&](int zz, bool vz) -> float {
-                     const int zc = M == 3 ? min(max(zz, 0), nz - 1) : 0;
-                     const unsigned ra = (unsigned)((zc * ny + ya) * nx), rb = (unsigned)((zc * ny + yb) * nx);
-                     const float a0 = (float)gld(src, ra + xa), a1 = (float)gld(src, ra + xb);
-                     const float b0 = (float)gld(src, rb + xa), b1 = (float)gld(src, rb + xb);
-                     const float t00 = vz && vya && vxa ? a0 + off : 0.f;
-                     const float t01 = vz && vya && vxb ? a1 + off : 0.f;
-                     const float t10 = vz && vyb && vxa ? b0 + off : 0.f;
-                     const float t11 = vz && vyb && vxb ? b1 + off : 0.f;
-                     const float top = fmaf(wt[0], t01 - t00, t00);
-                     const float bot = fmaf(wt[0], t11 - t10, t10);
-                     return fmaf(wt[1], bot - top, top);
-                  };
-                  const float v0 = bilinear(z0, M != 3 || (unsigned)z0 < (unsigned)nz);
-                  if (M == 3) {
-                     const float v1 = bilinear(z0 + 1, (unsigned)(z0 + 1) < (unsigned)nz);
-                     v = fmaf(wt[2], v1 - v0, v0);
-                  } else {
-                     v = v0;
-                  }
-               }
-               const int p = pb + y * nx + c;
-               if (MODE == MODE_WARP) {
-                  if (in) wout[p] = (v == 0.f) ? invalid : v;
-                  continue;
-               }
-               // no sample (v == 0): residual 0, outside the SSD; the interval's last pixel has no
-               // Jacobian row
-               const float e = (in && v != 0.f) ? v - rec.x : 0.f;
-               const float ej = (p != ch.end_p) ? e : 0.f;
-               const double ed = (double)e;
-               acc[0] = fma(ed, ed, acc[0]);
-               const float g[3] = {rec.y, rec.z, rec.w};
-#pragma unroll
-               for (int q = 0; q < M; q++) {
-                  const double ge = (double)(g[q] * ej);
-                  acc[1 + q] += ge;
-                  acc[1 + M + q] = fma(ge, fd, acc[1 + M + q]);
-               }
+            for (int u = 0; u < kU; u++) pixel(r + u * rpar, recs[u], fs[u]);
+         }
+         if constexpr (kU > 1) {
+            for (; r <= rl; r += rpar) {
+               const float4 rec = gld_bytes(rg4, __umul24((unsigned)r, nx16) + pb16);
+               const float fsv = (M == 3) ? gld_bytes(dfp, __umul24((unsigned)r, nx4) + pb4) : rec.w;
+               pixel(r, rec, fsv);
             }
          }
       }
