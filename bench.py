@@ -27,19 +27,46 @@ import numpy as np  # noqa: E402
 
 METRIC = "aligned frames/s @512²,30 nodes (≤0.02px err); p50/p99 latency; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pass_traffic_512x512_200f_nD31.txt")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
-def measured_traffic():
-    """HBM bytes per k_pass launch from the committed rocprof FETCH_SIZE/WRITE_SIZE summary
-    (counters scaled by a plain copy of known size), or None."""
+def profile_facts(name):
+    """`key: value` lines of a committed rocprof summary under profiles/ (HBM traffic per
+    launch from FETCH_SIZE/WRITE_SIZE scaled by a copy of known size, the commit the counters
+    were taken at, the measured limiter), or {}."""
+    out = {}
     try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
+        for line in open(os.path.join(PROFILES, name)):
+            k, sep, v = line.partition(":")
+            if sep and k.strip() in ("traffic_bytes_per_launch", "algorithmic_bytes_per_launch",
+                                      "commit", "limiter", "workload"):
+                v = v.strip()
+                try:
+                    v = float(v)
+                except ValueError:
+                    pass
+                out[k.strip()] = v
     except OSError:
         pass
-    return None
+    return out
+
+
+def traffic_of(name, achieved_gbs):
+    """The roofline's `traffic` (HBM bytes per launch, measured by rocprof for the same
+    workload) and its provenance, or (None, None)."""
+    f = profile_facts(name)
+    t = f.get("traffic_bytes_per_launch")
+    if t is None:
+        return None, None
+    src = {"file": "profiles/" + name, "commit": f.get("commit"), "workload": f.get("workload")}
+    a = f.get("algorithmic_bytes_per_launch")
+    if a:
+        src["dram_over_algorithmic"] = round(t / a, 3)
+        # DRAM bytes moved at the measured launch rate, against the 8 TB/s peak
+        src["dram_frac_of_peak"] = round(achieved_gbs * t / a / HBM_PEAK_GBS, 4)
+    if f.get("limiter"):
+        src["limiter"] = f["limiter"]
+    return t, src
 
 
 def log(*a):
@@ -228,29 +255,47 @@ def serving_512(F, args, ctx, plain_fps):
 
 def timed_steps(batch, steps):
     """`steps` untimed runs (graph replay where tuned), then `steps` runs with HIP events around
-    every fused pass. Returns (results, stats, wall s per run, pass ms, pass bytes, launches)."""
+    every fused pass. Returns (results, stats, wall s per run, pass timing sums)."""
     t = time.perf_counter()
     for _ in range(steps):
         xs, stats, _ = batch.run()
     wall = (time.perf_counter() - t) / steps
     batch.set_timing(True)
-    pms = pb = nl = 0
+    tot = {"pass_ms": 0.0, "pass_bytes": 0.0, "pass_px": 0.0, "pass_launches": 0}
     for _ in range(steps):
         _, _, tm = batch.run()
-        pms += tm["pass_ms"]
-        pb += tm["pass_bytes"]
-        nl += tm["pass_launches"]
+        for k in tot:
+            tot[k] += tm[k]
     batch.set_timing(False)
-    return xs, stats, wall, pms, pb, nl
+    return xs, stats, wall, tot
 
 
-def roofline_of(pass_ms, pass_bytes, launches, kernel):
+# bytes per pixel the pass reads beyond the algorithmic ones: the node fraction (3-D, its own
+# array) or the 2-D record's fourth float (the node fraction in the padding)
+RECORD_EXTRA_BYTES_PER_PX = 4
+
+
+def roofline_of(tm, kernel, traffic_file=None):
     """Pass roofline from HIP events around every fused pass (eager runs; a batch split into two
-    parts runs them one after the other in those runs, so each pass is timed alone)."""
-    gbs = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": kernel, "launches": int(launches),
-            "avg_launch_us": round(1e3 * pass_ms / max(launches, 1), 2)}
+    parts runs them one after the other in those runs, so each pass is timed alone). achieved =
+    algorithmic bytes (per active pixel: frame sample, reference intensity, n_dim gradient
+    components) / summed pass time; the bytes the pass actually requests (plus the node
+    fraction) are reported beside it."""
+    ms, launches = tm["pass_ms"], max(int(tm["pass_launches"]), 1)
+    gbs = tm["pass_bytes"] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    rec = (tm["pass_bytes"] + RECORD_EXTRA_BYTES_PER_PX * tm["pass_px"]) / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    out = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+           "bytes_per_px": "s_f + 4 + 4 n_dim (frame, reference, gradient)",
+           "launches": int(tm["pass_launches"]), "avg_launch_us": round(1e3 * ms / launches, 2),
+           "algorithmic_bytes_per_launch": round(tm["pass_bytes"] / launches),
+           "requested_bytes": {"achieved": round(rec, 1), "frac": round(rec / HBM_PEAK_GBS, 4),
+                               "bytes_per_px": "algorithmic + 4 (node fraction)"}}
+    if traffic_file:
+        out["traffic"], src = traffic_of(traffic_file, gbs)
+        if src:
+            out["traffic_source"] = src
+    return out
 
 
 def precision_3d(F, args, ctx):
@@ -276,7 +321,7 @@ def precision_3d(F, args, ctx):
     frames = [ref.register(synth.with_offset(f)) for f in st["frames"]]
     b = F.Batch(frames, **tr)
     b.run()
-    xs, stats, wall, pms, pb, nl = timed_steps(b, args.steps)
+    xs, stats, wall, tm = timed_steps(b, args.steps)
     n = len(frames)
     out = {"workload": f"256x256x32 float32, {n} volumes, nD=513 x 3 components, TR to 1e-4 rel or 50 "
                        f"iterations, batched", "volumes": n,
@@ -284,7 +329,8 @@ def precision_3d(F, args, ctx):
            "mvox_per_s": round(n * ref.n_px / wall / 1e6, 1),
            "evaluations_per_volume": {"mean": float(np.mean([s_["evaluations"] for s_ in stats])),
                                       "max": int(np.max([s_["evaluations"] for s_ in stats]))},
-           "roofline": roofline_of(pms, pb, nl, "k_pass<3> (fused trilinear warp + SSD + Jtr)")}
+           "roofline": roofline_of(tm, "k_pass<3> (fused trilinear warp + SSD + Jtr)",
+                                   "k_pass3_traffic_256x256x32_50v_nD513.txt")}
     if not args.no_cpu:
         k = min(args.c3_oracle, n)
         host = host_cpu(args.cpu_threads)
@@ -380,19 +426,44 @@ def uneven_batch(F, args, ctx):
         best, _, trials = tune.tune_batch(b, reps=2, sweeps=1)
         log(f"uneven_batch: autotuner {best} ({len(trials)} shapes)")
     b.run()
-    xs, stats, wall, pms, pb, nl = timed_steps(b, max(1, args.steps - 1))
+    xs, stats, wall, tm = timed_steps(b, max(1, args.steps - 1))
     out = {"workload": f"{len(stacks)} stacks, {n} frames, {vox / 1e9:.2f} Gvoxel per pass over all frames",
            "stacks": stacks, "frames": n, "frames_per_s": round(n / wall, 1), "ms_per_step": round(1e3 * wall, 2),
            "mvox_per_s": round(vox / wall / 1e6, 1), "tuning": b.tuning,
            "evaluations_per_frame": {"mean": float(np.mean([s_["evaluations"] for s_ in stats])),
                                      "max": int(np.max([s_["evaluations"] for s_ in stats]))},
-           "roofline": roofline_of(pms, pb, nl, "k_pass<2>, k_pass<3>")}
+           "roofline": roofline_of(tm, "k_pass<2>, k_pass<3>")}
     b.close()
     for f in frames:
         f.release()
     for r in refs:
         r.release()
     return out
+
+
+def large_working_set(F, args, ref, raw, tuning, tr, xs_ref):
+    """The headline workload with args.large_frames frames resident at once (the 200 distinct
+    frames each registered several times, so every copy is its own device buffer): frames,
+    reference record and work-item partials exceed the 256 MB last-level cache (MALL), so the
+    pass streams its frames from HBM. frames/s, the pass roofline by HIP events, and whether
+    every copy's result equals the 200-frame batch's for the same frame."""
+    n = args.large_frames
+    frames = [ref.register(raw[i % len(raw)]) for i in range(n)]
+    b = F.Batch(frames, **tr)
+    b.set_tuning(**tuning)
+    b.run()
+    xs, _, wall, tm = timed_steps(b, args.steps)
+    same = all(np.array_equal(xs[i], xs_ref[i % len(raw)]) for i in range(n))
+    b.close()
+    for f in frames:
+        f.release()
+    frame_bytes = sum(f.nbytes for f in raw[:1]) * n
+    return {"workload": f"512x512 uint16, {n} frames ({len(raw)} distinct, each a separate device copy), "
+                        f"nD=31, TR to 1e-4 rel or 50 iterations, batched",
+            "frames": n, "frame_bytes": int(frame_bytes), "frames_per_s": round(n / wall, 1),
+            "ms_per_step": round(1e3 * wall, 3), "same_result_as_200_frame_batch": same,
+            "roofline": roofline_of(tm, "k_pass<2> (fused bilinear warp + SSD + Jtr)",
+                                    f"k_pass_traffic_512x512_{n}f_nD31.txt")}
 
 
 def headline(F, args, ctx):
@@ -434,26 +505,20 @@ def headline(F, args, ctx):
     # the same steps again with HIP events around every fused pass (eager launches, since
     # events inside a replayed graph cannot be read): per-pass kernel time and bytes
     batch.set_timing(True)
-    pass_ms = 0.0
-    pass_bytes = 0.0
-    launches = 0
-    rounds = 0
+    tsum = {"pass_ms": 0.0, "pass_bytes": 0.0, "pass_px": 0.0, "pass_launches": 0}
     timed_wall = 0.0
     for k in range(args.steps):
         t = time.perf_counter()
         xs_t, _, tm = batch.run()
         timed_wall += time.perf_counter() - t
-        pass_ms += tm["pass_ms"]
-        pass_bytes += tm["pass_bytes"]
-        launches += tm["pass_launches"]
-        rounds += tm["tr_rounds"]
+        for f in tsum:
+            tsum[f] += tm[f]
         log(f"timed step {k}: {tm['wall_ms']:.1f} ms, {tm['pass_launches']} passes, "
             f"pass {tm['pass_ms']:.2f} ms")
     batch.set_timing(False)
     n = args.frames
     fps = n * args.steps / tot_wall
     ms_per_step = 1e3 * tot_wall / args.steps
-    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     evals = [s["evaluations"] for s in stats]
     same_timed = all(np.array_equal(a, b) for a, b in zip(xs, xs_t))
 
@@ -502,18 +567,21 @@ def headline(F, args, ctx):
                                  "p99": round(float(np.percentile(lat, 99)), 3)},
                   "same_result_as_batched": plain_same},
         "batched_vs_plain": round(fps / plain_fps, 2),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(),
-                     "kernel": "k_pass (fused warp + SSD + Jtr)",
-                     "measured": f"HIP events around every fused pass of {args.steps} more steps of "
-                                 f"the same workload (eager launches; the batch's two parts run one "
-                                 f"after the other, so each pass is timed alone)",
-                     "launches": launches, "avg_launch_us": round(1e3 * pass_ms / max(launches, 1), 2),
-                     "bytes_per_launch": pass_bytes / max(launches, 1),
-                     "isolated": {"avg_launch_us": round(1e3 * iso_ms, 2), "bytes_per_launch": iso_bytes,
-                                  "achieved": round(iso_bytes / (iso_ms * 1e-3) / 1e9, 1) if iso_ms else None}},
+        "roofline": roofline_of(tsum, "k_pass<2> (fused bilinear warp + SSD + Jtr)",
+                                "k_pass_traffic_512x512_200f_nD31.txt"),
     }
+    out["roofline"]["measured"] = (f"HIP events around every fused pass of {args.steps} more steps of the "
+                                   f"same workload (eager launches; the batch's two parts run one after the "
+                                   f"other, so each pass is timed alone)")
+    if iso_ms:
+        out["roofline"]["isolated"] = {"avg_launch_us": round(1e3 * iso_ms, 2), "bytes_per_launch": iso_bytes,
+                                       "achieved": round(iso_bytes / (iso_ms * 1e-3) / 1e9, 1),
+                                       "what": "one pass with every frame active, nothing else on the GPU"}
+    batch.close()
+    for f in frames:
+        f.release()
+    if args.large_frames:
+        out["large_working_set"] = large_working_set(F, args, ref, st["frames"], tuning, tr, xs)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -587,6 +655,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=200)
     ap.add_argument("--plain-frames", type=int, default=40)
+    ap.add_argument("--large-frames", type=int, default=1000,
+                    help="frames of the headline workload's run beyond the last-level cache (0: skip)")
     ap.add_argument("--cpu-frames", type=int, default=200)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="CPU baseline threads (0: hardware_concurrency within one GPU's share of the host)")

@@ -11,6 +11,7 @@
 #include <deque>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fa_kernels.hpp"
@@ -603,8 +604,11 @@ int event_pair(fa_batch* b, size_t i, hipEvent_t* s, hipEvent_t* e)
 }
 
 constexpr size_t kMaxTimedRounds = 4096;
-constexpr int kServerSplitSlots = 16;  // servers with at least this many slots run kServerParts
+constexpr int kServerSplitSlots = 16;  // a one-lane server with at least this many slots runs kServerParts
 constexpr int kServerParts = 2;        // slot batches (at least 8 slots each)
+constexpr int kMaxLanes = 3;           // server lanes (each on its own stream, beside the copy stream)
+constexpr int64_t kLaneSpan = 16;      // a lane's references span at most this ratio of pixels
+constexpr size_t kArenaMargin = (size_t)32 << 20;  // device memory a default arena leaves free
 constexpr int kSplitFrames = 64;  // batches of at least this many frames run as two parts
 
 int64_t max_rounds(const fa_batch* b) { return b->prm.max_evals > 0 ? (int64_t)b->prm.max_evals + 2 : 100000; }
@@ -684,6 +688,7 @@ int add_timing(fa_batch* b, fa_batch_timing* t)
    t->pass_ms += pass_ms;
    t->pass_launches += (int64_t)b->timed;
    t->pass_bytes += (double)b->h_counters->bytes;
+   t->pass_px += (double)b->h_counters->px;
    t->tr_rounds = std::max<int64_t>(t->tr_rounds, (int64_t)b->h_counters->rounds);
    return FA_OK;
 }
@@ -934,33 +939,60 @@ struct fa_server {
       float offset = 0.f;
       double* x = nullptr;
       int64_t tag = 0;
+      int64_t seq = 0;  // arrival order
    };
    enum : char { kFree = 0, kUploading = 1, kRunning = 2 };
    fa_context* ctx = nullptr;
-   // slot batches (one, or two on their own streams whose rounds interleave): each holds its
-   // slots' worth of state, sized for the largest reference; global slot j is local slot
-   // j - first[p] of part p, and the part's frames use the staging entries (h_bf, h_x,
-   // h_slots, d_slots) from first[p] on
+   // Lanes: the references grouped by kind and size. Each lane has its own slot batches
+   // (parts) on their own streams, launched and harvested on their own cadence, so a lane of
+   // small frames runs its rounds without waiting for the rounds of a lane of large ones. A
+   // server with one lane and 16 or more slots runs it as two parts whose rounds interleave
+   // (one part's update overlaps the other's pass). Global slot j is local slot j - first[p]
+   // of part p, whose frames use the staging entries (h_bf, h_x, h_slots) from
+   // first[p] on.
+   struct Lane {
+      std::vector<fa_reference*> refs;  // in the slot batches' launch order
+      int part_lo = 0, part_hi = 0;
+      std::deque<Req> pending;          // arrival order
+   };
+   std::vector<Lane> lanes;
+   std::vector<fa_reference*> refs;  // every served reference
+   std::vector<int> ref_lane;        // lane of refs[i]
    std::vector<fa_batch*> parts;
    std::vector<int> first;
-   std::vector<fa_reference*> refs;
-   int S = 0, nmax = 0;
+   std::vector<int> part_lane;
+   std::vector<char> launched;       // per part: rounds launched, results not yet harvested
+   std::vector<hipEvent_t> done_ev;  // per part: recorded after its launched rounds and copies
+   int S = 0;         // slots over all parts
+   int cap = 0;       // frames admitted at once (uploading or running), over all lanes
+   int admitted = 0;
+   int nmax = 0;
+   int64_t next_seq = 0;
    int part_of(int j) const
    {
       int p = 0;
       while (p + 1 < (int)first.size() && j >= first[p + 1]) p++;
       return p;
    }
-   std::deque<Req> pending;
+   int lane_of(const fa_reference* r) const
+   {
+      return ref_lane[std::find(refs.begin(), refs.end(), r) - refs.begin()];
+   }
    std::vector<std::vector<int>> free_slots;  // per part
-   // the part with the most free slots (frames spread evenly over the parts, so both streams
-   // carry work), or -1
-   int free_part() const
+   // the lane's part with the most free slots (frames spread evenly over a lane's parts, so
+   // both streams carry work), or -1
+   int free_part(const Lane& L) const
    {
       int best = -1;
-      for (int p = 0; p < (int)free_slots.size(); p++)
+      for (int p = L.part_lo; p < L.part_hi; p++)
          if (!free_slots[p].empty() && (best < 0 || free_slots[p].size() > free_slots[best].size())) best = p;
       return best;
+   }
+   size_t queued() const
+   {
+      size_t n = 0;
+      for (const Lane& L : lanes) n += L.pending.size();
+      return n;
    }
    std::vector<Req> slot_req;
    std::vector<char> sstate;
@@ -972,13 +1004,18 @@ struct fa_server {
    size_t arena_bytes = 0, arena_used = 0;
    std::map<size_t, size_t> free_blocks;
    hipStream_t cstream = nullptr;  // uploads (default priority)
-   BatchFrame* h_bf = nullptr;     // pinned staging
+   // pinned host staging, and the device's addresses of the same memory (the slot-start and
+   // publish kernels read and write it over the host link)
+   BatchFrame* h_bf = nullptr;
    double* h_x = nullptr;
    int32_t* h_slots = nullptr;
    TrState* h_state = nullptr;
    double* h_xall = nullptr;
-   int32_t* d_slots = nullptr;
-   bool launched = false;  // rounds launched by the last step, not yet harvested
+   BatchFrame* dh_bf = nullptr;
+   double* dh_x = nullptr;
+   int32_t* dh_slots = nullptr;
+   TrState* dh_state = nullptr;
+   double* dh_xall = nullptr;
 
    static size_t round_up(size_t n) { return (n + 255) & ~(size_t)255; }
 
@@ -1024,6 +1061,7 @@ struct fa_server {
          delete q;
       }
       for (auto e : up_ev) (void)hipEventDestroy(e);
+      for (auto e : done_ev) (void)hipEventDestroy(e);
       if (cstream) (void)hipStreamDestroy(cstream);
       dfree(d_arena);
       if (h_bf) (void)hipHostFree(h_bf);
@@ -1031,7 +1069,6 @@ struct fa_server {
       if (h_slots) (void)hipHostFree(h_slots);
       if (h_state) (void)hipHostFree(h_state);
       if (h_xall) (void)hipHostFree(h_xall);
-      dfree(d_slots);
    }
 };
 
@@ -1146,76 +1183,104 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
    return FA_OK;
 }
 
-// Waits for the rounds the previous step launched and returns its finished slots' results.
-int server_harvest(fa_server* sv)
+// Takes the results of the parts whose launched rounds have finished (their done events
+// have completed; nothing waits). *any: some part was harvested.
+int server_harvest(fa_server* sv, bool* any)
 {
-   if (!sv->launched) return FA_OK;
-   for (fa_batch* q : sv->parts) HIPCHK(hipStreamSynchronize(q->stream));
-   sv->launched = false;
-   for (int j = 0; j < sv->S; j++) {
-      if (sv->sstate[j] != fa_server::kRunning || !sv->h_state[j].done) continue;
-      const fa_server::Req& r = sv->slot_req[j];
-      const fa_reference* ref = r.f ? r.f->ref : r.ref;
-      const int n = ref->nD * ref->n_dim;
-      std::memcpy(r.x, sv->h_xall + (size_t)j * sv->nmax, n * sizeof(double));
-      fa_server_result res{};
-      res.tag = r.tag;
-      const TrState& t = sv->h_state[j];
-      res.stats.iterations = t.iterations;
-      res.stats.evaluations = t.evaluations;
-      res.stats.rejected = t.rejected;
-      res.stats.status = t.status;
-      res.stats.f_final = t.f;
-      res.stats.radius_final = t.radius;
-      sv->results.push_back(res);
-      if (sv->blk_size[j]) sv->arena_free(sv->blk_off[j], sv->blk_size[j]);
-      sv->blk_size[j] = 0;
-      sv->sstate[j] = fa_server::kFree;
-      sv->free_slots[sv->part_of(j)].push_back(j);
+   *any = false;
+   for (int p = 0; p < (int)sv->parts.size(); p++) {
+      if (!sv->launched[p]) continue;
+      const hipError_t q = hipEventQuery(sv->done_ev[p]);
+      if (q == hipErrorNotReady) continue;
+      HIPCHK(q);
+      sv->launched[p] = 0;
+      *any = true;
+      const int j0 = sv->first[p], j1 = j0 + sv->parts[p]->n_slots;
+      for (int j = j0; j < j1; j++) {
+         if (sv->sstate[j] != fa_server::kRunning || !sv->h_state[j].done) continue;
+         const fa_server::Req& r = sv->slot_req[j];
+         const fa_reference* ref = r.f ? r.f->ref : r.ref;
+         const int n = ref->nD * ref->n_dim;
+         // the part's parameter vectors as laid out on the device (slot strides of its lane)
+         const fa_batch* b = sv->parts[p];
+         std::memcpy(r.x, sv->h_xall + (size_t)j0 * sv->nmax + b->hframes[j - j0].x_base, n * sizeof(double));
+         fa_server_result res{};
+         res.tag = r.tag;
+         const TrState& t = sv->h_state[j];
+         res.stats.iterations = t.iterations;
+         res.stats.evaluations = t.evaluations;
+         res.stats.rejected = t.rejected;
+         res.stats.status = t.status;
+         res.stats.f_final = t.f;
+         res.stats.radius_final = t.radius;
+         sv->results.push_back(res);
+         if (sv->blk_size[j]) sv->arena_free(sv->blk_off[j], sv->blk_size[j]);
+         sv->blk_size[j] = 0;
+         sv->sstate[j] = fa_server::kFree;
+         sv->free_slots[p].push_back(j);
+         sv->admitted--;
+      }
    }
    return FA_OK;
 }
 
-int server_step(fa_server* sv, int rounds)
+// Takes queued requests in arrival order (over all lanes) while the server admits fewer than
+// cap frames, the request's lane has a free slot and, for host data, the frame arena has room
+// (admission by free memory: the oldest request that does not fit waits, and the ones behind
+// it with it). Host data is uploaded on the copy stream while running frames compute.
+int server_admit(fa_server* sv)
 {
-   // 0. results of the rounds the previous step launched
-   CHK(server_harvest(sv));
-   // 1. take queued requests, in arrival order, while a slot and (for host data) arena memory
-   //    are free; host data is uploaded on the copy stream while the running frames compute
    bool uploading = false;
-   while (sv->free_part() >= 0 && !sv->pending.empty()) {
-      fa_server::Req& r = sv->pending.front();
-      const int fp = sv->free_part();
+   while (sv->admitted < sv->cap) {
+      int li = -1;
+      for (int l = 0; l < (int)sv->lanes.size(); l++) {
+         const fa_server::Lane& L = sv->lanes[l];
+         if (L.pending.empty() || sv->free_part(L) < 0) continue;
+         if (li < 0 || L.pending.front().seq < sv->lanes[li].pending.front().seq) li = l;
+      }
+      if (li < 0) break;
+      fa_server::Lane& L = sv->lanes[li];
+      fa_server::Req& r = L.pending.front();
+      const int fp = sv->free_part(L);
       const int slot = sv->free_slots[fp].back();
       if (r.host) {
          const size_t nb = frame_bytes(r.ref, r.dtype);
          size_t off;
-         if (!sv->arena_alloc(nb, &off)) break;  // admission by free memory
+         if (!sv->arena_alloc(nb, &off)) break;
          sv->blk_off[slot] = off;
          sv->blk_size[slot] = nb;
          HIPCHK(hipMemcpyAsync(sv->d_arena + off, r.host, nb, hipMemcpyHostToDevice, sv->cstream));
-         HIPCHK(hipEventRecord(sv->up_ev[slot], sv->cstream));
       } else {
          sv->blk_size[slot] = 0;
-         HIPCHK(hipEventRecord(sv->up_ev[slot], sv->cstream));
       }
+      HIPCHK(hipEventRecord(sv->up_ev[slot], sv->cstream));
       sv->free_slots[fp].pop_back();
       sv->slot_req[slot] = r;
       sv->sstate[slot] = fa_server::kUploading;
-      sv->pending.pop_front();
+      L.pending.pop_front();
+      sv->admitted++;
       uploading = true;
    }
-   bool running = false;
-   for (int j = 0; j < sv->S; j++) running = running || sv->sstate[j] == fa_server::kRunning;
-   if (uploading && !running) HIPCHK(hipStreamSynchronize(sv->cstream));  // nothing to overlap with
-   // 2. start the frames whose data is on the device, part by part
+   bool busy = false;
+   for (char c : sv->launched) busy = busy || c;
+   if (uploading && !busy) HIPCHK(hipStreamSynchronize(sv->cstream));  // nothing to overlap with
+   return FA_OK;
+}
+
+// For every part without launched rounds: starts its frames whose data is on the device, then
+// launches `rounds` trust-region rounds over its running frames (one graph replay), the copies
+// of their states and parameter vectors to the host, and its done event.
+int server_launch(fa_server* sv, int rounds)
+{
    const int P = (int)sv->parts.size();
    for (int pi = 0; pi < P; pi++) {
+      if (sv->launched[pi]) continue;
       fa_batch* b = sv->parts[pi];
-      const int j0 = sv->first[pi], j1 = pi + 1 < P ? sv->first[pi + 1] : sv->S;
-      const int base = j0;  // the part's region of the staging arrays and d_slots
+      const fa_server::Lane& L = sv->lanes[sv->part_lane[pi]];
+      const int j0 = sv->first[pi], nj = b->n_slots;
+      const int base = j0;  // the part's region of the staging arrays
       int na = 0;
-      for (int slot = j0; slot < j1; slot++) {
+      for (int slot = j0; slot < j0 + nj; slot++) {
          if (sv->sstate[slot] != fa_server::kUploading) continue;
          const hipError_t q = hipEventQuery(sv->up_ev[slot]);
          if (q == hipErrorNotReady) continue;
@@ -1233,7 +1298,7 @@ int server_step(fa_server* sv, int rounds)
             bf.f.dtype = r.dtype;
             bf.f.offset = r.offset;
          }
-         bf.ref = (int)(std::find(sv->refs.begin(), sv->refs.end(), ref) - sv->refs.begin());
+         bf.ref = (int)(std::find(L.refs.begin(), L.refs.end(), ref) - L.refs.begin());
          bf.r = ref->dev;
          bf.trc = b->ref_trc[bf.ref];
          bf.n = ref->nD * ref->n_dim;
@@ -1245,41 +1310,59 @@ int server_step(fa_server* sv, int rounds)
          na++;
       }
       hipStream_t s = b->stream;
-      for (int i = base; i < base + na; i++) {
-         const int l = sv->h_slots[i];
-         HIPCHK(hipMemcpyAsync(b->d_frames + l, &sv->h_bf[i], sizeof(BatchFrame), hipMemcpyHostToDevice, s));
-         HIPCHK(hipMemcpyAsync(b->d_x + b->hframes[l].x_base, sv->h_x + (size_t)i * sv->nmax,
-                               sv->h_bf[i].n * sizeof(double), hipMemcpyHostToDevice, s));
-      }
-      if (na) {
-         HIPCHK(hipMemcpyAsync(sv->d_slots + base, sv->h_slots + base, na * sizeof(int32_t), hipMemcpyHostToDevice, s));
-         HIPCHK(launch_tr_init_slots(b->dev(), sv->d_slots + base, na, s));
-      }
-      running = running || na > 0;
-   }
-   if (!running) return FA_OK;
-   // 3. rounds over every running frame (one graph replay per part; the parts' streams
-   //    interleave, so one part's update overlaps the other's pass), then the states and
-   //    parameter vectors back to the host for the next step's harvest
-   for (int pi = 0; pi < P; pi++) {
-      fa_batch* b = sv->parts[pi];
-      hipStream_t s = b->stream;
-      const int j0 = sv->first[pi], nj = (pi + 1 < P ? sv->first[pi + 1] : sv->S) - j0;
+      // the part's stream touches no copy engine: slots start from the pinned staging and the
+      // results are published into pinned memory by kernels, so one lane's copies never wait
+      // behind another's in a shared copy-engine queue
+      if (na)
+         HIPCHK(launch_tr_start_slots(b->dev(), sv->dh_bf + base, sv->dh_x + (size_t)base * sv->nmax,
+                                      sv->dh_slots + base, na, sv->nmax, s));
       bool busy = false;
       for (int j = j0; j < j0 + nj; j++) busy = busy || sv->sstate[j] == fa_server::kRunning;
-      if (!busy) continue;  // its states on the host are still current (nothing ran)
+      if (!busy) continue;  // its states on the host are current (nothing ran)
       if (b->tune.use_graph) {
          CHK(ensure_graph(b, rounds));
          HIPCHK(hipGraphLaunch(b->gexec, s));
       } else {
          for (int r = 0; r < rounds; r++) CHK(run_round(b));
       }
-      CHK(run_poll(b));
-      HIPCHK(hipMemcpyAsync(sv->h_state + j0, b->d_state, nj * sizeof(TrState), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipMemcpyAsync(sv->h_xall + (size_t)j0 * sv->nmax, b->d_x, b->total_x * sizeof(double),
-                            hipMemcpyDeviceToHost, s));
+      HIPCHK(launch_publish(b->dev(), sv->dh_state + j0, sv->dh_xall + (size_t)j0 * sv->nmax, b->total_x, s));
+      HIPCHK(hipEventRecord(sv->done_ev[pi], s));
+      sv->launched[pi] = 1;
    }
-   sv->launched = true;
+   return FA_OK;
+}
+
+// How long a step waits for some part's rounds to finish when none had finished at its start.
+constexpr int kStepWaitUs = 500;
+
+// One server step: harvest the parts whose rounds finished, admit, start and launch the idle
+// parts; if no part had finished, wait up to kStepWaitUs for one to finish and do the same
+// again, so a step normally returns results and relaunches that part at once.
+int server_step(fa_server* sv, int rounds)
+{
+   bool any = false;
+   CHK(server_harvest(sv, &any));
+   CHK(server_admit(sv));
+   CHK(server_launch(sv, rounds));
+   if (any) return FA_OK;
+   const auto t0 = std::chrono::steady_clock::now();
+   for (;;) {
+      bool busy = false, ready = false;
+      for (int p = 0; p < (int)sv->parts.size(); p++) {
+         if (!sv->launched[p]) continue;
+         busy = true;
+         const hipError_t q = hipEventQuery(sv->done_ev[p]);
+         if (q == hipSuccess) ready = true;
+         else if (q != hipErrorNotReady) HIPCHK(q);
+      }
+      if (!busy) return FA_OK;
+      if (ready) break;
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kStepWaitUs)) return FA_OK;
+      std::this_thread::yield();
+   }
+   CHK(server_harvest(sv, &any));
+   CHK(server_admit(sv));
+   CHK(server_launch(sv, rounds));
    return FA_OK;
 }
 }  // namespace
@@ -2187,59 +2270,113 @@ int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs,
    auto* sv = new fa_server();
    sv->ctx = ctx;
    size_t largest = 0;
-   // the server's references in launch order (kind, then LDS need); slots index them
    for (int i = 0; i < n_refs; i++)
       if (std::find(sv->refs.begin(), sv->refs.end(), refs[i]) == sv->refs.end()) sv->refs.push_back(refs[i]);
-   order_refs(sv->refs);  // the slot batch's launch order
    for (fa_reference* r : sv->refs) {
       sv->nmax = std::max(sv->nmax, r->nD * r->n_dim);
       largest = std::max(largest, fa_server::round_up(frame_bytes(r, FA_F32)));
    }
-   sv->S = slots;
-   sv->arena_bytes = arena_bytes ? fa_server::round_up(arena_bytes) : largest * (size_t)slots;
-   // two slot batches when there are enough slots for both to fill their launches
-   const int P = slots >= kServerSplitSlots ? std::min(kServerParts, slots / 8) : 1;
-   int rc = FA_OK;
-   for (int pi = 0, j = 0; pi < P && rc == FA_OK; pi++) {
-      const int sp = slots / P + (pi < slots % P ? 1 : 0);
-      fa_batch* q = nullptr;
-      rc = slot_batch_build(ctx, sv->refs, sp, prm, &q);
-      if (rc != FA_OK) break;
-      sv->parts.push_back(q);
-      sv->first.push_back(j);
-      j += sp;
-   }
-   if (rc == FA_OK)
-      if ((rc = halloc(&sv->h_bf, slots)) || (rc = halloc(&sv->h_x, (size_t)slots * sv->nmax)) ||
-          (rc = halloc(&sv->h_slots, slots)) || (rc = halloc(&sv->h_state, slots)) ||
-          (rc = halloc(&sv->h_xall, (size_t)slots * sv->nmax)) || (rc = dalloc(&sv->d_slots, slots)) ||
-          (rc = dalloc(&sv->d_arena, sv->arena_bytes))) {
+   // lanes: by kind, then pixels; a new lane where a reference has more than kLaneSpan times
+   // the pixels of its lane's first; past kMaxLanes, the largest references share the last
+   {
+      std::vector<fa_reference*> rs = sv->refs;
+      std::stable_sort(rs.begin(), rs.end(), [](const fa_reference* a, const fa_reference* c) {
+         if (a->n_dim != c->n_dim) return a->n_dim < c->n_dim;
+         return a->n_px < c->n_px;
+      });
+      for (fa_reference* r : rs) {
+         if (sv->lanes.empty() || ((int)sv->lanes.size() < kMaxLanes &&
+                                   (sv->lanes.back().refs.front()->n_dim != r->n_dim ||
+                                    r->n_px > kLaneSpan * sv->lanes.back().refs.front()->n_px)))
+            sv->lanes.emplace_back();
+         sv->lanes.back().refs.push_back(r);
       }
+      sv->ref_lane.resize(sv->refs.size());
+      for (size_t i = 0; i < sv->refs.size(); i++)
+         for (int l = 0; l < (int)sv->lanes.size(); l++)
+            if (std::find(sv->lanes[l].refs.begin(), sv->lanes[l].refs.end(), sv->refs[i]) != sv->lanes[l].refs.end())
+               sv->ref_lane[i] = l;
+      for (auto& L : sv->lanes) order_refs(L.refs);  // the slot batch's launch order
+   }
+   sv->cap = slots;
+   // every lane can hold `slots` frames; a lone lane with enough slots runs as two parts
+   const int P1 = sv->lanes.size() == 1 && slots >= kServerSplitSlots ? std::min(kServerParts, slots / 8) : 1;
+   int rc = FA_OK;
+   for (int l = 0, j = 0; l < (int)sv->lanes.size() && rc == FA_OK; l++) {
+      fa_server::Lane& L = sv->lanes[l];
+      L.part_lo = (int)sv->parts.size();
+      for (int pi = 0; pi < P1 && rc == FA_OK; pi++) {
+         const int sp = slots / P1 + (pi < slots % P1 ? 1 : 0);
+         fa_batch* q = nullptr;
+         rc = slot_batch_build(ctx, L.refs, sp, prm, &q);
+         if (rc != FA_OK) break;
+         sv->parts.push_back(q);
+         sv->first.push_back(j);
+         sv->part_lane.push_back(l);
+         j += sp;
+      }
+      L.part_hi = (int)sv->parts.size();
+   }
+   sv->S = 0;
+   for (fa_batch* q : sv->parts) sv->S += q->n_slots;
+   // the frame arena: by default room for every slot's frame, within the device's free memory
+   // less a margin (GpuFrameWarper.cpp:113-130 sizes its frame count the same way)
+   if (rc == FA_OK) {
+      size_t free_b = 0, total_b = 0;
+      if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+         (void)hipGetLastError();
+         rc = set_err(FA_E_DEVICE, "hipMemGetInfo failed");
+      } else {
+         const size_t avail = free_b > kArenaMargin ? free_b - kArenaMargin : 0;
+         size_t want = arena_bytes ? fa_server::round_up(arena_bytes) : largest * (size_t)slots;
+         if (!arena_bytes) want = std::min(want, avail / 256 * 256);
+         if (want > avail || (!arena_bytes && want < largest))
+            rc = set_err(FA_E_OOM, "Not enough video memory to use GPU");
+         sv->arena_bytes = want;
+      }
+   }
+   const int S = sv->S;
+   if (rc == FA_OK)
+      if ((rc = halloc(&sv->h_bf, S)) || (rc = halloc(&sv->h_x, (size_t)S * sv->nmax)) ||
+          (rc = halloc(&sv->h_slots, S)) || (rc = halloc(&sv->h_state, S)) ||
+          (rc = halloc(&sv->h_xall, (size_t)S * sv->nmax)) || (rc = dalloc(&sv->d_arena, sv->arena_bytes))) {
+      }
+   if (rc == FA_OK &&
+       (hipHostGetDevicePointer((void**)&sv->dh_bf, sv->h_bf, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&sv->dh_x, sv->h_x, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&sv->dh_slots, sv->h_slots, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&sv->dh_state, sv->h_state, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&sv->dh_xall, sv->h_xall, 0) != hipSuccess)) {
+      (void)hipGetLastError();
+      rc = set_err(FA_E_DEVICE, "pinned staging is not mapped for the device");
+   }
    if (rc == FA_OK && hipStreamCreateWithFlags(&sv->cstream, hipStreamNonBlocking) != hipSuccess) {
       (void)hipGetLastError();
       sv->cstream = nullptr;
       rc = set_err(FA_E_DEVICE, "stream creation failed");
    }
-   for (int j = 0; rc == FA_OK && j < slots; j++) {
+   for (int j = 0; rc == FA_OK && j < S + (int)sv->parts.size(); j++) {
       hipEvent_t e;
       if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
          (void)hipGetLastError();
          rc = set_err(FA_E_DEVICE, "event creation failed");
          break;
       }
-      sv->up_ev.push_back(e);
+      if (j < S) sv->up_ev.push_back(e);
+      else sv->done_ev.push_back(e);
    }
    if (rc != FA_OK) {
       delete sv;
       return rc;
    }
    sv->free_blocks[0] = sv->arena_bytes;
-   sv->slot_req.resize(slots);
-   sv->sstate.assign(slots, fa_server::kFree);
-   sv->blk_off.assign(slots, 0);
-   sv->blk_size.assign(slots, 0);
+   sv->slot_req.resize(S);
+   sv->sstate.assign(S, fa_server::kFree);
+   sv->blk_off.assign(S, 0);
+   sv->blk_size.assign(S, 0);
+   sv->launched.assign(sv->parts.size(), 0);
    sv->free_slots.assign(sv->parts.size(), {});
-   for (int j = slots - 1; j >= 0; j--) sv->free_slots[sv->part_of(j)].push_back(j);
+   for (int j = S - 1; j >= 0; j--) sv->free_slots[sv->part_of(j)].push_back(j);
    for (fa_reference* r : sv->refs) r->live_frames++;  // the references outlive the server
    *out = sv;
    return FA_OK;
@@ -2265,7 +2402,8 @@ int fa_server_submit(fa_server* sv, fa_frame* frame, double* x_inout, int64_t ta
    r.f = frame;
    r.x = x_inout;
    r.tag = tag;
-   sv->pending.push_back(r);
+   r.seq = sv->next_seq++;
+   sv->lanes[sv->lane_of(frame->ref)].pending.push_back(r);
    return FA_OK;
 }
 
@@ -2286,7 +2424,8 @@ int fa_server_submit_host(fa_server* sv, fa_reference* ref, const void* data, in
    r.offset = add_offset;
    r.x = x_inout;
    r.tag = tag;
-   sv->pending.push_back(r);
+   r.seq = sv->next_seq++;
+   sv->lanes[sv->lane_of(ref)].pending.push_back(r);
    return FA_OK;
 }
 
@@ -2301,7 +2440,7 @@ int fa_server_step(fa_server* sv, int32_t rounds, int32_t* in_flight, int32_t* q
       for (char c : sv->sstate) n += c != fa_server::kFree;
       *in_flight = n;
    }
-   if (queued) *queued = (int32_t)sv->pending.size();
+   if (queued) *queued = (int32_t)sv->queued();
    return FA_OK;
 }
 

@@ -104,15 +104,27 @@ struct TrState {
        pad;
 };
 
+static_assert(sizeof(TrState) % 8 == 0, "trust-region states are copied as 8-byte words");
+static_assert(sizeof(BatchFrame) % 8 == 0, "frame records are copied as 8-byte words");
+
 // Counters the schedule kernel maintains for the host and the pass kernel.
 struct Counters {
    int32_t n_active_frames;
    int32_t upd_done;          // update workgroups finished in this round (folded schedule)
    int64_t n_active_work;
    int64_t n_active_work2;    // work items of active 2-D frames (their references come first)
-   unsigned long long bytes;  // algorithmic bytes scheduled so far
+   unsigned long long bytes;  // algorithmic bytes of the passes scheduled so far
+   unsigned long long px;     // active pixels of those passes
    unsigned long long rounds;
 };
+
+// Algorithmic bytes of one fused pass per pixel: the stored frame sample, the reference
+// intensity and its n_dim gradient components (s_f + 4 + 4 n_dim). The node fraction and the
+// 2-D record's padding, which the pass also reads, are not counted.
+__host__ __device__ constexpr int pass_bytes_per_px(int dtype, int n_dim)
+{
+   return (dtype == FA_U16 ? 2 : 4) + 4 + 4 * n_dim;
+}
 
 struct PassArgs {
    const RefDev* refs;

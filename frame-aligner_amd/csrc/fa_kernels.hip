@@ -1505,25 +1505,31 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
    __shared__ int32_t s_fill[kMaxRefs];
    __shared__ int32_t s_wf[1024 / 64];
    __shared__ int64_t s_ww[1024 / 64], s_ww2[1024 / 64];
-   __shared__ unsigned long long s_bytes;
+   __shared__ unsigned long long s_bytes, s_px;
    const int t = threadIdx.x, T = blockDim.x;
    const int n = B.n_frames, R = B.n_refs;
    for (int r = t; r < R; r += T) {
       s_cnt[r] = 0;
       s_fill[r] = 0;
    }
-   if (t == 0) s_bytes = 0;
+   if (t == 0) {
+      s_bytes = 0;
+      s_px = 0;
+   }
    __syncthreads();
-   unsigned long long bytes = 0;
+   unsigned long long bytes = 0, px = 0;
    for (int f = t; f < n; f += T) {
       if (B.state[f].done) continue;
       const int r = B.frames[f].ref;
       atomicAdd(&s_cnt[r], 1);
       const RefDev& RD = B.refs[r];
-      bytes += (unsigned long long)RD.n_px *
-               (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + (RD.n_dim == 3 ? 20 : 16));
+      bytes += (unsigned long long)RD.n_px * (unsigned long long)pass_bytes_per_px(B.frames[f].f.dtype, RD.n_dim);
+      px += (unsigned long long)RD.n_px;
    }
-   if (bytes) atomicAdd(&s_bytes, bytes);
+   if (bytes) {
+      atomicAdd(&s_bytes, bytes);
+      atomicAdd(&s_px, px);
+   }
    __syncthreads();
    // exclusive scans over the references (each thread a contiguous range of them)
    const int r0 = (int)((int64_t)R * t / T), r1 = (int)((int64_t)R * (t + 1) / T);
@@ -1579,6 +1585,7 @@ __global__ __launch_bounds__(1024) void k_schedule(BatchDev B, int account)
       B.counters->n_active_work2 = w2_in;
       if (account) {
          B.counters->bytes += s_bytes;
+         B.counters->px += s_px;
          B.counters->rounds += 1;
       }
    }
@@ -2012,8 +2019,51 @@ __device__ __forceinline__ void init_frame(const BatchDev& B, const int j, const
 
 __global__ void k_tr_init(BatchDev B, int eval_only) { init_frame(B, blockIdx.x, eval_only); }
 
-// Starts the listed slots of a serving batch.
-__global__ void k_tr_init_slots(BatchDev B, const int32_t* slots) { init_frame(B, slots[blockIdx.x], 0); }
+// Starts slots of a serving batch from pinned host staging, read directly over the host link
+// (no copy engine, whose queues other streams share): slot hslots[i] takes the frame record
+// hbf[i] and the start vector hx[i * nmax ...]; its trial displacement and trust-region state
+// are then reset as in init_frame.
+__global__ void k_tr_start_slots(BatchDev B, const BatchFrame* hbf, const double* hx, const int32_t* hslots,
+                                 int nmax)
+{
+   const int i = blockIdx.x;
+   const int j = hslots[i];
+   const BatchFrame* src = hbf + i;
+   const int m = src->r.n_dim, nD = src->r.nD;
+   const double* xs = hx + (size_t)i * nmax;
+   double* x = B.x + src->x_base;
+   double* Dt = B.Dtrial + src->d_base;
+   for (int k = threadIdx.x; k < nD; k += blockDim.x)
+      for (int c = 0; c < 3; c++) {
+         const double v = c < m ? xs[k * m + c] : 0.0;
+         if (c < m) x[k * m + c] = v;
+         Dt[3 * k + c] = v;
+      }
+   // the record word by word (it is a multiple of 8 bytes)
+   const uint64_t* ws = (const uint64_t*)src;
+   uint64_t* wd = (uint64_t*)(B.frames + j);
+   for (int k = threadIdx.x; k < (int)(sizeof(BatchFrame) / 8); k += blockDim.x) wd[k] = ws[k];
+   if (threadIdx.x == 0) {
+      TrState st = {};
+      st.radius = B.prm.radius;
+      st.radius0 = B.prm.radius;
+      st.phase = 0;
+      B.state[j] = st;
+   }
+}
+
+// Copies a serving batch's trust-region states and parameter vectors into pinned host memory,
+// written directly over the host link (no copy engine).
+__global__ void k_publish(BatchDev B, TrState* hstate, double* hx, int64_t total_x)
+{
+   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, T = (int64_t)gridDim.x * blockDim.x;
+   const int64_t ns = (int64_t)B.n_frames * (int64_t)(sizeof(TrState) / 8);
+   const uint64_t* ws = (const uint64_t*)B.state;
+   uint64_t* wd = (uint64_t*)hstate;
+   for (int64_t k = t; k < ns; k += T) wd[k] = ws[k];
+   for (int64_t k = t; k < total_x; k += T) hx[k] = B.x[k];
+   __threadfence_system();
+}
 
 // LDS use of the update (full = 1): the frame's pass partials, H, ichunk, x, p, g, the factor
 // blocks and the solve vector are all staged with one burst of loads, so the update's many
@@ -2239,8 +2289,7 @@ __device__ void fold_schedule(const BatchDev& B)
       const unsigned long long m = __ballot(act);
       if (act) {
          B.active_frames[total + __popcll(m & ((1ull << lane) - 1))] = f;
-         bytes += (unsigned long long)R.n_px *
-                  (unsigned long long)((B.frames[f].f.dtype == FA_U16 ? 2 : 4) + (R.n_dim == 3 ? 20 : 16));
+         bytes += (unsigned long long)R.n_px * (unsigned long long)pass_bytes_per_px(B.frames[f].f.dtype, R.n_dim);
       }
       total += __popcll(m);
    }
@@ -2257,6 +2306,7 @@ __device__ void fold_schedule(const BatchDev& B)
       B.counters->n_active_work2 = R.n_dim == 2 ? w : 0;
       if (total > 0) {
          B.counters->bytes += bytes;
+         B.counters->px += (unsigned long long)total * (unsigned long long)R.n_px;
          B.counters->rounds += 1;
       }
       __hip_atomic_store(&B.counters->upd_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2403,9 +2453,18 @@ hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s)
    return hipGetLastError();
 }
 
-hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, hipStream_t s)
+hipError_t launch_tr_start_slots(const BatchDev& B, const BatchFrame* hbf, const double* hx, const int32_t* hslots,
+                                 int n, int nmax, hipStream_t s)
 {
-   hipLaunchKernelGGL(k_tr_init_slots, dim3(n), dim3(64), 0, s, B, slots);
+   hipLaunchKernelGGL(k_tr_start_slots, dim3(n), dim3(64), 0, s, B, hbf, hx, hslots, nmax);
+   return hipGetLastError();
+}
+
+hipError_t launch_publish(const BatchDev& B, TrState* hstate, double* hx, int64_t total_x, hipStream_t s)
+{
+   const int64_t words = (int64_t)B.n_frames * (int64_t)(sizeof(TrState) / 8) + total_x;
+   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(64, (words + 255) / 256));
+   hipLaunchKernelGGL(k_publish, dim3(grid), dim3(256), 0, s, B, hstate, hx, total_x);
    return hipGetLastError();
 }
 

@@ -21,7 +21,9 @@ hipError_t launch_splat(const RefDev& R, const FrameDev& F, const double* D, flo
                         float offset, int grid, hipStream_t s);
 hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s);
 hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s);
-hipError_t launch_tr_init_slots(const BatchDev& B, const int32_t* slots, int n, hipStream_t s);
+hipError_t launch_tr_start_slots(const BatchDev& B, const BatchFrame* hbf, const double* hx, const int32_t* hslots,
+                                 int n, int nmax, hipStream_t s);
+hipError_t launch_publish(const BatchDev& B, TrState* hstate, double* hx, int64_t total_x, hipStream_t s);
 hipError_t launch_tr_update(const BatchDev& B, int cls, int lds_doubles, int full, int threads, hipStream_t s);
 hipError_t launch_tr_update_all(const BatchDev& B, int lds_doubles, uint32_t wide, uint32_t full, hipStream_t s);
 int tr_update_all_per_cu(int lds_doubles);

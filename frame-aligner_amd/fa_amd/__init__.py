@@ -86,7 +86,7 @@ class ServerResult(C.Structure):
 
 class BatchTiming(C.Structure):
     _fields_ = [("wall_ms", C.c_double), ("pass_ms", C.c_double), ("pass_launches", C.c_int64),
-                ("pass_bytes", C.c_double), ("tr_rounds", C.c_int64)]
+                ("pass_bytes", C.c_double), ("tr_rounds", C.c_int64), ("pass_px", C.c_double)]
 
 
 EXPORTS = [
@@ -602,8 +602,9 @@ class Server:
         return u.value, t.value
 
     def step(self, rounds=1):
-        """Collects what the previous step's rounds finished, admits queued frames and launches
-        rounds trust-region rounds without waiting for them: returns (in_flight, queued,
+        """Collects the results of the lanes whose rounds finished, admits queued frames and
+        launches `rounds` trust-region rounds on every lane with none in flight (waiting up to
+        0.5 ms for a lane to finish when none had): returns (in_flight, queued,
         [(tag, x, stats), ...])."""
         a, q = C.c_int32(), C.c_int32()
         _check(self.L.fa_server_step(self.h, int(rounds), C.byref(a), C.byref(q)))

@@ -101,8 +101,10 @@ typedef struct fa_batch_timing {
    double wall_ms;       /* host wall time of the run */
    double pass_ms;       /* sum of fused-pass kernel durations (HIP events on the batch stream) */
    int64_t pass_launches;
-   double pass_bytes;    /* algorithmic bytes of those launches: active pixels x bytes/pixel */
+   double pass_bytes;    /* algorithmic bytes of those launches: active pixels x (s_f + 4 + 4 n_dim),
+                            s_f = 2 (uint16 frames) or 4 (float): frame, reference, gradient */
    int64_t tr_rounds;    /* launch rounds (pass + update) issued */
+   double pass_px;       /* active pixels of those launches */
 } fa_batch_timing;
 
 /* Launch shape of the fused pass, chosen per batch by the autotuner (fa_amd.tune):
@@ -230,22 +232,30 @@ int fa_batch_eval(fa_batch* b, const double* D, double* err, double* jac);
 int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
                    double* x_inout, fa_align_stats* stats);
 
-/* Serving (continuous batching): a fixed set of slots, each sized for the largest of the
-   references the server serves, and a device frame arena of arena_bytes (0: slots times the
-   largest float frame). Submitted frames wait in a queue, in arrival order. Every
-   fa_server_step first waits for the rounds the previous step launched and harvests the
-   slots they finished: x_inout (caller memory, valid until the result is returned) receives
-   the result and fa_server_results lists the finished tags. It then takes queued frames
-   while a slot is free and (for host frames) the arena has room for the frame (admission by
-   free memory), uploads host frames on a copy stream that overlaps the running rounds,
-   starts the frames whose data is on the device, and launches `rounds` trust-region rounds
-   over every running slot (frames admitted at different times share the launches) without
-   waiting for them, so the device computes while the caller submits and reads results; a
-   frame's result is returned by the step after the one whose rounds finished it.
-   Each result is the same as fa_align's for that frame. Host frames should be pinned
-   (fa_host_alloc) for the upload to be asynchronous, and stay untouched until their result is
-   returned; registered frames stay registered until then. Use one thread per server.
-   There is no counterpart in the reference beyond its thread pool of addFrame tasks
+/* Serving (continuous batching). The references are grouped into lanes by kind (2-D, 3-D)
+   and size (a lane's largest reference has at most 16 times the pixels of its smallest; at
+   most 3 lanes). Each lane runs its own slot batch on its own stream (a server with one lane
+   and 16 or more slots runs it as two that interleave), with `slots` slots each sized for the
+   lane's largest reference; at most `slots` frames are admitted at once over all lanes. Host
+   frames go into a device frame arena of arena_bytes (0: slots times the largest float frame,
+   within the device's free memory less 32 MiB; FA_E_OOM "Not enough video memory to use GPU"
+   when even one frame does not fit, as GpuFrameWarper::setupReferenceInformation,
+   GpuFrameWarper.cpp:113-130). Submitted frames wait in arrival order.
+   fa_server_step: (1) takes the results of every lane whose launched rounds have finished:
+   x_inout (caller memory, valid until the result is returned) receives the result and
+   fa_server_results lists the finished tags; (2) admits queued frames in arrival order while
+   fewer than `slots` are admitted, the frame's lane has a free slot and (host frames) the
+   arena has room for it (admission by free memory: the oldest frame that does not fit waits,
+   and those behind it with it), uploading host frames on a copy stream that overlaps the
+   running rounds; (3) for every lane with no rounds in flight, starts its frames whose data is
+   on the device and launches `rounds` trust-region rounds over its running slots (frames
+   admitted at different times share the launches) without waiting for them. When no lane had
+   finished at (1), the step waits up to 0.5 ms for one to finish and repeats (1)-(3), so a
+   lane of small frames is harvested and relaunched on its own cadence, not on that of a lane
+   of large frames. Each result is the same as fa_align's for that frame. Host frames should
+   be pinned (fa_host_alloc) for the upload to be asynchronous, and stay untouched until their
+   result is returned; registered frames stay registered until then. Use one thread per
+   server. There is no counterpart in the reference beyond its thread pool of addFrame tasks
    (FrameAlignerMex/FrameAlignerMex.cpp:110-130). */
 typedef struct fa_server_result {
    int64_t tag;

@@ -70,8 +70,8 @@ def test_server_update_widths_in_one_launch(ctx):
 
 @pytest.mark.parametrize("slots", [4, 16])
 def test_server_same_alone_and_in_mix(ctx, slots):
-    """4 slots: fewer than the frames, some wait in the queue; 16 slots: two slot batches on
-    their own streams, both holding frames."""
+    """4 slots: fewer than the frames, some wait in the queue; 16 slots: every frame admitted
+    at once, the 2-D and 3-D frames in their own lanes (slot batches on their own streams)."""
     stacks, refs, frames = _setup(ctx)
     alone = [f.align(**KW) for f in frames]
     srv = F.Server(refs, slots=slots, **KW)
@@ -111,6 +111,40 @@ def test_server_same_alone_and_in_mix(ctx, slots):
             _, _, res = srv.step(rounds=2)
         assert np.array_equal(res[0][1], alone[i][0])
     srv.close()
+
+
+def test_server_lanes_same_alone(ctx):
+    """Three lanes: 2-D frames, small 3-D volumes, and 3-D volumes with more than 16 times their
+    voxels, each lane on its own stream and cadence. Frames arrive while the others are in
+    flight; every result is the frame's alone."""
+    a = small_stack((1, 64, 72), n_frames=4, nD=13, seed=61, dtype="uint16")
+    c = small_stack((6, 40, 36), n_frames=3, nD=19, seed=62, n_blobs=120, sigma=(1.5, 4.0))
+    d = small_stack((8, 128, 160), n_frames=2, nD=25, seed=63, n_blobs=600, sigma=(1.5, 4.0))
+    refs = [F.Reference(ctx, a["ref_w"], 13), F.Reference(ctx, c["ref_w"], 19), F.Reference(ctx, d["ref_w"], 25)]
+    frames = ([refs[0].register(f) for f in a["frames"]] + [refs[1].register(f) for f in c["frames_w"]] +
+              [refs[2].register(f) for f in d["frames_w"]])
+    alone = [f.align(**KW) for f in frames]
+    srv = F.Server(refs, slots=8, **KW)
+    order = [7, 0, 4, 8, 1, 5, 2, 6, 3]
+    got = {}
+    k = steps = 0
+    while len(got) < len(frames):
+        for _ in range(2):
+            if k < len(order):
+                srv.submit(frames[order[k]], tag=order[k])
+                k += 1
+        _, _, res = srv.step(rounds=2)
+        steps += 1
+        for tag, x, st in res:
+            got[tag] = (x, st)
+        assert steps < 20000
+    for i, (x1, s1) in enumerate(alone):
+        x2, s2 = got[i]
+        assert np.array_equal(x1, x2), i
+        assert (s1["evaluations"], s1["status"], s1["f_final"]) == (s2["evaluations"], s2["status"], s2["f_final"]), i
+    srv.close()
+    for f in frames:
+        f.release()
 
 
 def test_server_errors(ctx):
