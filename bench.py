@@ -69,6 +69,12 @@ def traffic_of(name, achieved_gbs):
     return t, src
 
 
+def parse_tuning(text):
+    """`key=value,...` over the launch-shape fields (lds_floats, blocks_per_cu, use_lds,
+    check_every, use_graph, tr_lds)."""
+    return {k: int(v) for k, v in (kv.split("=") for kv in text.split(",") if kv)}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -422,7 +428,9 @@ def uneven_batch(F, args, ctx):
     log(f"uneven_batch: {len(stacks)} stacks, {n} frames, {vox / 1e9:.2f} Gvoxel, generated in {time.time() - t0:.1f}s")
     b = F.Batch(frames, **tr)
     best = None
-    if not args.no_tune:
+    if args.set_tuning:
+        b.set_tuning(**parse_tuning(args.set_tuning))
+    elif not args.no_tune:
         best, _, trials = tune.tune_batch(b, reps=2, sweeps=1)
         log(f"uneven_batch: autotuner {best} ({len(trials)} shapes)")
     b.run()
@@ -489,7 +497,9 @@ def headline(F, args, ctx):
     ref = F.Reference(ctx, ref_w, nD, chunk_px=chunk)
     frames = [ref.register(f) for f in st["frames"]]  # uint16 counts, +1 on the device
     batch = F.Batch(frames, **tr)
-    if not args.no_tune:
+    if args.set_tuning:
+        batch.set_tuning(**parse_tuning(args.set_tuning))
+    elif not args.no_tune:
         best, best_t, trials = tune.tune_batch(batch, reps=5, sweeps=1)
         log(f"autotuner: {best} ({best_t * 1e3:.3f} ms per run, {len(trials)} shapes tried)")
     tuning = batch.tuning
@@ -557,7 +567,7 @@ def headline(F, args, ctx):
         "config": {"workload": "512x512 uint16, 200 frames, 30 resampling points (nD=31), "
                                "TR to 1e-4 rel or 50 iterations, batched",
                    "frames": n, "nD": nD, "shape": list(shape), "chunk_px": chunk,
-                   "tuning": tuning, "tuned": not args.no_tune},
+                   "tuning": tuning, "tuned": not args.no_tune and not args.set_tuning},
         "timed_ms_per_step": round(1e3 * timed_wall / args.steps, 3),
         "timed_same_result": same_timed,
         "mpx_per_s": round(fps * 512 * 512 / 1e6, 1),
@@ -675,6 +685,9 @@ def main():
     ap.add_argument("--serve-lower", default="1500,1000",
                     help="lower offered rates (frames/s) the serving section also runs")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape, no autotuning")
+    ap.add_argument("--set-tuning", default="",
+                    help="launch shape key=value,... applied instead of autotuning (e.g. the shape a tuned "
+                         "run reported), for profiling the same launches")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated one-pass timing")
     ap.add_argument("--sections", default="batch,serve,precision,uneven",
                     help="comma list of: batch (the headline 512^2 workload), serve, precision (3-D), "
