@@ -236,8 +236,11 @@ def serving_512(F, args, ctx, plain_fps):
            "plain_frames_per_s": round(plain_fps, 1), "runs": []}
     w, wa = serve.arrivals(1, 2000.0, 0.2, seed=args.seed + 6)
     loop.run(w, wa)  # warm-up
-    for rate in (3.0 * plain_fps, args.serve512_high):
-        t, a = serve.arrivals(1, rate, args.serve_seconds, seed=args.seed + 7)
+    for rate, dur in ((3.0 * plain_fps, args.serve_seconds), (args.serve512_high, args.serve_seconds),
+                      (args.serve512_sat, min(args.serve_seconds, 0.5))):
+        if not rate:
+            continue
+        t, a = serve.arrivals(1, rate, dur, seed=args.seed + 7)
         keep = set(range(0, len(t), max(1, len(t) // 16)))
         r = loop.run(t, a, keep=keep)
         s = serve.summarize(r, t, a, [st["frames"]])
@@ -247,9 +250,11 @@ def serving_512(F, args, ctx, plain_fps):
             x, _ = fr.align(**tr)
             same = same and bool(np.array_equal(x, r["kept"][i][0]))
             fr.release()
+        # every frame is uploaded from pinned host memory when admitted
         out["runs"].append({"offered_frames_per_s": s["offered_frames_per_s"], "frames_per_s": s["frames_per_s"],
                             "vs_plain": round(s["frames_per_s"] / plain_fps, 2), "latency_ms": s["latency_ms"],
                             "max_in_flight": s["max_in_flight"], "max_queued": s["max_queued"],
+                            "upload_gb_per_s": round(s["frames_per_s"] * st["frames"][0].nbytes / 1e9, 2),
                             "same_as_alone": same})
         log(f"serving 512^2 at {rate:.0f}/s: {s['frames_per_s']} frames/s, p50 {s['latency_ms']['p50']} ms, "
             f"p99 {s['latency_ms']['p99']} ms")
@@ -449,6 +454,30 @@ def uneven_batch(F, args, ctx):
     return out
 
 
+def end_to_end(F, args, ctx, st, tr):
+    """The headline workload through the aligner front end (FrameWarpAligner::addFrame,
+    FrameWarpAligner.cpp:117-223) for all frames at once: raw uint16 frames uploaded from host
+    memory, +1 (no blur), the three starts (zero twice and the rigid phase-correlation shift) with
+    the best one taken, the trust region, and the realigned uint16 frames copied back. frames/s
+    over whole calls, and the host-device bytes they move."""
+    al = F.Aligner(ctx, st["reference"], type="warp", n_resampling_points=30)
+    frames = list(st["frames"])
+    al.add_frames(frames[:8], outputs=("realigned",), **tr)  # plans and buffers
+    reps, t = max(1, args.steps), time.perf_counter()
+    for _ in range(reps):
+        D, res, imgs = al.add_frames(frames, outputs=("realigned",), **tr)
+    wall = (time.perf_counter() - t) / reps
+    al.close()
+    n = len(frames)
+    moved = n * frames[0].nbytes + imgs["realigned"].nbytes
+    starts = [r["start"] for r in res]
+    return {"workload": f"512x512 uint16, {n} raw frames per call, 30 resampling points (nD=31), three starts "
+                        f"with the rigid FFT start, TR to 1e-4 rel or 50 iterations, realigned frames returned",
+            "frames_per_s": round(n / wall, 1), "ms_per_call": round(1e3 * wall, 2), "calls": reps,
+            "host_device_bytes_per_call": int(moved), "host_device_gb_per_s": round(moved / wall / 1e9, 2),
+            "start_chosen": {str(k): starts.count(k) for k in sorted(set(starts))}}
+
+
 def large_working_set(F, args, ref, raw, tuning, tr, xs_ref):
     """The headline workload with args.large_frames frames resident at once (the 200 distinct
     frames each registered several times, so every copy is its own device buffer): frames,
@@ -590,6 +619,8 @@ def headline(F, args, ctx):
     batch.close()
     for f in frames:
         f.release()
+    if args.end_to_end:
+        out["end_to_end"] = end_to_end(F, args, ctx, st, tr)
     if args.large_frames:
         out["large_working_set"] = large_working_set(F, args, ref, st["frames"], tuning, tr, xs)
 
@@ -665,6 +696,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=200)
     ap.add_argument("--plain-frames", type=int, default=40)
+    ap.add_argument("--no-end-to-end", dest="end_to_end", action="store_false",
+                    help="skip the aligner front-end run of the headline workload")
     ap.add_argument("--large-frames", type=int, default=1000,
                     help="frames of the headline workload's run beyond the last-level cache (0: skip)")
     ap.add_argument("--cpu-frames", type=int, default=200)
@@ -680,6 +713,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve512-high", type=float, default=20000.0,
                     help="second offered rate (frames/s) of the 512^2 serving run")
+    ap.add_argument("--serve512-sat", type=float, default=100000.0,
+                    help="third offered rate (frames/s) of the 512^2 serving run, above what it sustains (0: skip)")
     ap.add_argument("--plain-fps", type=float, default=1400.0,
                     help="plain-path frames/s used for the 512^2 serving rate when the batch section is skipped")
     ap.add_argument("--serve-lower", default="1500,1000",

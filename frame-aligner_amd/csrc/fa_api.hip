@@ -108,6 +108,7 @@ struct fa_batch {
    fa_context* ctx = nullptr;
    std::mutex mu;                  // serialises runs, evaluations and tuning of this batch
    hipStream_t stream = nullptr;   // the batch's own stream (default priority)
+   bool own_stream = true;         // false: a stream of the context (serving lanes)
    // A batch of many frames runs as parts on their own streams, whose trust-region rounds
    // interleave so that one part's update overlaps another part's pass. Leaf batches have no
    // parts. part_first[i] is the caller's index of part i's first frame.
@@ -250,7 +251,7 @@ struct fa_batch {
       dfree(d_cov);
       dfree(d_trace);
       if (h_counters) (void)hipHostFree(h_counters);
-      if (stream) (void)hipStreamDestroy(stream);
+      if (stream && own_stream) (void)hipStreamDestroy(stream);
    }
 };
 
@@ -606,7 +607,7 @@ int event_pair(fa_batch* b, size_t i, hipEvent_t* s, hipEvent_t* e)
 constexpr size_t kMaxTimedRounds = 4096;
 constexpr int kServerSplitSlots = 16;  // a one-lane server with at least this many slots runs kServerParts
 constexpr int kServerParts = 2;        // slot batches (at least 8 slots each)
-constexpr int kMaxLanes = 3;           // server lanes (each on its own stream, beside the copy stream)
+constexpr int kMaxLanes = fa_context::kLaneStreams;  // server lanes (each on a lane stream of the context)
 constexpr int64_t kLaneSpan = 16;      // a lane's references span at most this ratio of pixels
 constexpr size_t kArenaMargin = (size_t)32 << 20;  // device memory a default arena leaves free
 constexpr int kSplitFrames = 64;  // batches of at least this many frames run as two parts
@@ -1062,7 +1063,6 @@ struct fa_server {
       }
       for (auto e : up_ev) (void)hipEventDestroy(e);
       for (auto e : done_ev) (void)hipEventDestroy(e);
-      if (cstream) (void)hipStreamDestroy(cstream);
       dfree(d_arena);
       if (h_bf) (void)hipHostFree(h_bf);
       if (h_x) (void)hipHostFree(h_x);
@@ -1090,7 +1090,7 @@ size_t frame_bytes(const fa_reference* r, int dtype) { return (size_t)r->n_px * 
 // A batch whose frames are slots: every slot is sized for the largest of the references and
 // starts finished (out of the schedule) until a request is admitted into it.
 int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, int S,
-                     const fa_tr_params* prm, fa_batch** out)
+                     const fa_tr_params* prm, hipStream_t stream, fa_batch** out)
 {
    auto* b = new fa_batch();
    b->ctx = ctx;
@@ -1160,11 +1160,8 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
        (rc = dalloc(&b->d_run_count, rdev.size())) || (rc = dalloc(&b->d_run_base, rdev.size() + 1)) ||
        (rc = dalloc(&b->d_counters, 1)) || (rc = halloc(&b->h_counters, 1)))
       return fail(rc);
-   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-      (void)hipGetLastError();
-      b->stream = nullptr;
-      return fail(set_err(FA_E_DEVICE, "stream creation failed"));
-   }
+   b->stream = stream;  // one of the context's lane streams
+   b->own_stream = false;
    std::vector<TrState> st(S);
    for (auto& t : st) {
       t = TrState{};
@@ -1611,7 +1608,14 @@ int fa_create(int device, fa_context** out)
    // keeps one; launches report their own errors
    c->lds_config = (int)configure_kernels(c->lds_per_block);
    (void)hipGetLastError();
-   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+   for (int l = 0; ok && l < fa_context::kLaneStreams; l++)
+      ok = hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking) == hipSuccess;
+   if (!ok) {
+      (void)hipGetLastError();
+      for (hipStream_t s : c->lane_stream)
+         if (s) (void)hipStreamDestroy(s);
+      if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
       return set_err(FA_E_DEVICE, "stream creation failed");
    }
@@ -1627,6 +1631,10 @@ int fa_destroy(fa_context* ctx)
    DeviceGuard g(ctx->device);
    (void)hipStreamSynchronize(ctx->stream);
    (void)hipStreamDestroy(ctx->stream);
+   for (hipStream_t s : ctx->lane_stream) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+   }
    delete ctx;
    return FA_OK;
 }
@@ -2308,7 +2316,8 @@ int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs,
       for (int pi = 0; pi < P1 && rc == FA_OK; pi++) {
          const int sp = slots / P1 + (pi < slots % P1 ? 1 : 0);
          fa_batch* q = nullptr;
-         rc = slot_batch_build(ctx, L.refs, sp, prm, &q);
+         // part k of the server runs on the context's lane stream k
+         rc = slot_batch_build(ctx, L.refs, sp, prm, ctx->lane_stream[sv->parts.size() % fa_context::kLaneStreams], &q);
          if (rc != FA_OK) break;
          sv->parts.push_back(q);
          sv->first.push_back(j);
@@ -2350,11 +2359,7 @@ int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs,
       (void)hipGetLastError();
       rc = set_err(FA_E_DEVICE, "pinned staging is not mapped for the device");
    }
-   if (rc == FA_OK && hipStreamCreateWithFlags(&sv->cstream, hipStreamNonBlocking) != hipSuccess) {
-      (void)hipGetLastError();
-      sv->cstream = nullptr;
-      rc = set_err(FA_E_DEVICE, "stream creation failed");
-   }
+   sv->cstream = ctx->stream;  // uploads: the context's stream, whose queue no lane shares
    for (int j = 0; rc == FA_OK && j < S + (int)sv->parts.size(); j++) {
       hipEvent_t e;
       if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {

@@ -185,8 +185,15 @@ struct fa_context {
    int lds_per_block = 65536;  // bytes of LDS one workgroup may allocate
    int lds_config = 0;         // hipError_t of raising the kernels' dynamic-LDS limit
    hipStream_t stream = nullptr;
+   // Streams of the serving lanes, created with the context's own stream, before any other
+   // stream of the context: the runtime maps streams onto a few hardware queues (4 by default)
+   // in creation order, so these get queues of their own, which streams created and destroyed
+   // later (batches, aligners) could otherwise share with them.
+   static constexpr int kLaneStreams = 3;
+   hipStream_t lane_stream[kLaneStreams] = {nullptr, nullptr, nullptr};
    std::mutex mu;
    int live_refs = 0;  // references not yet released; fa_destroy refuses while > 0
+   int live_servers = 0;  // servers using the lane streams
 };
 
 struct fa_reference {
