@@ -1368,9 +1368,16 @@ namespace fa {
 
 int set_error(int code, const std::string& msg) { return set_err(code, msg); }
 
+void batch_set_params(fa_batch* b, const fa_tr_params& p)
+{
+   std::lock_guard<std::mutex> lb(b->mu);
+   b->prm = p;
+   for (fa_batch* q : b->parts) q->prm = p;  // a changed set re-captures the round graphs
+}
+
 // A frame whose samples are already in device memory (d_data, nz*ny*nx values of dtype), owned
 // by the frame from now on (freed on deregistration).
-int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offset, fa_frame** out)
+int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offset, fa_frame** out, bool own)
 {
    std::lock_guard<std::mutex> lk(r->ctx->mu);
    auto* f = new fa_frame();
@@ -1378,6 +1385,7 @@ int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offse
    f->dtype = dtype;
    f->offset = offset;
    f->d_data = d_data;
+   f->own_data = own;
    r->live_frames++;
    r->frames.push_back(f);
    *out = f;
@@ -1996,7 +2004,7 @@ int fa_deregister_frame(fa_frame* f)
    delete f->single;
    f->single = nullptr;
    void* d = f->d_data;
-   dfree(d);
+   if (f->own_data) dfree(d);
    fa_reference* r = f->ref;
    r->live_frames--;
    r->frames.erase(std::remove(r->frames.begin(), r->frames.end(), f), r->frames.end());

@@ -18,6 +18,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -118,20 +119,23 @@ __global__ void k_add(float* v, int64_t n, float add)
    if (i < n) v[i] = v[i] + add;
 }
 
-// Binned (x, y summed in bins of f, y outer and x inner, in float) volume times the window.
+// Binned (x, y summed in bins of f, y outer and x inner, in float) volumes times the window,
+// for `count` frames (inputs nz*ny*nx floats apart, outputs nz*ph*pw doubles apart).
 __global__ void k_bin_window(const float* v, int nz, int ny, int nx, int f, int ph, int pw, const double* win,
-                             double* out)
+                             double* out, int count)
 {
-   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    const int64_t n = (int64_t)nz * ph * pw;
-   if (i >= n) return;
+   if (t >= n * count) return;
+   const int64_t fr = t / n, i = t - fr * n;
+   const float* vf = v + fr * ((int64_t)nz * ny * nx);
    const int X = (int)(i % pw);
    const int Y = (int)((i / pw) % ph);
    const int z = (int)(i / ((int64_t)pw * ph));
    float s = 0.f;
    for (int y = Y * f; y < min(ny, Y * f + f); y++)
-      for (int x = X * f; x < min(nx, X * f + f); x++) s += v[((int64_t)z * ny + y) * nx + x];
-   out[i] = (double)s * win[i];
+      for (int x = X * f; x < min(nx, X * f + f); x++) s += vf[((int64_t)z * ny + y) * nx + x];
+   out[t] = (double)s * win[i];
 }
 
 __global__ void k_conj(const double2* in, double2* out, int64_t n)
@@ -140,18 +144,20 @@ __global__ void k_conj(const double2* in, double2* out, int64_t n)
    if (i < n) out[i] = make_double2(in[i].x, -in[i].y);
 }
 
-__global__ void k_cross_power(double2* s, const double2* ref, int64_t n)
+// Cross-power spectra of `count` frames' spectra (n apart) with the reference's.
+__global__ void k_cross_power(double2* s, const double2* ref, int64_t n, int count)
 {
    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (i >= n) return;
-   const double2 a = s[i], b = ref[i];
+   if (i >= n * count) return;
+   const double2 a = s[i], b = ref[i % n];
    const double re = a.x * b.x - a.y * b.y, im = a.x * b.y + a.y * b.x;
    const double nrm = hypot(re, im) + DBL_MIN;
    s[i] = make_double2(re / nrm, im / nrm);
 }
 
 // First maximum: per workgroup (value, index), ties to the lower index; the last stage is one
-// workgroup over the partials.
+// workgroup over the partials. Frame blockIdx.y of the first stage reads v + y n and writes
+// partials y * gridDim.x + blockIdx.x; workgroup b of the last stage reduces frame b's m.
 __device__ __forceinline__ void better(double& v, int64_t& i, double v2, int64_t i2)
 {
    if (v2 > v || (v2 == v && i2 < i)) {
@@ -164,6 +170,9 @@ __global__ __launch_bounds__(kFB) void k_argmax(const double* v, int64_t n, doub
 {
    __shared__ double sv[kFB];
    __shared__ int64_t si[kFB];
+   v += (int64_t)blockIdx.y * n;
+   pv += (int64_t)blockIdx.y * gridDim.x;
+   pi += (int64_t)blockIdx.y * gridDim.x;
    double bv = -DBL_MAX;
    int64_t bi = INT64_MAX;
    for (int64_t i = (int64_t)blockIdx.x * kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB)
@@ -185,6 +194,9 @@ __global__ __launch_bounds__(kFB) void k_argmax_final(const double* pv, const in
 {
    __shared__ double sv[kFB];
    __shared__ int64_t si[kFB];
+   pv += (int64_t)blockIdx.x * m;
+   pi += (int64_t)blockIdx.x * m;
+   out += blockIdx.x;
    double bv = -DBL_MAX;
    int64_t bi = INT64_MAX;
    for (int i = threadIdx.x; i < m; i += kFB) better(bv, bi, pv[i], pi[i]);
@@ -237,14 +249,26 @@ __global__ void k_preserving(const float* wimg, const float* cov, int64_t n, uin
 
 // Masked moments for correlation() (Cv3dUtils.cpp:64-88): over pixels with cov != 0,
 // stage 1: count, sum x, sum x^2, sum y, sum y^2 in double (y = the record's reference value);
-// stage 2 (fm >= 0): sum of float((x - mx) * (y - my)) with mx, my the float means.
-// Per-workgroup partials, summed in a fixed order by k_sum_partials.
+// stage 2: sum of float((x - mx) * (y - my)) with mx, my the float means of stage 1's sums
+// (s1: kMoments per frame; mx = float(sum x / count)). Frame blockIdx.y reads x and cov n
+// apart; per-workgroup partials, summed in a fixed order by k_sum_partials.
 constexpr int kMoments = 5;
 
 __global__ __launch_bounds__(kFB) void k_moments(const float* x, const float4* rec, const float* cov, int64_t n,
-                                                 int stage, float mx, float my, double* part)
+                                                 int stage, const double* s1, double* part)
 {
    __shared__ double sm[kFB][kMoments];
+   const int fr = blockIdx.y;
+   x += (int64_t)fr * n;
+   cov += (int64_t)fr * n;
+   float mx = 0.f, my = 0.f;
+   if (stage == 2) {
+      const double* h = s1 + (int64_t)fr * kMoments;
+      if (h[0] != 0) {
+         mx = (float)(h[1] / h[0]);
+         my = (float)(h[3] / h[0]);
+      }
+   }
    double a[kMoments] = {0, 0, 0, 0, 0};
    for (int64_t i = (int64_t)blockIdx.x * kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB) {
       if (cov[i] == 0.f) continue;
@@ -267,27 +291,37 @@ __global__ __launch_bounds__(kFB) void k_moments(const float* x, const float4* r
          for (int k = 0; k < kMoments; k++) sm[threadIdx.x][k] += sm[threadIdx.x + off][k];
       __syncthreads();
    }
-   if (threadIdx.x < kMoments) part[(int64_t)blockIdx.x * kMoments + threadIdx.x] = sm[0][threadIdx.x];
+   if (threadIdx.x < kMoments)
+      part[((int64_t)fr * gridDim.x + blockIdx.x) * kMoments + threadIdx.x] = sm[0][threadIdx.x];
 }
 
-__global__ void k_sum_partials(const double* part, int m, double* out)
+// Frame blockIdx.x's m partials: lane l sums partials l, l + 64, ..., then a fixed tree over
+// the 64 lanes.
+__global__ __launch_bounds__(64) void k_sum_partials(const double* part, int m, double* out)
 {
-   if (threadIdx.x >= kMoments) return;
-   double s = 0;
-   for (int b = 0; b < m; b++) s += part[(int64_t)b * kMoments + threadIdx.x];
-   out[threadIdx.x] = s;
+   __shared__ double sm[64][kMoments];
+   part += (int64_t)blockIdx.x * m * kMoments;
+   double a[kMoments] = {0, 0, 0, 0, 0};
+   for (int b = threadIdx.x; b < m; b += 64)
+      for (int k = 0; k < kMoments; k++) a[k] += part[(int64_t)b * kMoments + k];
+   for (int k = 0; k < kMoments; k++) sm[threadIdx.x][k] = a[k];
+   __syncthreads();
+   for (int off = 32; off >= 1; off >>= 1) {
+      if (threadIdx.x < off)
+         for (int k = 0; k < kMoments; k++) sm[threadIdx.x][k] += sm[threadIdx.x + off][k];
+      __syncthreads();
+   }
+   if (threadIdx.x < kMoments) out[(int64_t)blockIdx.x * kMoments + threadIdx.x] = sm[0][threadIdx.x];
 }
 
-__global__ void k_count_nonzero(const float* v, int64_t n, unsigned long long* cnt)
+// correlation() from the two stages' sums of one frame (NaN where no pixel is covered).
+double correlation_of(const double* h, const double* h2)
 {
-   __shared__ unsigned long long sc;
-   if (threadIdx.x == 0) sc = 0;
-   __syncthreads();
-   unsigned long long c = 0;
-   for (int64_t i = (int64_t)blockIdx.x * kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB) c += v[i] != 0.f;
-   atomicAdd(&sc, c);
-   __syncthreads();
-   if (threadIdx.x == 0) atomicAdd(cnt, sc);
+   const double cnt = h[0];
+   if (cnt == 0) return NAN;
+   const double m1 = h[1] / cnt, m2 = h[3] / cnt;
+   const double s1 = std::sqrt(std::max(h[2] / cnt - m1 * m1, 0.0)), s2 = std::sqrt(std::max(h[4] / cnt - m2 * m2, 0.0));
+   return (h2[0] / cnt) / (s1 * s2);
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)((n + kFB - 1) / kFB); }
@@ -354,10 +388,61 @@ struct fa_aligner {
    float* d_raw = nullptr;  // raw frame as float
    float* d_tmp = nullptr;
    void* d_up = nullptr;    // raw upload
+   // add_frames state kept between calls: its device arrays (grown when a call needs more) and
+   // the frames and batch of the last call, reused while the frame count stays the same (the
+   // frames are views of the preprocessed-frame array, whose address then stays the same)
+   enum { kUp, kRawF, kPre, kBlurTmp, kReal, kSpec, kPv, kPi, kIdx, kD, kPart, kSums, kW, kW2, kWimg, kCov, kU16, kBufs };
+   struct Buf {
+      void* p = nullptr;
+      size_t bytes = 0;
+   };
+   Buf bufs[kBufs];
+   std::vector<fa_frame*> cfr;
+   fa_batch* cbatch = nullptr;
+
+   void drop_cached()
+   {
+      if (cbatch) fa_batch_destroy(cbatch);
+      cbatch = nullptr;
+      for (fa_frame* f : cfr) fa_deregister_frame(f);
+      cfr.clear();
+   }
+
+   // buffer `slot` with room for count values of T (its contents are not kept when it grows)
+   template <typename T>
+   int grow(int slot, size_t count, T** out)
+   {
+      const size_t need = std::max<size_t>(count, 1) * sizeof(T);
+      Buf& b = bufs[slot];
+      if (b.bytes < need) {
+         FCHK(hipStreamSynchronize(stream));
+         if (slot == kPre) drop_cached();  // the cached frames view the old array
+         if (b.p) (void)hipFree(b.p);
+         b.p = nullptr;
+         b.bytes = 0;
+         if (hipMalloc(&b.p, need) != hipSuccess) {
+            (void)hipGetLastError();
+            b.p = nullptr;
+            return set_error(FA_E_OOM, "Not enough memory to allocate");
+         }
+         b.bytes = need;
+      }
+      *out = (T*)b.p;
+      return FA_OK;
+   }
 
    ~fa_aligner()
    {
       if (stream) (void)hipStreamSynchronize(stream);
+      drop_cached();
+      for (Buf& b : bufs)
+         if (b.p) (void)hipFree(b.p);
+      for (auto& kv : bplans) {
+         if (kv.second.fwd) rocfft_plan_destroy(kv.second.fwd);
+         if (kv.second.inv) rocfft_plan_destroy(kv.second.inv);
+         if (kv.second.info) rocfft_execution_info_destroy(kv.second.info);
+         if (kv.second.work) (void)hipFree(kv.second.work);
+      }
       if (fwd) rocfft_plan_destroy(fwd);
       if (inv) rocfft_plan_destroy(inv);
       if (info) rocfft_execution_info_destroy(info);
@@ -389,28 +474,12 @@ struct fa_aligner {
       return FA_OK;
    }
 
-   // out = smoothStack(d_raw) + 1
-   int smooth_plus_one(float* out)
-   {
-      if (rp.smoothing > 0) {
-         hipLaunchKernelGGL(k_blur, dim3(grid_of(n_px)), dim3(kFB), 0, stream, (const float*)d_raw, d_tmp, nz, ny, nx,
-                            (const float*)d_kx, rx, 0, 0.f);
-         hipLaunchKernelGGL(k_blur, dim3(grid_of(n_px)), dim3(kFB), 0, stream, (const float*)d_tmp, out, nz, ny, nx,
-                            (const float*)d_ky, ry, 1, 1.f);
-      } else {
-         if (out != d_raw) FCHK(hipMemcpyAsync(out, d_raw, n_px * sizeof(float), hipMemcpyDeviceToDevice, stream));
-         hipLaunchKernelGGL(k_add, dim3(grid_of(n_px)), dim3(kFB), 0, stream, out, n_px, 1.f);
-      }
-      FCHK(hipGetLastError());
-      return FA_OK;
-   }
-
    // spectrum of the binned, windowed d_raw into d_spec
    int spectrum()
    {
       const int f = std::max(1, rp.spatial_binning);
       hipLaunchKernelGGL(k_bin_window, dim3(grid_of(n_el)), dim3(kFB), 0, stream, (const float*)d_raw, nz, ny, nx, f,
-                         ph, pw, (const double*)d_win, d_real);
+                         ph, pw, (const double*)d_win, d_real, 1);
       FCHK(hipGetLastError());
       void* in[1] = {d_real};
       void* out[1] = {d_spec};
@@ -418,23 +487,75 @@ struct fa_aligner {
       return FA_OK;
    }
 
-   // rigid shift of d_raw
-   int shift(double* s3)
+   // FP64 real-to-complex and complex-to-real plans for `count` frames at once (cached by count)
+   struct Plans {
+      rocfft_plan fwd = nullptr, inv = nullptr;
+      rocfft_execution_info info = nullptr;
+      void* work = nullptr;
+   };
+   std::map<int, Plans> bplans;
+   int batch_plans(int count, Plans** out)
    {
-      OK(spectrum());
-      hipLaunchKernelGGL(k_cross_power, dim3(grid_of(n_cplx)), dim3(kFB), 0, stream, d_spec,
-                         (const double2*)d_refspec, n_cplx);
+      auto it = bplans.find(count);
+      if (it == bplans.end()) {
+         Plans& P = bplans[count];
+         const size_t dims = pz > 1 ? 3 : 2;
+         const size_t len[3] = {(size_t)pw, (size_t)ph, (size_t)pz};
+         RCHK(rocfft_plan_create(&P.fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                                 rocfft_precision_double, dims, len, (size_t)count, nullptr));
+         RCHK(rocfft_plan_create(&P.inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                                 rocfft_precision_double, dims, len, (size_t)count, nullptr));
+         size_t w1 = 0, w2 = 0;
+         RCHK(rocfft_plan_get_work_buffer_size(P.fwd, &w1));
+         RCHK(rocfft_plan_get_work_buffer_size(P.inv, &w2));
+         RCHK(rocfft_execution_info_create(&P.info));
+         if (std::max(w1, w2) > 0) {
+            if (hipMalloc(&P.work, std::max(w1, w2)) != hipSuccess) {
+               (void)hipGetLastError();
+               P.work = nullptr;
+               return set_error(FA_E_OOM, "Not enough memory to allocate");
+            }
+            RCHK(rocfft_execution_info_set_work_buffer(P.info, P.work, std::max(w1, w2)));
+         }
+         RCHK(rocfft_execution_info_set_stream(P.info, stream));
+         it = bplans.find(count);
+      }
+      *out = &it->second;
+      return FA_OK;
+   }
+
+   // Rigid shifts of `count` frames (raw values as float, n_px apart) as flat indices of the
+   // correlation maximum in idx[0..count) (device), with the work arrays the caller provides
+   // (count times n_el doubles, n_cplx complex, n_part maxima).
+   int shifts(const float* rawf, int count, double* w_real, double2* w_spec, double* w_pv, int64_t* w_pi,
+              int64_t* idx)
+   {
+      Plans* P = nullptr;
+      OK(batch_plans(count, &P));
+      const int f = std::max(1, rp.spatial_binning);
+      hipLaunchKernelGGL(k_bin_window, dim3(grid_of(n_el * count)), dim3(kFB), 0, stream, rawf, nz, ny, nx, f, ph,
+                         pw, (const double*)d_win, w_real, count);
       FCHK(hipGetLastError());
-      void* in[1] = {d_spec};
-      void* out[1] = {d_real};
-      RCHK(rocfft_execute(inv, in, out, info));
-      hipLaunchKernelGGL(k_argmax, dim3(n_part), dim3(kFB), 0, stream, (const double*)d_real, n_el, d_pv, d_pi);
-      hipLaunchKernelGGL(k_argmax_final, dim3(1), dim3(kFB), 0, stream, (const double*)d_pv, (const int64_t*)d_pi,
-                         n_part, d_idx);
+      void* in[1] = {w_real};
+      void* out[1] = {w_spec};
+      RCHK(rocfft_execute(P->fwd, in, out, P->info));
+      hipLaunchKernelGGL(k_cross_power, dim3(grid_of(n_cplx * count)), dim3(kFB), 0, stream, w_spec,
+                         (const double2*)d_refspec, n_cplx, count);
       FCHK(hipGetLastError());
-      int64_t idx = 0;
-      FCHK(hipMemcpyAsync(&idx, d_idx, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-      FCHK(hipStreamSynchronize(stream));
+      void* in2[1] = {w_spec};
+      void* out2[1] = {w_real};
+      RCHK(rocfft_execute(P->inv, in2, out2, P->info));
+      hipLaunchKernelGGL(k_argmax, dim3(n_part, count), dim3(kFB), 0, stream, (const double*)w_real, n_el, w_pv, w_pi);
+      hipLaunchKernelGGL(k_argmax_final, dim3(count), dim3(kFB), 0, stream, (const double*)w_pv,
+                         (const int64_t*)w_pi, n_part, idx);
+      FCHK(hipGetLastError());
+      return FA_OK;
+   }
+
+   // the shift of a flat maximum index: indices above half the size wrapped negative, x and y
+   // scaled by the binning
+   void shift_of(int64_t idx, double* s3) const
+   {
       int x = (int)(idx % pw), y = (int)((idx / pw) % ph), z = (int)(idx / ((int64_t)pw * ph));
       if (x > pw / 2) x -= pw;
       if (y > ph / 2) y -= ph;
@@ -443,118 +564,139 @@ struct fa_aligner {
       s3[0] = (double)x * f;
       s3[1] = (double)y * f;
       s3[2] = (double)z;
-      return FA_OK;
    }
 
-   // correlation() of Cv3dUtils.cpp:64-88 between x (device floats) and the warper's reference
-   // (the record's first channel: blurred reference + 1) over the pixels where cov != 0
-   int correlation(const float* x, const float* cov, double* part, double* sum, int parts, double* out)
+   // rigid shift of d_raw
+   int shift(double* s3)
    {
-      const RefDev& R = ref->dev;
-      double h[kMoments];
-      hipLaunchKernelGGL(k_moments, dim3(parts), dim3(kFB), 0, stream, x, R.rg4, cov, n_px, 1, 0.f, 0.f, part);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, stream, (const double*)part, parts, sum);
-      FCHK(hipGetLastError());
-      FCHK(hipMemcpyAsync(h, sum, sizeof(h), hipMemcpyDeviceToHost, stream));
+      OK(shifts(d_raw, 1, d_real, d_spec, d_pv, d_pi, d_idx));
+      int64_t idx = 0;
+      FCHK(hipMemcpyAsync(&idx, d_idx, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
       FCHK(hipStreamSynchronize(stream));
-      const double cnt = h[0];
-      if (cnt == 0) {
-         *out = NAN;
-         return FA_OK;
-      }
-      const double m1 = h[1] / cnt, m2 = h[3] / cnt;
-      const double s1 = std::sqrt(std::max(h[2] / cnt - m1 * m1, 0.0)), s2 = std::sqrt(std::max(h[4] / cnt - m2 * m2, 0.0));
-      hipLaunchKernelGGL(k_moments, dim3(parts), dim3(kFB), 0, stream, x, R.rg4, cov, n_px, 2, (float)m1, (float)m2,
-                         part);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, stream, (const double*)part, parts, sum);
-      FCHK(hipGetLastError());
-      FCHK(hipMemcpyAsync(h, sum, sizeof(h), hipMemcpyDeviceToHost, stream));
-      FCHK(hipStreamSynchronize(stream));
-      *out = (h[0] / cnt) / (s1 * s2);
+      shift_of(idx, s3);
       return FA_OK;
    }
 
-   // Output images and metrics of n frames at D (n x nD x 3): the raw frame's splat (mask,
-   // preserving), its interpolated warp (realigned), the warp of the blurred frame + 1 (warped)
-   // and the correlations and coverage (FrameWarpAligner.cpp:218-240,317-346).
-   int products(const void* const* frames, int n, int dtype, const double* D_all, const std::vector<fa_frame*>& fr,
-                const fa_add_outputs* o, fa_add_result* res)
+   // out = smoothStack(in) + 1 for `count` frames (n_px apart); tmp: count frames (blur only)
+   int smooth_plus_one(const float* in, float* out, int count, float* tmp)
+   {
+      const int64_t n = n_px * count;
+      if (rp.smoothing > 0) {
+         // the blur runs slice by slice: the frames are count * nz slices
+         hipLaunchKernelGGL(k_blur, dim3(grid_of(n)), dim3(kFB), 0, stream, in, tmp, nz * count, ny, nx,
+                            (const float*)d_kx, rx, 0, 0.f);
+         hipLaunchKernelGGL(k_blur, dim3(grid_of(n)), dim3(kFB), 0, stream, (const float*)tmp, out, nz * count, ny, nx,
+                            (const float*)d_ky, ry, 1, 1.f);
+      } else {
+         if (out != in) FCHK(hipMemcpyAsync(out, in, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+         hipLaunchKernelGGL(k_add, dim3(grid_of(n)), dim3(kFB), 0, stream, out, n, 1.f);
+      }
+      FCHK(hipGetLastError());
+      return FA_OK;
+   }
+   int smooth_plus_one(float* out) { return smooth_plus_one(d_raw, out, 1, d_tmp); }
+
+   // Frames handled together by the batched phases of add_frames: at most 32, and at most
+   // about 1 GB of work arrays.
+   int chunk_frames(int n) const
+   {
+      const int64_t per = n_px * 4 * 6 + n_el * 8 + n_cplx * 16 + n_part * 16;
+      return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)n, 32, ((int64_t)1 << 30) / per}));
+   }
+
+   // Output images and metrics of n frames at D_all (host, n x nD x 3): the raw frame's splat
+   // (mask, preserving), its interpolated warp (realigned), the warp of the blurred frame + 1
+   // (warped) and the correlations and coverage (FrameWarpAligner.cpp:218-240,317-346). rawf:
+   // the raw frames as float, pre: the warper's frames (device, n_px apart). Frames run C at a
+   // time; the metrics' sums come back in one copy at the end.
+   int products(int n, const double* D_all, const float* rawf, const float* pre, const fa_add_outputs* o,
+                fa_add_result* res)
    {
       const RefDev& R = ref->dev;
       const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(R.n_chunks, (int64_t)ctx->n_cu * 8));
       const int parts = (int)std::min<int64_t>(1024, (n_px + kFB - 1) / kFB);
-      double *d_D = nullptr, *d_part = nullptr, *d_sum = nullptr;
-      float *d_w = nullptr, *d_wimg = nullptr, *d_cov = nullptr;
+      const int C = chunk_frames(n);
+      const int64_t cpx = (int64_t)C * n_px;
+      double *d_D = nullptr, *d_part = nullptr, *d_sums = nullptr;
+      float *d_w = nullptr, *d_w2 = nullptr, *d_wimg = nullptr, *d_cov = nullptr;
       uint16_t* d_u16 = nullptr;
-      unsigned long long* d_cnt = nullptr;
       auto done = [&](int rc) {
-         ffree(d_D);
-         ffree(d_part);
-         ffree(d_sum);
-         ffree(d_w);
-         ffree(d_wimg);
-         ffree(d_cov);
-         ffree(d_u16);
-         ffree(d_cnt);
+         (void)hipStreamSynchronize(stream);
          return rc;
       };
       int rc;
-      if ((rc = falloc(&d_D, 3 * (size_t)nD)) || (rc = falloc(&d_part, (size_t)parts * kMoments)) ||
-          (rc = falloc(&d_sum, kMoments)) || (rc = falloc(&d_w, n_px)) || (rc = falloc(&d_wimg, n_px)) ||
-          (rc = falloc(&d_cov, n_px)) || (rc = falloc(&d_u16, n_px)) || (rc = falloc(&d_cnt, 1)))
+      if ((rc = grow(kD, 3 * (size_t)nD * n, &d_D)) || (rc = grow(kPart, (size_t)C * parts * kMoments, &d_part)) ||
+          (rc = grow(kSums, (size_t)4 * n * kMoments, &d_sums)) || (rc = grow(kW, cpx, &d_w)) ||
+          (rc = grow(kW2, cpx, &d_w2)) || (rc = grow(kWimg, cpx, &d_wimg)) || (rc = grow(kCov, cpx, &d_cov)) ||
+          (rc = grow(kU16, cpx, &d_u16)))
          return done(rc);
-      for (int i = 0; i < n; i++) {
-         const double* D = D_all + (size_t)i * nD * 3;
-         const size_t off = (size_t)i * n_px;
-         if ((rc = upload(frames[i], dtype)) != FA_OK) return done(rc);
-         const FrameDev Fraw{d_raw, FA_F32, 0.f};
-         const FrameDev Fsm{fr[i]->d_data, FA_F32, 0.f};
-         if (hipMemcpyAsync(d_D, D, 3 * (size_t)nD * sizeof(double), hipMemcpyHostToDevice, stream) != hipSuccess ||
-             hipMemsetAsync(d_wimg, 0, n_px * sizeof(float), stream) != hipSuccess ||
-             hipMemsetAsync(d_cov, 0, n_px * sizeof(float), stream) != hipSuccess ||
-             launch_splat(R, Fraw, d_D, d_wimg, d_cov, 0.f, grid, stream) != hipSuccess)
-            return done(set_error(FA_E_DEVICE, "output warp failed"));
+      if (hipMemcpyAsync(d_D, D_all, 3 * (size_t)nD * n * sizeof(double), hipMemcpyHostToDevice, stream) != hipSuccess)
+         return done(set_error(FA_E_DEVICE, "output warp failed"));
+      auto fail = [&](const char* what) { (void)hipGetLastError(); return done(set_error(FA_E_DEVICE, what)); };
+      for (int j0 = 0; j0 < n; j0 += C) {
+         const int c = std::min(C, n - j0);
+         const int64_t off = (int64_t)j0 * n_px, m = (int64_t)c * n_px;
+         if (hipMemsetAsync(d_wimg, 0, m * sizeof(float), stream) != hipSuccess ||
+             hipMemsetAsync(d_cov, 0, m * sizeof(float), stream) != hipSuccess)
+            return fail("output warp failed");
+         const FrameDev Fraw{rawf + off, FA_F32, 0.f}, Fsm{pre + off, FA_F32, 0.f};
+         const double* Dc = d_D + 3 * (size_t)nD * j0;
+         if (launch_splat(R, Fraw, Dc, d_wimg, d_cov, 0.f, grid, stream, c) != hipSuccess)
+            return fail("output warp failed");
          if (o && o->mask &&
-             hipMemcpyAsync(o->mask + off, d_cov, n_px * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
-            return done(set_error(FA_E_DEVICE, "output copy failed"));
+             hipMemcpyAsync(o->mask + off, d_cov, m * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
+            return fail("output copy failed");
          if (o && o->preserving) {
-            hipLaunchKernelGGL(k_preserving, dim3(grid_of(n_px)), dim3(kFB), 0, stream, (const float*)d_wimg,
-                               (const float*)d_cov, n_px, d_u16);
-            if (hipMemcpyAsync(o->preserving + off, d_u16, n_px * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
-                    hipSuccess ||
-                hipStreamSynchronize(stream) != hipSuccess)
-               return done(set_error(FA_E_DEVICE, "output copy failed"));
+            hipLaunchKernelGGL(k_preserving, dim3(grid_of(m)), dim3(kFB), 0, stream, (const float*)d_wimg,
+                               (const float*)d_cov, m, d_u16);
+            if (hipMemcpyAsync(o->preserving + off, d_u16, m * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
+                hipSuccess)
+               return fail("output copy failed");
          }
          if (o && o->realigned) {
-            if (launch_warp(R, Fraw, d_D, d_w, 0.f, 0, 0, grid, stream) != hipSuccess)
-               return done(set_error(FA_E_DEVICE, "output warp failed"));
-            hipLaunchKernelGGL(k_to_u16, dim3(grid_of(n_px)), dim3(kFB), 0, stream, (const float*)d_w, n_px, d_u16);
-            if (hipMemcpyAsync(o->realigned + off, d_u16, n_px * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
-                    hipSuccess ||
-                hipStreamSynchronize(stream) != hipSuccess)
-               return done(set_error(FA_E_DEVICE, "output copy failed"));
+            if (launch_warp(R, Fraw, Dc, d_w, 0.f, 0, 0, grid, stream, c) != hipSuccess)
+               return fail("output warp failed");
+            hipLaunchKernelGGL(k_to_u16, dim3(grid_of(m)), dim3(kFB), 0, stream, (const float*)d_w, m, d_u16);
+            if (hipMemcpyAsync(o->realigned + off, d_u16, m * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
+                hipSuccess)
+               return fail("output copy failed");
          }
-         if (launch_warp(R, Fsm, d_D, d_w, 0.f, 0, 0, grid, stream) != hipSuccess)
-            return done(set_error(FA_E_DEVICE, "output warp failed"));
+         if (launch_warp(R, Fsm, Dc, d_w2, 0.f, 0, 0, grid, stream, c) != hipSuccess)
+            return fail("output warp failed");
          if (o && o->warped &&
-             hipMemcpyAsync(o->warped + off, d_w, n_px * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
-            return done(set_error(FA_E_DEVICE, "output copy failed"));
+             hipMemcpyAsync(o->warped + off, d_w2, m * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
+            return fail("output copy failed");
          if (res) {
-            unsigned long long cnt = 0;
-            if (hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), stream) != hipSuccess)
-               return done(set_error(FA_E_DEVICE, "metrics failed"));
-            hipLaunchKernelGGL(k_count_nonzero, dim3(parts), dim3(kFB), 0, stream, (const float*)d_cov, n_px, d_cnt);
-            if (hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, stream) != hipSuccess ||
-                hipStreamSynchronize(stream) != hipSuccess)
-               return done(set_error(FA_E_DEVICE, "metrics failed"));
-            res[i].coverage = (double)cnt / (double)n_px;
-            if ((rc = correlation(d_w, d_cov, d_part, d_sum, parts, &res[i].correlation)) != FA_OK ||
-                (rc = correlation((const float*)fr[i]->d_data, d_cov, d_part, d_sum, parts, &res[i].unaligned_correlation)) !=
-                    FA_OK)
-               return done(rc);
+            // correlation of the warped blurred frame and of the blurred frame itself with the
+            // reference over the covered pixels: stage 1 (whose count is the coverage), then
+            // stage 2 at its means
+            const float* xs[2] = {d_w2, pre + off};
+            for (int q = 0; q < 2; q++) {
+               double* s1 = d_sums + ((int64_t)2 * q * n + j0) * kMoments;
+               double* s2 = d_sums + ((int64_t)(2 * q + 1) * n + j0) * kMoments;
+               hipLaunchKernelGGL(k_moments, dim3(parts, c), dim3(kFB), 0, stream, xs[q], R.rg4, (const float*)d_cov,
+                                  n_px, 1, (const double*)nullptr, d_part);
+               hipLaunchKernelGGL(k_sum_partials, dim3(c), dim3(64), 0, stream, (const double*)d_part, parts, s1);
+               hipLaunchKernelGGL(k_moments, dim3(parts, c), dim3(kFB), 0, stream, xs[q], R.rg4, (const float*)d_cov,
+                                  n_px, 2, (const double*)s1, d_part);
+               hipLaunchKernelGGL(k_sum_partials, dim3(c), dim3(64), 0, stream, (const double*)d_part, parts, s2);
+            }
          }
-         if (hipStreamSynchronize(stream) != hipSuccess) return done(set_error(FA_E_DEVICE, "output copy failed"));
+         if (hipGetLastError() != hipSuccess) return fail("output warp failed");
       }
+      if (res) {
+         std::vector<double> sums((size_t)4 * n * kMoments);
+         if (hipMemcpyAsync(sums.data(), d_sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+             hipStreamSynchronize(stream) != hipSuccess)
+            return fail("metrics failed");
+         for (int i = 0; i < n; i++) {
+            const double* h = sums.data() + (size_t)i * kMoments;
+            res[i].coverage = h[0] / (double)n_px;  // pixels with coverage != 0 (an exact count)
+            res[i].correlation = correlation_of(h, h + (size_t)n * kMoments);
+            res[i].unaligned_correlation = correlation_of(h + (size_t)2 * n * kMoments, h + (size_t)3 * n * kMoments);
+         }
+      }
+      if (hipStreamSynchronize(stream) != hipSuccess) return fail("output copy failed");
       return done(FA_OK);
    }
 };
@@ -690,6 +832,21 @@ int fa_aligner_destroy(fa_aligner* a)
    return FA_OK;
 }
 
+int fa_aligner_clear_temp(fa_aligner* a)
+{
+   if (!a) return set_error(FA_E_ARG, "null argument");
+   std::lock_guard<std::mutex> lk(a->mu);
+   FCHK(hipSetDevice(a->ctx->device));
+   FCHK(hipStreamSynchronize(a->stream));
+   a->drop_cached();
+   for (auto& b : a->bufs) {
+      if (b.p) (void)hipFree(b.p);
+      b.p = nullptr;
+      b.bytes = 0;
+   }
+   return fa_clear_temp(a->ref);
+}
+
 int fa_aligner_reference(fa_aligner* a, fa_reference** ref, int32_t* nD)
 {
    if (!a) return set_error(FA_E_ARG, "null argument");
@@ -732,28 +889,63 @@ int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, i
    std::lock_guard<std::mutex> lk(a->mu);
    FCHK(hipSetDevice(a->ctx->device));
    const int nD = a->nD, m = a->n_dim, np = nD * m;
-   std::vector<fa_frame*> fr(n, nullptr);
+   const int64_t npx = a->n_px;
+   const size_t rb = a->raw_bytes(dtype);
+   const int C = a->chunk_frames(n);
    std::vector<double> shifts(3 * (size_t)n, 0.0);
-   fa_batch* b = nullptr;
+   // the call's device arrays (kept in the aligner between calls): raw uploads, raw values as
+   // float, the warper's frames (blur + 1, registered without a copy), and the rigid shift's
+   // work arrays for C frames
+   uint8_t* d_up = nullptr;
+   float *d_rawf = nullptr, *d_pre = nullptr, *d_btmp = nullptr;
+   double *w_real = nullptr, *w_pv = nullptr;
+   double2* w_spec = nullptr;
+   int64_t *w_pi = nullptr, *d_idx = nullptr;
+   // on an error the cached frames and batch are dropped (rebuilt by the next call)
    auto release = [&]() {
-      if (b) fa_batch_destroy(b);
-      for (fa_frame* f : fr)
-         if (f) fa_deregister_frame(f);
+      (void)hipStreamSynchronize(a->stream);
+      a->drop_cached();
    };
    int rc = FA_OK;
-   // 1. per frame: rigid shift of the raw frame, then the warper's frame (blur + 1) on the device
-   for (int i = 0; i < n && rc == FA_OK; i++) {
-      float* d_frame = nullptr;
-      if ((rc = a->upload(frames[i], dtype)) != FA_OK || (rc = a->shift(&shifts[3 * (size_t)i])) != FA_OK ||
-          (rc = falloc(&d_frame, a->n_px)) != FA_OK)
-         break;
-      if ((rc = a->smooth_plus_one(d_frame)) != FA_OK || hipStreamSynchronize(a->stream) != hipSuccess) {
-         (void)hipGetLastError();
-         ffree(d_frame);
-         if (rc == FA_OK) rc = set_error(FA_E_DEVICE, "frame preprocessing failed");
-         break;
-      }
-      rc = adopt_device_frame(a->ref, d_frame, FA_F32, 0.f, &fr[i]);
+   if ((rc = a->grow(fa_aligner::kUp, (size_t)n * rb, &d_up)) || (rc = a->grow(fa_aligner::kRawF, (size_t)n * npx, &d_rawf)) ||
+       (rc = a->grow(fa_aligner::kPre, (size_t)n * npx, &d_pre)) ||
+       (a->rp.smoothing > 0 && (rc = a->grow(fa_aligner::kBlurTmp, (size_t)C * npx, &d_btmp))) ||
+       (rc = a->grow(fa_aligner::kReal, (size_t)C * a->n_el, &w_real)) ||
+       (rc = a->grow(fa_aligner::kSpec, (size_t)C * a->n_cplx, &w_spec)) ||
+       (rc = a->grow(fa_aligner::kPv, (size_t)C * a->n_part, &w_pv)) ||
+       (rc = a->grow(fa_aligner::kPi, (size_t)C * a->n_part, &w_pi)) || (rc = a->grow(fa_aligner::kIdx, (size_t)n, &d_idx))) {
+      release();
+      return rc;
+   }
+   // 1. all frames uploaded and converted, then C at a time: rigid shifts of the raw frames and
+   //    the warper's frames (blur + 1); the shifts come back in one copy
+   for (int i = 0; i < n && rc == FA_OK; i++)
+      if (hipMemcpyAsync(d_up + (size_t)i * rb, frames[i], rb, hipMemcpyHostToDevice, a->stream) != hipSuccess)
+         rc = set_error(FA_E_DEVICE, "frame upload failed");
+   if (rc == FA_OK) {
+      hipLaunchKernelGGL(k_to_float, dim3(grid_of((int64_t)n * npx)), dim3(kFB), 0, a->stream, (const void*)d_up, dtype,
+                         (int64_t)n * npx, d_rawf);
+      if (hipGetLastError() != hipSuccess) rc = set_error(FA_E_DEVICE, "frame upload failed");
+   }
+   for (int j0 = 0; j0 < n && rc == FA_OK; j0 += C) {
+      const int c = std::min(C, n - j0);
+      if ((rc = a->shifts(d_rawf + (int64_t)j0 * npx, c, w_real, w_spec, w_pv, w_pi, d_idx + j0)) == FA_OK)
+         rc = a->smooth_plus_one(d_rawf + (int64_t)j0 * npx, d_pre + (int64_t)j0 * npx, c, d_btmp);
+   }
+   std::vector<int64_t> idx(n);
+   if (rc == FA_OK && (hipMemcpyAsync(idx.data(), d_idx, n * sizeof(int64_t), hipMemcpyDeviceToHost, a->stream) != hipSuccess ||
+                       hipStreamSynchronize(a->stream) != hipSuccess)) {
+      (void)hipGetLastError();
+      rc = set_error(FA_E_DEVICE, "frame preprocessing failed");
+   }
+   for (int i = 0; i < n && rc == FA_OK; i++) a->shift_of(idx[i], &shifts[3 * (size_t)i]);
+   // the frames (views of d_pre) and their batch: those of the last call when the count is
+   // the same, else new ones
+   if (rc == FA_OK && (int)a->cfr.size() != n) {
+      a->drop_cached();
+      a->cfr.assign(n, nullptr);
+      for (int i = 0; i < n && rc == FA_OK; i++)
+         rc = adopt_device_frame(a->ref, d_pre + (int64_t)i * npx, FA_F32, 0.f, &a->cfr[i], false);
    }
    if (rc != FA_OK) {
       release();
@@ -762,10 +954,15 @@ int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, i
    fa_tr_params p;
    if (prm) p = *prm;
    else fa_tr_default(&p);
-   if ((rc = fa_batch_create(fr.data(), n, &p, &b)) != FA_OK) {
-      release();
-      return rc;
+   if (!a->cbatch) {
+      if ((rc = fa_batch_create(a->cfr.data(), n, &p, &a->cbatch)) != FA_OK) {
+         release();
+         return rc;
+      }
+   } else {
+      batch_set_params(a->cbatch, p);
    }
+   fa_batch* b = a->cbatch;
    // 2. SSD at the zero start (twice in the reference) and at the rigid start
    std::vector<double> D0(3 * (size_t)nD * n, 0.0), Dr(3 * (size_t)nD * n), e0(n), er(n);
    for (int i = 0; i < n; i++)
@@ -811,8 +1008,8 @@ int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, i
       }
    }
    // 5. output images and metrics from the raw frame at the final D
-   if (outputs || results) rc = a->products(frames, n, dtype, D_out, fr, outputs, results);
-   release();
+   if (outputs || results) rc = a->products(n, D_out, d_rawf, d_pre, outputs, results);
+   if (rc != FA_OK) release();
    return rc;
 }
 

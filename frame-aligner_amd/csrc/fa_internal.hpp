@@ -231,6 +231,7 @@ struct fa_frame {
    int dtype = FA_F32;
    float offset = 0;
    void* d_data = nullptr;
+   bool own_data = true;        // false: d_data belongs to the caller of adopt_device_frame
    fa_batch* single = nullptr;  // cached one-frame batch for fa_align and single evaluations
    // Serialises the calls on this frame (fa_align, the single evaluations and warps, which use
    // the frame's own batch and stream); calls on different frames run concurrently.
@@ -240,5 +241,7 @@ struct fa_frame {
 namespace fa {
 // Sets the calling thread's fa_last_error() message; returns code.
 int set_error(int code, const std::string& msg);
-int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offset, fa_frame** out);
+int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offset, fa_frame** out, bool own = true);
+// Trust-region parameters of the next runs of batch b (and of its parts).
+void batch_set_params(fa_batch* b, const fa_tr_params& p);
 }  // namespace fa

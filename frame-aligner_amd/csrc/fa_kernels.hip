@@ -1416,11 +1416,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(M == 2 ?
    }
 }
 
+// Frame blockIdx.y of a launch over several: its samples fstride bytes, its D dstride doubles
+// and its output ostride floats after the first's.
 __global__ __launch_bounds__(kBlock) void k_warp(RefDev R, FrameDev F, const double* D,
                                                  float* out, float invalid, int lds_cap,
-                                                 int use_lds)
+                                                 int use_lds, int64_t fstride, int64_t dstride, int64_t ostride)
 {
    extern __shared__ float lds[];
+   F.data = (const char*)F.data + blockIdx.y * fstride;
+   D += blockIdx.y * dstride;
+   out += blockIdx.y * ostride;
    for (int c = blockIdx.x; c < R.n_chunks; c += gridDim.x) {
       dispatch_chunk<MODE_WARP>(R, F, R.chunks[c], D, lds, lds_cap, use_lds != 0, nullptr, out,
                                 invalid);
@@ -1478,8 +1483,13 @@ __device__ void splat_chunk(const RefDev& R, const FrameDev& F, const Chunk ch, 
 }
 
 __global__ __launch_bounds__(kBlock) void k_splat(RefDev R, FrameDev F, const double* D, float* wimg,
-                                                  float* cov, float offset)
+                                                  float* cov, float offset, int64_t fstride, int64_t dstride,
+                                                  int64_t ostride)
 {
+   F.data = (const char*)F.data + blockIdx.y * fstride;
+   D += blockIdx.y * dstride;
+   wimg += blockIdx.y * ostride;
+   cov += blockIdx.y * ostride;
    for (int c = blockIdx.x; c < R.n_chunks; c += gridDim.x) {
       const Chunk ch = R.chunks[c];
       if (R.n_dim == 2) {
@@ -2427,17 +2437,21 @@ hipError_t launch_pass(const PassArgs& a, int n_dim, int grid, hipStream_t s)
 }
 
 hipError_t launch_warp(const RefDev& R, const FrameDev& F, const double* D, float* out,
-                       float invalid, int lds_cap, int use_lds, int grid, hipStream_t s)
+                       float invalid, int lds_cap, int use_lds, int grid, hipStream_t s, int count)
 {
-   hipLaunchKernelGGL(k_warp, dim3(grid), dim3(kBlock), (size_t)lds_cap * sizeof(float), s, R, F, D,
-                      out, invalid, lds_cap, use_lds);
+   // frames of one launch lie one after the other: samples, nD x 3 displacements, outputs
+   const int64_t fstride = R.n_px * (F.dtype == FA_U16 ? 2 : 4);
+   hipLaunchKernelGGL(k_warp, dim3(grid, count), dim3(kBlock), (size_t)lds_cap * sizeof(float), s, R, F, D,
+                      out, invalid, lds_cap, use_lds, fstride, (int64_t)3 * R.nD, R.n_px);
    return hipGetLastError();
 }
 
 hipError_t launch_splat(const RefDev& R, const FrameDev& F, const double* D, float* wimg, float* cov,
-                        float offset, int grid, hipStream_t s)
+                        float offset, int grid, hipStream_t s, int count)
 {
-   hipLaunchKernelGGL(k_splat, dim3(grid), dim3(kBlock), 0, s, R, F, D, wimg, cov, offset);
+   const int64_t fstride = R.n_px * (F.dtype == FA_U16 ? 2 : 4);
+   hipLaunchKernelGGL(k_splat, dim3(grid, count), dim3(kBlock), 0, s, R, F, D, wimg, cov, offset, fstride,
+                      (int64_t)3 * R.nD, R.n_px);
    return hipGetLastError();
 }
 

@@ -15,10 +15,12 @@ hipError_t launch_hessian(const RefDev& R, double* partials, double* Hdiag, doub
                           double* L0, double* info, hipStream_t s);
 hipError_t launch_eigen(const RefDev& R, double* Q, double* ev, double* info, hipStream_t s);
 hipError_t launch_pass(const PassArgs& a, int n_dim, int grid, hipStream_t s);
+// count frames in one launch: frame i's samples, nD x 3 displacements and outputs follow the
+// first's at i times their sizes
 hipError_t launch_warp(const RefDev& R, const FrameDev& F, const double* D, float* out,
-                       float invalid, int lds_cap, int use_lds, int grid, hipStream_t s);
+                       float invalid, int lds_cap, int use_lds, int grid, hipStream_t s, int count = 1);
 hipError_t launch_splat(const RefDev& R, const FrameDev& F, const double* D, float* wimg, float* cov,
-                        float offset, int grid, hipStream_t s);
+                        float offset, int grid, hipStream_t s, int count = 1);
 hipError_t launch_schedule(const BatchDev& B, int account, hipStream_t s);
 hipError_t launch_tr_init(const BatchDev& B, int eval_only, hipStream_t s);
 hipError_t launch_tr_start_slots(const BatchDev& B, const BatchFrame* hbf, const double* hx, const int32_t* hslots,

@@ -102,7 +102,8 @@ EXPORTS = [
     "fa_server_submit", "fa_server_submit_host", "fa_server_step", "fa_server_results",
     "fa_server_memory", "fa_host_alloc", "fa_host_free",
     "fa_aligner_create", "fa_aligner_destroy", "fa_aligner_reference", "fa_aligner_preprocess",
-    "fa_aligner_rigid_shift", "fa_aligner_add_frames", "fa_aligner_realign_as_frame", "fa_shift_pixel",
+    "fa_aligner_rigid_shift", "fa_aligner_add_frames", "fa_aligner_clear_temp", "fa_aligner_realign_as_frame",
+    "fa_shift_pixel",
     "fa_write_realignment_info",
 ]
 
@@ -159,6 +160,7 @@ def load_library(path=None):
     L.fa_aligner_preprocess.argtypes = [P, P, i32, P]
     L.fa_aligner_rigid_shift.argtypes = [P, P, i32, P]
     L.fa_aligner_add_frames.argtypes = [P, P, i32, i32, C.POINTER(TrParams), P, C.POINTER(AddOutputs), P]
+    L.fa_aligner_clear_temp.argtypes = [P]
     L.fa_aligner_realign_as_frame.argtypes = [P, P, P, i32, i32, i32, P]
     L.fa_shift_pixel.argtypes = [P, P, P]
     L.fa_write_realignment_info.argtypes = [C.c_char_p, C.POINTER(ScanParams), i32, i32, P, P, P, P]
@@ -707,6 +709,11 @@ class Aligner:
                         stats=stats_dict(r.stats), correlation=r.correlation,
                         unaligned_correlation=r.unaligned_correlation, coverage=r.coverage) for r in res]
         return [D[i] for i in range(n)], results, imgs
+
+    def clear_temp(self):
+        """FrameWarpAligner::clearTemp: frees what add_frames keeps between calls and the
+        warper's cached working spaces."""
+        _check(self.L.fa_aligner_clear_temp(self.h))
 
     def realign_as_frame(self, D, frame, interpolate_missing=False, cv_type=0):
         """FrameWarpAligner::realignAsFrame: frame (as given, no blur or offset) realigned with D

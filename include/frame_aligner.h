@@ -234,9 +234,12 @@ int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm,
 
 /* Serving (continuous batching). The references are grouped into lanes by kind (2-D, 3-D)
    and size (a lane's largest reference has at most 16 times the pixels of its smallest; at
-   most 3 lanes). Each lane runs its own slot batch on its own stream (a server with one lane
-   and 16 or more slots runs it as two that interleave), with `slots` slots each sized for the
-   lane's largest reference; at most `slots` frames are admitted at once over all lanes. Host
+   most 3 lanes). Each lane runs its own slot batch on one of the context's three lane streams
+   (a server with one lane and 16 or more slots runs it as two that interleave), with `slots`
+   slots each sized for the lane's largest reference; at most `slots` frames are admitted at
+   once over all lanes. The lane streams are created by fa_create with the context's own stream,
+   so they keep hardware queues of their own; host frames are uploaded on the context's stream.
+   Servers of one context share its lane streams (their rounds then run one after another). Host
    frames go into a device frame arena of arena_bytes (0: slots times the largest float frame,
    within the device's free memory less 32 MiB; FA_E_OOM "Not enough video memory to use GPU"
    when even one frame does not fit, as GpuFrameWarper::setupReferenceInformation,
@@ -344,10 +347,18 @@ int fa_aligner_rigid_shift(fa_aligner* a, const void* raw, int32_t dtype, double
 /* FrameWarpAligner::addFrame (FrameWarpAligner.cpp:117-250) for n raw frames of dtype: blur
    and +1, the rigid start, the start with the smallest SSD, then (type Warp) the trust region
    of all n frames as one batch, then the output images and metrics. D_out: n x nD x 3;
-   outputs and results (n entries) may be NULL. prm NULL = the reference's stop rule. */
+   outputs and results (n entries) may be NULL. prm NULL = the reference's stop rule. The
+   frames run through batched phases (uploads, then rigid shifts and blur up to 32 frames at a
+   time, the batch's evaluations and trust region, the output warps), and the aligner keeps
+   its device arrays, and the frames and batch of the last call, for the next call with the
+   same frame count (fa_aligner_clear_temp frees them). */
 int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, int32_t dtype,
                           const fa_tr_params* prm, double* D_out, const fa_add_outputs* outputs,
                           fa_add_result* results);
+/* FrameWarpAligner::clearTemp (FrameWarpAligner.cpp:72-75, which calls the warper's
+   clearTemp): frees what fa_aligner_add_frames keeps between calls and the warper's cached
+   working spaces (fa_clear_temp). */
+int fa_aligner_clear_temp(fa_aligner* a);
 /* FrameWarpAligner::realignAsFrame (FrameWarpAligner.cpp:270-285): frame (nz*ny*nx values of
    dtype, warped as given: no blur, no offset) realigned with the displacements D (nD x 3) that
    were found for some frame. interpolate_missing = 1: warpImageInterpolated with invalid value

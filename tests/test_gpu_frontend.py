@@ -129,6 +129,32 @@ def test_translation_type_returns_the_start(ctx):
     al.close()
 
 
+def test_add_frames_kept_state_between_calls(ctx):
+    """The arrays, frames and batch add_frames keeps between calls: calls of the same and of
+    other frame counts, with other frames and stop rules, and after clear_temp, each give what
+    a fresh aligner gives for those frames."""
+    st = synth.make_stack((1, 96, 80), 7, 9, seed=14, n_blobs=120, dtype="uint16")
+    fr = list(st["frames"])
+    kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
+    calls = [(fr[:3], kw), (fr[3:6], kw), (fr[:3], dict(rel_delta=1e-3, max_iter=20, skip_repeat=1)),
+             (fr[1:6], kw), (fr[:2], kw)]
+    al = F.Aligner(ctx, st["reference"], n_resampling_points=8, smoothing=1.0)
+    for k, (frames, p) in enumerate(calls):
+        if k == 3:
+            al.clear_temp()
+        D1, r1, im1 = al.add_frames(frames, outputs=("realigned",), **p)
+        fresh = F.Aligner(ctx, st["reference"], n_resampling_points=8, smoothing=1.0)
+        D2, r2, im2 = fresh.add_frames(frames, outputs=("realigned",), **p)
+        fresh.close()
+        for a, b in zip(D1, D2):
+            assert np.array_equal(a, b), k
+        for a, b in zip(r1, r2):
+            assert a["stats"] == b["stats"] and a["start"] == b["start"], k
+            assert a["correlation"] == b["correlation"] and a["coverage"] == b["coverage"], k
+        assert np.array_equal(im1["realigned"], im2["realigned"]), k
+    al.close()
+
+
 @pytest.mark.parametrize("shape,n_res", [((1, 96, 80), 8), ((4, 40, 36), 5)])
 def test_add_frames_outputs_and_metrics(ctx, shape, n_res):
     """RealignmentResult's images and metrics (FrameWarpAligner.cpp:218-240,317-346) at the
