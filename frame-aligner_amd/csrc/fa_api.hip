@@ -431,7 +431,10 @@ void set_grids(fa_batch* b)
    for (auto& c : b->pcls) c.grid = c.work ? (int)std::max<int64_t>(1, std::min(c.work, cap)) : 0;
 }
 
-int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out)
+// stream: a stream of the context to run on (the parts of a split batch), or null for a
+// stream of the batch's own.
+int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out,
+                hipStream_t stream = nullptr)
 {
    *out = nullptr;
    if (n <= 0 || !frames) return set_err(FA_E_ARG, "empty batch");
@@ -525,7 +528,10 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
       b->h_counters = nullptr;
       return fail(set_err(FA_E_OOM, "Not enough memory to allocate"));
    }
-   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+   if (stream) {
+      b->stream = stream;
+      b->own_stream = false;
+   } else if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
       (void)hipGetLastError();
       b->stream = nullptr;
       return fail(set_err(FA_E_DEVICE, "stream creation failed"));
@@ -700,7 +706,14 @@ int ensure_graph(fa_batch* b, int K)
    if (b->gexec && b->graph_rounds == K && std::memcmp(&b->graph_prm, &b->prm, sizeof(fa_tr_params)) == 0)
       return FA_OK;
    b->drop_graph();
-   hipStream_t s = b->stream;
+   // captured on a stream of its own: the batch's stream may be a lane stream of the context,
+   // on which another thread could enqueue work during the capture
+   hipStream_t s = nullptr;
+   HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+   struct Drop {
+      hipStream_t s;
+      ~Drop() { (void)hipStreamDestroy(s); }
+   } drop{s};
    const BatchDev B = b->dev();
    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
    int rc = FA_OK;
@@ -865,8 +878,11 @@ int batch_create_split(fa_frame* const* frames, int32_t n, const fa_tr_params* p
    cut = std::max(1, cut);
    fa_batch* a = nullptr;
    fa_batch* c = nullptr;
-   CHK(batch_build(frames, cut, prm, &a));
-   int rc = batch_build(frames + cut, n - cut, prm, &c);
+   // the parts run on two lane streams of the context, which have hardware queues of their
+   // own, so their rounds interleave on the device
+   fa_context* ctx = frames[0]->ref->ctx;
+   CHK(batch_build(frames, cut, prm, &a, ctx->lane_stream[0]));
+   int rc = batch_build(frames + cut, n - cut, prm, &c, ctx->lane_stream[1]);
    if (rc != FA_OK) {
       delete a;
       return rc;
