@@ -11,7 +11,7 @@ OUT="$HERE/lib/libfa_hip.so"
   -Xclang -target-feature -Xclang -packed-fp32-ops \
   -I"$ROOT/include" -I"$HERE/csrc" \
   "$HERE/csrc/fa_kernels.hip" "$HERE/csrc/fa_api.hip" "$HERE/csrc/fa_frontend.hip" \
-  -L/opt/rocm/lib -lrocfft -Wl,-rpath,/opt/rocm/lib \
+  -L/opt/rocm/lib -lrocfft -ldl -Wl,-rpath,/opt/rocm/lib \
   -o "$OUT.tmp"
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"

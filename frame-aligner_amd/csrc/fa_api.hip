@@ -14,6 +14,10 @@
 #include <thread>
 #include <vector>
 
+#include <dlfcn.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "fa_kernels.hpp"
 
 using namespace fa;
@@ -168,6 +172,9 @@ struct fa_batch {
    // 1: one reference, one update launch: the update's last workgroup compacts the active
    // frames for the next round, so a round is a pass and an update (no schedule launch)
    int fold = 0;
+   // largest nD n_dim of the batch's references whose subproblem runs in the batched
+   // eigen-space kernels (RefDev::eig == 2), 0 when none has one
+   int eig_n = 0;
 
    BatchDev dev() const
    {
@@ -199,6 +206,7 @@ struct fa_batch {
       B.trace_cap = trace_cap;
       B.trace_rec = trace_rec;
       B.fold = fold;
+      B.split_eig = eig_n > 0 ? 1 : 0;
       return B;
    }
 
@@ -549,6 +557,8 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
    }
    set_grids(b);
    b->fold = (rs.size() == 1 && b->trc_lds.size() == 1) ? 1 : 0;
+   for (const RefDev& rd : refs)
+      if (rd.eig == 2) b->eig_n = std::max(b->eig_n, rd.nD * rd.n_dim);
    *out = b;
    return FA_OK;
 }
@@ -590,11 +600,13 @@ int launch_updates(fa_batch* b, hipStream_t s)
       const int lds = b->tune.tr_lds ? most : 0;
       if ((fit || !b->tune.tr_lds) && (int64_t)B.n_frames <= (int64_t)b->ctx->n_cu * tr_update_all_per_cu(lds)) {
          HIPCHK(launch_tr_update_all(B, lds, wide, lds ? full : 0u, s));
+         if (b->eig_n) HIPCHK(launch_eig_subproblems(B, b->eig_n, s));
          return FA_OK;
       }
    }
    for (size_t c = 0; c < nc; c++)
       HIPCHK(launch_tr_update(B, (int)c, b->tune.tr_lds ? b->trc_lds[c] : 0, b->trc_full[c], b->trc_threads[c], s));
+   if (b->eig_n) HIPCHK(launch_eig_subproblems(B, b->eig_n, s));
    return FA_OK;
 }
 
@@ -1192,6 +1204,8 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
       return fail(set_err(FA_E_DEVICE, "server upload failed"));
    }
    set_grids(b);
+   for (const RefDev& rd : rdev)
+      if (rd.eig == 2) b->eig_n = std::max(b->eig_n, rd.nD * rd.n_dim);
    *out = b;
    return FA_OK;
 }
@@ -1410,6 +1424,93 @@ int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offse
 
 }  // namespace fa
 
+// ---- eigendecomposition of long node chains -------------------------------------------------
+namespace {
+
+// rocSOLVER's symmetric divide-and-conquer eigensolver, opened on first use: only references
+// whose nD n_dim lies in (kEigMax, kEigBatchMax] need it, and the library is large. Without
+// it those references keep the cyclic-reduction subproblem.
+struct EigLib {
+   decltype(&rocblas_create_handle) create = nullptr;
+   decltype(&rocblas_destroy_handle) destroy = nullptr;
+   decltype(&rocblas_set_stream) set_stream = nullptr;
+   decltype(&rocsolver_dsyevd) syevd = nullptr;
+};
+
+const EigLib* eig_lib()
+{
+   static EigLib lib;
+   static std::once_flag once;
+   std::call_once(once, [] {
+      void* h = dlopen("librocsolver.so.0", RTLD_NOW | RTLD_LOCAL);
+      if (!h) return;
+      EigLib l;
+      l.create = (decltype(l.create))dlsym(h, "rocblas_create_handle");
+      l.destroy = (decltype(l.destroy))dlsym(h, "rocblas_destroy_handle");
+      l.set_stream = (decltype(l.set_stream))dlsym(h, "rocblas_set_stream");
+      l.syevd = (decltype(l.syevd))dlsym(h, "rocsolver_dsyevd");
+      if (l.create && l.destroy && l.set_stream && l.syevd) lib = l;
+   });
+   return lib.syevd ? &lib : nullptr;
+}
+
+// H = Q diag(ev) Q^T for the batched subproblem kernels (RefDev::eig == 2): the dense matrix
+// from the blocks H holds on the host, rocsolver_dsyevd on the stream. Its column-major output
+// leaves eigenvector i in row i of r->d_Q read row-major; ev ascending. False (and nothing
+// kept) when the solver is absent, fails or does not converge.
+bool large_eigen(fa_reference* r, hipStream_t s)
+{
+   const EigLib* L = eig_lib();
+   if (!L) return false;
+   const int m = r->n_dim, mm = m * m, nb = r->nD, n = nb * m;
+   std::vector<double> H((size_t)n * n, 0.0);
+   for (int k = 0; k < nb; k++)
+      for (int a = 0; a < m; a++)
+         for (int b = 0; b < m; b++) {
+            H[(size_t)(k * m + a) * n + k * m + b] = r->Hdiag[(size_t)k * mm + a * m + b];
+            if (k + 1 < nb) {
+               const double v = r->Hsub[(size_t)k * mm + a * m + b];  // H((k+1) m + a, k m + b)
+               H[(size_t)((k + 1) * m + a) * n + k * m + b] = v;
+               H[(size_t)(k * m + b) * n + (k + 1) * m + a] = v;
+            }
+         }
+   double* d_Q = nullptr;
+   double* d_ev = nullptr;
+   double* d_E = nullptr;
+   rocblas_int* d_info = nullptr;
+   rocblas_handle h = nullptr;
+   bool ok = dalloc(&d_Q, (size_t)n * n) == FA_OK && dalloc(&d_ev, n) == FA_OK && dalloc(&d_E, n) == FA_OK &&
+             dalloc(&d_info, 1) == FA_OK;
+   rocblas_int info = -1;
+   double ev0 = 0;
+   ok = ok && L->create(&h) == rocblas_status_success;
+   ok = ok && L->set_stream(h, s) == rocblas_status_success &&
+        hipMemcpyAsync(d_Q, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess &&
+        L->syevd(h, rocblas_evect_original, rocblas_fill_upper, n, d_Q, n, d_ev, d_E, d_info) ==
+            rocblas_status_success &&
+        hipMemcpyAsync(&info, d_info, sizeof info, hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipMemcpyAsync(&ev0, d_ev, sizeof ev0, hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess && info == 0;
+   if (h) L->destroy(h);
+   (void)hipGetLastError();
+   dfree(d_E);
+   dfree(d_info);
+   if (!ok) {
+      dfree(d_Q);
+      dfree(d_ev);
+      return false;
+   }
+   r->d_Q = d_Q;
+   r->d_ev = d_ev;
+   r->dev.Q = d_Q;
+   r->dev.ev = d_ev;
+   r->dev.ev_min = ev0;
+   r->dev.eig = 2;
+   return true;
+}
+
+}  // namespace
+
 // ---- allocation-free alignment --------------------------------------------------------------
 // fa_align_noalloc runs fa_align's kernels with every device buffer it needs (the frame's
 // trust-region vectors, factor scratch, trial displacement, pass partials, state and counters)
@@ -1528,6 +1629,7 @@ int align_noalloc(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_alig
    B.n_frames = 1;
    B.n_refs = 1;
    B.trace = nullptr;
+   B.split_eig = r->dev.eig == 2 ? 1 : 0;
    // the launch shape and update variant a one-frame batch of this reference takes
    const int lds_floats = (int)std::min<int64_t>(lds_floats_for(r), kLdsCapFloats);
    PassArgs pa{};
@@ -1568,6 +1670,7 @@ int align_noalloc(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_alig
          HIPCHK(launch_schedule(B, 1, s));
          HIPCHK(launch_pass(pa, r->n_dim, grid, s));
          HIPCHK(launch_tr_update(B, 0, tr_lds, tr_full, tr_nt, s));
+         if (B.split_eig) HIPCHK(launch_eig_subproblems(B, n, s));
       }
       HIPCHK(launch_schedule(B, 0, s));
       HIPCHK(hipMemcpyAsync(&hc, d_counters, sizeof hc, hipMemcpyDeviceToHost, s));
@@ -1876,6 +1979,8 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
       R.eig = info[0] != 0.0 ? 1 : 0;
       R.ev_min = info[1];
       R.q_lds = R.eig && tr_lds_full_base(r) + tr_lds_q(r) <= kTrFullDoubles ? 1 : 0;
+   } else if (nn > kEigMax && nn <= kEigBatchMax) {
+      large_eigen(r, s);  // else the cyclic-reduction subproblem
    }
    cleanup();
    ctx->live_refs++;

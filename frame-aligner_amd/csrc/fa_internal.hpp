@@ -49,6 +49,8 @@ static_assert(kStripWidth == kBlock, "a strip item has one column per fused-pass
 constexpr int kPad = 16;
 constexpr int kMaxRefs = 4096;  // distinct references in one batch
 constexpr int kEigMax = 90;     // largest nD * n_dim whose subproblem uses the eigendecomposition of H
+// largest nD * n_dim whose subproblem runs in the batched eigen-space kernels (Q: 72 MiB)
+constexpr int kEigBatchMax = 3072;
 
 // Per-pixel reference record read by every pass. 2-D: (R, gx, gy, Df) in one float4;
 // 3-D: (R, gx, gy, gz) plus Df in a separate float array. Df is the node fraction of
@@ -65,7 +67,8 @@ struct RefDev {
    const double* Q;        // eigenvectors of H, n x n row-major, column i for ev[i], then Q^T (valid when eig)
    const double* ev;       // eigenvalues of H
    ScanDev scan;
-   int32_t nx, ny, nz, nD, n_dim, n_chunks, h_pd, eig;
+   int32_t nx, ny, nz, nD, n_dim, n_chunks, h_pd;
+   int32_t eig;            // 0: none; 1: Q, Q^T (n <= 90, solved in the update); 2: Q only (batched kernels)
    int32_t q_lds;          // 1: an update that stages its working set in LDS also stages Q
    double eig_lo;          // Gershgorin lower bound of the spectrum of H
    double ev_min;          // smallest eigenvalue (valid when eig)
@@ -173,7 +176,17 @@ struct BatchDev {
    int32_t trace_cap;
    int32_t trace_rec; // kTraceFields + 3 * (largest nD n_dim of the batch)
    int32_t fold;      // 1: the update's last workgroup schedules the next round (one reference)
+   // 1: frames of references with a large eigendecomposition (RefDev::eig == 2) leave their
+   // subproblem to the batched eigen-space kernels launched after the update (k_eig_gemm,
+   // k_eig_secular); 0: they solve it by cyclic reduction in the update
+   int32_t split_eig;
 };
+
+// Trust-region states of a frame whose subproblem runs in the batched eigen-space kernels:
+// left by the update (qv = Q^T g to be formed), by the secular iteration with w = qv / (ev + lam)
+// to be turned into p = -Q w, or with p already zero (no solution). The next update takes the
+// last two as 1 (a step was proposed) and sets 1 when the frame stops.
+constexpr int kPhaseEigPending = 2, kPhaseEigW = 3, kPhaseEigZero = 4;
 
 }  // namespace fa
 
@@ -218,7 +231,7 @@ struct fa_reference {
    double* d_Hdiag = nullptr;
    double* d_Hsub = nullptr;
    double* d_L0 = nullptr;  // 3 nD n_dim^2: factor blocks A, C, W of H
-   double* d_Q = nullptr;   // eigenvectors and eigenvalues of H (nD n_dim <= kEigMax)
+   double* d_Q = nullptr;   // eigenvectors and eigenvalues of H (nD n_dim <= kEigBatchMax)
    double* d_ev = nullptr;
    fa::RefDev dev{};
    int live_frames = 0;

@@ -2222,10 +2222,17 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
       st.status = FA_STOP_MAX_EVALS;
       st.done = 1;
    }
-   if (!st.done) {
+   const bool batched_eig = R.eig == 2 && B.split_eig;
+   if (st.done && st.phase > 1) st.phase = 1;  // a phase the batched subproblem kernels left
+   if (!st.done && batched_eig) {
+      // the subproblem runs in the batched eigen-space kernels that follow the update (g is
+      // written back to the frame's vectors below when it was staged in LDS)
+      st.phase = kPhaseEigPending;
+   } else if (!st.done) {
       double lam;
-      st.sub_iters = R.eig ? tr_subproblem_eig<NT>(R, S, qs, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam)
-                           : tr_subproblem<M, NT>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
+      st.sub_iters = R.eig == 1
+                        ? tr_subproblem_eig<NT>(R, S, qs, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam)
+                        : tr_subproblem<M, NT>(R, S, S.g, st.radius, prm.sub_eps, prm.sub_max_iter, S.p, &lam);
       st.lambda = lam;
       st.phase = 1;
       double* Dt = B.Dtrial + bf.d_base;
@@ -2234,9 +2241,11 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
          for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < M ? S.x[i * M + c] + S.p[i * M + c] : 0.0;
    }
    if (B.trace && st.evaluations - 1 < B.trace_cap) {
-      // record of this evaluation (layout of kTraceFields), then x, g, p
+      // record of this evaluation (layout of kTraceFields), then x, g, p; a subproblem left to
+      // the batched kernels gets its fields and p from k_eig_trace
       __syncthreads();
-      const bool ran = !st.done;
+      const bool later = st.phase == kPhaseEigPending;
+      const bool ran = !st.done && !later;
       const double pn = ran ? sqrt(block_dot<NT>(n, S.p, S.p)) : 0.0;
       const double gn = sqrt(block_dot<NT>(n, S.g, S.g));
       const double gp = ran ? block_dot<NT>(n, S.g, S.p) : 0.0;
@@ -2251,7 +2260,7 @@ __device__ void tr_update_frame(const BatchDev& B, const int j, double* smem)
       for (int i = threadIdx.x; i < n; i += NT) {
          o[kTraceFields + i] = S.x[i];
          o[kTraceFields + n + i] = S.g[i];
-         o[kTraceFields + 2 * n + i] = S.p[i];
+         if (!later) o[kTraceFields + 2 * n + i] = S.p[i];
       }
    }
    if (threadIdx.x == 0) B.state[j] = st;
@@ -2374,6 +2383,278 @@ __global__ __launch_bounds__(256) void k_tr_update_all(BatchDev B, uint32_t wide
 }
 
 // ---- launchers ----------------------------------------------------------------------------
+
+// ---- batched subproblems through H = Q diag(ev) Q^T (references with RefDev::eig == 2) ----
+// A frame whose update left its subproblem pending (kPhaseEigPending) takes it in three steps
+// over the whole batch: qv = Q^T g for all of a reference's pending frames at once
+// (k_eig_gemm<false>: Q is read once per round rather than once per frame), the secular
+// iteration of tr_subproblem_eig on qv (k_eig_secular), then p = -Q w and the trial
+// displacement x + p (k_eig_gemm<true>, k_eig_apply); k_eig_trace completes the trace
+// records when the batch keeps one. The frame's next update takes phases kPhaseEigW and
+// kPhaseEigZero as 1 (a step was proposed). Here Q is the matrix of eigenvectors (column i for
+// ev[i]); RefDev::Q stores it transposed, eigenvector i in row i.
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+// 16-row blocks of the operator; 64 frames per workgroup; k split over kGemmSplit waves of a
+// workgroup and kGemmGroups workgroups. A frame's factor scratch holds the kGemmGroups
+// partial products (n each), then w (n): 5 n doubles, within its 3 nD n_dim^2 (>= 6 n).
+constexpr int kGemmRows = 16, kGemmCols = 64, kGemmSplit = 4, kGemmGroups = 4;
+
+// Partial products over k group blockIdx.z % kGemmGroups for output rows [16 bx, 16 bx + 16)
+// of reference blockIdx.y (when its eig == 2), for the frames among entries
+// [64 (bz / kGemmGroups), +64) of its active list that are in phase kPhaseEigPending
+// (TRANS false: qv = Q^T g, each row of RefDev::Q times g) or kPhaseEigW (TRANS true: Q w,
+// to be negated into p). Partial kg goes to the frame's factor scratch at kg n. FP64
+// MFMA 16x16x4 steps: wave w takes columns [16 (w & 3), +16) and the k range w >> 2 of the
+// group's four, whose sums are added in k order. An output element depends only on its row of
+// the operator and its frame's vector, summed in a fixed order, so a frame's result does not
+// depend on the other frames. Lanes of rows past n or columns past the frames read row n - 1 or
+// the first frame's vector; their outputs are not stored.
+template <bool TRANS>
+__global__ __launch_bounds__(256 * kGemmSplit) void k_eig_gemm(BatchDev B)
+{
+   const int r = blockIdx.y;
+   const RefDev& R = B.refs[r];
+   if (R.eig != 2) return;
+   const int n = R.nD * R.n_dim;
+   const int row0 = blockIdx.x * kGemmRows;
+   const int kg = blockIdx.z % kGemmGroups;
+   const int first = B.run_first[r], count = B.run_count[r];
+   const int base = (blockIdx.z / kGemmGroups) * kGemmCols;
+   if (row0 >= n || base >= count) return;
+   __shared__ int cols[kGemmCols];
+   __shared__ int s_nc;
+   __shared__ dbl4 part[kGemmSplit - 1][4][64];
+   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+   const int want = TRANS ? kPhaseEigW : kPhaseEigPending;
+   if (t < 64) {
+      const int i = base + t;
+      const int j = i < count ? B.active_frames[first + i] : -1;
+      const bool take = j >= 0 && B.state[j].phase == want;
+      const unsigned long long m = __ballot(take);
+      if (take) cols[__popcll(m & ((1ull << t) - 1))] = j;
+      if (t == 0) s_nc = __popcll(m);
+   }
+   __syncthreads();
+   const int nc = s_nc;
+   const int cb = w & 3, ks = kg * kGemmSplit + (w >> 2);  // k range ks of 16
+   const bool busy = cb * 16 < nc;  // uniform over the wave
+   dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+   if (busy) {
+      const int li = lane & 15, lk = lane >> 4;
+      // A operand: lane (li, lk) holds A[li][lk] of each 16x4 step, B operand: B[lk][li]
+      const int row = min(row0 + li, n - 1);
+      const int c = min(cb * 16 + li, nc - 1);
+      const BatchFrame& bf = B.frames[cols[c]];
+      const double* __restrict__ v = TRANS ? B.fac + bf.fac_base + kGemmGroups * n : B.vec + bf.vec_base + n;
+      const double* __restrict__ qa = TRANS ? R.Q + row : R.Q + (int64_t)row * n;
+      const int64_t qs = TRANS ? n : 1;
+      constexpr int KS = kGemmSplit * kGemmGroups;
+      const int span = ((n + 4 * KS - 1) / (4 * KS)) * 4;
+      const int k_lo = min(n, ks * span), k_hi = min(n, k_lo + span);
+      int k = k_lo;
+      dbl4 acc1 = {0.0, 0.0, 0.0, 0.0};
+      // eight steps' operands are loaded before their products, so their loads overlap
+      constexpr int U = 8;
+      for (; k + 4 * U <= k_hi; k += 4 * U) {
+         double a[U], b[U];
+#pragma unroll
+         for (int u = 0; u < U; u++) {
+            a[u] = qa[(k + 4 * u + lk) * qs];
+            b[u] = v[k + 4 * u + lk];
+         }
+#pragma unroll
+         for (int u = 0; u < U; u += 2) {
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u + 1], b[u + 1], acc1, 0, 0, 0);
+         }
+      }
+      for (; k < k_hi; k += 4) {
+         const bool in = k + lk < k_hi;
+         const double a0 = in ? qa[(k + lk) * qs] : 0.0, b0 = in ? v[k + lk] : 0.0;
+         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc, 0, 0, 0);
+      }
+      acc += acc1;
+   }
+   if (w >= 4) part[(w >> 2) - 1][cb][lane] = acc;
+   __syncthreads();
+   if (w < 4 && busy) {
+#pragma unroll
+      for (int q = 0; q < kGemmSplit - 1; q++) acc += part[q][cb][lane];
+      // lane holds D[(lane >> 4) + 4 e][lane & 15], e = 0..3 (the FP64 16x16x4 output layout)
+      const int c = cb * 16 + (lane & 15);
+      if (c < nc) {
+         double* o = B.fac + B.frames[cols[c]].fac_base + (int64_t)kg * n;
+#pragma unroll
+         for (int e = 0; e < 4; e++) {
+            const int row = row0 + (lane >> 4) + 4 * e;
+            if (row < n) o[row] = acc[e];
+         }
+      }
+   }
+}
+
+// p = -(sum of the kGemmGroups partial products of Q^T w, in order) and the trial displacement
+// x + p of a frame in phase kPhaseEigW.
+__global__ __launch_bounds__(256) void k_eig_apply(BatchDev B)
+{
+   const int j = blockIdx.x;
+   if (B.state[j].phase != kPhaseEigW) return;
+   const BatchFrame& bf = B.frames[j];
+   const int n = bf.n, m = bf.r.n_dim;
+   const double* P = B.fac + bf.fac_base;
+   const double* x = B.x + bf.x_base;
+   double* p = B.vec + bf.vec_base;
+   double* Dt = B.Dtrial + bf.d_base;
+   for (int i = threadIdx.x; i < n; i += 256) {
+      double s = P[i];
+#pragma unroll
+      for (int q = 1; q < kGemmGroups; q++) s += P[(int64_t)q * n + i];
+      const double pv = -s;
+      p[i] = pv;
+      const int node = i / m, comp = i - node * m;
+      Dt[3 * node + comp] = x[i] + pv;
+      if (m == 2 && comp == 1) Dt[3 * node + 2] = 0.0;
+   }
+}
+
+// The secular iteration of tr_subproblem_eig for a frame in phase kPhaseEigPending, from
+// qv = Q^T g (the sum of k_eig_gemm<false>'s partial products), held in registers with the
+// eigenvalues (at most kEigRegs per thread): lambda and the iteration count into its state,
+// and either w = qv / (ev + lambda) after the partial products (phase kPhaseEigW; then
+// k_eig_gemm<true> and k_eig_apply form p and x + p) or,
+// when the iteration found no lambda at which H + lambda I is positive definite or g = 0,
+// p = 0 and the trial displacement x (phase kPhaseEigZero).
+constexpr int kEigRegs = (kEigBatchMax + 255) / 256;
+
+__global__ __launch_bounds__(256) void k_eig_secular(BatchDev B)
+{
+   constexpr int NT = 256;
+   const int j = blockIdx.x;
+   TrState st = B.state[j];
+   if (st.phase != kPhaseEigPending) return;
+   const BatchFrame& bf = B.frames[j];
+   const RefDev& R = bf.r;
+   const int n = bf.n, t = threadIdx.x;
+   const double* g = B.vec + bf.vec_base + n;
+   const double* P = B.fac + bf.fac_base;  // the partial products of qv, in order
+   double qr[kEigRegs], er[kEigRegs];
+#pragma unroll
+   for (int u = 0; u < kEigRegs; u++) {
+      const int i = t + NT * u;
+      double q = 0.0;
+      if (i < n) {
+         q = P[i];
+#pragma unroll
+         for (int k = 1; k < kGemmGroups; k++) q += P[(int64_t)k * n + i];
+      }
+      qr[u] = q;
+      er[u] = i < n ? R.ev[i] : 1.0;
+   }
+   const double radius = st.radius, eps = B.prm.sub_eps;
+   const int max_iter = B.prm.sub_max_iter;
+   const double gnorm = sqrt(block_dot<NT>(n, g, g));
+   double lam = 0, lam_lo = 0, lam_hi = 0, lam_sol = -1;
+   int it = 1;
+   if (gnorm != 0) {
+      lam_hi = gnorm / radius + (R.eig_lo < 0 ? -R.eig_lo : 0);
+      for (it = 0; it < max_iter; it++) {
+         const bool ok = lam == 0 ? R.h_pd != 0 : R.ev_min + lam > 0;
+         if (!ok) {
+            lam_lo = lam;
+            lam = 0.5 * (lam_lo + lam_hi);
+            continue;
+         }
+         // the sums of tr_subproblem_eig, in the same per-thread order (i = t, t + NT, ...)
+         double s2 = 0, s3 = 0;
+#pragma unroll
+         for (int u = 0; u < kEigRegs; u++) {
+            if (t + NT * u < n) {
+               const double d = 1.0 / (er[u] + lam);
+               const double a = qr[u] * d;
+               s2 += a * a;
+               s3 += a * a * d;
+            }
+         }
+         const double pn = sqrt(block_sum<NT>(s2));
+         lam_sol = lam;
+         if (lam == 0 && pn <= radius) {
+            it++;
+            break;
+         }
+         if (fabs(pn - radius) <= eps * radius) {
+            it++;
+            break;
+         }
+         if (pn > radius) lam_lo = lam;
+         else lam_hi = lam;
+         const double qn = sqrt(block_sum<NT>(s3));
+         double lam_new = lam + (pn / qn) * (pn / qn) * (pn - radius) / radius;
+         if (!(lam_new > lam_lo && lam_new < lam_hi)) lam_new = 0.5 * (lam_lo + lam_hi);
+         lam = lam_new;
+      }
+   }
+   if (lam_sol >= 0) {
+      double* w = B.fac + bf.fac_base + kGemmGroups * n;
+#pragma unroll
+      for (int u = 0; u < kEigRegs; u++) {
+         const int i = t + NT * u;
+         if (i < n) w[i] = qr[u] / (er[u] + lam_sol);
+      }
+      st.lambda = lam_sol;
+      st.phase = kPhaseEigW;
+   } else {
+      double* p = B.vec + bf.vec_base;
+      const double* x = B.x + bf.x_base;
+      double* Dt = B.Dtrial + bf.d_base;
+      const int m = R.n_dim;
+      for (int i = t; i < n; i += NT) p[i] = 0.0;
+      for (int i = t; i < R.nD; i += NT)
+         for (int c = 0; c < 3; c++) Dt[3 * i + c] = c < m ? x[i * m + c] + 0.0 : 0.0;
+      st.lambda = 0;
+      st.phase = kPhaseEigZero;
+   }
+   st.sub_iters = it;
+   if (t == 0) B.state[j] = st;
+}
+
+// Trace records of the frames whose subproblem the batched kernels solved in this round
+// (phase kPhaseEigW or kPhaseEigZero): lambda, iterations, |p|, g'p and p (the update wrote the
+// rest of the record). Launched only when the batch records a trace.
+__global__ __launch_bounds__(256) void k_eig_trace(BatchDev B)
+{
+   constexpr int NT = 256;
+   const int j = blockIdx.x;
+   const TrState st = B.state[j];
+   if (st.phase != kPhaseEigW && st.phase != kPhaseEigZero) return;
+   if (st.evaluations - 1 >= B.trace_cap) return;
+   const BatchFrame& bf = B.frames[j];
+   const int n = bf.n;
+   const double* p = B.vec + bf.vec_base;
+   const double* g = p + n;
+   const double pn = sqrt(block_dot<NT>(n, p, p));
+   const double gp = block_dot<NT>(n, g, p);
+   double* o = B.trace + (int64_t)j * B.trace_cap * B.trace_rec + (int64_t)(st.evaluations - 1) * B.trace_rec;
+   if (threadIdx.x == 0) {
+      o[10] = st.lambda;
+      o[11] = (double)st.sub_iters;
+      o[12] = pn;
+      o[14] = gp;
+   }
+   for (int i = threadIdx.x; i < n; i += NT) o[kTraceFields + 2 * n + i] = p[i];
+}
+
+hipError_t launch_eig_subproblems(const BatchDev& B, int n_max, hipStream_t s)
+{
+   const dim3 gg((unsigned)((n_max + kGemmRows - 1) / kGemmRows), (unsigned)B.n_refs,
+                 (unsigned)((B.n_frames + kGemmCols - 1) / kGemmCols * kGemmGroups));
+   hipLaunchKernelGGL(k_eig_gemm<false>, gg, dim3(256 * kGemmSplit), 0, s, B);
+   hipLaunchKernelGGL(k_eig_secular, dim3(B.n_frames), dim3(256), 0, s, B);
+   hipLaunchKernelGGL(k_eig_gemm<true>, gg, dim3(256 * kGemmSplit), 0, s, B);
+   hipLaunchKernelGGL(k_eig_apply, dim3(B.n_frames), dim3(256), 0, s, B);
+   if (B.trace) hipLaunchKernelGGL(k_eig_trace, dim3(B.n_frames), dim3(256), 0, s, B);
+   return hipGetLastError();
+}
 
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

@@ -31,5 +31,9 @@ hipError_t launch_tr_update_all(const BatchDev& B, int lds_doubles, uint32_t wid
 int tr_update_all_per_cu(int lds_doubles);
 hipError_t configure_kernels(int max_lds_bytes);
 hipError_t launch_reduce_eval(const BatchDev& B, hipStream_t s);
+// The subproblems the update left pending (references with RefDev::eig == 2): qv = Q^T g for
+// all pending frames of a reference at once, the secular iteration per frame, p = -Q w, x + p.
+// n_max: the largest nD n_dim of those references.
+hipError_t launch_eig_subproblems(const BatchDev& B, int n_max, hipStream_t s);
 
 }  // namespace fa
