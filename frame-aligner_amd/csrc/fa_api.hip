@@ -1455,9 +1455,10 @@ const EigLib* eig_lib()
 }
 
 // H = Q diag(ev) Q^T for the batched subproblem kernels (RefDev::eig == 2): the dense matrix
-// from the blocks H holds on the host, rocsolver_dsyevd on the stream. Its column-major output
-// leaves eigenvector i in row i of r->d_Q read row-major; ev ascending. False (and nothing
-// kept) when the solver is absent, fails or does not converge.
+// from the blocks H holds on the host, rocsolver_dsyevd on the stream. r->d_Q gets Q
+// row-major (column i: the eigenvector of ev[i]), then Q^T, which is the solver's column-major
+// output read row-major; ev ascending. False (and nothing kept) when the solver is absent,
+// fails or does not converge.
 bool large_eigen(fa_reference* r, hipStream_t s)
 {
    const EigLib* L = eig_lib();
@@ -1479,15 +1480,17 @@ bool large_eigen(fa_reference* r, hipStream_t s)
    double* d_E = nullptr;
    rocblas_int* d_info = nullptr;
    rocblas_handle h = nullptr;
-   bool ok = dalloc(&d_Q, (size_t)n * n) == FA_OK && dalloc(&d_ev, n) == FA_OK && dalloc(&d_E, n) == FA_OK &&
+   bool ok = dalloc(&d_Q, 2 * (size_t)n * n) == FA_OK && dalloc(&d_ev, n) == FA_OK && dalloc(&d_E, n) == FA_OK &&
              dalloc(&d_info, 1) == FA_OK;
+   double* d_Qt = d_Q ? d_Q + (size_t)n * n : nullptr;
    rocblas_int info = -1;
    double ev0 = 0;
    ok = ok && L->create(&h) == rocblas_status_success;
    ok = ok && L->set_stream(h, s) == rocblas_status_success &&
-        hipMemcpyAsync(d_Q, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess &&
-        L->syevd(h, rocblas_evect_original, rocblas_fill_upper, n, d_Q, n, d_ev, d_E, d_info) ==
+        hipMemcpyAsync(d_Qt, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess &&
+        L->syevd(h, rocblas_evect_original, rocblas_fill_upper, n, d_Qt, n, d_ev, d_E, d_info) ==
             rocblas_status_success &&
+        launch_transpose(d_Qt, d_Q, n, s) == hipSuccess &&
         hipMemcpyAsync(&info, d_info, sizeof info, hipMemcpyDeviceToHost, s) == hipSuccess &&
         hipMemcpyAsync(&ev0, d_ev, sizeof ev0, hipMemcpyDeviceToHost, s) == hipSuccess &&
         hipStreamSynchronize(s) == hipSuccess && info == 0;

@@ -49,7 +49,7 @@ static_assert(kStripWidth == kBlock, "a strip item has one column per fused-pass
 constexpr int kPad = 16;
 constexpr int kMaxRefs = 4096;  // distinct references in one batch
 constexpr int kEigMax = 90;     // largest nD * n_dim whose subproblem uses the eigendecomposition of H
-// largest nD * n_dim whose subproblem runs in the batched eigen-space kernels (Q: 72 MiB)
+// largest nD * n_dim whose subproblem runs in the batched eigen-space kernels (Q, Q^T: 144 MiB)
 constexpr int kEigBatchMax = 3072;
 
 // Per-pixel reference record read by every pass. 2-D: (R, gx, gy, Df) in one float4;
@@ -68,7 +68,7 @@ struct RefDev {
    const double* ev;       // eigenvalues of H
    ScanDev scan;
    int32_t nx, ny, nz, nD, n_dim, n_chunks, h_pd;
-   int32_t eig;            // 0: none; 1: Q, Q^T (n <= 90, solved in the update); 2: Q only (batched kernels)
+   int32_t eig;            // 0: none; 1: Q, Q^T (n <= 90, solved in the update); 2: Q, Q^T (batched kernels)
    int32_t q_lds;          // 1: an update that stages its working set in LDS also stages Q
    double eig_lo;          // Gershgorin lower bound of the spectrum of H
    double ev_min;          // smallest eigenvalue (valid when eig)

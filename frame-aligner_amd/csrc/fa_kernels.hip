@@ -2392,7 +2392,7 @@ __global__ __launch_bounds__(256) void k_tr_update_all(BatchDev B, uint32_t wide
 // displacement x + p (k_eig_gemm<true>, k_eig_apply); k_eig_trace completes the trace
 // records when the batch keeps one. The frame's next update takes phases kPhaseEigW and
 // kPhaseEigZero as 1 (a step was proposed). Here Q is the matrix of eigenvectors (column i for
-// ev[i]); RefDev::Q stores it transposed, eigenvector i in row i.
+// ev[i]); RefDev::Q stores Q row-major, then Q^T (as for the in-update path).
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 // 16-row blocks of the operator; 64 frames per workgroup; k split over kGemmSplit waves of a
@@ -2403,8 +2403,7 @@ constexpr int kGemmRows = 16, kGemmCols = 64, kGemmSplit = 4, kGemmGroups = 4;
 // Partial products over k group blockIdx.z % kGemmGroups for output rows [16 bx, 16 bx + 16)
 // of reference blockIdx.y (when its eig == 2), for the frames among entries
 // [64 (bz / kGemmGroups), +64) of its active list that are in phase kPhaseEigPending
-// (TRANS false: qv = Q^T g, each row of RefDev::Q times g) or kPhaseEigW (TRANS true: Q w,
-// to be negated into p). Partial kg goes to the frame's factor scratch at kg n. FP64
+// (TRANS false: qv = Q^T g) or kPhaseEigW (TRANS true: Q w, to be negated into p). Partial kg goes to the frame's factor scratch at kg n. FP64
 // MFMA 16x16x4 steps: wave w takes columns [16 (w & 3), +16) and the k range w >> 2 of the
 // group's four, whose sums are added in k order. An output element depends only on its row of
 // the operator and its frame's vector, summed in a fixed order, so a frame's result does not
@@ -2447,8 +2446,10 @@ __global__ __launch_bounds__(256 * kGemmSplit) void k_eig_gemm(BatchDev B)
       const int c = min(cb * 16 + li, nc - 1);
       const BatchFrame& bf = B.frames[cols[c]];
       const double* __restrict__ v = TRANS ? B.fac + bf.fac_base + kGemmGroups * n : B.vec + bf.vec_base + n;
-      const double* __restrict__ qa = TRANS ? R.Q + row : R.Q + (int64_t)row * n;
-      const int64_t qs = TRANS ? n : 1;
+      // GEMM1 reads V (V[k n + i] = eigenvector i at k), GEMM2 V^T: either way the lanes of
+      // one k read 16 consecutive rows
+      const double* __restrict__ qa = R.Q + (TRANS ? (int64_t)n * n : 0) + row;
+      const int64_t qs = n;
       constexpr int KS = kGemmSplit * kGemmGroups;
       const int span = ((n + 4 * KS - 1) / (4 * KS)) * 4;
       const int k_lo = min(n, ks * span), k_hi = min(n, k_lo + span);
@@ -2494,28 +2495,25 @@ __global__ __launch_bounds__(256 * kGemmSplit) void k_eig_gemm(BatchDev B)
    }
 }
 
-// p = -(sum of the kGemmGroups partial products of Q^T w, in order) and the trial displacement
-// x + p of a frame in phase kPhaseEigW.
+// p = -(sum of the kGemmGroups partial products of Q w, in order) and the trial displacement
+// x + p of a frame in phase kPhaseEigW: elements [256 bx, +256) of frame blockIdx.y.
 __global__ __launch_bounds__(256) void k_eig_apply(BatchDev B)
 {
-   const int j = blockIdx.x;
-   if (B.state[j].phase != kPhaseEigW) return;
+   const int j = blockIdx.y;
    const BatchFrame& bf = B.frames[j];
    const int n = bf.n, m = bf.r.n_dim;
+   const int i = blockIdx.x * 256 + threadIdx.x;
+   if (i >= n || B.state[j].phase != kPhaseEigW) return;
    const double* P = B.fac + bf.fac_base;
-   const double* x = B.x + bf.x_base;
-   double* p = B.vec + bf.vec_base;
-   double* Dt = B.Dtrial + bf.d_base;
-   for (int i = threadIdx.x; i < n; i += 256) {
-      double s = P[i];
+   double s = P[i];
 #pragma unroll
-      for (int q = 1; q < kGemmGroups; q++) s += P[(int64_t)q * n + i];
-      const double pv = -s;
-      p[i] = pv;
-      const int node = i / m, comp = i - node * m;
-      Dt[3 * node + comp] = x[i] + pv;
-      if (m == 2 && comp == 1) Dt[3 * node + 2] = 0.0;
-   }
+   for (int q = 1; q < kGemmGroups; q++) s += P[(int64_t)q * n + i];
+   const double pv = -s;
+   B.vec[bf.vec_base + i] = pv;
+   const int node = i / m, comp = i - node * m;
+   double* Dt = B.Dtrial + bf.d_base + 3 * node;
+   Dt[comp] = B.x[bf.x_base + i] + pv;
+   if (m == 2 && comp == 1) Dt[2] = 0.0;
 }
 
 // The secular iteration of tr_subproblem_eig for a frame in phase kPhaseEigPending, from
@@ -2539,6 +2537,7 @@ __global__ __launch_bounds__(256) void k_eig_secular(BatchDev B)
    const double* g = B.vec + bf.vec_base + n;
    const double* P = B.fac + bf.fac_base;  // the partial products of qv, in order
    double qr[kEigRegs], er[kEigRegs];
+   double gg = 0;  // g'g in block_dot's per-thread order
 #pragma unroll
    for (int u = 0; u < kEigRegs; u++) {
       const int i = t + NT * u;
@@ -2547,13 +2546,14 @@ __global__ __launch_bounds__(256) void k_eig_secular(BatchDev B)
          q = P[i];
 #pragma unroll
          for (int k = 1; k < kGemmGroups; k++) q += P[(int64_t)k * n + i];
+         gg += g[i] * g[i];
       }
       qr[u] = q;
       er[u] = i < n ? R.ev[i] : 1.0;
    }
    const double radius = st.radius, eps = B.prm.sub_eps;
    const int max_iter = B.prm.sub_max_iter;
-   const double gnorm = sqrt(block_dot<NT>(n, g, g));
+   const double gnorm = sqrt(block_sum<NT>(gg));
    double lam = 0, lam_lo = 0, lam_hi = 0, lam_sol = -1;
    int it = 1;
    if (gnorm != 0) {
@@ -2644,6 +2644,26 @@ __global__ __launch_bounds__(256) void k_eig_trace(BatchDev B)
    for (int i = threadIdx.x; i < n; i += NT) o[kTraceFields + 2 * n + i] = p[i];
 }
 
+// dst = src^T for an n x n row-major matrix, through 32 x 32 tiles in LDS.
+__global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ src, double* __restrict__ dst, int n)
+{
+   __shared__ double tile[32][33];
+   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+   for (int r = ty; r < 32; r += 8)
+      if (by + r < n && bx + tx < n) tile[r][tx] = src[(int64_t)(by + r) * n + bx + tx];
+   __syncthreads();
+   for (int r = ty; r < 32; r += 8)
+      if (bx + r < n && by + tx < n) dst[(int64_t)(bx + r) * n + by + tx] = tile[tx][r];
+}
+
+hipError_t launch_transpose(const double* src, double* dst, int n, hipStream_t s)
+{
+   const dim3 g((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
+   hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, s, src, dst, n);
+   return hipGetLastError();
+}
+
 hipError_t launch_eig_subproblems(const BatchDev& B, int n_max, hipStream_t s)
 {
    const dim3 gg((unsigned)((n_max + kGemmRows - 1) / kGemmRows), (unsigned)B.n_refs,
@@ -2651,7 +2671,7 @@ hipError_t launch_eig_subproblems(const BatchDev& B, int n_max, hipStream_t s)
    hipLaunchKernelGGL(k_eig_gemm<false>, gg, dim3(256 * kGemmSplit), 0, s, B);
    hipLaunchKernelGGL(k_eig_secular, dim3(B.n_frames), dim3(256), 0, s, B);
    hipLaunchKernelGGL(k_eig_gemm<true>, gg, dim3(256 * kGemmSplit), 0, s, B);
-   hipLaunchKernelGGL(k_eig_apply, dim3(B.n_frames), dim3(256), 0, s, B);
+   hipLaunchKernelGGL(k_eig_apply, dim3((unsigned)((n_max + 255) / 256), (unsigned)B.n_frames), dim3(256), 0, s, B);
    if (B.trace) hipLaunchKernelGGL(k_eig_trace, dim3(B.n_frames), dim3(256), 0, s, B);
    return hipGetLastError();
 }

@@ -35,5 +35,7 @@ hipError_t launch_reduce_eval(const BatchDev& B, hipStream_t s);
 // all pending frames of a reference at once, the secular iteration per frame, p = -Q w, x + p.
 // n_max: the largest nD n_dim of those references.
 hipError_t launch_eig_subproblems(const BatchDev& B, int n_max, hipStream_t s);
+// dst = src^T (n x n, row-major, distinct buffers)
+hipError_t launch_transpose(const double* src, double* dst, int n, hipStream_t s);
 
 }  // namespace fa
