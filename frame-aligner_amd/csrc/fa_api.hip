@@ -172,9 +172,10 @@ struct fa_batch {
    // 1: one reference, one update launch: the update's last workgroup compacts the active
    // frames for the next round, so a round is a pass and an update (no schedule launch)
    int fold = 0;
-   // largest nD n_dim of the batch's references whose subproblem runs in the batched
-   // eigen-space kernels (RefDev::eig == 2), 0 when none has one
-   int eig_n = 0;
+   // tables of the batched eigen-space kernels (RefDev::eig == 2): n_eig_tiles product tiles
+   // of 4 ints, then the n_eig_frames frames or slots that may hold such a reference's frame
+   int32_t* d_eig = nullptr;
+   int n_eig_tiles = 0, n_eig_frames = 0;
 
    BatchDev dev() const
    {
@@ -206,7 +207,11 @@ struct fa_batch {
       B.trace_cap = trace_cap;
       B.trace_rec = trace_rec;
       B.fold = fold;
-      B.split_eig = eig_n > 0 ? 1 : 0;
+      B.split_eig = n_eig_tiles > 0 ? 1 : 0;
+      B.n_eig_tiles = n_eig_tiles;
+      B.n_eig_frames = n_eig_frames;
+      B.eig_tiles = d_eig;
+      B.eig_frames = d_eig ? d_eig + 4 * (int64_t)n_eig_tiles : nullptr;
       return B;
    }
 
@@ -258,6 +263,7 @@ struct fa_batch {
       dfree(d_warp);
       dfree(d_cov);
       dfree(d_trace);
+      dfree(d_eig);
       if (h_counters) (void)hipHostFree(h_counters);
       if (stream && own_stream) (void)hipStreamDestroy(stream);
    }
@@ -441,6 +447,31 @@ void set_grids(fa_batch* b)
 
 // stream: a stream of the context to run on (the parts of a split batch), or null for a
 // stream of the batch's own.
+// Tables of the batched eigen-space kernels: the product tiles of each reference with
+// RefDev::eig == 2 for frames_of[r] frames of it at most, and the frames (or slots) that may
+// hold such a reference's frame; uploaded on the batch's stream. Nothing without such a
+// reference.
+int set_eig_tables(fa_batch* b, const std::vector<RefDev>& refs, const std::vector<int>& frames_of,
+                   const std::vector<int32_t>& eig_frames)
+{
+   std::vector<int32_t> t;
+   for (size_t r = 0; r < refs.size(); r++)
+      if (refs[r].eig == 2 && frames_of[r] > 0) eig_tiles_of((int)r, refs[r].nD * refs[r].n_dim, frames_of[r], t);
+   if (t.empty() || eig_frames.empty()) return FA_OK;
+   const int tiles = (int)(t.size() / 4);
+   t.insert(t.end(), eig_frames.begin(), eig_frames.end());
+   int rc = dalloc(&b->d_eig, t.size());
+   if (rc) return rc;
+   if (hipMemcpyAsync(b->d_eig, t.data(), t.size() * sizeof(int32_t), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+       hipStreamSynchronize(b->stream) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_err(FA_E_DEVICE, "batch upload failed");
+   }
+   b->n_eig_tiles = tiles;
+   b->n_eig_frames = (int)eig_frames.size();
+   return FA_OK;
+}
+
 int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out,
                 hipStream_t stream = nullptr)
 {
@@ -557,8 +588,15 @@ int batch_build(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_
    }
    set_grids(b);
    b->fold = (rs.size() == 1 && b->trc_lds.size() == 1) ? 1 : 0;
-   for (const RefDev& rd : refs)
-      if (rd.eig == 2) b->eig_n = std::max(b->eig_n, rd.nD * rd.n_dim);
+   {
+      std::vector<int> frames_of(refs.size(), 0);
+      std::vector<int32_t> eig_frames;
+      for (int i = 0; i < n; i++) {
+         frames_of[b->hframes[i].ref]++;
+         if (refs[b->hframes[i].ref].eig == 2) eig_frames.push_back(i);
+      }
+      if ((rc = set_eig_tables(b, refs, frames_of, eig_frames))) return fail(rc);
+   }
    *out = b;
    return FA_OK;
 }
@@ -600,13 +638,13 @@ int launch_updates(fa_batch* b, hipStream_t s)
       const int lds = b->tune.tr_lds ? most : 0;
       if ((fit || !b->tune.tr_lds) && (int64_t)B.n_frames <= (int64_t)b->ctx->n_cu * tr_update_all_per_cu(lds)) {
          HIPCHK(launch_tr_update_all(B, lds, wide, lds ? full : 0u, s));
-         if (b->eig_n) HIPCHK(launch_eig_subproblems(B, b->eig_n, s));
+         HIPCHK(launch_eig_subproblems(B, s));
          return FA_OK;
       }
    }
    for (size_t c = 0; c < nc; c++)
       HIPCHK(launch_tr_update(B, (int)c, b->tune.tr_lds ? b->trc_lds[c] : 0, b->trc_full[c], b->trc_threads[c], s));
-   if (b->eig_n) HIPCHK(launch_eig_subproblems(B, b->eig_n, s));
+   HIPCHK(launch_eig_subproblems(B, s));
    return FA_OK;
 }
 
@@ -1204,8 +1242,12 @@ int slot_batch_build(fa_context* ctx, const std::vector<fa_reference*>& refs, in
       return fail(set_err(FA_E_DEVICE, "server upload failed"));
    }
    set_grids(b);
-   for (const RefDev& rd : rdev)
-      if (rd.eig == 2) b->eig_n = std::max(b->eig_n, rd.nD * rd.n_dim);
+   {
+      // any slot may hold a frame of any of the lane's references
+      std::vector<int32_t> slots(S);
+      for (int j = 0; j < S; j++) slots[j] = j;
+      if ((rc = set_eig_tables(b, rdev, std::vector<int>(rdev.size(), S), slots))) return fail(rc);
+   }
    *out = b;
    return FA_OK;
 }
@@ -1428,8 +1470,8 @@ int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offse
 namespace {
 
 // rocSOLVER's symmetric divide-and-conquer eigensolver, opened on first use: only references
-// whose nD n_dim lies in (kEigMax, kEigBatchMax] need it, and the library is large. Without
-// it those references keep the cyclic-reduction subproblem.
+// whose nD n_dim lies in [kEigBatchMin, kEigBatchMax] need it, and the library is large.
+// Without it those references keep the cyclic-reduction subproblem.
 struct EigLib {
    decltype(&rocblas_create_handle) create = nullptr;
    decltype(&rocblas_destroy_handle) destroy = nullptr;
@@ -1633,6 +1675,8 @@ int align_noalloc(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_alig
    B.n_refs = 1;
    B.trace = nullptr;
    B.split_eig = r->dev.eig == 2 ? 1 : 0;
+   B.n_eig_tiles = B.split_eig ? (n + 15) / 16 * 4 : 0;  // the kernels' tiles of one frame (no table)
+   B.n_eig_frames = B.split_eig;
    // the launch shape and update variant a one-frame batch of this reference takes
    const int lds_floats = (int)std::min<int64_t>(lds_floats_for(r), kLdsCapFloats);
    PassArgs pa{};
@@ -1673,7 +1717,7 @@ int align_noalloc(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_alig
          HIPCHK(launch_schedule(B, 1, s));
          HIPCHK(launch_pass(pa, r->n_dim, grid, s));
          HIPCHK(launch_tr_update(B, 0, tr_lds, tr_full, tr_nt, s));
-         if (B.split_eig) HIPCHK(launch_eig_subproblems(B, n, s));
+         HIPCHK(launch_eig_subproblems(B, s));
       }
       HIPCHK(launch_schedule(B, 0, s));
       HIPCHK(hipMemcpyAsync(&hc, d_counters, sizeof hc, hipMemcpyDeviceToHost, s));
@@ -1982,7 +2026,7 @@ int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, 
       R.eig = info[0] != 0.0 ? 1 : 0;
       R.ev_min = info[1];
       R.q_lds = R.eig && tr_lds_full_base(r) + tr_lds_q(r) <= kTrFullDoubles ? 1 : 0;
-   } else if (nn > kEigMax && nn <= kEigBatchMax) {
+   } else if (nn >= kEigBatchMin && nn <= kEigBatchMax) {
       large_eigen(r, s);  // else the cyclic-reduction subproblem
    }
    cleanup();

@@ -49,8 +49,10 @@ static_assert(kStripWidth == kBlock, "a strip item has one column per fused-pass
 constexpr int kPad = 16;
 constexpr int kMaxRefs = 4096;  // distinct references in one batch
 constexpr int kEigMax = 90;     // largest nD * n_dim whose subproblem uses the eigendecomposition of H
-// largest nD * n_dim whose subproblem runs in the batched eigen-space kernels (Q, Q^T: 144 MiB)
-constexpr int kEigBatchMax = 3072;
+// nD * n_dim range whose subproblem runs in the batched eigen-space kernels (Q, Q^T: up to
+// 144 MiB). They cost four short launches per round, which the cyclic reduction's update beats
+// on shorter chains (serving references of 231 to 843 parameters ran slower, C3's 1539 faster).
+constexpr int kEigBatchMin = 1000, kEigBatchMax = 3072;
 
 // Per-pixel reference record read by every pass. 2-D: (R, gx, gy, Df) in one float4;
 // 3-D: (R, gx, gy, gz) plus Df in a separate float array. Df is the node fraction of
@@ -180,6 +182,13 @@ struct BatchDev {
    // subproblem to the batched eigen-space kernels launched after the update (k_eig_gemm,
    // k_eig_secular); 0: they solve it by cyclic reduction in the update
    int32_t split_eig;
+   // The launches of those kernels: eig_tiles holds n_eig_tiles product tiles of 4 ints
+   // (reference, first row, first entry of its active list, k group), eig_frames the
+   // n_eig_frames frames (or slots) that may hold such a reference's frame; null tables: the
+   // tiles of reference 0 for one frame, and frames 0 .. n_eig_frames - 1.
+   int32_t n_eig_tiles, n_eig_frames;
+   const int32_t* eig_tiles;
+   const int32_t* eig_frames;
 };
 
 // Trust-region states of a frame whose subproblem runs in the batched eigen-space kernels:

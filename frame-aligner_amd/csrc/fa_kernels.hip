@@ -2400,9 +2400,10 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // partial products (n each), then w (n): 5 n doubles, within its 3 nD n_dim^2 (>= 6 n).
 constexpr int kGemmRows = 16, kGemmCols = 64, kGemmSplit = 4, kGemmGroups = 4;
 
-// Partial products over k group blockIdx.z % kGemmGroups for output rows [16 bx, 16 bx + 16)
-// of reference blockIdx.y (when its eig == 2), for the frames among entries
-// [64 (bz / kGemmGroups), +64) of its active list that are in phase kPhaseEigPending
+// Partial products of tile blockIdx.x of B.eig_tiles (without a table: the tiles of reference
+// 0 for one frame, in eig_tiles_of's order): over k group kg, for output rows
+// [row0, row0 + 16) of reference r (eig == 2), for the frames among entries [base, base + 64)
+// of its active list that are in phase kPhaseEigPending
 // (TRANS false: qv = Q^T g) or kPhaseEigW (TRANS true: Q w, to be negated into p). Partial kg goes to the frame's factor scratch at kg n. FP64
 // MFMA 16x16x4 steps: wave w takes columns [16 (w & 3), +16) and the k range w >> 2 of the
 // group's four, whose sums are added in k order. An output element depends only on its row of
@@ -2412,15 +2413,18 @@ constexpr int kGemmRows = 16, kGemmCols = 64, kGemmSplit = 4, kGemmGroups = 4;
 template <bool TRANS>
 __global__ __launch_bounds__(256 * kGemmSplit) void k_eig_gemm(BatchDev B)
 {
-   const int r = blockIdx.y;
+   int r = 0, row0 = (blockIdx.x / kGemmGroups) * kGemmRows, base = 0, kg = blockIdx.x % kGemmGroups;
+   if (B.eig_tiles) {
+      const int32_t* tl = B.eig_tiles + 4 * (int64_t)blockIdx.x;
+      r = tl[0];
+      row0 = tl[1];
+      base = tl[2];
+      kg = tl[3];
+   }
    const RefDev& R = B.refs[r];
-   if (R.eig != 2) return;
    const int n = R.nD * R.n_dim;
-   const int row0 = blockIdx.x * kGemmRows;
-   const int kg = blockIdx.z % kGemmGroups;
    const int first = B.run_first[r], count = B.run_count[r];
-   const int base = (blockIdx.z / kGemmGroups) * kGemmCols;
-   if (row0 >= n || base >= count) return;
+   if (base >= count) return;
    __shared__ int cols[kGemmCols];
    __shared__ int s_nc;
    __shared__ dbl4 part[kGemmSplit - 1][4][64];
@@ -2496,24 +2500,25 @@ __global__ __launch_bounds__(256 * kGemmSplit) void k_eig_gemm(BatchDev B)
 }
 
 // p = -(sum of the kGemmGroups partial products of Q w, in order) and the trial displacement
-// x + p of a frame in phase kPhaseEigW: elements [256 bx, +256) of frame blockIdx.y.
-__global__ __launch_bounds__(256) void k_eig_apply(BatchDev B)
+// x + p of frame eig_frames[blockIdx.x] when it is in phase kPhaseEigW.
+__global__ __launch_bounds__(1024) void k_eig_apply(BatchDev B)
 {
-   const int j = blockIdx.y;
+   const int j = B.eig_frames ? B.eig_frames[blockIdx.x] : blockIdx.x;
+   if (B.state[j].phase != kPhaseEigW) return;
    const BatchFrame& bf = B.frames[j];
    const int n = bf.n, m = bf.r.n_dim;
-   const int i = blockIdx.x * 256 + threadIdx.x;
-   if (i >= n || B.state[j].phase != kPhaseEigW) return;
    const double* P = B.fac + bf.fac_base;
-   double s = P[i];
+   for (int i = threadIdx.x; i < n; i += 1024) {
+      double s = P[i];
 #pragma unroll
-   for (int q = 1; q < kGemmGroups; q++) s += P[(int64_t)q * n + i];
-   const double pv = -s;
-   B.vec[bf.vec_base + i] = pv;
-   const int node = i / m, comp = i - node * m;
-   double* Dt = B.Dtrial + bf.d_base + 3 * node;
-   Dt[comp] = B.x[bf.x_base + i] + pv;
-   if (m == 2 && comp == 1) Dt[2] = 0.0;
+      for (int q = 1; q < kGemmGroups; q++) s += P[(int64_t)q * n + i];
+      const double pv = -s;
+      B.vec[bf.vec_base + i] = pv;
+      const int node = i / m, comp = i - node * m;
+      double* Dt = B.Dtrial + bf.d_base + 3 * node;
+      Dt[comp] = B.x[bf.x_base + i] + pv;
+      if (m == 2 && comp == 1) Dt[2] = 0.0;
+   }
 }
 
 // The secular iteration of tr_subproblem_eig for a frame in phase kPhaseEigPending, from
@@ -2528,7 +2533,7 @@ constexpr int kEigRegs = (kEigBatchMax + 255) / 256;
 __global__ __launch_bounds__(256) void k_eig_secular(BatchDev B)
 {
    constexpr int NT = 256;
-   const int j = blockIdx.x;
+   const int j = B.eig_frames ? B.eig_frames[blockIdx.x] : blockIdx.x;
    TrState st = B.state[j];
    if (st.phase != kPhaseEigPending) return;
    const BatchFrame& bf = B.frames[j];
@@ -2618,13 +2623,13 @@ __global__ __launch_bounds__(256) void k_eig_secular(BatchDev B)
    if (t == 0) B.state[j] = st;
 }
 
-// Trace records of the frames whose subproblem the batched kernels solved in this round
+// Trace records of the frames (eig_frames) whose subproblem the batched kernels solved in this round
 // (phase kPhaseEigW or kPhaseEigZero): lambda, iterations, |p|, g'p and p (the update wrote the
 // rest of the record). Launched only when the batch records a trace.
 __global__ __launch_bounds__(256) void k_eig_trace(BatchDev B)
 {
    constexpr int NT = 256;
-   const int j = blockIdx.x;
+   const int j = B.eig_frames ? B.eig_frames[blockIdx.x] : blockIdx.x;
    const TrState st = B.state[j];
    if (st.phase != kPhaseEigW && st.phase != kPhaseEigZero) return;
    if (st.evaluations - 1 >= B.trace_cap) return;
@@ -2664,15 +2669,26 @@ hipError_t launch_transpose(const double* src, double* dst, int n, hipStream_t s
    return hipGetLastError();
 }
 
-hipError_t launch_eig_subproblems(const BatchDev& B, int n_max, hipStream_t s)
+void eig_tiles_of(int ref, int n, int frames, std::vector<int32_t>& tiles)
 {
-   const dim3 gg((unsigned)((n_max + kGemmRows - 1) / kGemmRows), (unsigned)B.n_refs,
-                 (unsigned)((B.n_frames + kGemmCols - 1) / kGemmCols * kGemmGroups));
-   hipLaunchKernelGGL(k_eig_gemm<false>, gg, dim3(256 * kGemmSplit), 0, s, B);
-   hipLaunchKernelGGL(k_eig_secular, dim3(B.n_frames), dim3(256), 0, s, B);
-   hipLaunchKernelGGL(k_eig_gemm<true>, gg, dim3(256 * kGemmSplit), 0, s, B);
-   hipLaunchKernelGGL(k_eig_apply, dim3((unsigned)((n_max + 255) / 256), (unsigned)B.n_frames), dim3(256), 0, s, B);
-   if (B.trace) hipLaunchKernelGGL(k_eig_trace, dim3(B.n_frames), dim3(256), 0, s, B);
+   for (int base = 0; base < frames; base += kGemmCols)
+      for (int row0 = 0; row0 < n; row0 += kGemmRows)
+         for (int kg = 0; kg < kGemmGroups; kg++) {
+            tiles.push_back(ref);
+            tiles.push_back(row0);
+            tiles.push_back(base);
+            tiles.push_back(kg);
+         }
+}
+
+hipError_t launch_eig_subproblems(const BatchDev& B, hipStream_t s)
+{
+   if (B.n_eig_tiles <= 0 || B.n_eig_frames <= 0) return hipSuccess;
+   hipLaunchKernelGGL(k_eig_gemm<false>, dim3(B.n_eig_tiles), dim3(256 * kGemmSplit), 0, s, B);
+   hipLaunchKernelGGL(k_eig_secular, dim3(B.n_eig_frames), dim3(256), 0, s, B);
+   hipLaunchKernelGGL(k_eig_gemm<true>, dim3(B.n_eig_tiles), dim3(256 * kGemmSplit), 0, s, B);
+   hipLaunchKernelGGL(k_eig_apply, dim3(B.n_eig_frames), dim3(1024), 0, s, B);
+   if (B.trace) hipLaunchKernelGGL(k_eig_trace, dim3(B.n_eig_frames), dim3(256), 0, s, B);
    return hipGetLastError();
 }
 

@@ -32,9 +32,12 @@ int tr_update_all_per_cu(int lds_doubles);
 hipError_t configure_kernels(int max_lds_bytes);
 hipError_t launch_reduce_eval(const BatchDev& B, hipStream_t s);
 // The subproblems the update left pending (references with RefDev::eig == 2): qv = Q^T g for
-// all pending frames of a reference at once, the secular iteration per frame, p = -Q w, x + p.
-// n_max: the largest nD n_dim of those references.
-hipError_t launch_eig_subproblems(const BatchDev& B, int n_max, hipStream_t s);
+// all pending frames of a reference at once, the secular iteration per frame, p = -Q w, x + p,
+// over B.eig_tiles and B.eig_frames (nothing when they are empty).
+hipError_t launch_eig_subproblems(const BatchDev& B, hipStream_t s);
+// Appends the product tiles of reference index ref (n = nD n_dim parameters) for up to
+// `frames` active frames to tiles (4 ints each: ref, first row, first list entry, k group).
+void eig_tiles_of(int ref, int n, int frames, std::vector<int32_t>& tiles);
 // dst = src^T (n x n, row-major, distinct buffers)
 hipError_t launch_transpose(const double* src, double* dst, int n, hipStream_t s);
 

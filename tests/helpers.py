@@ -39,6 +39,11 @@ MANY_NODE_CASES = [
      dict(n_blobs=200, sigma=(1.5, 4.0), walk_sigma=0.05, sin_amp=1.5, z_amp=0.5)),
 ]
 
+# A chain of 1539 parameters (3-D, 513 nodes: 64 per plane), long enough for the batched
+# eigen-space subproblem kernels.
+LONG_CHAIN_CASE = ("3d_long_chain", (8, 64, 64), 8 * 64 + 1, dict(line=1.0, interline=1.0),
+                   dict(n_blobs=300, sigma=(1.5, 4.0), walk_sigma=0.05, sin_amp=1.5, z_amp=0.5))
+
 
 def truth_start(st, f, n_dim):
     """Parameter vector (D2col layout) of frame f's generator displacement."""

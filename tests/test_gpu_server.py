@@ -147,6 +147,39 @@ def test_server_lanes_same_alone(ctx):
         f.release()
 
 
+def test_server_long_chain_same_alone(ctx):
+    """A 3-D reference of 513 nodes (1539 parameters: its subproblems run in the batched
+    eigen-space kernels) shares a lane with a 19-node one; frames arrive while others are in
+    flight, and every result is the frame's alone."""
+    from helpers import LONG_CHAIN_CASE
+    _, shape, nD, scan, gen = LONG_CHAIN_CASE
+    c = small_stack((6, 40, 36), n_frames=2, nD=19, seed=71, n_blobs=120, sigma=(1.5, 4.0))
+    d = small_stack(shape, n_frames=3, nD=nD, seed=72, **gen)
+    refs = [F.Reference(ctx, c["ref_w"], 19), F.Reference(ctx, d["ref_w"], nD, **scan)]
+    frames = [refs[0].register(f) for f in c["frames_w"]] + [refs[1].register(f) for f in d["frames_w"]]
+    alone = [f.align(**KW) for f in frames]
+    srv = F.Server(refs, slots=4, **KW)
+    order = [2, 0, 3, 1, 4]
+    got = {}
+    k = steps = 0
+    while len(got) < len(frames):
+        if k < len(order):
+            srv.submit(frames[order[k]], tag=order[k])
+            k += 1
+        _, _, res = srv.step(rounds=2)
+        steps += 1
+        for tag, x, st in res:
+            got[tag] = (x, st)
+        assert steps < 20000
+    for i, (x1, s1) in enumerate(alone):
+        x2, s2 = got[i]
+        assert np.array_equal(x1, x2), i
+        assert (s1["evaluations"], s1["status"], s1["f_final"]) == (s2["evaluations"], s2["status"], s2["f_final"]), i
+    srv.close()
+    for f in frames:
+        f.release()
+
+
 def test_server_errors(ctx):
     st = small_stack((1, 64, 64), n_frames=1, nD=9, seed=44)
     r1 = F.Reference(ctx, st["ref_w"], 9)

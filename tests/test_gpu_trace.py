@@ -9,7 +9,7 @@ import pytest
 
 import oracle as O
 import fa_amd as F
-from helpers import MANY_NODE_CASES, PARITY_CASES, small_stack
+from helpers import LONG_CHAIN_CASE, MANY_NODE_CASES, PARITY_CASES, small_stack
 
 pytestmark = pytest.mark.gpu
 
@@ -60,10 +60,7 @@ def test_trace_matches_oracle(ctx, name, shape, nD, scan, gen):
     ref.release()
 
 
-LONG_CHAINS = [c for c in MANY_NODE_CASES] + [
-    ("3d_long_chain", (8, 64, 64), 8 * 64 + 1, dict(line=1.0, interline=1.0),
-     dict(n_blobs=300, sigma=(1.5, 4.0), walk_sigma=0.05, sin_amp=1.5, z_amp=0.5)),
-]
+LONG_CHAINS = [c for c in MANY_NODE_CASES] + [LONG_CHAIN_CASE]
 
 
 def banded_solve(H, m, rhs):
@@ -84,8 +81,9 @@ def banded_solve(H, m, rhs):
 
 @pytest.mark.parametrize("name,shape,nD,scan,gen", LONG_CHAINS, ids=[c[0] for c in LONG_CHAINS])
 def test_long_chain_steps_solve_the_subproblem(ctx, name, shape, nD, scan, gen):
-    """Chains of 300 to 1539 parameters (2-D and 3-D), whose subproblems the batched
-    eigen-space kernels solve: every traced step p with its lambda is the solution of
+    """Chains of 300 and 483 parameters (2-D and 3-D, subproblems by cyclic reduction) and 1539
+    (3-D, subproblems in the batched eigen-space kernels): every traced step p with its lambda
+    is the solution of
     (H + lambda I) p = -g (a banded FP64 solve of the reference's H, within 1e-9 relative), with
     H positive definite and |p| <= 1.1 radius at lambda = 0, H + lambda I positive definite and
     |p| within 0.1 radius of the radius at lambda > 0 (the subproblem's stop rule, boundary

@@ -42,6 +42,21 @@ def test_noalloc_matches_align_larger_then_smaller(ctx):
         ref.release()
 
 
+def test_noalloc_long_chain(ctx):
+    """A 1539-parameter chain, whose subproblems run in the batched eigen-space kernels (their
+    scratch is the frame's factor scratch in the array): the same result as fa_align."""
+    from helpers import LONG_CHAIN_CASE
+    ref, frames = _frames(ctx, LONG_CHAIN_CASE)
+    for fr in frames:
+        x1, s1 = fr.align(**KW)
+        x2, s2 = fr.align_noalloc(**KW)
+        assert np.array_equal(x1, x2)
+        assert (s1["evaluations"], s1["status"], s1["f_final"]) == (s2["evaluations"], s2["status"], s2["f_final"])
+    for fr in frames:
+        fr.release()
+    ref.release()
+
+
 def test_noalloc_reference_stop_rule_and_start(ctx):
     ref, frames = _frames(ctx, [c for c in PARITY_CASES if c[0] == "2d_interline"][0])
     x0 = np.random.default_rng(3).normal(0, 0.3, ref.nD * ref.n_dim)
