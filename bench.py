@@ -287,7 +287,7 @@ RECORD_EXTRA_BYTES_PER_PX = 4
 
 
 def roofline_of(tm, kernel, traffic_file=None):
-    """Pass roofline from HIP events around every fused pass (eager runs; a batch split into two
+    """Pass roofline from HIP events around every fused pass (eager runs; a batch split into
     parts runs them one after the other in those runs, so each pass is timed alone). achieved =
     algorithmic bytes (per active pixel: frame sample, reference intensity, n_dim gradient
     components) / summed pass time; the bytes the pass actually requests (plus the node
@@ -620,8 +620,15 @@ def headline(F, args, ctx):
                                 "k_pass_traffic_512x512_200f_nD31.txt"),
     }
     out["roofline"]["measured"] = (f"HIP events around every fused pass of {args.steps} more steps of the "
-                                   f"same workload (eager launches; the batch's two parts run one after the "
+                                   f"same workload (eager launches; the batch's parts run one after the "
                                    f"other, so each pass is timed alone)")
+    # the same bytes over the step time of `value`, where the parts' rounds interleave (one
+    # part's pass fills the GPU while another's tail drains): a whole-step rate that includes
+    # the trust-region updates and host polls
+    step_gbs = tsum["pass_bytes"] / args.steps / (ms_per_step * 1e-3) / 1e9
+    out["roofline"]["per_step"] = {"achieved": round(step_gbs, 1), "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                                   "what": "algorithmic pass bytes of a step over ms_per_step (parts interleaved; "
+                                           "updates and host polls included)"}
     if iso_ms:
         out["roofline"]["isolated"] = {"avg_launch_us": round(1e3 * iso_ms, 2), "bytes_per_launch": iso_bytes,
                                        "achieved": round(iso_bytes / (iso_ms * 1e-3) / 1e9, 1),

@@ -666,7 +666,10 @@ constexpr int kServerParts = 2;        // slot batches (at least 8 slots each)
 constexpr int kMaxLanes = fa_context::kLaneStreams;  // server lanes (each on a lane stream of the context)
 constexpr int64_t kLaneSpan = 16;      // a lane's references span at most this ratio of pixels
 constexpr size_t kArenaMargin = (size_t)32 << 20;  // device memory a default arena leaves free
-constexpr int kSplitFrames = 64;  // batches of at least this many frames run as two parts
+// Batches of at least kSplitFrames frames run as two parts, of at least kSplit3Frames as three
+// (one per lane stream of the context): 200 frames of 512^2 run 7 % faster in three parts than
+// in two, a fourth part (sharing a hardware queue) 28 % slower.
+constexpr int kSplitFrames = 64, kSplit3Frames = 144;
 
 int64_t max_rounds(const fa_batch* b) { return b->prm.max_evals > 0 ? (int64_t)b->prm.max_evals + 2 : 100000; }
 
@@ -915,37 +918,46 @@ int batch_eval(fa_batch* b, const double* D, double* err, double* jac)
    return FA_OK;
 }
 
-// A batch for the caller: a leaf, or two leaves over the two halves of the frames (balanced
-// by pixel count) when it is large enough to overlap their rounds.
+// A batch for the caller: a leaf, or two or three leaves over consecutive runs of the frames
+// (balanced by pixel count) when it is large enough to overlap their rounds. Each part runs on
+// its own lane stream of the context, which has a hardware queue of its own, so the parts'
+// rounds interleave on the device; a frame's result does not depend on the partition.
 int batch_create_split(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out)
 {
    if (n < kSplitFrames) return batch_build(frames, n, prm, out);
+   static_assert(fa_context::kLaneStreams >= 3, "a part per lane stream");
+   const int P = n >= kSplit3Frames ? 3 : 2;
    int64_t total = 0;
    for (int i = 0; i < n; i++) total += frames[i]->ref->n_px;
-   int cut = 0;
-   for (int64_t acc = 0; cut < n - 1 && 2 * (acc + frames[cut]->ref->n_px) <= total; cut++)
-      acc += frames[cut]->ref->n_px;
-   cut = std::max(1, cut);
-   fa_batch* a = nullptr;
-   fa_batch* c = nullptr;
-   // the parts run on two lane streams of the context, which have hardware queues of their
-   // own, so their rounds interleave on the device
-   fa_context* ctx = frames[0]->ref->ctx;
-   CHK(batch_build(frames, cut, prm, &a, ctx->lane_stream[0]));
-   int rc = batch_build(frames + cut, n - cut, prm, &c, ctx->lane_stream[1]);
-   if (rc != FA_OK) {
-      delete a;
-      return rc;
+   // part p ends after the frames whose pixels reach p / P of the total (at least one frame
+   // per part)
+   std::vector<int> cut = {0};
+   int64_t acc = 0;
+   int i = 0;
+   for (int p = 1; p < P; p++) {
+      while (i < n - (P - p) && P * (acc + frames[i]->ref->n_px) <= p * total) acc += frames[i++]->ref->n_px;
+      if (i <= cut.back()) acc += frames[i++]->ref->n_px;
+      cut.push_back(i);
    }
+   cut.push_back(n);
+   fa_context* ctx = frames[0]->ref->ctx;
    auto* b = new fa_batch();
-   b->ctx = a->ctx;
+   b->ctx = ctx;
    b->frames.assign(frames, frames + n);
-   b->prm = a->prm;
-   b->tune = a->tune;
-   b->parts = {a, c};
-   b->part_first = {0, cut};
-   b->total_x = a->total_x + c->total_x;
-   b->total_D = a->total_D + c->total_D;
+   for (int p = 0; p < P; p++) {
+      fa_batch* q = nullptr;
+      const int rc = batch_build(frames + cut[p], cut[p + 1] - cut[p], prm, &q, ctx->lane_stream[p]);
+      if (rc != FA_OK) {
+         delete b;
+         return rc;
+      }
+      b->parts.push_back(q);
+      b->part_first.push_back(cut[p]);
+      b->total_x += q->total_x;
+      b->total_D += q->total_D;
+   }
+   b->prm = b->parts[0]->prm;
+   b->tune = b->parts[0]->tune;
    *out = b;
    return FA_OK;
 }

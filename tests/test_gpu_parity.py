@@ -306,11 +306,12 @@ def test_errors_are_reported(ctx):
     assert e == 0.0 and not j.any()
 
 
-def test_split_batch_same_as_alone(ctx):
-    """A batch of >= 64 frames runs as two parts on two streams with interleaved rounds; every
-    frame's result is bit-identical to aligning it alone."""
-    a, _, ref_a = _pair(ctx, (1, 48, 40), 7, dict(line=1.0, interline=1.0), seed=41, n_frames=36)
-    b, _, ref_b = _pair(ctx, (1, 40, 56), 9, dict(line=1.0, interline=1.0), seed=42, n_frames=36,
+@pytest.mark.parametrize("per_ref", [36, 75])
+def test_split_batch_same_as_alone(ctx, per_ref):
+    """A batch of >= 64 frames runs as two parts (>= 144 frames: three) on their own streams
+    with interleaved rounds; every frame's result is bit-identical to aligning it alone."""
+    a, _, ref_a = _pair(ctx, (1, 48, 40), 7, dict(line=1.0, interline=1.0), seed=41, n_frames=per_ref)
+    b, _, ref_b = _pair(ctx, (1, 40, 56), 9, dict(line=1.0, interline=1.0), seed=42, n_frames=per_ref,
                         dtype="uint16")
     kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
     frames = [ref_a.register(f) for f in a["frames_w"]] + [ref_b.register(f) for f in b["frames"]]
@@ -318,7 +319,8 @@ def test_split_batch_same_as_alone(ctx):
     batch.set_timing(True)
     xs, stats, timing = batch.run()
     assert timing["pass_launches"] > 0 and timing["pass_bytes"] > 0
-    for i in (0, 17, 35, 36, 50, 71):
+    n = len(frames)
+    for i in sorted({0, n // 3 - 1, n // 3, n // 2 - 1, n // 2, 2 * n // 3, n - 1}):
         x1, s1 = frames[i].align(**kw)
         assert np.array_equal(x1, xs[i]), i
         assert s1["evaluations"] == stats[i]["evaluations"]
