@@ -57,10 +57,11 @@ def time_run(batch, reps=3, x0=None):
 
 
 def tune_batch(batch, reps=3, sweeps=2, space=None, log=None, x0=None, gain=0.02):
-    """Coordinate descent over fa_tuning; a candidate replaces the current best when it is
-    faster by more than `gain` (relative), so timing noise does not move the shape. Leaves the
-    batch at the best shape and returns (best tuning dict, best seconds per run, list of
-    (tuning, seconds) trials)."""
+    """Coordinate descent over fa_tuning: per knob, every candidate value is timed from the
+    current best, and the fastest (the first in the knob's order on a tie) replaces the current
+    best when it is faster by more than `gain` (relative), so timing noise does not move the
+    shape. Leaves the batch at the best shape and returns (best tuning dict, best seconds per
+    run, list of (tuning, seconds) trials)."""
     space = space or SPACE
     was_timing = getattr(batch, "_timing", False)
     batch.set_timing(False)
@@ -71,6 +72,7 @@ def tune_batch(batch, reps=3, sweeps=2, space=None, log=None, x0=None, gain=0.02
     for _ in range(sweeps):
         improved = False
         for knob, values in space.items():
+            top, top_t = None, None
             for v in values:
                 if best.get(knob) == v:
                     continue
@@ -80,8 +82,10 @@ def tune_batch(batch, reps=3, sweeps=2, space=None, log=None, x0=None, gain=0.02
                 trials.append((dict(cand), t))
                 if log:
                     log(f"tune {knob}={v}: {t * 1e3:.3f} ms (best {best_t * 1e3:.3f})")
-                if t < best_t * (1.0 - gain):
-                    best, best_t, improved = cand, t, True
+                if top_t is None or t < top_t:
+                    top, top_t = cand, t
+            if top is not None and top_t < best_t * (1.0 - gain):
+                best, best_t, improved = top, top_t, True
             batch.set_tuning(**best)
         if not improved:
             break

@@ -50,10 +50,19 @@ def test_gain_threshold_keeps_current_shape(fake_time):
     b = FakeBatch(lambda t: 1.0 if t == DEFAULT else 0.99)
     best, _, _ = tune.tune_batch(b, reps=1, sweeps=1)
     assert best == DEFAULT
-    # faster by 3 %: the first such candidate is taken
+    # faster by 3 %: of equally fast candidates the first in the knob's order is taken
     b = FakeBatch(lambda t: 1.0 if t["blocks_per_cu"] == 64 else 0.97)
     best, _, _ = tune.tune_batch(b, reps=1, sweeps=1)
     assert best["blocks_per_cu"] == tune.SPACE["blocks_per_cu"][0]
+
+
+def test_fastest_value_of_a_knob_wins(fake_time):
+    # 8 is 3 % faster than the default, 16 faster still but by less than the gain over 8: a
+    # greedy walk would stop at 8; the fastest value of the knob is taken
+    times = {8: 0.97, 16: 0.955, 32: 0.99, 64: 1.0}
+    b = FakeBatch(lambda t: times[t["blocks_per_cu"]])
+    best, best_t, _ = tune.tune_batch(b, reps=1, sweeps=1)
+    assert best["blocks_per_cu"] == 16 and best_t == pytest.approx(0.955)
 
 
 def test_timing_flag_restored(fake_time):
