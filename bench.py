@@ -161,6 +161,7 @@ def serving(F, args, ctx):
     arrival -> result. A sample of served results is checked against the same frame aligned
     alone (bit-identical) and, for the 256^2 2-D acquisition, against the CPU oracle."""
     from fa_amd import serve
+    native = args.serve_loop == "native"
     t0 = time.time()
     acqs = serve.make_acquisitions(seed=args.seed, pool=2)
     log(f"serving: generated {len(acqs)} acquisitions in {time.time() - t0:.1f}s")
@@ -170,20 +171,21 @@ def serving(F, args, ctx):
     srv = F.Server(refs, slots=args.serve_slots, **tr)
     loop = serve.ServeLoop(srv, refs, pools, rounds=args.serve_rounds)
     times, acq = serve.arrivals(len(acqs), args.serve_rate, args.serve_seconds, seed=args.seed + 1)
-    loop.run(times[:200], acq[:200])  # warm-up
+    loop.run(times[:200], acq[:200], native=native)  # warm-up
     keep = set(range(0, len(times), max(1, len(times) // 64)))
-    res = loop.run(times, acq, keep=keep)
+    res = loop.run(times, acq, keep=keep, native=native)
     out = serve.summarize(res, times, acq, pools)
     out["workload"] = ("8 acquisitions " + ", ".join("x".join(map(str, a["shape"][::-1])) + f"/{a['nD']}n"
                                                      for a in acqs) + f", Poisson arrivals {args.serve_rate:g}/s")
     out["slots"] = args.serve_slots
     out["rounds_per_step"] = args.serve_rounds
+    out["loop"] = args.serve_loop
     # the same stream at lower offered rates: the highest the GPU sustains is where the
     # latency stops growing with the run's length
     out["at_lower_rates"] = []
     for rate in [float(r) for r in args.serve_lower.split(",") if r]:
         t2, a2 = serve.arrivals(len(acqs), rate, args.serve_seconds, seed=args.seed + 2)
-        r2 = loop.run(t2, a2)
+        r2 = loop.run(t2, a2, native=native)
         s2 = serve.summarize(r2, t2, a2, pools)
         out["at_lower_rates"].append({"offered_frames_per_s": s2["offered_frames_per_s"],
                                       "frames_per_s": s2["frames_per_s"], "latency_ms": s2["latency_ms"],
@@ -226,6 +228,7 @@ def serving_512(F, args, ctx, plain_fps):
     server at 3x the plain path's frames/s (the batched-serving target's operating point) and at
     args.serve512_high; frames/s and p50/p99 arrival -> result latency, and served == alone on a
     sample."""
+    native = args.serve_loop == "native"
     from fa_amd import serve, synth
     st = synth.make_stack((1, 512, 512), 64, 31, seed=args.seed + 5, n_blobs=600, mean=50.0, dtype="uint16")
     ref = F.Reference(ctx, synth.with_offset(st["reference"]), 31)
@@ -233,16 +236,16 @@ def serving_512(F, args, ctx, plain_fps):
     srv = F.Server([ref], slots=args.serve_slots, **tr)
     loop = serve.ServeLoop(srv, [ref], [st["frames"]], rounds=args.serve_rounds)
     out = {"workload": "512x512 uint16, nD=31, one acquisition, 64 distinct frames cycled, Poisson arrivals",
-           "plain_frames_per_s": round(plain_fps, 1), "runs": []}
+           "plain_frames_per_s": round(plain_fps, 1), "loop": args.serve_loop, "runs": []}
     w, wa = serve.arrivals(1, 2000.0, 0.2, seed=args.seed + 6)
-    loop.run(w, wa)  # warm-up
+    loop.run(w, wa, native=native)  # warm-up
     for rate, dur in ((3.0 * plain_fps, args.serve_seconds), (args.serve512_high, args.serve_seconds),
                       (args.serve512_sat, min(args.serve_seconds, 0.5))):
         if not rate:
             continue
         t, a = serve.arrivals(1, rate, dur, seed=args.seed + 7)
         keep = set(range(0, len(t), max(1, len(t) // 16)))
-        r = loop.run(t, a, keep=keep)
+        r = loop.run(t, a, keep=keep, native=native)
         s = serve.summarize(r, t, a, [st["frames"]])
         same = True
         for i in sorted(keep):
@@ -727,6 +730,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-slots", type=int, default=128)
     ap.add_argument("--serve-rounds", type=int, default=3)
+    ap.add_argument("--serve-loop", choices=["native", "python"], default="native",
+                    help="request loop of the serving sections: in the library (fa_server_run_stream) or "
+                         "in Python over fa_server_submit_host / fa_server_step")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve512-high", type=float, default=20000.0,
                     help="second offered rate (frames/s) of the 512^2 serving run")

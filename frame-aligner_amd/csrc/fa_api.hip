@@ -2675,4 +2675,57 @@ int fa_server_results(fa_server* sv, fa_server_result* out, int32_t max, int32_t
    return FA_OK;
 }
 
+int fa_server_run_stream(fa_server* sv, int64_t n, const double* arrival_s, fa_reference* const* refs,
+                         const void* const* data, const int32_t* dtype, const float* add_offset,
+                         double* x_inout, const int64_t* x_offset, int32_t rounds, double* latency_s,
+                         fa_align_stats* stats, fa_stream_counts* counts)
+{
+   if (!sv || n < 0 || rounds < 1 || (n > 0 && (!arrival_s || !refs || !data || !dtype || !add_offset ||
+                                                !x_inout || !x_offset || !latency_s || !stats)))
+      return set_err(FA_E_ARG, "invalid argument");
+   for (int64_t i = 1; i < n; i++)
+      if (arrival_s[i] < arrival_s[i - 1]) return set_err(FA_E_ARG, "arrivals out of order");
+   using clock = std::chrono::steady_clock;
+   const auto t0 = clock::now();
+   auto since = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
+   fa_stream_counts c{};
+   std::vector<fa_server_result> buf(256);
+   int64_t next = 0, done = 0, busy = 0;
+   while (done < n) {
+      const double now = since();
+      while (next < n && arrival_s[next] <= now) {
+         CHK(fa_server_submit_host(sv, refs[next], data[next], dtype[next], add_offset[next],
+                                   x_inout + x_offset[next], next));
+         next++;
+         busy++;
+      }
+      if (busy == 0) {
+         // nothing submitted and not done: wait for the next arrival
+         std::this_thread::sleep_for(std::chrono::duration<double>(std::max(0.0, arrival_s[next] - since())));
+         continue;
+      }
+      int32_t in_flight = 0, queued = 0;
+      CHK(fa_server_step(sv, rounds, &in_flight, &queued));
+      c.steps++;
+      c.max_in_flight = std::max<int64_t>(c.max_in_flight, in_flight);
+      c.max_queued = std::max<int64_t>(c.max_queued, queued);
+      for (;;) {
+         int32_t got = 0;
+         CHK(fa_server_results(sv, buf.data(), (int32_t)buf.size(), &got));
+         const double t = since();
+         for (int32_t k = 0; k < got; k++) {
+            const int64_t i = buf[k].tag;
+            latency_s[i] = t - arrival_s[i];
+            stats[i] = buf[k].stats;
+            done++;
+            busy--;
+         }
+         if (got < (int32_t)buf.size()) break;
+      }
+   }
+   c.wall_s = since();
+   if (counts) *counts = c;
+   return FA_OK;
+}
+
 }  // extern "C"

@@ -84,6 +84,11 @@ class ServerResult(C.Structure):
     _fields_ = [("tag", C.c_int64), ("stats", AlignStats)]
 
 
+class StreamCounts(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("max_in_flight", C.c_int64), ("max_queued", C.c_int64),
+                ("wall_s", C.c_double)]
+
+
 class BatchTiming(C.Structure):
     _fields_ = [("wall_ms", C.c_double), ("pass_ms", C.c_double), ("pass_launches", C.c_int64),
                 ("pass_bytes", C.c_double), ("tr_rounds", C.c_int64), ("pass_px", C.c_double)]
@@ -99,7 +104,7 @@ EXPORTS = [
     "fa_batch_get_tuning", "fa_batch_set_tuning",
     "fa_batch_run", "fa_batch_eval", "fa_align_batch",
     "fa_context_stream", "fa_context_mem_info", "fa_context_info", "fa_server_create", "fa_server_destroy",
-    "fa_server_submit", "fa_server_submit_host", "fa_server_step", "fa_server_results",
+    "fa_server_submit", "fa_server_submit_host", "fa_server_step", "fa_server_results", "fa_server_run_stream",
     "fa_server_memory", "fa_host_alloc", "fa_host_free",
     "fa_aligner_create", "fa_aligner_destroy", "fa_aligner_reference", "fa_aligner_preprocess",
     "fa_aligner_rigid_shift", "fa_aligner_add_frames", "fa_aligner_clear_temp", "fa_aligner_realign_as_frame",
@@ -173,6 +178,7 @@ def load_library(path=None):
     L.fa_server_submit.argtypes = [P, P, P, C.c_int64]
     L.fa_server_step.argtypes = [P, i32, P, P]
     L.fa_server_results.argtypes = [P, P, i32, P]
+    L.fa_server_run_stream.argtypes = [P, C.c_int64, P, P, P, P, P, P, P, i32, P, P, P]
     L.fa_context_stream.argtypes = [P]
     L.fa_context_stream.restype = P
     _lib = L
@@ -621,6 +627,39 @@ class Server:
             if n.value < len(self._buf):
                 break
         return a.value, q.value, out
+
+    def run_stream(self, times, refs, frames, rounds=3, offsets=None):
+        """The request loop in the library (fa_server_run_stream): frame i (a host array, ideally
+        pinned, of refs[i]) arrives times[i] seconds after the call starts; returns (latency s
+        per frame, [(x, stats)] per frame, counts). The server must hold no other frames."""
+        n = len(times)
+        t = np.ascontiguousarray(times, np.float64)
+        hs = (C.c_void_p * n)(*[r.h for r in refs])
+        arrs = []
+        for a in frames:
+            a = a if isinstance(a, np.ndarray) else np.asarray(a)
+            if not a.flags.c_contiguous or a.dtype not in (np.uint16, np.float32):
+                raise ValueError("host frames are C-contiguous uint16 or float32")
+            arrs.append(a)
+        for a, r in zip(arrs, refs):
+            if a.size != r.n_px:
+                raise FaError(-1, "Frame is incorrect size")
+        ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
+        dt = np.array([FA_U16 if a.dtype == np.uint16 else FA_F32 for a in arrs], np.int32)
+        if offsets is None:
+            off = np.array([1.0 if d == FA_U16 else 0.0 for d in dt], np.float32)
+        else:
+            off = np.ascontiguousarray(offsets, np.float32)
+        sizes = np.array([r.nD * r.n_dim for r in refs], np.int64)
+        xoff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if n else np.zeros(0, np.int64)
+        x = np.zeros(int(sizes.sum()) if n else 0)
+        lat = np.zeros(n)
+        st = (AlignStats * max(n, 1))()
+        cnt = StreamCounts()
+        _check(self.L.fa_server_run_stream(self.h, n, _ptr(t), hs, ptrs, _ptr(dt), _ptr(off), _ptr(x), _ptr(xoff),
+                                           int(rounds), _ptr(lat), st, C.byref(cnt)))
+        res = [(x[xoff[i]:xoff[i] + sizes[i]].copy(), stats_dict(st[i])) for i in range(n)]
+        return lat, res, {f: getattr(cnt, f) for f, _ in StreamCounts._fields_}
 
 
 class Aligner:

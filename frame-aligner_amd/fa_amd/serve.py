@@ -70,9 +70,12 @@ class ServeLoop:
         for p in self.pinned:
             p.close()
 
-    def run(self, times, acq, keep=None):
+    def run(self, times, acq, keep=None, native=False):
         """Streams the arrivals in real time. Returns dict with per-frame latency (s), the
-        results of the frames whose index is in `keep`, and loop counters."""
+        results of the frames whose index is in `keep`, and loop counters. native: the loop runs
+        in the library (fa_server_run_stream) instead of here."""
+        if native:
+            return self._run_native(times, acq, keep)
         S = self.server
         n = len(times)
         lat = np.full(n, np.nan)
@@ -111,6 +114,20 @@ class ServeLoop:
         wall = time.perf_counter() - t0
         return dict(latency=lat, kept=kept, wall=wall, steps=steps, max_in_flight=max_flight,
                     max_queued=max_queued, evaluations=evals)
+
+    def _run_native(self, times, acq, keep=None):
+        n = len(times)
+        refs, frames = [], []
+        for i in range(n):
+            a = int(acq[i])
+            pool = self.pinned[a].array
+            refs.append(self.refs[a])
+            frames.append(pool[i % len(pool)])
+        lat, res, cnt = self.server.run_stream(times, refs, frames, rounds=self.rounds)
+        kept = {i: res[i] for i in range(n) if keep is not None and i in keep}
+        evals = np.array([r[1]["evaluations"] for r in res], np.int64)
+        return dict(latency=lat, kept=kept, wall=cnt["wall_s"], steps=cnt["steps"],
+                    max_in_flight=cnt["max_in_flight"], max_queued=cnt["max_queued"], evaluations=evals)
 
 
 def summarize(res, times, acq, pools):

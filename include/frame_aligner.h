@@ -276,6 +276,26 @@ int fa_server_submit_host(fa_server* server, fa_reference* ref, const void* data
                           float add_offset, double* x_inout, int64_t tag);
 int fa_server_step(fa_server* server, int32_t rounds, int32_t* in_flight, int32_t* queued);
 int fa_server_results(fa_server* server, fa_server_result* out, int32_t max, int32_t* n_out);
+/* A request loop in the library: arrival i, arrival_s[i] seconds after the call starts (in
+   non-decreasing order), submits host frame data[i] of refs[i] (dtype[i], add_offset[i], start
+   and result vector x_inout + x_offset[i]) with tag i, as fa_server_submit_host; the loop steps
+   the server with `rounds` while frames are in flight or queued, sleeps until the next arrival
+   otherwise, and returns when every frame's result is in: latency_s[i] = time of the step that
+   returned result i - arrival_s[i], stats[i] its trust-region statistics. The frames must stay
+   untouched until the call returns. The server should hold no other frames. It replaces a
+   host-language loop of fa_server_submit_host / fa_server_step / fa_server_results with the
+   same calls made natively (no counterpart in the reference beyond its MEX thread pool,
+   FrameAlignerMex/FrameAlignerMex.cpp:110-130). */
+typedef struct fa_stream_counts {
+   int64_t steps;          /* fa_server_step calls */
+   int64_t max_in_flight;  /* most frames admitted at once */
+   int64_t max_queued;     /* most frames waiting for a slot */
+   double wall_s;          /* seconds from the call's start to the last result */
+} fa_stream_counts;
+int fa_server_run_stream(fa_server* server, int64_t n, const double* arrival_s, fa_reference* const* refs,
+                         const void* const* data, const int32_t* dtype, const float* add_offset,
+                         double* x_inout, const int64_t* x_offset, int32_t rounds, double* latency_s,
+                         fa_align_stats* stats, fa_stream_counts* counts);
 /* Bytes of the frame arena in use and in total. */
 int fa_server_memory(const fa_server* server, uint64_t* used, uint64_t* total);
 /* Pinned host memory for frames handed to fa_server_submit_host. */

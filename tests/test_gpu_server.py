@@ -234,7 +234,10 @@ def test_server_host_frames_and_arena(ctx, slots):
         p.close()
 
 
-def test_serve_loop_small(ctx):
+@pytest.mark.parametrize("native", [False, True])
+def test_serve_loop_small(ctx, native):
+    """The request loop here and in the library (fa_server_run_stream): every served result is
+    the frame aligned alone."""
     acqs = serve.make_acquisitions(seed=5, shapes=[(1, 64, 64), (1, 96, 80), (4, 40, 36)], pool=2,
                                    node_range=(8, 12))
     refs = [F.Reference(ctx, serve.synth.with_offset(a["reference"]), a["nD"]) for a in acqs]
@@ -243,7 +246,7 @@ def test_serve_loop_small(ctx):
     times, acq = serve.arrivals(len(acqs), rate=400.0, duration=0.1, seed=6)
     loop = serve.ServeLoop(srv, refs, pools, rounds=2)
     keep = set(range(len(times)))
-    res = loop.run(times, acq, keep=keep)
+    res = loop.run(times, acq, keep=keep, native=native)
     assert np.all(np.isfinite(res["latency"])) and np.all(res["latency"] >= 0)
     # each served result is the same as aligning that frame alone
     for i in range(len(times)):
