@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "fa_internal.hpp"
@@ -30,6 +31,44 @@
 using namespace fa;
 
 namespace {
+
+// Host copies between pageable caller memory and the aligner's pinned staging: pieces of
+// `parts` (dst, src, bytes) triples spread over up to kCopyThreads threads (a single thread
+// moves a few GB/s into freshly allocated pages; the DMA side runs at PCIe rate).
+constexpr int kCopyThreads = 8;
+struct CopyPiece {
+   void* dst;
+   const void* src;
+   size_t bytes;
+};
+
+void par_copy(const std::vector<CopyPiece>& parts)
+{
+   size_t total = 0;
+   for (const auto& c : parts) total += c.bytes;
+   const int T = (int)std::max<size_t>(1, std::min<size_t>({(size_t)kCopyThreads, parts.size() * 4, total >> 21}));
+   if (T <= 1) {
+      for (const auto& c : parts) std::memcpy(c.dst, c.src, c.bytes);
+      return;
+   }
+   // thread t copies bytes [t total / T, (t + 1) total / T) of the concatenated pieces
+   auto run = [&](int t) {
+      const size_t lo = total * t / T, hi = total * (t + 1) / T;
+      size_t at = 0;
+      for (const auto& c : parts) {
+         const size_t a = std::max(lo, at), b = std::min(hi, at + c.bytes);
+         if (a < b) std::memcpy((char*)c.dst + (a - at), (const char*)c.src + (a - at), b - a);
+         at += c.bytes;
+         if (at >= hi) break;
+      }
+   };
+   std::vector<std::thread> th;
+   th.reserve(T - 1);
+   for (int t = 1; t < T; t++) th.emplace_back(run, t);
+   run(0);
+   for (auto& x : th) x.join();
+}
+
 
 #define FCHK(expr)                                                                                \
    do {                                                                                           \
@@ -397,6 +436,10 @@ struct fa_aligner {
       size_t bytes = 0;
    };
    Buf bufs[kBufs];
+   // pinned host staging of uploads and outputs, two buffers used in turn, and the events of
+   // the copies that last used them
+   Buf hbuf[2];
+   hipEvent_t hev[2] = {nullptr, nullptr};
    std::vector<fa_frame*> cfr;
    fa_batch* cbatch = nullptr;
 
@@ -431,10 +474,48 @@ struct fa_aligner {
       return FA_OK;
    }
 
+   // both pinned staging buffers with room for `bytes` (the stream drained first when they
+   // grow) and their events
+   int grow_host(size_t bytes)
+   {
+      for (int i = 0; i < 2; i++) {
+         if (!hev[i] && hipEventCreateWithFlags(&hev[i], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            hev[i] = nullptr;
+            return set_error(FA_E_DEVICE, "event creation failed");
+         }
+         Buf& b = hbuf[i];
+         if (b.bytes >= bytes) continue;
+         FCHK(hipStreamSynchronize(stream));
+         if (b.p) (void)hipHostFree(b.p);
+         b.p = nullptr;
+         b.bytes = 0;
+         if (hipHostMalloc(&b.p, std::max<size_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            b.p = nullptr;
+            return set_error(FA_E_OOM, "Not enough memory to allocate");
+         }
+         b.bytes = bytes;
+      }
+      return FA_OK;
+   }
+
+   void free_host()
+   {
+      for (Buf& b : hbuf) {
+         if (b.p) (void)hipHostFree(b.p);
+         b.p = nullptr;
+         b.bytes = 0;
+      }
+   }
+
    ~fa_aligner()
    {
       if (stream) (void)hipStreamSynchronize(stream);
       drop_cached();
+      free_host();
+      for (hipEvent_t e : hev)
+         if (e) (void)hipEventDestroy(e);
       for (Buf& b : bufs)
          if (b.p) (void)hipFree(b.p);
       for (auto& kv : bplans) {
@@ -633,9 +714,34 @@ struct fa_aligner {
       if (hipMemcpyAsync(d_D, D_all, 3 * (size_t)nD * n * sizeof(double), hipMemcpyHostToDevice, stream) != hipSuccess)
          return done(set_error(FA_E_DEVICE, "output warp failed"));
       auto fail = [&](const char* what) { (void)hipGetLastError(); return done(set_error(FA_E_DEVICE, what)); };
-      for (int j0 = 0; j0 < n; j0 += C) {
+      // host outputs leave through the two pinned buffers in turn: a chunk's images are copied
+      // there by the GPU, then into the caller's arrays by the host threads while the GPU works
+      // on the next chunk. Per chunk: mask (float), preserving, realigned (uint16), warped (float).
+      const size_t o_mask = 0, o_pres = o_mask + (o && o->mask ? (size_t)cpx * 4 : 0);
+      const size_t o_real = o_pres + (o && o->preserving ? (size_t)cpx * 2 : 0);
+      const size_t o_warp = o_real + (o && o->realigned ? (size_t)cpx * 2 : 0);
+      const size_t o_end = o_warp + (o && o->warped ? (size_t)cpx * 4 : 0);
+      if (o_end && (rc = grow_host(o_end))) return done(rc);
+      struct Staged {
+         int k = -1;
+         int64_t off = 0, m = 0;
+      } prev;
+      auto deliver = [&](const Staged& q) -> int {
+         if (q.k < 0 || !o_end) return FA_OK;
+         if (hipEventSynchronize(hev[q.k & 1]) != hipSuccess) return set_error(FA_E_DEVICE, "output copy failed");
+         const char* st = (const char*)hbuf[q.k & 1].p;
+         std::vector<CopyPiece> pc;
+         if (o->mask) pc.push_back({o->mask + q.off, st + o_mask, (size_t)q.m * 4});
+         if (o->preserving) pc.push_back({o->preserving + q.off, st + o_pres, (size_t)q.m * 2});
+         if (o->realigned) pc.push_back({o->realigned + q.off, st + o_real, (size_t)q.m * 2});
+         if (o->warped) pc.push_back({o->warped + q.off, st + o_warp, (size_t)q.m * 4});
+         par_copy(pc);
+         return FA_OK;
+      };
+      for (int j0 = 0, k = 0; j0 < n; j0 += C, k++) {
          const int c = std::min(C, n - j0);
          const int64_t off = (int64_t)j0 * n_px, m = (int64_t)c * n_px;
+         char* st = o_end ? (char*)hbuf[k & 1].p : nullptr;
          if (hipMemsetAsync(d_wimg, 0, m * sizeof(float), stream) != hipSuccess ||
              hipMemsetAsync(d_cov, 0, m * sizeof(float), stream) != hipSuccess)
             return fail("output warp failed");
@@ -644,12 +750,12 @@ struct fa_aligner {
          if (launch_splat(R, Fraw, Dc, d_wimg, d_cov, 0.f, grid, stream, c) != hipSuccess)
             return fail("output warp failed");
          if (o && o->mask &&
-             hipMemcpyAsync(o->mask + off, d_cov, m * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
+             hipMemcpyAsync(st + o_mask, d_cov, m * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
             return fail("output copy failed");
          if (o && o->preserving) {
             hipLaunchKernelGGL(k_preserving, dim3(grid_of(m)), dim3(kFB), 0, stream, (const float*)d_wimg,
                                (const float*)d_cov, m, d_u16);
-            if (hipMemcpyAsync(o->preserving + off, d_u16, m * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
+            if (hipMemcpyAsync(st + o_pres, d_u16, m * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
                 hipSuccess)
                return fail("output copy failed");
          }
@@ -657,14 +763,14 @@ struct fa_aligner {
             if (launch_warp(R, Fraw, Dc, d_w, 0.f, 0, 0, grid, stream, c) != hipSuccess)
                return fail("output warp failed");
             hipLaunchKernelGGL(k_to_u16, dim3(grid_of(m)), dim3(kFB), 0, stream, (const float*)d_w, m, d_u16);
-            if (hipMemcpyAsync(o->realigned + off, d_u16, m * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
+            if (hipMemcpyAsync(st + o_real, d_u16, m * sizeof(uint16_t), hipMemcpyDeviceToHost, stream) !=
                 hipSuccess)
                return fail("output copy failed");
          }
          if (launch_warp(R, Fsm, Dc, d_w2, 0.f, 0, 0, grid, stream, c) != hipSuccess)
             return fail("output warp failed");
          if (o && o->warped &&
-             hipMemcpyAsync(o->warped + off, d_w2, m * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
+             hipMemcpyAsync(st + o_warp, d_w2, m * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess)
             return fail("output copy failed");
          if (res) {
             // correlation of the warped blurred frame and of the blurred frame itself with the
@@ -683,7 +789,13 @@ struct fa_aligner {
             }
          }
          if (hipGetLastError() != hipSuccess) return fail("output warp failed");
+         if (o_end && hipEventRecord(hev[k & 1], stream) != hipSuccess) return fail("output copy failed");
+         // the previous chunk's images into the caller's arrays while this chunk runs (its
+         // buffer is the other one, free since that chunk's delivery two chunks ago)
+         if ((rc = deliver(prev))) return done(rc);
+         prev = Staged{k, off, m};
       }
+      if ((rc = deliver(prev))) return done(rc);
       if (res) {
          std::vector<double> sums((size_t)4 * n * kMoments);
          if (hipMemcpyAsync(sums.data(), d_sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost, stream) != hipSuccess ||
@@ -844,6 +956,7 @@ int fa_aligner_clear_temp(fa_aligner* a)
       b.p = nullptr;
       b.bytes = 0;
    }
+   a->free_host();
    return fa_clear_temp(a->ref);
 }
 
@@ -919,9 +1032,24 @@ int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, i
    }
    // 1. all frames uploaded and converted, then C at a time: rigid shifts of the raw frames and
    //    the warper's frames (blur + 1); the shifts come back in one copy
-   for (int i = 0; i < n && rc == FA_OK; i++)
-      if (hipMemcpyAsync(d_up + (size_t)i * rb, frames[i], rb, hipMemcpyHostToDevice, a->stream) != hipSuccess)
+   //    (U frames at a time through the two pinned buffers in turn: the host threads copy a
+   //    chunk while the previous one's DMA runs)
+   const int U = (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(32 << 20) / (int64_t)rb));
+   if (rc == FA_OK) rc = a->grow_host((size_t)U * rb);
+   for (int j0 = 0, k = 0; j0 < n && rc == FA_OK; j0 += U, k++) {
+      const int c = std::min(U, n - j0);
+      uint8_t* st = (uint8_t*)a->hbuf[k & 1].p;
+      if (k >= 2 && hipEventSynchronize(a->hev[k & 1]) != hipSuccess) {
          rc = set_error(FA_E_DEVICE, "frame upload failed");
+         break;
+      }
+      std::vector<CopyPiece> pc;
+      for (int i = 0; i < c; i++) pc.push_back({st + (size_t)i * rb, frames[j0 + i], rb});
+      par_copy(pc);
+      if (hipMemcpyAsync(d_up + (size_t)j0 * rb, st, (size_t)c * rb, hipMemcpyHostToDevice, a->stream) != hipSuccess ||
+          hipEventRecord(a->hev[k & 1], a->stream) != hipSuccess)
+         rc = set_error(FA_E_DEVICE, "frame upload failed");
+   }
    if (rc == FA_OK) {
       hipLaunchKernelGGL(k_to_float, dim3(grid_of((int64_t)n * npx)), dim3(kFB), 0, a->stream, (const void*)d_up, dtype,
                          (int64_t)n * npx, d_rawf);
