@@ -1030,11 +1030,13 @@ int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, i
       release();
       return rc;
    }
-   // 1. all frames uploaded and converted, then C at a time: rigid shifts of the raw frames and
-   //    the warper's frames (blur + 1); the shifts come back in one copy
-   //    (U frames at a time through the two pinned buffers in turn: the host threads copy a
-   //    chunk while the previous one's DMA runs)
-   const int U = (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(32 << 20) / (int64_t)rb));
+   // 1. the frames uploaded U at a time through the two pinned buffers in turn (the host
+   //    threads copy a chunk while the previous chunk's DMA and kernels run), each chunk
+   //    converted, then C frames at a time: rigid shifts of the raw frames and the warper's
+   //    frames (blur + 1); the shifts come back in one copy. U is a multiple of C, so the
+   //    rigid batches are the same as over all frames at once.
+   int U = (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(32 << 20) / (int64_t)rb));
+   if (U >= C) U = U / C * C;
    if (rc == FA_OK) rc = a->grow_host((size_t)U * rb);
    for (int j0 = 0, k = 0; j0 < n && rc == FA_OK; j0 += U, k++) {
       const int c = std::min(U, n - j0);
@@ -1047,18 +1049,21 @@ int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, i
       for (int i = 0; i < c; i++) pc.push_back({st + (size_t)i * rb, frames[j0 + i], rb});
       par_copy(pc);
       if (hipMemcpyAsync(d_up + (size_t)j0 * rb, st, (size_t)c * rb, hipMemcpyHostToDevice, a->stream) != hipSuccess ||
-          hipEventRecord(a->hev[k & 1], a->stream) != hipSuccess)
+          hipEventRecord(a->hev[k & 1], a->stream) != hipSuccess) {
          rc = set_error(FA_E_DEVICE, "frame upload failed");
-   }
-   if (rc == FA_OK) {
-      hipLaunchKernelGGL(k_to_float, dim3(grid_of((int64_t)n * npx)), dim3(kFB), 0, a->stream, (const void*)d_up, dtype,
-                         (int64_t)n * npx, d_rawf);
-      if (hipGetLastError() != hipSuccess) rc = set_error(FA_E_DEVICE, "frame upload failed");
-   }
-   for (int j0 = 0; j0 < n && rc == FA_OK; j0 += C) {
-      const int c = std::min(C, n - j0);
-      if ((rc = a->shifts(d_rawf + (int64_t)j0 * npx, c, w_real, w_spec, w_pv, w_pi, d_idx + j0)) == FA_OK)
-         rc = a->smooth_plus_one(d_rawf + (int64_t)j0 * npx, d_pre + (int64_t)j0 * npx, c, d_btmp);
+         break;
+      }
+      hipLaunchKernelGGL(k_to_float, dim3(grid_of((int64_t)c * npx)), dim3(kFB), 0, a->stream,
+                         (const void*)(d_up + (size_t)j0 * rb), dtype, (int64_t)c * npx, d_rawf + (int64_t)j0 * npx);
+      if (hipGetLastError() != hipSuccess) {
+         rc = set_error(FA_E_DEVICE, "frame upload failed");
+         break;
+      }
+      for (int i0 = j0; i0 < j0 + c && rc == FA_OK; i0 += C) {
+         const int cc = std::min(C, j0 + c - i0);
+         if ((rc = a->shifts(d_rawf + (int64_t)i0 * npx, cc, w_real, w_spec, w_pv, w_pi, d_idx + i0)) == FA_OK)
+            rc = a->smooth_plus_one(d_rawf + (int64_t)i0 * npx, d_pre + (int64_t)i0 * npx, cc, d_btmp);
+      }
    }
    std::vector<int64_t> idx(n);
    if (rc == FA_OK && (hipMemcpyAsync(idx.data(), d_idx, n * sizeof(int64_t), hipMemcpyDeviceToHost, a->stream) != hipSuccess ||
