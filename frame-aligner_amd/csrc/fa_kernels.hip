@@ -1144,12 +1144,14 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
    // (half an FP64 ulp of |d|; none when D[k+1] = D[k]).
    Stage S{0, 1, 0, 0, 0};
    bool use = false;
-   // tile sampling: Dsh = D[k] + K with integers K = -floor(lo) that make every pixel's
-   // displacement d + K >= 0 over the work item, so its integer part is a truncation and its
-   // fraction one v_fract (d + K rounds at ~1e-15 px)
-   double Dsh[3] = {0.0, 0.0, 0.0};
+   // Positions d + K with Dsh = D[k] + K and integers K = -floor(lo) that make every pixel's
+   // displacement d + K >= 0 over the work item: in a staged tile its integer part is a
+   // truncation and its fraction one v_fract (d + K rounds at ~1e-15 px). The global-load
+   // sampler takes the same d + K (floor, then -K), so an item gives the same bits staged or
+   // not, whichever LDS budget its launch has.
+   double Dsh[3] = {Da[0], Da[1], Da[2]};
    int Ksh[3] = {0, 0, 0};
-   if (want_lds && finite) {
+   if (finite) {
       const double lim = kPad - 3;
       const double fa = (double)((M == 3) ? R.df[ch.p0] : R.rg4[ch.p0].w);
       const double fb = (double)((M == 3) ? R.df[ch.p1 - 1] : R.rg4[ch.p1 - 1].w);
@@ -1162,7 +1164,12 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          lo[c] = fmin(va, vb) - e;
          hi[c] = fmax(va, vb) + e;
       }
-      if (lo[0] > -lim && hi[0] < lim) {
+#pragma unroll
+      for (int c = 0; c < M; c++) {
+         if (fabs(lo[c]) < 0x1p20) Ksh[c] = -(int)floor(lo[c]);
+         Dsh[c] = Da[c] + (double)Ksh[c];
+      }
+      if (want_lds && lo[0] > -lim && hi[0] < lim) {
          // taps y0, y0 + 1 with y0 = y + floor(d_y); likewise in z
          S.ya = r0 + (int)floor(lo[1]);
          S.nrows = r1 + (int)floor(hi[1]) + 1 - S.ya + 1;
@@ -1172,11 +1179,6 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
          }
          S.stride = ((ch.w + 2 * kPad) + 3) & ~3;
          use = (int64_t)S.nzs * S.nrows * S.stride <= lds_cap;
-#pragma unroll
-         for (int c = 0; c < M; c++) {
-            Ksh[c] = -(int)floor(lo[c]);
-            Dsh[c] = Da[c] + (double)Ksh[c];
-         }
       }
    }
    const int cx = ch.x0, cw = ch.w;  // columns of the item (cx = 0, cw = nx: whole rows)
@@ -1227,10 +1229,11 @@ __device__ void pass_chunk(const RefDev& R, const FrameDev& F, const Chunk ch,
                   fr = __builtin_amdgcn_fract(d);
                   ti[q] = (int)d;
                } else {
-                  const double d = fma(fd, Dd[q], Da[q]);
+                  // the staged sampler's d + K, then floor and -K
+                  const double d = fma(fd, Dd[q], Dsh[q]);
                   const double fl = floor(d);
                   fr = d - fl;
-                  ti[q] = (int)fl;
+                  ti[q] = (int)fl - Ksh[q];
                }
                // below 1 after the rounding to float too, so a weight of 1 never hides the
                // other tap

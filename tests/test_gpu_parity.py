@@ -10,7 +10,7 @@ import pytest
 
 import oracle as O
 import fa_amd as F
-from helpers import MANY_NODE_CASES, PARITY_CASES, rel_err, small_stack, truth_start
+from helpers import LONG_CHAIN_CASE, MANY_NODE_CASES, PARITY_CASES, rel_err, small_stack, truth_start
 
 pytestmark = pytest.mark.gpu
 
@@ -132,20 +132,24 @@ def test_error_and_jacobian_odd_width(ctx, name, shape, nD, scan, gen, dtype):
 
 
 @pytest.mark.parametrize("name,shape,nD,scan,gen",
-                         [c[:4] + (GEN.get(c[0], {}),) for c in CASES] + ODD_WIDTH_CASES + STRIP_CASES)
+                         [c[:4] + (GEN.get(c[0], {}),) for c in CASES] + ODD_WIDTH_CASES + STRIP_CASES
+                         + [LONG_CHAIN_CASE])
 def test_staged_and_global_sampling_agree(ctx, name, shape, nD, scan, gen):
     """Work items sample the frame from their LDS tile or, when it does not fit, from global
-    memory with the same arithmetic: a batch run with staging switched off gives the same bits."""
+    memory with the same arithmetic: a batch run with staging switched off, or with an LDS
+    budget that stages only some items, gives the same bits."""
     st, orc, ref = _pair(ctx, shape, nD, scan, dtype="uint16", **gen)
     kw = dict(rel_delta=1e-4, max_iter=50, skip_repeat=1)
     frames = [ref.register(f, offset=1.0) for f in st["frames"]]
     batch = F.Batch(frames, **kw)
     x1, s1, _ = batch.run()
-    batch.set_tuning(use_lds=0)
-    x0, s0, _ = batch.run()
-    for a, b, sa, sb in zip(x1, x0, s1, s0):
-        assert np.array_equal(a, b)
-        assert sa["evaluations"] == sb["evaluations"]
+    full = batch.tuning["lds_floats"]
+    for tune in (dict(use_lds=0), dict(use_lds=1, lds_floats=max(16, full * 2 // 3))):
+        batch.set_tuning(**tune)
+        x0, s0, _ = batch.run()
+        for a, b, sa, sb in zip(x1, x0, s1, s0):
+            assert np.array_equal(a, b), tune
+            assert sa["evaluations"] == sb["evaluations"]
     batch.close()
 
 
