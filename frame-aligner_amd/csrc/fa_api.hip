@@ -2691,15 +2691,26 @@ int fa_server_run_stream(fa_server* sv, int64_t n, const double* arrival_s, fa_r
    fa_stream_counts c{};
    std::vector<fa_server_result> buf(256);
    int64_t next = 0, done = 0, busy = 0;
-   while (done < n) {
+   // a frame that cannot be submitted ends the stream: the frames already submitted are run
+   // to their results (the server then holds nothing that points into the caller's arrays),
+   // and its error is returned
+   int stop_rc = FA_OK;
+   std::string stop_msg;
+   while (done < next || (stop_rc == FA_OK && done < n)) {
       const double now = since();
-      while (next < n && arrival_s[next] <= now) {
-         CHK(fa_server_submit_host(sv, refs[next], data[next], dtype[next], add_offset[next],
-                                   x_inout + x_offset[next], next));
+      while (stop_rc == FA_OK && next < n && arrival_s[next] <= now) {
+         const int rc = fa_server_submit_host(sv, refs[next], data[next], dtype[next], add_offset[next],
+                                              x_inout + x_offset[next], next);
+         if (rc != FA_OK) {
+            stop_rc = rc;
+            stop_msg = fa_last_error();
+            break;
+         }
          next++;
          busy++;
       }
       if (busy == 0) {
+         if (stop_rc != FA_OK) break;
          // nothing submitted and not done: wait for the next arrival
          std::this_thread::sleep_for(std::chrono::duration<double>(std::max(0.0, arrival_s[next] - since())));
          continue;
@@ -2725,6 +2736,7 @@ int fa_server_run_stream(fa_server* sv, int64_t n, const double* arrival_s, fa_r
    }
    c.wall_s = since();
    if (counts) *counts = c;
+   if (stop_rc != FA_OK) return set_err(stop_rc, stop_msg);
    return FA_OK;
 }
 

@@ -282,7 +282,10 @@ int fa_server_results(fa_server* server, fa_server_result* out, int32_t max, int
    the server with `rounds` while frames are in flight or queued, sleeps until the next arrival
    otherwise, and returns when every frame's result is in: latency_s[i] = time of the step that
    returned result i - arrival_s[i], stats[i] its trust-region statistics. The frames must stay
-   untouched until the call returns. The server should hold no other frames. It replaces a
+   untouched until the call returns. The server should hold no other frames. A frame that
+   cannot be submitted (fa_server_submit_host's errors) ends the stream: the frames submitted
+   before it are run to their results and its error is returned; after a device error the
+   server must not be stepped again. It replaces a
    host-language loop of fa_server_submit_host / fa_server_step / fa_server_results with the
    same calls made natively (no counterpart in the reference beyond its MEX thread pool,
    FrameAlignerMex/FrameAlignerMex.cpp:110-130). */

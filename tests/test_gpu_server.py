@@ -180,6 +180,31 @@ def test_server_long_chain_same_alone(ctx):
         f.release()
 
 
+def test_run_stream_error_drains_the_server(ctx):
+    """A frame the server cannot take (its reference is not served) ends the native stream with
+    the error after the frames submitted before it have run; the server is then empty and
+    serves a following stream whose results equal the frames aligned alone."""
+    st = small_stack((1, 64, 64), n_frames=3, nD=9, seed=51)
+    other = small_stack((1, 64, 64), n_frames=1, nD=9, seed=52)
+    r1 = F.Reference(ctx, st["ref_w"], 9)
+    r2 = F.Reference(ctx, other["ref_w"], 9)
+    srv = F.Server([r1], slots=4, **KW)
+    frames = [np.ascontiguousarray(f) for f in st["frames_w"]]
+    with pytest.raises(F.FaError):
+        srv.run_stream([0.0, 0.0, 0.001], [r1, r1, r2], [frames[0], frames[1], other["frames_w"][0]])
+    assert srv.step(1)[:2] == (0, 0)
+    lat, res, cnt = srv.run_stream([0.0, 0.0005, 0.001], [r1, r1, r1], frames)
+    assert np.all(lat >= 0) and cnt["steps"] > 0
+    for f, (x, s) in zip(frames, res):
+        fr = r1.register(f)
+        xa, sa = fr.align(**KW)
+        fr.release()
+        assert np.array_equal(x, xa) and s["evaluations"] == sa["evaluations"]
+    srv.close()
+    r1._release_handle()
+    r2._release_handle()
+
+
 def test_server_errors(ctx):
     st = small_stack((1, 64, 64), n_frames=1, nD=9, seed=44)
     r1 = F.Reference(ctx, st["ref_w"], 9)
