@@ -876,16 +876,24 @@ int batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timi
 }
 
 // One fused pass over every frame of a leaf batch at D (concatenated nD x 3 per frame).
-int leaf_eval(fa_batch* b, const double* D, double* err, double* jac)
+// One evaluation of a leaf at D, in two halves: the launches (nothing waits), then the
+// copies of the states (and Jtr) and their delivery, which wait for that leaf's stream.
+int leaf_eval_launch(fa_batch* b, const double* D)
 {
    hipStream_t s = b->stream;
-   const int n = (int)b->frames.size();
    BatchDev B = b->dev();
    HIPCHK(hipMemcpyAsync(b->d_D, D, b->total_D * sizeof(double), hipMemcpyHostToDevice, s));
    HIPCHK(launch_tr_init(B, 1, s));
    HIPCHK(launch_schedule(B, 1, s));
    CHK(launch_passes(b, s));
    HIPCHK(launch_reduce_eval(B, s));
+   return FA_OK;
+}
+
+int leaf_eval_collect(fa_batch* b, double* err, double* jac)
+{
+   hipStream_t s = b->stream;
+   const int n = (int)b->frames.size();
    std::vector<TrState> st(n);
    std::vector<double> vec;
    HIPCHK(hipMemcpyAsync(st.data(), b->d_state, n * sizeof(TrState), hipMemcpyDeviceToHost, s));
@@ -905,15 +913,27 @@ int leaf_eval(fa_batch* b, const double* D, double* err, double* jac)
    return FA_OK;
 }
 
+int leaf_eval(fa_batch* b, const double* D, double* err, double* jac)
+{
+   CHK(leaf_eval_launch(b, D));
+   return leaf_eval_collect(b, err, jac);
+}
+
+// A split batch launches every part's evaluation before collecting any, so the parts' passes
+// run together on their streams.
 int batch_eval(fa_batch* b, const double* D, double* err, double* jac)
 {
    if (b->parts.empty()) return leaf_eval(b, D, err, jac);
-   int64_t xo = 0, dof = 0;
+   int64_t dof = 0;
+   for (fa_batch* q : b->parts) {
+      CHK(leaf_eval_launch(q, D + dof));
+      dof += q->total_D;
+   }
+   int64_t xo = 0;
    for (size_t i = 0; i < b->parts.size(); i++) {
       fa_batch* q = b->parts[i];
-      CHK(leaf_eval(q, D + dof, err ? err + b->part_first[i] : nullptr, jac ? jac + xo : nullptr));
+      CHK(leaf_eval_collect(q, err ? err + b->part_first[i] : nullptr, jac ? jac + xo : nullptr));
       xo += q->total_x;
-      dof += q->total_D;
    }
    return FA_OK;
 }
