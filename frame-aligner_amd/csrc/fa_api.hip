@@ -2705,6 +2705,10 @@ int fa_server_run_stream(fa_server* sv, int64_t n, const double* arrival_s, fa_r
       return set_err(FA_E_ARG, "invalid argument");
    for (int64_t i = 1; i < n; i++)
       if (arrival_s[i] < arrival_s[i - 1]) return set_err(FA_E_ARG, "arrivals out of order");
+   // results are matched to arrivals by tag, so frames submitted before the call (queued,
+   // admitted or with results not yet taken) would be taken for arrivals of this stream
+   if (sv->admitted > 0 || sv->queued() > 0 || !sv->results.empty())
+      return set_err(FA_E_ARG, "the server holds frames submitted before the stream");
    using clock = std::chrono::steady_clock;
    const auto t0 = clock::now();
    auto since = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
@@ -2746,6 +2750,7 @@ int fa_server_run_stream(fa_server* sv, int64_t n, const double* arrival_s, fa_r
          const double t = since();
          for (int32_t k = 0; k < got; k++) {
             const int64_t i = buf[k].tag;
+            if (i < 0 || i >= next) return set_err(FA_E_STATE, "result of a frame not in the stream");
             latency_s[i] = t - arrival_s[i];
             stats[i] = buf[k].stats;
             done++;

@@ -282,7 +282,8 @@ int fa_server_results(fa_server* server, fa_server_result* out, int32_t max, int
    the server with `rounds` while frames are in flight or queued, sleeps until the next arrival
    otherwise, and returns when every frame's result is in: latency_s[i] = time of the step that
    returned result i - arrival_s[i], stats[i] its trust-region statistics. The frames must stay
-   untouched until the call returns. The server should hold no other frames. A frame that
+   untouched until the call returns. A server that holds other frames (queued, running or with
+   results not yet taken) is refused with FA_E_ARG, since results are matched by tag. A frame that
    cannot be submitted (fa_server_submit_host's errors) ends the stream: the frames submitted
    before it are run to their results and its error is returned; after a device error the
    server must not be stepped again. It replaces a

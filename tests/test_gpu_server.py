@@ -205,6 +205,37 @@ def test_run_stream_error_drains_the_server(ctx):
     r2._release_handle()
 
 
+def test_run_stream_refuses_a_server_holding_frames(ctx):
+    """The native stream matches results to its arrivals by tag, so a server that still holds
+    a frame submitted before the call is refused (FA_E_ARG) and keeps that frame, whose result
+    comes back through step as usual; the emptied server then takes the stream, and both
+    results equal the frames aligned alone."""
+    st = small_stack((1, 64, 64), n_frames=2, nD=9, seed=53)
+    r1 = F.Reference(ctx, st["ref_w"], 9)
+    srv = F.Server([r1], slots=4, **KW)
+    frames = [np.ascontiguousarray(f) for f in st["frames_w"]]
+    srv.submit_host(r1, frames[0], tag=7)
+    with pytest.raises(F.FaError) as e:
+        srv.run_stream([0.0], [r1], [frames[1]])
+    assert e.value.code == -1
+    got = []
+    for _ in range(2000):
+        a, q, out = srv.step(1)
+        got += out
+        if a == 0 and q == 0:
+            break
+    assert [t for t, _, _ in got] == [7]
+    lat, res, cnt = srv.run_stream([0.0], [r1], [frames[1]])
+    assert lat[0] >= 0 and cnt["steps"] > 0
+    for f, x in zip(frames, [got[0][1], res[0][0]]):
+        fr = r1.register(f)
+        xa, _ = fr.align(**KW)
+        fr.release()
+        assert np.array_equal(x, xa)
+    srv.close()
+    r1._release_handle()
+
+
 def test_server_errors(ctx):
     st = small_stack((1, 64, 64), n_frames=1, nD=9, seed=44)
     r1 = F.Reference(ctx, st["ref_w"], 9)
