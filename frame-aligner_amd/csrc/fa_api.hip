@@ -1472,6 +1472,16 @@ namespace fa {
 
 int set_error(int code, const std::string& msg) { return set_err(code, msg); }
 
+int fa_caught() noexcept
+{
+   try {
+      g_last_error = "host memory or threads exhausted";
+   } catch (...) {
+      g_last_error.clear();
+   }
+   return FA_E_OOM;
+}
+
 void batch_set_params(fa_batch* b, const fa_tr_params& p)
 {
    std::lock_guard<std::mutex> lb(b->mu);
@@ -1793,56 +1803,64 @@ void fa_tr_default(fa_tr_params* p)
 
 int fa_create(int device, fa_context** out)
 {
-   if (!out) return set_err(FA_E_ARG, "null output");
-   *out = nullptr;
-   int count = 0;
-   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+   try {
+      if (!out) return set_err(FA_E_ARG, "null output");
+      *out = nullptr;
+      int count = 0;
+      if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+         (void)hipGetLastError();
+         return set_err(FA_E_DEVICE, "No supported GPU");
+      }
+      hipDeviceProp_t prop;
+      HIPCHK(hipGetDeviceProperties(&prop, device));
+      if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+         return set_err(FA_E_DEVICE, std::string("No supported GPU (found ") + prop.gcnArchName + ")");
+      HIPCHK(hipSetDevice(device));
+      scratch_init(device);  // the allocation-free entry point's array on this GPU
+      auto* c = new fa_context();
+      c->device = device;
+      c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+      c->lds_per_block = (int)std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
+      // raises the dynamic-LDS limit of the kernels that use more than 64 KB where the runtime
+      // keeps one; launches report their own errors
+      c->lds_config = (int)configure_kernels(c->lds_per_block);
       (void)hipGetLastError();
-      return set_err(FA_E_DEVICE, "No supported GPU");
+      bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+      for (int l = 0; ok && l < fa_context::kLaneStreams; l++)
+         ok = hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking) == hipSuccess;
+      if (!ok) {
+         (void)hipGetLastError();
+         for (hipStream_t s : c->lane_stream)
+            if (s) (void)hipStreamDestroy(s);
+         if (c->stream) (void)hipStreamDestroy(c->stream);
+         delete c;
+         return set_err(FA_E_DEVICE, "stream creation failed");
+      }
+      *out = c;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   hipDeviceProp_t prop;
-   HIPCHK(hipGetDeviceProperties(&prop, device));
-   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-      return set_err(FA_E_DEVICE, std::string("No supported GPU (found ") + prop.gcnArchName + ")");
-   HIPCHK(hipSetDevice(device));
-   scratch_init(device);  // the allocation-free entry point's array on this GPU
-   auto* c = new fa_context();
-   c->device = device;
-   c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-   c->lds_per_block = (int)std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
-   // raises the dynamic-LDS limit of the kernels that use more than 64 KB where the runtime
-   // keeps one; launches report their own errors
-   c->lds_config = (int)configure_kernels(c->lds_per_block);
-   (void)hipGetLastError();
-   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
-   for (int l = 0; ok && l < fa_context::kLaneStreams; l++)
-      ok = hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking) == hipSuccess;
-   if (!ok) {
-      (void)hipGetLastError();
-      for (hipStream_t s : c->lane_stream)
-         if (s) (void)hipStreamDestroy(s);
-      if (c->stream) (void)hipStreamDestroy(c->stream);
-      delete c;
-      return set_err(FA_E_DEVICE, "stream creation failed");
-   }
-   *out = c;
-   return FA_OK;
 }
 
 int fa_destroy(fa_context* ctx)
 {
-   if (!ctx) return FA_OK;
-   if (ctx->live_refs > 0)
-      return set_err(FA_E_STATE, "context still has live references; release them first");
-   DeviceGuard g(ctx->device);
-   (void)hipStreamSynchronize(ctx->stream);
-   (void)hipStreamDestroy(ctx->stream);
-   for (hipStream_t s : ctx->lane_stream) {
-      (void)hipStreamSynchronize(s);
-      (void)hipStreamDestroy(s);
+   try {
+      if (!ctx) return FA_OK;
+      if (ctx->live_refs > 0)
+         return set_err(FA_E_STATE, "context still has live references; release them first");
+      DeviceGuard g(ctx->device);
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipStreamDestroy(ctx->stream);
+      for (hipStream_t s : ctx->lane_stream) {
+         (void)hipStreamSynchronize(s);
+         (void)hipStreamDestroy(s);
+      }
+      delete ctx;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   delete ctx;
-   return FA_OK;
 }
 
 void* fa_context_stream(fa_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
@@ -1850,849 +1868,997 @@ void* fa_context_stream(fa_context* ctx) { return ctx ? (void*)ctx->stream : nul
 int fa_set_reference(fa_context* ctx, const float* ref, int32_t nz, int32_t ny, int32_t nx,
                      int32_t nD, const fa_scan_params* scan, int32_t chunk_px, fa_reference** out)
 {
-   if (!ctx || !ref || !scan || !out) return set_err(FA_E_ARG, "null argument");
-   *out = nullptr;
-   if (nz < 1 || ny < 2 || nx < 2) return set_err(FA_E_ARG, "Reference must be three dimensional");
-   const int64_t n_px = (int64_t)nz * ny * nx;
-   // pixel indices are 32-bit, with room for the pass to step a few blocks past an item's end
-   if (n_px > (int64_t)INT32_MAX - 4 * kBlock) return set_err(FA_E_ARG, "volume too large");
-   if (nD < 2 || nD - 1 > 65535 || nD <= nz) return set_err(FA_E_ARG, "invalid number of nodes");
-   if (scan->bidirectional) return set_err(FA_E_ARG, "bidirectional scans are not supported");
-   std::lock_guard<std::mutex> lk(ctx->mu);
-   DeviceGuard g(ctx->device);
-   hipStream_t s = ctx->stream;
+   try {
+      if (!ctx || !ref || !scan || !out) return set_err(FA_E_ARG, "null argument");
+      *out = nullptr;
+      if (nz < 1 || ny < 2 || nx < 2) return set_err(FA_E_ARG, "Reference must be three dimensional");
+      const int64_t n_px = (int64_t)nz * ny * nx;
+      // pixel indices are 32-bit, with room for the pass to step a few blocks past an item's end
+      if (n_px > (int64_t)INT32_MAX - 4 * kBlock) return set_err(FA_E_ARG, "volume too large");
+      if (nD < 2 || nD - 1 > 65535 || nD <= nz) return set_err(FA_E_ARG, "invalid number of nodes");
+      if (scan->bidirectional) return set_err(FA_E_ARG, "bidirectional scans are not supported");
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      DeviceGuard g(ctx->device);
+      hipStream_t s = ctx->stream;
 
-   auto* r = new fa_reference();
-   r->ctx = ctx;
-   r->nz = nz;
-   r->ny = ny;
-   r->nx = nx;
-   r->nD = nD;
-   r->n_dim = nz > 1 ? 3 : 2;
-   r->n_px = n_px;
-   // Frames at least 768 wide whose width is a multiple of kStripWidth are cut into strips of
-   // kStripWidth columns: a work item then stages a narrow tile of many rows instead of a few
-   // whole rows (less staging per pixel, more pixels per item for the same LDS).
-   const bool strips = nx >= 3 * kStripWidth && nx % kStripWidth == 0;
-   r->item_w = strips ? kStripWidth : nx;
-   // default work item: strips 4096 pixels (16 rows); else 8 rows within 2048..8192 pixels
-   // (2-D), 2048 pixels or one row of a wider frame (3-D, which stages two planes per item)
-   r->chunk_px = chunk_px > 0 ? chunk_px
-                 : strips     ? 16 * kStripWidth
-                 : nz > 1     ? std::max(2048, nx)
-                              : (int)std::min<int64_t>(8192, std::max<int64_t>(2048, 8 * (int64_t)nx));
-   derive_scan(*scan, nz, ny, nx, nD, r->scan_host, &r->scan);
-   const int m = r->n_dim;
+      auto* r = new fa_reference();
+      r->ctx = ctx;
+      r->nz = nz;
+      r->ny = ny;
+      r->nx = nx;
+      r->nD = nD;
+      r->n_dim = nz > 1 ? 3 : 2;
+      r->n_px = n_px;
+      // Frames at least 768 wide whose width is a multiple of kStripWidth are cut into strips of
+      // kStripWidth columns: a work item then stages a narrow tile of many rows instead of a few
+      // whole rows (less staging per pixel, more pixels per item for the same LDS).
+      const bool strips = nx >= 3 * kStripWidth && nx % kStripWidth == 0;
+      r->item_w = strips ? kStripWidth : nx;
+      // default work item: strips 4096 pixels (16 rows); else 8 rows within 2048..8192 pixels
+      // (2-D), 2048 pixels or one row of a wider frame (3-D, which stages two planes per item)
+      r->chunk_px = chunk_px > 0 ? chunk_px
+                    : strips     ? 16 * kStripWidth
+                    : nz > 1     ? std::max(2048, nx)
+                                 : (int)std::min<int64_t>(8192, std::max<int64_t>(2048, 8 * (int64_t)nx));
+      derive_scan(*scan, nz, ny, nx, nD, r->scan_host, &r->scan);
+      const int m = r->n_dim;
 
-   float* d_T = nullptr;
-   uint16_t* d_Di = nullptr;
-   int32_t* d_begin = nullptr;
-   int32_t* d_flags = nullptr;
-   double* d_hpart = nullptr;
-   double* d_info = nullptr;
-   auto cleanup = [&]() {
-      dfree(d_T);
-      dfree(d_Di);
-      dfree(d_begin);
-      dfree(d_flags);
-      dfree(d_hpart);
-      dfree(d_info);
-   };
-   auto fail = [&](int code) {
-      cleanup();
-      fa_reference_release(r);
-      return code;
-   };
-   int rc;
-   if ((rc = dalloc(&d_T, n_px)) || (rc = dalloc(&d_Di, n_px)) || (rc = dalloc(&d_begin, nD)) ||
-       (rc = dalloc(&d_flags, 1)) || (rc = dalloc(&d_info, 2)) ||
-       (rc = dalloc(&r->d_rg4, (size_t)n_px)) ||
-       (m == 3 && (rc = dalloc(&r->d_df, (size_t)n_px))))
-      return fail(rc);
+      float* d_T = nullptr;
+      uint16_t* d_Di = nullptr;
+      int32_t* d_begin = nullptr;
+      int32_t* d_flags = nullptr;
+      double* d_hpart = nullptr;
+      double* d_info = nullptr;
+      auto cleanup = [&]() {
+         dfree(d_T);
+         dfree(d_Di);
+         dfree(d_begin);
+         dfree(d_flags);
+         dfree(d_hpart);
+         dfree(d_info);
+      };
+      auto fail = [&](int code) {
+         cleanup();
+         fa_reference_release(r);
+         return code;
+      };
+      int rc;
+      if ((rc = dalloc(&d_T, n_px)) || (rc = dalloc(&d_Di, n_px)) || (rc = dalloc(&d_begin, nD)) ||
+          (rc = dalloc(&d_flags, 1)) || (rc = dalloc(&d_info, 2)) ||
+          (rc = dalloc(&r->d_rg4, (size_t)n_px)) ||
+          (m == 3 && (rc = dalloc(&r->d_df, (size_t)n_px))))
+         return fail(rc);
 
-   // node map on the device, interval starts back to the host
-   std::vector<int32_t> begin(nD, INT32_MAX);
-   uint16_t last_i = 0;
-   int32_t flags = 0;
-   if (hipMemcpyAsync(d_T, ref, n_px * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess ||
-       hipMemcpyAsync(d_begin, begin.data(), nD * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
-       hipMemsetAsync(d_flags, 0, sizeof(int32_t), s) != hipSuccess ||
-       launch_node_map(r->scan, nx, ny, n_px, d_Di, nullptr, s) != hipSuccess ||
-       launch_interval_begin(d_Di, n_px, nD, d_begin, d_flags, s) != hipSuccess ||
-       hipMemcpyAsync(begin.data(), d_begin, nD * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-       hipMemcpyAsync(&last_i, d_Di + (n_px - 1), sizeof(uint16_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-       hipMemcpyAsync(&flags, d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-       hipStreamSynchronize(s) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(set_err(FA_E_DEVICE, "node map failed"));
-   }
-   if (flags & 2)
-      return fail(set_err(FA_E_ARG, "scan timing is not monotonic in scan order "
-                                    "(interframe_duration < frame duration)"));
-   if ((flags & 1) || last_i != nD - 2) return fail(set_err(FA_E_NODE_MAP, "Something went wrong"));
-
-   // D_range exactly as precomputeInterp leaves it (FrameWarper.cpp:269-283)
-   r->D_range.assign(2 * nD, 0);
-   int last = -1;
-   for (int i = 0; i <= nD - 2; i++) {
-      if (begin[i] == INT32_MAX) continue;
-      r->D_range[2 * i] = begin[i];
-      if (last >= 0) r->D_range[2 * last + 1] = begin[i] - 1;
-      last = i;
-   }
-   r->D_range[2 * last + 1] = (int32_t)(n_px - 1);
-
-   // work chunks: the part of an interval in one plane is cut into the fewest items of at
-   // most chunk_px pixels, of equal length (to one pixel), so no item is a short remainder
-   const int64_t plane = (int64_t)nx * ny;
-   r->ichunk.assign(nD, 0);
-   r->item_rows = 1;
-   for (int i = 0; i <= nD - 2; i++) {
-      r->ichunk[i] = (int32_t)r->chunks.size();
-      if (begin[i] == INT32_MAX) continue;
-      int64_t b0 = r->D_range[2 * i], e0 = r->D_range[2 * i + 1];
-      for (int64_t s0 = b0; s0 <= e0;) {
-         const int64_t s1 = std::min<int64_t>(e0, (s0 / plane + 1) * plane - 1);  // segment end
-         if (!strips) {
-            const int64_t len = s1 - s0 + 1, m = (len + r->chunk_px - 1) / r->chunk_px;
-            const int64_t base = len / m, extra = len % m;
-            int64_t p = s0;
-            for (int64_t t = 0; t < m; t++) {
-               const int64_t q = p + base + (t < extra ? 1 : 0) - 1;
-               r->chunks.push_back(Chunk{(int32_t)p, (int32_t)(q + 1), i, (int32_t)e0, 0, nx});
-               r->item_rows = std::max<int>(r->item_rows, (int)(q / nx - p / nx + 1));
-               p = q + 1;
-            }
-         } else {
-            // the segment's rows in the fewest equal groups of at most chunk_px / kStripWidth
-            // rows, each group cut into strips; an item runs from the strip's first pixel in
-            // the segment to its last
-            const int64_t ra = s0 / nx, rb = s1 / nx, nrow = rb - ra + 1;
-            const int64_t rmax = std::max<int64_t>(1, r->chunk_px / kStripWidth);
-            const int64_t m = (nrow + rmax - 1) / rmax, base = nrow / m, extra = nrow % m;
-            int64_t ga = ra;
-            for (int64_t t = 0; t < m; t++) {
-               const int64_t gb = ga + base + (t < extra ? 1 : 0) - 1;
-               for (int cx = 0; cx < nx; cx += kStripWidth) {
-                  // first pixel: in row ga unless the segment starts past the strip there
-                  int64_t p0 = ga * nx + cx;
-                  if (p0 + kStripWidth - 1 < s0) p0 = (ga + 1) * nx + cx;
-                  p0 = std::max(p0, s0);
-                  // last pixel: in row gb unless the segment ends before the strip there
-                  int64_t pl = gb * nx + cx + kStripWidth - 1;
-                  if (gb * nx + cx > s1) pl = (gb - 1) * nx + cx + kStripWidth - 1;
-                  pl = std::min(pl, s1);
-                  if (pl < p0) continue;
-                  r->chunks.push_back(Chunk{(int32_t)p0, (int32_t)(pl + 1), i, (int32_t)e0, cx, kStripWidth});
-                  r->item_rows = std::max<int>(r->item_rows, (int)(pl / nx - p0 / nx + 1));
-               }
-               ga = gb + 1;
-            }
-         }
-         s0 = s1 + 1;
+      // node map on the device, interval starts back to the host
+      std::vector<int32_t> begin(nD, INT32_MAX);
+      uint16_t last_i = 0;
+      int32_t flags = 0;
+      if (hipMemcpyAsync(d_T, ref, n_px * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess ||
+          hipMemcpyAsync(d_begin, begin.data(), nD * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
+          hipMemsetAsync(d_flags, 0, sizeof(int32_t), s) != hipSuccess ||
+          launch_node_map(r->scan, nx, ny, n_px, d_Di, nullptr, s) != hipSuccess ||
+          launch_interval_begin(d_Di, n_px, nD, d_begin, d_flags, s) != hipSuccess ||
+          hipMemcpyAsync(begin.data(), d_begin, nD * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(&last_i, d_Di + (n_px - 1), sizeof(uint16_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(&flags, d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+         (void)hipGetLastError();
+         return fail(set_err(FA_E_DEVICE, "node map failed"));
       }
-   }
-   r->ichunk[nD - 1] = (int32_t)r->chunks.size();
-   const int nc = (int)r->chunks.size();
-   const int mm = m * m;
-   if ((rc = dalloc(&r->d_chunks, nc)) || (rc = dalloc(&r->d_ichunk, nD)) ||
-       (rc = dalloc(&r->d_Hdiag, (size_t)nD * mm)) || (rc = dalloc(&r->d_Hsub, (size_t)nD * mm)) ||
-       (rc = dalloc(&r->d_L0, (size_t)3 * nD * mm)) ||
-       (rc = dalloc(&d_hpart, (size_t)nc * kHessStride)))
-      return fail(rc);
+      if (flags & 2)
+         return fail(set_err(FA_E_ARG, "scan timing is not monotonic in scan order "
+                                       "(interframe_duration < frame duration)"));
+      if ((flags & 1) || last_i != nD - 2) return fail(set_err(FA_E_NODE_MAP, "Something went wrong"));
 
-   RefDev& R = r->dev;
-   R.rg4 = r->d_rg4;
-   R.df = r->d_df;
-   R.chunks = r->d_chunks;
-   R.ichunk = r->d_ichunk;
-   R.Hdiag = r->d_Hdiag;
-   R.Hsub = r->d_Hsub;
-   R.L0 = r->d_L0;
-   R.scan = r->scan;
-   R.nx = nx;
-   R.ny = ny;
-   R.nz = nz;
-   R.nD = nD;
-   R.n_dim = m;
-   R.n_chunks = nc;
-   R.h_pd = 0;
-   R.eig_lo = 0;
-   R.n_px = n_px;
+      // D_range exactly as precomputeInterp leaves it (FrameWarper.cpp:269-283)
+      r->D_range.assign(2 * nD, 0);
+      int last = -1;
+      for (int i = 0; i <= nD - 2; i++) {
+         if (begin[i] == INT32_MAX) continue;
+         r->D_range[2 * i] = begin[i];
+         if (last >= 0) r->D_range[2 * last + 1] = begin[i] - 1;
+         last = i;
+      }
+      r->D_range[2 * last + 1] = (int32_t)(n_px - 1);
 
-   double info[2] = {0, 0};
-   r->Hdiag.assign((size_t)nD * mm, 0);
-   r->Hsub.assign((size_t)nD * mm, 0);
-   if (hipMemcpyAsync(r->d_chunks, r->chunks.data(), nc * sizeof(Chunk), hipMemcpyHostToDevice, s) != hipSuccess ||
-       hipMemcpyAsync(r->d_ichunk, r->ichunk.data(), nD * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
-       hipMemsetAsync(r->d_Hsub, 0, (size_t)nD * mm * sizeof(double), s) != hipSuccess ||
-       hipMemsetAsync(r->d_L0, 0, (size_t)3 * nD * mm * sizeof(double), s) != hipSuccess ||
-       launch_reference_record(d_T, nz, ny, nx, m, r->scan, r->d_rg4, r->d_df, s) != hipSuccess ||
-       launch_hessian(R, d_hpart, r->d_Hdiag, r->d_Hsub, r->d_L0, d_info, s) != hipSuccess ||
-       hipMemcpyAsync(info, d_info, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
-       hipMemcpyAsync(r->Hdiag.data(), r->d_Hdiag, (size_t)nD * mm * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
-       hipMemcpyAsync(r->Hsub.data(), r->d_Hsub, (size_t)nD * mm * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
-       hipStreamSynchronize(s) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(set_err(FA_E_DEVICE, "reference set-up failed"));
-   }
-   r->h_pd = info[0] != 0.0;
-   r->eig_lo = info[1];
-   R.h_pd = r->h_pd;
-   R.eig_lo = r->eig_lo;
-   // small problems: eigendecomposition of H for the trust-region subproblem
-   R.eig = 0;
-   R.q_lds = 0;
-   R.ev_min = 0;
-   const int nn = nD * m, mp = nn + (nn & 1);
-   if (nn <= kEigMax && (int64_t)(2 * mp * mp + mp) * (int64_t)sizeof(double) <= ctx->lds_per_block) {
-      if ((rc = dalloc(&r->d_Q, 2 * (size_t)nn * nn)) || (rc = dalloc(&r->d_ev, nn))) return fail(rc);  // Q, Q^T
-      R.Q = r->d_Q;
-      R.ev = r->d_ev;
-      if (launch_eigen(R, r->d_Q, r->d_ev, d_info, s) != hipSuccess ||
+      // work chunks: the part of an interval in one plane is cut into the fewest items of at
+      // most chunk_px pixels, of equal length (to one pixel), so no item is a short remainder
+      const int64_t plane = (int64_t)nx * ny;
+      r->ichunk.assign(nD, 0);
+      r->item_rows = 1;
+      for (int i = 0; i <= nD - 2; i++) {
+         r->ichunk[i] = (int32_t)r->chunks.size();
+         if (begin[i] == INT32_MAX) continue;
+         int64_t b0 = r->D_range[2 * i], e0 = r->D_range[2 * i + 1];
+         for (int64_t s0 = b0; s0 <= e0;) {
+            const int64_t s1 = std::min<int64_t>(e0, (s0 / plane + 1) * plane - 1);  // segment end
+            if (!strips) {
+               const int64_t len = s1 - s0 + 1, m = (len + r->chunk_px - 1) / r->chunk_px;
+               const int64_t base = len / m, extra = len % m;
+               int64_t p = s0;
+               for (int64_t t = 0; t < m; t++) {
+                  const int64_t q = p + base + (t < extra ? 1 : 0) - 1;
+                  r->chunks.push_back(Chunk{(int32_t)p, (int32_t)(q + 1), i, (int32_t)e0, 0, nx});
+                  r->item_rows = std::max<int>(r->item_rows, (int)(q / nx - p / nx + 1));
+                  p = q + 1;
+               }
+            } else {
+               // the segment's rows in the fewest equal groups of at most chunk_px / kStripWidth
+               // rows, each group cut into strips; an item runs from the strip's first pixel in
+               // the segment to its last
+               const int64_t ra = s0 / nx, rb = s1 / nx, nrow = rb - ra + 1;
+               const int64_t rmax = std::max<int64_t>(1, r->chunk_px / kStripWidth);
+               const int64_t m = (nrow + rmax - 1) / rmax, base = nrow / m, extra = nrow % m;
+               int64_t ga = ra;
+               for (int64_t t = 0; t < m; t++) {
+                  const int64_t gb = ga + base + (t < extra ? 1 : 0) - 1;
+                  for (int cx = 0; cx < nx; cx += kStripWidth) {
+                     // first pixel: in row ga unless the segment starts past the strip there
+                     int64_t p0 = ga * nx + cx;
+                     if (p0 + kStripWidth - 1 < s0) p0 = (ga + 1) * nx + cx;
+                     p0 = std::max(p0, s0);
+                     // last pixel: in row gb unless the segment ends before the strip there
+                     int64_t pl = gb * nx + cx + kStripWidth - 1;
+                     if (gb * nx + cx > s1) pl = (gb - 1) * nx + cx + kStripWidth - 1;
+                     pl = std::min(pl, s1);
+                     if (pl < p0) continue;
+                     r->chunks.push_back(Chunk{(int32_t)p0, (int32_t)(pl + 1), i, (int32_t)e0, cx, kStripWidth});
+                     r->item_rows = std::max<int>(r->item_rows, (int)(pl / nx - p0 / nx + 1));
+                  }
+                  ga = gb + 1;
+               }
+            }
+            s0 = s1 + 1;
+         }
+      }
+      r->ichunk[nD - 1] = (int32_t)r->chunks.size();
+      const int nc = (int)r->chunks.size();
+      const int mm = m * m;
+      if ((rc = dalloc(&r->d_chunks, nc)) || (rc = dalloc(&r->d_ichunk, nD)) ||
+          (rc = dalloc(&r->d_Hdiag, (size_t)nD * mm)) || (rc = dalloc(&r->d_Hsub, (size_t)nD * mm)) ||
+          (rc = dalloc(&r->d_L0, (size_t)3 * nD * mm)) ||
+          (rc = dalloc(&d_hpart, (size_t)nc * kHessStride)))
+         return fail(rc);
+
+      RefDev& R = r->dev;
+      R.rg4 = r->d_rg4;
+      R.df = r->d_df;
+      R.chunks = r->d_chunks;
+      R.ichunk = r->d_ichunk;
+      R.Hdiag = r->d_Hdiag;
+      R.Hsub = r->d_Hsub;
+      R.L0 = r->d_L0;
+      R.scan = r->scan;
+      R.nx = nx;
+      R.ny = ny;
+      R.nz = nz;
+      R.nD = nD;
+      R.n_dim = m;
+      R.n_chunks = nc;
+      R.h_pd = 0;
+      R.eig_lo = 0;
+      R.n_px = n_px;
+
+      double info[2] = {0, 0};
+      r->Hdiag.assign((size_t)nD * mm, 0);
+      r->Hsub.assign((size_t)nD * mm, 0);
+      if (hipMemcpyAsync(r->d_chunks, r->chunks.data(), nc * sizeof(Chunk), hipMemcpyHostToDevice, s) != hipSuccess ||
+          hipMemcpyAsync(r->d_ichunk, r->ichunk.data(), nD * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
+          hipMemsetAsync(r->d_Hsub, 0, (size_t)nD * mm * sizeof(double), s) != hipSuccess ||
+          hipMemsetAsync(r->d_L0, 0, (size_t)3 * nD * mm * sizeof(double), s) != hipSuccess ||
+          launch_reference_record(d_T, nz, ny, nx, m, r->scan, r->d_rg4, r->d_df, s) != hipSuccess ||
+          launch_hessian(R, d_hpart, r->d_Hdiag, r->d_Hsub, r->d_L0, d_info, s) != hipSuccess ||
           hipMemcpyAsync(info, d_info, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(r->Hdiag.data(), r->d_Hdiag, (size_t)nD * mm * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(r->Hsub.data(), r->d_Hsub, (size_t)nD * mm * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
           hipStreamSynchronize(s) != hipSuccess) {
          (void)hipGetLastError();
          return fail(set_err(FA_E_DEVICE, "reference set-up failed"));
       }
-      R.eig = info[0] != 0.0 ? 1 : 0;
-      R.ev_min = info[1];
-      R.q_lds = R.eig && tr_lds_full_base(r) + tr_lds_q(r) <= kTrFullDoubles ? 1 : 0;
-   } else if (nn >= kEigBatchMin && nn <= kEigBatchMax) {
-      large_eigen(r, s);  // else the cyclic-reduction subproblem
+      r->h_pd = info[0] != 0.0;
+      r->eig_lo = info[1];
+      R.h_pd = r->h_pd;
+      R.eig_lo = r->eig_lo;
+      // small problems: eigendecomposition of H for the trust-region subproblem
+      R.eig = 0;
+      R.q_lds = 0;
+      R.ev_min = 0;
+      const int nn = nD * m, mp = nn + (nn & 1);
+      if (nn <= kEigMax && (int64_t)(2 * mp * mp + mp) * (int64_t)sizeof(double) <= ctx->lds_per_block) {
+         if ((rc = dalloc(&r->d_Q, 2 * (size_t)nn * nn)) || (rc = dalloc(&r->d_ev, nn))) return fail(rc);  // Q, Q^T
+         R.Q = r->d_Q;
+         R.ev = r->d_ev;
+         if (launch_eigen(R, r->d_Q, r->d_ev, d_info, s) != hipSuccess ||
+             hipMemcpyAsync(info, d_info, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+             hipStreamSynchronize(s) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(set_err(FA_E_DEVICE, "reference set-up failed"));
+         }
+         R.eig = info[0] != 0.0 ? 1 : 0;
+         R.ev_min = info[1];
+         R.q_lds = R.eig && tr_lds_full_base(r) + tr_lds_q(r) <= kTrFullDoubles ? 1 : 0;
+      } else if (nn >= kEigBatchMin && nn <= kEigBatchMax) {
+         large_eigen(r, s);  // else the cyclic-reduction subproblem
+      }
+      cleanup();
+      ctx->live_refs++;
+      r->counted = true;
+      *out = r;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   cleanup();
-   ctx->live_refs++;
-   r->counted = true;
-   *out = r;
-   return FA_OK;
 }
 
 int fa_reference_release(fa_reference* r)
 {
-   if (!r) return FA_OK;
-   if (r->live_frames > 0)
-      return set_err(FA_E_STATE, "reference still has registered frames or servers; release them first");
-   DeviceGuard g(r->ctx->device);
-   if (r->counted) r->ctx->live_refs--;
-   dfree(r->d_rg4);
-   dfree(r->d_df);
-   dfree(r->d_chunks);
-   dfree(r->d_ichunk);
-   dfree(r->d_Hdiag);
-   dfree(r->d_Hsub);
-   dfree(r->d_L0);
-   dfree(r->d_Q);
-   dfree(r->d_ev);
-   delete r;
-   return FA_OK;
+   try {
+      if (!r) return FA_OK;
+      if (r->live_frames > 0)
+         return set_err(FA_E_STATE, "reference still has registered frames or servers; release them first");
+      DeviceGuard g(r->ctx->device);
+      if (r->counted) r->ctx->live_refs--;
+      dfree(r->d_rg4);
+      dfree(r->d_df);
+      dfree(r->d_chunks);
+      dfree(r->d_ichunk);
+      dfree(r->d_Hdiag);
+      dfree(r->d_Hsub);
+      dfree(r->d_L0);
+      dfree(r->d_Q);
+      dfree(r->d_ev);
+      delete r;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_reference_info(const fa_reference* r, int32_t* o)
 {
-   if (!r || !o) return set_err(FA_E_ARG, "null argument");
-   o[0] = r->nz;
-   o[1] = r->ny;
-   o[2] = r->nx;
-   o[3] = r->nD;
-   o[4] = r->n_dim;
-   o[5] = (int32_t)r->chunks.size();
-   return FA_OK;
+   try {
+      if (!r || !o) return set_err(FA_E_ARG, "null argument");
+      o[0] = r->nz;
+      o[1] = r->ny;
+      o[2] = r->nx;
+      o[3] = r->nD;
+      o[4] = r->n_dim;
+      o[5] = (int32_t)r->chunks.size();
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_get_hessian(const fa_reference* r, double* H)
 {
-   if (!r || !H) return set_err(FA_E_ARG, "null argument");
-   const int m = r->n_dim, n = r->nD * m, mm = m * m;
-   std::fill(H, H + (size_t)n * n, 0.0);
-   for (int k = 0; k < r->nD; k++)
-      for (int a = 0; a < m; a++)
-         for (int c = 0; c < m; c++) {
-            H[(size_t)(k * m + a) * n + k * m + c] = r->Hdiag[(size_t)k * mm + a * m + c];
-            if (k + 1 < r->nD) {
-               double v = r->Hsub[(size_t)k * mm + a * m + c];
-               H[(size_t)((k + 1) * m + a) * n + k * m + c] = v;
-               H[(size_t)(k * m + c) * n + (k + 1) * m + a] = v;
+   try {
+      if (!r || !H) return set_err(FA_E_ARG, "null argument");
+      const int m = r->n_dim, n = r->nD * m, mm = m * m;
+      std::fill(H, H + (size_t)n * n, 0.0);
+      for (int k = 0; k < r->nD; k++)
+         for (int a = 0; a < m; a++)
+            for (int c = 0; c < m; c++) {
+               H[(size_t)(k * m + a) * n + k * m + c] = r->Hdiag[(size_t)k * mm + a * m + c];
+               if (k + 1 < r->nD) {
+                  double v = r->Hsub[(size_t)k * mm + a * m + c];
+                  H[(size_t)((k + 1) * m + a) * n + k * m + c] = v;
+                  H[(size_t)(k * m + c) * n + (k + 1) * m + a] = v;
+               }
             }
-         }
-   return FA_OK;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_get_node_map(const fa_reference* r, uint16_t* Di, float* Df, int32_t* D_range)
 {
-   if (!r) return set_err(FA_E_ARG, "null argument");
-   if (D_range) std::memcpy(D_range, r->D_range.data(), r->D_range.size() * sizeof(int32_t));
-   if (!Di && !Df) return FA_OK;
-   std::lock_guard<std::mutex> lk(r->ctx->mu);
-   DeviceGuard g(r->ctx->device);
-   hipStream_t s = r->ctx->stream;
-   uint16_t* d_Di = nullptr;
-   float* d_Df = nullptr;
-   CHK(dalloc(&d_Di, r->n_px));
-   if (dalloc(&d_Df, r->n_px) != FA_OK) {
+   try {
+      if (!r) return set_err(FA_E_ARG, "null argument");
+      if (D_range) std::memcpy(D_range, r->D_range.data(), r->D_range.size() * sizeof(int32_t));
+      if (!Di && !Df) return FA_OK;
+      std::lock_guard<std::mutex> lk(r->ctx->mu);
+      DeviceGuard g(r->ctx->device);
+      hipStream_t s = r->ctx->stream;
+      uint16_t* d_Di = nullptr;
+      float* d_Df = nullptr;
+      CHK(dalloc(&d_Di, r->n_px));
+      if (dalloc(&d_Df, r->n_px) != FA_OK) {
+         dfree(d_Di);
+         return FA_E_OOM;
+      }
+      int rc = FA_OK;
+      if (launch_node_map(r->scan, r->nx, r->ny, r->n_px, d_Di, d_Df, s) != hipSuccess ||
+          (Di && hipMemcpyAsync(Di, d_Di, r->n_px * sizeof(uint16_t), hipMemcpyDeviceToHost, s) != hipSuccess) ||
+          (Df && hipMemcpyAsync(Df, d_Df, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess) ||
+          hipStreamSynchronize(s) != hipSuccess) {
+         (void)hipGetLastError();
+         rc = set_err(FA_E_DEVICE, "node map failed");
+      }
       dfree(d_Di);
-      return FA_E_OOM;
+      dfree(d_Df);
+      return rc;
+   } catch (...) {
+      return fa_caught();
    }
-   int rc = FA_OK;
-   if (launch_node_map(r->scan, r->nx, r->ny, r->n_px, d_Di, d_Df, s) != hipSuccess ||
-       (Di && hipMemcpyAsync(Di, d_Di, r->n_px * sizeof(uint16_t), hipMemcpyDeviceToHost, s) != hipSuccess) ||
-       (Df && hipMemcpyAsync(Df, d_Df, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess) ||
-       hipStreamSynchronize(s) != hipSuccess) {
-      (void)hipGetLastError();
-      rc = set_err(FA_E_DEVICE, "node map failed");
-   }
-   dfree(d_Di);
-   dfree(d_Df);
-   return rc;
 }
 
 int fa_get_gradient(const fa_reference* r, float* gx, float* gy, float* gz)
 {
-   if (!r) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(r->ctx->mu);
-   DeviceGuard g(r->ctx->device);
-   const int m = r->n_dim;
-   std::vector<float> rg((size_t)r->n_px * 4);
-   HIPCHK(hipMemcpyAsync(rg.data(), r->d_rg4, rg.size() * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
-   HIPCHK(hipStreamSynchronize(r->ctx->stream));
-   for (int64_t p = 0; p < r->n_px; p++) {
-      if (gx) gx[p] = rg[p * 4 + 1];
-      if (gy) gy[p] = rg[p * 4 + 2];
-      if (gz) gz[p] = m == 3 ? rg[p * 4 + 3] : 0.f;
+   try {
+      if (!r) return set_err(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(r->ctx->mu);
+      DeviceGuard g(r->ctx->device);
+      const int m = r->n_dim;
+      std::vector<float> rg((size_t)r->n_px * 4);
+      HIPCHK(hipMemcpyAsync(rg.data(), r->d_rg4, rg.size() * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
+      HIPCHK(hipStreamSynchronize(r->ctx->stream));
+      for (int64_t p = 0; p < r->n_px; p++) {
+         if (gx) gx[p] = rg[p * 4 + 1];
+         if (gy) gy[p] = rg[p * 4 + 2];
+         if (gz) gz[p] = m == 3 ? rg[p * 4 + 3] : 0.f;
+      }
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   return FA_OK;
 }
 
 int fa_register_frame(fa_reference* r, const void* data, int32_t dtype, float add_offset, fa_frame** out)
 {
-   if (!r || !data || !out) return set_err(FA_E_ARG, "null argument");
-   *out = nullptr;
-   if (dtype != FA_F32 && dtype != FA_U16) return set_err(FA_E_ARG, "unsupported frame dtype");
-   std::lock_guard<std::mutex> lk(r->ctx->mu);
-   DeviceGuard g(r->ctx->device);
-   auto* f = new fa_frame();
-   f->ref = r;
-   f->dtype = dtype;
-   f->offset = add_offset;
-   size_t bytes = (size_t)r->n_px * (dtype == FA_U16 ? 2 : 4);
-   char* d = nullptr;
-   int rc = dalloc(&d, bytes);
-   if (rc) {
-      delete f;
-      return rc;
+   try {
+      if (!r || !data || !out) return set_err(FA_E_ARG, "null argument");
+      *out = nullptr;
+      if (dtype != FA_F32 && dtype != FA_U16) return set_err(FA_E_ARG, "unsupported frame dtype");
+      std::lock_guard<std::mutex> lk(r->ctx->mu);
+      DeviceGuard g(r->ctx->device);
+      auto* f = new fa_frame();
+      f->ref = r;
+      f->dtype = dtype;
+      f->offset = add_offset;
+      size_t bytes = (size_t)r->n_px * (dtype == FA_U16 ? 2 : 4);
+      char* d = nullptr;
+      int rc = dalloc(&d, bytes);
+      if (rc) {
+         delete f;
+         return rc;
+      }
+      f->d_data = d;
+      if (hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, r->ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(r->ctx->stream) != hipSuccess) {
+         (void)hipGetLastError();
+         dfree(d);
+         delete f;
+         return set_err(FA_E_DEVICE, "frame upload failed");
+      }
+      r->live_frames++;
+      r->frames.push_back(f);
+      *out = f;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   f->d_data = d;
-   if (hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, r->ctx->stream) != hipSuccess ||
-       hipStreamSynchronize(r->ctx->stream) != hipSuccess) {
-      (void)hipGetLastError();
-      dfree(d);
-      delete f;
-      return set_err(FA_E_DEVICE, "frame upload failed");
-   }
-   r->live_frames++;
-   r->frames.push_back(f);
-   *out = f;
-   return FA_OK;
 }
 
 int fa_deregister_frame(fa_frame* f)
 {
-   if (!f) return FA_OK;
-   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
-   std::unique_lock<std::mutex> lf(f->mu);  // waits for a call on the frame still running
-   DeviceGuard g(f->ref->ctx->device);
-   delete f->single;
-   f->single = nullptr;
-   void* d = f->d_data;
-   if (f->own_data) dfree(d);
-   fa_reference* r = f->ref;
-   r->live_frames--;
-   r->frames.erase(std::remove(r->frames.begin(), r->frames.end(), f), r->frames.end());
-   lf.unlock();
-   delete f;
-   return FA_OK;
+   try {
+      if (!f) return FA_OK;
+      std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+      std::unique_lock<std::mutex> lf(f->mu);  // waits for a call on the frame still running
+      DeviceGuard g(f->ref->ctx->device);
+      delete f->single;
+      f->single = nullptr;
+      void* d = f->d_data;
+      if (f->own_data) dfree(d);
+      fa_reference* r = f->ref;
+      r->live_frames--;
+      r->frames.erase(std::remove(r->frames.begin(), r->frames.end(), f), r->frames.end());
+      lf.unlock();
+      delete f;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_clear_temp(fa_reference* r)
 {
-   if (!r) return set_err(FA_E_ARG, "null reference");
-   std::lock_guard<std::mutex> lk(r->ctx->mu);
-   DeviceGuard g(r->ctx->device);
-   for (fa_frame* f : r->frames) {
-      std::lock_guard<std::mutex> lf(f->mu);
-      delete f->single;  // rebuilt at the frame's next fa_align or single evaluation
-      f->single = nullptr;
+   try {
+      if (!r) return set_err(FA_E_ARG, "null reference");
+      std::lock_guard<std::mutex> lk(r->ctx->mu);
+      DeviceGuard g(r->ctx->device);
+      for (fa_frame* f : r->frames) {
+         std::lock_guard<std::mutex> lf(f->mu);
+         delete f->single;  // rebuilt at the frame's next fa_align or single evaluation
+         f->single = nullptr;
+      }
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   return FA_OK;
 }
 
 int fa_get_error(fa_frame* f, const double* D, double* err)
 {
-   if (!f || !D || !err) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->mu);
-   DeviceGuard g(f->ref->ctx->device);
-   fa_batch* b;
-   CHK(single_batch(f, &b));
-   return batch_eval(b, D, err, nullptr);
+   try {
+      if (!f || !D || !err) return set_err(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(f->mu);
+      DeviceGuard g(f->ref->ctx->device);
+      fa_batch* b;
+      CHK(single_batch(f, &b));
+      return batch_eval(b, D, err, nullptr);
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_get_jacobian(fa_frame* f, const double* D, double* jac, double* err)
 {
-   if (!f || !D || !jac) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->mu);
-   DeviceGuard g(f->ref->ctx->device);
-   fa_batch* b;
-   CHK(single_batch(f, &b));
-   double e = 0;
-   CHK(batch_eval(b, D, &e, jac));
-   if (err) *err = e;
-   return FA_OK;
+   try {
+      if (!f || !D || !jac) return set_err(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(f->mu);
+      DeviceGuard g(f->ref->ctx->device);
+      fa_batch* b;
+      CHK(single_batch(f, &b));
+      double e = 0;
+      CHK(batch_eval(b, D, &e, jac));
+      if (err) *err = e;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_warp_interpolated(fa_frame* f, const double* D, float* out, float invalid_value)
 {
-   if (!f || !D || !out) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->mu);
-   DeviceGuard g(f->ref->ctx->device);
-   fa_batch* b;
-   CHK(single_batch(f, &b));
-   fa_reference* r = f->ref;
-   hipStream_t s = b->stream;
-   if (!b->d_warp) {
-      CHK(dalloc(&b->d_warp, r->n_px));
-      b->warp_px = r->n_px;
+   try {
+      if (!f || !D || !out) return set_err(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(f->mu);
+      DeviceGuard g(f->ref->ctx->device);
+      fa_batch* b;
+      CHK(single_batch(f, &b));
+      fa_reference* r = f->ref;
+      hipStream_t s = b->stream;
+      if (!b->d_warp) {
+         CHK(dalloc(&b->d_warp, r->n_px));
+         b->warp_px = r->n_px;
+      }
+      FrameDev F;
+      F.data = f->d_data;
+      F.dtype = f->dtype;
+      F.offset = f->offset;
+      int grid = (int)std::min<int64_t>((int64_t)r->chunks.size(), (int64_t)r->ctx->n_cu * kBlocksPerCu);
+      HIPCHK(hipMemcpyAsync(b->d_D, D, 3 * (size_t)r->nD * sizeof(double), hipMemcpyHostToDevice, s));
+      HIPCHK(launch_warp(r->dev, F, b->d_D, b->d_warp, invalid_value, b->tune.lds_floats, b->tune.use_lds, grid, s));
+      HIPCHK(hipMemcpyAsync(out, b->d_warp, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   FrameDev F;
-   F.data = f->d_data;
-   F.dtype = f->dtype;
-   F.offset = f->offset;
-   int grid = (int)std::min<int64_t>((int64_t)r->chunks.size(), (int64_t)r->ctx->n_cu * kBlocksPerCu);
-   HIPCHK(hipMemcpyAsync(b->d_D, D, 3 * (size_t)r->nD * sizeof(double), hipMemcpyHostToDevice, s));
-   HIPCHK(launch_warp(r->dev, F, b->d_D, b->d_warp, invalid_value, b->tune.lds_floats, b->tune.use_lds, grid, s));
-   HIPCHK(hipMemcpyAsync(out, b->d_warp, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
-   HIPCHK(hipStreamSynchronize(s));
-   return FA_OK;
 }
 
 int fa_warp_image(fa_frame* f, const double* D, float* wimg, float* coverage, int32_t apply_offset)
 {
-   if (!f || !D || !wimg || !coverage) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->mu);
-   DeviceGuard g(f->ref->ctx->device);
-   fa_batch* b;
-   CHK(single_batch(f, &b));
-   fa_reference* r = f->ref;
-   hipStream_t s = b->stream;
-   if (!b->d_warp) {
-      CHK(dalloc(&b->d_warp, r->n_px));
-      b->warp_px = r->n_px;
+   try {
+      if (!f || !D || !wimg || !coverage) return set_err(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(f->mu);
+      DeviceGuard g(f->ref->ctx->device);
+      fa_batch* b;
+      CHK(single_batch(f, &b));
+      fa_reference* r = f->ref;
+      hipStream_t s = b->stream;
+      if (!b->d_warp) {
+         CHK(dalloc(&b->d_warp, r->n_px));
+         b->warp_px = r->n_px;
+      }
+      if (!b->d_cov) CHK(dalloc(&b->d_cov, r->n_px));
+      FrameDev F;
+      F.data = f->d_data;
+      F.dtype = f->dtype;
+      F.offset = f->offset;
+      int grid = (int)std::min<int64_t>((int64_t)r->chunks.size(), (int64_t)r->ctx->n_cu * kBlocksPerCu);
+      HIPCHK(hipMemcpyAsync(b->d_D, D, 3 * (size_t)r->nD * sizeof(double), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemsetAsync(b->d_warp, 0, r->n_px * sizeof(float), s));
+      HIPCHK(hipMemsetAsync(b->d_cov, 0, r->n_px * sizeof(float), s));
+      HIPCHK(launch_splat(r->dev, F, b->d_D, b->d_warp, b->d_cov, apply_offset ? f->offset : 0.f, grid, s));
+      HIPCHK(hipMemcpyAsync(wimg, b->d_warp, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(coverage, b->d_cov, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   if (!b->d_cov) CHK(dalloc(&b->d_cov, r->n_px));
-   FrameDev F;
-   F.data = f->d_data;
-   F.dtype = f->dtype;
-   F.offset = f->offset;
-   int grid = (int)std::min<int64_t>((int64_t)r->chunks.size(), (int64_t)r->ctx->n_cu * kBlocksPerCu);
-   HIPCHK(hipMemcpyAsync(b->d_D, D, 3 * (size_t)r->nD * sizeof(double), hipMemcpyHostToDevice, s));
-   HIPCHK(hipMemsetAsync(b->d_warp, 0, r->n_px * sizeof(float), s));
-   HIPCHK(hipMemsetAsync(b->d_cov, 0, r->n_px * sizeof(float), s));
-   HIPCHK(launch_splat(r->dev, F, b->d_D, b->d_warp, b->d_cov, apply_offset ? f->offset : 0.f, grid, s));
-   HIPCHK(hipMemcpyAsync(wimg, b->d_warp, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
-   HIPCHK(hipMemcpyAsync(coverage, b->d_cov, r->n_px * sizeof(float), hipMemcpyDeviceToHost, s));
-   HIPCHK(hipStreamSynchronize(s));
-   return FA_OK;
 }
 
 int fa_warp_point(const fa_reference* r, const double* D, int32_t x, int32_t y, int32_t z,
                   int32_t binning, double* out3)
 {
-   if (!r || !D || !out3) return set_err(FA_E_ARG, "null argument");
-   if (binning < 1) return set_err(FA_E_ARG, "spatial binning must be >= 1");
-   double factor = 1.0 / binning;
-   int xs = (int)(x / factor);
-   int ys = (int)(y / factor);
-   out3[0] = out3[1] = out3[2] = 0;
-   if (xs < 0 || xs >= r->nx || ys < 0 || ys >= r->ny || z < 0 || z >= r->nz) return FA_OK;
-   double t = (double)z * r->scan.interframe + (double)ys * r->scan.interline + (double)xs * r->scan.pixel;
-   double u = t / r->scan.stack * r->scan.nDm1;
-   double ip = std::trunc(u);
-   double f = (double)(float)(u - ip);
-   int i = (int)ip;
-   for (int c = 0; c < 3; c++) {
-      double a = (c < r->n_dim) ? D[3 * (i + 1) + c] : 0.0, b = (c < r->n_dim) ? D[3 * i + c] : 0.0;
-      out3[c] = (f * a + (1 - f) * b) * factor;
+   try {
+      if (!r || !D || !out3) return set_err(FA_E_ARG, "null argument");
+      if (binning < 1) return set_err(FA_E_ARG, "spatial binning must be >= 1");
+      double factor = 1.0 / binning;
+      int xs = (int)(x / factor);
+      int ys = (int)(y / factor);
+      out3[0] = out3[1] = out3[2] = 0;
+      if (xs < 0 || xs >= r->nx || ys < 0 || ys >= r->ny || z < 0 || z >= r->nz) return FA_OK;
+      double t = (double)z * r->scan.interframe + (double)ys * r->scan.interline + (double)xs * r->scan.pixel;
+      double u = t / r->scan.stack * r->scan.nDm1;
+      double ip = std::trunc(u);
+      double f = (double)(float)(u - ip);
+      int i = (int)ip;
+      for (int c = 0; c < 3; c++) {
+         double a = (c < r->n_dim) ? D[3 * (i + 1) + c] : 0.0, b = (c < r->n_dim) ? D[3 * i + c] : 0.0;
+         out3[c] = (f * a + (1 - f) * b) * factor;
+      }
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   return FA_OK;
 }
 
 int fa_align(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_align_stats* stats)
 {
-   if (!f || !x_inout) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->mu);
-   DeviceGuard g(f->ref->ctx->device);
-   fa_batch* b;
-   CHK(single_batch(f, &b));
-   if (prm) b->prm = *prm;
-   else fa_tr_default(&b->prm);
-   return batch_run(b, x_inout, stats, nullptr);
+   try {
+      if (!f || !x_inout) return set_err(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(f->mu);
+      DeviceGuard g(f->ref->ctx->device);
+      fa_batch* b;
+      CHK(single_batch(f, &b));
+      if (prm) b->prm = *prm;
+      else fa_tr_default(&b->prm);
+      return batch_run(b, x_inout, stats, nullptr);
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_align_noalloc(fa_frame* f, const fa_tr_params* prm, double* x_inout, fa_align_stats* stats)
 {
-   if (!f || !x_inout) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
-   DeviceGuard g(f->ref->ctx->device);
-   return align_noalloc(f, prm, x_inout, stats);
+   try {
+      if (!f || !x_inout) return set_err(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(f->ref->ctx->mu);
+      DeviceGuard g(f->ref->ctx->device);
+      return align_noalloc(f, prm, x_inout, stats);
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_batch_create(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, fa_batch** out)
 {
-   if (!out || !frames || n <= 0 || !frames[0] || !frames[0]->ref) return set_err(FA_E_ARG, "invalid batch");
-   fa_context* ctx = frames[0]->ref->ctx;
-   std::lock_guard<std::mutex> lk(ctx->mu);
-   DeviceGuard g(ctx->device);
-   return batch_create_split(frames, n, prm, out);
+   try {
+      if (!out || !frames || n <= 0 || !frames[0] || !frames[0]->ref) return set_err(FA_E_ARG, "invalid batch");
+      fa_context* ctx = frames[0]->ref->ctx;
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      DeviceGuard g(ctx->device);
+      return batch_create_split(frames, n, prm, out);
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_batch_destroy(fa_batch* b)
 {
-   if (!b) return FA_OK;
-   std::lock_guard<std::mutex> lk(b->ctx->mu);
-   DeviceGuard g(b->ctx->device);
-   delete b;
-   return FA_OK;
+   try {
+      if (!b) return FA_OK;
+      std::lock_guard<std::mutex> lk(b->ctx->mu);
+      DeviceGuard g(b->ctx->device);
+      delete b;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_batch_set_timing(fa_batch* b, int32_t enable)
 {
-   if (!b) return set_err(FA_E_ARG, "null batch");
-   b->timing = enable ? 1 : 0;
-   for (fa_batch* q : b->parts) q->timing = b->timing;
-   return FA_OK;
+   try {
+      if (!b) return set_err(FA_E_ARG, "null batch");
+      b->timing = enable ? 1 : 0;
+      for (fa_batch* q : b->parts) q->timing = b->timing;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_batch_set_trace(fa_batch* b, int32_t max_records)
 {
-   if (!b) return set_err(FA_E_ARG, "null batch");
-   if (max_records < 0 || max_records > 100000) return set_err(FA_E_ARG, "max_records out of range");
-   DeviceGuard g(b->ctx->device);
-   const int rec = kTraceFields + 3 * max_params(b);
-   if (b->parts.empty()) return leaf_set_trace(b, max_records, rec);
-   for (fa_batch* q : b->parts) CHK(leaf_set_trace(q, max_records, rec));
-   b->trace_cap = max_records;
-   b->trace_rec = max_records > 0 ? rec : 0;
-   return FA_OK;
+   try {
+      if (!b) return set_err(FA_E_ARG, "null batch");
+      if (max_records < 0 || max_records > 100000) return set_err(FA_E_ARG, "max_records out of range");
+      DeviceGuard g(b->ctx->device);
+      const int rec = kTraceFields + 3 * max_params(b);
+      if (b->parts.empty()) return leaf_set_trace(b, max_records, rec);
+      for (fa_batch* q : b->parts) CHK(leaf_set_trace(q, max_records, rec));
+      b->trace_cap = max_records;
+      b->trace_rec = max_records > 0 ? rec : 0;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_batch_get_trace(fa_batch* b, double* out)
 {
-   if (!b || !out) return set_err(FA_E_ARG, "null argument");
-   if (b->trace_cap <= 0) return set_err(FA_E_STATE, "tracing is off");
-   DeviceGuard g(b->ctx->device);
-   if (b->parts.empty()) return leaf_get_trace(b, out, 0, b->trace_cap, b->trace_rec);
-   for (size_t i = 0; i < b->parts.size(); i++)
-      CHK(leaf_get_trace(b->parts[i], out, b->part_first[i], b->trace_cap, b->trace_rec));
-   return FA_OK;
+   try {
+      if (!b || !out) return set_err(FA_E_ARG, "null argument");
+      if (b->trace_cap <= 0) return set_err(FA_E_STATE, "tracing is off");
+      DeviceGuard g(b->ctx->device);
+      if (b->parts.empty()) return leaf_get_trace(b, out, 0, b->trace_cap, b->trace_rec);
+      for (size_t i = 0; i < b->parts.size(); i++)
+         CHK(leaf_get_trace(b->parts[i], out, b->part_first[i], b->trace_cap, b->trace_rec));
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_batch_get_tuning(const fa_batch* b, fa_tuning* t)
 {
-   if (!b || !t) return set_err(FA_E_ARG, "null argument");
-   *t = b->tune;
-   const fa_batch* leaf = b->parts.empty() ? b : b->parts[0];
-   bool any = false;
-   for (int l : leaf->trc_lds) any = any || l > 0;
-   t->tr_lds = b->tune.tr_lds && any;
-   return FA_OK;
+   try {
+      if (!b || !t) return set_err(FA_E_ARG, "null argument");
+      *t = b->tune;
+      const fa_batch* leaf = b->parts.empty() ? b : b->parts[0];
+      bool any = false;
+      for (int l : leaf->trc_lds) any = any || l > 0;
+      t->tr_lds = b->tune.tr_lds && any;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_batch_set_tuning(fa_batch* b, const fa_tuning* t)
 {
-   if (!b || !t) return set_err(FA_E_ARG, "null argument");
-   if (t->lds_floats < 0 || t->lds_floats > 15360 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
-       t->check_every < 1 || t->check_every > 64 || t->tr_lds < 0 || t->tr_lds > 1)
-      return set_err(FA_E_ARG, "tuning out of range (lds_floats <= 15360, 1 <= blocks_per_cu <= 64)");
-   std::lock_guard<std::mutex> lk(b->mu);
-   b->tune = *t;
-   if (b->parts.empty()) {
-      set_grids(b);
-      b->drop_graph();
+   try {
+      if (!b || !t) return set_err(FA_E_ARG, "null argument");
+      if (t->lds_floats < 0 || t->lds_floats > 15360 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
+          t->check_every < 1 || t->check_every > 64 || t->tr_lds < 0 || t->tr_lds > 1)
+         return set_err(FA_E_ARG, "tuning out of range (lds_floats <= 15360, 1 <= blocks_per_cu <= 64)");
+      std::lock_guard<std::mutex> lk(b->mu);
+      b->tune = *t;
+      if (b->parts.empty()) {
+         set_grids(b);
+         b->drop_graph();
+      }
+      for (fa_batch* q : b->parts) {
+         q->tune = *t;
+         set_grids(q);
+         q->drop_graph();
+      }
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   for (fa_batch* q : b->parts) {
-      q->tune = *t;
-      set_grids(q);
-      q->drop_graph();
-   }
-   return FA_OK;
 }
 
 int fa_batch_run(fa_batch* b, double* x_inout, fa_align_stats* stats, fa_batch_timing* timing)
 {
-   if (!b || !x_inout) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(b->mu);
-   DeviceGuard g(b->ctx->device);
-   return batch_run(b, x_inout, stats, timing);
+   try {
+      if (!b || !x_inout) return set_err(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(b->mu);
+      DeviceGuard g(b->ctx->device);
+      return batch_run(b, x_inout, stats, timing);
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_batch_eval(fa_batch* b, const double* D, double* err, double* jac)
 {
-   if (!b || !D) return set_err(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(b->mu);
-   DeviceGuard g(b->ctx->device);
-   return batch_eval(b, D, err, jac);
+   try {
+      if (!b || !D) return set_err(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(b->mu);
+      DeviceGuard g(b->ctx->device);
+      return batch_eval(b, D, err, jac);
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_align_batch(fa_frame* const* frames, int32_t n, const fa_tr_params* prm, double* x_inout,
                    fa_align_stats* stats)
 {
-   fa_batch* b = nullptr;
-   CHK(fa_batch_create(frames, n, prm, &b));
-   int rc = fa_batch_run(b, x_inout, stats, nullptr);
-   fa_batch_destroy(b);
-   return rc;
+   try {
+      fa_batch* b = nullptr;
+      CHK(fa_batch_create(frames, n, prm, &b));
+      int rc = fa_batch_run(b, x_inout, stats, nullptr);
+      fa_batch_destroy(b);
+      return rc;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_context_info(const fa_context* ctx, int32_t* o)
 {
-   if (!ctx || !o) return set_err(FA_E_ARG, "null argument");
-   o[0] = ctx->device;
-   o[1] = ctx->n_cu;
-   o[2] = ctx->lds_per_block;
-   o[3] = 64;
-   o[4] = ctx->lds_config;
-   return FA_OK;
+   try {
+      if (!ctx || !o) return set_err(FA_E_ARG, "null argument");
+      o[0] = ctx->device;
+      o[1] = ctx->n_cu;
+      o[2] = ctx->lds_per_block;
+      o[3] = 64;
+      o[4] = ctx->lds_config;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_context_mem_info(fa_context* ctx, uint64_t* free_bytes, uint64_t* total_bytes)
 {
-   if (!ctx || !free_bytes || !total_bytes) return set_err(FA_E_ARG, "null argument");
-   DeviceGuard g(ctx->device);
-   size_t f = 0, t = 0;
-   HIPCHK(hipMemGetInfo(&f, &t));
-   *free_bytes = f;
-   *total_bytes = t;
-   return FA_OK;
+   try {
+      if (!ctx || !free_bytes || !total_bytes) return set_err(FA_E_ARG, "null argument");
+      DeviceGuard g(ctx->device);
+      size_t f = 0, t = 0;
+      HIPCHK(hipMemGetInfo(&f, &t));
+      *free_bytes = f;
+      *total_bytes = t;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_server_create(fa_context* ctx, fa_reference* const* refs, int32_t n_refs, int32_t slots,
                      uint64_t arena_bytes, const fa_tr_params* prm, fa_server** out)
 {
-   if (!ctx || !refs || n_refs <= 0 || slots <= 0 || !out) return set_err(FA_E_ARG, "invalid server arguments");
-   *out = nullptr;
-   for (int i = 0; i < n_refs; i++)
-      if (!refs[i] || refs[i]->ctx != ctx) return set_err(FA_E_ARG, "reference of another context");
-   std::lock_guard<std::mutex> lk(ctx->mu);
-   DeviceGuard g(ctx->device);
-   auto* sv = new fa_server();
-   sv->ctx = ctx;
-   size_t largest = 0;
-   for (int i = 0; i < n_refs; i++)
-      if (std::find(sv->refs.begin(), sv->refs.end(), refs[i]) == sv->refs.end()) sv->refs.push_back(refs[i]);
-   for (fa_reference* r : sv->refs) {
-      sv->nmax = std::max(sv->nmax, r->nD * r->n_dim);
-      largest = std::max(largest, fa_server::round_up(frame_bytes(r, FA_F32)));
-   }
-   // lanes: by kind, then pixels; a new lane where a reference has more than kLaneSpan times
-   // the pixels of its lane's first; past kMaxLanes, the largest references share the last
-   {
-      std::vector<fa_reference*> rs = sv->refs;
-      std::stable_sort(rs.begin(), rs.end(), [](const fa_reference* a, const fa_reference* c) {
-         if (a->n_dim != c->n_dim) return a->n_dim < c->n_dim;
-         return a->n_px < c->n_px;
-      });
-      for (fa_reference* r : rs) {
-         if (sv->lanes.empty() || ((int)sv->lanes.size() < kMaxLanes &&
-                                   (sv->lanes.back().refs.front()->n_dim != r->n_dim ||
-                                    r->n_px > kLaneSpan * sv->lanes.back().refs.front()->n_px)))
-            sv->lanes.emplace_back();
-         sv->lanes.back().refs.push_back(r);
+   try {
+      if (!ctx || !refs || n_refs <= 0 || slots <= 0 || !out) return set_err(FA_E_ARG, "invalid server arguments");
+      *out = nullptr;
+      for (int i = 0; i < n_refs; i++)
+         if (!refs[i] || refs[i]->ctx != ctx) return set_err(FA_E_ARG, "reference of another context");
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      DeviceGuard g(ctx->device);
+      auto* sv = new fa_server();
+      sv->ctx = ctx;
+      size_t largest = 0;
+      for (int i = 0; i < n_refs; i++)
+         if (std::find(sv->refs.begin(), sv->refs.end(), refs[i]) == sv->refs.end()) sv->refs.push_back(refs[i]);
+      for (fa_reference* r : sv->refs) {
+         sv->nmax = std::max(sv->nmax, r->nD * r->n_dim);
+         largest = std::max(largest, fa_server::round_up(frame_bytes(r, FA_F32)));
       }
-      sv->ref_lane.resize(sv->refs.size());
-      for (size_t i = 0; i < sv->refs.size(); i++)
-         for (int l = 0; l < (int)sv->lanes.size(); l++)
-            if (std::find(sv->lanes[l].refs.begin(), sv->lanes[l].refs.end(), sv->refs[i]) != sv->lanes[l].refs.end())
-               sv->ref_lane[i] = l;
-      for (auto& L : sv->lanes) order_refs(L.refs);  // the slot batch's launch order
-   }
-   sv->cap = slots;
-   // every lane can hold `slots` frames; a lone lane with enough slots runs as two parts
-   const int P1 = sv->lanes.size() == 1 && slots >= kServerSplitSlots ? std::min(kServerParts, slots / 8) : 1;
-   int rc = FA_OK;
-   for (int l = 0, j = 0; l < (int)sv->lanes.size() && rc == FA_OK; l++) {
-      fa_server::Lane& L = sv->lanes[l];
-      L.part_lo = (int)sv->parts.size();
-      for (int pi = 0; pi < P1 && rc == FA_OK; pi++) {
-         const int sp = slots / P1 + (pi < slots % P1 ? 1 : 0);
-         fa_batch* q = nullptr;
-         // part k of the server runs on the context's lane stream k
-         rc = slot_batch_build(ctx, L.refs, sp, prm, ctx->lane_stream[sv->parts.size() % fa_context::kLaneStreams], &q);
-         if (rc != FA_OK) break;
-         sv->parts.push_back(q);
-         sv->first.push_back(j);
-         sv->part_lane.push_back(l);
-         j += sp;
+      // lanes: by kind, then pixels; a new lane where a reference has more than kLaneSpan times
+      // the pixels of its lane's first; past kMaxLanes, the largest references share the last
+      {
+         std::vector<fa_reference*> rs = sv->refs;
+         std::stable_sort(rs.begin(), rs.end(), [](const fa_reference* a, const fa_reference* c) {
+            if (a->n_dim != c->n_dim) return a->n_dim < c->n_dim;
+            return a->n_px < c->n_px;
+         });
+         for (fa_reference* r : rs) {
+            if (sv->lanes.empty() || ((int)sv->lanes.size() < kMaxLanes &&
+                                      (sv->lanes.back().refs.front()->n_dim != r->n_dim ||
+                                       r->n_px > kLaneSpan * sv->lanes.back().refs.front()->n_px)))
+               sv->lanes.emplace_back();
+            sv->lanes.back().refs.push_back(r);
+         }
+         sv->ref_lane.resize(sv->refs.size());
+         for (size_t i = 0; i < sv->refs.size(); i++)
+            for (int l = 0; l < (int)sv->lanes.size(); l++)
+               if (std::find(sv->lanes[l].refs.begin(), sv->lanes[l].refs.end(), sv->refs[i]) != sv->lanes[l].refs.end())
+                  sv->ref_lane[i] = l;
+         for (auto& L : sv->lanes) order_refs(L.refs);  // the slot batch's launch order
       }
-      L.part_hi = (int)sv->parts.size();
-   }
-   sv->S = 0;
-   for (fa_batch* q : sv->parts) sv->S += q->n_slots;
-   // the frame arena: by default room for every slot's frame, within the device's free memory
-   // less a margin (GpuFrameWarper.cpp:113-130 sizes its frame count the same way)
-   if (rc == FA_OK) {
-      size_t free_b = 0, total_b = 0;
-      if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      sv->cap = slots;
+      // every lane can hold `slots` frames; a lone lane with enough slots runs as two parts
+      const int P1 = sv->lanes.size() == 1 && slots >= kServerSplitSlots ? std::min(kServerParts, slots / 8) : 1;
+      int rc = FA_OK;
+      for (int l = 0, j = 0; l < (int)sv->lanes.size() && rc == FA_OK; l++) {
+         fa_server::Lane& L = sv->lanes[l];
+         L.part_lo = (int)sv->parts.size();
+         for (int pi = 0; pi < P1 && rc == FA_OK; pi++) {
+            const int sp = slots / P1 + (pi < slots % P1 ? 1 : 0);
+            fa_batch* q = nullptr;
+            // part k of the server runs on the context's lane stream k
+            rc = slot_batch_build(ctx, L.refs, sp, prm, ctx->lane_stream[sv->parts.size() % fa_context::kLaneStreams], &q);
+            if (rc != FA_OK) break;
+            sv->parts.push_back(q);
+            sv->first.push_back(j);
+            sv->part_lane.push_back(l);
+            j += sp;
+         }
+         L.part_hi = (int)sv->parts.size();
+      }
+      sv->S = 0;
+      for (fa_batch* q : sv->parts) sv->S += q->n_slots;
+      // the frame arena: by default room for every slot's frame, within the device's free memory
+      // less a margin (GpuFrameWarper.cpp:113-130 sizes its frame count the same way)
+      if (rc == FA_OK) {
+         size_t free_b = 0, total_b = 0;
+         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            (void)hipGetLastError();
+            rc = set_err(FA_E_DEVICE, "hipMemGetInfo failed");
+         } else {
+            const size_t avail = free_b > kArenaMargin ? free_b - kArenaMargin : 0;
+            size_t want = arena_bytes ? fa_server::round_up(arena_bytes) : largest * (size_t)slots;
+            if (!arena_bytes) want = std::min(want, avail / 256 * 256);
+            if (want > avail || (!arena_bytes && want < largest))
+               rc = set_err(FA_E_OOM, "Not enough video memory to use GPU");
+            sv->arena_bytes = want;
+         }
+      }
+      const int S = sv->S;
+      if (rc == FA_OK)
+         if ((rc = halloc(&sv->h_bf, S)) || (rc = halloc(&sv->h_x, (size_t)S * sv->nmax)) ||
+             (rc = halloc(&sv->h_slots, S)) || (rc = halloc(&sv->h_state, S)) ||
+             (rc = halloc(&sv->h_xall, (size_t)S * sv->nmax)) || (rc = dalloc(&sv->d_arena, sv->arena_bytes))) {
+         }
+      if (rc == FA_OK &&
+          (hipHostGetDevicePointer((void**)&sv->dh_bf, sv->h_bf, 0) != hipSuccess ||
+           hipHostGetDevicePointer((void**)&sv->dh_x, sv->h_x, 0) != hipSuccess ||
+           hipHostGetDevicePointer((void**)&sv->dh_slots, sv->h_slots, 0) != hipSuccess ||
+           hipHostGetDevicePointer((void**)&sv->dh_state, sv->h_state, 0) != hipSuccess ||
+           hipHostGetDevicePointer((void**)&sv->dh_xall, sv->h_xall, 0) != hipSuccess)) {
          (void)hipGetLastError();
-         rc = set_err(FA_E_DEVICE, "hipMemGetInfo failed");
-      } else {
-         const size_t avail = free_b > kArenaMargin ? free_b - kArenaMargin : 0;
-         size_t want = arena_bytes ? fa_server::round_up(arena_bytes) : largest * (size_t)slots;
-         if (!arena_bytes) want = std::min(want, avail / 256 * 256);
-         if (want > avail || (!arena_bytes && want < largest))
-            rc = set_err(FA_E_OOM, "Not enough video memory to use GPU");
-         sv->arena_bytes = want;
+         rc = set_err(FA_E_DEVICE, "pinned staging is not mapped for the device");
       }
-   }
-   const int S = sv->S;
-   if (rc == FA_OK)
-      if ((rc = halloc(&sv->h_bf, S)) || (rc = halloc(&sv->h_x, (size_t)S * sv->nmax)) ||
-          (rc = halloc(&sv->h_slots, S)) || (rc = halloc(&sv->h_state, S)) ||
-          (rc = halloc(&sv->h_xall, (size_t)S * sv->nmax)) || (rc = dalloc(&sv->d_arena, sv->arena_bytes))) {
+      sv->cstream = ctx->stream;  // uploads: the context's stream, whose queue no lane shares
+      for (int j = 0; rc == FA_OK && j < S + (int)sv->parts.size(); j++) {
+         hipEvent_t e;
+         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            rc = set_err(FA_E_DEVICE, "event creation failed");
+            break;
+         }
+         if (j < S) sv->up_ev.push_back(e);
+         else sv->done_ev.push_back(e);
       }
-   if (rc == FA_OK &&
-       (hipHostGetDevicePointer((void**)&sv->dh_bf, sv->h_bf, 0) != hipSuccess ||
-        hipHostGetDevicePointer((void**)&sv->dh_x, sv->h_x, 0) != hipSuccess ||
-        hipHostGetDevicePointer((void**)&sv->dh_slots, sv->h_slots, 0) != hipSuccess ||
-        hipHostGetDevicePointer((void**)&sv->dh_state, sv->h_state, 0) != hipSuccess ||
-        hipHostGetDevicePointer((void**)&sv->dh_xall, sv->h_xall, 0) != hipSuccess)) {
-      (void)hipGetLastError();
-      rc = set_err(FA_E_DEVICE, "pinned staging is not mapped for the device");
-   }
-   sv->cstream = ctx->stream;  // uploads: the context's stream, whose queue no lane shares
-   for (int j = 0; rc == FA_OK && j < S + (int)sv->parts.size(); j++) {
-      hipEvent_t e;
-      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-         (void)hipGetLastError();
-         rc = set_err(FA_E_DEVICE, "event creation failed");
-         break;
+      if (rc != FA_OK) {
+         delete sv;
+         return rc;
       }
-      if (j < S) sv->up_ev.push_back(e);
-      else sv->done_ev.push_back(e);
+      sv->free_blocks[0] = sv->arena_bytes;
+      sv->slot_req.resize(S);
+      sv->sstate.assign(S, fa_server::kFree);
+      sv->blk_off.assign(S, 0);
+      sv->blk_size.assign(S, 0);
+      sv->launched.assign(sv->parts.size(), 0);
+      sv->free_slots.assign(sv->parts.size(), {});
+      for (int j = S - 1; j >= 0; j--) sv->free_slots[sv->part_of(j)].push_back(j);
+      for (fa_reference* r : sv->refs) r->live_frames++;  // the references outlive the server
+      *out = sv;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   if (rc != FA_OK) {
-      delete sv;
-      return rc;
-   }
-   sv->free_blocks[0] = sv->arena_bytes;
-   sv->slot_req.resize(S);
-   sv->sstate.assign(S, fa_server::kFree);
-   sv->blk_off.assign(S, 0);
-   sv->blk_size.assign(S, 0);
-   sv->launched.assign(sv->parts.size(), 0);
-   sv->free_slots.assign(sv->parts.size(), {});
-   for (int j = S - 1; j >= 0; j--) sv->free_slots[sv->part_of(j)].push_back(j);
-   for (fa_reference* r : sv->refs) r->live_frames++;  // the references outlive the server
-   *out = sv;
-   return FA_OK;
 }
 
 int fa_server_destroy(fa_server* sv)
 {
-   if (!sv) return FA_OK;
-   std::lock_guard<std::mutex> lk(sv->ctx->mu);
-   DeviceGuard g(sv->ctx->device);
-   for (fa_reference* r : sv->refs) r->live_frames--;
-   delete sv;
-   return FA_OK;
+   try {
+      if (!sv) return FA_OK;
+      std::lock_guard<std::mutex> lk(sv->ctx->mu);
+      DeviceGuard g(sv->ctx->device);
+      for (fa_reference* r : sv->refs) r->live_frames--;
+      delete sv;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_server_submit(fa_server* sv, fa_frame* frame, double* x_inout, int64_t tag)
 {
-   if (!sv || !frame || !x_inout) return set_err(FA_E_ARG, "null argument");
-   if (std::find(sv->refs.begin(), sv->refs.end(), frame->ref) == sv->refs.end())
-      return set_err(FA_E_ARG, "frame of a reference the server does not serve");
-   std::lock_guard<std::mutex> lk(sv->ctx->mu);
-   fa_server::Req r;
-   r.f = frame;
-   r.x = x_inout;
-   r.tag = tag;
-   r.seq = sv->next_seq++;
-   sv->lanes[sv->lane_of(frame->ref)].pending.push_back(r);
-   return FA_OK;
+   try {
+      if (!sv || !frame || !x_inout) return set_err(FA_E_ARG, "null argument");
+      if (std::find(sv->refs.begin(), sv->refs.end(), frame->ref) == sv->refs.end())
+         return set_err(FA_E_ARG, "frame of a reference the server does not serve");
+      std::lock_guard<std::mutex> lk(sv->ctx->mu);
+      fa_server::Req r;
+      r.f = frame;
+      r.x = x_inout;
+      r.tag = tag;
+      r.seq = sv->next_seq++;
+      sv->lanes[sv->lane_of(frame->ref)].pending.push_back(r);
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_server_submit_host(fa_server* sv, fa_reference* ref, const void* data, int32_t dtype,
                           float add_offset, double* x_inout, int64_t tag)
 {
-   if (!sv || !ref || !data || !x_inout) return set_err(FA_E_ARG, "null argument");
-   if (dtype != FA_F32 && dtype != FA_U16) return set_err(FA_E_ARG, "unsupported frame type");
-   if (std::find(sv->refs.begin(), sv->refs.end(), ref) == sv->refs.end())
-      return set_err(FA_E_ARG, "frame of a reference the server does not serve");
-   if (fa_server::round_up(frame_bytes(ref, dtype)) > sv->arena_bytes)
-      return set_err(FA_E_OOM, "frame larger than the server's frame memory");
-   std::lock_guard<std::mutex> lk(sv->ctx->mu);
-   fa_server::Req r;
-   r.ref = ref;
-   r.host = data;
-   r.dtype = dtype;
-   r.offset = add_offset;
-   r.x = x_inout;
-   r.tag = tag;
-   r.seq = sv->next_seq++;
-   sv->lanes[sv->lane_of(ref)].pending.push_back(r);
-   return FA_OK;
+   try {
+      if (!sv || !ref || !data || !x_inout) return set_err(FA_E_ARG, "null argument");
+      if (dtype != FA_F32 && dtype != FA_U16) return set_err(FA_E_ARG, "unsupported frame type");
+      if (std::find(sv->refs.begin(), sv->refs.end(), ref) == sv->refs.end())
+         return set_err(FA_E_ARG, "frame of a reference the server does not serve");
+      if (fa_server::round_up(frame_bytes(ref, dtype)) > sv->arena_bytes)
+         return set_err(FA_E_OOM, "frame larger than the server's frame memory");
+      std::lock_guard<std::mutex> lk(sv->ctx->mu);
+      fa_server::Req r;
+      r.ref = ref;
+      r.host = data;
+      r.dtype = dtype;
+      r.offset = add_offset;
+      r.x = x_inout;
+      r.tag = tag;
+      r.seq = sv->next_seq++;
+      sv->lanes[sv->lane_of(ref)].pending.push_back(r);
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_server_step(fa_server* sv, int32_t rounds, int32_t* in_flight, int32_t* queued)
 {
-   if (!sv || rounds < 1) return set_err(FA_E_ARG, "invalid argument");
-   std::lock_guard<std::mutex> lk(sv->ctx->mu);
-   DeviceGuard g(sv->ctx->device);
-   CHK(server_step(sv, rounds));
-   if (in_flight) {
-      int n = 0;
-      for (char c : sv->sstate) n += c != fa_server::kFree;
-      *in_flight = n;
+   try {
+      if (!sv || rounds < 1) return set_err(FA_E_ARG, "invalid argument");
+      std::lock_guard<std::mutex> lk(sv->ctx->mu);
+      DeviceGuard g(sv->ctx->device);
+      CHK(server_step(sv, rounds));
+      if (in_flight) {
+         int n = 0;
+         for (char c : sv->sstate) n += c != fa_server::kFree;
+         *in_flight = n;
+      }
+      if (queued) *queued = (int32_t)sv->queued();
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   if (queued) *queued = (int32_t)sv->queued();
-   return FA_OK;
 }
 
 int fa_server_memory(const fa_server* sv, uint64_t* used, uint64_t* total)
 {
-   if (!sv || !used || !total) return set_err(FA_E_ARG, "null argument");
-   *used = sv->arena_used;
-   *total = sv->arena_bytes;
-   return FA_OK;
+   try {
+      if (!sv || !used || !total) return set_err(FA_E_ARG, "null argument");
+      *used = sv->arena_used;
+      *total = sv->arena_bytes;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_host_alloc(uint64_t bytes, void** out)
 {
-   if (!out) return set_err(FA_E_ARG, "null argument");
-   *out = nullptr;
-   if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
-      (void)hipGetLastError();
+   try {
+      if (!out) return set_err(FA_E_ARG, "null argument");
       *out = nullptr;
-      return set_err(FA_E_OOM, "Not enough memory to allocate");
+      if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+         (void)hipGetLastError();
+         *out = nullptr;
+         return set_err(FA_E_OOM, "Not enough memory to allocate");
+      }
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   return FA_OK;
 }
 
 int fa_host_free(void* p)
 {
-   if (p) HIPCHK(hipHostFree(p));
-   return FA_OK;
+   try {
+      if (p) HIPCHK(hipHostFree(p));
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_server_results(fa_server* sv, fa_server_result* out, int32_t max, int32_t* n_out)
 {
-   if (!sv || !n_out || (max > 0 && !out)) return set_err(FA_E_ARG, "null argument");
-   int n = 0;
-   while (n < max && !sv->results.empty()) {
-      out[n++] = sv->results.front();
-      sv->results.pop_front();
+   try {
+      if (!sv || !n_out || (max > 0 && !out)) return set_err(FA_E_ARG, "null argument");
+      int n = 0;
+      while (n < max && !sv->results.empty()) {
+         out[n++] = sv->results.front();
+         sv->results.pop_front();
+      }
+      *n_out = n;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   *n_out = n;
-   return FA_OK;
 }
 
 int fa_server_run_stream(fa_server* sv, int64_t n, const double* arrival_s, fa_reference* const* refs,
@@ -2700,69 +2866,73 @@ int fa_server_run_stream(fa_server* sv, int64_t n, const double* arrival_s, fa_r
                          double* x_inout, const int64_t* x_offset, int32_t rounds, double* latency_s,
                          fa_align_stats* stats, fa_stream_counts* counts)
 {
-   if (!sv || n < 0 || rounds < 1 || (n > 0 && (!arrival_s || !refs || !data || !dtype || !add_offset ||
-                                                !x_inout || !x_offset || !latency_s || !stats)))
-      return set_err(FA_E_ARG, "invalid argument");
-   for (int64_t i = 1; i < n; i++)
-      if (arrival_s[i] < arrival_s[i - 1]) return set_err(FA_E_ARG, "arrivals out of order");
-   // results are matched to arrivals by tag, so frames submitted before the call (queued,
-   // admitted or with results not yet taken) would be taken for arrivals of this stream
-   if (sv->admitted > 0 || sv->queued() > 0 || !sv->results.empty())
-      return set_err(FA_E_ARG, "the server holds frames submitted before the stream");
-   using clock = std::chrono::steady_clock;
-   const auto t0 = clock::now();
-   auto since = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
-   fa_stream_counts c{};
-   std::vector<fa_server_result> buf(256);
-   int64_t next = 0, done = 0, busy = 0;
-   // a frame that cannot be submitted ends the stream: the frames already submitted are run
-   // to their results (the server then holds nothing that points into the caller's arrays),
-   // and its error is returned
-   int stop_rc = FA_OK;
-   std::string stop_msg;
-   while (done < next || (stop_rc == FA_OK && done < n)) {
-      const double now = since();
-      while (stop_rc == FA_OK && next < n && arrival_s[next] <= now) {
-         const int rc = fa_server_submit_host(sv, refs[next], data[next], dtype[next], add_offset[next],
-                                              x_inout + x_offset[next], next);
-         if (rc != FA_OK) {
-            stop_rc = rc;
-            stop_msg = fa_last_error();
-            break;
+   try {
+      if (!sv || n < 0 || rounds < 1 || (n > 0 && (!arrival_s || !refs || !data || !dtype || !add_offset ||
+                                                   !x_inout || !x_offset || !latency_s || !stats)))
+         return set_err(FA_E_ARG, "invalid argument");
+      for (int64_t i = 1; i < n; i++)
+         if (arrival_s[i] < arrival_s[i - 1]) return set_err(FA_E_ARG, "arrivals out of order");
+      // results are matched to arrivals by tag, so frames submitted before the call (queued,
+      // admitted or with results not yet taken) would be taken for arrivals of this stream
+      if (sv->admitted > 0 || sv->queued() > 0 || !sv->results.empty())
+         return set_err(FA_E_ARG, "the server holds frames submitted before the stream");
+      using clock = std::chrono::steady_clock;
+      const auto t0 = clock::now();
+      auto since = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
+      fa_stream_counts c{};
+      std::vector<fa_server_result> buf(256);
+      int64_t next = 0, done = 0, busy = 0;
+      // a frame that cannot be submitted ends the stream: the frames already submitted are run
+      // to their results (the server then holds nothing that points into the caller's arrays),
+      // and its error is returned
+      int stop_rc = FA_OK;
+      std::string stop_msg;
+      while (done < next || (stop_rc == FA_OK && done < n)) {
+         const double now = since();
+         while (stop_rc == FA_OK && next < n && arrival_s[next] <= now) {
+            const int rc = fa_server_submit_host(sv, refs[next], data[next], dtype[next], add_offset[next],
+                                                 x_inout + x_offset[next], next);
+            if (rc != FA_OK) {
+               stop_rc = rc;
+               stop_msg = fa_last_error();
+               break;
+            }
+            next++;
+            busy++;
          }
-         next++;
-         busy++;
-      }
-      if (busy == 0) {
-         if (stop_rc != FA_OK) break;
-         // nothing submitted and not done: wait for the next arrival
-         std::this_thread::sleep_for(std::chrono::duration<double>(std::max(0.0, arrival_s[next] - since())));
-         continue;
-      }
-      int32_t in_flight = 0, queued = 0;
-      CHK(fa_server_step(sv, rounds, &in_flight, &queued));
-      c.steps++;
-      c.max_in_flight = std::max<int64_t>(c.max_in_flight, in_flight);
-      c.max_queued = std::max<int64_t>(c.max_queued, queued);
-      for (;;) {
-         int32_t got = 0;
-         CHK(fa_server_results(sv, buf.data(), (int32_t)buf.size(), &got));
-         const double t = since();
-         for (int32_t k = 0; k < got; k++) {
-            const int64_t i = buf[k].tag;
-            if (i < 0 || i >= next) return set_err(FA_E_STATE, "result of a frame not in the stream");
-            latency_s[i] = t - arrival_s[i];
-            stats[i] = buf[k].stats;
-            done++;
-            busy--;
+         if (busy == 0) {
+            if (stop_rc != FA_OK) break;
+            // nothing submitted and not done: wait for the next arrival
+            std::this_thread::sleep_for(std::chrono::duration<double>(std::max(0.0, arrival_s[next] - since())));
+            continue;
          }
-         if (got < (int32_t)buf.size()) break;
+         int32_t in_flight = 0, queued = 0;
+         CHK(fa_server_step(sv, rounds, &in_flight, &queued));
+         c.steps++;
+         c.max_in_flight = std::max<int64_t>(c.max_in_flight, in_flight);
+         c.max_queued = std::max<int64_t>(c.max_queued, queued);
+         for (;;) {
+            int32_t got = 0;
+            CHK(fa_server_results(sv, buf.data(), (int32_t)buf.size(), &got));
+            const double t = since();
+            for (int32_t k = 0; k < got; k++) {
+               const int64_t i = buf[k].tag;
+               if (i < 0 || i >= next) return set_err(FA_E_STATE, "result of a frame not in the stream");
+               latency_s[i] = t - arrival_s[i];
+               stats[i] = buf[k].stats;
+               done++;
+               busy--;
+            }
+            if (got < (int32_t)buf.size()) break;
+         }
       }
+      c.wall_s = since();
+      if (counts) *counts = c;
+      if (stop_rc != FA_OK) return set_err(stop_rc, stop_msg);
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   c.wall_s = since();
-   if (counts) *counts = c;
-   if (stop_rc != FA_OK) return set_err(stop_rc, stop_msg);
-   return FA_OK;
 }
 
 }  // extern "C"

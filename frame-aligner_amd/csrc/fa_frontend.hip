@@ -62,9 +62,19 @@ void par_copy(const std::vector<CopyPiece>& parts)
          if (at >= hi) break;
       }
    };
+   // a thread that cannot be started (no exception may leave the C interface) leaves its share
+   // and the ones after it to this thread
    std::vector<std::thread> th;
-   th.reserve(T - 1);
-   for (int t = 1; t < T; t++) th.emplace_back(run, t);
+   int started = 1;
+   try {
+      th.reserve(T - 1);
+      for (int t = 1; t < T; t++) {
+         th.emplace_back(run, t);
+         started++;
+      }
+   } catch (...) {
+   }
+   for (int t = started; t < T; t++) run(t);
    run(0);
    for (auto& x : th) x.join();
 }
@@ -906,352 +916,392 @@ extern "C" {
 int fa_aligner_create(fa_context* ctx, const fa_realign_params* rp, const fa_scan_params* scan,
                       const void* reference, int32_t dtype, fa_aligner** out)
 {
-   if (!ctx || !rp || !scan || !reference || !out) return set_error(FA_E_ARG, "null argument");
-   *out = nullptr;
-   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
-   if (rp->type < FA_REALIGN_NONE || rp->type > FA_REALIGN_WARP || rp->spatial_binning < 1 ||
-       rp->n_resampling_points < 1 || !(rp->smoothing >= 0))
-      return set_error(FA_E_ARG, "invalid realignment parameters");
-   if (scan->n_x < 1 || scan->n_y < 1 || scan->n_z < 1) return set_error(FA_E_ARG, "Reference must be three dimensional");
-   if (hipSetDevice(ctx->device) != hipSuccess) {
-      (void)hipGetLastError();
-      return set_error(FA_E_DEVICE, "No supported GPU");
+   try {
+      if (!ctx || !rp || !scan || !reference || !out) return set_error(FA_E_ARG, "null argument");
+      *out = nullptr;
+      if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+      if (rp->type < FA_REALIGN_NONE || rp->type > FA_REALIGN_WARP || rp->spatial_binning < 1 ||
+          rp->n_resampling_points < 1 || !(rp->smoothing >= 0))
+         return set_error(FA_E_ARG, "invalid realignment parameters");
+      if (scan->n_x < 1 || scan->n_y < 1 || scan->n_z < 1) return set_error(FA_E_ARG, "Reference must be three dimensional");
+      if (hipSetDevice(ctx->device) != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(FA_E_DEVICE, "No supported GPU");
+      }
+      auto* a = new fa_aligner();
+      a->ctx = ctx;
+      a->rp = *rp;
+      a->scan = *scan;
+      a->nz = scan->n_z;
+      a->ny = scan->n_y;
+      a->nx = scan->n_x;
+      a->n_px = (int64_t)a->nz * a->ny * a->nx;
+      a->n_dim = a->nz > 1 ? 3 : 2;
+      a->nD = rp->n_resampling_points * a->nz + 1;
+      const int rc = aligner_init(a, reference, dtype);
+      if (rc != FA_OK) {
+         delete a;
+         return rc;
+      }
+      *out = a;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
    }
-   auto* a = new fa_aligner();
-   a->ctx = ctx;
-   a->rp = *rp;
-   a->scan = *scan;
-   a->nz = scan->n_z;
-   a->ny = scan->n_y;
-   a->nx = scan->n_x;
-   a->n_px = (int64_t)a->nz * a->ny * a->nx;
-   a->n_dim = a->nz > 1 ? 3 : 2;
-   a->nD = rp->n_resampling_points * a->nz + 1;
-   const int rc = aligner_init(a, reference, dtype);
-   if (rc != FA_OK) {
-      delete a;
-      return rc;
-   }
-   *out = a;
-   return FA_OK;
 }
 
 int fa_aligner_destroy(fa_aligner* a)
 {
-   if (!a) return FA_OK;
-   (void)hipSetDevice(a->ctx->device);
-   delete a;
-   return FA_OK;
+   try {
+      if (!a) return FA_OK;
+      (void)hipSetDevice(a->ctx->device);
+      delete a;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_aligner_clear_temp(fa_aligner* a)
 {
-   if (!a) return set_error(FA_E_ARG, "null argument");
-   std::lock_guard<std::mutex> lk(a->mu);
-   FCHK(hipSetDevice(a->ctx->device));
-   FCHK(hipStreamSynchronize(a->stream));
-   a->drop_cached();
-   for (auto& b : a->bufs) {
-      if (b.p) (void)hipFree(b.p);
-      b.p = nullptr;
-      b.bytes = 0;
+   try {
+      if (!a) return set_error(FA_E_ARG, "null argument");
+      std::lock_guard<std::mutex> lk(a->mu);
+      FCHK(hipSetDevice(a->ctx->device));
+      FCHK(hipStreamSynchronize(a->stream));
+      a->drop_cached();
+      for (auto& b : a->bufs) {
+         if (b.p) (void)hipFree(b.p);
+         b.p = nullptr;
+         b.bytes = 0;
+      }
+      a->free_host();
+      return fa_clear_temp(a->ref);
+   } catch (...) {
+      return fa_caught();
    }
-   a->free_host();
-   return fa_clear_temp(a->ref);
 }
 
 int fa_aligner_reference(fa_aligner* a, fa_reference** ref, int32_t* nD)
 {
-   if (!a) return set_error(FA_E_ARG, "null argument");
-   if (ref) *ref = a->ref;
-   if (nD) *nD = a->nD;
-   return FA_OK;
+   try {
+      if (!a) return set_error(FA_E_ARG, "null argument");
+      if (ref) *ref = a->ref;
+      if (nD) *nD = a->nD;
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_aligner_preprocess(fa_aligner* a, const void* raw, int32_t dtype, float* out)
 {
-   if (!a || !raw || !out) return set_error(FA_E_ARG, "null argument");
-   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
-   std::lock_guard<std::mutex> lk(a->mu);
-   FCHK(hipSetDevice(a->ctx->device));
-   OK(a->upload(raw, dtype));
-   OK(a->smooth_plus_one(a->d_raw));  // in place (the blur passes through d_tmp)
-   FCHK(hipMemcpyAsync(out, a->d_raw, a->n_px * sizeof(float), hipMemcpyDeviceToHost, a->stream));
-   FCHK(hipStreamSynchronize(a->stream));
-   return FA_OK;
+   try {
+      if (!a || !raw || !out) return set_error(FA_E_ARG, "null argument");
+      if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+      std::lock_guard<std::mutex> lk(a->mu);
+      FCHK(hipSetDevice(a->ctx->device));
+      OK(a->upload(raw, dtype));
+      OK(a->smooth_plus_one(a->d_raw));  // in place (the blur passes through d_tmp)
+      FCHK(hipMemcpyAsync(out, a->d_raw, a->n_px * sizeof(float), hipMemcpyDeviceToHost, a->stream));
+      FCHK(hipStreamSynchronize(a->stream));
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_aligner_rigid_shift(fa_aligner* a, const void* raw, int32_t dtype, double* shift3)
 {
-   if (!a || !raw || !shift3) return set_error(FA_E_ARG, "null argument");
-   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
-   std::lock_guard<std::mutex> lk(a->mu);
-   FCHK(hipSetDevice(a->ctx->device));
-   OK(a->upload(raw, dtype));
-   return a->shift(shift3);
+   try {
+      if (!a || !raw || !shift3) return set_error(FA_E_ARG, "null argument");
+      if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+      std::lock_guard<std::mutex> lk(a->mu);
+      FCHK(hipSetDevice(a->ctx->device));
+      OK(a->upload(raw, dtype));
+      return a->shift(shift3);
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_aligner_add_frames(fa_aligner* a, const void* const* frames, int32_t n, int32_t dtype,
                           const fa_tr_params* prm, double* D_out, const fa_add_outputs* outputs,
                           fa_add_result* results)
 {
-   if (!a || !frames || n <= 0 || !D_out) return set_error(FA_E_ARG, "null argument");
-   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
-   for (int i = 0; i < n; i++)
-      if (!frames[i]) return set_error(FA_E_ARG, "null frame");
-   std::lock_guard<std::mutex> lk(a->mu);
-   FCHK(hipSetDevice(a->ctx->device));
-   const int nD = a->nD, m = a->n_dim, np = nD * m;
-   const int64_t npx = a->n_px;
-   const size_t rb = a->raw_bytes(dtype);
-   const int C = a->chunk_frames(n);
-   std::vector<double> shifts(3 * (size_t)n, 0.0);
-   // the call's device arrays (kept in the aligner between calls): raw uploads, raw values as
-   // float, the warper's frames (blur + 1, registered without a copy), and the rigid shift's
-   // work arrays for C frames
-   uint8_t* d_up = nullptr;
-   float *d_rawf = nullptr, *d_pre = nullptr, *d_btmp = nullptr;
-   double *w_real = nullptr, *w_pv = nullptr;
-   double2* w_spec = nullptr;
-   int64_t *w_pi = nullptr, *d_idx = nullptr;
-   // on an error the cached frames and batch are dropped (rebuilt by the next call)
-   auto release = [&]() {
-      (void)hipStreamSynchronize(a->stream);
-      a->drop_cached();
-   };
-   int rc = FA_OK;
-   if ((rc = a->grow(fa_aligner::kUp, (size_t)n * rb, &d_up)) || (rc = a->grow(fa_aligner::kRawF, (size_t)n * npx, &d_rawf)) ||
-       (rc = a->grow(fa_aligner::kPre, (size_t)n * npx, &d_pre)) ||
-       (a->rp.smoothing > 0 && (rc = a->grow(fa_aligner::kBlurTmp, (size_t)C * npx, &d_btmp))) ||
-       (rc = a->grow(fa_aligner::kReal, (size_t)C * a->n_el, &w_real)) ||
-       (rc = a->grow(fa_aligner::kSpec, (size_t)C * a->n_cplx, &w_spec)) ||
-       (rc = a->grow(fa_aligner::kPv, (size_t)C * a->n_part, &w_pv)) ||
-       (rc = a->grow(fa_aligner::kPi, (size_t)C * a->n_part, &w_pi)) || (rc = a->grow(fa_aligner::kIdx, (size_t)n, &d_idx))) {
-      release();
-      return rc;
-   }
-   // 1. the frames uploaded U at a time through the two pinned buffers in turn (the host
-   //    threads copy a chunk while the previous chunk's DMA and kernels run), each chunk
-   //    converted, then C frames at a time: rigid shifts of the raw frames and the warper's
-   //    frames (blur + 1); the shifts come back in one copy. U is a multiple of C, so the
-   //    rigid batches are the same as over all frames at once.
-   int U = (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(32 << 20) / (int64_t)rb));
-   if (U >= C) U = U / C * C;
-   if (rc == FA_OK) rc = a->grow_host((size_t)U * rb);
-   for (int j0 = 0, k = 0; j0 < n && rc == FA_OK; j0 += U, k++) {
-      const int c = std::min(U, n - j0);
-      uint8_t* st = (uint8_t*)a->hbuf[k & 1].p;
-      if (k >= 2 && hipEventSynchronize(a->hev[k & 1]) != hipSuccess) {
-         rc = set_error(FA_E_DEVICE, "frame upload failed");
-         break;
-      }
-      std::vector<CopyPiece> pc;
-      for (int i = 0; i < c; i++) pc.push_back({st + (size_t)i * rb, frames[j0 + i], rb});
-      par_copy(pc);
-      if (hipMemcpyAsync(d_up + (size_t)j0 * rb, st, (size_t)c * rb, hipMemcpyHostToDevice, a->stream) != hipSuccess ||
-          hipEventRecord(a->hev[k & 1], a->stream) != hipSuccess) {
-         rc = set_error(FA_E_DEVICE, "frame upload failed");
-         break;
-      }
-      hipLaunchKernelGGL(k_to_float, dim3(grid_of((int64_t)c * npx)), dim3(kFB), 0, a->stream,
-                         (const void*)(d_up + (size_t)j0 * rb), dtype, (int64_t)c * npx, d_rawf + (int64_t)j0 * npx);
-      if (hipGetLastError() != hipSuccess) {
-         rc = set_error(FA_E_DEVICE, "frame upload failed");
-         break;
-      }
-      for (int i0 = j0; i0 < j0 + c && rc == FA_OK; i0 += C) {
-         const int cc = std::min(C, j0 + c - i0);
-         if ((rc = a->shifts(d_rawf + (int64_t)i0 * npx, cc, w_real, w_spec, w_pv, w_pi, d_idx + i0)) == FA_OK)
-            rc = a->smooth_plus_one(d_rawf + (int64_t)i0 * npx, d_pre + (int64_t)i0 * npx, cc, d_btmp);
-      }
-   }
-   std::vector<int64_t> idx(n);
-   if (rc == FA_OK && (hipMemcpyAsync(idx.data(), d_idx, n * sizeof(int64_t), hipMemcpyDeviceToHost, a->stream) != hipSuccess ||
-                       hipStreamSynchronize(a->stream) != hipSuccess)) {
-      (void)hipGetLastError();
-      rc = set_error(FA_E_DEVICE, "frame preprocessing failed");
-   }
-   for (int i = 0; i < n && rc == FA_OK; i++) a->shift_of(idx[i], &shifts[3 * (size_t)i]);
-   // the frames (views of d_pre) and their batch: those of the last call when the count is
-   // the same, else new ones
-   if (rc == FA_OK && (int)a->cfr.size() != n) {
-      a->drop_cached();
-      a->cfr.assign(n, nullptr);
-      for (int i = 0; i < n && rc == FA_OK; i++)
-         rc = adopt_device_frame(a->ref, d_pre + (int64_t)i * npx, FA_F32, 0.f, &a->cfr[i], false);
-   }
-   if (rc != FA_OK) {
-      release();
-      return rc;
-   }
-   fa_tr_params p;
-   if (prm) p = *prm;
-   else fa_tr_default(&p);
-   if (!a->cbatch) {
-      if ((rc = fa_batch_create(a->cfr.data(), n, &p, &a->cbatch)) != FA_OK) {
+   try {
+      if (!a || !frames || n <= 0 || !D_out) return set_error(FA_E_ARG, "null argument");
+      if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+      for (int i = 0; i < n; i++)
+         if (!frames[i]) return set_error(FA_E_ARG, "null frame");
+      std::lock_guard<std::mutex> lk(a->mu);
+      FCHK(hipSetDevice(a->ctx->device));
+      const int nD = a->nD, m = a->n_dim, np = nD * m;
+      const int64_t npx = a->n_px;
+      const size_t rb = a->raw_bytes(dtype);
+      const int C = a->chunk_frames(n);
+      std::vector<double> shifts(3 * (size_t)n, 0.0);
+      // the call's device arrays (kept in the aligner between calls): raw uploads, raw values as
+      // float, the warper's frames (blur + 1, registered without a copy), and the rigid shift's
+      // work arrays for C frames
+      uint8_t* d_up = nullptr;
+      float *d_rawf = nullptr, *d_pre = nullptr, *d_btmp = nullptr;
+      double *w_real = nullptr, *w_pv = nullptr;
+      double2* w_spec = nullptr;
+      int64_t *w_pi = nullptr, *d_idx = nullptr;
+      // on an error the cached frames and batch are dropped (rebuilt by the next call)
+      auto release = [&]() {
+         (void)hipStreamSynchronize(a->stream);
+         a->drop_cached();
+      };
+      int rc = FA_OK;
+      if ((rc = a->grow(fa_aligner::kUp, (size_t)n * rb, &d_up)) || (rc = a->grow(fa_aligner::kRawF, (size_t)n * npx, &d_rawf)) ||
+          (rc = a->grow(fa_aligner::kPre, (size_t)n * npx, &d_pre)) ||
+          (a->rp.smoothing > 0 && (rc = a->grow(fa_aligner::kBlurTmp, (size_t)C * npx, &d_btmp))) ||
+          (rc = a->grow(fa_aligner::kReal, (size_t)C * a->n_el, &w_real)) ||
+          (rc = a->grow(fa_aligner::kSpec, (size_t)C * a->n_cplx, &w_spec)) ||
+          (rc = a->grow(fa_aligner::kPv, (size_t)C * a->n_part, &w_pv)) ||
+          (rc = a->grow(fa_aligner::kPi, (size_t)C * a->n_part, &w_pi)) || (rc = a->grow(fa_aligner::kIdx, (size_t)n, &d_idx))) {
          release();
          return rc;
       }
-   } else {
-      batch_set_params(a->cbatch, p);
-   }
-   fa_batch* b = a->cbatch;
-   // 2. SSD at the zero start (twice in the reference) and at the rigid start
-   std::vector<double> D0(3 * (size_t)nD * n, 0.0), Dr(3 * (size_t)nD * n), e0(n), er(n);
-   for (int i = 0; i < n; i++)
-      for (int k = 0; k < nD; k++)
-         for (int c = 0; c < 3; c++) Dr[((size_t)i * nD + k) * 3 + c] = shifts[3 * (size_t)i + c];
-   if ((rc = fa_batch_eval(b, D0.data(), e0.data(), nullptr)) != FA_OK ||
-       (rc = fa_batch_eval(b, Dr.data(), er.data(), nullptr)) != FA_OK) {
-      release();
-      return rc;
-   }
-   // 3. start: the first smallest of {e0, e0, er}; x in D2col order
-   std::vector<double> x((size_t)np * n, 0.0);
-   std::vector<int> start(n, 0);
-   for (int i = 0; i < n; i++) {
-      start[i] = er[i] < e0[i] ? 2 : 0;
-      if (start[i] == 2)
+      // 1. the frames uploaded U at a time through the two pinned buffers in turn (the host
+      //    threads copy a chunk while the previous chunk's DMA and kernels run), each chunk
+      //    converted, then C frames at a time: rigid shifts of the raw frames and the warper's
+      //    frames (blur + 1); the shifts come back in one copy. U is a multiple of C, so the
+      //    rigid batches are the same as over all frames at once.
+      int U = (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(32 << 20) / (int64_t)rb));
+      if (U >= C) U = U / C * C;
+      if (rc == FA_OK) rc = a->grow_host((size_t)U * rb);
+      for (int j0 = 0, k = 0; j0 < n && rc == FA_OK; j0 += U, k++) {
+         const int c = std::min(U, n - j0);
+         uint8_t* st = (uint8_t*)a->hbuf[k & 1].p;
+         if (k >= 2 && hipEventSynchronize(a->hev[k & 1]) != hipSuccess) {
+            rc = set_error(FA_E_DEVICE, "frame upload failed");
+            break;
+         }
+         std::vector<CopyPiece> pc;
+         for (int i = 0; i < c; i++) pc.push_back({st + (size_t)i * rb, frames[j0 + i], rb});
+         par_copy(pc);
+         if (hipMemcpyAsync(d_up + (size_t)j0 * rb, st, (size_t)c * rb, hipMemcpyHostToDevice, a->stream) != hipSuccess ||
+             hipEventRecord(a->hev[k & 1], a->stream) != hipSuccess) {
+            rc = set_error(FA_E_DEVICE, "frame upload failed");
+            break;
+         }
+         hipLaunchKernelGGL(k_to_float, dim3(grid_of((int64_t)c * npx)), dim3(kFB), 0, a->stream,
+                            (const void*)(d_up + (size_t)j0 * rb), dtype, (int64_t)c * npx, d_rawf + (int64_t)j0 * npx);
+         if (hipGetLastError() != hipSuccess) {
+            rc = set_error(FA_E_DEVICE, "frame upload failed");
+            break;
+         }
+         for (int i0 = j0; i0 < j0 + c && rc == FA_OK; i0 += C) {
+            const int cc = std::min(C, j0 + c - i0);
+            if ((rc = a->shifts(d_rawf + (int64_t)i0 * npx, cc, w_real, w_spec, w_pv, w_pi, d_idx + i0)) == FA_OK)
+               rc = a->smooth_plus_one(d_rawf + (int64_t)i0 * npx, d_pre + (int64_t)i0 * npx, cc, d_btmp);
+         }
+      }
+      std::vector<int64_t> idx(n);
+      if (rc == FA_OK && (hipMemcpyAsync(idx.data(), d_idx, n * sizeof(int64_t), hipMemcpyDeviceToHost, a->stream) != hipSuccess ||
+                          hipStreamSynchronize(a->stream) != hipSuccess)) {
+         (void)hipGetLastError();
+         rc = set_error(FA_E_DEVICE, "frame preprocessing failed");
+      }
+      for (int i = 0; i < n && rc == FA_OK; i++) a->shift_of(idx[i], &shifts[3 * (size_t)i]);
+      // the frames (views of d_pre) and their batch: those of the last call when the count is
+      // the same, else new ones
+      if (rc == FA_OK && (int)a->cfr.size() != n) {
+         a->drop_cached();
+         a->cfr.assign(n, nullptr);
+         for (int i = 0; i < n && rc == FA_OK; i++)
+            rc = adopt_device_frame(a->ref, d_pre + (int64_t)i * npx, FA_F32, 0.f, &a->cfr[i], false);
+      }
+      if (rc != FA_OK) {
+         release();
+         return rc;
+      }
+      fa_tr_params p;
+      if (prm) p = *prm;
+      else fa_tr_default(&p);
+      if (!a->cbatch) {
+         if ((rc = fa_batch_create(a->cfr.data(), n, &p, &a->cbatch)) != FA_OK) {
+            release();
+            return rc;
+         }
+      } else {
+         batch_set_params(a->cbatch, p);
+      }
+      fa_batch* b = a->cbatch;
+      // 2. SSD at the zero start (twice in the reference) and at the rigid start
+      std::vector<double> D0(3 * (size_t)nD * n, 0.0), Dr(3 * (size_t)nD * n), e0(n), er(n);
+      for (int i = 0; i < n; i++)
          for (int k = 0; k < nD; k++)
-            for (int c = 0; c < m; c++) x[(size_t)i * np + k * m + c] = shifts[3 * (size_t)i + c];
-   }
-   // 4. trust region (Warp); None keeps D = 0 (NullFrameAligner), Translation the start
-   std::vector<fa_align_stats> st(n);
-   std::memset(st.data(), 0, st.size() * sizeof(fa_align_stats));
-   if (a->rp.type == FA_REALIGN_WARP) {
-      if ((rc = fa_batch_run(b, x.data(), st.data(), nullptr)) != FA_OK) {
+            for (int c = 0; c < 3; c++) Dr[((size_t)i * nD + k) * 3 + c] = shifts[3 * (size_t)i + c];
+      if ((rc = fa_batch_eval(b, D0.data(), e0.data(), nullptr)) != FA_OK ||
+          (rc = fa_batch_eval(b, Dr.data(), er.data(), nullptr)) != FA_OK) {
          release();
          return rc;
       }
-   } else if (a->rp.type == FA_REALIGN_NONE) {
-      std::fill(x.begin(), x.end(), 0.0);
-   }
-   for (int i = 0; i < n; i++) {
-      double* D = D_out + (size_t)i * nD * 3;
-      for (int k = 0; k < nD; k++)
-         for (int c = 0; c < 3; c++) D[k * 3 + c] = c < m ? x[(size_t)i * np + k * m + c] : 0.0;
-      if (results) {
-         fa_add_result& r = results[i];
-         std::memset(&r, 0, sizeof(r));
-         r.start = start[i];
-         for (int c = 0; c < 3; c++) r.rigid_shift[c] = shifts[3 * (size_t)i + c];
-         r.start_error[0] = r.start_error[1] = e0[i];
-         r.start_error[2] = er[i];
-         r.stats = st[i];
+      // 3. start: the first smallest of {e0, e0, er}; x in D2col order
+      std::vector<double> x((size_t)np * n, 0.0);
+      std::vector<int> start(n, 0);
+      for (int i = 0; i < n; i++) {
+         start[i] = er[i] < e0[i] ? 2 : 0;
+         if (start[i] == 2)
+            for (int k = 0; k < nD; k++)
+               for (int c = 0; c < m; c++) x[(size_t)i * np + k * m + c] = shifts[3 * (size_t)i + c];
       }
+      // 4. trust region (Warp); None keeps D = 0 (NullFrameAligner), Translation the start
+      std::vector<fa_align_stats> st(n);
+      std::memset(st.data(), 0, st.size() * sizeof(fa_align_stats));
+      if (a->rp.type == FA_REALIGN_WARP) {
+         if ((rc = fa_batch_run(b, x.data(), st.data(), nullptr)) != FA_OK) {
+            release();
+            return rc;
+         }
+      } else if (a->rp.type == FA_REALIGN_NONE) {
+         std::fill(x.begin(), x.end(), 0.0);
+      }
+      for (int i = 0; i < n; i++) {
+         double* D = D_out + (size_t)i * nD * 3;
+         for (int k = 0; k < nD; k++)
+            for (int c = 0; c < 3; c++) D[k * 3 + c] = c < m ? x[(size_t)i * np + k * m + c] : 0.0;
+         if (results) {
+            fa_add_result& r = results[i];
+            std::memset(&r, 0, sizeof(r));
+            r.start = start[i];
+            for (int c = 0; c < 3; c++) r.rigid_shift[c] = shifts[3 * (size_t)i + c];
+            r.start_error[0] = r.start_error[1] = e0[i];
+            r.start_error[2] = er[i];
+            r.stats = st[i];
+         }
+      }
+      // 5. output images and metrics from the raw frame at the final D
+      if (outputs || results) rc = a->products(n, D_out, d_rawf, d_pre, outputs, results);
+      if (rc != FA_OK) release();
+      return rc;
+   } catch (...) {
+      return fa_caught();
    }
-   // 5. output images and metrics from the raw frame at the final D
-   if (outputs || results) rc = a->products(n, D_out, d_rawf, d_pre, outputs, results);
-   if (rc != FA_OK) release();
-   return rc;
 }
 
 int fa_aligner_realign_as_frame(fa_aligner* a, const double* D, const void* frame, int32_t dtype,
                                 int32_t interpolate_missing, int32_t cv_type, void* out)
 {
-   if (!a || !D || !frame || !out) return set_error(FA_E_ARG, "null argument");
-   if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
-   if (!interpolate_missing && cv_type != 0 && cv_type != 2 && cv_type != 5)
-      return set_error(FA_E_ARG, "cv_type must be 0 (CV_8U), 2 (CV_16U) or 5 (CV_32F)");
-   std::lock_guard<std::mutex> lk(a->mu);
-   FCHK(hipSetDevice(a->ctx->device));
-   const RefDev& R = a->ref->dev;
-   const int64_t n_px = a->n_px;
-   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(R.n_chunks, (int64_t)a->ctx->n_cu * 8));
-   double* d_D = nullptr;
-   float *d_w = nullptr, *d_cov = nullptr;
-   void* d_out = nullptr;
-   auto done = [&](int rc) {
-      ffree(d_D);
-      ffree(d_w);
-      ffree(d_cov);
-      if (d_out) (void)hipFree(d_out);
-      return rc;
-   };
-   int rc;
-   if ((rc = falloc(&d_D, 3 * (size_t)a->nD)) || (rc = falloc(&d_w, n_px)) || (rc = falloc(&d_cov, n_px)))
-      return done(rc);
-   OK(a->upload(frame, dtype));  // d_raw = the frame as float, as given
-   const FrameDev Fraw{a->d_raw, FA_F32, 0.f};
-   if (hipMemcpyAsync(d_D, D, 3 * (size_t)a->nD * sizeof(double), hipMemcpyHostToDevice, a->stream) != hipSuccess)
-      return done(set_error(FA_E_DEVICE, "upload failed"));
-   size_t out_bytes = n_px * sizeof(float);
-   const void* src = d_w;
-   if (interpolate_missing) {
-      // warpImageInterpolated, invalid value 0
-      if (launch_warp(R, Fraw, d_D, d_w, 0.f, 0, 0, grid, a->stream) != hipSuccess)
-         return done(set_error(FA_E_DEVICE, "warp failed"));
-   } else {
-      // warpImage splat; the weighted values, not divided by the coverage, converted
-      if (hipMemsetAsync(d_w, 0, n_px * sizeof(float), a->stream) != hipSuccess ||
-          hipMemsetAsync(d_cov, 0, n_px * sizeof(float), a->stream) != hipSuccess ||
-          launch_splat(R, Fraw, d_D, d_w, d_cov, 0.f, grid, a->stream) != hipSuccess)
-         return done(set_error(FA_E_DEVICE, "warp failed"));
-      if (cv_type != 5) {
-         out_bytes = n_px * (cv_type == 0 ? 1 : 2);
-         if (hipMalloc(&d_out, out_bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            return done(set_error(FA_E_DEVICE, "device allocation failed"));
+   try {
+      if (!a || !D || !frame || !out) return set_error(FA_E_ARG, "null argument");
+      if (dtype != FA_F32 && dtype != FA_U16) return set_error(FA_E_ARG, "unsupported frame dtype");
+      if (!interpolate_missing && cv_type != 0 && cv_type != 2 && cv_type != 5)
+         return set_error(FA_E_ARG, "cv_type must be 0 (CV_8U), 2 (CV_16U) or 5 (CV_32F)");
+      std::lock_guard<std::mutex> lk(a->mu);
+      FCHK(hipSetDevice(a->ctx->device));
+      const RefDev& R = a->ref->dev;
+      const int64_t n_px = a->n_px;
+      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(R.n_chunks, (int64_t)a->ctx->n_cu * 8));
+      double* d_D = nullptr;
+      float *d_w = nullptr, *d_cov = nullptr;
+      void* d_out = nullptr;
+      auto done = [&](int rc) {
+         ffree(d_D);
+         ffree(d_w);
+         ffree(d_cov);
+         if (d_out) (void)hipFree(d_out);
+         return rc;
+      };
+      int rc;
+      if ((rc = falloc(&d_D, 3 * (size_t)a->nD)) || (rc = falloc(&d_w, n_px)) || (rc = falloc(&d_cov, n_px)))
+         return done(rc);
+      OK(a->upload(frame, dtype));  // d_raw = the frame as float, as given
+      const FrameDev Fraw{a->d_raw, FA_F32, 0.f};
+      if (hipMemcpyAsync(d_D, D, 3 * (size_t)a->nD * sizeof(double), hipMemcpyHostToDevice, a->stream) != hipSuccess)
+         return done(set_error(FA_E_DEVICE, "upload failed"));
+      size_t out_bytes = n_px * sizeof(float);
+      const void* src = d_w;
+      if (interpolate_missing) {
+         // warpImageInterpolated, invalid value 0
+         if (launch_warp(R, Fraw, d_D, d_w, 0.f, 0, 0, grid, a->stream) != hipSuccess)
+            return done(set_error(FA_E_DEVICE, "warp failed"));
+      } else {
+         // warpImage splat; the weighted values, not divided by the coverage, converted
+         if (hipMemsetAsync(d_w, 0, n_px * sizeof(float), a->stream) != hipSuccess ||
+             hipMemsetAsync(d_cov, 0, n_px * sizeof(float), a->stream) != hipSuccess ||
+             launch_splat(R, Fraw, d_D, d_w, d_cov, 0.f, grid, a->stream) != hipSuccess)
+            return done(set_error(FA_E_DEVICE, "warp failed"));
+         if (cv_type != 5) {
+            out_bytes = n_px * (cv_type == 0 ? 1 : 2);
+            if (hipMalloc(&d_out, out_bytes) != hipSuccess) {
+               (void)hipGetLastError();
+               return done(set_error(FA_E_DEVICE, "device allocation failed"));
+            }
+            if (cv_type == 0)
+               hipLaunchKernelGGL(k_to_u8, dim3(grid_of(n_px)), dim3(kFB), 0, a->stream, (const float*)d_w, n_px,
+                                  (uint8_t*)d_out);
+            else
+               hipLaunchKernelGGL(k_to_u16, dim3(grid_of(n_px)), dim3(kFB), 0, a->stream, (const float*)d_w, n_px,
+                                  (uint16_t*)d_out);
+            src = d_out;
          }
-         if (cv_type == 0)
-            hipLaunchKernelGGL(k_to_u8, dim3(grid_of(n_px)), dim3(kFB), 0, a->stream, (const float*)d_w, n_px,
-                               (uint8_t*)d_out);
-         else
-            hipLaunchKernelGGL(k_to_u16, dim3(grid_of(n_px)), dim3(kFB), 0, a->stream, (const float*)d_w, n_px,
-                               (uint16_t*)d_out);
-         src = d_out;
       }
+      if (hipGetLastError() != hipSuccess || hipMemcpyAsync(out, src, out_bytes, hipMemcpyDeviceToHost, a->stream) != hipSuccess ||
+          hipStreamSynchronize(a->stream) != hipSuccess)
+         return done(set_error(FA_E_DEVICE, "realignment failed"));
+      return done(FA_OK);
+   } catch (...) {
+      return fa_caught();
    }
-   if (hipGetLastError() != hipSuccess || hipMemcpyAsync(out, src, out_bytes, hipMemcpyDeviceToHost, a->stream) != hipSuccess ||
-       hipStreamSynchronize(a->stream) != hipSuccess)
-      return done(set_error(FA_E_DEVICE, "realignment failed"));
-   return done(FA_OK);
 }
 
 int fa_shift_pixel(const fa_reference* ref, const double* D, double* xyz)
 {
-   if (!ref || !D || !xyz) return set_error(FA_E_ARG, "null argument");
-   double loc[3];
-   OK(fa_warp_point(ref, D, (int32_t)xyz[0], (int32_t)xyz[1], (int32_t)xyz[2], 1, loc));
-   for (int c = 0; c < 3; c++) xyz[c] -= loc[c];
-   return FA_OK;
+   try {
+      if (!ref || !D || !xyz) return set_error(FA_E_ARG, "null argument");
+      double loc[3];
+      OK(fa_warp_point(ref, D, (int32_t)xyz[0], (int32_t)xyz[1], (int32_t)xyz[2], 1, loc));
+      for (int c = 0; c < 3; c++) xyz[c] -= loc[c];
+      return FA_OK;
+   } catch (...) {
+      return fa_caught();
+   }
 }
 
 int fa_write_realignment_info(const char* filename, const fa_scan_params* scan, int32_t n, int32_t nD,
                               const double* D, const double* unaligned_correlation, const double* correlation,
                               const double* coverage)
 {
-   if (!filename || !scan || n < 0 || nD < 1 || (n > 0 && (!D || !unaligned_correlation || !correlation || !coverage)))
-      return set_error(FA_E_ARG, "invalid argument");
-   if (n == 0) return FA_OK;  // nothing stored: the reference writes no file
-   FILE* fh = std::fopen(filename, "w");
-   if (!fh) return set_error(FA_E_ARG, std::string("cannot write ") + filename);
-   // ImageScanParameters constructor (ImageScanParameters.h:7-20)
-   const int ny = std::max(scan->n_y, 1);
-   double interline = scan->interline_duration;
-   if (interline < scan->line_duration) interline = 1.01 * scan->line_duration;
-   const double frame = ny * interline;
-   // std::ostream's default formatting of doubles is printf's %g
-   std::fprintf(fh, "Realignment Information Version, 2\n");
-   std::fprintf(fh, "Line Duration,%g\n", scan->line_duration);
-   std::fprintf(fh, "Interline Duration,%g\n", interline);
-   std::fprintf(fh, "Frame Duration,%g\n", frame);
-   std::fprintf(fh, "Interframe Duration,%g\n", scan->interframe_duration);
-   std::fprintf(fh, "Lines,%d\n", ny);
-   std::fprintf(fh, "Frames Per Stack,%d\n", scan->n_z);
-   std::fprintf(fh, "Frame, UnalignedCorrelation, Correlation, Coverage");
-   for (int j = 0; j < nD; j++) std::fprintf(fh, ", dx_%d, dy_%d, dz_%d", j, j, j);
-   std::fprintf(fh, "\n");
-   for (int i = 0; i < n; i++) {
-      std::fprintf(fh, "%d,%g,%g, %g", i, unaligned_correlation[i], correlation[i], coverage[i]);
-      for (int j = 0; j < nD; j++) {
-         const double* d = D + ((size_t)i * nD + j) * 3;
-         std::fprintf(fh, ", %g, %g, %g", d[0], d[1], d[2]);
-      }
+   try {
+      if (!filename || !scan || n < 0 || nD < 1 || (n > 0 && (!D || !unaligned_correlation || !correlation || !coverage)))
+         return set_error(FA_E_ARG, "invalid argument");
+      if (n == 0) return FA_OK;  // nothing stored: the reference writes no file
+      FILE* fh = std::fopen(filename, "w");
+      if (!fh) return set_error(FA_E_ARG, std::string("cannot write ") + filename);
+      // ImageScanParameters constructor (ImageScanParameters.h:7-20)
+      const int ny = std::max(scan->n_y, 1);
+      double interline = scan->interline_duration;
+      if (interline < scan->line_duration) interline = 1.01 * scan->line_duration;
+      const double frame = ny * interline;
+      // std::ostream's default formatting of doubles is printf's %g
+      std::fprintf(fh, "Realignment Information Version, 2\n");
+      std::fprintf(fh, "Line Duration,%g\n", scan->line_duration);
+      std::fprintf(fh, "Interline Duration,%g\n", interline);
+      std::fprintf(fh, "Frame Duration,%g\n", frame);
+      std::fprintf(fh, "Interframe Duration,%g\n", scan->interframe_duration);
+      std::fprintf(fh, "Lines,%d\n", ny);
+      std::fprintf(fh, "Frames Per Stack,%d\n", scan->n_z);
+      std::fprintf(fh, "Frame, UnalignedCorrelation, Correlation, Coverage");
+      for (int j = 0; j < nD; j++) std::fprintf(fh, ", dx_%d, dy_%d, dz_%d", j, j, j);
       std::fprintf(fh, "\n");
+      for (int i = 0; i < n; i++) {
+         std::fprintf(fh, "%d,%g,%g, %g", i, unaligned_correlation[i], correlation[i], coverage[i]);
+         for (int j = 0; j < nD; j++) {
+            const double* d = D + ((size_t)i * nD + j) * 3;
+            std::fprintf(fh, ", %g, %g, %g", d[0], d[1], d[2]);
+         }
+         std::fprintf(fh, "\n");
+      }
+      const bool ok = std::fclose(fh) == 0;
+      return ok ? FA_OK : set_error(FA_E_ARG, std::string("cannot write ") + filename);
+   } catch (...) {
+      return fa_caught();
    }
-   const bool ok = std::fclose(fh) == 0;
-   return ok ? FA_OK : set_error(FA_E_ARG, std::string("cannot write ") + filename);
 }
 
 }  // extern "C"

@@ -263,6 +263,9 @@ struct fa_frame {
 namespace fa {
 // Sets the calling thread's fa_last_error() message; returns code.
 int set_error(int code, const std::string& msg);
+// The C entry points' handler for a C++ exception (host memory or threads exhausted), which
+// must not leave the C interface: sets the message without throwing and returns FA_E_OOM.
+int fa_caught() noexcept;
 int adopt_device_frame(fa_reference* r, void* d_data, int32_t dtype, float offset, fa_frame** out, bool own = true);
 // Trust-region parameters of the next runs of batch b (and of its parts).
 void batch_set_params(fa_batch* b, const fa_tr_params& p);
