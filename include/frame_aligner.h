@@ -37,7 +37,7 @@ extern "C" {
 
 #define FA_OK 0
 #define FA_E_ARG (-1)           /* invalid argument / size ("Frame is incorrect size") */
-#define FA_E_OOM (-2)           /* device allocation failed ("Not enough memory to allocate") */
+#define FA_E_OOM (-2)           /* device allocation failed ("Not enough memory to allocate"), or host memory or threads exhausted */
 #define FA_E_UNKNOWN_FRAME (-3) /* handle not registered ("Unrecognised frame") */
 #define FA_E_DEVICE (-4)        /* HIP runtime error or no supported GPU ("No supported GPU") */
 #define FA_E_NODE_MAP (-5)      /* node map inconsistent ("Something went wrong") */
